@@ -1,0 +1,1104 @@
+/*
+ * sv_oracle.c -- FP64 CPU restatement of the SUNSET LABFM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  Never linked into
+ * the product library.
+ *
+ * The restatement follows the reference's algorithm and loop structure; each
+ * function names the reference file:line it follows.  Known deviations:
+ *   - per-node neighbour sort is by (r^2, j) (stable); the reference quicksort
+ *     (neighbours.F90:316-341) is unstable for equal r^2, so lists are compared
+ *     after the same canonical sort;
+ *   - svd_solve (svdlib.F90:22-306, Golub-Reinsch) is replaced by a one-sided
+ *     Jacobi SVD (Hestenes 1958) with the same "skip exactly-zero singular
+ *     values" semantics; both agree to ~cond(A)*eps;
+ *   - DI conformation RHS uses the full gradients of cxx and cyy and
+ *     szz = fr*(czz-1) (the reference reads uninitialised memory there,
+ *     rhs_di.F90:188-190, :421-424, :455-470); czz == 1 in 2D and is not stored;
+ *   - optional Kolmogorov forcing f_x = kol_amp*sin(kol_k*y) added as a momentum
+ *     source, as the forcing line in rhs.F90:275 (commented out in
+ *     rhs_lc.F90:263 / rhs_di.F90:264).
+ */
+#include "sv_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double PI = 3.141592653589793238462643383279502884197;
+static const double SS = 2.0;                  /* common_parameter.F90:68 */
+static const double VERYSMALL = 1.0e-30, DOUBLESMALL = 1.0e-16;
+
+/* RK3(2)4[2R+]C coefficients, common_parameter.F90:75-94 */
+#define RKA1 (11847461282814.0 / 36547543011857.0)
+#define RKA2 (3943225443063.0 / 7078155732230.0)
+#define RKA3 (-346793006927.0 / 4029903576067.0)
+#define RKB1 (1017324711453.0 / 9774461848756.0)
+#define RKB2 (8237718856693.0 / 13685301971492.0)
+#define RKB3 (57731312506979.0 / 19404895981398.0)
+#define RKB4 (-101169746363290.0 / 37734290219643.0)
+#define RKBH1 (15763415370699.0 / 46270243929542.0)
+#define RKBH2 (514528521746.0 / 5659431552419.0)
+#define RKBH3 (27030193851939.0 / 9429696342944.0)
+#define RKBH4 (-69544964788955.0 / 30262026368149.0)
+/* PID constants, common_parameter.F90:97-101 */
+#define PID_TOL 1.0e-5
+#define PID_A (0.49 / 2.0)
+#define PID_B (0.34 / 2.0)
+#define PID_C (0.1 / 2.0)
+#define PID_K 0.9
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+static int nsize_of(int m) { return (m * m + 3 * m) / 2; }
+
+/* ------------------------------------------------------------------------ */
+/* Domain                                                                    */
+/* ------------------------------------------------------------------------ */
+int or_domain_init(or_domain *d, int npfb, const double *x, const double *y,
+                   const double *s, const double *h, const double *hs,
+                   const int *node_type, double xmin, double xmax, double ymin,
+                   double ymax, const int bc[4], double smax, int morder,
+                   int forder, double L_char) {
+  memset(d, 0, sizeof(*d));
+  d->npfb = npfb;
+  d->np = npfb;
+  d->cap = 12 * npfb + 16;     /* setup_domain.F90:158-159 allocates 12*npfb */
+  d->x = calloc(d->cap, sizeof(double));
+  d->y = calloc(d->cap, sizeof(double));
+  d->h = calloc(d->cap, sizeof(double));
+  d->hs = calloc(d->cap, sizeof(double));
+  d->s = calloc(d->cap, sizeof(double));
+  d->node_type = calloc(d->cap, sizeof(int));
+  d->irel = calloc(d->cap, sizeof(int));
+  d->vrel = calloc(d->cap, sizeof(int));
+  if (!d->x || !d->y || !d->h || !d->hs || !d->s || !d->node_type || !d->irel || !d->vrel)
+    return -1;
+  for (int i = 0; i < npfb; i++) {
+    d->x[i] = x[i]; d->y[i] = y[i]; d->s[i] = s[i];
+    d->h[i] = h[i]; d->hs[i] = hs[i];
+    d->node_type[i] = node_type ? node_type[i] : 999;
+    d->irel[i] = -1; d->vrel[i] = 0;
+  }
+  d->xmin = xmin; d->xmax = xmax; d->ymin = ymin; d->ymax = ymax;
+  for (int k = 0; k < 4; k++) d->bc[k] = bc[k];
+  d->smax = smax;
+  /* setup_domain.F90:105: h0 = hovs*smax, h2 = h0*h0.  hovs per order
+   * (common_parameter.F90:45-56) */
+  double hovs = (forder == 4) ? 1.8 : (forder == 6) ? 2.4 : (forder == 10) ? 3.1 : 2.7;
+  d->h0 = hovs * smax;
+  d->h2 = d->h0 * d->h0;
+  d->morder = morder; d->forder = forder;
+  d->L_char = L_char;
+  return 0;
+}
+
+void or_domain_free(or_domain *d) {
+  free(d->x); free(d->y); free(d->h); free(d->hs); free(d->s);
+  free(d->node_type); free(d->irel); free(d->vrel);
+  free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
+  free(d->w_grad); free(d->w_lap); free(d->w_hyp);
+  free(d->gsum); free(d->lsum); free(d->hsum); free(d->vol);
+  memset(d, 0, sizeof(*d));
+}
+
+/* create_mirror_particles, mirror_boundaries.F90:34-261.  Ghost k gets parent
+ * irel[k]; positions use the reference's left-to-right arithmetic so that the
+ * neighbour distance tests reproduce bit for bit. */
+static int add_ghost(or_domain *d, int i, double gx, double gy, int vr) {
+  if (d->np >= d->cap) return -1;
+  int k = d->np++;
+  d->x[k] = gx; d->y[k] = gy; d->irel[k] = i; d->vrel[k] = vr;
+  return 0;
+}
+
+int or_create_ghosts(or_domain *d) {
+  const double mlayer_buff = 2.0;
+  const int xl = d->bc[0], xu = d->bc[1], yl = d->bc[2], yu = d->bc[3];
+  if ((xl == 1) != (xu == 1) || (yl == 1) != (yu == 1)) return -2; /* :61-69 */
+  const double xmin = d->xmin, xmax = d->xmax, ymin = d->ymin, ymax = d->ymax;
+  d->np = d->npfb;
+  for (int i = 0; i < d->npfb; i++) {
+    const double xi = d->x[i], yi = d->y[i];
+    const double lim = SS * d->h[i] * mlayer_buff;
+    int rc = 0;
+    if (xi <= xmin + lim) {                                   /* :82-94 */
+      if (xl == 1) rc |= add_ghost(d, i, xi + xmax - xmin, yi, 1);
+      else if (xl == 2) rc |= add_ghost(d, i, 2.0 * xmin - xi, yi, 2);
+    }
+    if (xi >= xmax - lim) {                                   /* :96-108 */
+      if (xu == 1) rc |= add_ghost(d, i, xi - xmax + xmin, yi, 1);
+      else if (xu == 2) rc |= add_ghost(d, i, 2.0 * xmax - xi, yi, 2);
+    }
+    if (yi <= ymin + lim) {                                   /* :111-128 */
+      if (yl == 1) rc |= add_ghost(d, i, xi, yi + ymax - ymin, 1);
+      else if (yl == 2) rc |= add_ghost(d, i, xi, 2.0 * ymin - yi, 3);
+      else if (yl == 3) rc |= add_ghost(d, i, xi, 2.0 * ymin - yi, 4);
+    }
+    if (yi >= ymax - lim) {                                   /* :130-147 */
+      if (yu == 1) rc |= add_ghost(d, i, xi, yi - ymax + ymin, 1);
+      else if (yu == 2) rc |= add_ghost(d, i, xi, 2.0 * ymax - yi, 3);
+      else if (yu == 3) rc |= add_ghost(d, i, xi, 2.0 * ymax - yi, 4);
+    }
+    /* corners, :149-240 */
+    const double cx[4] = {xmin, xmax, xmin, xmax};
+    const double cy[4] = {ymin, ymin, ymax, ymax};
+    const int bx[4] = {xl, xu, xl, xu};
+    const int by[4] = {yl, yl, yu, yu};
+    for (int c = 0; c < 4; c++) {
+      double dx = cx[c] - xi, dy = cy[c] - yi;
+      double cdist = sqrt(dx * dx + dy * dy + 0.0 * 0.0);
+      if (cdist <= lim && bx[c] != 0 && by[c] != 0) {
+        double px, py; int vr = 0, ok = 1;
+        int perx = (bx[c] == 1), pery = (by[c] == 1);
+        int symx = (bx[c] == 2), symy = (by[c] == 2);
+        if (c == 0 || c == 2) px = perx ? xi + xmax - xmin : 2.0 * xmin - xi;
+        else px = perx ? xi - xmax + xmin : 2.0 * xmax - xi;
+        if (c == 0 || c == 1) py = pery ? yi + ymax - ymin : 2.0 * ymin - yi;
+        else py = pery ? yi - ymax + ymin : 2.0 * ymax - yi;
+        if (perx && pery) vr = 1;
+        else if (symx && pery) vr = 2;
+        else if (perx && symy) vr = 3;
+        else if (symx && symy) vr = 4;
+        else ok = 0;   /* other corner combinations create no entry in the reference */
+        if (ok) rc |= add_ghost(d, i, px, py, vr);
+        else {         /* reference increments imp without setting position */
+          rc |= add_ghost(d, i, 0.0, 0.0, 0);
+        }
+      }
+    }
+    if (rc) return -1;
+  }
+  for (int k = d->npfb; k < d->np; k++) {                     /* :245-254 */
+    int p = d->irel[k];
+    d->node_type[k] = d->node_type[p];
+    d->h[k] = d->h[p]; d->hs[k] = d->hs[p]; d->s[k] = d->s[p];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Neighbours: neighbours.F90:32-341                                         */
+/* ------------------------------------------------------------------------ */
+static double r2_of(const or_domain *d, int i, int j) {
+  /* neighbours.F90:260-261: rij = ri - rj; dot_product over 3 components */
+  double dx = d->x[i] - d->x[j];
+  double dy = d->y[i] - d->y[j];
+  double t = dx * dx;
+  t = t + dy * dy;
+  t = t + 0.0;
+  return t;
+}
+
+typedef struct { double r2; int j; } or_pair;
+static int pair_cmp(const void *a, const void *b) {
+  const or_pair *p = a, *q = b;
+  if (p->r2 < q->r2) return -1;
+  if (p->r2 > q->r2) return 1;
+  return (p->j > q->j) - (p->j < q->j);
+}
+
+int or_find_neighbours(or_domain *d, int nplink) {
+  /* cell_generator :75-93 */
+  const double sup_size = SS * d->h0;
+  const double sizeX = d->xmax - d->xmin, sizeY = d->ymax - d->ymin;
+  const double unoss = 1.0 / (1.0 * sup_size);
+  const double xmaxt = d->xmax + sizeX, xmint = d->xmin - sizeX;
+  const double ymaxt = d->ymax + sizeY, ymint = d->ymin - sizeY;
+  const int ncx = (int)((xmaxt - xmint) * unoss) + 1;
+  const int ncy = (int)((ymaxt - ymint) * unoss) + 1;
+  const long nct = (long)ncx * ncy;
+  const int np = d->np, npfb = d->npfb;
+  /* divide :160-206: counting sort of all np nodes into cells */
+  int *cellpart = malloc(sizeof(int) * np);
+  long *ist = calloc(nct + 2, sizeof(long));
+  int *ip = malloc(sizeof(int) * np);
+  for (int k = 0; k < np; k++) {
+    int icell = (int)((d->x[k] - xmint) * unoss);
+    int jcell = (int)((d->y[k] - ymint) * unoss);
+    if (icell < 0) icell = 0;
+    if (icell >= ncx) icell = ncx - 1;
+    if (jcell < 0) jcell = 0;
+    if (jcell >= ncy) jcell = ncy - 1;
+    cellpart[k] = icell + jcell * ncx;
+    ist[cellpart[k] + 1]++;
+  }
+  for (long c = 0; c < nct; c++) ist[c + 1] += ist[c];
+  {
+    long *fill = calloc(nct + 1, sizeof(long));
+    for (int k = 0; k < np; k++) {
+      long c = cellpart[k];
+      ip[ist[c] + fill[c]++] = k;
+    }
+    free(fill);
+  }
+  free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
+  d->nplink = nplink;
+  d->ij_count = calloc(npfb, sizeof(int));
+  d->ij_count_small = calloc(npfb, sizeof(int));
+  d->ij_link = calloc((size_t)npfb * nplink, sizeof(int));
+  int overflow = 0;
+  /* neighboring_list_parallel :208-270 + order_neighbours :272-314 */
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : overflow)
+  for (int i = 0; i < npfb; i++) {
+    or_pair *buf = malloc(sizeof(or_pair) * (size_t)nplink);
+    int cnt = 0;
+    const int ic = cellpart[i];
+    const int icx = ic % ncx, icy = ic / ncx;
+    const double hh = d->h[i] * SS;
+    const double st2 = hh * hh;
+    for (int oy = -1; oy <= 1; oy++)
+      for (int ox = -1; ox <= 1; ox++) {
+        int cx = icx + ox, cy = icy + oy;
+        if (cx < 0 || cx >= ncx || cy < 0 || cy >= ncy) continue;
+        long c = (long)cy * ncx + cx;
+        for (long jj = ist[c]; jj < ist[c + 1]; jj++) {
+          int j = ip[jj];
+          double r2 = r2_of(d, i, j);
+          if (r2 <= st2) {
+            if (cnt < nplink) { buf[cnt].r2 = r2; buf[cnt].j = j; }
+            cnt++;
+          }
+        }
+      }
+    if (cnt > nplink) { overflow = 1; cnt = nplink; }
+    qsort(buf, cnt, sizeof(or_pair), pair_cmp);
+    const double hs2 = d->hs[i] * SS;
+    const double st2s = hs2 * hs2;
+    int ks = 0;
+    for (int k = 0; k < cnt; k++) {
+      d->ij_link[(size_t)i * nplink + k] = buf[k].j;
+      if (buf[k].r2 <= st2s) ks++;
+    }
+    d->ij_count[i] = cnt;
+    d->ij_count_small[i] = ks;
+    free(buf);
+  }
+  free(cellpart); free(ist); free(ip);
+  return overflow ? -3 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernels and bases                                                         */
+/* ------------------------------------------------------------------------ */
+/* Wendland C2, rbfs.F90:48-57 (normalisation 7/(4*pi*h2), h2 from
+ * setup_domain.F90:105) */
+double or_wab(double q, double h2) {
+  double t = 1.0 - 0.5 * q;
+  t = pow(t, 4.0);
+  if (q <= 2.0 && q > 0.0) return (2.0 * q + 1.0) * (7.0 / (4.0 * PI * h2)) * t;
+  return 0.0;
+}
+
+/* Taylor monomials x^a y^b/(a! b!) in graded order, labf.F90:1815-1920 */
+void or_monomials(int order, double x, double y, double *out) {
+  double xp[11], yp[11];
+  xp[0] = 1.0; yp[0] = 1.0; xp[1] = x; yp[1] = y;
+  for (int k = 2; k <= order; k++) { xp[k] = xp[k - 1] * x; yp[k] = yp[k - 1] * y; }
+  static const double fact[11] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880, 3628800};
+  int idx = 0;
+  for (int dg = 1; dg <= order; dg++)
+    for (int b = 0; b <= dg; b++) {
+      int a = dg - b;
+      double coef = 1.0 / (fact[a] * fact[b]);
+      double v;
+      if (b == 0) v = (a == 1) ? x : coef * xp[a];
+      else if (a == 0) v = (b == 1) ? y : coef * yp[b];
+      else if (a == 1 && b == 1) v = x * y;
+      else if (a == 1) v = (coef * x) * yp[b];
+      else if (b == 1) v = (coef * xp[a]) * y;
+      else v = (coef * xp[a]) * yp[b];
+      out[idx++] = v;
+    }
+}
+
+/* Probabilists' Hermite polynomials as written in labf.F90:2329-2391 */
+static double hermite(int n, double z) {
+  double z2, z3, z4;
+  switch (n) {
+  case 0: return 1.0;
+  case 1: return z;
+  case 2: return z * z - 1.0;
+  case 3: return z * z * z - 3.0 * z;
+  case 4: return z * z * z * z - 6.0 * z * z + 3.0;
+  case 5: z2 = z * z; z3 = z * z2; return z2 * z3 - 10.0 * z3 + 15.0 * z;
+  case 6: z2 = z * z; z3 = z * z2; return z3 * z3 - 15.0 * z2 * z2 + 45.0 * z2 - 15.0;
+  case 7: z2 = z * z; z3 = z * z2; z4 = z3 * z;
+    return z4 * z3 - 21.0 * z3 * z2 + 105.0 * z3 - 105.0 * z;
+  case 8: z2 = z * z; z4 = z2 * z2;
+    return z4 * z4 - 28.0 * z4 * z2 + 210.0 * z4 - 420.0 * z2 + 105.0;
+  case 9: z2 = z * z; z3 = z * z2; z4 = z3 * z;
+    return z4 * z3 * z2 - 36.0 * z4 * z3 + 378.0 * z3 * z2 - 1260.0 * z3 + 945.0 * z;
+  case 10: z2 = z * z; z4 = z2 * z2;
+    return z4 * z4 * z2 - 45.0 * z4 * z4 + 630.0 * z4 * z2 - 3150.0 * z4 + 4725.0 * z2 - 945.0;
+  }
+  return 0.0;
+}
+
+/* Hermite ABFs (ABF==2), labf.F90:2017-2120: He_p(x)*He_q(y) graded order */
+void or_abfs(int order, double x, double y, double *out) {
+  double hx[11], hy[11];
+  for (int k = 0; k <= order; k++) { hx[k] = hermite(k, x); hy[k] = hermite(k, y); }
+  int idx = 0;
+  for (int dg = 1; dg <= order; dg++)
+    for (int b = 0; b <= dg; b++) {
+      int a = dg - b;
+      double v;
+      if (b == 0) v = hx[a];
+      else if (a == 0) v = hy[b];
+      else v = hx[a] * hy[b];
+      out[idx++] = v;
+    }
+}
+
+/* Solve A x = b by SVD, A = U S V^T, x = V S^+ U^T b, skipping singular values
+ * that are exactly zero (svdlib.F90:282-303 semantics).  One-sided Jacobi
+ * (Hestenes): orthogonalise the columns of W = A by plane rotations, V
+ * accumulates them; then sigma_j = |W_j| and U_j = W_j / sigma_j.
+ * A is column-major (Fortran layout).  Returns 0, or -1 if not converged. */
+int or_svd_solve(int n, const double *A, double *b) {
+  double *W = malloc(sizeof(double) * n * n);
+  double *V = calloc((size_t)n * n, sizeof(double));
+  memcpy(W, A, sizeof(double) * n * n);
+  for (int j = 0; j < n; j++) V[j * n + j] = 1.0;
+  int converged = 0;
+  for (int sweep = 0; sweep < 80 && !converged; sweep++) {
+    converged = 1;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double *wp = W + (size_t)p * n, *wq = W + (size_t)q * n;
+        double al = 0, be = 0, ga = 0;
+        for (int r = 0; r < n; r++) { al += wp[r] * wp[r]; be += wq[r] * wq[r]; ga += wp[r] * wq[r]; }
+        if (ga == 0.0) continue;
+        if (fabs(ga) <= 1.0e-15 * sqrt(al * be)) continue;
+        converged = 0;
+        double zeta = (be - al) / (2.0 * ga);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        for (int r = 0; r < n; r++) {
+          double a1 = wp[r], a2 = wq[r];
+          wp[r] = c * a1 - s * a2; wq[r] = s * a1 + c * a2;
+        }
+        double *vp = V + (size_t)p * n, *vq = V + (size_t)q * n;
+        for (int r = 0; r < n; r++) {
+          double a1 = vp[r], a2 = vq[r];
+          vp[r] = c * a1 - s * a2; vq[r] = s * a1 + c * a2;
+        }
+      }
+  }
+  double *tmp = malloc(sizeof(double) * n);
+  for (int j = 0; j < n; j++) {
+    double *wj = W + (size_t)j * n;
+    double s2 = 0, ub = 0;
+    for (int r = 0; r < n; r++) { s2 += wj[r] * wj[r]; ub += wj[r] * b[r]; }
+    /* U_j^T b / sigma_j = (W_j^T b)/sigma_j^2 */
+    tmp[j] = (s2 != 0.0) ? ub / s2 : 0.0;
+  }
+  for (int r = 0; r < n; r++) {
+    double acc = 0;
+    for (int j = 0; j < n; j++) acc += V[(size_t)j * n + r] * tmp[j];
+    b[r] = acc;
+  }
+  free(W); free(V); free(tmp);
+  return converged ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Weights: labf.F90:34-231 (derivatives), :233-394 (filter), :453-503 (vol) */
+/* ------------------------------------------------------------------------ */
+/* Row-reduction of the moment matrix near walls (labf.F90:126-148): rows and
+ * columns beyond nkeep become identity. */
+static void reduce_order(double *A, int n, int nkeep) {
+  for (int c = nkeep; c < n; c++)
+    for (int r = 0; r < n; r++) A[(size_t)c * n + r] = 0.0;  /* cols >= nkeep */
+  for (int r = nkeep; r < n; r++) {
+    for (int c = 0; c < n; c++) A[(size_t)c * n + r] = 0.0;
+    A[(size_t)r * n + r] = 1.0;
+  }
+}
+
+static void node_weights(or_domain *d, int i, int morder, int forder) {
+  const int nG = nsize_of(morder);
+  const int nF = nsize_of(forder);
+  const int nL = nF > nG ? nF : nG;
+  double *A = calloc((size_t)nL * nL, sizeof(double));
+  double *g = malloc(sizeof(double) * nL), *xv = malloc(sizeof(double) * nL);
+  double *bx = malloc(sizeof(double) * nL), *by = malloc(sizeof(double) * nL);
+  double *bl = malloc(sizeof(double) * nL), *bh = malloc(sizeof(double) * nL);
+  const size_t base = (size_t)i * d->nplink;
+  /* ---- derivative system, hh = h_small, k <= ij_count_small ---- */
+  double hh = d->hs[i];
+  memset(A, 0, sizeof(double) * nG * nG);
+  for (int k = 0; k < d->ij_count_small[i]; k++) {
+    int j = d->ij_link[base + k];
+    double x = -(d->x[i] - d->x[j]), y = -(d->y[i] - d->y[j]);
+    double rad = sqrt(x * x + y * y) / hh;
+    double ff1 = or_wab(rad, d->h2);
+    or_abfs(morder, x / hh, y / hh, g);
+    for (int a = 0; a < nG; a++) g[a] *= ff1;
+    or_monomials(morder, x / hh, y / hh, xv);
+    for (int r = 0; r < nG; r++)          /* amatx(i1,:) += xvec(i1)*gvec(:) */
+      for (int c = 0; c < nG; c++) A[(size_t)c * nG + r] += xv[r] * g[c];
+  }
+  int nt = d->node_type[i];
+  if (morder >= 7 && (nt == 998 || nt < 0)) reduce_order(A, nG, 27);
+  if (morder >= 5 && (nt == -1 || nt == -2)) reduce_order(A, nG, 14);
+  memset(bx, 0, sizeof(double) * nG); bx[0] = 1.0;
+  memset(by, 0, sizeof(double) * nG); by[1] = 1.0;
+  double *bxx = malloc(sizeof(double) * nG), *byy = malloc(sizeof(double) * nG);
+  memset(bxx, 0, sizeof(double) * nG); bxx[2] = 1.0;
+  memset(byy, 0, sizeof(double) * nG); byy[4] = 1.0;
+  or_svd_solve(nG, A, bx);
+  or_svd_solve(nG, A, by);
+  or_svd_solve(nG, A, bxx);
+  or_svd_solve(nG, A, byy);
+  for (int a = 0; a < nG; a++) bl[a] = bxx[a] + byy[a];
+  for (int k = 0; k < d->ij_count[i]; k++) {     /* labf.F90:183-215 */
+    int j = d->ij_link[base + k];
+    double x = -(d->x[i] - d->x[j]), y = -(d->y[i] - d->y[j]);
+    double rad = sqrt(x * x + y * y) / hh;
+    double ff1 = or_wab(rad, d->h2);
+    or_abfs(morder, x / hh, y / hh, g);
+    double sx = 0, sy = 0, sl = 0;
+    for (int a = 0; a < nG; a++) {
+      double ga = g[a] * ff1;
+      sx += bx[a] * ga; sy += by[a] * ga; sl += bl[a] * ga;
+    }
+    d->w_grad[(base + k) * 2 + 0] = sx / hh;
+    d->w_grad[(base + k) * 2 + 1] = sy / hh;
+    d->w_lap[base + k] = sl / hh / hh;
+  }
+  for (int k = d->ij_count[i]; k < d->nplink; k++) {
+    d->w_grad[(base + k) * 2 + 0] = 0; d->w_grad[(base + k) * 2 + 1] = 0; d->w_lap[base + k] = 0;
+  }
+  free(bxx); free(byy);
+  /* ---- filter system, hh = h, k <= ij_count, order forder (:233-394) ---- */
+  hh = d->h[i];
+  memset(A, 0, sizeof(double) * nF * nF);
+  for (int k = 0; k < d->ij_count[i]; k++) {
+    int j = d->ij_link[base + k];
+    double x = -(d->x[i] - d->x[j]), y = -(d->y[i] - d->y[j]);
+    double rad = sqrt(x * x + y * y) / hh;
+    double ff1 = or_wab(rad, d->h2);
+    or_abfs(forder, x / hh, y / hh, g);
+    for (int a = 0; a < nF; a++) g[a] *= ff1;
+    or_monomials(forder, x / hh, y / hh, xv);
+    for (int r = 0; r < nF; r++)
+      for (int c = 0; c < nF; c++) A[(size_t)c * nF + r] += xv[r] * g[c];
+  }
+  if (forder >= 7 && (nt == 998 || nt < 0)) reduce_order(A, nF, 27);
+  memset(bh, 0, sizeof(double) * nF);
+  /* hyperviscous RHS patterns, labf.F90:325-334 (0-based indices) */
+  if (forder <= 5) { bh[9] = -1; bh[11] = -2; bh[13] = -1; }
+  else if (forder <= 7) { bh[20] = 1; bh[22] = 3; bh[24] = 3; bh[26] = 1; }
+  else if (forder <= 9) { bh[35] = -1; bh[37] = -4; bh[39] = -6; bh[41] = -4; bh[43] = -1; }
+  else { bh[54] = 1; bh[56] = 5; bh[58] = 10; bh[60] = 10; bh[62] = 5; bh[64] = 1; }
+  if (forder >= 7 && (nt == 998 || nt < 0)) {
+    memset(bh, 0, sizeof(double) * nF);
+    bh[20] = 1; bh[22] = 3; bh[24] = 3; bh[26] = 1;
+  }
+  if (forder >= 5 && nt == -1) {
+    reduce_order(A, nF, 14);
+    memset(bh, 0, sizeof(double) * nF);
+    bh[9] = -1; bh[11] = -2; bh[13] = -1;
+  }
+  or_svd_solve(nF, A, bh);
+  for (int k = 0; k < d->ij_count[i]; k++) {
+    int j = d->ij_link[base + k];
+    double x = -(d->x[i] - d->x[j]), y = -(d->y[i] - d->y[j]);
+    double rad = sqrt(x * x + y * y) / hh;
+    double ff1 = or_wab(rad, d->h2);
+    or_abfs(forder, x / hh, y / hh, g);
+    double sh = 0;
+    for (int a = 0; a < nF; a++) sh += bh[a] * (g[a] * ff1);
+    d->w_hyp[base + k] = sh;
+  }
+  for (int k = d->ij_count[i]; k < d->nplink; k++) d->w_hyp[base + k] = 0;
+  /* ---- node volume, labf.F90:453-503 ---- */
+  double vs = 0;
+  for (int k = 0; k < d->ij_count[i]; k++) {
+    int j = d->ij_link[base + k];
+    double x = -(d->x[i] - d->x[j]), y = -(d->y[i] - d->y[j]);
+    double rad = sqrt(x * x + y * y) / hh;
+    vs += (9.0 / PI) * exp(-9.0 * rad * rad) / (hh * hh);
+  }
+  d->vol[i] = 1.0 / vs;
+  free(A); free(g); free(xv); free(bx); free(by); free(bl); free(bh);
+}
+
+static int alloc_weights(or_domain *d) {
+  size_t L = (size_t)d->npfb * d->nplink;
+  free(d->w_grad); free(d->w_lap); free(d->w_hyp); free(d->vol);
+  free(d->gsum); free(d->lsum); free(d->hsum);
+  d->w_grad = calloc(2 * L, sizeof(double));
+  d->w_lap = calloc(L, sizeof(double));
+  d->w_hyp = calloc(L, sizeof(double));
+  d->vol = calloc(d->npfb, sizeof(double));
+  d->gsum = calloc(2 * (size_t)d->npfb, sizeof(double));
+  d->lsum = calloc(d->npfb, sizeof(double));
+  d->hsum = calloc(d->npfb, sizeof(double));
+  return (d->w_grad && d->w_lap && d->w_hyp && d->vol && d->gsum && d->lsum && d->hsum) ? 0 : -1;
+}
+
+int or_calc_labf_weights(or_domain *d) {
+  /* The reference loop is serial (labf.F90:82); OpenMP here only speeds the
+   * checker up -- the per-node arithmetic is independent. */
+  if (alloc_weights(d)) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < d->npfb; i++) node_weights(d, i, d->morder, d->forder);
+  return 0;
+}
+
+int or_calc_labf_weights_ordermap(or_domain *d, const int *order_of_node, int nthreads) {
+  if (alloc_weights(d)) return -1;
+  if (nthreads > 0) or_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < d->npfb; i++) {
+    int m = order_of_node ? order_of_node[i] : d->morder;
+    node_weights(d, i, m, m);
+  }
+  return 0;
+}
+
+/* calc_labf_sums, labf.F90:396-451 (interior part) */
+void or_calc_labf_sums(or_domain *d) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {
+    const size_t base = (size_t)i * d->nplink;
+    double sh = 0, sx = 0, sy = 0, sl = 0;
+    for (int k = 0; k < d->ij_count[i]; k++) {
+      sh += d->w_hyp[base + k];
+      sx += d->w_grad[(base + k) * 2 + 0];
+      sy += d->w_grad[(base + k) * 2 + 1];
+      sl += d->w_lap[base + k];
+    }
+    d->hsum[i] = sh;
+    d->gsum[2 * i + 0] = sx / d->L_char;
+    d->gsum[2 * i + 1] = sy / d->L_char;
+    for (int k = 0; k < d->nplink; k++) {
+      d->w_grad[(base + k) * 2 + 0] /= d->L_char;
+      d->w_grad[(base + k) * 2 + 1] /= d->L_char;
+    }
+    d->lsum[i] = sl / d->L_char / d->L_char;
+    for (int k = 0; k < d->nplink; k++) d->w_lap[base + k] = d->w_lap[base + k] / d->L_char / d->L_char;
+  }
+}
+
+/* filter_coefficients, labf.F90:1723-1811 (interior nodes) */
+void or_filter_coefficients(or_domain *d) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {
+    const size_t base = (size_t)i * d->nplink;
+    double lscal = 2.0 * d->h[i] * sqrt(PI / (double)d->ij_count[i]);
+    double tmp = (2.0 / 3.0) * PI / lscal;
+    double lsum = 0;
+    for (int k = 0; k < d->ij_count[i]; k++) {
+      int j = d->ij_link[base + k];
+      double x = d->x[j] - d->x[i], y = d->y[j] - d->y[i];
+      double fji = 1.0 - cos(tmp * x) * cos(tmp * y);
+      lsum += fji * d->w_hyp[base + k];
+    }
+    double fc = (2.0 / 3.0) / lsum;
+    int nt = d->node_type[i];
+    (void)nt;  /* wall-row reductions need fd_parent (boundary nodes): not in scope */
+    double hs = 0;
+    for (int k = 0; k < d->ij_count[i]; k++) {
+      d->w_hyp[base + k] = d->w_hyp[base + k] * fc;
+      hs += d->w_hyp[base + k];
+    }
+    d->hsum[i] = hs;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stencil operators: derivatives.F90                                        */
+/* ------------------------------------------------------------------------ */
+void or_calc_gradient(const or_domain *d, const double *phi, double *gx, double *gy) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {          /* derivatives.F90:36-45 */
+    const size_t base = (size_t)i * d->nplink;
+    double tx = 0, ty = 0;
+    for (int k = 0; k < d->ij_count_small[i]; k++) {
+      double pj = phi[d->ij_link[base + k]];
+      tx = tx + pj * d->w_grad[(base + k) * 2 + 0];
+      ty = ty + pj * d->w_grad[(base + k) * 2 + 1];
+    }
+    gx[i] = tx - phi[i] * d->gsum[2 * i + 0];
+    gy[i] = ty - phi[i] * d->gsum[2 * i + 1];
+  }
+}
+
+void or_calc_laplacian(const or_domain *d, const double *phi, double *lap) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {          /* derivatives.F90:84-93 */
+    const size_t base = (size_t)i * d->nplink;
+    double t = 0;
+    for (int k = 0; k < d->ij_count_small[i]; k++)
+      t = t + phi[d->ij_link[base + k]] * d->w_lap[base + k];
+    lap[i] = t - phi[i] * d->lsum[i];
+  }
+}
+
+void or_calc_gradient_only(const or_domain *d, int dim, const double *phi, double *g) {
+  if (dim < 1 || dim > 2) {                    /* 2D: third component is zero */
+    for (int i = 0; i < d->npfb; i++) g[i] = 0.0;
+    return;
+  }
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {          /* derivatives.F90:428-437 */
+    const size_t base = (size_t)i * d->nplink;
+    double t = 0;
+    for (int k = 0; k < d->ij_count_small[i]; k++)
+      t = t + phi[d->ij_link[base + k]] * d->w_grad[(base + k) * 2 + dim - 1];
+    g[i] = t - phi[i] * d->gsum[2 * i + dim - 1];
+  }
+}
+
+void or_calc_filter_term(const or_domain *d, const double *phi, double *out) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {          /* derivatives.F90:480-488 */
+    const size_t base = (size_t)i * d->nplink;
+    double t = 0;
+    for (int k = 0; k < d->ij_count[i]; k++)
+      t = t + phi[d->ij_link[base + k]] * d->w_hyp[base + k];
+    out[i] = t - phi[i] * d->hsum[i];
+  }
+}
+
+void or_calc_filtered_var(const or_domain *d, double *phi) {
+  double *f = malloc(sizeof(double) * d->npfb);
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) {          /* derivatives.F90:373-383 */
+    const size_t base = (size_t)i * d->nplink;
+    double t = 0;
+    for (int k = 0; k < d->ij_count[i]; k++)
+      t = t + phi[d->ij_link[base + k]] * d->w_hyp[base + k];
+    f[i] = phi[i] + (t - phi[i] * d->hsum[i]);
+  }
+  memcpy(phi, f, sizeof(double) * d->npfb);
+  free(f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Conformation transforms: conf_transforms.F90                             */
+/* ------------------------------------------------------------------------ */
+/* 2D eigens, conf_transforms.F90:399-444.  R column-major: R[0]=R11,
+ * R[1]=R21, R[2]=R12, R[3]=R22. */
+void or_eigens2(double a, double b, double d, double l[2], double R[4]) {
+  if (fabs(b) <= DOUBLESMALL) {
+    l[0] = a; l[1] = d;
+    R[0] = 1.0; R[1] = 0.0; R[2] = 0.0; R[3] = 1.0;
+    return;
+  }
+  double tr = a + d;
+  double det = a * d - b * b;
+  double temp = 0.25 * tr * tr - det;
+  if (temp <= DOUBLESMALL) temp = 0.0;
+  temp = sqrt(temp);
+  l[0] = 0.5 * tr + temp;
+  l[1] = 0.5 * tr - temp;
+  double r11 = l[0] - d, r21 = b;
+  temp = sqrt(r11 * r11 + r21 * r21);
+  r11 = r11 / temp; r21 = r21 / temp;
+  R[0] = r11; R[1] = r21; R[2] = -r21; R[3] = r11;
+}
+
+/* matmul(R, matmul(diag(L), R^T)), returning the (1,1),(2,1),(2,2) entries */
+static void recompose(const double R[4], double l1, double l2, double *c11, double *c21,
+                      double *c22) {
+  /* inner = diag(L) * R^T : inner(a,b) = L_a * R(b,a) */
+  double i11 = l1 * R[0], i12 = l1 * R[1];
+  double i21 = l2 * R[2], i22 = l2 * R[3];
+  /* C = R * inner */
+  *c11 = R[0] * i11 + R[2] * i21;
+  *c21 = R[1] * i11 + R[3] * i21;
+  *c22 = R[1] * i12 + R[3] * i22;
+}
+
+/* log_conf_c_from_psi, conf_transforms.F90:133-169 */
+void or_c_from_psi(double pxx, double pxy, double pyy, double *cxx, double *cxy, double *cyy) {
+  double l[2], R[4];
+  or_eigens2(pxx, pxy, pyy, l, R);
+  recompose(R, exp(l[0]), exp(l[1]), cxx, cxy, cyy);
+}
+
+/* log_conf_psi_from_c, conf_transforms.F90:177-215 */
+void or_psi_from_c(double cxx, double cxy, double cyy, double *pxx, double *pxy, double *pyy) {
+  double l[2], R[4];
+  or_eigens2(cxx, cxy, cyy, l, R);
+  recompose(R, log(l[0]), log(l[1]), pxx, pxy, pyy);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Physics                                                                   */
+/* ------------------------------------------------------------------------ */
+void or_phys_derive(or_phys *p) {            /* load_data.F90:132-148 */
+  p->rho_char = 1.0;
+  p->visc_total = p->rho_char * p->U_char * p->L_char / p->Re;
+  p->visc_solvent = p->beta * p->visc_total;
+  p->visc_polymeric = (1.0 - p->beta) * p->visc_total;
+  p->lambda = p->Wi * p->L_char / p->U_char;
+  p->csq = (p->U_char / p->Ma) * (p->U_char / p->Ma);
+}
+
+int or_state_alloc(or_state *st, int np) {
+  memset(st, 0, sizeof(*st));
+  st->np = np;
+  double **f[] = {&st->ro, &st->rou, &st->rov, &st->u, &st->v, &st->pxx, &st->pxy,
+                  &st->pyy, &st->cxx, &st->cxy, &st->cyy};
+  for (unsigned k = 0; k < sizeof(f) / sizeof(f[0]); k++) {
+    *f[k] = calloc(np, sizeof(double));
+    if (!*f[k]) return -1;
+  }
+  return 0;
+}
+
+void or_state_free(or_state *st) {
+  free(st->ro); free(st->rou); free(st->rov); free(st->u); free(st->v);
+  free(st->pxx); free(st->pxy); free(st->pyy); free(st->cxx); free(st->cxy); free(st->cyy);
+  memset(st, 0, sizeof(*st));
+}
+
+/* reapply_mirror_bcs, mirror_boundaries.F90:263-352 (no pgrad) */
+void or_reapply_mirror_bcs(const or_domain *d, const or_phys *p, or_state *st) {
+  for (int j = d->npfb; j < d->np; j++) {
+    int i = d->irel[j], vr = d->vrel[j];
+    st->ro[j] = st->ro[i];
+    double su = (vr == 2 || vr == 4) ? -1.0 : 1.0;
+    double sv = (vr == 3 || vr == 4) ? -1.0 : 1.0;
+    st->rou[j] = su * st->rou[i];
+    st->rov[j] = sv * st->rov[i];
+    double sxy = (vr == 2 || vr == 3) ? -1.0 : 1.0;
+    if (p->form == OR_FORM_LC) {
+      st->pxx[j] = st->pxx[i]; st->pxy[j] = sxy * st->pxy[i]; st->pyy[j] = st->pyy[i];
+    } else {
+      st->cxx[j] = st->cxx[i]; st->cxy[j] = sxy * st->cxy[i]; st->cyy[j] = st->cyy[i];
+    }
+  }
+}
+
+static void velocity_from_momentum(const or_domain *d, or_state *st) {
+#pragma omp parallel for
+  for (int i = 0; i < d->np; i++) {            /* step.F90:514-529 */
+    double ooro = 1.0 / st->ro[i];
+    st->u[i] = st->rou[i] * ooro;
+    st->v[i] = st->rov[i] * ooro;
+  }
+}
+
+static void c_from_psi_all(const or_domain *d, or_state *st) {
+#pragma omp parallel for
+  for (int i = 0; i < d->np; i++)              /* step.F90:531-567 */
+    or_c_from_psi(st->pxx[i], st->pxy[i], st->pyy[i], &st->cxx[i], &st->cxy[i], &st->cyy[i]);
+}
+
+void or_set_initial(const or_domain *d, const or_phys *p, or_state *st, const double *ro,
+                    const double *u, const double *v, const double *cxx,
+                    const double *cxy, const double *cyy) {
+  for (int i = 0; i < d->npfb; i++) {
+    st->ro[i] = ro[i]; st->u[i] = u[i]; st->v[i] = v[i];
+    st->cxx[i] = cxx[i]; st->cxy[i] = cxy[i]; st->cyy[i] = cyy[i];
+    if (p->form == OR_FORM_LC)
+      or_psi_from_c(cxx[i], cxy[i], cyy[i], &st->pxx[i], &st->pxy[i], &st->pyy[i]);
+  }
+  for (int i = d->npfb; i < d->np; i++) {       /* setup_flow.F90:260-267 */
+    st->ro[i] = 1.0; st->u[i] = 0; st->v[i] = 0;
+    st->cxx[i] = 1.0; st->cxy[i] = 0.0; st->cyy[i] = 1.0;
+  }
+  for (int i = 0; i < d->np; i++) {             /* setup_flow.F90:74-80 */
+    st->rou[i] = st->ro[i] * st->u[i];
+    st->rov[i] = st->ro[i] * st->v[i];
+  }
+  or_reapply_mirror_bcs(d, p, st);
+  if (p->form == OR_FORM_LC) c_from_psi_all(d, st);
+  for (int i = d->npfb; i < d->np; i++) {       /* setup_flow.F90:125-131 */
+    st->u[i] = st->rou[i] / st->ro[i];
+    st->v[i] = st->rov[i] / st->ro[i];
+  }
+  st->dt = 1.0e-6;                              /* setup_flow.F90:149-151 */
+  st->emax_np1 = PID_TOL; st->emax_n = PID_TOL; st->emax_nm1 = PID_TOL;
+  st->time = 0.0; st->itime = 0;
+}
+
+/* calc_all_rhs + calc_rhs_ro + calc_rhs_rovel + calc_rhs_logconf/calc_rhs_c,
+ * rhs_lc.F90:47-559 and rhs_di.F90:384-476 (interior nodes, 2D) */
+void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, int iRKstep,
+                     double *r_ro, double *r_rou, double *r_rov, double *r_xx,
+                     double *r_xy, double *r_yy) {
+  const int n = d->npfb;
+  double *g[22];
+  for (int k = 0; k < 22; k++) g[k] = malloc(sizeof(double) * n);
+  double *rox = g[0], *roy = g[1], *ux = g[2], *uy = g[3], *vx = g[4], *vy = g[5];
+  double *lapu = g[6], *lapv = g[7], *cxxx = g[8], *cxxy = g[9], *cxyx = g[10], *cxyy = g[11];
+  double *cyyx = g[12], *cyyy = g[13];
+  double *pxxx = g[14], *pxxy = g[15], *pxyx = g[16], *pxyy = g[17], *pyyx = g[18], *pyyy = g[19];
+  double *lcxx = g[20], *tmpl = g[21];
+  /* gradients of primary variables (rhs_lc.F90:64-66) */
+  or_calc_gradient(d, st->ro, rox, roy);
+  or_calc_gradient(d, st->u, ux, uy);
+  or_calc_gradient(d, st->v, vx, vy);
+  /* rhs_rovel (rhs_lc.F90:178-190) */
+  or_calc_laplacian(d, st->u, lapu);
+  or_calc_laplacian(d, st->v, lapv);
+  if (p->form == OR_FORM_LC) {
+    or_calc_gradient_only(d, 1, st->cxx, cxxx);
+    or_calc_gradient(d, st->cxy, cxyx, cxyy);
+    or_calc_gradient_only(d, 2, st->cyy, cyyy);
+  } else {   /* DI: full gradients of cxx, cyy (see header) */
+    or_calc_gradient(d, st->cxx, cxxx, cxxy);
+    or_calc_gradient(d, st->cxy, cxyx, cxyy);
+    or_calc_gradient(d, st->cyy, cyyx, cyyy);
+  }
+  const double csq = p->csq;
+  const double coef_solvent = p->visc_solvent;
+  const double coef_polymeric = p->visc_polymeric / p->lambda;
+#pragma omp parallel for
+  for (int i = 0; i < n; i++) {
+    /* calc_rhs_ro, rhs_lc.F90:127-141 */
+    double tmp_scal = st->u[i] * rox[i] + st->v[i] * roy[i] + 0.0 * 0.0;
+    double divv = ux[i] + vy[i] + 0.0;
+    r_ro[i] = -st->ro[i] * divv - tmp_scal;
+    /* calc_rhs_rovel, rhs_lc.F90:217-272 */
+    double tsu = st->ro[i] * (st->u[i] * ux[i] + st->v[i] * uy[i] + 0.0 * 0.0) - st->u[i] * r_ro[i];
+    double tsv = st->ro[i] * (st->u[i] * vx[i] + st->v[i] * vy[i] + 0.0 * 0.0) - st->v[i] * r_ro[i];
+    double fvu = coef_solvent * lapu[i];
+    double fvv = coef_solvent * lapv[i];
+    fvu = fvu + coef_polymeric * (cxxx[i] + cxyy[i] + 0.0);
+    fvv = fvv + coef_polymeric * (cxyx[i] + cyyy[i] + 0.0);
+    double bfu = st->ro[i] * (p->grav[0] + p->drive[0]);
+    double bfv = st->ro[i] * (p->grav[1] + p->drive[1]);
+    if (p->kol_amp != 0.0) bfu = bfu + p->kol_amp * sin(p->kol_k * d->y[i]);
+    r_rou[i] = -tsu - csq * rox[i] + bfu + fvu;
+    r_rov[i] = -tsv - csq * roy[i] + bfv + fvv;
+  }
+  if (p->form == OR_FORM_LC) {
+    or_calc_gradient(d, st->pxx, pxxx, pxxy);   /* rhs_lc.F90:400-402 */
+    or_calc_gradient(d, st->pxy, pxyx, pxyy);
+    or_calc_gradient(d, st->pyy, pyyx, pyyy);
+    const double eps = p->epsPTT, lam = p->lambda;
+    const int first = (st->itime == 1 && iRKstep == 1);
+#pragma omp parallel for
+    for (int i = 0; i < n; i++) {               /* rhs_lc.F90:414-558 */
+      double u = st->u[i], v = st->v[i];
+      double adxx = -(u * pxxx[i] + v * pxxy[i] + 0.0);
+      double adxy = -(u * pxyx[i] + v * pxyy[i] + 0.0);
+      double adyy = -(u * pyyx[i] + v * pyyy[i] + 0.0);
+      double L[2], R[4];
+      or_eigens2(st->pxx[i], st->pxy[i], st->pyy[i], L, R);
+      L[0] = exp(L[0]); L[1] = exp(L[1]);
+      double tr_c = L[0] + L[1];
+      /* gradu_local(a,b) = d u_a / d x_b */
+      double G11 = ux[i], G12 = uy[i], G21 = vx[i], G22 = vy[i];
+      /* M = R^T (G R) */
+      double GR11 = G11 * R[0] + G12 * R[1], GR12 = G11 * R[2] + G12 * R[3];
+      double GR21 = G21 * R[0] + G22 * R[1], GR22 = G21 * R[2] + G22 * R[3];
+      double M11 = R[0] * GR11 + R[1] * GR21, M12 = R[0] * GR12 + R[1] * GR22;
+      double M21 = R[2] * GR11 + R[3] * GR21, M22 = R[2] * GR12 + R[3] * GR22;
+      double om = L[1] * M12 + L[0] * M21;
+      if (fabs(L[1] - L[0]) <= VERYSMALL) om = 0.0;
+      else om = om / (L[1] - L[0]);
+      /* B = R diag(M11,M22) R^T ; Omega = R [[0,om],[-om,0]] R^T */
+      double B11, B12, B21, B22, O11, O12, O21, O22;
+      {
+        double i11 = M11 * R[0], i12 = M11 * R[1], i21 = M22 * R[2], i22 = M22 * R[3];
+        B11 = R[0] * i11 + R[2] * i21; B12 = R[0] * i12 + R[2] * i22;
+        B21 = R[1] * i11 + R[3] * i21; B22 = R[1] * i12 + R[3] * i22;
+      }
+      {
+        /* inner = Mom * R^T, Mom = [[0,om],[-om,0]] */
+        double i11 = om * R[2], i12 = om * R[3];
+        double i21 = -om * R[0], i22 = -om * R[1];
+        O11 = R[0] * i11 + R[2] * i21; O12 = R[0] * i12 + R[2] * i22;
+        O21 = R[1] * i11 + R[3] * i21; O22 = R[1] * i12 + R[3] * i22;
+      }
+      if (first) {                               /* rhs_lc.F90:503-506 */
+        O11 = O12 = O21 = O22 = 0.0;
+        B11 = 0.5 * (G11 + G11); B12 = 0.5 * (G12 + G21);
+        B21 = 0.5 * (G21 + G12); B22 = 0.5 * (G22 + G22);
+      }
+      double P11 = st->pxx[i], P12 = st->pxy[i], P22 = st->pyy[i];
+      /* UC = Om*Psi - Psi*Om + 2B */
+      double UC11 = (O11 * P11 + O12 * P12) - (P11 * O11 + P12 * O21) + 2.0 * B11;
+      double UC12 = (O11 * P12 + O12 * P22) - (P11 * O12 + P12 * O22) + 2.0 * B12;
+      double UC22 = (O21 * P12 + O22 * P22) - (P12 * O12 + P22 * O22) + 2.0 * B22;
+      /* sPTT relaxation, rhs_lc.F90:538-545 */
+      double fR = 1.0 - 2.0 * eps + eps * tr_c;
+      double C1 = fR / L[0] - fR, C2 = fR / L[1] - fR;
+      double RX11, RX12, RX22;
+      {
+        double i11 = C1 * R[0], i12 = C1 * R[1], i21 = C2 * R[2], i22 = C2 * R[3];
+        RX11 = (R[0] * i11 + R[2] * i21) / lam;
+        RX12 = (R[0] * i12 + R[2] * i22) / lam;
+        RX22 = (R[1] * i12 + R[3] * i22) / lam;
+      }
+      (void)B21;
+      r_xx[i] = adxx + UC11 + RX11;
+      r_xy[i] = adxy + UC12 + RX12;
+      r_yy[i] = adyy + UC22 + RX22;
+    }
+  } else {
+    const double eps = p->epsPTT, lam = p->lambda, Md = p->Mdiff;
+    double *lcxy = NULL, *lcyy = NULL;
+    if (Md != 0.0) {
+      lcxy = malloc(sizeof(double) * n); lcyy = malloc(sizeof(double) * n);
+      or_calc_laplacian(d, st->cxx, lcxx);        /* rhs_di.F90:398-401 */
+      or_calc_laplacian(d, st->cxy, lcxy);
+      or_calc_laplacian(d, st->cyy, lcyy);
+    }
+#pragma omp parallel for
+    for (int i = 0; i < n; i++) {               /* rhs_di.F90:417-475 */
+      double u = st->u[i], v = st->v[i];
+      double cxx = st->cxx[i], cxy = st->cxy[i], cyy = st->cyy[i], czz = 1.0;
+      double adxx = -(u * cxxx[i] + v * cxxy[i] + 0.0);
+      double adxy = -(u * cxyx[i] + v * cxyy[i] + 0.0);
+      double adyy = -(u * cyyx[i] + v * cyyy[i] + 0.0);
+      double ucxx = 2.0 * ux[i] * cxx + 2.0 * uy[i] * cxy + 0.0;
+      double ucxy = ux[i] * cxy + uy[i] * cyy + 0.0 + vx[i] * cxx + vy[i] * cxy + 0.0;
+      double ucyy = 2.0 * vx[i] * cxy + 2.0 * vy[i] * cyy + 0.0;
+      double fr = -(1.0 - eps * 2.0 + eps * (cxx + cyy + czz)) / lam;
+      double sxx = fr * (cxx - 1.0), sxy = fr * cxy, syy = fr * (cyy - 1.0);
+      double mxx = 0, mxy = 0, myy = 0;
+      if (Md != 0.0) { mxx = Md * lcxx[i]; mxy = Md * lcxy[i]; myy = Md * lcyy[i]; }
+      r_xx[i] = adxx + ucxx + sxx + mxx;
+      r_xy[i] = adxy + ucxy + sxy + mxy;
+      r_yy[i] = adyy + ucyy + syy + myy;
+    }
+    free(lcxy); free(lcyy);
+  }
+  for (int k = 0; k < 22; k++) free(g[k]);
+  (void)tmpl;
+}
+
+/* filter_variables, rhs_lc.F90:818-893 / rhs_di.F90:667-720 (mcorr on) */
+void or_filter_variables(const or_domain *d, const or_phys *p, or_state *st) {
+  or_calc_filtered_var(d, st->ro);
+  or_calc_filtered_var(d, st->rou);
+  or_calc_filtered_var(d, st->rov);
+  if (p->form == OR_FORM_LC) {
+    or_calc_filtered_var(d, st->pxx);
+    or_calc_filtered_var(d, st->pxy);
+    or_calc_filtered_var(d, st->pyy);
+  } else {
+    or_calc_filtered_var(d, st->cxx);
+    or_calc_filtered_var(d, st->cxy);
+    or_calc_filtered_var(d, st->cyy);
+  }
+  double tm = 0, tv = 0;
+  for (int i = 0; i < d->npfb; i++) {
+    tm = tm + (st->ro[i] - p->rho_char) * d->vol[i];
+    tv = tv + d->vol[i];
+  }
+  double dro = tm / tv;
+  for (int i = 0; i < d->npfb; i++) st->ro[i] = st->ro[i] - dro;
+}
+
+/* set_tstep, step.F90:569-622 */
+void or_set_tstep(const or_domain *d, const or_phys *p, or_state *st) {
+  double dt_cfl = 1.0e10, dt_visc = 1.0e10;
+  const double c = sqrt(p->csq);
+  for (int i = 0; i < d->npfb; i++) {
+    double umag = sqrt(st->u[i] * st->u[i] + st->v[i] * st->v[i] + 0.0);
+    double upc = umag + c;
+    if (d->node_type[i] == 2) dt_cfl = fmin(dt_cfl, 0.8 * d->s[i] / upc);
+    else dt_cfl = fmin(dt_cfl, d->s[i] / upc);
+    dt_visc = fmin(dt_visc, d->s[i] * d->s[i] * st->ro[i] / p->visc_total);
+  }
+  st->dt_cfl = 1.0 * dt_cfl * p->L_char;
+  st->dt_par = (2.0 / 3.0) * dt_visc * p->L_char * p->L_char;
+}
+
+/* set_tstep_PID, step.F90:624-682 */
+void or_set_tstep_pid(or_state *st) {
+  double facA = PID_A * log(PID_TOL / st->emax_np1);
+  double facB = PID_B * log(st->emax_n / PID_TOL);
+  double facC = PID_C * log(PID_TOL / st->emax_nm1);
+  double f = PID_K * exp(facA + facB + facC);
+  f = fmax(0.5, f);
+  f = fmin(1.02, f);
+  st->dt = st->dt * f;
+  double dt_max = fmin(st->dt_cfl, st->dt_par);
+  if (st->dt > dt_max) st->dt = dt_max;
+}
+
+/* step_rk3_4S_2R_EE, step.F90:218-512 */
+void or_step_rk3(const or_domain *d, const or_phys *p, or_state *st) {
+  const int n = d->npfb;
+  const double a[3] = {RKA1, RKA2, RKA3};
+  const double b[4] = {RKB1, RKB2, RKB3, RKB4};
+  const double bh[4] = {RKBH1, RKBH2, RKBH3, RKBH4};
+  double RKa[3], RKb[4], RKbmbh[4], RKc[4];
+  const double cc[4] = {0.0, RKA1, RKA2 + RKB1, 1.0};
+  st->emax_nm1 = st->emax_n; st->emax_n = st->emax_np1;     /* :240 */
+  for (int k = 0; k < 3; k++) RKa[k] = st->dt * a[k];
+  for (int k = 0; k < 4; k++) { RKb[k] = st->dt * b[k]; RKbmbh[k] = st->dt * (b[k] - bh[k]); RKc[k] = st->dt * cc[k]; }
+  double *reg[6], *rhs[6], *eacc[6];
+  for (int f = 0; f < 6; f++) {
+    reg[f] = malloc(sizeof(double) * n);
+    rhs[f] = malloc(sizeof(double) * n);
+    eacc[f] = calloc(n, sizeof(double));
+  }
+  double *U[6];
+  U[0] = st->ro; U[1] = st->rou; U[2] = st->rov;
+  if (p->form == OR_FORM_LC) { U[3] = st->pxx; U[4] = st->pxy; U[5] = st->pyy; }
+  else { U[3] = st->cxx; U[4] = st->cxy; U[5] = st->cyy; }
+  for (int f = 0; f < 6; f++) memcpy(reg[f], U[f], sizeof(double) * n);
+  const double time0 = st->time;
+  for (int k = 0; k < 3; k++) {                                /* :281-359 */
+    st->time = time0 + RKc[k];
+    or_calc_all_rhs(d, p, st, k + 1, rhs[0], rhs[1], rhs[2], rhs[3], rhs[4], rhs[5]);
+#pragma omp parallel for
+    for (int i = 0; i < n; i++)
+      for (int f = 0; f < 6; f++) {
+        U[f][i] = reg[f][i] + RKa[k] * rhs[f][i];
+        reg[f][i] = reg[f][i] + RKb[k] * rhs[f][i];
+        eacc[f][i] = eacc[f][i] + RKbmbh[k] * rhs[f][i];
+      }
+    or_reapply_mirror_bcs(d, p, st);
+    velocity_from_momentum(d, st);
+    if (p->form == OR_FORM_LC) c_from_psi_all(d, st);
+  }
+  st->time = time0 + RKc[3];
+  or_calc_all_rhs(d, p, st, 4, rhs[0], rhs[1], rhs[2], rhs[3], rhs[4], rhs[5]);
+  double en[6] = {0, 0, 0, 0, 0, 0};
+  const double norm[6] = {1.0, 1.0 / p->U_char, 1.0 / p->U_char, 1.0, 1.0, 1.0};
+  for (int i = 0; i < n; i++)                                  /* :375-460 (serial) */
+    for (int f = 0; f < 6; f++) {
+      U[f][i] = reg[f][i] + RKb[3] * rhs[f][i];
+      eacc[f][i] = eacc[f][i] + RKbmbh[3] * rhs[f][i];
+      en[f] = fmax(en[f], fabs(eacc[f][i]) * norm[f]);
+    }
+  double emax_conf = fmax(en[3], fmax(en[4], en[5]));
+  st->emax_np1 = fmax(emax_conf, fmax(en[0], fmax(en[1], fmax(en[2], DOUBLESMALL))));
+  for (int f = 0; f < 6; f++) { free(reg[f]); free(rhs[f]); free(eacc[f]); }
+  or_reapply_mirror_bcs(d, p, st);                             /* :489 */
+  or_filter_variables(d, p, st);                               /* :495 */
+  or_reapply_mirror_bcs(d, p, st);                             /* :499 */
+  velocity_from_momentum(d, st);
+  if (p->form == OR_FORM_LC) c_from_psi_all(d, st);
+}
+
+void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps) {
+  for (int s = 0; s < nsteps; s++) {                           /* sunset.F90:75-106 */
+    or_set_tstep(d, p, st);
+    if (p->dt_mode == OR_DT_PID) or_set_tstep_pid(st);
+    or_step_rk3(d, p, st);
+    st->itime++;
+  }
+}

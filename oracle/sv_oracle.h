@@ -1,0 +1,135 @@
+/*
+ * sv_oracle.h -- FP64 CPU restatement of the SUNSET LABFM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in the product (sunset-viscoelastic_amd/) links or calls it.
+ *
+ * Every routine restates the reference algorithm (file:line given per function
+ * in sv_oracle.c) with the reference's loop structure: one OpenMP sweep per
+ * derivative call, ghost ("mirror") nodes stored explicitly at ids >= npfb,
+ * neighbour lists ij_link[i*nplink + k] sorted by distance.  Indices are
+ * 0-based here (the reference is 1-based).
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   or_create_ghosts     -- bit-exact vs the flang-built reference (oracle/_ref)
+ *   or_find_neighbours   -- bit-exact vs oracle/_ref and vs brute force O(N^2)
+ *   or_wab/monomials/abfs-- closed forms (Wendland C2, Hermite recurrence, Taylor)
+ *   or_svd_solve         -- LAPACK (numpy) on random systems; oracle/_ref weights
+ *   or_labf_weights etc. -- oracle/_ref to 1e-10; moment conditions; polynomial
+ *                           exactness
+ *   stencil operators    -- dense matvec identity; oracle/_ref derivatives
+ *   or_calc_all_rhs (LC) -- oracle/_ref fields after N RK3 steps
+ *   or_calc_all_rhs (DI) -- closed forms only (reference DI reads uninitialised
+ *                           memory, rhs_di.F90:188-190,421-424); parity for the
+ *                           DI form is pinned by relaxation/shear closed forms
+ *                           and LC<->DI consistency, not by the reference.
+ *   or_step_rk3          -- oracle/_ref (LC); Butcher order conditions
+ */
+#ifndef SV_ORACLE_H
+#define SV_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_FORM_DI = 0, OR_FORM_LC = 1 };
+enum { OR_DT_PID = 0, OR_DT_FIXED = 1 };
+
+/* Geometry + stencils: the state the reference keeps in common_vars.F90:51-95 */
+typedef struct {
+  int npfb, np, nplink, cap;     /* real nodes, real+ghost nodes, max links, alloc */
+  double *x, *y, *h, *hs, *s;    /* [cap] positions, h, h_small, spacing */
+  int *node_type, *irel, *vrel;  /* [cap]; irel/vrel valid for ghosts */
+  double xmin, xmax, ymin, ymax, smax, h0, h2;
+  int bc[4];                     /* xL xU yL yU: 0 none 1 periodic 2 symmetric 3 no-slip */
+  int morder, forder;
+  int *ij_count, *ij_count_small, *ij_link; /* [npfb], [npfb], [npfb*nplink] */
+  double *w_grad;                /* [npfb][nplink][2] */
+  double *w_lap, *w_hyp;         /* [npfb][nplink] */
+  double *gsum, *lsum, *hsum, *vol; /* [npfb][2], [npfb], [npfb], [npfb] */
+  double L_char;
+} or_domain;
+
+typedef struct {
+  double L_char, U_char, Re, Wi, beta, epsPTT, Ma, Mdiff;
+  double grav[3], drive[3];
+  double kol_amp, kol_k;         /* f_x = kol_amp*sin(kol_k*y); not in reference LC/DI */
+  int form, dt_mode;
+  /* derived as load_data.F90:132-148 */
+  double rho_char, visc_total, visc_solvent, visc_polymeric, lambda, csq;
+} or_phys;
+
+typedef struct {
+  int np;
+  double *ro, *rou, *rov, *u, *v;            /* [np] */
+  double *pxx, *pxy, *pyy;                   /* log-conformation psi [np] (LC) */
+  double *cxx, *cxy, *cyy;                   /* conformation [np] */
+  double dt, time, dt_cfl, dt_par;
+  double emax_np1, emax_n, emax_nm1;
+  int itime;
+} or_state;
+
+/* ---- domain ---- */
+int  or_domain_init(or_domain *d, int npfb, const double *x, const double *y,
+                    const double *s, const double *h, const double *hs,
+                    const int *node_type, double xmin, double xmax, double ymin,
+                    double ymax, const int bc[4], double smax, int morder,
+                    int forder, double L_char);
+void or_domain_free(or_domain *d);
+int  or_create_ghosts(or_domain *d);
+int  or_find_neighbours(or_domain *d, int nplink);
+int  or_calc_labf_weights(or_domain *d);     /* grad, lap, hyp(raw), vol */
+void or_calc_labf_sums(or_domain *d);
+void or_filter_coefficients(or_domain *d);
+/* per-node order map (nullable): n-system size chosen per node (config C3) */
+int  or_calc_labf_weights_ordermap(or_domain *d, const int *order_of_node,
+                                   int nthreads);
+
+/* ---- kernels / bases ---- */
+double or_wab(double q, double h2);
+void   or_monomials(int order, double x, double y, double *out);
+void   or_abfs(int order, double x, double y, double *out);
+int    or_svd_solve(int n, const double *A, double *b);  /* A column-major n*n */
+void   or_eigens2(double a, double b, double d, double l[2], double R[4]);
+void   or_c_from_psi(double pxx, double pxy, double pyy, double *cxx, double *cxy,
+                     double *cyy);
+void   or_psi_from_c(double cxx, double cxy, double cyy, double *pxx, double *pxy,
+                     double *pyy);
+
+/* ---- stencil operators (phi sized np, outputs sized npfb) ---- */
+void or_calc_gradient(const or_domain *d, const double *phi, double *gx, double *gy);
+void or_calc_laplacian(const or_domain *d, const double *phi, double *lap);
+void or_calc_gradient_only(const or_domain *d, int dim, const double *phi, double *g);
+void or_calc_filtered_var(const or_domain *d, double *phi);
+void or_calc_filter_term(const or_domain *d, const double *phi, double *out);
+
+/* ---- physics ---- */
+void or_phys_derive(or_phys *p);
+int  or_state_alloc(or_state *st, int np);
+void or_state_free(or_state *st);
+/* Initial conditions as hardcode_initial_conditions/initial_solution
+ * (setup_flow.F90:24-173): ro,u,v,c given on real nodes; psi=log(c). */
+void or_set_initial(const or_domain *d, const or_phys *p, or_state *st,
+                    const double *ro, const double *u, const double *v,
+                    const double *cxx, const double *cxy, const double *cyy);
+void or_reapply_mirror_bcs(const or_domain *d, const or_phys *p, or_state *st);
+/* calc_all_rhs: outputs rhs_* [npfb]; iRKstep as in rhs_lc.F90:503 */
+void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st,
+                     int iRKstep, double *r_ro, double *r_rou, double *r_rov,
+                     double *r_xx, double *r_xy, double *r_yy);
+void or_filter_variables(const or_domain *d, const or_phys *p, or_state *st);
+void or_set_tstep(const or_domain *d, const or_phys *p, or_state *st);
+void or_set_tstep_pid(or_state *st);
+void or_step_rk3(const or_domain *d, const or_phys *p, or_state *st);
+/* time loop as sunset.F90:75-106 (set_tstep, PID unless fixed dt, step, itime++) */
+void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps);
+
+int  or_num_threads(void);
+void or_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

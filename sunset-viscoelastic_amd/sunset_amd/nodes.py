@@ -1,0 +1,138 @@
+"""Node sets, the IPART text format and the benchmark configurations.
+
+The IPART format is the reference's node-set file (read by
+``setup_domain.F90:93-199``, written by ``source/gen/datclass.F90:734-745``)::
+
+    nb npfb smax smin
+    xmin xmax ymin ymax
+    xbcL xbcU ybcL ybcU            (0 none, 1 periodic, 2 symmetric, 3 no-slip)
+    [nprocsX nprocsY nprocsZ / nprocs / nprocs lines "start end"]   (optional)
+    gid x y type nx ny s           (one line per node; type 999 = fluid)
+
+Node sets here are seeded jittered lattices (numpy PCG64, seed recorded in
+``NodeSet.meta``), written in x-major order like the reference generator's
+x-sorted output (``datclass.F90:718``).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+# stencil-size multipliers h = hovs*s per order, common_parameter.F90:45-56
+HOVS = {4: 1.8, 6: 2.4, 8: 2.7, 10: 3.1}
+
+
+@dataclasses.dataclass
+class NodeSet:
+    x: np.ndarray
+    y: np.ndarray
+    s: np.ndarray
+    h: np.ndarray
+    h_small: np.ndarray
+    node_type: np.ndarray
+    xmin: float
+    xmax: float
+    ymin: float
+    ymax: float
+    bc: tuple = (1, 1, 1, 1)
+    smax: float = 0.0
+    meta: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def npfb(self) -> int:
+        return int(self.x.shape[0])
+
+
+def jittered_lattice(nx: int, ny: Optional[int] = None, *, seed: int = 0,
+                     jitter: float = 0.3, h_over_s: float = 1.3,
+                     hs_over_s: Optional[float] = None, box=(0.0, 1.0, 0.0, 1.0),
+                     bc=(1, 1, 1, 1)) -> NodeSet:
+    """Periodic jittered lattice: x = (i+0.5)s + U(-jitter, jitter)s."""
+    ny = nx if ny is None else ny
+    x0, x1, y0, y1 = box
+    sx = (x1 - x0) / nx
+    sy = (y1 - y0) / ny
+    s = min(sx, sy)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ii = ii.ravel()
+    jj = jj.ravel()
+    jx = rng.uniform(-jitter, jitter, ii.shape[0])
+    jy = rng.uniform(-jitter, jitter, ii.shape[0])
+    x = x0 + (ii + 0.5) * sx + jx * sx
+    y = y0 + (jj + 0.5) * sy + jy * sy
+    n = x.shape[0]
+    sv = np.full(n, s)
+    h = sv * h_over_s
+    hs = sv * (h_over_s if hs_over_s is None else hs_over_s)
+    return NodeSet(x=x, y=y, s=sv, h=h, h_small=hs,
+                   node_type=np.full(n, 999, dtype=np.int32),
+                   xmin=x0, xmax=x1, ymin=y0, ymax=y1, bc=tuple(bc), smax=s,
+                   meta=dict(generator="PCG64", seed=seed, nx=nx, ny=ny,
+                             jitter=jitter, h_over_s=h_over_s))
+
+
+def write_ipart(path: str, ns: NodeSet, nprocs_schedule: bool = False) -> None:
+    """Write the IPART text file (datclass.F90:734-745 layout)."""
+    nb = int(np.count_nonzero((ns.node_type >= 0) & (ns.node_type <= 2)))
+    with open(path, "w") as f:
+        f.write(f"{nb} {ns.npfb} {ns.smax!r} {float(ns.s.min())!r}\n")
+        f.write(f"{ns.xmin!r} {ns.xmax!r} {ns.ymin!r} {ns.ymax!r}\n")
+        f.write(" ".join(str(int(b)) for b in ns.bc) + "\n")
+        if nprocs_schedule:
+            f.write("1 1 1\n1\n")
+            f.write(f"1 {ns.npfb}\n")
+        for i in range(ns.npfb):
+            f.write(f"{i + 1} {ns.x[i]!r} {ns.y[i]!r} {int(ns.node_type[i])} "
+                    f"0.0 0.0 {ns.s[i]!r}\n")
+
+
+def read_ipart(path: str, order: int = 8) -> NodeSet:
+    """Read an IPART file (with or without a decomposition schedule).
+
+    h = hovs*s as setup_domain.F90:180 (hovs for the given order), h_small = h.
+    Boundary nodes (types 0-2) are read but their FD rows are not generated.
+    """
+    with open(path) as f:
+        lines = [ln.split() for ln in f if ln.strip()]
+    nb, npfb = int(lines[0][0]), int(lines[0][1])
+    smax = float(lines[0][2].replace("D", "E").replace("d", "e"))
+    xmin, xmax, ymin, ymax = (float(v) for v in lines[1][:4])
+    bc = tuple(int(v) for v in lines[2][:4])
+    k = 3
+    # optional schedule: a 3-int line, then nprocs, then nprocs pairs
+    if len(lines[k]) == 3 and all(_isint(v) for v in lines[k]):
+        nprocs = int(lines[k + 1][0])
+        k += 2 + nprocs
+    rows = lines[k:k + npfb]
+    arr = np.array([[float(v) for v in r[:7]] for r in rows])
+    x, y, typ, s = arr[:, 1], arr[:, 2], arr[:, 3].astype(np.int32), arr[:, 6]
+    h = s * HOVS.get(order, 2.7)
+    return NodeSet(x=x, y=y, s=s, h=h, h_small=h.copy(), node_type=typ,
+                   xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, bc=bc, smax=smax,
+                   meta=dict(source=path, nb=nb))
+
+
+def _isint(v: str) -> bool:
+    try:
+        int(v)
+        return True
+    except ValueError:
+        return False
+
+
+def nplink_default(order: int) -> int:
+    """initial_setup's neighbour-list capacity, setup_domain.F90:50."""
+    return 2 * 4 * int(math.ceil(2.0 * HOVS.get(order, 2.7))) ** 2
+
+
+def taylor_green_ic(ns: NodeSet):
+    """u = sin(2 pi x)cos(2 pi y), v = -cos(2 pi x) sin(2 pi y), rho = 1, c = I."""
+    n = ns.npfb
+    u = np.sin(2 * np.pi * ns.x) * np.cos(2 * np.pi * ns.y)
+    v = -np.cos(2 * np.pi * ns.x) * np.sin(2 * np.pi * ns.y)
+    return dict(ro=np.ones(n), u=u, v=v, cxx=np.ones(n), cxy=np.zeros(n),
+                cyy=np.ones(n))

@@ -1,0 +1,148 @@
+"""The CPU oracle against the reference: committed golden vectors (always) and
+the flang-built reference in oracle/_ref (when present)."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+import refdrv
+from parity_util import (FIELD_RTOL, WEIGHT_RTOL, csr_from_rows, field_rel, golden_nodeset,
+                         golden_phys, ipart_nodeset, load_golden, per_node_rel, taylor_green_ic)
+
+
+@pytest.fixture(scope="module", params=["c1_m4", "m4_h18", "m6_20"])
+def case(request):
+    g = load_golden(request.param)
+    ns = golden_nodeset(g)
+    o = OL.OracleSim(ns, int(g["order"])).setup()
+    return g, ns, o
+
+
+def test_ghosts_and_neighbours_bit_exact(case):
+    g, ns, o = case
+    x, y = o.xy()
+    n = ns.npfb
+    assert np.array_equal(x[n:], g["gx"]) and np.array_equal(y[n:], g["gy"])
+    assert np.array_equal(o.irel()[n:], g["irel"])
+    cnt, cs, link = o.lists()
+    assert np.array_equal(cnt, g["ij_count"]) and np.array_equal(cs, g["ij_count_small"])
+    ptr, col, _ = csr_from_rows(cnt, link, x, y, x[:n], y[:n])
+    assert np.array_equal(ptr, g["ptr"]) and np.array_equal(col, g["col"])
+
+
+def test_weights_match_reference(case):
+    g, ns, o = case
+    x, y = o.xy()
+    n = ns.npfb
+    cnt, cs, link = o.lists()
+    W = o.weight_arrays()
+    ptr, col, (wx, wy, wl, wh) = csr_from_rows(
+        cnt, link, x, y, x[:n], y[:n],
+        [W["w_grad"][:, :, 0], W["w_grad"][:, :, 1], W["w_lap"], W["w_hyp"]])
+    for a, b in ((wx, g["wx"]), (wy, g["wy"]), (wl, g["wl"]), (wh, g["wh"])):
+        assert per_node_rel(ptr, a, b) < WEIGHT_RTOL
+    assert np.array_equal(W["vol"], g["vol"]) or np.allclose(W["vol"], g["vol"], rtol=1e-14)
+    # weight sums cancel: scale each node's error by its sum of |w|
+    absx = np.add.reduceat(np.abs(g["wx"]), ptr[:-1])
+    absy = np.add.reduceat(np.abs(g["wy"]), ptr[:-1])
+    absl = np.add.reduceat(np.abs(g["wl"]), ptr[:-1])
+    absh = np.add.reduceat(np.abs(g["wh"]), ptr[:-1])
+    assert (np.abs(W["gsum"][:, 0] - g["gsum"][:, 0]) / absx).max() < WEIGHT_RTOL
+    assert (np.abs(W["gsum"][:, 1] - g["gsum"][:, 1]) / absy).max() < WEIGHT_RTOL
+    assert (np.abs(W["lsum"] - g["lsum"]) / absl).max() < WEIGHT_RTOL
+    assert (np.abs(W["hsum"] - g["hsum"]) / absh).max() < WEIGHT_RTOL
+
+
+def test_operators_match_reference(case):
+    g, ns, o = case
+    ro = o.extend(np.ones(ns.npfb))
+    gx, gy = o.gradient(ro)
+    # grad of a constant: both are sums of weights minus their stored sum
+    scale = np.abs(o.weight_arrays()["gsum"]).max()
+    assert np.abs(gx - g["grad_ro_x"]).max() < 1e-12 * max(scale, 1.0)
+    lap = o.laplacian(ro)
+    assert np.abs(lap - g["lap_ro"]).max() < 1e-12 * max(np.abs(o.weight_arrays()["lsum"]).max(), 1)
+
+
+def test_fields_after_steps_match_reference(case):
+    g, ns, o = case
+    ph = golden_phys(g)
+    o.make_phys(form="lc", **ph)
+    o.set_initial(taylor_green_ic(ns))
+    o.run(int(g["nsteps"]))
+    F = o.fields()
+    assert F["itime"] == int(g["itime"])
+    for k in ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy"):
+        assert field_rel(F[k], g[f"f_{k}"]) < FIELD_RTOL, k
+    ctl = g["ctl"]
+    assert abs(F["dt"] - ctl[0]) <= 1e-10 * ctl[0]
+    assert abs(F["time"] - ctl[1]) <= 1e-10 * ctl[1]
+    assert abs(F["emax_np1"] - ctl[2]) <= 1e-3 * ctl[2] + 1e-15
+
+
+@pytest.mark.skipif(not refdrv.available(4), reason="oracle/_ref not built")
+def test_ipart_fixture_neighbours_and_weights_vs_live_reference():
+    ns = ipart_nodeset(order=4)
+    o = OL.OracleSim(ns, 4).setup()
+    ic = dict(ro=np.ones(ns.npfb), u=np.zeros(ns.npfb), v=np.zeros(ns.npfb),
+              cxx=np.ones(ns.npfb), cxy=np.zeros(ns.npfb), cyy=np.ones(ns.npfb))
+    ref, _ = refdrv.run_reference(ns, 4, ic, nsteps=0)
+    x, y = o.xy()
+    n = ns.npfb
+    assert np.array_equal(x, ref["x"]) and np.array_equal(y, ref["y"])
+    cnt, cs, link = o.lists()
+    assert np.array_equal(cnt, ref["ij_count"]) and np.array_equal(cs, ref["ij_count_small"])
+    W = o.weight_arrays()
+    ptr, col, (wx, wl) = csr_from_rows(cnt, link, x, y, x[:n], y[:n],
+                                       [W["w_grad"][:, :, 0], W["w_lap"]])
+    rptr, rcol, (rwx, rwl) = csr_from_rows(ref["ij_count"], ref["ij_link"], x, y, x[:n], y[:n],
+                                           [ref["w_grad"][:, :, 0], ref["w_lap"]])
+    assert np.array_equal(col, rcol)
+    assert per_node_rel(ptr, wx, rwx) < WEIGHT_RTOL
+    assert per_node_rel(ptr, wl, rwl) < WEIGHT_RTOL
+
+
+@pytest.mark.skipif(not refdrv.available(8), reason="oracle/_ref not built")
+def test_order8_oracle_vs_extended_precision_and_reference():
+    """Order 8: the reference's Golub-Reinsch SVD carries ~1e-8 error; the
+    oracle's Jacobi SVD is pinned against a 40-digit solve of the same system."""
+    mp = pytest.importorskip("mpmath")
+    from math import factorial
+    from sunset_amd.nodes import jittered_lattice
+    mp.mp.dps = 40
+    ns = jittered_lattice(18, seed=3, h_over_s=2.7)
+    o = OL.OracleSim(ns, 8).setup()
+    x, y = o.xy()
+    cnt, cs, link = o.lists()
+    W = o.weight_arrays()
+    idx = [(d - b, b) for d in range(1, 9) for b in range(d + 1)]
+
+    def he(k, z):
+        a, b = mp.mpf(1), z
+        if k == 0:
+            return a
+        for m in range(1, k):
+            a, b = b, z * b - m * a
+        return b
+    for i in (0, 101, 250):
+        hh = mp.mpf(ns.h_small[i])
+        js = link[i, :cs[i]]
+        A = mp.zeros(len(idx))
+        G = []
+        for j in js:
+            X = (mp.mpf(x[j]) - mp.mpf(x[i])) / hh
+            Y = (mp.mpf(y[j]) - mp.mpf(y[i])) / hh
+            q = mp.sqrt(X * X + Y * Y)
+            w = (2 * q + 1) * (1 - q / 2) ** 4 if 0 < q <= 2 else mp.mpf(0)
+            gv = [he(a, X) * he(b, Y) * w for a, b in idx]
+            G.append(gv)
+            mv = [X ** a * Y ** b / (factorial(a) * factorial(b)) for a, b in idx]
+            for r in range(len(idx)):
+                for c in range(len(idx)):
+                    A[r, c] += mv[r] * gv[c]
+        rhs = mp.zeros(len(idx), 1)
+        rhs[0] = 1
+        psi = mp.lu_solve(A, rhs)
+        wex = np.array([float(sum(psi[a] * G[k][a] for a in range(len(idx))) / hh)
+                        for k in range(len(js))])
+        wo = W["w_grad"][i, :len(js), 0]
+        assert np.abs(wo - wex).max() / np.abs(wex).max() < WEIGHT_RTOL
