@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark: RK3 node-steps/s in FP64 on the headline workload.
+
+Workload (one MI355X): 1024 x 1024 seeded jittered periodic lattice (~1M
+nodes) on the unit square, LABFM order 6, h = h_small = 2.4 s, log-conformation
+sPTT (eps = 0.25, Wi = 2, beta = 0.1, Re = 1, Ma = 0.1), Kolmogorov forcing
+f_x = sin(4 pi y), PID time step from dt = 1e-6 as the reference starts.  A
+step is one full RK3(2)4[2R+]C step: 4 right-hand sides + stage updates,
+filter, mass correction, time-step control, all on the device with inputs
+resident in HBM.
+
+Printed line (one JSON object): value = node-steps/s (nodes x steps / time of
+the timed steps); roofline of the dominant kernel (the RK stage sweep)
+measured live with HIP events on the library's stream; weight solves/s of the
+device weight setup; cpu_baseline = the reference's own OpenMP RK3 loop
+(flang build in oracle/_ref) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sunset-viscoelastic_amd"))
+
+METRIC = "RK3 node-steps/s FP64 @1M nodes; weight solves/s; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+PHYS = dict(Re=1.0, Wi=2.0, beta=0.1, eps=0.25, Ma=0.1, kol_amp=1.0, kol_k=4.0 * math.pi)
+
+
+def stage_bytes_per_node(ks_mean):
+    """Algorithmic HBM bytes per node of the four LC stage sweeps (mean):
+    28 B per link (int32 index + 3 FP64 weights) + own-node traffic
+    (stage 1: 316 B, stages 2-3: 364 B, stage 4: 244 B)."""
+    return 28.0 * ks_mean + (316.0 + 364.0 + 364.0 + 244.0) / 4.0
+
+
+def step_bytes_per_node(ks_mean, k_mean):
+    """Whole step: 4 stages + filter (12 B per link + 212 B) + apply (128 B)."""
+    return 4 * 28.0 * ks_mean + 1288.0 + 12.0 * k_mean + 212.0 + 128.0
+
+
+def cpu_baseline(n_sample, steps, threads):
+    """The reference's RK3 loop (oracle/_ref, flang/OpenMP) on an n_sample^2
+    lattice of the same workload, its neighbour lists and weights loaded from
+    this library's parity-checked ones (the reference's serial weight setup
+    would take minutes).  Returns node-steps/s from the median step time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import refdrv
+    import sunset_amd
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+    if not refdrv.available(6):
+        return None
+    ns = jittered_lattice(n_sample, seed=2, h_over_s=2.4)
+    s = sunset_amd.Solver(morder=6, form="lc")
+    s.set_nodes(ns)
+    cnt, cs, link = s.find_neighbours()
+    s.calc_weights()
+    s.calc_sums()
+    s.filter_coefficients()
+    W = s.weights(link.shape[1])
+    s.close()
+    st = dict(nplink=link.shape[1], ij_count=cnt, ij_count_small=cs, ij_link=link, **W)
+    ic = taylor_green_ic(ns)
+    # the reference's LC/DI right-hand sides have no Kolmogorov line (rhs_lc.F90:263);
+    # the forcing is one multiply-add per node and does not change the timing
+    phys = {k: v for k, v in PHYS.items() if not k.startswith("kol")}
+    _, times = refdrv.run_reference(ns, 6, ic, nsteps=steps + 1, phys=phys, stencil=st,
+                                    fields_only=True, threads=threads, timeout=900)
+    ts = sorted(times["steps"][1:])
+    med = ts[len(ts) // 2]
+    return dict(value=ns.npfb / med, unit="node-steps/s", cores=threads, kind="reference",
+                sample=f"{n_sample}x{n_sample} lattice ({ns.npfb} nodes), same order/physics, "
+                       f"{steps} timed steps after 1 warm-up, median step "
+                       f"{med * 1e3:.1f} ms, OMP_NUM_THREADS={threads}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024, help="lattice side (n^2 nodes)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--graph", action="store_true", help="time hipGraph replay instead")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import sunset_amd
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+
+    ns = jittered_lattice(args.n, seed=2, h_over_s=2.4)
+    print(f"[bench] {ns.npfb} nodes: setup", file=sys.stderr, flush=True)
+    s = sunset_amd.Solver(morder=6, form="lc")
+    s.set_nodes(ns)
+    s.find_neighbours(copy=False)
+    t_neigh = s.timing("neighbours")
+    s.calc_weights()
+    t_w = s.timing("weights")
+    n_sys = s.timing("systems")
+    s.calc_sums()
+    s.filter_coefficients()
+    st = s.stencil_stats()
+    s.set_phys(**PHYS)
+    s.set_initial(taylor_green_ic(ns))
+    print(f"[bench] neighbours {t_neigh:.1f} ms, weights {t_w:.1f} ms "
+          f"({n_sys:.0f} systems), K={st['K_mean']:.2f}; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    s.run(args.warmup, graph=args.graph)
+    s.fields()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    if args.graph:
+        t0 = time.perf_counter()
+        s.run(args.steps, graph=True)
+        total_ms = (time.perf_counter() - t0) * 1e3
+        kern = None
+    else:
+        kern, total_ms = s.run_profiled(args.steps)
+    F = s.fields()
+    if not all(np.isfinite(F[k]).all() for k in ("ro", "u", "v", "pxx", "pxy", "pyy")):
+        raise SystemExit("bench.py: non-finite state after the timed steps")
+    n = ns.npfb
+    ms_step = total_ms / args.steps
+    value = n * args.steps / (total_ms / 1e3)
+
+    roof = None
+    if kern is not None:
+        t_stage = np.mean([kern[k] for k in ("stage1", "stage2", "stage3", "stage4")])
+        b_stage = stage_bytes_per_node(st["Ks_mean"]) * n
+        ach = b_stage / (t_stage / 1e3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_stage (RK stage sweep, mean of stages 1-4)",
+                    bytes_per_launch=int(b_stage), ms_per_launch=round(t_stage, 4))
+    step_gbs = step_bytes_per_node(st["Ks_mean"], st["K_mean"]) * n / (ms_step / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        print(f"[bench] cpu baseline: reference on {args.cpu_n}^2, {threads} threads",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "node-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered lattice, Taylor-Green initial velocity)",
+        "config": {"workload": "1024^2 jittered periodic lattice, LABFM order 6, h=2.4s, "
+                               "log-conformation sPTT eps=0.25 Wi=2 beta=0.1 Re=1 Ma=0.1, "
+                               "Kolmogorov f_x=sin(4 pi y), PID dt",
+                   "nodes": n, "K_mean": round(st["K_mean"], 2), "order": 6,
+                   "timing": "hipGraph replay" if args.graph else "eager, HIP events"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "step_hbm_gbs": round(step_gbs, 1),
+        "weight_solves_per_s": round(n / (t_w / 1e3), 1),
+        "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
+        "adjoint_forward_ratio": None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * sunset_amd.h -- C ABI of the MI355X (gfx950) LABFM hot-path library
+ * (libsunset_amd.so).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative SV_E_* code;
+ *    sv_last_error(ctx) gives the message.  Host pointers are never retained
+ *    past a call.  One host thread per context; calls are not reentrant.
+ *  - Host arrays use the reference's layouts: node arrays are indexed by node
+ *    (0-based here), neighbour lists are ij_link(nplink, npfb) column-major
+ *    (node-contiguous), 1-based, ghost ids > npfb (common_vars.F90:87-89), and
+ *    weights are ij_w_grad(2, nplink, npfb), ij_w_lap / ij_w_hyp(nplink, npfb)
+ *    (common_vars.F90:92-95).  Any output pointer may be NULL (not copied).
+ *  - Entry points ending in _dev take device pointers and a hipStream_t passed
+ *    as void* (NULL = the context's stream); they neither allocate nor copy.
+ *  - All arithmetic is FP64.
+ */
+#ifndef SUNSET_AMD_H
+#define SUNSET_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sv_ctx sv_ctx;
+
+enum sv_status {
+  SV_OK = 0,
+  SV_E_ARG = -1,        /* invalid argument or call order */
+  SV_E_HIP = -2,        /* HIP runtime error */
+  SV_E_NOMEM = -3,      /* device allocation failed */
+  SV_E_OVERFLOW = -4,   /* neighbour list longer than nplink */
+  SV_E_SINGULAR = -5,   /* a moment matrix had a zero pivot */
+  SV_E_STATE = -6       /* operation needs a step that has not been done */
+};
+
+enum sv_form { SV_FORM_DI = 0, SV_FORM_LC = 1 };      /* -Ddi / -Dlc (Makefile:73-83) */
+enum sv_dt_mode { SV_DT_PID = 0, SV_DT_FIXED = 1 };
+enum sv_op {                                          /* derivatives.F90 routines */
+  SV_OP_GRAD = 0,        /* calc_gradient          :26-71   */
+  SV_OP_LAP = 1,         /* calc_laplacian         :73-116  */
+  SV_OP_GRAD_X = 2,      /* calc_gradient_only(1)  :416-464 */
+  SV_OP_GRAD_Y = 3,      /* calc_gradient_only(2)  :416-464 */
+  SV_OP_FILTER = 4,      /* calc_filtered_var      :360-414 */
+  SV_OP_FILTER_TERM = 5  /* calc_filter_term       :466-511 */
+};
+
+typedef struct {
+  int32_t morder;      /* derivative order (4,6,8,10): -Dmorder, common_parameter.F90:57-66 */
+  int32_t forder;      /* filter order: -Dforder, common_parameter.F90:45-56 */
+  int32_t form;        /* SV_FORM_DI or SV_FORM_LC */
+  int32_t device;      /* HIP device ordinal */
+} sv_config;
+
+typedef struct {       /* control.in quantities (load_data.F90:21-159) */
+  double L_char, U_char, Re, Wi, beta, epsPTT, Ma, Mdiff;
+  double grav[3], driving_force[3];
+  /* Kolmogorov forcing f_x = kol_amp*sin(kol_k*y) added to rhs_rou as the
+   * forcing line of rhs.F90:275 (the LC/DI forms have it commented out,
+   * rhs_lc.F90:263, rhs_di.F90:264).  0 disables it. */
+  double kol_amp, kol_k;
+} sv_phys;
+
+/* ---- context --------------------------------------------------------- */
+int  sv_create(const sv_config *cfg, sv_ctx **out);
+void sv_destroy(sv_ctx *ctx);
+const char *sv_last_error(const sv_ctx *ctx);
+const char *sv_version(void);
+
+/* ---- domain and nodes: replaces build_domain's IPART read
+ *      (setup_domain.F90:93-202) and create_mirror_particles
+ *      (mirror_boundaries.F90:34-261) ------------------------------------ */
+/* IPART header: box, bc flags (xL xU yL yU; 0 none 1 periodic 2 symmetric
+ * 3 no-slip) and smax (sets h0 = hovs*smax, h2 = h0^2: setup_domain.F90:105) */
+int sv_set_domain(sv_ctx *ctx, double xmin, double xmax, double ymin, double ymax,
+                  const int32_t bc[4], double smax, double L_char);
+/* Real nodes.  Ghost (mirror) nodes are created on the device, numbered and
+ * positioned exactly as create_mirror_particles does. */
+int sv_set_nodes(sv_ctx *ctx, int32_t npfb, const double *x, const double *y,
+                 const double *s, const double *h, const double *h_small,
+                 const int32_t *node_type);
+/* np = npfb + ghosts; x,y sized np; irelation/vrelation sized np-npfb
+ * (1-based parents, as common_vars.F90:101) */
+int sv_get_ghosts(sv_ctx *ctx, int32_t *np, double *x, double *y, int32_t *irelation,
+                  int32_t *vrelation, int32_t cap);
+
+/* ---- neighbours: find_neighbours (neighbours.F90:32-341) ---------------- */
+/* Lists sorted by (r^2, j) (the reference's quicksort orders equal r^2
+ * arbitrarily).  Host copies optional; the device lists stay resident. */
+int sv_find_neighbours(sv_ctx *ctx, int32_t nplink, int32_t *ij_count,
+                       int32_t *ij_count_small, int32_t *ij_link);
+
+/* ---- weights: labf.F90 -------------------------------------------------- */
+/* calc_labf_weights (labf.F90:34-231) incl. calc_filter_weights (:233-394)
+ * and calc_node_volumes (:453-503).  order_map (nullable, per real node):
+ * LABFM order of that node's derivative and filter systems, overriding
+ * morder/forder (variable-order node sets). */
+int sv_calc_labf_weights(sv_ctx *ctx, const int32_t *order_map);
+int sv_calc_labf_sums(sv_ctx *ctx);           /* labf.F90:396-451 */
+int sv_filter_coefficients(sv_ctx *ctx);      /* labf.F90:1723-1811 */
+/* Host copies in the reference layouts (see top); any pointer may be NULL */
+int sv_get_weights(sv_ctx *ctx, int32_t nplink, double *ij_w_grad, double *ij_w_lap,
+                   double *ij_w_hyp, double *ij_w_grad_sum, double *ij_w_lap_sum,
+                   double *ij_w_hyp_sum, double *vol);
+/* Device time of the last call, milliseconds: "neighbours", "weights" (all
+ * weight-solve launches), "step" (the last sv_run); "systems" gives the number
+ * of moment systems solved by the last sv_calc_labf_weights. */
+int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
+/* Tuning switches: "weights_persistent" (0/1) solves all moment systems of an
+ * order in one persistent launch fed from a work queue. */
+int sv_set_option(sv_ctx *ctx, const char *key, double value);
+
+/* ---- derivative operators: derivatives.F90 ------------------------------ */
+/* phi sized np (ghost values used as given, as the reference does);
+ * out sized npfb, or 2*npfb (x then y) for SV_OP_GRAD; FILTER writes phi in place
+ * (first npfb entries). */
+int sv_apply(sv_ctx *ctx, int32_t op, double *phi, double *out);
+int sv_calc_gradient(sv_ctx *ctx, const double *phi, double *gradphi);   /* :26-71 */
+int sv_calc_laplacian(sv_ctx *ctx, const double *phi, double *lapphi);   /* :73-116 */
+int sv_calc_gradient_only(sv_ctx *ctx, int32_t dim, const double *phi,
+                          double *gradphi);                              /* :416-464 */
+int sv_calc_filtered_var(sv_ctx *ctx, double *phi);                      /* :360-414 */
+int sv_calc_filter_term(sv_ctx *ctx, const double *phi, double *filtphi);/* :466-511 */
+/* Device variant over fields stored on real nodes only (ghosts resolved to
+ * their periodic parents): nfields fields, each npfb doubles. */
+int sv_apply_dev(sv_ctx *ctx, int32_t op, int32_t nfields, const double *const *phi,
+                 double *const *out, void *stream);
+
+/* ---- physics and time stepping: rhs_lc.F90 / rhs_di.F90 / step.F90 ----- */
+int sv_set_phys(sv_ctx *ctx, const sv_phys *phys);
+/* initial_solution (setup_flow.F90:24-173) with the fields given on real
+ * nodes: rou = ro*u, psi = log(c) (log_conf_psi_from_c), dt = 1e-6,
+ * emax = pid_tol unless set by sv_set_control. */
+int sv_set_initial(sv_ctx *ctx, const double *ro, const double *u, const double *v,
+                   const double *cxx, const double *cxy, const double *cyy);
+/* restart-style control state (setup_flow.F90:495-545) */
+int sv_set_control(sv_ctx *ctx, double dt, double time, int32_t itime, double emax_np1,
+                   double emax_n, double emax_nm1, int32_t dt_mode);
+int sv_get_control(sv_ctx *ctx, double *dt, double *time, int32_t *itime, double *emax3,
+                   double *dt_cfl, double *dt_parabolic);
+/* calc_all_rhs (rhs_lc.F90:47-117 / rhs_di.F90:47-117) on the current state;
+ * iRKstep as the reference's stage counter (the LC first-stage rule,
+ * rhs_lc.F90:503-506, applies at itime==1, iRKstep==1).  Outputs sized npfb. */
+int sv_calc_all_rhs(sv_ctx *ctx, int32_t iRKstep, double *rhs_ro, double *rhs_rou,
+                    double *rhs_rov, double *rhs_xx, double *rhs_xy, double *rhs_yy);
+/* nsteps of the main loop (sunset.F90:75-106): set_tstep, set_tstep_PID
+ * (PID mode), step_rk3_4S_2R_EE (step.F90:218-512), itime += 1.  State stays
+ * on the device; dt lives on the device.  use_graph != 0 replays one captured
+ * step as a hipGraph. */
+int sv_run(sv_ctx *ctx, int32_t nsteps, int32_t use_graph);
+/* As sv_run (eager launches) with HIP events recorded on the context's stream
+ * between the launches of every step.  kernel_ms[8]: mean device time per step
+ * of each launch, in launch order: dt control, stage 1, stage 2, stage 3,
+ * stage 4, filter, mass-correction reduce, state apply.  total_ms: whole run. */
+int sv_run_profiled(sv_ctx *ctx, int32_t nsteps, double *kernel_ms, double *total_ms);
+/* stats[5]: mean ij_count, mean ij_count_small, max ij_count, SELL entries
+ * (with padding), number of ghost nodes */
+int sv_get_stencil_stats(sv_ctx *ctx, double *stats);
+/* Fields on real nodes (any pointer may be NULL): psi* only in the LC form. */
+int sv_get_state(sv_ctx *ctx, double *ro, double *rou, double *rov, double *u, double *v,
+                 double *psixx, double *psixy, double *psiyy, double *cxx, double *cxy,
+                 double *cyy);
+/* Synchronise the context's stream */
+int sv_sync(sv_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -9,7 +9,7 @@
 !! setup_flow.F90:24-173) and calls the reference's own routines in the order of
 !! the main program (sunset.F90:36-106).  Nothing here alters reference source.
 !!
-!! usage: sunset_ref <input.bin> <output.bin>
+!! usage: sunset_ref <input.bin> <output.bin> [stencil.bin]
 !!   input (stream, little endian):
 !!     int32(12): npfb nsteps dt_mode(0 PID,1 fixed) nplink(0=default) flags
 !!                xbcL xbcU ybcL ybcU itime0 0 0
@@ -17,6 +17,14 @@
 !!                 Mdiff grav(3) drive(3) dt0 time0 emax_np1 emax_n emax_nm1
 !!     real64(npfb) x y s h h_small ; int32(npfb) node_type ;
 !!     real64(npfb) ro u v cxx cxy cyy
+!!   flags: bit 1 (2) -- take neighbour lists, weights, sums and volumes from
+!!          stencil.bin instead of find_neighbours/calc_labf_weights (timing runs
+!!          of the RK3 loop at sizes where the serial weight setup would take
+!!          minutes); layout: int32 nplink, ij_count(npfb), ij_count_small(npfb),
+!!          ij_link(nplink,npfb); real64 ij_w_grad(2,nplink,npfb),
+!!          ij_w_lap(nplink,npfb), ij_w_hyp(nplink,npfb), ij_w_grad_sum(2,npfb),
+!!          ij_w_lap_sum(npfb), ij_w_hyp_sum(npfb), vol(npfb)
+!!          bit 2 (4) -- write only the fields after the steps (no lists/weights)
 !!   output: see write section below (mirrored by tests/refdrv.py)
 program sunset_ref
   use kind_parameters
@@ -30,7 +38,7 @@ program sunset_ref
   use step
   use omp_lib
   implicit none
-  character(len=512) :: fin,fout
+  character(len=512) :: fin,fout,fst
   integer(ikind) :: hdr(12),nsteps,dt_mode,flags,i,k,nm,itime0
   real(rkind) :: rh(24)
   real(rkind),allocatable :: tmp(:),tro(:),tu(:),tv(:),tcxx(:),tcxy(:),tcyy(:)
@@ -40,6 +48,7 @@ program sunset_ref
 
   call get_command_argument(1,fin)
   call get_command_argument(2,fout)
+  call get_command_argument(3,fst)
   open(30,file=trim(fin),access='stream',form='unformatted',status='old')
   read(30) hdr
   read(30) rh
@@ -84,16 +93,27 @@ program sunset_ref
      internal_list(i) = i
   end do
 
-  t0 = omp_get_wtime()
-  call find_neighbours
-  t1 = omp_get_wtime()
-  write(6,*) "REFTIME find_neighbours",t1-t0
-  t0 = omp_get_wtime()
-  call calc_labf_weights
-  t1 = omp_get_wtime()
-  write(6,*) "REFTIME calc_labf_weights",t1-t0
-  call calc_labf_sums
-  call filter_coefficients
+  if(iand(flags,2).ne.0) then
+     open(32,file=trim(fst),access='stream',form='unformatted',status='old')
+     read(32) nplink
+     allocate(ij_count(npfb),ij_count_small(npfb),ij_link(nplink,npfb))
+     allocate(ij_w_grad(2,nplink,npfb),ij_w_lap(nplink,npfb),ij_w_hyp(nplink,npfb))
+     allocate(ij_w_grad_sum(2,npfb),ij_w_lap_sum(npfb),ij_w_hyp_sum(npfb),vol(npfb))
+     read(32) ij_count,ij_count_small,ij_link
+     read(32) ij_w_grad,ij_w_lap,ij_w_hyp,ij_w_grad_sum,ij_w_lap_sum,ij_w_hyp_sum,vol
+     close(32)
+  else
+     t0 = omp_get_wtime()
+     call find_neighbours
+     t1 = omp_get_wtime()
+     write(6,*) "REFTIME find_neighbours",t1-t0
+     t0 = omp_get_wtime()
+     call calc_labf_weights
+     t1 = omp_get_wtime()
+     write(6,*) "REFTIME calc_labf_weights",t1-t0
+     call calc_labf_sums
+     call filter_coefficients
+  end if
 
   !! load_control_data_all derived quantities (load_data.F90:132-148)
   rho_char = one
@@ -145,6 +165,7 @@ program sunset_ref
   call calc_filter_term(ro,f1)
 
   open(31,file=trim(fout),access='stream',form='unformatted',status='replace')
+  if(iand(flags,4).eq.0) then
   write(31) np,nplink
   write(31) rp(1:np,1),rp(1:np,2)
   write(31) irelation(npfb+1:np),vrelation(npfb+1:np)
@@ -153,14 +174,17 @@ program sunset_ref
   write(31) ij_w_grad(1:2,1:nplink,1:npfb),ij_w_lap(1:nplink,1:npfb),ij_w_hyp(1:nplink,1:npfb)
   write(31) ij_w_grad_sum(1:2,1:npfb),ij_w_lap_sum(1:npfb),ij_w_hyp_sum(1:npfb),vol(1:npfb)
   write(31) g2(1:npfb,1),g2(1:npfb,2),l1(1:npfb),f1(1:npfb)
+  end if
 
   !! Time loop (sunset.F90:75-106) without output/statistics
   t0 = omp_get_wtime()
   do k=1,nsteps
+     t1 = omp_get_wtime()
      call set_tstep
      if(dt_mode.eq.0) call set_tstep_PID
      call step_rk3_4S_2R_EE
      itime = itime + 1
+     write(6,*) "REFSTEP",k,omp_get_wtime()-t1
   end do
   t1 = omp_get_wtime()
   write(6,*) "REFTIME rk3_steps",nsteps,t1-t0

@@ -1,0 +1,411 @@
+// k_nodes.hip -- ghost (mirror) creation, cell-list neighbour search and the
+// SELL-64 layout of the stencils.
+//
+// Reference behaviour reproduced:
+//  * create_mirror_particles (mirror_boundaries.F90:34-261): ghosts within
+//    ss*h*2 of periodic/symmetric/no-slip edges, numbered node by node in the
+//    order left/right, lower/upper, four corners; positions with the same
+//    left-to-right arithmetic (contraction off) so distances are bit-identical.
+//  * find_neighbours (neighbours.F90:32-341): every j with
+//    |r_i - r_j|^2 <= (h_i*ss)^2 (self included), r^2 = dx*dx + dy*dy without
+//    FMA contraction, lists sorted by (r^2, j); ij_count_small counts
+//    r^2 <= (h_small_i*ss)^2.
+// The cell grid is this library's own: cells of side ss*max(h) over the box
+// padded by its size on each side, so a 3x3 block of cells covers every
+// stencil whatever h_i is.
+#include <hipcub/hipcub.hpp>
+
+#include "sv_internal.h"
+
+#define SS 2.0
+
+namespace {
+
+struct GhostDom {
+  double xmin, xmax, ymin, ymax;
+  int xl, xu, yl, yu;
+};
+
+// Enumerate the ghosts of node i in reference order.  emit(gx, gy, vrel).
+template <class F>
+__device__ void ghost_enum(const GhostDom &g, double xi, double yi, double hi, F emit) {
+#pragma clang fp contract(off)
+  const double lim = SS * hi * 2.0;
+  if (xi <= g.xmin + lim) {
+    if (g.xl == 1) emit(xi + g.xmax - g.xmin, yi, 1);
+    else if (g.xl == 2) emit(2.0 * g.xmin - xi, yi, 2);
+  }
+  if (xi >= g.xmax - lim) {
+    if (g.xu == 1) emit(xi - g.xmax + g.xmin, yi, 1);
+    else if (g.xu == 2) emit(2.0 * g.xmax - xi, yi, 2);
+  }
+  if (yi <= g.ymin + lim) {
+    if (g.yl == 1) emit(xi, yi + g.ymax - g.ymin, 1);
+    else if (g.yl == 2) emit(xi, 2.0 * g.ymin - yi, 3);
+    else if (g.yl == 3) emit(xi, 2.0 * g.ymin - yi, 4);
+  }
+  if (yi >= g.ymax - lim) {
+    if (g.yu == 1) emit(xi, yi - g.ymax + g.ymin, 1);
+    else if (g.yu == 2) emit(xi, 2.0 * g.ymax - yi, 3);
+    else if (g.yu == 3) emit(xi, 2.0 * g.ymax - yi, 4);
+  }
+  const double cx[4] = {g.xmin, g.xmax, g.xmin, g.xmax};
+  const double cy[4] = {g.ymin, g.ymin, g.ymax, g.ymax};
+  const int bx[4] = {g.xl, g.xu, g.xl, g.xu};
+  const int by[4] = {g.yl, g.yl, g.yu, g.yu};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double dx = cx[c] - xi, dy = cy[c] - yi;
+    const double cd = sqrt(dx * dx + dy * dy + 0.0 * 0.0);
+    if (cd <= lim && bx[c] != 0 && by[c] != 0) {
+      const bool perx = bx[c] == 1, pery = by[c] == 1;
+      const bool symx = bx[c] == 2, symy = by[c] == 2;
+      double px, py;
+      if (c == 0 || c == 2) px = perx ? xi + g.xmax - g.xmin : 2.0 * g.xmin - xi;
+      else px = perx ? xi - g.xmax + g.xmin : 2.0 * g.xmax - xi;
+      if (c == 0 || c == 1) py = pery ? yi + g.ymax - g.ymin : 2.0 * g.ymin - yi;
+      else py = pery ? yi - g.ymax + g.ymin : 2.0 * g.ymax - yi;
+      int vr = 0;
+      if (perx && pery) vr = 1;
+      else if (symx && pery) vr = 2;
+      else if (perx && symy) vr = 3;
+      else if (symx && symy) vr = 4;
+      if (vr) emit(px, py, vr);
+      else emit(0.0, 0.0, 0);   // the reference counts the ghost but leaves rp = 0
+    }
+  }
+}
+
+__global__ void k_ghost_count(int npfb, GhostDom g, const double *x, const double *y,
+                              const double *h, int *gcount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npfb) return;
+  int n = 0;
+  ghost_enum(g, x[i], y[i], h[i], [&](double, double, int) { n++; });
+  gcount[i] = n;
+}
+
+__global__ void k_ghost_write(int npfb, GhostDom g, double *x, double *y, double *h, double *hs,
+                              double *s, int *nt, int *irel, int *vrel, const int *goff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npfb) return;
+  int k = npfb + goff[i];
+  const double hi = h[i], hsi = hs[i], si = s[i];
+  const int nti = nt[i];
+  ghost_enum(g, x[i], y[i], hi, [&](double gx, double gy, int vr) {
+    x[k] = gx; y[k] = gy; irel[k] = i; vrel[k] = vr;
+    h[k] = hi; hs[k] = hsi; s[k] = si; nt[k] = nti;
+    k++;
+  });
+  irel[i] = i;
+  vrel[i] = 0;
+}
+
+struct Grid {
+  double x0, y0, inv;
+  int ncx, ncy;
+};
+
+__device__ __forceinline__ int cell_of(const Grid &G, double x, double y, int *cx, int *cy) {
+  int ix = (int)((x - G.x0) * G.inv);
+  int iy = (int)((y - G.y0) * G.inv);
+  ix = ix < 0 ? 0 : (ix >= G.ncx ? G.ncx - 1 : ix);
+  iy = iy < 0 ? 0 : (iy >= G.ncy ? G.ncy - 1 : iy);
+  *cx = ix;
+  *cy = iy;
+  return iy * G.ncx + ix;
+}
+
+__global__ void k_cell_count(int np, Grid G, const double *x, const double *y, int *cellof,
+                             int *ccount) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  int cx, cy;
+  const int c = cell_of(G, x[k], y[k], &cx, &cy);
+  cellof[k] = c;
+  atomicAdd(&ccount[c], 1);
+}
+
+__global__ void k_cell_fill(int np, const int *cellof, const int *cstart, int *cfill, int *ip) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int c = cellof[k];
+  const int pos = atomicAdd(&cfill[c], 1);
+  ip[cstart[c] + pos] = k;
+}
+
+__device__ __forceinline__ double dist2(double xi, double yi, double xj, double yj) {
+#pragma clang fp contract(off)
+  const double dx = xi - xj, dy = yi - yj;
+  double t = dx * dx;
+  t = t + dy * dy;
+  return t;
+}
+
+// pass 0: count; pass 1: write (col, r2) then sort the row by (r2, col)
+template <int PASS>
+__global__ void k_neigh(int npfb, Grid G, const double *x, const double *y, const double *h,
+                        const double *hs, const int *cellof, const int *cstart, const int *ip,
+                        int *cnt, int *cnt_small, const int64_t *row_ptr, int *col, double *r2a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npfb) return;
+  const double xi = x[i], yi = y[i];
+  const double hh = h[i] * SS;
+  const double st2 = hh * hh;
+  const int c = cellof[i];
+  const int cx = c % G.ncx, cy = c / G.ncx;
+  int n = 0;
+  int64_t base = 0;
+  if (PASS == 1) base = row_ptr[i];
+  for (int oy = -1; oy <= 1; oy++) {
+    const int yy = cy + oy;
+    if (yy < 0 || yy >= G.ncy) continue;
+    for (int ox = -1; ox <= 1; ox++) {
+      const int xx = cx + ox;
+      if (xx < 0 || xx >= G.ncx) continue;
+      const int cc = yy * G.ncx + xx;
+      const int e = cstart[cc + 1];
+      for (int jj = cstart[cc]; jj < e; jj++) {
+        const int j = ip[jj];
+        const double r2 = dist2(xi, yi, x[j], y[j]);
+        if (r2 <= st2) {
+          if (PASS == 1) {
+            col[base + n] = j;
+            r2a[base + n] = r2;
+          }
+          n++;
+        }
+      }
+    }
+  }
+  if (PASS == 0) {
+    cnt[i] = n;
+    return;
+  }
+  // insertion sort of the row by (r2, j); rows are short (tens to ~200)
+  for (int a = 1; a < n; a++) {
+    const double rv = r2a[base + a];
+    const int jv = col[base + a];
+    int b = a - 1;
+    while (b >= 0) {
+      const double rb = r2a[base + b];
+      const int jb = col[base + b];
+      if (rb < rv || (rb == rv && jb < jv)) break;
+      r2a[base + b + 1] = rb;
+      col[base + b + 1] = jb;
+      b--;
+    }
+    r2a[base + b + 1] = rv;
+    col[base + b + 1] = jv;
+  }
+  const double hsm = hs[i] * SS;
+  const double st2s = hsm * hsm;
+  int ks = 0;
+  for (int a = 0; a < n; a++) ks += (r2a[base + a] <= st2s) ? 1 : 0;
+  cnt_small[i] = ks;
+}
+
+__global__ void k_sell_pack(int npfb, const int64_t *sell_off, const int *sell_w,
+                            const int64_t *row_ptr, const int *cnt, const int *col_raw,
+                            const int *irel, const double *wx, const double *wy,
+                            const double *wl, const double *wh, int *scol, int *scol_raw,
+                            double *swx, double *swy, double *swl, double *swh) {
+  // one block of 64 threads per (slice, link-chunk of 4)
+  const int s = blockIdx.x;
+  const int l = threadIdx.x;
+  const int i = s * SV_WAVE + l;
+  const int W = sell_w[s];
+  const int64_t off = sell_off[s];
+  const int k0 = blockIdx.y * 4;
+  for (int k = k0; k < W && k < k0 + 4; k++) {
+    const int64_t d = off + (int64_t)k * SV_WAVE + l;
+    if (i < npfb && k < cnt[i]) {
+      const int64_t sidx = row_ptr[i] + k;
+      const int j = col_raw[sidx];
+      scol_raw[d] = j;
+      scol[d] = irel[j];
+      swx[d] = wx ? wx[sidx] : 0.0;
+      swy[d] = wy ? wy[sidx] : 0.0;
+      swl[d] = wl ? wl[sidx] : 0.0;
+      swh[d] = wh ? wh[sidx] : 0.0;
+    } else {
+      const int self = i < npfb ? i : 0;
+      scol_raw[d] = self;
+      scol[d] = self;
+      swx[d] = 0.0;
+      swy[d] = 0.0;
+      swl[d] = 0.0;
+      swh[d] = 0.0;
+    }
+  }
+}
+
+}  // namespace
+
+static int scan_int(sv_ctx *c, const int *in, int *out, int n) {
+  size_t tb = 0;
+  SV_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, c->stream));
+  void *tmp = nullptr;
+  SV_HIP(c, hipMalloc(&tmp, tb > 0 ? tb : 16));
+  SV_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  SV_HIP(c, hipFree(tmp));
+  return 0;
+}
+
+int sv_dev_create_ghosts(sv_ctx *c) {
+  const int n = c->npfb;
+  GhostDom g{c->xmin, c->xmax, c->ymin, c->ymax, c->bc[0], c->bc[1], c->bc[2], c->bc[3]};
+  int *gcount = nullptr, *goff = nullptr;
+  SV_HIP(c, hipMalloc(&gcount, sizeof(int) * (n + 1)));
+  SV_HIP(c, hipMalloc(&goff, sizeof(int) * (n + 1)));
+  SV_HIP(c, hipMemsetAsync(gcount, 0, sizeof(int) * (n + 1), c->stream));
+  const int B = 256, nb = (n + B - 1) / B;
+  k_ghost_count<<<nb, B, 0, c->stream>>>(n, g, c->x, c->y, c->h, gcount);
+  SV_HIP(c, hipGetLastError());
+  if (int rc = scan_int(c, gcount, goff, n + 1)) return rc;
+  int ng = 0;
+  SV_HIP(c, hipMemcpy(&ng, goff + n, sizeof(int), hipMemcpyDeviceToHost));
+  if (n + ng > c->cap) {
+    (void)hipFree(gcount);
+    (void)hipFree(goff);
+    return sv_fail(c, SV_E_ARG, "ghost count %d exceeds node capacity %d", ng, c->cap - n);
+  }
+  k_ghost_write<<<nb, B, 0, c->stream>>>(n, g, c->x, c->y, c->h, c->hs, c->s, c->node_type,
+                                         c->irel, c->vrel, goff);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  SV_HIP(c, hipFree(gcount));
+  SV_HIP(c, hipFree(goff));
+  c->np = n + ng;
+  return 0;
+}
+
+int sv_dev_find_neighbours(sv_ctx *c) {
+  hipEvent_t e0, e1;
+  SV_HIP(c, hipEventCreate(&e0));
+  SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventRecord(e0, c->stream));
+  const int np = c->np, npfb = c->npfb;
+  const double Lx = c->xmax - c->xmin, Ly = c->ymax - c->ymin;
+  double csz = SS * c->hmax;
+  // keep the padded grid below ~64M cells
+  const double area = 9.0 * Lx * Ly;
+  if (area / (csz * csz) > 6.4e7) csz = sqrt(area / 6.4e7);
+  Grid G;
+  G.x0 = c->xmin - Lx;
+  G.y0 = c->ymin - Ly;
+  G.inv = 1.0 / csz;
+  G.ncx = (int)(3.0 * Lx / csz) + 1;
+  G.ncy = (int)(3.0 * Ly / csz) + 1;
+  const int nct = G.ncx * G.ncy;
+  int *cellof = nullptr, *ccount = nullptr, *cstart = nullptr, *ip = nullptr;
+  SV_HIP(c, hipMalloc(&cellof, sizeof(int) * np));
+  SV_HIP(c, hipMalloc(&ccount, sizeof(int) * (nct + 1)));
+  SV_HIP(c, hipMalloc(&cstart, sizeof(int) * (nct + 1)));
+  SV_HIP(c, hipMalloc(&ip, sizeof(int) * np));
+  SV_HIP(c, hipMemsetAsync(ccount, 0, sizeof(int) * (nct + 1), c->stream));
+  const int B = 256;
+  k_cell_count<<<(np + B - 1) / B, B, 0, c->stream>>>(np, G, c->x, c->y, cellof, ccount);
+  SV_HIP(c, hipGetLastError());
+  if (int rc = scan_int(c, ccount, cstart, nct + 1)) return rc;
+  SV_HIP(c, hipMemsetAsync(ccount, 0, sizeof(int) * (nct + 1), c->stream));
+  k_cell_fill<<<(np + B - 1) / B, B, 0, c->stream>>>(np, cellof, cstart, ccount, ip);
+  SV_HIP(c, hipGetLastError());
+  // counts
+  if (!c->cnt) {
+    if (sv_alloc(c, (void **)&c->cnt, sizeof(int) * npfb)) return SV_E_NOMEM;
+    if (sv_alloc(c, (void **)&c->cnt_small, sizeof(int) * npfb)) return SV_E_NOMEM;
+    if (sv_alloc(c, (void **)&c->row_ptr, sizeof(int64_t) * (npfb + 1))) return SV_E_NOMEM;
+  }
+  const int nb = (npfb + B - 1) / B;
+  k_neigh<0><<<nb, B, 0, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cellof, cstart, ip,
+                                      c->cnt, c->cnt_small, nullptr, nullptr, nullptr);
+  SV_HIP(c, hipGetLastError());
+  std::vector<int> hc(npfb);
+  SV_HIP(c, hipMemcpyAsync(hc.data(), c->cnt, sizeof(int) * npfb, hipMemcpyDeviceToHost,
+                           c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<int64_t> rp(npfb + 1);
+  rp[0] = 0;
+  int maxk = 0;
+  for (int i = 0; i < npfb; i++) {
+    rp[i + 1] = rp[i] + hc[i];
+    maxk = hc[i] > maxk ? hc[i] : maxk;
+  }
+  c->maxK = maxk;
+  c->nnz = rp[npfb];
+  SV_HIP(c, hipMemcpyAsync(c->row_ptr, rp.data(), sizeof(int64_t) * (npfb + 1),
+                           hipMemcpyHostToDevice, c->stream));
+  if (c->col_raw) sv_free(c, c->col_raw);
+  if (sv_alloc(c, (void **)&c->col_raw, sizeof(int) * (c->nnz + 1))) return SV_E_NOMEM;
+  double *r2a = nullptr;
+  SV_HIP(c, hipMalloc(&r2a, sizeof(double) * (c->nnz + 1)));
+  k_neigh<1><<<nb, B, 0, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cellof, cstart, ip,
+                                      c->cnt, c->cnt_small, c->row_ptr, c->col_raw, r2a);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipEventRecord(e1, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  c->t_neigh = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SV_HIP(c, hipFree(r2a));
+  SV_HIP(c, hipFree(cellof));
+  SV_HIP(c, hipFree(ccount));
+  SV_HIP(c, hipFree(cstart));
+  SV_HIP(c, hipFree(ip));
+  c->have_lists = true;
+  c->sell_dirty = true;
+  return 0;
+}
+
+int sv_dev_build_sell(sv_ctx *c) {
+  const int npfb = c->npfb;
+  const int ns = (npfb + SV_WAVE - 1) / SV_WAVE;
+  std::vector<int> hc(npfb);
+  SV_HIP(c, hipMemcpy(hc.data(), c->cnt, sizeof(int) * npfb, hipMemcpyDeviceToHost));
+  std::vector<int> w(ns);
+  std::vector<int64_t> off(ns + 1);
+  off[0] = 0;
+  int wmax = 0;
+  for (int s = 0; s < ns; s++) {
+    int m = 0;
+    for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < npfb; l++)
+      m = hc[s * SV_WAVE + l] > m ? hc[s * SV_WAVE + l] : m;
+    w[s] = m;
+    wmax = m > wmax ? m : wmax;
+    off[s + 1] = off[s] + (int64_t)m * SV_WAVE;
+  }
+  if (c->nslices != ns || c->sell_nnz != off[ns]) {
+    void *bufs[] = {c->sell_off, c->sell_w, c->scol, c->scol_raw, c->swx, c->swy, c->swl, c->swh};
+    for (void *b : bufs)
+      if (b) sv_free(c, b);
+    c->nslices = ns;
+    c->sell_nnz = off[ns];
+    const size_t L = (size_t)c->sell_nnz + 1;
+    if (sv_alloc(c, (void **)&c->sell_off, sizeof(int64_t) * (ns + 1)) ||
+        sv_alloc(c, (void **)&c->sell_w, sizeof(int) * (ns + 1)) ||
+        sv_alloc(c, (void **)&c->scol, sizeof(int) * L) ||
+        sv_alloc(c, (void **)&c->scol_raw, sizeof(int) * L) ||
+        sv_alloc(c, (void **)&c->swx, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->swy, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->swl, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->swh, sizeof(double) * L))
+      return SV_E_NOMEM;
+  }
+  SV_HIP(c, hipMemcpy(c->sell_off, off.data(), sizeof(int64_t) * (ns + 1),
+                      hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->sell_w, w.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+  dim3 grid(ns, (wmax + 3) / 4);
+  k_sell_pack<<<grid, SV_WAVE, 0, c->stream>>>(npfb, c->sell_off, c->sell_w, c->row_ptr, c->cnt,
+                                               c->col_raw, c->irel, c->wx, c->wy, c->wl, c->wh,
+                                               c->scol, c->scol_raw, c->swx, c->swy, c->swl,
+                                               c->swh);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  c->sell_dirty = false;
+  return 0;
+}

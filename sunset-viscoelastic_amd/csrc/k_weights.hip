@@ -1,0 +1,611 @@
+// k_weights.hip -- LABFM weight solves (calc_labf_weights, labf.F90:34-231;
+// calc_filter_weights :233-394; calc_node_volumes :453-503; calc_labf_sums
+// :396-451; filter_coefficients :1723-1811).
+//
+// One wavefront per moment system.  For node i with LABFM order m (n =
+// (m^2+3m)/2 unknowns) and stencil length K:
+//   1. the moment matrix A = sum_k X_k g_k^T (X_k: Taylor monomials of the
+//      scaled offset, g_k: Hermite ABFs times the Wendland C2 kernel) is
+//      contracted on the FP64 matrix cores, v_mfma_f64_16x16x4_f64, with
+//      32-neighbour chunks of X and g staged in LDS;
+//   2. A is factorised in LDS by LU with partial pivoting (one column per lane);
+//   3. the right-hand sides (d/dx, d/dy, Laplacian, hyperviscous filter) are
+//      solved and improved by two steps of iterative refinement with the
+//      residual accumulated in double-double, which brings the solution to the
+//      exact solution of the FP64 matrix (the reference's Golub-Reinsch SVD,
+//      svdlib.F90, is only within cond(A)*eps of it);
+//   4. weights w_k = (psi . g_k)/hh are evaluated one neighbour per lane and
+//      written in CSR order.
+// The derivative system (h_small, small stencil) and the filter system (h, full
+// stencil) are one system when they coincide (same order, h_small == h, same
+// stencil), and then share the factorisation (4 right-hand sides).
+#include <algorithm>
+
+#include "sv_internal.h"
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int nsize(int m) { return (m * m + 3 * m) / 2; }
+constexpr int npad(int m) { return ((nsize(m) + 15) / 16) * 16; }
+constexpr double fact(int k) { return k <= 1 ? 1.0 : k * fact(k - 1); }
+
+// Probabilists' Hermite polynomials He_0..He_M, written as labf.F90:2329-2391
+template <int M>
+__device__ __forceinline__ void hermite_all(double z, double *H) {
+  H[0] = 1.0;
+  H[1] = z;
+  if (M >= 2) H[2] = z * z - 1.0;
+  if (M >= 3) H[3] = z * z * z - 3.0 * z;
+  if (M >= 4) H[4] = z * z * z * z - 6.0 * z * z + 3.0;
+  if (M >= 5) {
+    const double z2 = z * z, z3 = z * z2;
+    H[5] = z2 * z3 - 10.0 * z3 + 15.0 * z;
+    if (M >= 6) H[6] = z3 * z3 - 15.0 * z2 * z2 + 45.0 * z2 - 15.0;
+    if (M >= 7) {
+      const double z4 = z3 * z;
+      H[7] = z4 * z3 - 21.0 * z3 * z2 + 105.0 * z3 - 105.0 * z;
+    }
+    if (M >= 8) {
+      const double z4 = z2 * z2;
+      H[8] = z4 * z4 - 28.0 * z4 * z2 + 210.0 * z4 - 420.0 * z2 + 105.0;
+    }
+  }
+}
+
+// Wendland C2, rbfs.F90:48-57
+__device__ __forceinline__ double wab(double q, double h2) {
+  double t = 1.0 - 0.5 * q;
+  t = t * t;
+  t = t * t;
+  return (q <= 2.0 && q > 0.0) ? (2.0 * q + 1.0) * (7.0 / (4.0 * SV_PI * h2)) * t : 0.0;
+}
+
+// Row of Taylor monomials x^a y^b/(a!b!) in graded order (labf.F90:1815-1920)
+template <int M>
+__device__ __forceinline__ void mono_row(double x, double y, double *dst) {
+  double xp[M + 1], yp[M + 1];
+  xp[0] = 1.0; yp[0] = 1.0; xp[1] = x; yp[1] = y;
+#pragma unroll
+  for (int k = 2; k <= M; k++) { xp[k] = xp[k - 1] * x; yp[k] = yp[k - 1] * y; }
+  int idx = 0;
+#pragma unroll
+  for (int d = 1; d <= M; d++)
+#pragma unroll
+    for (int b = 0; b <= d; b++) {
+      const int a = d - b;
+      const double coef = 1.0 / (fact(a) * fact(b));
+      double v;
+      if (b == 0) v = (a == 1) ? x : coef * xp[a];
+      else if (a == 0) v = (b == 1) ? y : coef * yp[b];
+      else if (a == 1 && b == 1) v = x * y;
+      else if (a == 1) v = (coef * x) * yp[b];
+      else if (b == 1) v = (coef * xp[a]) * y;
+      else v = (coef * xp[a]) * yp[b];
+      dst[idx++] = v;
+    }
+}
+
+// Row of Hermite ABFs He_a(x) He_b(y) * w (labf.F90:2017-2120)
+template <int M>
+__device__ __forceinline__ void abf_row(double x, double y, double w, double *dst) {
+  double hx[M + 1], hy[M + 1];
+  hermite_all<M>(x, hx);
+  hermite_all<M>(y, hy);
+  int idx = 0;
+#pragma unroll
+  for (int d = 1; d <= M; d++)
+#pragma unroll
+    for (int b = 0; b <= d; b++) {
+      const int a = d - b;
+      const double v = (b == 0) ? hx[a] : (a == 0) ? hy[b] : hx[a] * hy[b];
+      dst[idx++] = v * w;
+    }
+}
+
+// psi . g for 4 right-hand sides, g evaluated on the fly (psi row-major [n][4])
+template <int M>
+__device__ __forceinline__ void abf_dot4(double x, double y, double w, const double *psi,
+                                         double *out) {
+  double hx[M + 1], hy[M + 1];
+  hermite_all<M>(x, hx);
+  hermite_all<M>(y, hy);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int idx = 0;
+#pragma unroll
+  for (int d = 1; d <= M; d++)
+#pragma unroll
+    for (int b = 0; b <= d; b++) {
+      const int a = d - b;
+      const double v = ((b == 0) ? hx[a] : (a == 0) ? hy[b] : hx[a] * hy[b]) * w;
+      a0 += psi[idx * 4 + 0] * v;
+      a1 += psi[idx * 4 + 1] * v;
+      a2 += psi[idx * 4 + 2] * v;
+      a3 += psi[idx * 4 + 3] * v;
+      idx++;
+    }
+  out[0] = a0; out[1] = a1; out[2] = a2; out[3] = a3;
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Hyperviscous filter right-hand side, labf.F90:325-334 (0-based indices)
+__device__ __forceinline__ double hyp_rhs(int m, int r) {
+  if (m <= 5) return r == 9 ? -1.0 : r == 11 ? -2.0 : r == 13 ? -1.0 : 0.0;
+  if (m <= 7) return (r == 20 || r == 26) ? 1.0 : (r == 22 || r == 24) ? 3.0 : 0.0;
+  if (m <= 9)
+    return (r == 35 || r == 43) ? -1.0 : (r == 37 || r == 41) ? -4.0 : r == 39 ? -6.0 : 0.0;
+  return 0.0;
+}
+
+enum { SYS_D = 1, SYS_F = 2 };
+
+struct SysArgs {
+  const int *sys_node;       // node per system
+  const int *sys_kind;       // SYS_D | SYS_F bits
+  int nsys;
+  const double *x, *y, *h, *hs;
+  const int *node_type;
+  const int64_t *row_ptr;
+  const int *col, *cnt, *cnt_small;
+  double h2;
+  double *wx, *wy, *wl, *wh, *vol;
+  int *singular;
+  int *queue;                // persistent launch: next system to take
+};
+
+template <int M>
+struct Lds {
+  static constexpr int N = nsize(M), NP = npad(M), LDA = NP + 1, CH = 32;
+  static constexpr int CHUNK = 2 * CH * NP;          // X and G chunks
+  static constexpr int A0SZ = NP * LDA;
+  static constexpr int U0 = CHUNK > A0SZ ? CHUNK : A0SZ;
+  static constexpr int TOTAL = U0 + NP * LDA + 3 * NP * 4;   // doubles
+};
+
+// In-place solve of LU x = P b for the 4 columns of S ([NP][4]).
+template <int N, int LDA>
+__device__ __forceinline__ void lu_solve4(const double *A, const int *piv, double *S, int l) {
+  if (l < 4)
+    for (int k = 0; k < N; k++) {
+      const int p = piv[k];
+      if (p != k) {
+        const double t = S[k * 4 + l];
+        S[k * 4 + l] = S[p * 4 + l];
+        S[p * 4 + l] = t;
+      }
+    }
+  __syncthreads();
+  for (int k = 0; k < N - 1; k++) {
+    for (int e = l; e < (N - k - 1) * 4; e += SV_WAVE) {
+      const int r = k + 1 + (e >> 2), q = e & 3;
+      S[r * 4 + q] -= A[r * LDA + k] * S[k * 4 + q];
+    }
+    __syncthreads();
+  }
+  for (int k = N - 1; k >= 0; k--) {
+    if (l < 4) {
+      const double d = A[k * LDA + k];
+      S[k * 4 + l] = S[k * 4 + l] / (d == 0.0 ? 1.0 : d);
+    }
+    __syncthreads();
+    for (int e = l; e < k * 4; e += SV_WAVE) {
+      const int r = e >> 2, q = e & 3;
+      S[r * 4 + q] -= A[r * LDA + k] * S[k * 4 + q];
+    }
+    __syncthreads();
+  }
+}
+
+// Solve one system; the block is one wavefront.
+template <int M>
+__device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) {
+  using L = Lds<M>;
+  constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH, T = NP / 16;
+  double *Xc = lds;                 // [CH][NP]
+  double *Gc = lds + CH * NP;       // [CH][NP]
+  double *A0 = lds;                 // [NP][LDA] (reuses the chunks after assembly)
+  double *A = lds + L::U0;          // [NP][LDA] LU workspace
+  double *Bm = A + NP * LDA;        // [NP][4] right-hand sides
+  double *Rm = Bm + NP * 4;         // [NP][4] residual / correction
+  double *Xs = Rm + NP * 4;         // [NP][4] solution
+  const int l = threadIdx.x;
+  const int i = S_.sys_node[sys];
+  const int kind = S_.sys_kind[sys];
+  const bool sysD = kind & SYS_D, sysF = kind & SYS_F;
+  const double hh = sysD ? S_.hs[i] : S_.h[i];
+  const int Kuse = sysD ? S_.cnt_small[i] : S_.cnt[i];
+  const int Kfull = S_.cnt[i];
+  const int64_t base = S_.row_ptr[i];
+  const double xi = S_.x[i], yi = S_.y[i];
+
+  // ---- 1. assembly on the matrix cores ----
+  dbl4 acc[T][T];
+#pragma unroll
+  for (int a = 0; a < T; a++)
+#pragma unroll
+    for (int b = 0; b < T; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int c0 = 0; c0 < Kuse; c0 += CH) {
+    const int kk = c0 + (l & (CH - 1));
+    double *row = ((l >> 5) == 0 ? Xc : Gc) + (l & (CH - 1)) * NP;
+    if (kk < Kuse) {
+      const int j = S_.col[base + kk];
+      const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
+      const double xs = x / hh, ys = y / hh;
+      if ((l >> 5) == 0) {
+        mono_row<M>(xs, ys, row);
+      } else {
+        const double rad = sqrt(x * x + y * y) / hh;
+        abf_row<M>(xs, ys, wab(rad, S_.h2), row);
+      }
+#pragma unroll
+      for (int a = N; a < NP; a++) row[a] = 0.0;
+    } else {
+#pragma unroll
+      for (int a = 0; a < NP; a++) row[a] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < CH / 4; ks++) {
+      const int kr = ks * 4 + (l >> 4);
+#pragma unroll
+      for (int ti = 0; ti < T; ti++) {
+        const double av = Xc[kr * NP + ti * 16 + (l & 15)];
+#pragma unroll
+        for (int tj = 0; tj < T; tj++) {
+          const double bv = Gc[kr * NP + tj * 16 + (l & 15)];
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ti][tj], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D layout of a 16x16 f64 tile: lane l holds rows 4*(l>>4)+v of column l&15
+#pragma unroll
+  for (int ti = 0; ti < T; ti++)
+#pragma unroll
+    for (int tj = 0; tj < T; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = ti * 16 + 4 * (l >> 4) + v, c = tj * 16 + (l & 15);
+        A[r * LDA + c] = acc[ti][tj][v];
+      }
+  __syncthreads();
+  // order reduction near walls by identity padding (labf.F90:126-148, :311-350)
+  const int nt = S_.node_type[i];
+  int nkeep = N, mf = M;
+  if (M >= 7 && (nt == 998 || nt < 0)) { nkeep = 27; mf = 6; }
+  if (M >= 5 && sysD && (nt == -1 || nt == -2)) nkeep = 14;
+  if (M >= 5 && sysF && nt == -1) { nkeep = 14; mf = 4; }
+  if (nkeep < N && l < N) {
+    for (int r = 0; r < N; r++)
+      if (r >= nkeep || l >= nkeep) A[r * LDA + l] = (r == l) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int e = l; e < N * LDA; e += SV_WAVE) A0[e] = A[e];
+  // right-hand sides: q0 d/dx, q1 d/dy, q2 Laplacian (e3+e5), q3 filter
+  for (int e = l; e < NP * 4; e += SV_WAVE) {
+    const int r = e >> 2, q = e & 3;
+    double v = 0.0;
+    if (r < N) {
+      if (q == 0) v = (sysD && r == 0) ? 1.0 : 0.0;
+      else if (q == 1) v = (sysD && r == 1) ? 1.0 : 0.0;
+      else if (q == 2) v = (sysD && (r == 2 || r == 4)) ? 1.0 : 0.0;
+      else v = sysF ? hyp_rhs(mf, r) : 0.0;
+    }
+    Bm[e] = v;
+    Rm[e] = v;
+    Xs[e] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- 2. LU with partial pivoting, one column per lane ----
+  bool sing = false;
+  for (int k = 0; k < N; k++) {
+    double v = (l >= k && l < N) ? fabs(A[l * LDA + k]) : -1.0;
+    int p = l;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double vo = __shfl_xor(v, o);
+      const int po = __shfl_xor(p, o);
+      if (vo > v || (vo == v && po < p)) { v = vo; p = po; }
+    }
+    if (v == 0.0) sing = true;
+    if (l == 0) piv[k] = p;
+    if (p != k && l < N) {
+      const double t = A[k * LDA + l];
+      A[k * LDA + l] = A[p * LDA + l];
+      A[p * LDA + l] = t;
+    }
+    __syncthreads();
+    const double pv = A[k * LDA + k];
+    const double pivot = (pv == 0.0) ? 1.0 : pv;
+    if (l > k && l < N) A[l * LDA + k] = A[l * LDA + k] / pivot;
+    __syncthreads();
+    if (l > k && l < N) {
+      const double akc = A[k * LDA + l];
+      for (int r = k + 1; r < N; r++) A[r * LDA + l] -= A[r * LDA + k] * akc;
+    }
+    __syncthreads();
+  }
+
+  // ---- 3. solve, then two refinement steps with a double-double residual ----
+  for (int it = 0; it < 3; it++) {
+    if (it > 0) {
+      for (int e = l; e < N * 4; e += SV_WAVE) {
+        const int r = e >> 2, q = e & 3;
+        double sh = 0.0, sl = 0.0;
+        for (int cc = 0; cc < N; cc++) {
+          const double a = A0[r * LDA + cc], xv = Xs[cc * 4 + q];
+          const double p = a * xv;
+          const double pe = fma(a, xv, -p);
+          double t;
+          two_sum(sh, p, sh, t);
+          sl += t + pe;
+        }
+        double d, de;
+        two_sum(Bm[e], -sh, d, de);
+        Rm[e] = d + (de - sl);
+      }
+      __syncthreads();
+    }
+    lu_solve4<N, LDA>(A, piv, Rm, l);
+    for (int e = l; e < N * 4; e += SV_WAVE) Xs[e] = (it == 0) ? Rm[e] : Xs[e] + Rm[e];
+    __syncthreads();
+  }
+
+  // ---- 4. weights, one neighbour per lane (k <= ij_count, labf.F90:183-215) ----
+  double vsum = 0.0;
+  for (int k = l; k < Kfull; k += SV_WAVE) {
+    const int j = S_.col[base + k];
+    const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
+    const double rad = sqrt(x * x + y * y) / hh;
+    double o[4];
+    abf_dot4<M>(x / hh, y / hh, wab(rad, S_.h2), Xs, o);
+    if (sysD) {
+      S_.wx[base + k] = o[0] / hh;
+      S_.wy[base + k] = o[1] / hh;
+      S_.wl[base + k] = o[2] / hh / hh;
+    }
+    if (sysF) {
+      S_.wh[base + k] = o[3];
+      vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
+    }
+  }
+  if (sysF) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) vsum += __shfl_xor(vsum, o);
+    if (l == 0) S_.vol[i] = 1.0 / vsum;
+  }
+  if (sing && l == 0) atomicAdd(S_.singular, 1);
+  __syncthreads();
+}
+
+template <int M>
+__global__ __launch_bounds__(SV_WAVE) void k_weights(SysArgs a) {
+  extern __shared__ double lds[];
+  __shared__ int piv[64];
+  const int sys = blockIdx.x;
+  if (sys >= a.nsys) return;
+  solve_system<M>(a, sys, lds, piv);
+}
+
+// Persistent variant: a fixed grid takes systems from a queue; systems are
+// sorted by (order, stencil length) so every wave works on the longest
+// remaining systems first.
+template <int M>
+__global__ __launch_bounds__(SV_WAVE) void k_weights_persistent(SysArgs a) {
+  extern __shared__ double lds[];
+  __shared__ int piv[64];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int sys = next;
+    __syncthreads();
+    if (sys >= a.nsys) break;
+    solve_system<M>(a, sys, lds, piv);
+  }
+}
+
+// calc_labf_sums (labf.F90:396-451), one thread per node, sequential in k
+__global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double L, double *wx,
+                       double *wy, double *wl, const double *wh, double *gsx, double *gsy,
+                       double *lsum, double *hsum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npfb) return;
+  const int64_t b = row_ptr[i];
+  const int K = cnt[i];
+  double sh = 0.0, sx = 0.0, sy = 0.0, sl = 0.0;
+  for (int k = 0; k < K; k++) {
+    sh += wh[b + k];
+    sx += wx[b + k];
+    sy += wy[b + k];
+    sl += wl[b + k];
+  }
+  hsum[i] = sh;
+  gsx[i] = sx / L;
+  gsy[i] = sy / L;
+  lsum[i] = sl / L / L;
+  for (int k = 0; k < K; k++) {
+    wx[b + k] = wx[b + k] / L;
+    wy[b + k] = wy[b + k] / L;
+    wl[b + k] = wl[b + k] / L / L;
+  }
+}
+
+// filter_coefficients (labf.F90:1723-1811), interior nodes
+__global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt,
+                                const int *col, const double *x, const double *y,
+                                const double *h, double *wh, double *hsum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npfb) return;
+  const int64_t b = row_ptr[i];
+  const int K = cnt[i];
+  const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
+  const double tmp = (2.0 / 3.0) * SV_PI / lscal;
+  double ls = 0.0;
+  for (int k = 0; k < K; k++) {
+    const int j = col[b + k];
+    const double dx = x[j] - x[i], dy = y[j] - y[i];
+    const double f = 1.0 - cos(tmp * dx) * cos(tmp * dy);
+    ls = ls + f * wh[b + k];
+  }
+  const double fc = (2.0 / 3.0) / ls;
+  double hs = 0.0;
+  for (int k = 0; k < K; k++) {
+    const double w = wh[b + k] * fc;
+    wh[b + k] = w;
+    hs = hs + w;
+  }
+  hsum[i] = hs;
+}
+
+template <int M>
+int launch_order(sv_ctx *c, SysArgs a, int first, int count, bool persistent) {
+  if (count <= 0) return 0;
+  const size_t shm = sizeof(double) * Lds<M>::TOTAL;
+  SysArgs b = a;
+  b.sys_node = a.sys_node + first;
+  b.sys_kind = a.sys_kind + first;
+  b.nsys = count;
+  if (!persistent) {
+    k_weights<M><<<count, SV_WAVE, shm, c->stream>>>(b);
+  } else {
+    int dev = c->device, ncu = 0, occ = 0;
+    SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_persistent<M>,
+                                                            SV_WAVE, shm));
+    if (occ < 1) occ = 1;
+    int grid = ncu * occ;
+    if (grid > count) grid = count;
+    SV_HIP(c, hipMemsetAsync(a.queue, 0, sizeof(int), c->stream));
+    k_weights_persistent<M><<<grid, SV_WAVE, shm, c->stream>>>(b);
+  }
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// Build the system lists (one per order), launch, and check for zero pivots.
+int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
+  const int n = c->npfb;
+  std::vector<int> cnt(n), cs(n);
+  std::vector<double> h(n), hs(n);
+  SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(cs.data(), c->cnt_small, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(h.data(), c->h, sizeof(double) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(hs.data(), c->hs, sizeof(double) * n, hipMemcpyDeviceToHost));
+  // systems grouped by order: entries (order, K, node, kind)
+  struct Sys { int m, K, node, kind; };
+  std::vector<Sys> sys;
+  sys.reserve(2 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    const int md = order_map_host ? order_map_host[i] : c->cfg.morder;
+    const int mf = order_map_host ? order_map_host[i] : c->cfg.forder;
+    for (int m : {md, mf})
+      if (m != 4 && m != 6 && m != 8)
+        return sv_fail(c, SV_E_ARG, "LABFM order %d not supported on the device (4, 6, 8)", m);
+    if (md == mf && h[i] == hs[i] && cnt[i] == cs[i]) {
+      sys.push_back({md, cnt[i], i, SYS_D | SYS_F});
+    } else {
+      sys.push_back({md, cs[i], i, SYS_D});
+      sys.push_back({mf, cnt[i], i, SYS_F});
+    }
+  }
+  std::stable_sort(sys.begin(), sys.end(), [](const Sys &a, const Sys &b) {
+    return a.m != b.m ? a.m < b.m : a.K > b.K;
+  });
+  std::vector<int> hn(sys.size()), hk(sys.size());
+  for (size_t s = 0; s < sys.size(); s++) { hn[s] = sys[s].node; hk[s] = sys[s].kind; }
+  int *dn = nullptr, *dk = nullptr, *dq = nullptr;
+  SV_HIP(c, hipMalloc(&dn, sizeof(int) * (sys.size() + 1)));
+  SV_HIP(c, hipMalloc(&dk, sizeof(int) * (sys.size() + 1)));
+  SV_HIP(c, hipMalloc(&dq, sizeof(int) * 4));
+  SV_HIP(c, hipMemcpy(dn, hn.data(), sizeof(int) * sys.size(), hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(dk, hk.data(), sizeof(int) * sys.size(), hipMemcpyHostToDevice));
+  const size_t L = (size_t)c->nnz + 1;
+  if (!c->wx) {
+    if (sv_alloc(c, (void **)&c->wx, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->wy, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->wl, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->wh, sizeof(double) * L) ||
+        sv_alloc(c, (void **)&c->vol, sizeof(double) * n) ||
+        sv_alloc(c, (void **)&c->gsx, sizeof(double) * n) ||
+        sv_alloc(c, (void **)&c->gsy, sizeof(double) * n) ||
+        sv_alloc(c, (void **)&c->lsum, sizeof(double) * n) ||
+        sv_alloc(c, (void **)&c->hsum, sizeof(double) * n) ||
+        sv_alloc(c, (void **)&c->singular, sizeof(int) * 4))
+      return SV_E_NOMEM;
+  }
+  SV_HIP(c, hipMemsetAsync(c->wx, 0, sizeof(double) * L, c->stream));
+  SV_HIP(c, hipMemsetAsync(c->wy, 0, sizeof(double) * L, c->stream));
+  SV_HIP(c, hipMemsetAsync(c->wl, 0, sizeof(double) * L, c->stream));
+  SV_HIP(c, hipMemsetAsync(c->wh, 0, sizeof(double) * L, c->stream));
+  SV_HIP(c, hipMemsetAsync(c->singular, 0, sizeof(int) * 4, c->stream));
+  SysArgs a{dn, dk, (int)sys.size(), c->x, c->y, c->h, c->hs, c->node_type, c->row_ptr,
+            c->col_raw, c->cnt, c->cnt_small, c->h2, c->wx, c->wy, c->wl, c->wh, c->vol,
+            c->singular, dq};
+  hipEvent_t e0, e1;
+  SV_HIP(c, hipEventCreate(&e0));
+  SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventRecord(e0, c->stream));
+  int first = 0;
+  for (int m : {4, 6, 8}) {
+    int count = 0;
+    while (first + count < (int)sys.size() && sys[first + count].m == m) count++;
+    int rc = 0;
+    if (m == 4) rc = launch_order<4>(c, a, first, count, c->weights_persistent);
+    if (m == 6) rc = launch_order<6>(c, a, first, count, c->weights_persistent);
+    if (m == 8) rc = launch_order<8>(c, a, first, count, c->weights_persistent);
+    if (rc) return rc;
+    first += count;
+  }
+  SV_HIP(c, hipEventRecord(e1, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  c->t_weights = ms;
+  c->n_systems = (int)sys.size();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int nsing = 0;
+  SV_HIP(c, hipMemcpy(&nsing, c->singular, sizeof(int), hipMemcpyDeviceToHost));
+  SV_HIP(c, hipFree(dn));
+  SV_HIP(c, hipFree(dk));
+  SV_HIP(c, hipFree(dq));
+  c->have_weights = true;
+  c->have_sums = false;
+  c->have_filter = false;
+  c->sell_dirty = true;
+  c->n_singular = nsing;
+  return 0;
+}
+
+int sv_dev_sums(sv_ctx *c) {
+  const int n = c->npfb, B = 256;
+  k_sums<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->L_char, c->wx, c->wy,
+                                               c->wl, c->wh, c->gsx, c->gsy, c->lsum, c->hsum);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  c->have_sums = true;
+  c->sell_dirty = true;
+  return 0;
+}
+
+int sv_dev_filter_coeffs(sv_ctx *c) {
+  const int n = c->npfb, B = 256;
+  k_filter_coeffs<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->col_raw, c->x,
+                                                        c->y, c->h, c->wh, c->hsum);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  c->have_filter = true;
+  c->sell_dirty = true;
+  return 0;
+}

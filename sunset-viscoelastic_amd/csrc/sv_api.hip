@@ -1,0 +1,658 @@
+// sv_api.hip -- the extern "C" boundary of libsunset_amd.so (declared in
+// include/sunset_amd.h).  Host-side orchestration only: argument checks,
+// uploads/downloads in the reference layouts, and launches of the kernels in
+// k_nodes / k_weights / k_stencil / k_step.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "sv_internal.h"
+
+int sv_fail(sv_ctx *c, int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int sv_alloc(sv_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return sv_fail(c, SV_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  c->allocs.push_back(*p);
+  return 0;
+}
+
+void sv_free(sv_ctx *c, void *p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+#define SV_CHECK(c, cond, ...)                        \
+  do {                                                \
+    if (!(cond)) return sv_fail((c), SV_E_ARG, __VA_ARGS__); \
+  } while (0)
+
+static int set_device(sv_ctx *c) {
+  SV_HIP(c, hipSetDevice(c->device));
+  return 0;
+}
+
+extern "C" {
+
+const char *sv_version(void) { return "sunset_amd 0.1 (gfx950)"; }
+
+int sv_create(const sv_config *cfg, sv_ctx **out) {
+  if (!cfg || !out) return SV_E_ARG;
+  *out = nullptr;
+  sv_ctx *c = new sv_ctx();
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  if (cfg->form != SV_FORM_DI && cfg->form != SV_FORM_LC) {
+    delete c;
+    return SV_E_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return SV_E_HIP;
+  }
+  *out = c;
+  return 0;
+}
+
+void sv_destroy(sv_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *sv_last_error(const sv_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int sv_sync(sv_ctx *c) {
+  SV_CHECK(c, c, "null context");
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sv_set_domain(sv_ctx *c, double xmin, double xmax, double ymin, double ymax,
+                  const int32_t bc[4], double smax, double L_char) {
+  SV_CHECK(c, c && bc, "null argument");
+  SV_CHECK(c, xmax > xmin && ymax > ymin && smax > 0 && L_char > 0, "bad domain");
+  const bool perx = bc[0] == 1, pery = bc[2] == 1;
+  SV_CHECK(c, perx == (bc[1] == 1) && pery == (bc[3] == 1),
+           "periodic boundary conditions incorrectly specified (mirror_boundaries.F90:61-69)");
+  c->xmin = xmin; c->xmax = xmax; c->ymin = ymin; c->ymax = ymax;
+  for (int k = 0; k < 4; k++) c->bc[k] = bc[k];
+  c->smax = smax;
+  // h0 = hovs*smax (setup_domain.F90:105), hovs from the filter order
+  const int fo = c->cfg.forder;
+  const double hovs = fo == 4 ? 1.8 : fo == 6 ? 2.4 : fo == 10 ? 3.1 : 2.7;
+  c->h0 = hovs * smax;
+  c->h2 = c->h0 * c->h0;
+  c->L_char = L_char;
+  c->have_domain = true;
+  return 0;
+}
+
+int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, const double *s,
+                 const double *h, const double *h_small, const int32_t *node_type) {
+  SV_CHECK(c, c && x && y && s && h && h_small, "null argument");
+  SV_CHECK(c, c->have_domain, "sv_set_domain must come first");
+  SV_CHECK(c, npfb > 0, "npfb must be positive");
+  if (int rc = set_device(c)) return rc;
+  for (int i = 0; i < npfb; i++)
+    SV_CHECK(c, h[i] > 0 && h_small[i] > 0 && h_small[i] <= h[i] && s[i] > 0,
+             "node %d: need 0 < h_small <= h and s > 0", i);
+  // drop everything derived from an earlier node set
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  c->x = c->y = c->h = c->hs = c->s = nullptr;
+  c->node_type = c->irel = c->vrel = nullptr;
+  c->cnt = c->cnt_small = nullptr; c->row_ptr = nullptr; c->col_raw = nullptr;
+  c->wx = c->wy = c->wl = c->wh = nullptr;
+  c->gsx = c->gsy = c->lsum = c->hsum = c->vol = nullptr;
+  c->sell_off = nullptr; c->sell_w = nullptr; c->scol = c->scol_raw = nullptr;
+  c->swx = c->swy = c->swl = c->swh = nullptr; c->nslices = 0; c->sell_nnz = 0;
+  c->singular = nullptr; c->fx = nullptr; c->partial = nullptr; c->ctrl = nullptr;
+  c->U = c->Uf = c->reg = c->eacc = sv_fields{};
+  c->GA = c->GB = sv_gather{};
+  c->tmp_phi = c->tmp_out = nullptr; c->tmp_cap = 0;
+  if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+  c->graph_valid = false;
+  c->have_lists = c->have_weights = c->have_sums = c->have_filter = false;
+  c->have_state = false;
+  c->npfb = npfb;
+  c->cap = 9 * npfb + 64;      // at most 8 ghosts per node
+  const size_t D = sizeof(double) * c->cap, I = sizeof(int) * c->cap;
+  if (sv_alloc(c, (void **)&c->x, D) || sv_alloc(c, (void **)&c->y, D) ||
+      sv_alloc(c, (void **)&c->h, D) || sv_alloc(c, (void **)&c->hs, D) ||
+      sv_alloc(c, (void **)&c->s, D) || sv_alloc(c, (void **)&c->node_type, I) ||
+      sv_alloc(c, (void **)&c->irel, I) || sv_alloc(c, (void **)&c->vrel, I))
+    return SV_E_NOMEM;
+  const size_t n8 = sizeof(double) * npfb;
+  SV_HIP(c, hipMemcpy(c->x, x, n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->y, y, n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->s, s, n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->h, h, n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->hs, h_small, n8, hipMemcpyHostToDevice));
+  if (node_type) {
+    SV_HIP(c, hipMemcpy(c->node_type, node_type, sizeof(int) * npfb, hipMemcpyHostToDevice));
+  } else {
+    std::vector<int> nt(npfb, 999);
+    SV_HIP(c, hipMemcpy(c->node_type, nt.data(), sizeof(int) * npfb, hipMemcpyHostToDevice));
+  }
+  double hm = 0;
+  for (int i = 0; i < npfb; i++) hm = h[i] > hm ? h[i] : hm;
+  c->hmax = hm;
+  return sv_dev_create_ghosts(c);
+}
+
+int sv_get_ghosts(sv_ctx *c, int32_t *np, double *x, double *y, int32_t *irelation,
+                  int32_t *vrelation, int32_t cap) {
+  SV_CHECK(c, c && c->np > 0, "no nodes");
+  if (np) *np = c->np;
+  const int ng = c->np - c->npfb;
+  if (x || y) SV_CHECK(c, cap >= c->np, "capacity %d < np %d", cap, c->np);
+  if (x) SV_HIP(c, hipMemcpy(x, c->x, sizeof(double) * c->np, hipMemcpyDeviceToHost));
+  if (y) SV_HIP(c, hipMemcpy(y, c->y, sizeof(double) * c->np, hipMemcpyDeviceToHost));
+  if (irelation || vrelation) {
+    std::vector<int> ir(c->np), vr(c->np);
+    SV_HIP(c, hipMemcpy(ir.data(), c->irel, sizeof(int) * c->np, hipMemcpyDeviceToHost));
+    SV_HIP(c, hipMemcpy(vr.data(), c->vrel, sizeof(int) * c->np, hipMemcpyDeviceToHost));
+    for (int g = 0; g < ng; g++) {
+      if (irelation) irelation[g] = ir[c->npfb + g] + 1;
+      if (vrelation) vrelation[g] = vr[c->npfb + g];
+    }
+  }
+  return 0;
+}
+
+int sv_find_neighbours(sv_ctx *c, int32_t nplink, int32_t *ij_count, int32_t *ij_count_small,
+                       int32_t *ij_link) {
+  SV_CHECK(c, c && c->npfb > 0, "sv_set_nodes must come first");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = sv_dev_find_neighbours(c)) return rc;
+  c->nplink = nplink;
+  if (ij_count)
+    SV_HIP(c, hipMemcpy(ij_count, c->cnt, sizeof(int) * c->npfb, hipMemcpyDeviceToHost));
+  if (ij_count_small)
+    SV_HIP(c, hipMemcpy(ij_count_small, c->cnt_small, sizeof(int) * c->npfb,
+                        hipMemcpyDeviceToHost));
+  if (ij_link) {
+    if (nplink < c->maxK)
+      return sv_fail(c, SV_E_OVERFLOW, "a node has %d neighbours > nplink %d", c->maxK, nplink);
+    std::vector<int> col(c->nnz), cnt(c->npfb);
+    std::vector<int64_t> rp(c->npfb + 1);
+    SV_HIP(c, hipMemcpy(col.data(), c->col_raw, sizeof(int) * c->nnz, hipMemcpyDeviceToHost));
+    SV_HIP(c, hipMemcpy(rp.data(), c->row_ptr, sizeof(int64_t) * (c->npfb + 1),
+                        hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->npfb; i++) {
+      int32_t *dst = ij_link + (size_t)i * nplink;
+      const int K = (int)(rp[i + 1] - rp[i]);
+      for (int k = 0; k < K; k++) dst[k] = col[rp[i] + k] + 1;
+      for (int k = K; k < nplink; k++) dst[k] = 0;
+    }
+  }
+  return 0;
+}
+
+int sv_calc_labf_weights(sv_ctx *c, const int32_t *order_map) {
+  SV_CHECK(c, c && c->have_lists, "sv_find_neighbours must come first");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = sv_dev_weights(c, order_map)) return rc;
+  if (c->n_singular > 0)
+    return sv_fail(c, SV_E_SINGULAR, "%d moment systems had a zero pivot", c->n_singular);
+  return 0;
+}
+
+int sv_calc_labf_sums(sv_ctx *c) {
+  SV_CHECK(c, c && c->have_weights, "sv_calc_labf_weights must come first");
+  if (int rc = set_device(c)) return rc;
+  return sv_dev_sums(c);
+}
+
+int sv_filter_coefficients(sv_ctx *c) {
+  SV_CHECK(c, c && c->have_sums, "sv_calc_labf_sums must come first");
+  if (int rc = set_device(c)) return rc;
+  return sv_dev_filter_coeffs(c);
+}
+
+static int ensure_sell(sv_ctx *c) {
+  if (!c->sell_dirty) return 0;
+  return sv_dev_build_sell(c);
+}
+
+int sv_get_weights(sv_ctx *c, int32_t nplink, double *w_grad, double *w_lap, double *w_hyp,
+                   double *gsum, double *lsum, double *hsum, double *vol) {
+  SV_CHECK(c, c && c->have_weights, "no weights");
+  if (int rc = set_device(c)) return rc;
+  const int n = c->npfb;
+  std::vector<int64_t> rp(n + 1);
+  SV_HIP(c, hipMemcpy(rp.data(), c->row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+  if (w_grad || w_lap || w_hyp) SV_CHECK(c, nplink >= c->maxK, "nplink %d < %d", nplink, c->maxK);
+  auto fetch = [&](const double *src, std::vector<double> &dst) -> int {
+    dst.resize(c->nnz);
+    SV_HIP(c, hipMemcpy(dst.data(), src, sizeof(double) * c->nnz, hipMemcpyDeviceToHost));
+    return 0;
+  };
+  std::vector<double> a, b;
+  if (w_grad) {
+    if (fetch(c->wx, a) || fetch(c->wy, b)) return SV_E_HIP;
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < nplink; k++) {
+        const bool in = k < rp[i + 1] - rp[i];
+        w_grad[((size_t)i * nplink + k) * 2 + 0] = in ? a[rp[i] + k] : 0.0;
+        w_grad[((size_t)i * nplink + k) * 2 + 1] = in ? b[rp[i] + k] : 0.0;
+      }
+  }
+  if (w_lap) {
+    if (fetch(c->wl, a)) return SV_E_HIP;
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < nplink; k++)
+        w_lap[(size_t)i * nplink + k] = k < rp[i + 1] - rp[i] ? a[rp[i] + k] : 0.0;
+  }
+  if (w_hyp) {
+    if (fetch(c->wh, a)) return SV_E_HIP;
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < nplink; k++)
+        w_hyp[(size_t)i * nplink + k] = k < rp[i + 1] - rp[i] ? a[rp[i] + k] : 0.0;
+  }
+  if (gsum) {
+    SV_CHECK(c, c->have_sums, "sv_calc_labf_sums not done");
+    std::vector<double> gx(n), gy(n);
+    SV_HIP(c, hipMemcpy(gx.data(), c->gsx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    SV_HIP(c, hipMemcpy(gy.data(), c->gsy, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) { gsum[2 * i] = gx[i]; gsum[2 * i + 1] = gy[i]; }
+  }
+  if (lsum) SV_HIP(c, hipMemcpy(lsum, c->lsum, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (hsum) SV_HIP(c, hipMemcpy(hsum, c->hsum, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (vol) SV_HIP(c, hipMemcpy(vol, c->vol, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
+  SV_CHECK(c, c && what && ms, "null argument");
+  if (!strcmp(what, "neighbours")) *ms = c->t_neigh;
+  else if (!strcmp(what, "weights")) *ms = c->t_weights;
+  else if (!strcmp(what, "step")) *ms = c->t_step;
+  else if (!strcmp(what, "systems")) *ms = c->n_systems;
+  else return sv_fail(c, SV_E_ARG, "unknown timing '%s'", what);
+  return 0;
+}
+
+int sv_set_option(sv_ctx *c, const char *key, double value) {
+  SV_CHECK(c, c && key, "null argument");
+  if (!strcmp(key, "weights_persistent")) c->weights_persistent = value != 0.0;
+  else return sv_fail(c, SV_E_ARG, "unknown option '%s'", key);
+  return 0;
+}
+
+static int ensure_tmp(sv_ctx *c) {
+  const int need = c->np > 2 * c->npfb ? c->np : 2 * c->npfb;
+  if (c->tmp_cap >= need) return 0;
+  if (c->tmp_phi) sv_free(c, c->tmp_phi);
+  if (c->tmp_out) sv_free(c, c->tmp_out);
+  if (sv_alloc(c, (void **)&c->tmp_phi, sizeof(double) * need) ||
+      sv_alloc(c, (void **)&c->tmp_out, sizeof(double) * need))
+    return SV_E_NOMEM;
+  c->tmp_cap = need;
+  return 0;
+}
+
+int sv_apply(sv_ctx *c, int32_t op, double *phi, double *out) {
+  SV_CHECK(c, c && phi, "null argument");
+  SV_CHECK(c, c->have_weights && c->have_sums, "weights and sums must be computed first");
+  SV_CHECK(c, op >= SV_OP_GRAD && op <= SV_OP_FILTER_TERM, "bad operator");
+  if ((op == SV_OP_FILTER || op == SV_OP_FILTER_TERM))
+    SV_CHECK(c, c->have_filter, "sv_filter_coefficients must come first");
+  if (op != SV_OP_FILTER) SV_CHECK(c, out, "null output");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_sell(c)) return rc;
+  if (int rc = ensure_tmp(c)) return rc;
+  SV_HIP(c, hipMemcpy(c->tmp_phi, phi, sizeof(double) * c->np, hipMemcpyHostToDevice));
+  if (int rc = sv_dev_apply(c, op, c->tmp_phi, c->tmp_out, true, c->stream)) return rc;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t nout = (op == SV_OP_GRAD ? 2 : 1) * (size_t)c->npfb;
+  SV_HIP(c, hipMemcpy(op == SV_OP_FILTER ? phi : out, c->tmp_out, sizeof(double) * nout,
+                      hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sv_calc_gradient(sv_ctx *c, const double *phi, double *g) {
+  return sv_apply(c, SV_OP_GRAD, const_cast<double *>(phi), g);
+}
+int sv_calc_laplacian(sv_ctx *c, const double *phi, double *l) {
+  return sv_apply(c, SV_OP_LAP, const_cast<double *>(phi), l);
+}
+int sv_calc_gradient_only(sv_ctx *c, int32_t dim, const double *phi, double *g) {
+  SV_CHECK(c, c, "null context");
+  if (dim == 3) {  // 2D: third component is zero (derivatives.F90:455-458)
+    SV_CHECK(c, g, "null output");
+    memset(g, 0, sizeof(double) * c->npfb);
+    return 0;
+  }
+  SV_CHECK(c, dim == 1 || dim == 2, "dim must be 1, 2 or 3");
+  return sv_apply(c, dim == 1 ? SV_OP_GRAD_X : SV_OP_GRAD_Y, const_cast<double *>(phi), g);
+}
+int sv_calc_filtered_var(sv_ctx *c, double *phi) { return sv_apply(c, SV_OP_FILTER, phi, nullptr); }
+int sv_calc_filter_term(sv_ctx *c, const double *phi, double *f) {
+  return sv_apply(c, SV_OP_FILTER_TERM, const_cast<double *>(phi), f);
+}
+
+int sv_apply_dev(sv_ctx *c, int32_t op, int32_t nfields, const double *const *phi,
+                 double *const *out, void *stream) {
+  SV_CHECK(c, c && phi && out && nfields >= 0, "null argument");
+  SV_CHECK(c, c->have_weights && c->have_sums, "weights and sums must be computed first");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_sell(c)) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  for (int f = 0; f < nfields; f++)
+    if (int rc = sv_dev_apply(c, op, phi[f], out[f], false, st)) return rc;
+  return 0;
+}
+
+static void derive_phys(sv_ctx *c) {
+  const sv_phys &p = c->phys;
+  sv_physk &k = c->pk;
+  k.rho_char = 1.0;                                    // load_data.F90:132-148
+  k.visc_total = k.rho_char * p.U_char * p.L_char / p.Re;
+  k.coef_solvent = p.beta * k.visc_total;
+  const double visc_polymeric = (1.0 - p.beta) * k.visc_total;
+  k.lambda = p.Wi * p.L_char / p.U_char;
+  k.coef_polymeric = visc_polymeric / k.lambda;
+  k.csq = (p.U_char / p.Ma) * (p.U_char / p.Ma);
+  k.cs = sqrt(k.csq);
+  k.eps = p.epsPTT;
+  k.Mdiff = p.Mdiff;
+  k.bfx = p.grav[0] + p.driving_force[0];
+  k.bfy = p.grav[1] + p.driving_force[1];
+  k.kol_amp = p.kol_amp;
+  k.kol_k = p.kol_k;
+  k.U_char = p.U_char;
+  k.L_char = p.L_char;
+  k.err_norm_rou = 1.0 / (1.0 * p.U_char);
+}
+
+static int alloc_fields(sv_ctx *c, sv_fields *f) {
+  const size_t n = sizeof(double) * c->npfb;
+  double **ptrs[] = {&f->ro, &f->rou, &f->rov, &f->pxx, &f->pxy, &f->pyy};
+  for (double **p : ptrs)
+    if (sv_alloc(c, (void **)p, n)) return SV_E_NOMEM;
+  f->cxx = f->cxy = f->cyy = nullptr;
+  return 0;
+}
+
+static int alloc_gather(sv_ctx *c, sv_gather *g) {
+  const size_t n = sizeof(double) * c->npfb;
+  double **ptrs[] = {&g->ro, &g->u, &g->v, &g->pxx, &g->pxy, &g->pyy, &g->cxx, &g->cxy, &g->cyy};
+  for (double **p : ptrs)
+    if (sv_alloc(c, (void **)p, n)) return SV_E_NOMEM;
+  return 0;
+}
+
+static int ensure_state_buffers(sv_ctx *c) {
+  if (c->U.ro) return 0;
+  if (alloc_fields(c, &c->U) || alloc_fields(c, &c->Uf) || alloc_fields(c, &c->reg) ||
+      alloc_fields(c, &c->eacc) || alloc_gather(c, &c->GA) || alloc_gather(c, &c->GB))
+    return SV_E_NOMEM;
+  const int nb = (c->npfb + 255) / 256;
+  c->npartial = 4 * nb;
+  if (sv_alloc(c, (void **)&c->partial, sizeof(double) * (c->npartial + 8)) ||
+      sv_alloc(c, (void **)&c->ctrl, sizeof(sv_ctrl)) ||
+      sv_alloc(c, (void **)&c->fx, sizeof(double) * c->npfb))
+    return SV_E_NOMEM;
+  return 0;
+}
+
+int sv_set_phys(sv_ctx *c, const sv_phys *p) {
+  SV_CHECK(c, c && p, "null argument");
+  SV_CHECK(c, p->Re > 0 && p->Wi > 0 && p->Ma > 0 && p->U_char > 0 && p->L_char > 0,
+           "Re, Wi, Ma, U_char and L_char must be positive");
+  c->phys = *p;
+  derive_phys(c);
+  c->have_phys = true;
+  c->graph_valid = false;
+  return 0;
+}
+
+// log_conf_psi_from_c on the host (setup only), conf_transforms.F90:177-215
+static void psi_from_c(double a, double b, double d, double *pxx, double *pxy, double *pyy) {
+  double l1, l2, R[4];
+  if (fabs(b) <= 1.0e-16) {
+    l1 = a; l2 = d; R[0] = 1; R[1] = 0; R[2] = 0; R[3] = 1;
+  } else {
+    const double tr = a + d, det = a * d - b * b;
+    double t = 0.25 * tr * tr - det;
+    if (t <= 1.0e-16) t = 0.0;
+    t = sqrt(t);
+    l1 = 0.5 * tr + t;
+    l2 = 0.5 * tr - t;
+    double r11 = l1 - d, r21 = b;
+    const double nrm = sqrt(r11 * r11 + r21 * r21);
+    r11 /= nrm; r21 /= nrm;
+    R[0] = r11; R[1] = r21; R[2] = -r21; R[3] = r11;
+  }
+  const double e1 = log(l1), e2 = log(l2);
+  const double i11 = e1 * R[0], i12 = e1 * R[1], i21 = e2 * R[2], i22 = e2 * R[3];
+  *pxx = R[0] * i11 + R[2] * i21;
+  *pxy = R[1] * i11 + R[3] * i21;
+  *pyy = R[1] * i12 + R[3] * i22;
+}
+
+int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v,
+                   const double *cxx, const double *cxy, const double *cyy) {
+  SV_CHECK(c, c && ro && u && v && cxx && cxy && cyy, "null argument");
+  SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
+  SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_sell(c)) return rc;
+  if (int rc = ensure_state_buffers(c)) return rc;
+  const int n = c->npfb;
+  std::vector<double> a(n), b(n), d(n), e(n), f(n), g(n);
+  for (int i = 0; i < n; i++) {
+    SV_CHECK(c, ro[i] > 0, "node %d: density must be positive", i);
+    a[i] = ro[i];
+    b[i] = ro[i] * u[i];
+    d[i] = ro[i] * v[i];
+    if (c->cfg.form == SV_FORM_LC) {
+      SV_CHECK(c, cxx[i] * cyy[i] - cxy[i] * cxy[i] > 0 && cxx[i] > 0,
+               "node %d: conformation tensor not positive definite", i);
+      psi_from_c(cxx[i], cxy[i], cyy[i], &e[i], &f[i], &g[i]);
+    } else {
+      e[i] = cxx[i]; f[i] = cxy[i]; g[i] = cyy[i];
+    }
+  }
+  const size_t n8 = sizeof(double) * n;
+  SV_HIP(c, hipMemcpy(c->U.ro, a.data(), n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->U.rou, b.data(), n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->U.rov, d.data(), n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->U.pxx, e.data(), n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->U.pxy, f.data(), n8, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(c->U.pyy, g.data(), n8, hipMemcpyHostToDevice));
+  // Kolmogorov forcing per node, and the node-volume sum for the mass correction
+  std::vector<double> y(n), vol(n);
+  SV_HIP(c, hipMemcpy(y.data(), c->y, n8, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(vol.data(), c->vol, n8, hipMemcpyDeviceToHost));
+  double tv = 0.0;
+  for (int i = 0; i < n; i++) tv += vol[i];
+  c->volsum = tv;
+  for (int i = 0; i < n; i++) a[i] = c->pk.kol_amp * sin(c->pk.kol_k * y[i]);
+  SV_HIP(c, hipMemcpy(c->fx, a.data(), n8, hipMemcpyHostToDevice));
+  sv_ctrl h{};
+  h.dt = 1.0e-6;                         // setup_flow.F90:149-151
+  h.emax_np1 = h.emax_n = h.emax_nm1 = SV_PID_TOL;
+  h.dt_mode = SV_DT_PID;
+  SV_HIP(c, hipMemcpy(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice));
+  if (c->pk.kol_amp == 0.0) {
+    // no forcing: skip the per-node load in the stage kernels
+  }
+  if (int rc = sv_dev_init_state(c)) return rc;
+  c->have_state = true;
+  c->graph_valid = false;
+  return 0;
+}
+
+int sv_set_control(sv_ctx *c, double dt, double time, int32_t itime, double e1, double e2,
+                   double e3, int32_t dt_mode) {
+  SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_CHECK(c, dt > 0 && e1 > 0 && e2 > 0 && e3 > 0, "dt and error norms must be positive");
+  SV_CHECK(c, dt_mode == SV_DT_PID || dt_mode == SV_DT_FIXED, "bad dt mode");
+  if (int rc = set_device(c)) return rc;
+  sv_ctrl h{};
+  SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+  h.dt = dt; h.time = time; h.itime = itime;
+  h.emax_np1 = e1; h.emax_n = e2; h.emax_nm1 = e3;
+  h.dt_mode = dt_mode;
+  SV_HIP(c, hipMemcpy(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int sv_get_control(sv_ctx *c, double *dt, double *time, int32_t *itime, double *emax3,
+                   double *dt_cfl, double *dt_par) {
+  SV_CHECK(c, c && c->have_state, "no state");
+  if (int rc = set_device(c)) return rc;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  sv_ctrl h{};
+  SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+  if (dt) *dt = h.dt;
+  if (time) *time = h.time;
+  if (itime) *itime = h.itime;
+  if (emax3) { emax3[0] = h.emax_np1; emax3[1] = h.emax_n; emax3[2] = h.emax_nm1; }
+  if (dt_cfl) *dt_cfl = h.dt_cfl;
+  if (dt_par) *dt_par = h.dt_par;
+  if (h.flags & 1) return sv_fail(c, SV_E_STATE, "non-finite value in the state");
+  return 0;
+}
+
+int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *r2, double *r3,
+                    double *r4, double *r5) {
+  SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  if (int rc = set_device(c)) return rc;
+  double *out = nullptr;
+  SV_HIP(c, hipMalloc(&out, sizeof(double) * 6 * c->npfb));
+  int rc = sv_dev_rhs(c, iRKstep, out);
+  if (!rc) {
+    double *dst[6] = {r0, r1, r2, r3, r4, r5};
+    for (int f = 0; f < 6; f++)
+      if (dst[f])
+        SV_HIP(c, hipMemcpy(dst[f], out + (size_t)f * c->npfb, sizeof(double) * c->npfb,
+                            hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(out);
+  return rc;
+}
+
+int sv_run(sv_ctx *c, int32_t nsteps, int32_t use_graph) {
+  SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_CHECK(c, nsteps >= 0, "nsteps must be >= 0");
+  if (int rc = set_device(c)) return rc;
+  hipEvent_t e0, e1;
+  SV_HIP(c, hipEventCreate(&e0));
+  SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventRecord(e0, c->stream));
+  if (use_graph && nsteps > 0) {
+    if (!c->graph_valid) {
+      if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+      hipGraph_t g;
+      SV_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = sv_dev_step(c, nullptr);
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (rc) return rc;
+      SV_HIP(c, ce);
+      SV_HIP(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+      SV_HIP(c, hipGraphDestroy(g));
+      c->graph_valid = true;
+    }
+    for (int s = 0; s < nsteps; s++) SV_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+  } else {
+    for (int s = 0; s < nsteps; s++)
+      if (int rc = sv_dev_step(c, nullptr)) return rc;
+  }
+  SV_HIP(c, hipEventRecord(e1, c->stream));
+  SV_HIP(c, hipEventSynchronize(e1));
+  float ms = 0;
+  SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  c->t_step = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int sv_run_profiled(sv_ctx *c, int32_t nsteps, double *kernel_ms, double *total_ms) {
+  SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_CHECK(c, nsteps > 0, "nsteps must be > 0");
+  if (int rc = set_device(c)) return rc;
+  const int NE = SV_NKERNELS + 1;
+  std::vector<hipEvent_t> ev((size_t)nsteps * NE);
+  for (auto &e : ev) SV_HIP(c, hipEventCreate(&e));
+  for (int s = 0; s < nsteps; s++)
+    if (int rc = sv_dev_step(c, ev.data() + (size_t)s * NE)) return rc;
+  SV_HIP(c, hipEventSynchronize(ev.back()));
+  double acc[SV_NKERNELS] = {0};
+  for (int s = 0; s < nsteps; s++)
+    for (int k = 0; k < SV_NKERNELS; k++) {
+      float ms = 0;
+      SV_HIP(c, hipEventElapsedTime(&ms, ev[(size_t)s * NE + k], ev[(size_t)s * NE + k + 1]));
+      acc[k] += ms;
+    }
+  float tot = 0;
+  SV_HIP(c, hipEventElapsedTime(&tot, ev.front(), ev.back()));
+  for (int k = 0; k < SV_NKERNELS; k++)
+    if (kernel_ms) kernel_ms[k] = acc[k] / nsteps;
+  if (total_ms) *total_ms = tot;
+  c->t_step = tot;
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+int sv_get_stencil_stats(sv_ctx *c, double *stats) {
+  SV_CHECK(c, c && c->have_lists && stats, "no stencils");
+  std::vector<int> cnt(c->npfb), cs(c->npfb);
+  SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * c->npfb, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(cs.data(), c->cnt_small, sizeof(int) * c->npfb, hipMemcpyDeviceToHost));
+  double a = 0, b = 0;
+  int mx = 0;
+  for (int i = 0; i < c->npfb; i++) { a += cnt[i]; b += cs[i]; mx = cnt[i] > mx ? cnt[i] : mx; }
+  stats[0] = a / c->npfb;          // mean ij_count
+  stats[1] = b / c->npfb;          // mean ij_count_small
+  stats[2] = mx;                   // max ij_count
+  stats[3] = (double)c->sell_nnz;  // SELL entries incl. padding (0 before the first use)
+  stats[4] = c->np - c->npfb;      // ghosts
+  return 0;
+}
+
+int sv_get_state(sv_ctx *c, double *ro, double *rou, double *rov, double *u, double *v,
+                 double *pxx, double *pxy, double *pyy, double *cxx, double *cxy, double *cyy) {
+  SV_CHECK(c, c && c->have_state, "no state");
+  if (int rc = set_device(c)) return rc;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t n8 = sizeof(double) * c->npfb;
+  struct { double *dst; const double *src; } m[] = {
+      {ro, c->U.ro}, {rou, c->U.rou}, {rov, c->U.rov}, {u, c->GA.u}, {v, c->GA.v},
+      {pxx, c->cfg.form == SV_FORM_LC ? c->U.pxx : nullptr},
+      {pxy, c->cfg.form == SV_FORM_LC ? c->U.pxy : nullptr},
+      {pyy, c->cfg.form == SV_FORM_LC ? c->U.pyy : nullptr},
+      {cxx, c->GA.cxx}, {cxy, c->GA.cxy}, {cyy, c->GA.cyy}};
+  for (auto &e : m)
+    if (e.dst && e.src) SV_HIP(c, hipMemcpy(e.dst, e.src, n8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// sv_internal.h -- device data layout and kernel entry points shared by the
+// translation units of libsunset_amd.so.
+//
+// Layout in HBM (per context):
+//   node arrays  x, y, h, hs, s, node_type     [np]      (real nodes first, then
+//                                                        ghosts, reference order)
+//   irel                                       [np]      ghost -> parent (0-based),
+//                                                        identity for real nodes
+//   CSR          row_ptr [npfb+1], col_raw [nnz]         sorted by (r^2, j);
+//                                                        col_raw keeps ghost ids
+//                wx, wy, wl, wh [nnz]                     weights in CSR order
+//   SELL-64      slices of 64 consecutive real nodes; entry (lane l, link k) of
+//                slice s at sell_off[s] + k*64 + l; width = max links in slice.
+//                scol_raw (ghost ids), scol (ghosts folded to parents),
+//                swx, swy, swl, swh.  Padding: col = own node, weight 0.
+//   sums         gsx, gsy, lsum, hsum, vol      [npfb]
+//   state        structure of arrays on real nodes only (ghosts are resolved
+//                through scol at gather time).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/sunset_amd.h"
+
+#define SV_WAVE 64
+#define SV_PI 3.141592653589793238462643383279502884197
+
+// RK3(2)4[2R+]C, common_parameter.F90:75-94
+#define SV_RKA1 (11847461282814.0 / 36547543011857.0)
+#define SV_RKA2 (3943225443063.0 / 7078155732230.0)
+#define SV_RKA3 (-346793006927.0 / 4029903576067.0)
+#define SV_RKB1 (1017324711453.0 / 9774461848756.0)
+#define SV_RKB2 (8237718856693.0 / 13685301971492.0)
+#define SV_RKB3 (57731312506979.0 / 19404895981398.0)
+#define SV_RKB4 (-101169746363290.0 / 37734290219643.0)
+#define SV_RKBH1 (15763415370699.0 / 46270243929542.0)
+#define SV_RKBH2 (514528521746.0 / 5659431552419.0)
+#define SV_RKBH3 (27030193851939.0 / 9429696342944.0)
+#define SV_RKBH4 (-69544964788955.0 / 30262026368149.0)
+// PID, common_parameter.F90:97-101
+#define SV_PID_TOL 1.0e-5
+#define SV_PID_A (0.49 / 2.0)
+#define SV_PID_B (0.34 / 2.0)
+#define SV_PID_C (0.1 / 2.0)
+#define SV_PID_K 0.9
+
+// Device-resident control block: everything the time loop decides lives here so
+// a captured step replays without host involvement.
+struct sv_ctrl {
+  double dt, time, time0, dt_cfl, dt_par;
+  double emax_np1, emax_n, emax_nm1;
+  double RKa[3], RKb[4], RKbmbh[4], RKc[4];
+  double dro;
+  int itime, dt_mode;
+  int flags;   // bit 0: non-finite value met in a reduction
+};
+
+// Physical constants passed by value to the stage kernels
+struct sv_physk {
+  double csq, cs, visc_total, coef_solvent, coef_polymeric, lambda, eps, Mdiff;
+  double bfx, bfy, kol_amp, kol_k, rho_char, U_char, L_char;
+  double err_norm_rou;   // 1/U_char (setup_flow.F90:156)
+};
+
+// Fields of the state (real nodes).  LC: p* = log-conformation; DI: p* unused.
+struct sv_fields {
+  double *ro, *rou, *rov, *pxx, *pxy, *pyy, *cxx, *cxy, *cyy;
+};
+// Gather buffer: what neighbours read in a stage.
+struct sv_gather {
+  double *ro, *u, *v, *pxx, *pxy, *pyy, *cxx, *cxy, *cyy;
+};
+
+struct sv_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sv_config cfg{};
+  std::string err;
+  // domain
+  bool have_domain = false;
+  double xmin = 0, xmax = 1, ymin = 0, ymax = 1, smax = 0, h0 = 0, h2 = 0, L_char = 1;
+  int bc[4] = {1, 1, 1, 1};
+  // nodes
+  int npfb = 0, np = 0, cap = 0;
+  double *x = nullptr, *y = nullptr, *h = nullptr, *hs = nullptr, *s = nullptr;
+  int *node_type = nullptr, *irel = nullptr, *vrel = nullptr;
+  double hmax = 0;
+  // neighbours
+  bool have_lists = false;
+  int nplink = 0;
+  int64_t nnz = 0;
+  int *cnt = nullptr, *cnt_small = nullptr;
+  int64_t *row_ptr = nullptr;
+  int *col_raw = nullptr;
+  int maxK = 0;
+  // weights (CSR) and SELL
+  bool have_weights = false, have_sums = false, have_filter = false;
+  double *wx = nullptr, *wy = nullptr, *wl = nullptr, *wh = nullptr;
+  double *gsx = nullptr, *gsy = nullptr, *lsum = nullptr, *hsum = nullptr, *vol = nullptr;
+  int nslices = 0;
+  int64_t sell_nnz = 0;
+  int64_t *sell_off = nullptr;
+  int *sell_w = nullptr;
+  int *scol = nullptr, *scol_raw = nullptr;
+  double *swx = nullptr, *swy = nullptr, *swl = nullptr, *swh = nullptr;
+  bool sell_dirty = true;
+  int *singular = nullptr;  // count of zero pivots in the last solve
+  int n_singular = 0, n_systems = 0;
+  bool weights_persistent = false;
+  // physics/state
+  bool have_phys = false, have_state = false;
+  sv_physk pk{};
+  sv_phys phys{};
+  sv_fields U{}, Uf{}, reg{}, eacc{};
+  sv_gather GA{}, GB{};
+  double *fx = nullptr;    // per-node Kolmogorov forcing
+  double *partial = nullptr; int npartial = 0;
+  double volsum = 0.0;     // sum of node volumes (the mass-correction denominator)
+  sv_ctrl *ctrl = nullptr;
+  // scratch for drop-in ops
+  double *tmp_phi = nullptr, *tmp_out = nullptr; int tmp_cap = 0;
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_valid = false;
+  // timings (ms)
+  double t_neigh = 0, t_weights = 0, t_step = 0;
+  // device buffer registry for cleanup
+  std::vector<void *> allocs;
+};
+
+// ---- helpers (sv_api.cpp) ----
+int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
+#define SV_HIP(c, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return sv_fail((c), SV_E_HIP, "%s failed: %s (%s:%d)", #call,            \
+                     hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+int sv_alloc(sv_ctx *c, void **p, size_t bytes);
+void sv_free(sv_ctx *c, void *p);
+
+// ---- kernels: nodes/neighbours (k_nodes.hip) ----
+int sv_dev_create_ghosts(sv_ctx *c);
+int sv_dev_find_neighbours(sv_ctx *c);
+int sv_dev_build_sell(sv_ctx *c);
+// ---- kernels: weights (k_weights.hip) ----
+int sv_dev_weights(sv_ctx *c, const int *order_map_host);
+int sv_dev_sums(sv_ctx *c);
+int sv_dev_filter_coeffs(sv_ctx *c);
+// ---- kernels: stencil operators (k_stencil.hip) ----
+int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_cols,
+                 hipStream_t st);
+// ---- kernels: time stepping (k_step.hip) ----
+int sv_dev_init_state(sv_ctx *c);
+// one step; ev (nullable) gets SV_NKERNELS+1 events recorded between launches
+#define SV_NKERNELS 8
+int sv_dev_step(sv_ctx *c, hipEvent_t *ev = nullptr);
+int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6);
+int sv_dev_dt_init(sv_ctx *c);
+
+// XCD-aware block index: consecutive logical blocks land on the same XCD (one
+// L2), so neighbouring slices share cached field lines.  Blocks are dispatched
+// round-robin over the 8 XCDs of an MI355X.
+__device__ __forceinline__ int sv_xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, idx = b >> 3;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
+}

@@ -1,0 +1,309 @@
+"""ctypes binding of libsunset_amd.so (the C ABI in include/sunset_amd.h).
+
+`Solver` mirrors the reference's call sequence (sunset.F90:36-106):
+set_domain/set_nodes (build_domain), find_neighbours, calc_labf_weights,
+calc_labf_sums, filter_coefficients, set_phys/set_initial (initial_solution),
+run (the time loop).  Errors raise `SunsetError` with the library's message, as
+the reference stops with a message.  There is no CPU fallback: the library must
+be built (``python -c "import __graft_entry__ as g; g.build()"``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libsunset_amd.so")
+
+FORM_DI, FORM_LC = 0, 1
+DT_PID, DT_FIXED = 0, 1
+OP_GRAD, OP_LAP, OP_GRAD_X, OP_GRAD_Y, OP_FILTER, OP_FILTER_TERM = range(6)
+
+PD = C.POINTER(C.c_double)
+PI32 = C.POINTER(C.c_int32)
+
+
+class SunsetError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("morder", C.c_int32), ("forder", C.c_int32), ("form", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class Phys(C.Structure):
+    _fields_ = [("L_char", C.c_double), ("U_char", C.c_double), ("Re", C.c_double),
+                ("Wi", C.c_double), ("beta", C.c_double), ("epsPTT", C.c_double),
+                ("Ma", C.c_double), ("Mdiff", C.c_double), ("grav", C.c_double * 3),
+                ("driving_force", C.c_double * 3), ("kol_amp", C.c_double),
+                ("kol_k", C.c_double)]
+
+
+_lib = None
+
+EXPORTS = [
+    "sv_create", "sv_destroy", "sv_last_error", "sv_version", "sv_set_domain", "sv_set_nodes",
+    "sv_get_ghosts", "sv_find_neighbours", "sv_calc_labf_weights", "sv_calc_labf_sums",
+    "sv_filter_coefficients", "sv_get_weights", "sv_get_timing", "sv_set_option", "sv_apply",
+    "sv_calc_gradient", "sv_calc_laplacian", "sv_calc_gradient_only", "sv_calc_filtered_var",
+    "sv_calc_filter_term", "sv_apply_dev", "sv_set_phys", "sv_set_initial", "sv_set_control",
+    "sv_get_control", "sv_calc_all_rhs", "sv_run", "sv_get_state", "sv_sync",
+    "sv_run_profiled", "sv_get_stencil_stats",
+]
+
+KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
+
+
+def lib(path: Optional[str] = None):
+    """Load the library (no compute happens here, so this works without a GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise SunsetError(f"libsunset_amd.so not built: {p} is missing "
+                          "(run __graft_entry__.build())")
+    L = C.CDLL(p)
+    L.sv_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.sv_destroy.argtypes = [C.c_void_p]
+    L.sv_destroy.restype = None
+    L.sv_last_error.argtypes = [C.c_void_p]
+    L.sv_last_error.restype = C.c_char_p
+    L.sv_version.restype = C.c_char_p
+    L.sv_set_domain.argtypes = [C.c_void_p] + [C.c_double] * 4 + [PI32, C.c_double, C.c_double]
+    L.sv_set_nodes.argtypes = [C.c_void_p, C.c_int32] + [PD] * 5 + [PI32]
+    L.sv_get_ghosts.argtypes = [C.c_void_p, PI32, PD, PD, PI32, PI32, C.c_int32]
+    L.sv_find_neighbours.argtypes = [C.c_void_p, C.c_int32, PI32, PI32, PI32]
+    L.sv_calc_labf_weights.argtypes = [C.c_void_p, PI32]
+    L.sv_calc_labf_sums.argtypes = [C.c_void_p]
+    L.sv_filter_coefficients.argtypes = [C.c_void_p]
+    L.sv_get_weights.argtypes = [C.c_void_p, C.c_int32] + [PD] * 7
+    L.sv_get_timing.argtypes = [C.c_void_p, C.c_char_p, PD]
+    L.sv_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+    L.sv_apply.argtypes = [C.c_void_p, C.c_int32, PD, PD]
+    L.sv_calc_gradient.argtypes = [C.c_void_p, PD, PD]
+    L.sv_calc_laplacian.argtypes = [C.c_void_p, PD, PD]
+    L.sv_calc_gradient_only.argtypes = [C.c_void_p, C.c_int32, PD, PD]
+    L.sv_calc_filtered_var.argtypes = [C.c_void_p, PD]
+    L.sv_calc_filter_term.argtypes = [C.c_void_p, PD, PD]
+    L.sv_apply_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                               C.POINTER(C.c_void_p), C.c_void_p]
+    L.sv_set_phys.argtypes = [C.c_void_p, C.POINTER(Phys)]
+    L.sv_set_initial.argtypes = [C.c_void_p] + [PD] * 6
+    L.sv_set_control.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_double,
+                                 C.c_double, C.c_double, C.c_int32]
+    L.sv_get_control.argtypes = [C.c_void_p, PD, PD, PI32, PD, PD, PD]
+    L.sv_calc_all_rhs.argtypes = [C.c_void_p, C.c_int32] + [PD] * 6
+    L.sv_run.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.sv_get_state.argtypes = [C.c_void_p] + [PD] * 11
+    L.sv_sync.argtypes = [C.c_void_p]
+    L.sv_run_profiled.argtypes = [C.c_void_p, C.c_int32, PD, PD]
+    L.sv_get_stencil_stats.argtypes = [C.c_void_p, PD]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(PD)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(PI32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One device context: a node set, its stencils and weights, and a state."""
+
+    def __init__(self, morder: int = 6, forder: Optional[int] = None, form: str = "lc",
+                 device: int = 0):
+        L = lib()
+        self.morder = morder
+        self.forder = morder if forder is None else forder
+        self.form = FORM_LC if form == "lc" else FORM_DI
+        cfg = Config(morder, self.forder, self.form, device)
+        h = C.c_void_p()
+        rc = L.sv_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise SunsetError(f"sv_create failed ({rc}): is a HIP device visible?")
+        self.h = h
+        self.npfb = 0
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != 0:
+            msg = lib().sv_last_error(self.h).decode()
+            raise SunsetError(f"error {rc}: {msg}")
+
+    # ---- setup ----
+    def set_nodes(self, ns):
+        bc = np.array(ns.bc, dtype=np.int32)
+        self._ck(lib().sv_set_domain(self.h, ns.xmin, ns.xmax, ns.ymin, ns.ymax, _i(bc),
+                                     float(ns.smax), 1.0))
+        self._arr = [_f64(a) for a in (ns.x, ns.y, ns.s, ns.h, ns.h_small)]
+        nt = np.ascontiguousarray(ns.node_type, dtype=np.int32)
+        self._ck(lib().sv_set_nodes(self.h, ns.npfb, *[_d(a) for a in self._arr], _i(nt)))
+        self.npfb = ns.npfb
+        npv = C.c_int32()
+        self._ck(lib().sv_get_ghosts(self.h, C.byref(npv), None, None, None, None, 0))
+        self.np = npv.value
+        return self
+
+    def ghosts(self):
+        x = np.zeros(self.np)
+        y = np.zeros(self.np)
+        ng = self.np - self.npfb
+        ir = np.zeros(max(ng, 1), dtype=np.int32)
+        vr = np.zeros(max(ng, 1), dtype=np.int32)
+        self._ck(lib().sv_get_ghosts(self.h, None, _d(x), _d(y), _i(ir), _i(vr), self.np))
+        return x, y, ir[:ng] - 1, vr[:ng]
+
+    def find_neighbours(self, nplink: int = 0, copy: bool = True):
+        if not copy:
+            self._ck(lib().sv_find_neighbours(self.h, nplink, None, None, None))
+            return None
+        cnt = np.zeros(self.npfb, dtype=np.int32)
+        cs = np.zeros(self.npfb, dtype=np.int32)
+        self._ck(lib().sv_find_neighbours(self.h, 0, _i(cnt), _i(cs), None))
+        L = int(cnt.max()) if nplink <= 0 else nplink
+        link = np.zeros((self.npfb, L), dtype=np.int32)
+        self._ck(lib().sv_find_neighbours(self.h, L, None, None, _i(link)))
+        return cnt, cs, link - 1
+
+    def calc_weights(self, order_map=None, persistent: bool = False):
+        self._ck(lib().sv_set_option(self.h, b"weights_persistent", 1.0 if persistent else 0.0))
+        om = None if order_map is None else np.ascontiguousarray(order_map, dtype=np.int32)
+        self._ck(lib().sv_calc_labf_weights(self.h, _i(om)))
+
+    def calc_sums(self):
+        self._ck(lib().sv_calc_labf_sums(self.h))
+
+    def filter_coefficients(self):
+        self._ck(lib().sv_filter_coefficients(self.h))
+
+    def setup(self, ns, order_map=None):
+        self.set_nodes(ns)
+        self.find_neighbours(copy=False)
+        self.calc_weights(order_map)
+        self.calc_sums()
+        self.filter_coefficients()
+        return self
+
+    def timing(self, what: str) -> float:
+        v = C.c_double()
+        self._ck(lib().sv_get_timing(self.h, what.encode(), C.byref(v)))
+        return v.value
+
+    def weights(self, nplink: int):
+        n = self.npfb
+        wg = np.zeros((n, nplink, 2))
+        wl = np.zeros((n, nplink))
+        wh = np.zeros((n, nplink))
+        gs = np.zeros((n, 2))
+        ls, hs, vol = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._ck(lib().sv_get_weights(self.h, nplink, _d(wg), _d(wl), _d(wh), _d(gs), _d(ls),
+                                      _d(hs), _d(vol)))
+        return dict(w_grad=wg, w_lap=wl, w_hyp=wh, gsum=gs, lsum=ls, hsum=hs, vol=vol)
+
+    # ---- drop-in operators (phi sized np) ----
+    def gradient(self, phi):
+        phi = _f64(phi)
+        out = np.zeros(2 * self.npfb)
+        self._ck(lib().sv_calc_gradient(self.h, _d(phi), _d(out)))
+        return out[: self.npfb], out[self.npfb:]
+
+    def laplacian(self, phi):
+        phi = _f64(phi)
+        out = np.zeros(self.npfb)
+        self._ck(lib().sv_calc_laplacian(self.h, _d(phi), _d(out)))
+        return out
+
+    def gradient_only(self, dim, phi):
+        phi = _f64(phi)
+        out = np.zeros(self.npfb)
+        self._ck(lib().sv_calc_gradient_only(self.h, dim, _d(phi), _d(out)))
+        return out
+
+    def filter_term(self, phi):
+        phi = _f64(phi)
+        out = np.zeros(self.npfb)
+        self._ck(lib().sv_calc_filter_term(self.h, _d(phi), _d(out)))
+        return out
+
+    def filtered_var(self, phi):
+        phi = np.array(phi, dtype=np.float64)
+        self._ck(lib().sv_calc_filtered_var(self.h, _d(phi)))
+        return phi[: self.npfb]
+
+    # ---- physics ----
+    def set_phys(self, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1, Mdiff=0.0, kol_amp=0.0,
+                 kol_k=0.0, U_char=1.0, L_char=1.0, grav=(0, 0, 0), drive=(0, 0, 0)):
+        p = Phys()
+        p.L_char, p.U_char, p.Re, p.Wi, p.beta, p.epsPTT, p.Ma, p.Mdiff = (
+            L_char, U_char, Re, Wi, beta, eps, Ma, Mdiff)
+        for k in range(3):
+            p.grav[k] = grav[k]
+            p.driving_force[k] = drive[k]
+        p.kol_amp, p.kol_k = kol_amp, kol_k
+        self._ck(lib().sv_set_phys(self.h, C.byref(p)))
+
+    def set_initial(self, ic, dt=None, itime=0, dt_mode="pid", time=0.0,
+                    emax=(1e-5, 1e-5, 1e-5)):
+        arrs = [_f64(ic[k]) for k in ("ro", "u", "v", "cxx", "cxy", "cyy")]
+        self._ck(lib().sv_set_initial(self.h, *[_d(a) for a in arrs]))
+        self._ck(lib().sv_set_control(self.h, 1.0e-6 if dt is None else dt, time, itime,
+                                      emax[0], emax[1], emax[2],
+                                      DT_PID if dt_mode == "pid" else DT_FIXED))
+
+    def run(self, nsteps: int, graph: bool = False):
+        self._ck(lib().sv_run(self.h, nsteps, 1 if graph else 0))
+
+    def run_profiled(self, nsteps: int):
+        """Eager steps with HIP events between launches: (ms per kernel, total ms)."""
+        k = np.zeros(len(KERNELS))
+        tot = C.c_double()
+        self._ck(lib().sv_run_profiled(self.h, nsteps, _d(k), C.byref(tot)))
+        return dict(zip(KERNELS, k)), tot.value
+
+    def stencil_stats(self):
+        st = np.zeros(5)
+        self._ck(lib().sv_get_stencil_stats(self.h, _d(st)))
+        return dict(K_mean=st[0], Ks_mean=st[1], K_max=int(st[2]), sell_entries=int(st[3]),
+                    ghosts=int(st[4]))
+
+    def rhs(self, iRKstep=1):
+        outs = [np.zeros(self.npfb) for _ in range(6)]
+        self._ck(lib().sv_calc_all_rhs(self.h, iRKstep, *[_d(a) for a in outs]))
+        return dict(zip(("ro", "rou", "rov", "xx", "xy", "yy"), outs))
+
+    def fields(self):
+        n = self.npfb
+        keys = ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy")
+        out = {k: np.zeros(n) for k in keys}
+        self._ck(lib().sv_get_state(self.h, *[_d(out[k]) for k in keys]))
+        dt, tm, dc, dp = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        it = C.c_int32()
+        em = np.zeros(3)
+        self._ck(lib().sv_get_control(self.h, C.byref(dt), C.byref(tm), C.byref(it), _d(em),
+                                      C.byref(dc), C.byref(dp)))
+        out.update(dt=dt.value, time=tm.value, itime=it.value, emax_np1=em[0], emax_n=em[1],
+                   emax_nm1=em[2], dt_cfl=dc.value, dt_par=dp.value)
+        return out

@@ -24,12 +24,16 @@ def available(order: int) -> bool:
 
 def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e-6,
                   time0=0.0, emax=(1e-5, 1e-5, 1e-5), itime0=0, nplink=0, threads=4,
-                  timeout=600):
+                  timeout=600, stencil=None, fields_only=False):
+    """stencil (optional): dict(nplink, ij_count, ij_count_small, ij_link (0-based,
+    [npfb, nplink]), w_grad [npfb, nplink, 2], w_lap, w_hyp, gsum [npfb, 2], lsum,
+    hsum, vol) loaded instead of the reference's own neighbour/weight setup."""
     phys = dict(L_char=1.0, U_char=1.0, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
                 Mdiff=0.0, grav=(0, 0, 0), drive=(0, 0, 0)) | (phys or {})
     n = ns.npfb
-    hdr = np.array([n, nsteps, 0 if dt_mode == "pid" else 1, nplink, 0, *ns.bc, itime0, 0, 0],
-                   dtype=np.int32)
+    flags = (2 if stencil is not None else 0) | (4 if fields_only else 0)
+    hdr = np.array([n, nsteps, 0 if dt_mode == "pid" else 1, nplink, flags, *ns.bc, itime0, 0,
+                    0], dtype=np.int32)
     rh = np.array([ns.xmin, ns.xmax, ns.ymin, ns.ymax, ns.smax, phys["L_char"], phys["U_char"],
                    phys["Re"], phys["Wi"], phys["beta"], phys["eps"], phys["Ma"], phys["Mdiff"],
                    *phys["grav"], *phys["drive"], dt0, time0, *emax], dtype=np.float64)
@@ -44,21 +48,35 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
             f.write(np.ascontiguousarray(ns.node_type, dtype=np.int32).tobytes())
             for k in ("ro", "u", "v", "cxx", "cxy", "cyy"):
                 f.write(np.ascontiguousarray(ic[k], dtype=np.float64).tobytes())
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        res = subprocess.run([ref_binary(order), fin, fout], cwd=td, env=env,
-                             capture_output=True, text=True, timeout=timeout)
+        args = [ref_binary(order), fin, fout]
+        if stencil is not None:
+            fst = os.path.join(td, "stencil.bin")
+            with open(fst, "wb") as f:
+                f.write(np.int32(stencil["nplink"]).tobytes())
+                f.write(np.ascontiguousarray(stencil["ij_count"], dtype=np.int32).tobytes())
+                f.write(np.ascontiguousarray(stencil["ij_count_small"], dtype=np.int32).tobytes())
+                f.write(np.ascontiguousarray(stencil["ij_link"] + 1, dtype=np.int32).tobytes())
+                for k in ("w_grad", "w_lap", "w_hyp", "gsum", "lsum", "hsum", "vol"):
+                    f.write(np.ascontiguousarray(stencil[k], dtype=np.float64).tobytes())
+            args.append(fst)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        res = subprocess.run(args, cwd=td, env=env, capture_output=True, text=True,
+                             timeout=timeout)
         if res.returncode != 0:
             raise RuntimeError(f"reference failed rc={res.returncode}\n{res.stdout}\n{res.stderr}")
         raw = open(fout, "rb").read()
-    times = {}
+    times = {"steps": []}
     for line in res.stdout.splitlines():
         if "REFTIME" in line:
             parts = line.split()
             times[parts[1]] = float(parts[-1])
-    return _parse(raw, n, nsteps), times
+        if "REFSTEP" in line:
+            times["steps"].append(float(line.split()[-1]))
+    return _parse(raw, n, nsteps, fields_only), times
 
 
-def _parse(raw, n, nsteps):
+def _parse(raw, n, nsteps, fields_only=False):
     off = 0
 
     def take(cnt, dt):
@@ -67,8 +85,11 @@ def _parse(raw, n, nsteps):
         off += a.nbytes
         return a.copy()
 
+    out = {}
+    if fields_only:
+        return _parse_fields(out, take, n)
     np_, nplink = take(2, np.int32)
-    out = dict(np=int(np_), nplink=int(nplink))
+    out.update(np=int(np_), nplink=int(nplink))
     out["x"] = take(np_, np.float64)
     out["y"] = take(np_, np.float64)
     ng = np_ - n
@@ -88,6 +109,10 @@ def _parse(raw, n, nsteps):
     out["grad_ro_y"] = take(n, np.float64)
     out["lap_ro"] = take(n, np.float64)
     out["filt_ro"] = take(n, np.float64)
+    return _parse_fields(out, take, n)
+
+
+def _parse_fields(out, take, n):
     for k in ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy"):
         out[k] = take(n, np.float64)
     ctl = take(7, np.float64)
