@@ -265,14 +265,15 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
     }
     __syncthreads();
   }
-  // D layout of a 16x16 f64 tile: lane l holds rows 4*(l>>4)+v of column l&15
+  // D layout of a 16x16 f64 tile (v_mfma_f64_16x16x4_f64): lane l holds rows
+  // (l>>4) + 4v, v = 0..3, of column l&15
 #pragma unroll
   for (int ti = 0; ti < T; ti++)
 #pragma unroll
     for (int tj = 0; tj < T; tj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = ti * 16 + 4 * (l >> 4) + v, c = tj * 16 + (l & 15);
+        const int r = ti * 16 + (l >> 4) + 4 * v, c = tj * 16 + (l & 15);
         A[r * LDA + c] = acc[ti][tj][v];
       }
   __syncthreads();

@@ -1,0 +1,173 @@
+!! sv_capi.F90 -- iso_c_binding interfaces to libsunset_amd.so (the C ABI
+!! declared in include/sunset_amd.h), and the device context shared by the
+!! drop-in modules neighbours / labf / derivatives of this directory.
+!!
+!! The context is created on first use from the reference's own globals
+!! (common_vars.F90): the IPART header (xmin..ymax, x/ybcond_*, smax, L_char)
+!! and the real nodes 1..npfb (rp, s, h, h_small, node_type).  Orders come from
+!! the same -Dmorder/-Dforder switches the reference is built with, the
+!! conformation form from -Ddi/-Dlc.  Every nonzero status stops the program
+!! with the library's message, as the reference stops on errors.
+module sv_capi
+  use iso_c_binding
+  use kind_parameters
+  use common_parameter
+  use common_vars
+  implicit none
+
+  type, bind(C) :: sv_config_t
+     integer(c_int32_t) :: deriv_order, filt_order, conf_form, gpu
+  end type sv_config_t
+
+  type, bind(C) :: sv_phys_t
+     real(c_double) :: L_char, U_char, Re, Wi, beta, epsPTT, Ma, Mdiff
+     real(c_double) :: grav(3), driving_force(3), kol_amp, kol_k
+  end type sv_phys_t
+
+  integer(c_int32_t), parameter :: SV_OP_GRAD = 0, SV_OP_LAP = 1, SV_OP_GRAD_X = 2, &
+       SV_OP_GRAD_Y = 3, SV_OP_FILTER = 4, SV_OP_FILTER_TERM = 5
+
+  interface
+     integer(c_int) function sv_create(cfg, out) bind(C, name='sv_create')
+       import :: c_int, c_ptr, sv_config_t
+       type(sv_config_t), intent(in) :: cfg
+       type(c_ptr), intent(out) :: out
+     end function sv_create
+     subroutine sv_destroy(ctx) bind(C, name='sv_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine sv_destroy
+     type(c_ptr) function sv_last_error(ctx) bind(C, name='sv_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function sv_last_error
+     integer(c_int) function sv_set_domain(ctx, xmin, xmax, ymin, ymax, bc, smax, lchar) &
+          bind(C, name='sv_set_domain')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: xmin, xmax, ymin, ymax
+       integer(c_int32_t), intent(in) :: bc(4)
+       real(c_double), value :: smax, lchar
+     end function sv_set_domain
+     integer(c_int) function sv_set_nodes(ctx, n, x, y, s, h, hs, nt) bind(C, name='sv_set_nodes')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       real(c_double), intent(in) :: x(*), y(*), s(*), h(*), hs(*)
+       integer(c_int32_t), intent(in) :: nt(*)
+     end function sv_set_nodes
+     integer(c_int) function sv_find_neighbours(ctx, nplink, cnt, cnts, link) &
+          bind(C, name='sv_find_neighbours')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nplink
+       integer(c_int32_t), intent(out) :: cnt(*), cnts(*), link(*)
+     end function sv_find_neighbours
+     integer(c_int) function sv_calc_labf_weights(ctx, order_map) &
+          bind(C, name='sv_calc_labf_weights')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, order_map
+     end function sv_calc_labf_weights
+     integer(c_int) function sv_calc_labf_sums(ctx) bind(C, name='sv_calc_labf_sums')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function sv_calc_labf_sums
+     integer(c_int) function sv_filter_coefficients(ctx) bind(C, name='sv_filter_coefficients')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function sv_filter_coefficients
+     integer(c_int) function sv_get_weights(ctx, nplink, wg, wl, wh, gs, ls, hs, vol) &
+          bind(C, name='sv_get_weights')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nplink
+       type(c_ptr), value :: wg, wl, wh, gs, ls, hs, vol
+     end function sv_get_weights
+     integer(c_int) function sv_get_ghosts(ctx, np, x, y, irel, vrel, cap) &
+          bind(C, name='sv_get_ghosts')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: np
+       type(c_ptr), value :: x, y, irel, vrel
+       integer(c_int32_t), value :: cap
+     end function sv_get_ghosts
+     integer(c_int) function sv_apply(ctx, op, phi, out) bind(C, name='sv_apply')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: op
+       real(c_double), intent(inout) :: phi(*)
+       real(c_double), intent(out) :: out(*)
+     end function sv_apply
+  end interface
+
+  type(c_ptr), save :: sv_ctx = c_null_ptr
+  integer(ikind), save :: sv_ctx_npfb = -1
+
+contains
+
+  subroutine sv_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: k
+    if (rc == 0) return
+    text = ''
+    call c_f_pointer(sv_last_error(sv_ctx), msg, [512])
+    do k = 1, 512
+       if (msg(k) == c_null_char) exit
+       text(k:k) = msg(k)
+    end do
+    write(6,*) "sunset_amd: ", where, " failed (", rc, "): ", trim(text)
+    stop
+  end subroutine sv_check
+
+  !! Create the context if needed and upload the current real nodes.
+  subroutine sv_upload_nodes()
+    type(sv_config_t) :: cfg
+    integer(c_int32_t) :: bc(4)
+    integer(c_int) :: rc
+    if (.not. c_associated(sv_ctx)) then
+#if morder==4
+       cfg%deriv_order = 4
+#elif morder==6
+       cfg%deriv_order = 6
+#elif morder==10
+       cfg%deriv_order = 10
+#else
+       cfg%deriv_order = 8
+#endif
+#if forder==4
+       cfg%filt_order = 4
+#elif forder==6
+       cfg%filt_order = 6
+#elif forder==10
+       cfg%filt_order = 10
+#else
+       cfg%filt_order = 8
+#endif
+#ifdef di
+       cfg%conf_form = 0
+#else
+       cfg%conf_form = 1
+#endif
+       cfg%gpu = 0
+       rc = sv_create(cfg, sv_ctx)
+       if (rc /= 0) then
+          write(6,*) "sunset_amd: sv_create failed (", rc, "): no HIP device?"
+          stop
+       end if
+    end if
+    if (nb /= 0) then
+       write(6,*) "sunset_amd: boundary nodes (nb>0) are not supported by the device library"
+       stop
+    end if
+    bc = (/ xbcond_L, xbcond_U, ybcond_L, ybcond_U /)
+    call sv_check(sv_set_domain(sv_ctx, xmin, xmax, ymin, ymax, bc, smax, L_char), &
+                  "sv_set_domain")
+    call sv_check(sv_set_nodes(sv_ctx, npfb, rp(1:npfb,1), rp(1:npfb,2), s(1:npfb), &
+                  h(1:npfb), h_small(1:npfb), node_type(1:npfb)), "sv_set_nodes")
+    sv_ctx_npfb = npfb
+  end subroutine sv_upload_nodes
+
+end module sv_capi

@@ -22,9 +22,15 @@ def available(order: int) -> bool:
     return os.path.exists(ref_binary(order))
 
 
+def dropin_binary(order: int) -> str:
+    """The reference's time loop linked with this library's Fortran drop-in
+    neighbours / labf / derivatives modules (oracle/Makefile target dropin)."""
+    return os.path.join(REF_DIR, f"sunset_dropin_lc_m{order}")
+
+
 def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e-6,
                   time0=0.0, emax=(1e-5, 1e-5, 1e-5), itime0=0, nplink=0, threads=4,
-                  timeout=600, stencil=None, fields_only=False):
+                  timeout=600, stencil=None, fields_only=False, binary=None):
     """stencil (optional): dict(nplink, ij_count, ij_count_small, ij_link (0-based,
     [npfb, nplink]), w_grad [npfb, nplink, 2], w_lap, w_hyp, gsum [npfb, 2], lsum,
     hsum, vol) loaded instead of the reference's own neighbour/weight setup."""
@@ -48,7 +54,7 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
             f.write(np.ascontiguousarray(ns.node_type, dtype=np.int32).tobytes())
             for k in ("ro", "u", "v", "cxx", "cxy", "cyy"):
                 f.write(np.ascontiguousarray(ic[k], dtype=np.float64).tobytes())
-        args = [ref_binary(order), fin, fout]
+        args = [binary or ref_binary(order), fin, fout]
         if stencil is not None:
             fst = os.path.join(td, "stencil.bin")
             with open(fst, "wb") as f:
