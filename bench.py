@@ -78,6 +78,55 @@ def cpu_baseline(n_sample, steps, threads):
                        f"{med * 1e3:.1f} ms, OMP_NUM_THREADS={threads}")
 
 
+def c3_weights(smin, smax):
+    """Weight setup on the cylinder-in-channel node set: spacing smin at the
+    cylinder growing linearly to smax over 1.5, +/-0.3 s jitter, LABFM order 8
+    inside and 4 within 3 s of any wall.  Times the one-block-per-system launch
+    and the persistent work-queue launch (HIP events around the solves)."""
+    import numpy as np
+    import sunset_amd
+    from sunset_amd.nodes import cylinder_channel
+    ns, om = cylinder_channel(smin=smin, smax=smax)
+    s = sunset_amd.Solver(morder=8, form="lc")
+    s.set_nodes(ns)
+    s.find_neighbours(copy=False)
+    t_n = s.timing("neighbours")
+    out = dict(nodes=ns.npfb, order8=int((om == 8).sum()), order4=int((om == 4).sum()),
+               spacing=[smin, smax], neighbour_ms=round(t_n, 2))
+    for name, pers in (("plain", False), ("persistent", True)):
+        s.calc_weights(order_map=om, persistent=pers)
+        t = s.timing("weights")
+        out[f"{name}_ms"] = round(t, 2)
+        out["systems"] = int(s.timing("systems"))
+    s.calc_sums()
+    s.filter_coefficients()
+    st = s.stencil_stats()
+    s.close()
+    best = min(out["plain_ms"], out["persistent_ms"])
+    out.update(K_mean=round(st["K_mean"], 2), K_max=int(st["K_max"]),
+               weight_solves_per_s=round(ns.npfb / (best / 1e3), 1))
+    return out
+
+
+def cpu_weights_baseline(n_sample):
+    """The reference's calc_labf_weights (serial loop, labf.F90:82; derivative
+    and filter systems, Golub-Reinsch SVDs) on an n_sample^2 order-8 lattice,
+    h = 2.7 s; returns node weight sets per second."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refdrv
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+    if not refdrv.available(8):
+        return None
+    ns = jittered_lattice(n_sample, seed=2, h_over_s=2.7)
+    _, times = refdrv.run_reference(ns, 8, taylor_green_ic(ns), nsteps=0, fields_only=True,
+                                    threads=1, timeout=900)
+    t = times["calc_labf_weights"]
+    return dict(value=round(ns.npfb / t, 1), unit="node weight sets/s", cores=1,
+                kind="reference",
+                sample=f"{n_sample}x{n_sample} lattice ({ns.npfb} nodes), order 8, h=2.7s, "
+                       f"calc_labf_weights {t:.2f} s (serial loop)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +137,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--graph", action="store_true", help="time hipGraph replay instead")
+    ap.add_argument("--c3", choices=("full", "spec", "none"), default="full",
+                    help="cylinder-channel weight setup: full = spacing 0.0013..0.0052 "
+                         "(~1.2M nodes), spec = 0.004..0.016 (~126k nodes)")
+    ap.add_argument("--cpu-weights-n", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -150,6 +203,16 @@ def main():
               file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
 
+    c3 = None
+    if args.c3 != "none":
+        sp = (0.0013, 0.0052) if args.c3 == "full" else (0.004, 0.016)
+        print(f"[bench] cylinder-channel weights, spacing {sp}", file=sys.stderr, flush=True)
+        c3 = c3_weights(*sp)
+        if not args.no_cpu:
+            print(f"[bench] cpu weights baseline: reference on {args.cpu_weights_n}^2 order 8",
+                  file=sys.stderr, flush=True)
+            c3["cpu_baseline"] = cpu_weights_baseline(args.cpu_weights_n)
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "node-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -165,6 +228,7 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "step_hbm_gbs": round(step_gbs, 1),
         "weight_solves_per_s": round(n / (t_w / 1e3), 1),
+        "cylinder_channel_weights": c3,
         "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
         "adjoint_forward_ratio": None,

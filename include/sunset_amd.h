@@ -164,6 +164,29 @@ int sv_get_state(sv_ctx *ctx, double *ro, double *rou, double *rov, double *u, d
 /* Synchronise the context's stream */
 int sv_sync(sv_ctx *ctx);
 
+/* ---- Ensembles: many independent simulations advanced together ----------
+ * No single counterpart in the reference, which runs one simulation per
+ * process (sunset.F90:36-106); an ensemble is N such runs.  Each member is a
+ * context taken through the whole setup and sv_set_initial (its own nodes,
+ * orders, stencils, physics and control state); all members must have the
+ * same conformation form and device.  The ensemble context holds copies of the
+ * members' device data in one concatenated layout, so every launch of a step,
+ * and one captured hipGraph (sv_run with use_graph), advances all members;
+ * each keeps its own dt / PID history / itime (set_tstep and set_tstep_PID per
+ * member, step.F90:569-682) and its own mass correction (rhs_lc.F90:871-890).
+ * On an ensemble context: sv_run, sv_run_profiled, sv_sync, sv_destroy and the
+ * two accessors below; the setup entry points return SV_E_ARG. */
+int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **ensemble);
+int32_t sv_ensemble_size(const sv_ctx *ensemble);
+/* sv_get_state / sv_get_control of one member (sv_get_state on an ensemble
+ * context is member 0) */
+int sv_ensemble_get_state(sv_ctx *ensemble, int32_t member, double *ro, double *rou,
+                          double *rov, double *u, double *v, double *psixx, double *psixy,
+                          double *psiyy, double *cxx, double *cxy, double *cyy);
+int sv_ensemble_get_control(sv_ctx *ensemble, int32_t member, double *dt, double *time,
+                            int32_t *itime, double *emax3, double *dt_cfl,
+                            double *dt_parabolic);
+
 #ifdef __cplusplus
 }
 #endif

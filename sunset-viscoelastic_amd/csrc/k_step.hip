@@ -15,16 +15,38 @@
 //                            the conformation tensor of the filtered psi
 //   k_reduce      (1 block)  mass correction (rhs_lc.F90:871-890), emax,
 //                            itime += 1
-//   k_apply                  rho -= drho, u = rou/rho, dt partial minima
+//   k_apply_state            rho -= drho, u = rou/rho, dt partial minima
 // Ghost nodes do not exist at run time: periodic links point at the parent
 // (mirror copies are exact copies, mirror_boundaries.F90:263-352).
 // Reductions are a fixed-order two-level tree, so results are bitwise
 // reproducible run to run.
+//
+// Memory layout of the state (records, see sv_internal.h): what a neighbour
+// gather needs of node j sits in one 128-byte record (LC) or 64-byte record
+// (DI), so a link costs one cache line instead of one line per field.
 #include "sv_internal.h"
 
 namespace {
 
-constexpr int BS = 256;   // threads per block of the node kernels
+constexpr int BS = SV_BLOCK;   // threads per block of the node kernels
+
+// The node handled by this thread: segment of the block, concatenated index i,
+// a safe index ii (= i, or the segment's first node for an idle lane).
+struct NodeRef {
+  int sg, i, ii;
+  bool act;
+};
+
+__device__ __forceinline__ NodeRef node_ref(const int *blk_seg, const sv_segk *seg, int bid) {
+  NodeRef r;
+  r.sg = blk_seg[bid];
+  const sv_segk &S = seg[r.sg];
+  const int li = (bid - S.blk0) * BS + (int)threadIdx.x;
+  r.act = li < S.npfb;
+  r.i = S.base + li;
+  r.ii = r.act ? r.i : S.base;
+  return r;
+}
 
 // ---- conformation algebra (conf_transforms.F90) -------------------------
 // 2D eigens (:399-444); R column-major R[0]=R11 R[1]=R21 R[2]=R12 R[3]=R22
@@ -115,86 +137,154 @@ __device__ __forceinline__ void rhs_logconf(double u, double v, double ux, doubl
   ryy = adyy + UC22 + (R[1] * i12 + R[3] * i22) / lam;
 }
 
+// ---- record access ----------------------------------------------------
+struct Rec9 {   // gathered fields of one node
+  double ro, u, v, cxx, cxy, cyy, pxx, pxy, pyy;
+};
+
+template <int FORM>
+__device__ __forceinline__ Rec9 load_g(const double *G, int j) {
+  constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
+  const double2 *p = reinterpret_cast<const double2 *>(G + (size_t)j * S);
+  Rec9 r;
+  const double2 a = p[0], b = p[1], c = p[2];
+  r.ro = a.x; r.u = a.y; r.v = b.x; r.cxx = b.y; r.cxy = c.x; r.cyy = c.y;
+  if (FORM == SV_FORM_LC) {
+    const double2 d = p[3];
+    r.pxx = d.x; r.pxy = d.y;
+    r.pyy = G[(size_t)j * S + SV_G_PYY];
+  } else {
+    r.pxx = r.pxy = r.pyy = 0.0;
+  }
+  return r;
+}
+
+template <int FORM>
+__device__ __forceinline__ void store_g(double *G, int i, const Rec9 &r) {
+  constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
+  double2 *p = reinterpret_cast<double2 *>(G + (size_t)i * S);
+  p[0] = double2{r.ro, r.u};
+  p[1] = double2{r.v, r.cxx};
+  p[2] = double2{r.cxy, r.cyy};
+  if (FORM == SV_FORM_LC) {
+    p[3] = double2{r.pxx, r.pxy};
+    G[(size_t)i * S + SV_G_PYY] = r.pyy;
+  }
+}
+
+__device__ __forceinline__ void load_u(const double *U, int i, double f[6]) {
+  const double2 *p = reinterpret_cast<const double2 *>(U + (size_t)i * SV_US);
+  const double2 a = p[0], b = p[1], c = p[2];
+  f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = c.x; f[5] = c.y;
+}
+
+__device__ __forceinline__ void store_u(double *U, int i, const double f[6]) {
+  double2 *p = reinterpret_cast<double2 *>(U + (size_t)i * SV_US);
+  p[0] = double2{f[0], f[1]};
+  p[1] = double2{f[2], f[3]};
+  p[2] = double2{f[4], f[5]};
+}
+
 struct StageArgs {
-  int npfb;
+  int rhs_stride;       // STAGE 0: length of each rhs_out array
+  const int *blk_seg;
+  const sv_segk *seg;
   const int64_t *off;
   const int *kcnt;
   const int *col;
   const double *wx, *wy, *wl;
   const double *gsx, *gsy, *lsum;
-  sv_gather G;        // gathered (stage input)
-  sv_fields reg_in;   // register 1 input (stage 1: the state)
-  sv_fields reg_out;  // register 1 output
-  sv_fields e;        // error accumulators
-  sv_fields Uout;     // stage 4 result
-  sv_gather Gout;     // stages 1-3: next gather buffer
+  const double *G;      // gathered records (stage input)
+  const double *reg_in; // register 1 input (stage 1: the state)
+  double *reg_out;      // register 1 output
+  double *e;            // error accumulators
+  double *Uout;         // stage 4 result
+  double *Gout;         // stages 1-3: next gather records
+  double *rhs_out;      // STAGE 0: six arrays of npfb
   const double *fx;
-  sv_ctrl *ctrl;
-  sv_physk pk;
-  double *partial;    // stage 4: per-block max error
-  int rk1;            // STAGE 0 (right-hand side only): reference stage counter is 1
+  const sv_ctrl *ctrl;  // [nseg]
+  double *partial;      // stage 4: per-block max error
+  int rk1;              // STAGE 0: reference stage counter is 1
 };
 
-// STAGE 1..4: RK stages; STAGE 0: write the right-hand side into reg_out.
+// STAGE 1..4: RK stages; STAGE 0: write the right-hand side into rhs_out.
 template <int FORM, int STAGE, bool MDIFF>
 __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   __shared__ double red[BS / SV_WAVE];
   const int bid = sv_xcd_block();
-  const int i = bid * BS + threadIdx.x;
-  const bool act = i < a.npfb;
-  const int ii = act ? i : 0;
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int i = nr.i, ii = nr.ii;
+  const bool act = nr.act;
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
-  // accumulators
   double rx = 0, ry = 0, ux = 0, uy = 0, vx = 0, vy = 0, lu = 0, lv = 0;
-  double axx = 0, ayx = 0, axy = 0, ayy = 0, bxx = 0, byx = 0, bxy = 0, byy = 0;
+  double axx = 0, ayx = 0, axy = 0, ayy = 0, bxy = 0, byy = 0;
   double cxx_l = 0, cxy_l = 0, cyy_l = 0;
   double q1x = 0, q1y = 0, q2x = 0, q2y = 0, q3x = 0, q3y = 0;
-  for (int k = 0; k < K; k++) {
-    const int64_t e = b + (int64_t)k * SV_WAVE;
-    const int j = a.col[e];
-    const double wxk = a.wx[e], wyk = a.wy[e], wlk = a.wl[e];
-    const double roj = a.G.ro[j], uj = a.G.u[j], vj = a.G.v[j];
-    rx += roj * wxk; ry += roj * wyk;
-    ux += uj * wxk; uy += uj * wyk; lu += uj * wlk;
-    vx += vj * wxk; vy += vj * wyk; lv += vj * wlk;
-    const double cxxj = a.G.cxx[j], cxyj = a.G.cxy[j], cyyj = a.G.cyy[j];
+  auto accum = [&](const Rec9 &r, double wxk, double wyk, double wlk) {
+    rx += r.ro * wxk; ry += r.ro * wyk;
+    ux += r.u * wxk; uy += r.u * wyk; lu += r.u * wlk;
+    vx += r.v * wxk; vy += r.v * wyk; lv += r.v * wlk;
     if (FORM == SV_FORM_LC) {
-      axx += cxxj * wxk;                      // calc_gradient_only(1,cxx)
-      axy += cxyj * wxk; ayy += cxyj * wyk;   // calc_gradient(cxy)
-      byy += cyyj * wyk;                      // calc_gradient_only(2,cyy)
-      const double p1 = a.G.pxx[j], p2 = a.G.pxy[j], p3 = a.G.pyy[j];
-      q1x += p1 * wxk; q1y += p1 * wyk;
-      q2x += p2 * wxk; q2y += p2 * wyk;
-      q3x += p3 * wxk; q3y += p3 * wyk;
+      axx += r.cxx * wxk;                         // calc_gradient_only(1,cxx)
+      axy += r.cxy * wxk; ayy += r.cxy * wyk;     // calc_gradient(cxy)
+      byy += r.cyy * wyk;                         // calc_gradient_only(2,cyy)
+      q1x += r.pxx * wxk; q1y += r.pxx * wyk;
+      q2x += r.pxy * wxk; q2y += r.pxy * wyk;
+      q3x += r.pyy * wxk; q3y += r.pyy * wyk;
     } else {
-      axx += cxxj * wxk; ayx += cxxj * wyk;
-      axy += cxyj * wxk; ayy += cxyj * wyk;
-      bxy += cyyj * wxk; byy += cyyj * wyk;
-      if (MDIFF) { cxx_l += cxxj * wlk; cxy_l += cxyj * wlk; cyy_l += cyyj * wlk; }
+      axx += r.cxx * wxk; ayx += r.cxx * wyk;
+      axy += r.cxy * wxk; ayy += r.cxy * wyk;
+      bxy += r.cyy * wxk; byy += r.cyy * wyk;
+      if (MDIFF) { cxx_l += r.cxx * wlk; cxy_l += r.cxy * wlk; cyy_l += r.cyy * wlk; }
     }
+  };
+  // Links in pairs, the column indices of the next pair loaded before the
+  // current pair is consumed, so each lane keeps two record gathers and the
+  // next index loads in flight.  Summation stays sequential in k.
+  constexpr int U = 2;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? a.col[b + (int64_t)u * SV_WAVE] : ii;
+  for (int k = 0; k < K; k += U) {
+    Rec9 r[U];
+    double wxk[U], wyk[U], wlk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k + u < K) {
+        const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+        r[u] = load_g<FORM>(a.G, jn[u]);
+        wxk[u] = a.wx[e]; wyk[u] = a.wy[e]; wlk[u] = a.wl[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kn = k + U + u;
+      jn[u] = (kn < K) ? a.col[b + (int64_t)kn * SV_WAVE] : ii;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k + u < K) accum(r[u], wxk[u], wyk[u], wlk[u]);
   }
-  (void)bxx; (void)byx;
   double emax_loc = 0.0;
   if (act) {
     const double sx = a.gsx[i], sy = a.gsy[i], sl = a.lsum[i];
-    const double ro = a.G.ro[i], u = a.G.u[i], v = a.G.v[i];
-    const double cxx = a.G.cxx[i], cxy = a.G.cxy[i], cyy = a.G.cyy[i];
+    const Rec9 s = load_g<FORM>(a.G, i);
+    const double ro = s.ro, u = s.u, v = s.v, cxx = s.cxx, cxy = s.cxy, cyy = s.cyy;
     rx = rx - ro * sx; ry = ry - ro * sy;
     ux = ux - u * sx; uy = uy - u * sy; lu = lu - u * sl;
     vx = vx - v * sx; vy = vy - v * sy; lv = lv - v * sl;
-    const sv_physk &P = a.pk;
+    const sv_physk &P = a.seg[nr.sg].pk;
+    const sv_ctrl *C = a.ctrl + nr.sg;
     // calc_rhs_ro (rhs_lc.F90:127-141)
     const double r_ro = -ro * (ux + vy + 0.0) - (u * rx + v * ry + 0.0 * 0.0);
     // calc_rhs_rovel (rhs_lc.F90:217-272)
     const double tsu = ro * (u * ux + v * uy + 0.0 * 0.0) - u * r_ro;
     const double tsv = ro * (u * vx + v * vy + 0.0 * 0.0) - v * r_ro;
-    double dcxx_x, dcxy_x, dcxy_y, dcyy_y;
-    dcxx_x = axx - cxx * sx;
-    dcxy_x = axy - cxy * sx;
-    dcxy_y = ayy - cxy * sy;
-    if (FORM == SV_FORM_LC) dcyy_y = byy - cyy * sy;
-    else dcyy_y = byy - cyy * sy;
+    const double dcxx_x = axx - cxx * sx;
+    const double dcxy_x = axy - cxy * sx;
+    const double dcxy_y = ayy - cxy * sy;
+    const double dcyy_y = byy - cyy * sy;
     double fvu = P.coef_solvent * lu;
     double fvv = P.coef_solvent * lv;
     fvu = fvu + P.coef_polymeric * (dcxx_x + dcxy_y + 0.0);
@@ -206,11 +296,10 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
     const double r_rov = -tsv - P.csq * ry + bfv + fvv;
     double r_xx, r_xy, r_yy;
     if (FORM == SV_FORM_LC) {
-      const double pxx = a.G.pxx[i], pxy = a.G.pxy[i], pyy = a.G.pyy[i];
-      const bool first = (STAGE == 1 || (STAGE == 0 && a.rk1)) && (a.ctrl->itime == 1);
-      rhs_logconf(u, v, ux, uy, vx, vy, pxx, pxy, pyy, q1x - pxx * sx, q1y - pxx * sy,
-                  q2x - pxy * sx, q2y - pxy * sy, q3x - pyy * sx, q3y - pyy * sy, first, P.eps,
-                  P.lambda, r_xx, r_xy, r_yy);
+      const bool first = (STAGE == 1 || (STAGE == 0 && a.rk1)) && (C->itime == 1);
+      rhs_logconf(u, v, ux, uy, vx, vy, s.pxx, s.pxy, s.pyy, q1x - s.pxx * sx,
+                  q1y - s.pxx * sy, q2x - s.pxy * sx, q2y - s.pxy * sy, q3x - s.pyy * sx,
+                  q3y - s.pyy * sy, first, P.eps, P.lambda, r_xx, r_xy, r_yy);
     } else {
       // calc_rhs_c (rhs_di.F90:417-475) with full gradients of cxx, cyy
       const double gcxx_y = ayx - cxx * sy, gcyy_x = bxy - cyy * sx;
@@ -233,55 +322,49 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
     }
     // ---- RK update (step.F90:291-343, :375-409) ----
     const double rhs[6] = {r_ro, r_rou, r_rov, r_xx, r_xy, r_yy};
-    double *const rin[6] = {a.reg_in.ro, a.reg_in.rou, a.reg_in.rov, a.reg_in.pxx,
-                            a.reg_in.pxy, a.reg_in.pyy};
-    double *const eacc[6] = {a.e.ro, a.e.rou, a.e.rov, a.e.pxx, a.e.pxy, a.e.pyy};
-    const sv_ctrl *C = a.ctrl;
-    double Un[6];
     if (STAGE == 0) {
-      a.reg_out.ro[i] = r_ro; a.reg_out.rou[i] = r_rou; a.reg_out.rov[i] = r_rov;
-      a.reg_out.pxx[i] = r_xx; a.reg_out.pxy[i] = r_xy; a.reg_out.pyy[i] = r_yy;
+#pragma unroll
+      for (int f = 0; f < 6; f++) a.rhs_out[(size_t)f * a.rhs_stride + i] = rhs[f];
     } else if (STAGE <= 3) {
       const double ra = C->RKa[STAGE - 1], rb = C->RKb[STAGE - 1], rm = C->RKbmbh[STAGE - 1];
-      double *const rout[6] = {a.reg_out.ro, a.reg_out.rou, a.reg_out.rov, a.reg_out.pxx,
-                               a.reg_out.pxy, a.reg_out.pyy};
+      double r1[6], eo[6], Un[6], ro_[6];
+      load_u(a.reg_in, i, r1);
+      if (STAGE > 1) load_u(a.e, i, eo);
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        const double r1 = rin[f][i];
-        Un[f] = r1 + ra * rhs[f];
-        rout[f][i] = r1 + rb * rhs[f];
-        eacc[f][i] = (STAGE == 1 ? 0.0 : eacc[f][i]) + rm * rhs[f];
+        Un[f] = r1[f] + ra * rhs[f];
+        ro_[f] = r1[f] + rb * rhs[f];
+        eo[f] = (STAGE == 1 ? 0.0 : eo[f]) + rm * rhs[f];
       }
+      store_u(a.reg_out, i, ro_);
+      store_u(a.e, i, eo);
       // derived fields for the next stage (step.F90:350-356)
+      Rec9 g;
       const double ooro = 1.0 / Un[0];
-      a.Gout.ro[i] = Un[0];
-      a.Gout.u[i] = Un[1] * ooro;
-      a.Gout.v[i] = Un[2] * ooro;
+      g.ro = Un[0];
+      g.u = Un[1] * ooro;
+      g.v = Un[2] * ooro;
       if (FORM == SV_FORM_LC) {
-        a.Gout.pxx[i] = Un[3];
-        a.Gout.pxy[i] = Un[4];
-        a.Gout.pyy[i] = Un[5];
-        double c1, c2, c3;
-        c_from_psi(Un[3], Un[4], Un[5], c1, c2, c3);
-        a.Gout.cxx[i] = c1;
-        a.Gout.cxy[i] = c2;
-        a.Gout.cyy[i] = c3;
+        g.pxx = Un[3]; g.pxy = Un[4]; g.pyy = Un[5];
+        c_from_psi(Un[3], Un[4], Un[5], g.cxx, g.cxy, g.cyy);
       } else {
-        a.Gout.cxx[i] = Un[3];
-        a.Gout.cxy[i] = Un[4];
-        a.Gout.cyy[i] = Un[5];
+        g.cxx = Un[3]; g.cxy = Un[4]; g.cyy = Un[5];
+        g.pxx = g.pxy = g.pyy = 0.0;
       }
+      store_g<FORM>(a.Gout, i, g);
     } else {
       const double rb = C->RKb[3], rm = C->RKbmbh[3];
-      double *const uo[6] = {a.Uout.ro, a.Uout.rou, a.Uout.rov, a.Uout.pxx, a.Uout.pxy,
-                             a.Uout.pyy};
+      double r1[6], eo[6], uo[6];
+      load_u(a.reg_in, i, r1);
+      load_u(a.e, i, eo);
       const double nrm[6] = {1.0, P.err_norm_rou, P.err_norm_rou, 1.0, 1.0, 1.0};
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        uo[f][i] = rin[f][i] + rb * rhs[f];
-        const double ef = eacc[f][i] + rm * rhs[f];
+        uo[f] = r1[f] + rb * rhs[f];
+        const double ef = eo[f] + rm * rhs[f];
         emax_loc = fmax(emax_loc, fabs(ef) * nrm[f]);
       }
+      store_u(a.Uout, i, uo);
     }
   }
   if (STAGE == 4) {
@@ -298,15 +381,15 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
 }
 
 struct FilterArgs {
-  int npfb;
+  const int *blk_seg;
+  const sv_segk *seg;
   const int64_t *off;
   const int *kcnt;
   const int *col;
   const double *wh, *hsum, *vol;
-  sv_fields U;       // gathered
-  sv_fields Uf;      // filtered output
-  sv_gather Gout;    // psi (LC) and c of the filtered state
-  double rho_char;
+  const double *U;   // gathered state records
+  double *Uf;        // filtered state records
+  double *Gout;      // psi (LC) and c of the filtered state
   double *partial;   // per-block sum of (ro_f - rho_char)*vol
 };
 
@@ -314,41 +397,55 @@ template <int FORM>
 __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
   __shared__ double red[BS];
   const int bid = sv_xcd_block();
-  const int i = bid * BS + threadIdx.x;
-  const bool act = i < a.npfb;
-  const int ii = act ? i : 0;
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int i = nr.i, ii = nr.ii;
+  const bool act = nr.act;
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
-  double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
-  for (int k = 0; k < K; k++) {
-    const int64_t e = b + (int64_t)k * SV_WAVE;
-    const int j = a.col[e];
-    const double w = a.wh[e];
-    t0 += a.U.ro[j] * w;
-    t1 += a.U.rou[j] * w;
-    t2 += a.U.rov[j] * w;
-    t3 += a.U.pxx[j] * w;
-    t4 += a.U.pxy[j] * w;
-    t5 += a.U.pyy[j] * w;
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  // pairs of links with the next pair's column indices prefetched, as k_stage
+  constexpr int U = 2;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? a.col[b + (int64_t)u * SV_WAVE] : ii;
+  for (int k = 0; k < K; k += U) {
+    double f[U][6], w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k + u < K) {
+        load_u(a.U, jn[u], f[u]);
+        w[u] = a.wh[b + (int64_t)(k + u) * SV_WAVE];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kn = k + U + u;
+      jn[u] = (kn < K) ? a.col[b + (int64_t)kn * SV_WAVE] : ii;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k + u < K) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) t[q] += f[u][q] * w[u];
+      }
   }
   double m = 0.0;
   if (act) {
     const double hs = a.hsum[i];
-    const double f0 = a.U.ro[i], f1 = a.U.rou[i], f2 = a.U.rov[i];
-    const double f3 = a.U.pxx[i], f4 = a.U.pxy[i], f5 = a.U.pyy[i];
-    const double g0 = f0 + (t0 - f0 * hs), g1 = f1 + (t1 - f1 * hs), g2 = f2 + (t2 - f2 * hs);
-    const double g3 = f3 + (t3 - f3 * hs), g4 = f4 + (t4 - f4 * hs), g5 = f5 + (t5 - f5 * hs);
-    a.Uf.ro[i] = g0; a.Uf.rou[i] = g1; a.Uf.rov[i] = g2;
-    a.Uf.pxx[i] = g3; a.Uf.pxy[i] = g4; a.Uf.pyy[i] = g5;
+    double f[6], g[6];
+    load_u(a.U, i, f);
+#pragma unroll
+    for (int q = 0; q < 6; q++) g[q] = f[q] + (t[q] - f[q] * hs);
+    store_u(a.Uf, i, g);
+    constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
+    double *G = a.Gout + (size_t)i * S;
+    double c1 = g[3], c2 = g[4], c3 = g[5];
     if (FORM == SV_FORM_LC) {
-      a.Gout.pxx[i] = g3; a.Gout.pxy[i] = g4; a.Gout.pyy[i] = g5;
-      double c1, c2, c3;
-      c_from_psi(g3, g4, g5, c1, c2, c3);
-      a.Gout.cxx[i] = c1; a.Gout.cxy[i] = c2; a.Gout.cyy[i] = c3;
-    } else {
-      a.Gout.cxx[i] = g3; a.Gout.cxy[i] = g4; a.Gout.cyy[i] = g5;
+      G[SV_G_PXX] = g[3]; G[SV_G_PXY] = g[4]; G[SV_G_PYY] = g[5];
+      c_from_psi(g[3], g[4], g[5], c1, c2, c3);
     }
-    m = (g0 - a.rho_char) * a.vol[i];
+    G[SV_G_CXX] = c1; G[SV_G_CXY] = c2; G[SV_G_CYY] = c3;
+    m = (g[0] - a.seg[nr.sg].pk.rho_char) * a.vol[i];
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -359,14 +456,18 @@ __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
   if (threadIdx.x == 0) a.partial[bid] = red[0];
 }
 
-// mass correction value, emax and the end-of-step counters
-__global__ __launch_bounds__(1024) void k_reduce(int nb, const double *pmass, const double *perr,
-                                                 double volsum, sv_ctrl *ctrl) {
+// mass correction value, emax and the end-of-step counters; one block per
+// segment over that segment's per-block partials
+__global__ __launch_bounds__(1024) void k_reduce(const sv_segk *seg, const double *pmass,
+                                                 const double *perr, sv_ctrl *ctrl_all) {
   __shared__ double s1[1024], s2[1024];
+  const sv_segk &S = seg[blockIdx.x];
+  sv_ctrl *ctrl = ctrl_all + blockIdx.x;
+  const double volsum = S.volsum;
   double a = 0.0, m = 0.0;
-  for (int k = threadIdx.x; k < nb; k += 1024) {
-    a += pmass[k];
-    m = fmax(m, perr[k]);
+  for (int k = threadIdx.x; k < S.nblk; k += 1024) {
+    a += pmass[S.blk0 + k];
+    m = fmax(m, perr[S.blk0 + k]);
   }
   s1[threadIdx.x] = a;
   s2[threadIdx.x] = m;
@@ -388,12 +489,14 @@ __global__ __launch_bounds__(1024) void k_reduce(int nb, const double *pmass, co
 }
 
 struct ApplyArgs {
-  int npfb;
-  sv_fields Uf, U;
-  sv_gather G;
+  int gs;
+  const int *blk_seg;
+  const sv_segk *seg;
+  const double *Uf;
+  double *U;
+  double *G;
   const double *s;
-  const double *dro;     // &ctrl->dro, or nullptr for no correction
-  double cs, visc_total;
+  const sv_ctrl *ctrl;   // ctrl[sg].dro is subtracted; nullptr for no correction
   double *pcfl, *pvisc;  // per-block minima for set_tstep
 };
 
@@ -402,21 +505,25 @@ struct ApplyArgs {
 __global__ __launch_bounds__(BS) void k_apply_state(ApplyArgs a) {
   __shared__ double r1[BS], r2[BS];
   const int bid = sv_xcd_block();
-  const int i = bid * BS + threadIdx.x;
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int i = nr.i;
   double mc = 1.0e10, mv = 1.0e10;
-  if (i < a.npfb) {
-    const double dro = a.dro ? *a.dro : 0.0;
-    const double ro = a.Uf.ro[i] - dro;
-    const double rou = a.Uf.rou[i], rov = a.Uf.rov[i];
-    a.U.ro[i] = ro; a.U.rou[i] = rou; a.U.rov[i] = rov;
-    a.U.pxx[i] = a.Uf.pxx[i]; a.U.pxy[i] = a.Uf.pxy[i]; a.U.pyy[i] = a.Uf.pyy[i];
-    const double ooro = 1.0 / ro;
-    const double u = rou * ooro, v = rov * ooro;
-    a.G.ro[i] = ro; a.G.u[i] = u; a.G.v[i] = v;
+  if (nr.act) {
+    const sv_physk &P = a.seg[nr.sg].pk;
+    const double dro = a.ctrl ? a.ctrl[nr.sg].dro : 0.0;
+    double f[6];
+    load_u(a.Uf, i, f);
+    f[0] = f[0] - dro;
+    store_u(a.U, i, f);
+    const double ooro = 1.0 / f[0];
+    const double u = f[1] * ooro, v = f[2] * ooro;
+    double *G = a.G + (size_t)i * a.gs;
+    reinterpret_cast<double2 *>(G)[0] = double2{f[0], u};
+    G[SV_G_V] = v;
     const double um = sqrt(u * u + v * v + 0.0);
     const double si = a.s[i];
-    mc = si / (um + a.cs);
-    mv = si * si * ro / a.visc_total;
+    mc = si / (um + P.cs);
+    mv = si * si * f[0] / P.visc_total;
   }
   r1[threadIdx.x] = mc;
   r2[threadIdx.x] = mv;
@@ -433,13 +540,17 @@ __global__ __launch_bounds__(BS) void k_apply_state(ApplyArgs a) {
 
 // set_tstep (step.F90:569-622) + set_tstep_PID (:624-682) + the start of
 // step_rk3_4S_2R_EE (:239-246)
-__global__ __launch_bounds__(1024) void k_dt(int nb, const double *pcfl, const double *pvisc,
-                                             double L_char, sv_ctrl *c) {
+// one block per segment
+__global__ __launch_bounds__(1024) void k_dt(const sv_segk *seg, const double *pcfl,
+                                             const double *pvisc, sv_ctrl *ctrl_all) {
   __shared__ double s1[1024], s2[1024];
+  const sv_segk &S = seg[blockIdx.x];
+  sv_ctrl *c = ctrl_all + blockIdx.x;
+  const double L_char = S.pk.L_char;
   double a = 1.0e10, m = 1.0e10;
-  for (int k = threadIdx.x; k < nb; k += 1024) {
-    a = fmin(a, pcfl[k]);
-    m = fmin(m, pvisc[k]);
+  for (int k = threadIdx.x; k < S.nblk; k += 1024) {
+    a = fmin(a, pcfl[S.blk0 + k]);
+    m = fmin(m, pvisc[S.blk0 + k]);
   }
   s1[threadIdx.x] = a;
   s2[threadIdx.x] = m;
@@ -481,25 +592,30 @@ __global__ __launch_bounds__(1024) void k_dt(int nb, const double *pcfl, const d
   c->time0 = c->time;
 }
 
-// initial derived fields: u, v and c (LC: from psi) of the state
+// initial gather records: psi (LC) and c of the state (ro, u, v by k_apply_state)
 template <int FORM>
-__global__ void k_init_gather(int npfb, sv_fields U, sv_gather G) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npfb) return;
-  G.ro[i] = U.ro[i];
+__global__ __launch_bounds__(BS) void k_init_gather(const int *blk_seg, const sv_segk *seg,
+                                                     const double *U, double *G) {
+  const NodeRef nr = node_ref(blk_seg, seg, (int)blockIdx.x);
+  if (!nr.act) return;
+  const int i = nr.i;
+  double f[6];
+  load_u(U, i, f);
+  constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
+  double *g = G + (size_t)i * S;
+  double c1 = f[3], c2 = f[4], c3 = f[5];
   if (FORM == SV_FORM_LC) {
-    G.pxx[i] = U.pxx[i]; G.pxy[i] = U.pxy[i]; G.pyy[i] = U.pyy[i];
-    double c1, c2, c3;
-    c_from_psi(U.pxx[i], U.pxy[i], U.pyy[i], c1, c2, c3);
-    G.cxx[i] = c1; G.cxy[i] = c2; G.cyy[i] = c3;
-  } else {
-    G.cxx[i] = U.pxx[i]; G.cxy[i] = U.pxy[i]; G.cyy[i] = U.pyy[i];
+    g[SV_G_PXX] = f[3]; g[SV_G_PXY] = f[4]; g[SV_G_PYY] = f[5];
+    c_from_psi(f[3], f[4], f[5], c1, c2, c3);
   }
+  g[SV_G_CXX] = c1; g[SV_G_CXY] = c2; g[SV_G_CYY] = c3;
 }
 
-StageArgs stage_args(sv_ctx *c, int stage, const sv_gather &Gin, const sv_gather &Gout) {
+StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   StageArgs a{};
-  a.npfb = c->npfb;
+  a.rhs_stride = c->npfb;
+  a.blk_seg = c->blk_seg;
+  a.seg = c->segk;
   a.off = c->sell_off;
   a.kcnt = c->cnt_small;
   a.col = c->scol;
@@ -511,16 +627,15 @@ StageArgs stage_args(sv_ctx *c, int stage, const sv_gather &Gin, const sv_gather
   a.e = c->eacc;
   a.Uout = c->U;
   a.Gout = Gout;
-  a.fx = c->fx;
+  a.fx = c->any_forcing ? c->fx : nullptr;
   a.ctrl = c->ctrl;
-  a.pk = c->pk;
   a.partial = c->partial;
   return a;
 }
 
 template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
-  if (FORM == SV_FORM_DI && c->pk.Mdiff != 0.0)
+  if (FORM == SV_FORM_DI && c->any_mdiff)
     k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
   else
     k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
@@ -529,7 +644,7 @@ void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
 // ev (nullable): SV_NKERNELS+1 events recorded between the launches of one step
 template <int FORM>
 int step_form(sv_ctx *c, hipEvent_t *ev) {
-  const int nb = (c->npfb + BS - 1) / BS;
+  const int nb = c->nblk;
   double *pm = c->partial, *pe = c->partial + nb, *pc = c->partial + 2 * nb,
          *pv = c->partial + 3 * nb;
   int ke = 0;
@@ -537,23 +652,21 @@ int step_form(sv_ctx *c, hipEvent_t *ev) {
     if (ev) (void)hipEventRecord(ev[ke++], c->stream);
   };
   mark();
-  k_dt<<<1, 1024, 0, c->stream>>>(nb, pc, pv, c->L_char, c->ctrl);
+  k_dt<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pc, pv, c->ctrl);
   mark();
-  StageArgs a1 = stage_args(c, 1, c->GA, c->GB);
-  launch_stage<FORM, 1>(c, a1, nb);
+  launch_stage<FORM, 1>(c, stage_args(c, 1, c->GA, c->GB), nb);
   mark();
-  StageArgs a2 = stage_args(c, 2, c->GB, c->GA);
-  launch_stage<FORM, 2>(c, a2, nb);
+  launch_stage<FORM, 2>(c, stage_args(c, 2, c->GB, c->GA), nb);
   mark();
-  StageArgs a3 = stage_args(c, 3, c->GA, c->GB);
-  launch_stage<FORM, 3>(c, a3, nb);
+  launch_stage<FORM, 3>(c, stage_args(c, 3, c->GA, c->GB), nb);
   mark();
   StageArgs a4 = stage_args(c, 4, c->GB, c->GA);
   a4.partial = pe;
   launch_stage<FORM, 4>(c, a4, nb);
   mark();
   FilterArgs f{};
-  f.npfb = c->npfb;
+  f.blk_seg = c->blk_seg;
+  f.seg = c->segk;
   f.off = c->sell_off;
   f.kcnt = c->cnt;
   f.col = c->scol;
@@ -563,21 +676,20 @@ int step_form(sv_ctx *c, hipEvent_t *ev) {
   f.U = c->U;
   f.Uf = c->Uf;
   f.Gout = c->GA;
-  f.rho_char = c->pk.rho_char;
   f.partial = pm;
   k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
   mark();
-  k_reduce<<<1, 1024, 0, c->stream>>>(nb, pm, pe, c->volsum, c->ctrl);
+  k_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pm, pe, c->ctrl);
   mark();
   ApplyArgs ap{};
-  ap.npfb = c->npfb;
+  ap.gs = c->gs;
+  ap.blk_seg = c->blk_seg;
+  ap.seg = c->segk;
   ap.Uf = c->Uf;
   ap.U = c->U;
   ap.G = c->GA;
   ap.s = c->s;
-  ap.dro = &c->ctrl->dro;
-  ap.cs = c->pk.cs;
-  ap.visc_total = c->pk.visc_total;
+  ap.ctrl = c->ctrl;
   ap.pcfl = pc;
   ap.pvisc = pv;
   k_apply_state<<<nb, BS, 0, c->stream>>>(ap);
@@ -588,28 +700,34 @@ int step_form(sv_ctx *c, hipEvent_t *ev) {
 
 }  // namespace
 
-int sv_dev_init_state(sv_ctx *c) {
-  const int nb = (c->npfb + BS - 1) / BS;
-  if (c->cfg.form == SV_FORM_LC)
-    k_init_gather<SV_FORM_LC><<<nb, BS, 0, c->stream>>>(c->npfb, c->U, c->GA);
-  else
-    k_init_gather<SV_FORM_DI><<<nb, BS, 0, c->stream>>>(c->npfb, c->U, c->GA);
-  // u, v and the set_tstep partial minima of the initial state (no correction)
+int sv_dev_init_partials(sv_ctx *c) {
+  // u, v and the set_tstep partial minima of the current state (no correction)
+  const int nb = c->nblk;
   ApplyArgs ap{};
-  ap.npfb = c->npfb;
+  ap.gs = c->gs;
+  ap.blk_seg = c->blk_seg;
+  ap.seg = c->segk;
   ap.Uf = c->U;
   ap.U = c->U;
   ap.G = c->GA;
   ap.s = c->s;
-  ap.dro = nullptr;
-  ap.cs = c->pk.cs;
-  ap.visc_total = c->pk.visc_total;
+  ap.ctrl = nullptr;
   ap.pcfl = c->partial + 2 * nb;
   ap.pvisc = c->partial + 3 * nb;
   k_apply_state<<<nb, BS, 0, c->stream>>>(ap);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int sv_dev_init_state(sv_ctx *c) {
+  const int nb = c->nblk;
+  if (c->cfg.form == SV_FORM_LC)
+    k_init_gather<SV_FORM_LC><<<nb, BS, 0, c->stream>>>(c->blk_seg, c->segk, c->U, c->GA);
+  else
+    k_init_gather<SV_FORM_DI><<<nb, BS, 0, c->stream>>>(c->blk_seg, c->segk, c->U, c->GA);
+  SV_HIP(c, hipGetLastError());
+  return sv_dev_init_partials(c);
 }
 
 int sv_dev_step(sv_ctx *c, hipEvent_t *ev) {
@@ -619,15 +737,38 @@ int sv_dev_step(sv_ctx *c, hipEvent_t *ev) {
 // calc_all_rhs on the current state (interior nodes): the stage kernel in its
 // right-hand-side-only form, written into the six arrays of out6 (npfb each).
 int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6) {
-  const int nb = (c->npfb + BS - 1) / BS;
-  const size_t n = c->npfb;
+  const int nb = c->nblk;
   StageArgs a = stage_args(c, 1, c->GA, c->GB);
   a.rk1 = (iRKstep == 1);
-  a.reg_out = sv_fields{out6, out6 + n, out6 + 2 * n, out6 + 3 * n, out6 + 4 * n, out6 + 5 * n,
-                        nullptr, nullptr, nullptr};
+  a.rhs_out = out6;
   if (c->cfg.form == SV_FORM_LC) launch_stage<SV_FORM_LC, 0>(c, a, nb);
   else launch_stage<SV_FORM_DI, 0>(c, a, nb);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+namespace {
+__global__ void k_offset_int(int *dst, int64_t n, int add) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] += add;
+}
+__global__ void k_offset_i64(int64_t *dst, int64_t n, int64_t add) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] += add;
+}
+}  // namespace
+
+int sv_dev_offset_int(sv_ctx *c, int *dst, int64_t n, int add) {
+  if (n <= 0) return 0;
+  k_offset_int<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(dst, n, add);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int sv_dev_offset_i64(sv_ctx *c, int64_t *dst, int64_t n, int64_t add) {
+  if (n <= 0) return 0;
+  k_offset_i64<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(dst, n, add);
+  SV_HIP(c, hipGetLastError());
   return 0;
 }

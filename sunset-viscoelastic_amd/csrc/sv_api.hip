@@ -44,6 +44,11 @@ void sv_free(sv_ctx *c, void *p) {
     if (!(cond)) return sv_fail((c), SV_E_ARG, __VA_ARGS__); \
   } while (0)
 
+int sv_ensure_sell(sv_ctx *c) {
+  if (!c->sell_dirty) return 0;
+  return sv_dev_build_sell(c);
+}
+
 static int set_device(sv_ctx *c) {
   SV_HIP(c, hipSetDevice(c->device));
   return 0;
@@ -94,6 +99,7 @@ int sv_sync(sv_ctx *c) {
 int sv_set_domain(sv_ctx *c, double xmin, double xmax, double ymin, double ymax,
                   const int32_t bc[4], double smax, double L_char) {
   SV_CHECK(c, c && bc, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, xmax > xmin && ymax > ymin && smax > 0 && L_char > 0, "bad domain");
   const bool perx = bc[0] == 1, pery = bc[2] == 1;
   SV_CHECK(c, perx == (bc[1] == 1) && pery == (bc[3] == 1),
@@ -115,6 +121,7 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
                  const double *h, const double *h_small, const int32_t *node_type) {
   SV_CHECK(c, c && x && y && s && h && h_small, "null argument");
   SV_CHECK(c, c->have_domain, "sv_set_domain must come first");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, npfb > 0, "npfb must be positive");
   if (int rc = set_device(c)) return rc;
   for (int i = 0; i < npfb; i++)
@@ -131,8 +138,9 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   c->sell_off = nullptr; c->sell_w = nullptr; c->scol = c->scol_raw = nullptr;
   c->swx = c->swy = c->swl = c->swh = nullptr; c->nslices = 0; c->sell_nnz = 0;
   c->singular = nullptr; c->fx = nullptr; c->partial = nullptr; c->ctrl = nullptr;
-  c->U = c->Uf = c->reg = c->eacc = sv_fields{};
-  c->GA = c->GB = sv_gather{};
+  c->U = c->Uf = c->reg = c->eacc = nullptr;
+  c->GA = c->GB = nullptr;
+  c->segk = nullptr; c->blk_seg = nullptr; c->hseg.clear();
   c->tmp_phi = c->tmp_out = nullptr; c->tmp_cap = 0;
   if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
   c->graph_valid = false;
@@ -234,10 +242,7 @@ int sv_filter_coefficients(sv_ctx *c) {
   return sv_dev_filter_coeffs(c);
 }
 
-static int ensure_sell(sv_ctx *c) {
-  if (!c->sell_dirty) return 0;
-  return sv_dev_build_sell(c);
-}
+static int ensure_sell(sv_ctx *c) { return sv_ensure_sell(c); }
 
 int sv_get_weights(sv_ctx *c, int32_t nplink, double *w_grad, double *w_lap, double *w_hyp,
                    double *gsum, double *lsum, double *hsum, double *vol) {
@@ -390,34 +395,50 @@ static void derive_phys(sv_ctx *c) {
   k.err_norm_rou = 1.0 / (1.0 * p.U_char);
 }
 
-static int alloc_fields(sv_ctx *c, sv_fields *f) {
-  const size_t n = sizeof(double) * c->npfb;
-  double **ptrs[] = {&f->ro, &f->rou, &f->rov, &f->pxx, &f->pxy, &f->pyy};
-  for (double **p : ptrs)
-    if (sv_alloc(c, (void **)p, n)) return SV_E_NOMEM;
-  f->cxx = f->cxy = f->cyy = nullptr;
-  return 0;
-}
-
-static int alloc_gather(sv_ctx *c, sv_gather *g) {
-  const size_t n = sizeof(double) * c->npfb;
-  double **ptrs[] = {&g->ro, &g->u, &g->v, &g->pxx, &g->pxy, &g->pyy, &g->cxx, &g->cxy, &g->cyy};
-  for (double **p : ptrs)
-    if (sv_alloc(c, (void **)p, n)) return SV_E_NOMEM;
-  return 0;
-}
-
 static int ensure_state_buffers(sv_ctx *c) {
-  if (c->U.ro) return 0;
-  if (alloc_fields(c, &c->U) || alloc_fields(c, &c->Uf) || alloc_fields(c, &c->reg) ||
-      alloc_fields(c, &c->eacc) || alloc_gather(c, &c->GA) || alloc_gather(c, &c->GB))
+  if (c->U) return 0;
+  c->gs = c->cfg.form == SV_FORM_LC ? SV_GS_LC : SV_GS_DI;
+  c->nseg = 1;
+  c->nblk = (c->npfb + SV_BLOCK - 1) / SV_BLOCK;
+  c->ntot = c->nblk * SV_BLOCK;
+  const size_t nu = sizeof(double) * SV_US * (size_t)c->ntot;
+  const size_t ng = sizeof(double) * c->gs * (size_t)c->ntot;
+  if (sv_alloc(c, (void **)&c->U, nu) || sv_alloc(c, (void **)&c->Uf, nu) ||
+      sv_alloc(c, (void **)&c->reg, nu) || sv_alloc(c, (void **)&c->eacc, nu) ||
+      sv_alloc(c, (void **)&c->GA, ng) || sv_alloc(c, (void **)&c->GB, ng))
     return SV_E_NOMEM;
-  const int nb = (c->npfb + 255) / 256;
-  c->npartial = 4 * nb;
+  SV_HIP(c, hipMemset(c->GA, 0, ng));
+  SV_HIP(c, hipMemset(c->GB, 0, ng));
+  c->npartial = 4 * c->nblk;
   if (sv_alloc(c, (void **)&c->partial, sizeof(double) * (c->npartial + 8)) ||
       sv_alloc(c, (void **)&c->ctrl, sizeof(sv_ctrl)) ||
-      sv_alloc(c, (void **)&c->fx, sizeof(double) * c->npfb))
+      sv_alloc(c, (void **)&c->fx, sizeof(double) * c->ntot) ||
+      sv_alloc(c, (void **)&c->segk, sizeof(sv_segk)) ||
+      sv_alloc(c, (void **)&c->blk_seg, sizeof(int) * c->nblk))
     return SV_E_NOMEM;
+  SV_HIP(c, hipMemset(c->fx, 0, sizeof(double) * c->ntot));
+  SV_HIP(c, hipMemset(c->blk_seg, 0, sizeof(int) * c->nblk));
+  return 0;
+}
+
+// The single segment of a plain context: physics constants, volume sum and the
+// per-node Kolmogorov forcing (setup only).
+static int upload_segment(sv_ctx *c) {
+  const int n = c->npfb;
+  std::vector<double> y(n), vol(n), fx(n);
+  SV_HIP(c, hipMemcpy(y.data(), c->y, sizeof(double) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(vol.data(), c->vol, sizeof(double) * n, hipMemcpyDeviceToHost));
+  double tv = 0.0;
+  for (int i = 0; i < n; i++) tv += vol[i];
+  c->volsum = tv;
+  for (int i = 0; i < n; i++) fx[i] = c->pk.kol_amp * sin(c->pk.kol_k * y[i]);
+  SV_HIP(c, hipMemcpy(c->fx, fx.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  c->hseg.assign(1, sv_segk{});
+  sv_segk &S = c->hseg[0];
+  S.base = 0; S.npfb = n; S.blk0 = 0; S.nblk = c->nblk; S.volsum = tv; S.pk = c->pk;
+  SV_HIP(c, hipMemcpy(c->segk, &S, sizeof(S), hipMemcpyHostToDevice));
+  c->any_forcing = c->pk.kol_amp != 0.0;
+  c->any_mdiff = c->pk.Mdiff != 0.0;
   return 0;
 }
 
@@ -425,10 +446,16 @@ int sv_set_phys(sv_ctx *c, const sv_phys *p) {
   SV_CHECK(c, c && p, "null argument");
   SV_CHECK(c, p->Re > 0 && p->Wi > 0 && p->Ma > 0 && p->U_char > 0 && p->L_char > 0,
            "Re, Wi, Ma, U_char and L_char must be positive");
+  SV_CHECK(c, !c->is_ensemble, "ensemble context: physics is set per member before "
+                                 "sv_ensemble_create");
   c->phys = *p;
   derive_phys(c);
   c->have_phys = true;
   c->graph_valid = false;
+  if (c->have_state) {
+    if (int rc = set_device(c)) return rc;
+    return upload_segment(c);
+  }
   return 0;
 }
 
@@ -459,6 +486,7 @@ static void psi_from_c(double a, double b, double d, double *pxx, double *pxy, d
 int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v,
                    const double *cxx, const double *cxy, const double *cyy) {
   SV_CHECK(c, c && ro && u && v && cxx && cxy && cyy, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
   SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
   if (int rc = set_device(c)) return rc;
@@ -479,30 +507,18 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
       e[i] = cxx[i]; f[i] = cxy[i]; g[i] = cyy[i];
     }
   }
-  const size_t n8 = sizeof(double) * n;
-  SV_HIP(c, hipMemcpy(c->U.ro, a.data(), n8, hipMemcpyHostToDevice));
-  SV_HIP(c, hipMemcpy(c->U.rou, b.data(), n8, hipMemcpyHostToDevice));
-  SV_HIP(c, hipMemcpy(c->U.rov, d.data(), n8, hipMemcpyHostToDevice));
-  SV_HIP(c, hipMemcpy(c->U.pxx, e.data(), n8, hipMemcpyHostToDevice));
-  SV_HIP(c, hipMemcpy(c->U.pxy, f.data(), n8, hipMemcpyHostToDevice));
-  SV_HIP(c, hipMemcpy(c->U.pyy, g.data(), n8, hipMemcpyHostToDevice));
-  // Kolmogorov forcing per node, and the node-volume sum for the mass correction
-  std::vector<double> y(n), vol(n);
-  SV_HIP(c, hipMemcpy(y.data(), c->y, n8, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(vol.data(), c->vol, n8, hipMemcpyDeviceToHost));
-  double tv = 0.0;
-  for (int i = 0; i < n; i++) tv += vol[i];
-  c->volsum = tv;
-  for (int i = 0; i < n; i++) a[i] = c->pk.kol_amp * sin(c->pk.kol_k * y[i]);
-  SV_HIP(c, hipMemcpy(c->fx, a.data(), n8, hipMemcpyHostToDevice));
+  std::vector<double> rec((size_t)SV_US * n, 0.0);
+  for (int i = 0; i < n; i++) {
+    double *r = rec.data() + (size_t)SV_US * i;
+    r[0] = a[i]; r[1] = b[i]; r[2] = d[i]; r[3] = e[i]; r[4] = f[i]; r[5] = g[i];
+  }
+  SV_HIP(c, hipMemcpy(c->U, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+  if (int rc = upload_segment(c)) return rc;
   sv_ctrl h{};
   h.dt = 1.0e-6;                         // setup_flow.F90:149-151
   h.emax_np1 = h.emax_n = h.emax_nm1 = SV_PID_TOL;
   h.dt_mode = SV_DT_PID;
   SV_HIP(c, hipMemcpy(c->ctrl, &h, sizeof(h), hipMemcpyHostToDevice));
-  if (c->pk.kol_amp == 0.0) {
-    // no forcing: skip the per-node load in the stage kernels
-  }
   if (int rc = sv_dev_init_state(c)) return rc;
   c->have_state = true;
   c->graph_valid = false;
@@ -512,6 +528,7 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
 int sv_set_control(sv_ctx *c, double dt, double time, int32_t itime, double e1, double e2,
                    double e3, int32_t dt_mode) {
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, dt > 0 && e1 > 0 && e2 > 0 && e3 > 0, "dt and error norms must be positive");
   SV_CHECK(c, dt_mode == SV_DT_PID || dt_mode == SV_DT_FIXED, "bad dt mode");
   if (int rc = set_device(c)) return rc;
@@ -524,13 +541,23 @@ int sv_set_control(sv_ctx *c, double dt, double time, int32_t itime, double e1, 
   return 0;
 }
 
+int sv_ensemble_get_control(sv_ctx *c, int32_t member, double *dt, double *time,
+                            int32_t *itime, double *emax3, double *dt_cfl, double *dt_par);
+
 int sv_get_control(sv_ctx *c, double *dt, double *time, int32_t *itime, double *emax3,
                    double *dt_cfl, double *dt_par) {
+  return sv_ensemble_get_control(c, 0, dt, time, itime, emax3, dt_cfl, dt_par);
+}
+
+int sv_ensemble_get_control(sv_ctx *c, int32_t member, double *dt, double *time,
+                            int32_t *itime, double *emax3, double *dt_cfl, double *dt_par) {
   SV_CHECK(c, c && c->have_state, "no state");
+  SV_CHECK(c, member >= 0 && member < c->nseg, "member %d out of range [0, %d)", member,
+           c->nseg);
   if (int rc = set_device(c)) return rc;
   SV_HIP(c, hipStreamSynchronize(c->stream));
   sv_ctrl h{};
-  SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(&h, c->ctrl + member, sizeof(h), hipMemcpyDeviceToHost));
   if (dt) *dt = h.dt;
   if (time) *time = h.time;
   if (itime) *itime = h.itime;
@@ -544,6 +571,7 @@ int sv_get_control(sv_ctx *c, double *dt, double *time, int32_t *itime, double *
 int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *r2, double *r3,
                     double *r4, double *r5) {
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   if (int rc = set_device(c)) return rc;
   double *out = nullptr;
   SV_HIP(c, hipMalloc(&out, sizeof(double) * 6 * c->npfb));
@@ -638,20 +666,39 @@ int sv_get_stencil_stats(sv_ctx *c, double *stats) {
   return 0;
 }
 
+int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, double *rov,
+                          double *u, double *v, double *pxx, double *pxy, double *pyy,
+                          double *cxx, double *cxy, double *cyy);
+
 int sv_get_state(sv_ctx *c, double *ro, double *rou, double *rov, double *u, double *v,
                  double *pxx, double *pxy, double *pyy, double *cxx, double *cxy, double *cyy) {
+  return sv_ensemble_get_state(c, 0, ro, rou, rov, u, v, pxx, pxy, pyy, cxx, cxy, cyy);
+}
+
+int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, double *rov,
+                          double *u, double *v, double *pxx, double *pxy, double *pyy,
+                          double *cxx, double *cxy, double *cyy) {
   SV_CHECK(c, c && c->have_state, "no state");
+  SV_CHECK(c, member >= 0 && member < c->nseg, "member %d out of range [0, %d)", member,
+           c->nseg);
   if (int rc = set_device(c)) return rc;
+  const sv_segk &S = c->hseg[member];
+  const double *U = c->U + (size_t)S.base * SV_US;
+  const double *GA = c->GA + (size_t)S.base * c->gs;
   SV_HIP(c, hipStreamSynchronize(c->stream));
-  const size_t n8 = sizeof(double) * c->npfb;
-  struct { double *dst; const double *src; } m[] = {
-      {ro, c->U.ro}, {rou, c->U.rou}, {rov, c->U.rov}, {u, c->GA.u}, {v, c->GA.v},
-      {pxx, c->cfg.form == SV_FORM_LC ? c->U.pxx : nullptr},
-      {pxy, c->cfg.form == SV_FORM_LC ? c->U.pxy : nullptr},
-      {pyy, c->cfg.form == SV_FORM_LC ? c->U.pyy : nullptr},
-      {cxx, c->GA.cxx}, {cxy, c->GA.cxy}, {cyy, c->GA.cyy}};
+  // strided copies out of the records
+  const bool lc = c->cfg.form == SV_FORM_LC;
+  struct { double *dst; const double *src; int stride; } m[] = {
+      {ro, U + 0, SV_US}, {rou, U + 1, SV_US}, {rov, U + 2, SV_US},
+      {u, GA + SV_G_U, c->gs}, {v, GA + SV_G_V, c->gs},
+      {pxx, lc ? U + 3 : nullptr, SV_US}, {pxy, lc ? U + 4 : nullptr, SV_US},
+      {pyy, lc ? U + 5 : nullptr, SV_US},
+      {cxx, GA + SV_G_CXX, c->gs}, {cxy, GA + SV_G_CXY, c->gs},
+      {cyy, GA + SV_G_CYY, c->gs}};
   for (auto &e : m)
-    if (e.dst && e.src) SV_HIP(c, hipMemcpy(e.dst, e.src, n8, hipMemcpyDeviceToHost));
+    if (e.dst && e.src)
+      SV_HIP(c, hipMemcpy2D(e.dst, sizeof(double), e.src, sizeof(double) * e.stride,
+                            sizeof(double), S.npfb, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -1,0 +1,153 @@
+// sv_ensemble.hip -- one device context holding several independent simulations
+// (members of different sizes, orders and physics) so that one launch per
+// kernel, and one captured hipGraph per step, advances all of them.
+//
+// Layout: every member's per-node arrays (SELL stencils, sums, state records)
+// are concatenated, member m starting at node base_m, a multiple of SV_BLOCK,
+// with its SELL entries offset and its column indices rebased.  Each block of
+// the node kernels therefore belongs to one member; the per-member reductions
+// (time step, error norm, mass correction) run one block per member over that
+// member's block range, and each member keeps its own control block (dt, PID
+// history, itime) and physics constants.  The members' own contexts stay valid
+// and unchanged: the ensemble holds copies taken at creation.
+#include <string.h>
+
+#include "sv_internal.h"
+
+namespace {
+
+int copy_d2d(sv_ctx *E, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return 0;
+  SV_HIP(E, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, E->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
+  if (!members || !out || nmembers < 1) return SV_E_ARG;
+  *out = nullptr;
+  const sv_ctx *m0 = members[0];
+  if (!m0) return SV_E_ARG;
+  for (int m = 0; m < nmembers; m++) {
+    const sv_ctx *c = members[m];
+    if (!c || c->is_ensemble || !c->have_state || c->cfg.form != m0->cfg.form ||
+        c->device != m0->device)
+      return SV_E_ARG;
+  }
+  sv_config cfg = m0->cfg;
+  sv_ctx *E = nullptr;
+  if (int rc = sv_create(&cfg, &E)) return rc;
+  auto fail = [&](int rc) {
+    sv_destroy(E);
+    return rc;
+  };
+  E->is_ensemble = true;
+  E->nseg = nmembers;
+  E->gs = m0->gs;
+  // layout
+  E->hseg.resize(nmembers);
+  std::vector<int64_t> sell_base(nmembers);
+  int64_t nodes = 0, sell = 0;
+  int blocks = 0;
+  for (int m = 0; m < nmembers; m++) {
+    sv_ctx *c = members[m];
+    if (hipSetDevice(c->device) != hipSuccess) return fail(SV_E_HIP);
+    if (int rc = sv_ensure_sell(c)) return fail(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SV_E_HIP);
+    sv_segk &S = E->hseg[m];
+    S.base = (int)nodes;
+    S.npfb = c->npfb;
+    S.blk0 = blocks;
+    S.nblk = (c->npfb + SV_BLOCK - 1) / SV_BLOCK;
+    S.volsum = c->volsum;
+    S.pk = c->pk;
+    sell_base[m] = sell;
+    blocks += S.nblk;
+    nodes += (int64_t)S.nblk * SV_BLOCK;
+    sell += c->sell_nnz;
+    E->any_forcing |= c->pk.kol_amp != 0.0;
+    E->any_mdiff |= c->pk.Mdiff != 0.0;
+  }
+  if (nodes > INT32_MAX / 2) {
+    sv_destroy(E);
+    return SV_E_ARG;
+  }
+  (void)hipSetDevice(E->device);
+  E->ntot = (int)nodes;
+  E->nblk = blocks;
+  E->npfb = E->ntot;
+  E->nslices = E->ntot / SV_WAVE;
+  E->sell_nnz = sell;
+  const size_t N = (size_t)E->ntot, D = sizeof(double), I = sizeof(int);
+  void **bufs[] = {(void **)&E->sell_off, (void **)&E->cnt, (void **)&E->cnt_small,
+                   (void **)&E->scol, (void **)&E->swx, (void **)&E->swy, (void **)&E->swl,
+                   (void **)&E->swh, (void **)&E->gsx, (void **)&E->gsy, (void **)&E->lsum,
+                   (void **)&E->hsum, (void **)&E->vol, (void **)&E->s, (void **)&E->fx,
+                   (void **)&E->U, (void **)&E->Uf, (void **)&E->reg, (void **)&E->eacc,
+                   (void **)&E->GA, (void **)&E->GB, (void **)&E->partial, (void **)&E->ctrl,
+                   (void **)&E->segk, (void **)&E->blk_seg};
+  const size_t sizes[] = {sizeof(int64_t) * (E->nslices + 1), I * N, I * N,
+                          I * (size_t)sell, D * (size_t)sell, D * (size_t)sell,
+                          D * (size_t)sell, D * (size_t)sell, D * N, D * N, D * N, D * N, D * N,
+                          D * N, D * N, D * SV_US * N, D * SV_US * N, D * SV_US * N,
+                          D * SV_US * N, D * E->gs * N, D * E->gs * N,
+                          D * (4 * (size_t)blocks + 8), sizeof(sv_ctrl) * nmembers,
+                          sizeof(sv_segk) * nmembers, I * (size_t)blocks};
+  for (size_t k = 0; k < sizeof(sizes) / sizeof(sizes[0]); k++)
+    if (sv_alloc(E, bufs[k], sizes[k])) return fail(SV_E_NOMEM);
+  E->npartial = 4 * blocks;
+  if (hipMemsetAsync(E->sell_off, 0, sizes[0], E->stream) != hipSuccess ||
+      hipMemsetAsync(E->cnt, 0, I * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->cnt_small, 0, I * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->fx, 0, D * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->s, 0, D * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->U, 0, D * SV_US * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->GA, 0, D * E->gs * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(E->GB, 0, D * E->gs * N, E->stream) != hipSuccess)
+    return fail(SV_E_HIP);
+  std::vector<int> bseg(blocks);
+  for (int m = 0; m < nmembers; m++) {
+    const sv_ctx *c = members[m];
+    const sv_segk &S = E->hseg[m];
+    const size_t n = (size_t)S.npfb, b = (size_t)S.base;
+    const int64_t sb = sell_base[m];
+    for (int k = 0; k < S.nblk; k++) bseg[S.blk0 + k] = m;
+    int rc = copy_d2d(E, E->sell_off + b / SV_WAVE, c->sell_off,
+                      sizeof(int64_t) * (size_t)c->nslices);
+    if (!rc) rc = sv_dev_offset_i64(E, E->sell_off + b / SV_WAVE, c->nslices, sb);
+    if (!rc) rc = copy_d2d(E, E->scol + sb, c->scol, I * (size_t)c->sell_nnz);
+    if (!rc) rc = sv_dev_offset_int(E, E->scol + sb, c->sell_nnz, S.base);
+    const double *wsrc[4] = {c->swx, c->swy, c->swl, c->swh};
+    double *wdst[4] = {E->swx, E->swy, E->swl, E->swh};
+    for (int k = 0; k < 4 && !rc; k++)
+      rc = copy_d2d(E, wdst[k] + sb, wsrc[k], D * (size_t)c->sell_nnz);
+    if (!rc) rc = copy_d2d(E, E->cnt + b, c->cnt, I * n);
+    if (!rc) rc = copy_d2d(E, E->cnt_small + b, c->cnt_small, I * n);
+    const double *nsrc[7] = {c->gsx, c->gsy, c->lsum, c->hsum, c->vol, c->s, c->fx};
+    double *ndst[7] = {E->gsx, E->gsy, E->lsum, E->hsum, E->vol, E->s, E->fx};
+    for (int k = 0; k < 7 && !rc; k++) rc = copy_d2d(E, ndst[k] + b, nsrc[k], D * n);
+    if (!rc) rc = copy_d2d(E, E->U + b * SV_US, c->U, D * SV_US * n);
+    if (!rc) rc = copy_d2d(E, E->GA + b * E->gs, c->GA, D * E->gs * n);
+    if (!rc) rc = copy_d2d(E, E->ctrl + m, c->ctrl, sizeof(sv_ctrl));
+    if (rc) return fail(rc);
+  }
+  if (hipMemcpyAsync(E->segk, E->hseg.data(), sizeof(sv_segk) * nmembers,
+                     hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+      hipMemcpyAsync(E->blk_seg, bseg.data(), I * (size_t)blocks, hipMemcpyHostToDevice,
+                     E->stream) != hipSuccess ||
+      hipStreamSynchronize(E->stream) != hipSuccess)
+    return fail(SV_E_HIP);
+  E->sell_dirty = false;
+  E->have_phys = true;
+  E->have_state = true;
+  if (int rc = sv_dev_init_partials(E)) return fail(rc);
+  *out = E;
+  return 0;
+}
+
+int32_t sv_ensemble_size(const sv_ctx *c) { return c ? c->nseg : 0; }
+
+}  // extern "C"

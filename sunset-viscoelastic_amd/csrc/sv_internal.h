@@ -14,8 +14,8 @@
 //                scol_raw (ghost ids), scol (ghosts folded to parents),
 //                swx, swy, swl, swh.  Padding: col = own node, weight 0.
 //   sums         gsx, gsy, lsum, hsum, vol      [npfb]
-//   state        structure of arrays on real nodes only (ghosts are resolved
-//                through scol at gather time).
+//   state        records on real nodes only (ghosts are resolved through scol
+//                at gather time), see below.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -64,13 +64,37 @@ struct sv_physk {
   double err_norm_rou;   // 1/U_char (setup_flow.F90:156)
 };
 
-// Fields of the state (real nodes).  LC: p* = log-conformation; DI: p* unused.
-struct sv_fields {
-  double *ro, *rou, *rov, *pxx, *pxy, *pyy, *cxx, *cxy, *cyy;
-};
-// Gather buffer: what neighbours read in a stage.
-struct sv_gather {
-  double *ro, *u, *v, *pxx, *pxy, *pyy, *cxx, *cxy, *cyy;
+// State records (real nodes only; ghosts are resolved through the folded
+// stencil columns).  Gather record of node j, what a neighbour sweep reads:
+//   [ro, u, v, cxx, cxy, cyy, psixx, psixy, psiyy]  (LC: 16 doubles = 128 B)
+//   [ro, u, v, cxx, cxy, cyy]                        (DI:  8 doubles =  64 B)
+// State / register / error records: [ro, rou, rov, f1, f2, f3] in 8 doubles,
+// f = psi (LC) or c (DI).
+#define SV_GS_LC 16
+#define SV_GS_DI 8
+#define SV_US 8
+#define SV_G_RO 0
+#define SV_G_U 1
+#define SV_G_V 2
+#define SV_G_CXX 3
+#define SV_G_CXY 4
+#define SV_G_CYY 5
+#define SV_G_PXX 6
+#define SV_G_PXY 7
+#define SV_G_PYY 8
+
+// A segment is one simulation inside a context.  A single context has one; an
+// ensemble context concatenates its members, each starting at a multiple of
+// SV_BLOCK nodes, so every block of the node kernels lies in one member and
+// per-block partial reductions are per-member partial reductions.
+#define SV_BLOCK 256
+struct sv_segk {
+  int base;      // first node (concatenated numbering), a multiple of SV_BLOCK
+  int npfb;      // real nodes
+  int blk0;      // first logical block
+  int nblk;      // blocks
+  double volsum; // sum of node volumes (the mass-correction denominator)
+  sv_physk pk;
 };
 
 struct sv_ctx {
@@ -113,12 +137,20 @@ struct sv_ctx {
   bool have_phys = false, have_state = false;
   sv_physk pk{};
   sv_phys phys{};
-  sv_fields U{}, Uf{}, reg{}, eacc{};
-  sv_gather GA{}, GB{};
+  double *U = nullptr, *Uf = nullptr, *reg = nullptr, *eacc = nullptr;  // [npfb][SV_US]
+  double *GA = nullptr, *GB = nullptr;                                 // [npfb][gs]
+  int gs = SV_GS_LC;
   double *fx = nullptr;    // per-node Kolmogorov forcing
   double *partial = nullptr; int npartial = 0;
   double volsum = 0.0;     // sum of node volumes (the mass-correction denominator)
-  sv_ctrl *ctrl = nullptr;
+  sv_ctrl *ctrl = nullptr;  // [nseg]
+  // segments: ntot = padded node count of the state arrays, nblk = blocks
+  int nseg = 1, ntot = 0, nblk = 0;
+  sv_segk *segk = nullptr;  // [nseg]
+  int *blk_seg = nullptr;   // [nblk] segment of each logical block
+  bool is_ensemble = false;
+  bool any_forcing = false, any_mdiff = false;  // over segments
+  std::vector<sv_segk> hseg;
   // scratch for drop-in ops
   double *tmp_phi = nullptr, *tmp_out = nullptr; int tmp_cap = 0;
   // graph
@@ -140,6 +172,7 @@ int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
                      hipGetErrorString(e_), __FILE__, __LINE__);               \
   } while (0)
 int sv_alloc(sv_ctx *c, void **p, size_t bytes);
+int sv_ensure_sell(sv_ctx *c);
 void sv_free(sv_ctx *c, void *p);
 
 // ---- kernels: nodes/neighbours (k_nodes.hip) ----
@@ -159,7 +192,11 @@ int sv_dev_init_state(sv_ctx *c);
 #define SV_NKERNELS 8
 int sv_dev_step(sv_ctx *c, hipEvent_t *ev = nullptr);
 int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6);
-int sv_dev_dt_init(sv_ctx *c);
+// per-block partial minima of set_tstep for the current state (no correction)
+int sv_dev_init_partials(sv_ctx *c);
+// dst[k] += add for k < n (index rebasing when members are concatenated)
+int sv_dev_offset_int(sv_ctx *c, int *dst, int64_t n, int add);
+int sv_dev_offset_i64(sv_ctx *c, int64_t *dst, int64_t n, int64_t add);
 
 // XCD-aware block index: consecutive logical blocks land on the same XCD (one
 // L2), so neighbouring slices share cached field lines.  Blocks are dispatched

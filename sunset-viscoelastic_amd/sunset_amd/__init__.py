@@ -2,5 +2,5 @@
 
 Python host of the C ABI in include/sunset_amd.h: node sets and formats
 (`nodes`), the device solver (`core.Solver`)."""
-from .core import Solver, SunsetError, lib  # noqa: F401
+from .core import Ensemble, Solver, SunsetError, lib  # noqa: F401
 from . import nodes  # noqa: F401

@@ -52,7 +52,8 @@ EXPORTS = [
     "sv_calc_gradient", "sv_calc_laplacian", "sv_calc_gradient_only", "sv_calc_filtered_var",
     "sv_calc_filter_term", "sv_apply_dev", "sv_set_phys", "sv_set_initial", "sv_set_control",
     "sv_get_control", "sv_calc_all_rhs", "sv_run", "sv_get_state", "sv_sync",
-    "sv_run_profiled", "sv_get_stencil_stats",
+    "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
+    "sv_ensemble_get_state", "sv_ensemble_get_control",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -103,6 +104,12 @@ def lib(path: Optional[str] = None):
     L.sv_sync.argtypes = [C.c_void_p]
     L.sv_run_profiled.argtypes = [C.c_void_p, C.c_int32, PD, PD]
     L.sv_get_stencil_stats.argtypes = [C.c_void_p, PD]
+    L.sv_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+    L.sv_ensemble_size.argtypes = [C.c_void_p]
+    L.sv_ensemble_size.restype = C.c_int32
+    L.sv_ensemble_get_state.argtypes = [C.c_void_p, C.c_int32] + [PD] * 11
+    L.sv_ensemble_get_control.argtypes = [C.c_void_p, C.c_int32, PD, PD, C.POINTER(C.c_int32),
+                                          PD, PD, PD]
     _lib = L
     return L
 
@@ -304,6 +311,74 @@ class Solver:
         em = np.zeros(3)
         self._ck(lib().sv_get_control(self.h, C.byref(dt), C.byref(tm), C.byref(it), _d(em),
                                       C.byref(dc), C.byref(dp)))
+        out.update(dt=dt.value, time=tm.value, itime=it.value, emax_np1=em[0], emax_n=em[1],
+                   emax_nm1=em[2], dt_cfl=dc.value, dt_par=dp.value)
+        return out
+
+
+_FIELD_KEYS = ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy")
+
+
+class Ensemble:
+    """Several Solver contexts (each fully set up, with an initial state)
+    advanced together: one launch per kernel and one hipGraph per step for all
+    members.  The members' own contexts are left unchanged."""
+
+    def __init__(self, members):
+        L = lib()
+        members = list(members)
+        if not members:
+            raise SunsetError("an ensemble needs at least one member")
+        arr = (C.c_void_p * len(members))(*[m.h.value for m in members])
+        h = C.c_void_p()
+        rc = L.sv_ensemble_create(arr, len(members), C.byref(h))
+        if rc != 0:
+            msgs = "; ".join(L.sv_last_error(m.h).decode() for m in members)
+            raise SunsetError(f"sv_ensemble_create failed ({rc}) {msgs}")
+        self.h = h
+        self.sizes = [m.npfb for m in members]
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise SunsetError(f"error {rc}: {lib().sv_last_error(self.h).decode()}")
+
+    def run(self, nsteps: int, graph: bool = False):
+        self._ck(lib().sv_run(self.h, nsteps, 1 if graph else 0))
+
+    def run_profiled(self, nsteps: int):
+        k = np.zeros(len(KERNELS))
+        tot = C.c_double()
+        self._ck(lib().sv_run_profiled(self.h, nsteps, _d(k), C.byref(tot)))
+        return dict(zip(KERNELS, k)), tot.value
+
+    def step_ms(self) -> float:
+        v = C.c_double()
+        self._ck(lib().sv_get_timing(self.h, b"step", C.byref(v)))
+        return v.value
+
+    def fields(self, member: int):
+        n = self.sizes[member]
+        out = {k: np.zeros(n) for k in _FIELD_KEYS}
+        self._ck(lib().sv_ensemble_get_state(self.h, member, *[_d(out[k]) for k in _FIELD_KEYS]))
+        dt, tm, dc, dp = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        it = C.c_int32()
+        em = np.zeros(3)
+        self._ck(lib().sv_ensemble_get_control(self.h, member, C.byref(dt), C.byref(tm),
+                                               C.byref(it), _d(em), C.byref(dc), C.byref(dp)))
         out.update(dt=dt.value, time=tm.value, itime=it.value, emax_np1=em[0], emax_n=em[1],
                    emax_nm1=em[2], dt_cfl=dc.value, dt_par=dp.value)
         return out

@@ -141,6 +141,15 @@ class OracleSim:
         lib().or_calc_labf_sums(C.byref(self.d))
         lib().or_filter_coefficients(C.byref(self.d))
 
+    def weights_ordermap(self, order_map, nthreads=0):
+        """Weights with a per-node LABFM order (derivative and filter system)."""
+        om = np.ascontiguousarray(order_map, dtype=np.int32)
+        assert om.shape == (self.d.npfb,)
+        self._om = om
+        assert lib().or_calc_labf_weights_ordermap(C.byref(self.d), _pi(om), nthreads) == 0
+        lib().or_calc_labf_sums(C.byref(self.d))
+        lib().or_filter_coefficients(C.byref(self.d))
+
     def setup(self):
         self.ghosts()
         self.neighbours()

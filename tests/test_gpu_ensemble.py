@@ -1,0 +1,77 @@
+"""Ensembles: members of different sizes, orders and physics advanced together
+through sv_ensemble_create / sv_run.  Each member of the ensemble must end
+bit for bit where the same member run alone ends (same kernels, same
+per-member reductions in the same order), eager and graph-replayed, for
+several steps with PID time-step control (per-member dt histories diverge)."""
+import math
+
+import numpy as np
+import pytest
+
+from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [  # (lattice side, order, Wi, beta, eps, Kolmogorov amplitude)
+    (20, 4, 0.3, 0.2, 0.0, 0.0),
+    (33, 6, 2.0, 0.1, 0.25, 1.0),
+    (24, 6, 7.0, 0.8, 0.0, 0.5),
+    (17, 4, 1.0, 0.5, 0.1, 0.0),
+]
+KEYS = ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy")
+
+
+def _member(sunset, n, order, Wi, beta, eps, kol, form, seed):
+    ns = jittered_lattice(n, seed=seed, h_over_s={4: 1.8, 6: 2.4}[order])
+    s = sunset.Solver(morder=order, form=form).setup(ns)
+    s.set_phys(Re=1.0, Wi=Wi, beta=beta, eps=eps, Ma=0.1, kol_amp=kol, kol_k=4 * math.pi,
+               Mdiff=1e-3 if (form == "di" and seed % 2) else 0.0)
+    ic = taylor_green_ic(ns)
+    ic["cxx"] = 1.0 + 0.1 * np.sin(2 * np.pi * ns.y) ** 2
+    s.set_initial(ic)
+    return s
+
+
+@pytest.mark.parametrize("form", ["lc", "di"])
+def test_ensemble_matches_members_run_alone(form):
+    import sunset_amd
+    print(f"[ensemble] form {form}, {len(MEMBERS)} members", flush=True)
+    steps = 12
+    alone = []
+    for k, m in enumerate(MEMBERS):
+        s = _member(sunset_amd, *m, form=form, seed=k)
+        s.run(steps)
+        alone.append(s.fields())
+        s.close()
+    for graph in (False, True):
+        members = [_member(sunset_amd, *m, form=form, seed=k) for k, m in enumerate(MEMBERS)]
+        E = sunset_amd.Ensemble(members)
+        assert len(E) == len(MEMBERS)
+        E.run(steps // 2, graph=graph)
+        _ = E.fields(0)                      # host synchronisation between replays
+        E.run(steps - steps // 2, graph=graph)
+        for k in range(len(MEMBERS)):
+            F = E.fields(k)
+            assert F["itime"] == steps
+            for key in KEYS:
+                if form == "di" and key.startswith("p"):
+                    continue
+                assert np.array_equal(F[key], alone[k][key]), (graph, k, key)
+            assert F["dt"] == alone[k]["dt"] and F["time"] == alone[k]["time"], (graph, k)
+        # the members' own contexts were not advanced
+        assert members[0].fields()["itime"] == 0
+        E.close()
+        for s in members:
+            s.close()
+    dts = [a["dt"] for a in alone]
+    assert len(set(dts)) > 1      # the PID controllers really are per member
+
+
+def test_ensemble_rejects_mixed_forms():
+    import sunset_amd
+    a = _member(sunset_amd, 16, 4, 1.0, 0.5, 0.0, 0.0, form="lc", seed=0)
+    b = _member(sunset_amd, 16, 4, 1.0, 0.5, 0.0, 0.0, form="di", seed=1)
+    with pytest.raises(sunset_amd.SunsetError):
+        sunset_amd.Ensemble([a, b])
+    a.close()
+    b.close()
