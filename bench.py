@@ -127,6 +127,96 @@ def cpu_weights_baseline(n_sample):
                        f"calc_labf_weights {t:.2f} s (serial loop)")
 
 
+def ensemble_leg(members=32, steps=100, seed=7):
+    """32 simulations of different sizes in one context: N_m = round(exp(U(ln 16k,
+    ln 256k))) nodes (square jittered lattices), LABFM order 4 or 6, Wi =
+    exp(U(ln 0.1, ln 10)), beta = U(0.1, 0.9); log-conformation Oldroyd-B,
+    Re = 1, Ma = 0.1, Kolmogorov forcing sin(4 pi y), PID dt.  Times `steps`
+    steps of the whole ensemble (one hipGraph per step) against the same
+    members run one after another (each its own graph)."""
+    import numpy as np
+    import sunset_amd
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+    rng = np.random.Generator(np.random.PCG64(seed))
+    specs = []
+    for m in range(members):
+        N = int(round(math.exp(rng.uniform(math.log(16384), math.log(262144)))))
+        side = max(16, int(round(math.sqrt(N))))
+        order = int(rng.choice([4, 6]))
+        Wi = math.exp(rng.uniform(math.log(0.1), math.log(10.0)))
+        beta = rng.uniform(0.1, 0.9)
+        specs.append((side, order, Wi, beta))
+
+    def build():
+        out = []
+        for k, (side, order, Wi, beta) in enumerate(specs):
+            ns = jittered_lattice(side, seed=100 + k, h_over_s={4: 1.8, 6: 2.4}[order])
+            s = sunset_amd.Solver(morder=order, form="lc").setup(ns)
+            s.set_phys(Re=1.0, Wi=Wi, beta=beta, eps=0.0, Ma=0.1, kol_amp=1.0,
+                       kol_k=4 * math.pi)
+            s.set_initial(taylor_green_ic(ns))
+            out.append(s)
+        return out
+
+    mem = build()
+    nodes = sum(s.npfb for s in mem)
+    t_seq = 0.0
+    for s in mem:
+        s.run(2, graph=True)
+        s.run(steps, graph=True)
+        t_seq += s.timing("step")
+    for s in mem:
+        s.close()
+    mem = build()
+    E = sunset_amd.Ensemble(mem)
+    E.run(2, graph=True)
+    E.run(steps, graph=True)
+    t_ens = E.step_ms()
+    ok = all(np.isfinite(E.fields(k)["ro"]).all() for k in range(len(mem)))
+    E.close()
+    for s in mem:
+        s.close()
+    if not ok:
+        raise SystemExit("bench.py: non-finite ensemble state")
+    return dict(members=members, nodes=nodes, steps=steps,
+                sizes=[side * side for side, *_ in specs],
+                orders=[o for _, o, _, _ in specs],
+                ensemble_ms=round(t_ens, 2), sequential_ms=round(t_seq, 2),
+                node_steps_per_s=round(nodes * steps / (t_ens / 1e3), 1),
+                sequential_node_steps_per_s=round(nodes * steps / (t_seq / 1e3), 1),
+                speedup=round(t_seq / t_ens, 2))
+
+
+def adjoint_leg(n=128, steps=400, every=20):
+    """Checkpointed reverse mode on a 128^2 lattice, order 6, direct-form
+    Oldroyd-B (Wi = 1, beta = 0.5, Re = 1, Ma = 0.1), fluid at rest plus a
+    random-phase velocity perturbation of amplitude 0.01 (seed 5), Kolmogorov
+    forcing sin(4 pi y), fixed dt = 2e-5; loss (1/T) sum_t mean(tr c - 2) over
+    400 steps, a checkpoint every 20 steps.  Ratio = device time of the
+    gradient (recompute + adjoint sweep) over the forward rollout."""
+    import sunset_amd
+    from sunset_amd.nodes import jittered_lattice, perturbed_rest_ic
+    ns = jittered_lattice(n, seed=5, h_over_s=2.4)
+    s = sunset_amd.Solver(morder=6, form="di").setup(ns)
+    s.set_phys(Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1, kol_amp=1.0, kol_k=4 * math.pi)
+    ic = perturbed_rest_ic(ns, amp=0.01, seed=5)
+    U0 = dict(ro=ic["ro"], rou=ic["ro"] * ic["u"], rov=ic["ro"] * ic["v"], cxx=ic["cxx"],
+              cxy=ic["cxy"], cyy=ic["cyy"])
+    s.set_state(U0, dt=2e-5)
+    s.adj_setup(every)
+    s.rollout_forward(steps)           # warm-up (allocations, first launches)
+    s.rollout_vjp(1.0)
+    s.set_state(U0, dt=2e-5)
+    L = s.rollout_forward(steps)
+    g = s.rollout_vjp(1.0)
+    fwd, bwd = s.adj_timing()
+    s.close()
+    import numpy as np
+    gn = math.sqrt(sum(float(np.dot(v, v)) for v in g.values()))
+    return dict(nodes=ns.npfb, steps=steps, checkpoint_every=every, loss=L, grad_norm=gn,
+                forward_ms=round(fwd, 2), vjp_ms=round(bwd, 2), ratio=round(bwd / fwd, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +231,8 @@ def main():
                     help="cylinder-channel weight setup: full = spacing 0.0013..0.0052 "
                          "(~1.2M nodes), spec = 0.004..0.016 (~126k nodes)")
     ap.add_argument("--cpu-weights-n", type=int, default=64)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -213,6 +305,15 @@ def main():
                   file=sys.stderr, flush=True)
             c3["cpu_baseline"] = cpu_weights_baseline(args.cpu_weights_n)
 
+    ens = None
+    if not args.no_ensemble:
+        print("[bench] ensemble: 32 members", file=sys.stderr, flush=True)
+        ens = ensemble_leg()
+    adj = None
+    if not args.no_adjoint:
+        print("[bench] adjoint: 128^2 DI rollout of 400 steps", file=sys.stderr, flush=True)
+        adj = adjoint_leg()
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "node-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -231,7 +332,9 @@ def main():
         "cylinder_channel_weights": c3,
         "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
-        "adjoint_forward_ratio": None,
+        "ensemble": ens,
+        "adjoint": adj,
+        "adjoint_forward_ratio": adj["ratio"] if adj else None,
     }
     print(json.dumps(out))
 

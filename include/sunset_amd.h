@@ -187,6 +187,31 @@ int sv_ensemble_get_control(sv_ctx *ensemble, int32_t member, double *dt, double
                             int32_t *itime, double *emax3, double *dt_cfl,
                             double *dt_parabolic);
 
+/* ---- Reverse mode (DI form, fixed time step) ----------------------------
+ * No counterpart in the reference (it has no adjoint); these are the
+ * vector-Jacobian products of the reference's own operators: calc_all_rhs
+ * (rhs_di.F90:47-476), the RK3(2)4[2R+]C stages and registers
+ * (step.F90:281-409), the hyperviscous filter (derivatives.F90:360-414) and the
+ * mass correction (rhs_di.F90:667-720), with stencil transposes built once
+ * (no atomics, deterministic).  State vectors are (ro, rou, rov, cxx, cxy, cyy)
+ * on real nodes, six arrays of npfb.
+ * sv_adj_setup: after sv_set_initial and sv_set_control(..., SV_DT_FIXED);
+ * checkpoint_every = steps between stored states in a rollout (the backward
+ * sweep recomputes one interval at a time). */
+int sv_adj_setup(sv_ctx *ctx, int32_t checkpoint_every);
+/* ubar = (d calc_all_rhs / d state)^T rbar at the current state */
+int sv_rhs_vjp(sv_ctx *ctx, const double *const rbar[6], double *const ubar[6]);
+/* nsteps from the current state, storing checkpoints; loss = (1/nsteps)
+ * sum_{t=1..nsteps} mean_i (cxx_i + cyy_i - 2) of the state after each step */
+int sv_rollout_forward(sv_ctx *ctx, int32_t nsteps, double *loss);
+/* gradient of  loss_bar*loss + <ubar_final, state at the end>  with respect to
+ * the state at the start of the last sv_rollout_forward (ubar_final nullable);
+ * the context's state is left at the end of the rollout */
+int sv_rollout_vjp(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
+                   double *const grad0[6]);
+/* device time (ms) of the last sv_rollout_forward and sv_rollout_vjp */
+int sv_adj_timing(sv_ctx *ctx, double *forward_ms, double *vjp_ms);
+
 #ifdef __cplusplus
 }
 #endif

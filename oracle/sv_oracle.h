@@ -25,6 +25,8 @@
  *                           DI form is pinned by relaxation/shear closed forms
  *                           and LC<->DI consistency, not by the reference.
  *   or_step_rk3          -- oracle/_ref (LC); Butcher order conditions
+ *   or_*_tlm (DI, fixed dt) -- base trajectory bit-identical to or_run; tangent
+ *                           vs central finite differences of or_run
  */
 #ifndef SV_ORACLE_H
 #define SV_ORACLE_H
@@ -128,6 +130,23 @@ void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps);
 
 int  or_num_threads(void);
 void or_set_num_threads(int n);
+
+/* ---- tangent-linear model of the DI step at a fixed dt (sv_oracle_tlm.c) ---- */
+typedef struct {
+  int np;
+  double *ro, *rou, *rov, *u, *v, *cxx, *cxy, *cyy;   /* [np] tangents */
+} or_tangent;
+int  or_tangent_alloc(or_tangent *t, int np);
+void or_tangent_free(or_tangent *t);
+void or_tangent_set(const or_domain *d, const or_state *st, or_tangent *t, const double *dro,
+                    const double *drou, const double *drov, const double *dcxx,
+                    const double *dcxy, const double *dcyy);
+void or_calc_all_rhs_di_tlm(const or_domain *d, const or_phys *p, const or_state *st,
+                            const or_tangent *t, double *const dr[6]);
+int  or_step_rk3_di_tlm(const or_domain *d, const or_phys *p, or_state *st, or_tangent *t);
+double or_loss_trace(const or_domain *d, const or_state *st);
+int  or_rollout_tlm(const or_domain *d, const or_phys *p, or_state *st, or_tangent *t,
+                    int nsteps, double *loss, double *dloss);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,733 @@
+// k_adjoint.hip -- hand-written reverse mode of the direct-form (DI) RK3 step
+// at a fixed time step: the vector-Jacobian product of calc_all_rhs
+// (rhs_di.F90:47-476 with the full-gradient reading of calc_rhs_c), of the
+// four RK stages and registers (step.F90:281-409), of the hyperviscous filter
+// (calc_filtered_var, derivatives.F90:360-414) and of the mass correction
+// (rhs_di.F90:667-720), and checkpointed rollouts of a trace loss.
+//
+// Stencil transposes without atomics: every forward link (i -> j, weights w)
+// is stored once more in a transposed SELL-64 layout keyed by j, so the
+// adjoint of "D_i = sum_k w_ik G_j - G_i sum_k w_ik" is a gather
+//   Gbar_j += sum_{(i,k): j_k = i's link} w_ik Dbar_i
+// over the transposed list of j in a fixed order (ascending i): deterministic.
+//
+// Per RK stage the adjoint is two launches:
+//   k_adj_point  pointwise: Rbar_i and the stage tape (own gather record G_i and
+//                derivative values D_i written by the forward stage) -> the
+//                adjoints of the 14 derivative values Dbar_i (+3 Laplacians of
+//                c with Mdiff) and the own-node adjoint of G_i
+//   k_adj_trans  transposed gather of Dbar, G -> U (u = rou/ro), and the RK
+//                register recurrences that give the previous stage's Rbar
+// and per step: the mass-correction sum, its pointwise correction and the
+// transposed filter gather.  Rollouts keep the state every `checkpoint`
+// steps; the adjoint recomputes one checkpoint interval with the tape on and
+// walks it backwards.
+#include <string.h>
+
+#include "sv_internal.h"
+
+namespace {
+
+constexpr int BS = SV_BLOCK;
+
+__device__ __forceinline__ void ld6(const double *p, double f[6]) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  const double2 a = q[0], b = q[1], c = q[2];
+  f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = c.x; f[5] = c.y;
+}
+
+__device__ __forceinline__ void st6(double *p, const double f[6]) {
+  double2 *q = reinterpret_cast<double2 *>(p);
+  q[0] = double2{f[0], f[1]};
+  q[1] = double2{f[2], f[3]};
+  q[2] = double2{f[4], f[5]};
+}
+
+struct NodeRef {
+  int sg, i, ii;
+  bool act;
+};
+
+__device__ __forceinline__ NodeRef node_ref(const int *blk_seg, const sv_segk *seg, int bid) {
+  NodeRef r;
+  r.sg = blk_seg[bid];
+  const sv_segk &S = seg[r.sg];
+  const int li = (bid - S.blk0) * BS + (int)threadIdx.x;
+  r.act = li < S.npfb;
+  r.i = S.base + li;
+  r.ii = r.act ? r.i : S.base;
+  return r;
+}
+
+// ---- pointwise adjoint of the DI right-hand side ------------------------
+// Dbar record [ntot][16]: rx ry | ux uy lu | vx vy lv | cxx_x cxx_y | cxy_x
+// cxy_y | cyy_x cyy_y | 0 0; with Mdiff a second record [ntot][4]: the
+// Laplacian adjoints of cxx, cxy, cyy.
+struct PointArgs {
+  const int *blk_seg;
+  const sv_segk *seg;
+  const double *tG, *tD;     // stage tape
+  const double *Rbar;        // [ntot][8]
+  const double *gsx, *gsy, *lsum;
+  double *dbar, *dbarm, *own;
+};
+
+template <bool MDIFF>
+__global__ __launch_bounds__(BS) void k_adj_point(PointArgs a) {
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
+  if (!nr.act) return;
+  const int i = nr.i;
+  const sv_physk &P = a.seg[nr.sg].pk;
+  double g[6], rb[6];
+  ld6(a.tG + (size_t)i * SV_US, g);
+  ld6(a.Rbar + (size_t)i * SV_US, rb);
+  const double ro = g[0], u = g[1], v = g[2], cxx = g[3], cxy = g[4], cyy = g[5];
+  const double *D = a.tD + (size_t)i * SV_TAPE_D;
+  const double rx = D[0], ry = D[1], ux = D[2], uy = D[3], vx = D[5], vy = D[6];
+  const double gxx_x = D[8], gxx_y = D[9], gxy_x = D[10], gxy_y = D[11];
+  const double gyy_x = D[12], gyy_y = D[13];
+  const double Ab = rb[0], Bu = rb[1], Bv = rb[2], Xb = rb[3], Yb = rb[4], Zb = rb[5];
+  // forward intermediates the adjoint needs
+  const double r_ro = -ro * (ux + vy) - (u * rx + v * ry);
+  const double fr = -(1.0 - P.eps * 2.0 + P.eps * (cxx + cyy + 1.0)) / P.lambda;
+  double bro = 0, bu = 0, bv = 0, bcxx = 0, bcxy = 0, bcyy = 0;
+  double brx = 0, bry = 0, bux = 0, buy = 0, blu = 0, bvx = 0, bvy = 0, blv = 0;
+  double bgxx_x = 0, bgxx_y = 0, bgxy_x = 0, bgxy_y = 0, bgyy_x = 0, bgyy_y = 0;
+  // r_rou = -tsu - csq*rx + ro*bfx + fx + cs*lu + cp*(cxx_x + cxy_y)
+  const double btsu = -Bu, btsv = -Bv;
+  brx += -P.csq * Bu;
+  bry += -P.csq * Bv;
+  bro += P.bfx * Bu + P.bfy * Bv;
+  blu += P.coef_solvent * Bu;
+  blv += P.coef_solvent * Bv;
+  bgxx_x += P.coef_polymeric * Bu;
+  bgxy_y += P.coef_polymeric * Bu;
+  bgxy_x += P.coef_polymeric * Bv;
+  bgyy_y += P.coef_polymeric * Bv;
+  // tsu = ro*(u*ux + v*uy) - u*r_ro ; tsv = ro*(u*vx + v*vy) - v*r_ro
+  bro += (u * ux + v * uy) * btsu + (u * vx + v * vy) * btsv;
+  bu += ro * ux * btsu - r_ro * btsu + ro * vx * btsv;
+  bv += ro * uy * btsu + ro * vy * btsv - r_ro * btsv;
+  bux += ro * u * btsu;
+  buy += ro * v * btsu;
+  bvx += ro * u * btsv;
+  bvy += ro * v * btsv;
+  const double Ar = Ab - u * btsu - v * btsv;
+  // r_ro = -ro*(ux + vy) - (u*rx + v*ry)
+  bro += -(ux + vy) * Ar;
+  bux += -ro * Ar;
+  bvy += -ro * Ar;
+  bu += -rx * Ar;
+  brx += -u * Ar;
+  bv += -ry * Ar;
+  bry += -v * Ar;
+  // advection of c
+  bu += -(gxx_x * Xb + gxy_x * Yb + gyy_x * Zb);
+  bv += -(gxx_y * Xb + gxy_y * Yb + gyy_y * Zb);
+  bgxx_x += -u * Xb; bgxx_y += -v * Xb;
+  bgxy_x += -u * Yb; bgxy_y += -v * Yb;
+  bgyy_x += -u * Zb; bgyy_y += -v * Zb;
+  // upper-convected terms
+  bux += 2.0 * cxx * Xb + cxy * Yb;
+  buy += 2.0 * cxy * Xb + cyy * Yb;
+  bvx += cxx * Yb + 2.0 * cxy * Zb;
+  bvy += cxy * Yb + 2.0 * cyy * Zb;
+  bcxx += 2.0 * ux * Xb + vx * Yb;
+  bcxy += 2.0 * uy * Xb + (ux + vy) * Yb + 2.0 * vx * Zb;
+  bcyy += uy * Yb + 2.0 * vy * Zb;
+  // relaxation fr*(c - I), fr = -(1 - 2 eps + eps*(cxx + cyy + 1))/lambda
+  bcxx += fr * Xb;
+  bcxy += fr * Yb;
+  bcyy += fr * Zb;
+  const double bfr = (cxx - 1.0) * Xb + cxy * Yb + (cyy - 1.0) * Zb;
+  const double dfr = -P.eps / P.lambda * bfr;
+  bcxx += dfr;
+  bcyy += dfr;
+  double blxx = 0, blxy = 0, blyy = 0;
+  if (MDIFF) { blxx = P.Mdiff * Xb; blxy = P.Mdiff * Yb; blyy = P.Mdiff * Zb; }
+  // D_i = sum_k w G_j - G_i * sum: own-node share
+  const double sx = a.gsx[i], sy = a.gsy[i], sl = a.lsum[i];
+  double o[6];
+  o[0] = bro - (sx * brx + sy * bry);
+  o[1] = bu - (sx * bux + sy * buy + sl * blu);
+  o[2] = bv - (sx * bvx + sy * bvy + sl * blv);
+  o[3] = bcxx - (sx * bgxx_x + sy * bgxx_y + sl * blxx);
+  o[4] = bcxy - (sx * bgxy_x + sy * bgxy_y + sl * blxy);
+  o[5] = bcyy - (sx * bgyy_x + sy * bgyy_y + sl * blyy);
+  st6(a.own + (size_t)i * SV_US, o);
+  double2 *T = reinterpret_cast<double2 *>(a.dbar + (size_t)i * SV_TAPE_D);
+  T[0] = double2{brx, bry}; T[1] = double2{bux, buy}; T[2] = double2{blu, bvx};
+  T[3] = double2{bvy, blv}; T[4] = double2{bgxx_x, bgxx_y}; T[5] = double2{bgxy_x, bgxy_y};
+  T[6] = double2{bgyy_x, bgyy_y}; T[7] = double2{0.0, 0.0};
+  if (MDIFF) {
+    double2 *M = reinterpret_cast<double2 *>(a.dbarm + (size_t)i * 4);
+    M[0] = double2{blxx, blxy};
+    M[1] = double2{blyy, 0.0};
+  }
+}
+
+// ---- transposed gather + G->U + RK register recurrences ------------------
+struct TransArgs {
+  const int *blk_seg;
+  const sv_segk *seg;
+  const int64_t *toff;       // transposed SELL
+  const int *tcnt, *tcol;
+  const double *twx, *twy, *twl;
+  const double *dbar, *dbarm, *own;
+  const double *tG;          // stage tape (ro, u, v of node j)
+  const sv_ctrl *ctrl;       // RK coefficients (dt included)
+  double *Ubar;              // STAGE 1 / 0: output
+  double *Rbar;              // STAGE 2..4: next (earlier) stage's Rbar
+  double *regbar;            // STAGE 2..4: in/out
+  double seed;               // STAGE 1: loss seed added to cxx and cyy
+  double *psum;              // STAGE 1: per-block sum of Ubar_ro (mass correction)
+};
+
+// STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar)
+template <int STAGE, bool MDIFF>
+__global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
+  __shared__ double red[BS];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int j = nr.i, jj = nr.ii;
+  const int64_t b = a.toff[jj >> 6] + (jj & 63);
+  const int K = nr.act ? a.tcnt[jj] : 0;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < K; k++) {
+    const int64_t e = b + (int64_t)k * SV_WAVE;
+    const int i = a.tcol[e];
+    const double wx = a.twx[e], wy = a.twy[e], wl = a.twl[e];
+    const double2 *d = reinterpret_cast<const double2 *>(a.dbar + (size_t)i * SV_TAPE_D);
+    const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
+    acc[0] += wx * d0.x + wy * d0.y;
+    acc[1] += wx * d1.x + wy * d1.y + wl * d2.x;
+    acc[2] += wx * d2.y + wy * d3.x + wl * d3.y;
+    acc[3] += wx * d4.x + wy * d4.y;
+    acc[4] += wx * d5.x + wy * d5.y;
+    acc[5] += wx * d6.x + wy * d6.y;
+    if (MDIFF) {
+      const double2 *m = reinterpret_cast<const double2 *>(a.dbarm + (size_t)i * 4);
+      const double2 m0 = m[0], m1 = m[1];
+      acc[3] += wl * m0.x;
+      acc[4] += wl * m0.y;
+      acc[5] += wl * m1.x;
+    }
+  }
+  double part = 0.0;
+  if (nr.act) {
+    double o[6], g[6];
+    ld6(a.own + (size_t)j * SV_US, o);
+    ld6(a.tG + (size_t)j * SV_US, g);
+    double G[6];
+#pragma unroll
+    for (int f = 0; f < 6; f++) G[f] = o[f] + acc[f];
+    // G = (ro, rou/ro, rov/ro, c) -> U
+    const double ooro = 1.0 / g[0];
+    double Ub[6];
+    Ub[0] = G[0] - (G[1] * g[1] + G[2] * g[2]) * ooro;
+    Ub[1] = G[1] * ooro;
+    Ub[2] = G[2] * ooro;
+    Ub[3] = G[3]; Ub[4] = G[4]; Ub[5] = G[5];
+    const sv_ctrl *C = a.ctrl + nr.sg;
+    if (STAGE == 0) {
+      st6(a.Ubar + (size_t)j * SV_US, Ub);
+    } else if (STAGE == 1) {
+      double rg[6];
+      ld6(a.regbar + (size_t)j * SV_US, rg);
+#pragma unroll
+      for (int f = 0; f < 6; f++) Ub[f] += rg[f];
+      Ub[3] += a.seed;
+      Ub[5] += a.seed;
+      st6(a.Ubar + (size_t)j * SV_US, Ub);
+      part = Ub[0];
+    } else {
+      // U_k = reg_{k-2} + a_{k-1} R_{k-1};  reg_{k-1} = reg_{k-2} + b_{k-1} R_{k-1}
+      const double ra = C->RKa[STAGE - 2], rbk = C->RKb[STAGE - 2];
+      double rg[6], Rn[6];
+      ld6(a.regbar + (size_t)j * SV_US, rg);
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        Rn[f] = ra * Ub[f] + rbk * rg[f];
+        rg[f] = Ub[f] + rg[f];
+      }
+      st6(a.regbar + (size_t)j * SV_US, rg);
+      st6(a.Rbar + (size_t)j * SV_US, Rn);
+    }
+  }
+  if (STAGE == 1) {
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.psum[bid] = red[0];
+  }
+}
+
+// per-block sums of Ubar_ro (the start of a backward sweep)
+__global__ __launch_bounds__(BS) void k_adj_sum(const int *blk_seg, const sv_segk *seg,
+                                                const double *Ubar, double *psum) {
+  __shared__ double red[BS];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(blk_seg, seg, bid);
+  red[threadIdx.x] = nr.act ? Ubar[(size_t)nr.i * SV_US] : 0.0;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) psum[bid] = red[0];
+}
+
+// mass-correction adjoint: sbar[seg] = sum_j Ubar_ro,j / sum_j vol_j (one block
+// per segment), then Ubar_ro,i -= vol_i * sbar
+__global__ __launch_bounds__(1024) void k_adj_sum_reduce(const sv_segk *seg, const double *psum,
+                                                         double *sbar) {
+  __shared__ double s1[1024];
+  const sv_segk &S = seg[blockIdx.x];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += 1024) a += psum[S.blk0 + k];
+  s1[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) s1[threadIdx.x] += s1[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sbar[blockIdx.x] = s1[0] / S.volsum;
+}
+
+__global__ __launch_bounds__(BS) void k_adj_mcorr(const int *blk_seg, const sv_segk *seg,
+                                                  const double *sbar, const double *vol,
+                                                  double *Ubar) {
+  const NodeRef nr = node_ref(blk_seg, seg, sv_xcd_block());
+  if (!nr.act) return;
+  Ubar[(size_t)nr.i * SV_US] -= vol[nr.i] * sbar[nr.sg];
+}
+
+// filter adjoint: Ustar_j = Uf_j - hsum_j Uf_j + sum_t wh_t Uf_{i_t}; then
+// regbar_3 = Ustar, Rbar_4 = b_4 Ustar
+__global__ __launch_bounds__(BS) void k_adj_filter(const int *blk_seg, const sv_segk *seg,
+                                                   const int64_t *toff, const int *tcnt,
+                                                   const int *tcol, const double *twh,
+                                                   const double *hsum, const double *Ubar,
+                                                   const sv_ctrl *ctrl, double *regbar,
+                                                   double *Rbar) {
+  const NodeRef nr = node_ref(blk_seg, seg, sv_xcd_block());
+  const int j = nr.i, jj = nr.ii;
+  const int64_t b = toff[jj >> 6] + (jj & 63);
+  const int K = nr.act ? tcnt[jj] : 0;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < K; k++) {
+    const int64_t e = b + (int64_t)k * SV_WAVE;
+    const int i = tcol[e];
+    const double w = twh[e];
+    double f[6];
+    ld6(Ubar + (size_t)i * SV_US, f);
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[q] += w * f[q];
+  }
+  if (!nr.act) return;
+  double f[6], out[6], rb[6];
+  ld6(Ubar + (size_t)j * SV_US, f);
+  const double hs = hsum[j];
+  const double b4 = ctrl[nr.sg].RKb[3];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    out[q] = f[q] - hs * f[q] + acc[q];
+    rb[q] = b4 * out[q];
+  }
+  st6(regbar + (size_t)j * SV_US, out);
+  st6(Rbar + (size_t)j * SV_US, rb);
+}
+
+// trace loss of the state: per-block sums of (cxx + cyy - 2), then one block
+// per segment adds mean/T into lacc[seg]
+__global__ __launch_bounds__(BS) void k_loss_part(const int *blk_seg, const sv_segk *seg,
+                                                  const double *U, double *part) {
+  __shared__ double red[BS];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(blk_seg, seg, bid);
+  double v = 0.0;
+  if (nr.act) {
+    const double *r = U + (size_t)nr.i * SV_US;
+    v = r[3] + r[5] - 2.0;
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[bid] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_loss_acc(const sv_segk *seg, const double *part,
+                                                   double *lacc) {
+  __shared__ double s1[1024];
+  const sv_segk &S = seg[blockIdx.x];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += 1024) a += part[S.blk0 + k];
+  s1[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) s1[threadIdx.x] += s1[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lacc[blockIdx.x] += s1[0] / S.npfb;
+}
+
+__global__ void k_zero(double *p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+}  // namespace
+
+// ---- host side ------------------------------------------------------------
+struct sv_adj {
+  int every = 20;            // checkpoint interval (steps)
+  // transposed stencil (SELL-64 over j)
+  int64_t *toff = nullptr;
+  int *tcnt = nullptr, *tcol = nullptr;
+  double *twx = nullptr, *twy = nullptr, *twl = nullptr, *twh = nullptr;
+  int64_t tnnz = 0;
+  // adjoint records [ntot][8] and stencil-adjoint records
+  double *Ubar = nullptr, *Rbar = nullptr, *regbar = nullptr, *own = nullptr;
+  double *dbar = nullptr, *dbarm = nullptr;
+  double *psum = nullptr, *sbar = nullptr, *lpart = nullptr, *lacc = nullptr;
+  // rollout: checkpoints of U records, the end state, the tape of one interval
+  int T = 0;                 // steps of the last forward rollout
+  std::vector<double *> ckpt;
+  std::vector<sv_ctrl> ckpt_ctrl;
+  double *Uend = nullptr;
+  sv_ctrl ctrl_end{};
+  std::vector<double *> tape;    // [every][8]
+  double t_fwd = 0, t_bwd = 0;   // ms of the last rollout / vjp
+};
+
+void sv_adj_free(sv_ctx *c) {
+  delete c->adj;
+  c->adj = nullptr;
+}
+
+static int adj_alloc(sv_ctx *c, double **p, size_t n) {
+  return sv_alloc(c, (void **)p, sizeof(double) * (n ? n : 1));
+}
+
+static int zero_dev(sv_ctx *c, double *p, int64_t n) {
+  if (n <= 0) return 0;
+  k_zero<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// Transposed stencil of the folded SELL lists: for each j the entries (i, k)
+// with scol = j, in ascending i, stage weights zero beyond ij_count_small.
+static int build_transpose(sv_ctx *c, sv_adj *A) {
+  const int n = c->npfb, ns = c->nslices;
+  std::vector<int> cnt(n), cs(n), scol(c->sell_nnz);
+  std::vector<int64_t> off(ns);
+  std::vector<double> wx(c->sell_nnz), wy(c->sell_nnz), wl(c->sell_nnz), wh(c->sell_nnz);
+  SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(cs.data(), c->cnt_small, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(off.data(), c->sell_off, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+  const size_t L = sizeof(double) * c->sell_nnz;
+  SV_HIP(c, hipMemcpy(scol.data(), c->scol, sizeof(int) * c->sell_nnz, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wx.data(), c->swx, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wy.data(), c->swy, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wl.data(), c->swl, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wh.data(), c->swh, L, hipMemcpyDeviceToHost));
+  std::vector<int> tc(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < cnt[i]; k++) {
+      const int j = scol[off[i >> 6] + (int64_t)k * SV_WAVE + (i & 63)];
+      if (j < 0 || j >= n) return sv_fail(c, SV_E_STATE, "stencil column out of range");
+      tc[j]++;
+    }
+  const int nsl = (n + SV_WAVE - 1) / SV_WAVE;
+  std::vector<int64_t> toff(nsl + 1);
+  int64_t tot = 0;
+  for (int s = 0; s < nsl; s++) {
+    int w = 0;
+    for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) w = std::max(w, tc[s * SV_WAVE + l]);
+    toff[s] = tot;
+    tot += (int64_t)w * SV_WAVE;
+  }
+  toff[nsl] = tot;
+  A->tnnz = tot;
+  std::vector<int> tcol(tot);
+  std::vector<double> tx(tot, 0.0), ty(tot, 0.0), tl(tot, 0.0), th(tot, 0.0);
+  for (int s = 0; s < nsl; s++)       // padding: own node, zero weights
+    for (int64_t e = toff[s]; e < toff[s + 1]; e++)
+      tcol[e] = std::min(s * SV_WAVE + (int)((e - toff[s]) % SV_WAVE), n - 1);
+  std::vector<int> fill(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < cnt[i]; k++) {
+      const int64_t e = off[i >> 6] + (int64_t)k * SV_WAVE + (i & 63);
+      const int j = scol[e];
+      const int64_t d = toff[j >> 6] + (int64_t)(fill[j]++) * SV_WAVE + (j & 63);
+      tcol[d] = i;
+      const bool small = k < cs[i];
+      tx[d] = small ? wx[e] : 0.0;
+      ty[d] = small ? wy[e] : 0.0;
+      tl[d] = small ? wl[e] : 0.0;
+      th[d] = wh[e];
+    }
+  if (sv_alloc(c, (void **)&A->toff, sizeof(int64_t) * (nsl + 1)) ||
+      sv_alloc(c, (void **)&A->tcnt, sizeof(int) * n) ||
+      sv_alloc(c, (void **)&A->tcol, sizeof(int) * (tot + 1)) || adj_alloc(c, &A->twx, tot) ||
+      adj_alloc(c, &A->twy, tot) || adj_alloc(c, &A->twl, tot) || adj_alloc(c, &A->twh, tot))
+    return SV_E_NOMEM;
+  SV_HIP(c, hipMemcpy(A->toff, toff.data(), sizeof(int64_t) * (nsl + 1), hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->tcnt, tc.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->tcol, tcol.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->twx, tx.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->twy, ty.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->twl, tl.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(A->twh, th.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int ensure_tape(sv_ctx *c, sv_adj *A) {
+  const size_t N = (size_t)c->ntot;
+  while ((int)A->tape.size() < 8 * A->every) {
+    double *p = nullptr;
+    const size_t per = ((A->tape.size() & 1) ? SV_TAPE_D : SV_US) * N;
+    if (adj_alloc(c, &p, per)) return SV_E_NOMEM;
+    A->tape.push_back(p);
+  }
+  return 0;
+}
+
+// Jt Rbar at one tape slot: pointwise + transposed gather for the given stage
+template <int STAGE>
+static int adj_stage(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
+  const int k = (STAGE == 0 ? 1 : STAGE) - 1;
+  PointArgs p{c->blk_seg, c->segk, tape[2 * k], tape[2 * k + 1], A->Rbar, c->gsx, c->gsy,
+              c->lsum, A->dbar, A->dbarm, A->own};
+  if (c->any_mdiff) k_adj_point<true><<<c->nblk, BS, 0, c->stream>>>(p);
+  else k_adj_point<false><<<c->nblk, BS, 0, c->stream>>>(p);
+  TransArgs t{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twx, A->twy, A->twl,
+              A->dbar, A->dbarm, A->own, tape[2 * k], c->ctrl, A->Ubar, A->Rbar,
+              A->regbar, seed, A->psum};
+  if (c->any_mdiff) k_adj_trans<STAGE, true><<<c->nblk, BS, 0, c->stream>>>(t);
+  else k_adj_trans<STAGE, false><<<c->nblk, BS, 0, c->stream>>>(t);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// adjoint of one step (tape slot s): Ubar (after the step, with its mass sums
+// in psum) -> Ubar (before the step) + seed on cxx, cyy
+static int adj_step(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
+  k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, A->sbar);
+  k_adj_mcorr<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->sbar, c->vol, A->Ubar);
+  k_adj_filter<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol,
+                                              A->twh, c->hsum, A->Ubar, c->ctrl, A->regbar,
+                                              A->Rbar);
+  SV_HIP(c, hipGetLastError());
+  if (int rc = adj_stage<4>(c, A, tape, 0.0)) return rc;
+  if (int rc = adj_stage<3>(c, A, tape, 0.0)) return rc;
+  if (int rc = adj_stage<2>(c, A, tape, 0.0)) return rc;
+  return adj_stage<1>(c, A, tape, seed);
+}
+
+static int loss_accumulate(sv_ctx *c, sv_adj *A) {
+  k_loss_part<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, c->U, A->lpart);
+  k_loss_acc<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->lpart, A->lacc);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+static int upload_records(sv_ctx *c, double *dst, const double *const f[6]) {
+  const int n = c->npfb;
+  std::vector<double> rec((size_t)SV_US * c->ntot, 0.0);
+  for (int q = 0; q < 6; q++)
+    if (f && f[q])
+      for (int i = 0; i < n; i++) rec[(size_t)SV_US * i + q] = f[q][i];
+  SV_HIP(c, hipMemcpy(dst, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int download_records(sv_ctx *c, const double *src, double *const f[6]) {
+  for (int q = 0; q < 6; q++)
+    if (f[q])
+      SV_HIP(c, hipMemcpy2D(f[q], sizeof(double), src + q, sizeof(double) * SV_US,
+                            sizeof(double), c->npfb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+#define SV_ADJ_CHECK(c, cond, ...)                            \
+  do {                                                        \
+    if (!(cond)) return sv_fail((c), SV_E_ARG, __VA_ARGS__);  \
+  } while (0)
+
+extern "C" {
+
+int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
+  SV_ADJ_CHECK(c, c && c->have_state, "sv_set_initial must come first");
+  SV_ADJ_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  SV_ADJ_CHECK(c, c->cfg.form == SV_FORM_DI, "the adjoint is implemented for the DI form");
+  SV_ADJ_CHECK(c, checkpoint_every >= 1 && checkpoint_every <= 1000,
+               "checkpoint interval must be in [1, 1000]");
+  SV_HIP(c, hipSetDevice(c->device));
+  sv_ctrl h{};
+  SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+  SV_ADJ_CHECK(c, h.dt_mode == SV_DT_FIXED,
+               "the adjoint needs a fixed time step (sv_set_control with SV_DT_FIXED)");
+  if (!c->adj) {
+    sv_adj *A = new sv_adj();
+    c->adj = A;
+    const size_t R = (size_t)SV_US * c->ntot;
+    if (int rc = build_transpose(c, A)) return rc;
+    if (adj_alloc(c, &A->Ubar, R) || adj_alloc(c, &A->Rbar, R) || adj_alloc(c, &A->regbar, R) ||
+        adj_alloc(c, &A->own, R) || adj_alloc(c, &A->dbar, (size_t)SV_TAPE_D * c->ntot) ||
+        adj_alloc(c, &A->dbarm, (size_t)4 * c->ntot) || adj_alloc(c, &A->psum, c->nblk) ||
+        adj_alloc(c, &A->sbar, c->nseg) || adj_alloc(c, &A->lpart, c->nblk) ||
+        adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R))
+      return SV_E_NOMEM;
+  }
+  c->adj->every = checkpoint_every;
+  return 0;
+}
+
+// calc_all_rhs vector-Jacobian product at the current state: ubar = J^T rbar
+int sv_rhs_vjp(sv_ctx *c, const double *const rbar[6], double *const ubar[6]) {
+  SV_ADJ_CHECK(c, c && c->adj, "sv_adj_setup must come first");
+  SV_ADJ_CHECK(c, rbar && ubar, "null argument");
+  sv_adj *A = c->adj;
+  SV_HIP(c, hipSetDevice(c->device));
+  if (int rc = ensure_tape(c, A)) return rc;
+  double *scratch = nullptr;
+  SV_HIP(c, hipMalloc(&scratch, sizeof(double) * 6 * (size_t)c->npfb));
+  int rc = sv_dev_rhs(c, 1, scratch, A->tape[0], A->tape[1]);
+  (void)hipFree(scratch);
+  if (rc) return rc;
+  if (int r2 = upload_records(c, A->Rbar, rbar)) return r2;
+  if (int r2 = adj_stage<0>(c, A, A->tape.data(), 0.0)) return r2;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  return download_records(c, A->Ubar, ubar);
+}
+
+// nsteps of the fixed-dt loop from the current state, storing checkpoints;
+// *loss = (1/nsteps) sum_t mean_i (cxx + cyy - 2) after each step
+int sv_rollout_forward(sv_ctx *c, int32_t nsteps, double *loss) {
+  SV_ADJ_CHECK(c, c && c->adj, "sv_adj_setup must come first");
+  SV_ADJ_CHECK(c, nsteps >= 1, "nsteps must be >= 1");
+  sv_adj *A = c->adj;
+  SV_HIP(c, hipSetDevice(c->device));
+  {
+    sv_ctrl h{};
+    SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));
+    SV_ADJ_CHECK(c, h.dt_mode == SV_DT_FIXED, "the adjoint needs a fixed time step");
+  }
+  const int nck = (nsteps + A->every - 1) / A->every;
+  const size_t R = (size_t)SV_US * c->ntot;
+  while ((int)A->ckpt.size() < nck) {
+    double *p = nullptr;
+    if (adj_alloc(c, &p, R)) return SV_E_NOMEM;
+    A->ckpt.push_back(p);
+  }
+  A->ckpt_ctrl.assign(nck, sv_ctrl{});
+  if (int rc = zero_dev(c, A->lacc, c->nseg)) return rc;
+  hipEvent_t e0, e1;
+  SV_HIP(c, hipEventCreate(&e0));
+  SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventRecord(e0, c->stream));
+  for (int s = 0; s < nsteps; s++) {
+    if (s % A->every == 0) {
+      const int k = s / A->every;
+      SV_HIP(c, hipMemcpyAsync(A->ckpt[k], c->U, sizeof(double) * R, hipMemcpyDeviceToDevice,
+                               c->stream));
+      SV_HIP(c, hipMemcpyAsync(&A->ckpt_ctrl[k], c->ctrl, sizeof(sv_ctrl),
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+    if (int rc = sv_dev_step(c, nullptr)) return rc;
+    if (int rc = loss_accumulate(c, A)) return rc;
+  }
+  SV_HIP(c, hipEventRecord(e1, c->stream));
+  SV_HIP(c, hipMemcpyAsync(A->Uend, c->U, sizeof(double) * R, hipMemcpyDeviceToDevice,
+                           c->stream));
+  SV_HIP(c, hipMemcpyAsync(&A->ctrl_end, c->ctrl, sizeof(sv_ctrl), hipMemcpyDeviceToHost,
+                           c->stream));
+  double acc = 0.0;
+  SV_HIP(c, hipMemcpyAsync(&acc, A->lacc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  A->t_fwd = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  A->T = nsteps;
+  if (loss) *loss = acc / nsteps;
+  return 0;
+}
+
+// Gradient of  loss_bar * L + <ubar_final, U_T>  with respect to the state
+// (ro, rou, rov, cxx, cxy, cyy) at the start of the last sv_rollout_forward.
+// ubar_final may be NULL (or hold NULL fields).  The context's state is left
+// at the end of the rollout.
+int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6],
+                   double *const grad0[6]) {
+  SV_ADJ_CHECK(c, c && c->adj && c->adj->T > 0, "sv_rollout_forward must come first");
+  SV_ADJ_CHECK(c, grad0, "null output");
+  sv_adj *A = c->adj;
+  SV_HIP(c, hipSetDevice(c->device));
+  if (int rc = ensure_tape(c, A)) return rc;
+  const int T = A->T, E = A->every;
+  const double seed = loss_bar / ((double)T * c->npfb);
+  const size_t R = (size_t)SV_US * c->ntot;
+  if (int rc = upload_records(c, A->Ubar, ubar_final)) return rc;
+  // seed of the loss term at step T
+  {
+    std::vector<double> u((size_t)SV_US * c->ntot);
+    SV_HIP(c, hipMemcpy(u.data(), A->Ubar, sizeof(double) * u.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->npfb; i++) {
+      u[(size_t)SV_US * i + 3] += seed;
+      u[(size_t)SV_US * i + 5] += seed;
+    }
+    SV_HIP(c, hipMemcpy(A->Ubar, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice));
+  }
+  hipEvent_t e0, e1;
+  SV_HIP(c, hipEventCreate(&e0));
+  SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventRecord(e0, c->stream));
+  k_adj_sum<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
+  SV_HIP(c, hipGetLastError());
+  const int nck = (T + E - 1) / E;
+  for (int k = nck - 1; k >= 0; k--) {
+    const int s0 = k * E, s1 = std::min(T, s0 + E);
+    // recompute the interval from its checkpoint with the tape on
+    SV_HIP(c, hipMemcpyAsync(c->U, A->ckpt[k], sizeof(double) * R, hipMemcpyDeviceToDevice,
+                             c->stream));
+    SV_HIP(c, hipMemcpyAsync(c->ctrl, &A->ckpt_ctrl[k], sizeof(sv_ctrl), hipMemcpyHostToDevice,
+                             c->stream));
+    if (int rc = sv_dev_init_state(c)) return rc;
+    for (int s = s0; s < s1; s++)
+      if (int rc = sv_dev_step_tape(c, A->tape.data() + 8 * (s - s0))) return rc;
+    for (int s = s1 - 1; s >= s0; s--)
+      if (int rc = adj_step(c, A, A->tape.data() + 8 * (s - s0), s >= 1 ? seed : 0.0))
+        return rc;
+  }
+  SV_HIP(c, hipEventRecord(e1, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  A->t_bwd = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (int rc = download_records(c, A->Ubar, grad0)) return rc;
+  // leave the state at the end of the rollout
+  SV_HIP(c, hipMemcpy(c->U, A->Uend, sizeof(double) * R, hipMemcpyDeviceToDevice));
+  SV_HIP(c, hipMemcpy(c->ctrl, &A->ctrl_end, sizeof(sv_ctrl), hipMemcpyHostToDevice));
+  return sv_dev_init_state(c);
+}
+
+int sv_adj_timing(sv_ctx *c, double *fwd_ms, double *vjp_ms) {
+  SV_ADJ_CHECK(c, c && c->adj, "sv_adj_setup must come first");
+  if (fwd_ms) *fwd_ms = c->adj->t_fwd;
+  if (vjp_ms) *vjp_ms = c->adj->t_bwd;
+  return 0;
+}
+
+}  // extern "C"

@@ -30,6 +30,26 @@ namespace {
 
 constexpr int BS = SV_BLOCK;   // threads per block of the node kernels
 
+// Gather-loop tuning (compile time): links processed per iteration, and
+// whether the streamed stencil arrays (columns, weights; read once per sweep)
+// are loaded non-temporally so they do not displace the gathered records in
+// the caches.
+#ifndef SV_GATHER_UNROLL
+#define SV_GATHER_UNROLL 2
+#endif
+#ifndef SV_STREAM_NT
+#define SV_STREAM_NT 0
+#endif
+
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+#if SV_STREAM_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
 // The node handled by this thread: segment of the block, concatenated index i,
 // a safe index ii (= i, or the segment's first node for an idle lane).
 struct NodeRef {
@@ -205,69 +225,32 @@ struct StageArgs {
   const sv_ctrl *ctrl;  // [nseg]
   double *partial;      // stage 4: per-block max error
   int rk1;              // STAGE 0: reference stage counter is 1
+  // DI only, nullable: the stage's own gather record [ntot][8] and its
+  // derivative values [ntot][16] (rx ry ux uy lu vx vy lv, cxx_x cxx_y, cxy_x
+  // cxy_y, cyy_x cyy_y, 0 0), the tape the adjoint sweep reads
+  double *tapeG, *tapeD;
 };
 
-// STAGE 1..4: RK stages; STAGE 0: write the right-hand side into rhs_out.
+// The neighbour sums of one node in a stage: gradients and Laplacians before
+// the own-node term (sum_k w_k f_j), as named in the right-hand side.
+struct Sums {
+  double rx, ry, ux, uy, vx, vy, lu, lv, axx, ayx, axy, ayy, bxy, byy, cxx_l, cxy_l, cyy_l, q1x, q1y, q2x, q2y, q3x, q3y;
+};
+
+// Pointwise part of a stage for one node: the right-hand side from the
+// gathered sums and the own record, then the RK update.  Returns the node's
+// error norm contribution (stage 4).
 template <int FORM, int STAGE, bool MDIFF>
-__global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
-  __shared__ double red[BS / SV_WAVE];
-  const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
-  const int i = nr.i, ii = nr.ii;
-  const bool act = nr.act;
-  const int64_t b = a.off[ii >> 6] + (ii & 63);
-  const int K = act ? a.kcnt[ii] : 0;
-  double rx = 0, ry = 0, ux = 0, uy = 0, vx = 0, vy = 0, lu = 0, lv = 0;
-  double axx = 0, ayx = 0, axy = 0, ayy = 0, bxy = 0, byy = 0;
-  double cxx_l = 0, cxy_l = 0, cyy_l = 0;
-  double q1x = 0, q1y = 0, q2x = 0, q2y = 0, q3x = 0, q3y = 0;
-  auto accum = [&](const Rec9 &r, double wxk, double wyk, double wlk) {
-    rx += r.ro * wxk; ry += r.ro * wyk;
-    ux += r.u * wxk; uy += r.u * wyk; lu += r.u * wlk;
-    vx += r.v * wxk; vy += r.v * wyk; lv += r.v * wlk;
-    if (FORM == SV_FORM_LC) {
-      axx += r.cxx * wxk;                         // calc_gradient_only(1,cxx)
-      axy += r.cxy * wxk; ayy += r.cxy * wyk;     // calc_gradient(cxy)
-      byy += r.cyy * wyk;                         // calc_gradient_only(2,cyy)
-      q1x += r.pxx * wxk; q1y += r.pxx * wyk;
-      q2x += r.pxy * wxk; q2y += r.pxy * wyk;
-      q3x += r.pyy * wxk; q3y += r.pyy * wyk;
-    } else {
-      axx += r.cxx * wxk; ayx += r.cxx * wyk;
-      axy += r.cxy * wxk; ayy += r.cxy * wyk;
-      bxy += r.cyy * wxk; byy += r.cyy * wyk;
-      if (MDIFF) { cxx_l += r.cxx * wlk; cxy_l += r.cxy * wlk; cyy_l += r.cyy * wlk; }
-    }
-  };
-  // Links in pairs, the column indices of the next pair loaded before the
-  // current pair is consumed, so each lane keeps two record gathers and the
-  // next index loads in flight.  Summation stays sequential in k.
-  constexpr int U = 2;
-  int jn[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) jn[u] = (u < K) ? a.col[b + (int64_t)u * SV_WAVE] : ii;
-  for (int k = 0; k < K; k += U) {
-    Rec9 r[U];
-    double wxk[U], wyk[U], wlk[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (k + u < K) {
-        const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
-        r[u] = load_g<FORM>(a.G, jn[u]);
-        wxk[u] = a.wx[e]; wyk[u] = a.wy[e]; wlk[u] = a.wl[e];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int kn = k + U + u;
-      jn[u] = (kn < K) ? a.col[b + (int64_t)kn * SV_WAVE] : ii;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (k + u < K) accum(r[u], wxk[u], wyk[u], wlk[u]);
-  }
+__device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef &nr,
+                                              const Sums &S) {
+  const int i = nr.i;
+  double rx = S.rx, ry = S.ry, ux = S.ux, uy = S.uy, vx = S.vx, vy = S.vy, lu = S.lu, lv = S.lv;
+  double axx = S.axx, ayx = S.ayx, axy = S.axy, ayy = S.ayy, bxy = S.bxy, byy = S.byy, cxx_l = S.cxx_l, cxy_l = S.cxy_l;
+  double cyy_l = S.cyy_l, q1x = S.q1x, q1y = S.q1y, q2x = S.q2x, q2y = S.q2y, q3x = S.q3x, q3y = S.q3y;
+  (void)ayx; (void)bxy; (void)cxx_l; (void)cxy_l; (void)cyy_l;
+  (void)q1x; (void)q1y; (void)q2x; (void)q2y; (void)q3x; (void)q3y;
   double emax_loc = 0.0;
-  if (act) {
+  {
     const double sx = a.gsx[i], sy = a.gsy[i], sl = a.lsum[i];
     const Rec9 s = load_g<FORM>(a.G, i);
     const double ro = s.ro, u = s.u, v = s.v, cxx = s.cxx, cxy = s.cxy, cyy = s.cyy;
@@ -319,6 +302,14 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
       r_xx = adxx + ucxx + fr * (cxx - 1.0) + mxx;
       r_xy = adxy + ucxy + fr * cxy + mxy;
       r_yy = adyy + ucyy + fr * (cyy - 1.0) + myy;
+      if (a.tapeD) {
+        double2 *T = reinterpret_cast<double2 *>(a.tapeD + (size_t)i * SV_TAPE_D);
+        T[0] = double2{rx, ry}; T[1] = double2{ux, uy}; T[2] = double2{lu, vx};
+        T[3] = double2{vy, lv}; T[4] = double2{dcxx_x, gcxx_y}; T[5] = double2{dcxy_x, dcxy_y};
+        T[6] = double2{gcyy_x, dcyy_y}; T[7] = double2{0.0, 0.0};
+        double2 *Q = reinterpret_cast<double2 *>(a.tapeG + (size_t)i * SV_US);
+        Q[0] = double2{ro, u}; Q[1] = double2{v, cxx}; Q[2] = double2{cxy, cyy};
+      }
     }
     // ---- RK update (step.F90:291-343, :375-409) ----
     const double rhs[6] = {r_ro, r_rou, r_rov, r_xx, r_xy, r_yy};
@@ -367,6 +358,73 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
       store_u(a.Uout, i, uo);
     }
   }
+  return emax_loc;
+}
+
+// STAGE 1..4: RK stages; STAGE 0: write the right-hand side into rhs_out.
+template <int FORM, int STAGE, bool MDIFF>
+__global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
+  __shared__ double red[BS / SV_WAVE];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int ii = nr.ii;
+  const bool act = nr.act;
+  const int64_t b = a.off[ii >> 6] + (ii & 63);
+  const int K = act ? a.kcnt[ii] : 0;
+  double rx = 0, ry = 0, ux = 0, uy = 0, vx = 0, vy = 0, lu = 0, lv = 0;
+  double axx = 0, ayx = 0, axy = 0, ayy = 0, bxy = 0, byy = 0;
+  double cxx_l = 0, cxy_l = 0, cyy_l = 0;
+  double q1x = 0, q1y = 0, q2x = 0, q2y = 0, q3x = 0, q3y = 0;
+  auto accum = [&](const Rec9 &r, double wxk, double wyk, double wlk) {
+    rx += r.ro * wxk; ry += r.ro * wyk;
+    ux += r.u * wxk; uy += r.u * wyk; lu += r.u * wlk;
+    vx += r.v * wxk; vy += r.v * wyk; lv += r.v * wlk;
+    if (FORM == SV_FORM_LC) {
+      axx += r.cxx * wxk;                         // calc_gradient_only(1,cxx)
+      axy += r.cxy * wxk; ayy += r.cxy * wyk;     // calc_gradient(cxy)
+      byy += r.cyy * wyk;                         // calc_gradient_only(2,cyy)
+      q1x += r.pxx * wxk; q1y += r.pxx * wyk;
+      q2x += r.pxy * wxk; q2y += r.pxy * wyk;
+      q3x += r.pyy * wxk; q3y += r.pyy * wyk;
+    } else {
+      axx += r.cxx * wxk; ayx += r.cxx * wyk;
+      axy += r.cxy * wxk; ayy += r.cxy * wyk;
+      bxy += r.cyy * wxk; byy += r.cyy * wyk;
+      if (MDIFF) { cxx_l += r.cxx * wlk; cxy_l += r.cxy * wlk; cyy_l += r.cyy * wlk; }
+    }
+  };
+  // Links in pairs, the column indices of the next pair loaded before the
+  // current pair is consumed, so each lane keeps two record gathers and the
+  // next index loads in flight.  Summation stays sequential in k.
+  constexpr int U = SV_GATHER_UNROLL;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : ii;
+  for (int k = 0; k < K; k += U) {
+    Rec9 r[U];
+    double wxk[U], wyk[U], wlk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k + u < K) {
+        const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+        r[u] = load_g<FORM>(a.G, jn[u]);
+        wxk[u] = ld_stream(a.wx + e); wyk[u] = ld_stream(a.wy + e); wlk[u] = ld_stream(a.wl + e);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kn = k + U + u;
+      jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : ii;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k + u < K) accum(r[u], wxk[u], wyk[u], wlk[u]);
+  }
+  double emax_loc = 0.0;
+  if (act) {
+    const Sums S{rx, ry, ux, uy, vx, vy, lu, lv, axx, ayx, axy, ayy, bxy, byy, cxx_l, cxy_l, cyy_l, q1x, q1y, q2x, q2y, q3x, q3y};
+    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
+  }
   if (STAGE == 4) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) emax_loc = fmax(emax_loc, __shfl_xor(emax_loc, o));
@@ -375,6 +433,119 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
     if (threadIdx.x == 0) {
       double m = red[0];
       for (int w = 1; w < BS / SV_WAVE; w++) m = fmax(m, red[w]);
+      a.partial[bid] = m;
+    }
+  }
+}
+
+// Cooperative-gather form of k_stage: in the neighbour sweep each node is
+// served by LPN lanes (8 for the 128-byte LC record, 4 for the 64-byte DI
+// record), lane q loading 16 bytes (slot q) of every neighbour record, so one
+// gather instruction of a wavefront touches 64/LPN cache lines instead of 64.
+// Each lane accumulates the six products of its two fields with (wx, wy, wl);
+// the sums a node needs go through LDS to one lane per node for the pointwise
+// part.  Summation order per sum is sequential in k as in k_stage, so the
+// results are bitwise the same.
+//   LC slots: (ro,u) (v,cxx) (cxy,cyy) (pxx,pxy) (pyy,-);  DI: (ro,u) (v,cxx) (cxy,cyy)
+template <int FORM>
+struct Coop {
+  static constexpr int LPN = FORM == SV_FORM_LC ? 8 : 4;   // lanes per node
+  static constexpr int NPR = SV_WAVE / LPN;                // nodes per round
+  static constexpr int NQ = FORM == SV_FORM_LC ? 5 : 3;    // record slots used
+  static constexpr int NSUM = FORM == SV_FORM_LC ? 18 : 17;
+  static constexpr int RS2 = (FORM == SV_FORM_LC ? SV_GS_LC : SV_GS_DI) / 2;
+};
+
+template <int FORM, int STAGE, bool MDIFF>
+__global__ __launch_bounds__(BS) void k_stage_coop(StageArgs a) {
+  using CP = Coop<FORM>;
+  __shared__ double sm[BS / SV_WAVE][SV_WAVE][CP::NSUM];
+  __shared__ double red[BS / SV_WAVE];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int lane = threadIdx.x & (SV_WAVE - 1), w = threadIdx.x / SV_WAVE;
+  const int g = lane / CP::LPN, q = lane % CP::LPN;
+  const sv_segk &SG = a.seg[nr.sg];
+  const int wlocal = (bid - SG.blk0) * BS + w * SV_WAVE;   // wave's first node, local
+  const int wbase = SG.base + wlocal;
+  const int64_t soff = wlocal < SG.npfb ? a.off[wbase >> 6] : 0;
+  const double2 *G2 = reinterpret_cast<const double2 *>(a.G);
+  const bool qa = q < CP::NQ;
+  for (int r = 0; r < CP::LPN; r++) {
+    const int l = r * CP::NPR + g;
+    const bool nact = wlocal + l < SG.npfb;
+    const int K = nact ? a.kcnt[wbase + l] : 0;
+    const int64_t b = soff + l;
+    double P0 = 0, P1 = 0, P2 = 0, P3 = 0, P4 = 0, P5 = 0;
+    constexpr int U = SV_GATHER_UNROLL;
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : wbase;
+    for (int k = 0; k < K; k += U) {
+      double2 d[U];
+      double wxk[U], wyk[U], wlk[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k + u < K) {
+          const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+          d[u] = qa ? G2[(size_t)jn[u] * CP::RS2 + q] : double2{0.0, 0.0};
+          wxk[u] = ld_stream(a.wx + e); wyk[u] = ld_stream(a.wy + e);
+          wlk[u] = ld_stream(a.wl + e);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int kn = k + U + u;
+        jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : wbase;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (k + u < K) {
+          P0 += d[u].x * wxk[u]; P1 += d[u].x * wyk[u]; P2 += d[u].x * wlk[u];
+          P3 += d[u].y * wxk[u]; P4 += d[u].y * wyk[u]; P5 += d[u].y * wlk[u];
+        }
+    }
+    if (nact && qa) {
+      double *o = sm[w][l];
+      if (FORM == SV_FORM_LC) {
+        if (q == 0) { o[0] = P0; o[1] = P1; o[2] = P3; o[3] = P4; o[4] = P5; }
+        else if (q == 1) { o[5] = P0; o[6] = P1; o[7] = P2; o[8] = P3; }
+        else if (q == 2) { o[9] = P0; o[10] = P1; o[11] = P4; }
+        else if (q == 3) { o[12] = P0; o[13] = P1; o[14] = P3; o[15] = P4; }
+        else { o[16] = P0; o[17] = P1; }
+      } else {
+        if (q == 0) { o[0] = P0; o[1] = P1; o[2] = P3; o[3] = P4; o[4] = P5; }
+        else if (q == 1) { o[5] = P0; o[6] = P1; o[7] = P2; o[8] = P3; o[9] = P4; o[10] = P5; }
+        else { o[11] = P0; o[12] = P1; o[13] = P2; o[14] = P3; o[15] = P4; o[16] = P5; }
+      }
+    }
+  }
+  __syncthreads();
+  double emax_loc = 0.0;
+  if (nr.act) {
+    const double *o = sm[w][lane];
+    Sums S{};
+    S.rx = o[0]; S.ry = o[1]; S.ux = o[2]; S.uy = o[3]; S.lu = o[4];
+    S.vx = o[5]; S.vy = o[6]; S.lv = o[7]; S.axx = o[8];
+    if (FORM == SV_FORM_LC) {
+      S.axy = o[9]; S.ayy = o[10]; S.byy = o[11];
+      S.q1x = o[12]; S.q1y = o[13]; S.q2x = o[14]; S.q2y = o[15]; S.q3x = o[16]; S.q3y = o[17];
+    } else {
+      S.ayx = o[9]; S.cxx_l = o[10];
+      S.axy = o[11]; S.ayy = o[12]; S.cxy_l = o[13]; S.bxy = o[14]; S.byy = o[15];
+      S.cyy_l = o[16];
+    }
+    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
+  }
+  if (STAGE == 4) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) emax_loc = fmax(emax_loc, __shfl_xor(emax_loc, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emax_loc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int v = 1; v < BS / SV_WAVE; v++) m = fmax(m, red[v]);
       a.partial[bid] = m;
     }
   }
@@ -404,23 +575,23 @@ __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
   const int K = act ? a.kcnt[ii] : 0;
   double t[6] = {0, 0, 0, 0, 0, 0};
   // pairs of links with the next pair's column indices prefetched, as k_stage
-  constexpr int U = 2;
+  constexpr int U = SV_GATHER_UNROLL;
   int jn[U];
 #pragma unroll
-  for (int u = 0; u < U; u++) jn[u] = (u < K) ? a.col[b + (int64_t)u * SV_WAVE] : ii;
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : ii;
   for (int k = 0; k < K; k += U) {
     double f[U][6], w[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k + u < K) {
         load_u(a.U, jn[u], f[u]);
-        w[u] = a.wh[b + (int64_t)(k + u) * SV_WAVE];
+        w[u] = ld_stream(a.wh + b + (int64_t)(k + u) * SV_WAVE);
       }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int kn = k + U + u;
-      jn[u] = (kn < K) ? a.col[b + (int64_t)kn * SV_WAVE] : ii;
+      jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : ii;
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
@@ -633,17 +804,25 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   return a;
 }
 
+#ifndef SV_COOP_GATHER
+#define SV_COOP_GATHER 0
+#endif
+
 template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
-  if (FORM == SV_FORM_DI && c->any_mdiff)
-    k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
-  else
-    k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+  const bool coop = SV_COOP_GATHER != 0;
+  if (FORM == SV_FORM_DI && c->any_mdiff) {
+    if (coop) k_stage_coop<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
+    else k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
+  } else {
+    if (coop) k_stage_coop<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+    else k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+  }
 }
 
 // ev (nullable): SV_NKERNELS+1 events recorded between the launches of one step
 template <int FORM>
-int step_form(sv_ctx *c, hipEvent_t *ev) {
+int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   const int nb = c->nblk;
   double *pm = c->partial, *pe = c->partial + nb, *pc = c->partial + 2 * nb,
          *pv = c->partial + 3 * nb;
@@ -654,15 +833,18 @@ int step_form(sv_ctx *c, hipEvent_t *ev) {
   mark();
   k_dt<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pc, pv, c->ctrl);
   mark();
-  launch_stage<FORM, 1>(c, stage_args(c, 1, c->GA, c->GB), nb);
+  StageArgs as[4] = {stage_args(c, 1, c->GA, c->GB), stage_args(c, 2, c->GB, c->GA),
+                     stage_args(c, 3, c->GA, c->GB), stage_args(c, 4, c->GB, c->GA)};
+  as[3].partial = pe;
+  if (tape)
+    for (int k = 0; k < 4; k++) { as[k].tapeG = tape[2 * k]; as[k].tapeD = tape[2 * k + 1]; }
+  launch_stage<FORM, 1>(c, as[0], nb);
   mark();
-  launch_stage<FORM, 2>(c, stage_args(c, 2, c->GB, c->GA), nb);
+  launch_stage<FORM, 2>(c, as[1], nb);
   mark();
-  launch_stage<FORM, 3>(c, stage_args(c, 3, c->GA, c->GB), nb);
+  launch_stage<FORM, 3>(c, as[2], nb);
   mark();
-  StageArgs a4 = stage_args(c, 4, c->GB, c->GA);
-  a4.partial = pe;
-  launch_stage<FORM, 4>(c, a4, nb);
+  launch_stage<FORM, 4>(c, as[3], nb);
   mark();
   FilterArgs f{};
   f.blk_seg = c->blk_seg;
@@ -731,16 +913,24 @@ int sv_dev_init_state(sv_ctx *c) {
 }
 
 int sv_dev_step(sv_ctx *c, hipEvent_t *ev) {
-  return c->cfg.form == SV_FORM_LC ? step_form<SV_FORM_LC>(c, ev) : step_form<SV_FORM_DI>(c, ev);
+  return c->cfg.form == SV_FORM_LC ? step_form<SV_FORM_LC>(c, ev, nullptr)
+                                   : step_form<SV_FORM_DI>(c, ev, nullptr);
+}
+
+int sv_dev_step_tape(sv_ctx *c, double *const tape[8]) {
+  if (c->cfg.form != SV_FORM_DI) return sv_fail(c, SV_E_STATE, "the tape is DI-only");
+  return step_form<SV_FORM_DI>(c, nullptr, tape);
 }
 
 // calc_all_rhs on the current state (interior nodes): the stage kernel in its
 // right-hand-side-only form, written into the six arrays of out6 (npfb each).
-int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6) {
+int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6, double *tapeG, double *tapeD) {
   const int nb = c->nblk;
   StageArgs a = stage_args(c, 1, c->GA, c->GB);
   a.rk1 = (iRKstep == 1);
   a.rhs_out = out6;
+  a.tapeG = tapeG;
+  a.tapeD = tapeD;
   if (c->cfg.form == SV_FORM_LC) launch_stage<SV_FORM_LC, 0>(c, a, nb);
   else launch_stage<SV_FORM_DI, 0>(c, a, nb);
   SV_HIP(c, hipGetLastError());

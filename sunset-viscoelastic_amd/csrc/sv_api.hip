@@ -85,6 +85,7 @@ void sv_destroy(sv_ctx *c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  sv_adj_free(c);
   delete c;
 }
 
@@ -130,6 +131,7 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   // drop everything derived from an earlier node set
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  sv_adj_free(c);
   c->x = c->y = c->h = c->hs = c->s = nullptr;
   c->node_type = c->irel = c->vrel = nullptr;
   c->cnt = c->cnt_small = nullptr; c->row_ptr = nullptr; c->col_raw = nullptr;

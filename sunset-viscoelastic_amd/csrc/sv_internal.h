@@ -73,6 +73,9 @@ struct sv_physk {
 #define SV_GS_LC 16
 #define SV_GS_DI 8
 #define SV_US 8
+struct sv_adj;   // adjoint work space (k_adjoint.hip)
+struct sv_ctx;
+void sv_adj_free(sv_ctx *c);
 #define SV_G_RO 0
 #define SV_G_U 1
 #define SV_G_V 2
@@ -149,6 +152,7 @@ struct sv_ctx {
   sv_segk *segk = nullptr;  // [nseg]
   int *blk_seg = nullptr;   // [nblk] segment of each logical block
   bool is_ensemble = false;
+  sv_adj *adj = nullptr;
   bool any_forcing = false, any_mdiff = false;  // over segments
   std::vector<sv_segk> hseg;
   // scratch for drop-in ops
@@ -191,7 +195,12 @@ int sv_dev_init_state(sv_ctx *c);
 // one step; ev (nullable) gets SV_NKERNELS+1 events recorded between launches
 #define SV_NKERNELS 8
 int sv_dev_step(sv_ctx *c, hipEvent_t *ev = nullptr);
-int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6);
+int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6, double *tapeG = nullptr,
+               double *tapeD = nullptr);
+// DI step that also writes, per stage k, the own gather record to tape[2k]
+// ([ntot][SV_US]) and the derivative values to tape[2k+1] ([ntot][SV_TAPE_D])
+#define SV_TAPE_D 16
+int sv_dev_step_tape(sv_ctx *c, double *const tape[8]);
 // per-block partial minima of set_tstep for the current state (no correction)
 int sv_dev_init_partials(sv_ctx *c);
 // dst[k] += add for k < n (index rebasing when members are concatenated)

@@ -53,7 +53,8 @@ EXPORTS = [
     "sv_calc_filter_term", "sv_apply_dev", "sv_set_phys", "sv_set_initial", "sv_set_control",
     "sv_get_control", "sv_calc_all_rhs", "sv_run", "sv_get_state", "sv_sync",
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
-    "sv_ensemble_get_state", "sv_ensemble_get_control",
+    "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
+    "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_timing",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -104,6 +105,11 @@ def lib(path: Optional[str] = None):
     L.sv_sync.argtypes = [C.c_void_p]
     L.sv_run_profiled.argtypes = [C.c_void_p, C.c_int32, PD, PD]
     L.sv_get_stencil_stats.argtypes = [C.c_void_p, PD]
+    L.sv_adj_setup.argtypes = [C.c_void_p, C.c_int32]
+    L.sv_rhs_vjp.argtypes = [C.c_void_p, C.POINTER(PD), C.POINTER(PD)]
+    L.sv_rollout_forward.argtypes = [C.c_void_p, C.c_int32, PD]
+    L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
+    L.sv_adj_timing.argtypes = [C.c_void_p, PD, PD]
     L.sv_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.sv_ensemble_size.argtypes = [C.c_void_p]
     L.sv_ensemble_size.restype = C.c_int32
@@ -300,6 +306,50 @@ class Solver:
         outs = [np.zeros(self.npfb) for _ in range(6)]
         self._ck(lib().sv_calc_all_rhs(self.h, iRKstep, *[_d(a) for a in outs]))
         return dict(zip(("ro", "rou", "rov", "xx", "xy", "yy"), outs))
+
+    # ---- reverse mode (DI, fixed dt) ----
+    STATE_KEYS = ("ro", "rou", "rov", "cxx", "cxy", "cyy")
+
+    def _ptrs(self, d, alloc=False):
+        if d is None:
+            return None, []
+        arrs = [np.zeros(self.npfb) if alloc else _f64(d[k]) for k in self.STATE_KEYS]
+        return (PD * 6)(*[_d(a) for a in arrs]), arrs
+
+    def adj_setup(self, checkpoint_every: int = 20):
+        self._ck(lib().sv_adj_setup(self.h, checkpoint_every))
+
+    def rhs_vjp(self, rbar):
+        """(d calc_all_rhs / d state)^T rbar; rbar keyed ro, rou, rov, cxx, cxy, cyy."""
+        pin, keep = self._ptrs(rbar)
+        pout, out = self._ptrs(None, alloc=False)[0], [np.zeros(self.npfb) for _ in range(6)]
+        pout = (PD * 6)(*[_d(a) for a in out])
+        self._ck(lib().sv_rhs_vjp(self.h, pin, pout))
+        return dict(zip(self.STATE_KEYS, out))
+
+    def rollout_forward(self, nsteps: int) -> float:
+        L = C.c_double()
+        self._ck(lib().sv_rollout_forward(self.h, nsteps, C.byref(L)))
+        return L.value
+
+    def rollout_vjp(self, loss_bar: float = 1.0, ubar_final=None):
+        pin, keep = self._ptrs(ubar_final)
+        out = [np.zeros(self.npfb) for _ in range(6)]
+        pout = (PD * 6)(*[_d(a) for a in out])
+        self._ck(lib().sv_rollout_vjp(self.h, loss_bar, pin, pout))
+        return dict(zip(self.STATE_KEYS, out))
+
+    def adj_timing(self):
+        f, b = C.c_double(), C.c_double()
+        self._ck(lib().sv_adj_timing(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def set_state(self, U, dt=None, itime=0, time=0.0):
+        """Set (ro, rou, rov, cxx, cxy, cyy) directly (DI) with a fixed dt."""
+        ro = _f64(U["ro"])
+        ic = dict(ro=ro, u=_f64(U["rou"]) / ro, v=_f64(U["rov"]) / ro, cxx=U["cxx"],
+                  cxy=U["cxy"], cyy=U["cyy"])
+        self.set_initial(ic, dt=dt, itime=itime, dt_mode="fixed", time=time)
 
     def fields(self):
         n = self.npfb

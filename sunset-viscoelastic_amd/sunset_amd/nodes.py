@@ -138,6 +138,30 @@ def taylor_green_ic(ns: NodeSet):
                 cyy=np.ones(n))
 
 
+def perturbed_rest_ic(ns: NodeSet, amp: float = 0.01, seed: int = 0, kmax: int = 4):
+    """Fluid at rest (rho = 1, c = I) plus a divergence-free random-phase velocity
+    perturbation: streamfunction sum_k a_k sin(2 pi k.x + phi_k) / |k| over
+    integer wavevectors 0 < |k_x|, |k_y| <= kmax (a_k ~ U(0,1), phi_k ~ U(0, 2 pi),
+    numpy PCG64 with `seed`), u = d(psi)/dy, v = -d(psi)/dx, scaled so that
+    max |u, v| over the nodes equals amp."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x, y = ns.x, ns.y
+    u = np.zeros_like(x)
+    v = np.zeros_like(x)
+    for kx in range(-kmax, kmax + 1):
+        for ky in range(1, kmax + 1):
+            a = rng.uniform(0.0, 1.0)
+            ph = rng.uniform(0.0, 2 * np.pi)
+            kk = math.hypot(kx, ky)
+            arg = 2 * np.pi * (kx * x + ky * y) + ph
+            u += a * 2 * np.pi * ky * np.cos(arg) / kk
+            v -= a * 2 * np.pi * kx * np.cos(arg) / kk
+    scale = amp / max(np.abs(u).max(), np.abs(v).max())
+    n = ns.npfb
+    return dict(ro=np.ones(n), u=u * scale, v=v * scale, cxx=np.ones(n), cxy=np.zeros(n),
+                cyy=np.ones(n))
+
+
 def cylinder_channel(smin: float = 0.004, smax: float = 0.016, grow: float = 1.5,
                      box=(0.0, 8.0, 0.0, 2.0), centre=(2.0, 1.0), radius: float = 0.5,
                      seed: int = 3, jitter: float = 0.3, h_over_s: float = 2.7,

@@ -48,6 +48,11 @@ class State(C.Structure):
                 ("emax_nm1", C.c_double), ("itime", C.c_int)]
 
 
+class Tangent(C.Structure):
+    _fields_ = [("np", C.c_int), ("ro", PD), ("rou", PD), ("rov", PD), ("u", PD), ("v", PD),
+                ("cxx", PD), ("cxy", PD), ("cyy", PD)]
+
+
 _lib = None
 
 
@@ -78,6 +83,16 @@ def lib():
         L.or_set_initial.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State)] + [PD] * 6
         L.or_run.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State), C.c_int]
         L.or_calc_labf_weights_ordermap.argtypes = [C.POINTER(Domain), PI, C.c_int]
+        L.or_tangent_alloc.argtypes = [C.POINTER(Tangent), C.c_int]
+        L.or_tangent_free.argtypes = [C.POINTER(Tangent)]
+        L.or_tangent_set.argtypes = [C.POINTER(Domain), C.POINTER(State),
+                                     C.POINTER(Tangent)] + [PD] * 6
+        L.or_calc_all_rhs_di_tlm.argtypes = [C.POINTER(Domain), C.POINTER(Phys),
+                                             C.POINTER(State), C.POINTER(Tangent), C.POINTER(PD)]
+        L.or_rollout_tlm.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State),
+                                     C.POINTER(Tangent), C.c_int, PD, PD]
+        L.or_loss_trace.argtypes = [C.POINTER(Domain), C.POINTER(State)]
+        L.or_loss_trace.restype = C.c_double
         L.or_set_num_threads.argtypes = [C.c_int]
         L.or_num_threads.restype = C.c_int
         _lib = L
@@ -265,3 +280,40 @@ class OracleSim:
         out.update(dt=st.dt, time=st.time, emax_np1=st.emax_np1, emax_n=st.emax_n,
                    emax_nm1=st.emax_nm1, dt_cfl=st.dt_cfl, dt_par=st.dt_par, itime=st.itime)
         return out
+
+    # ---- tangent-linear model (DI, fixed dt) ----
+    STATE_KEYS = ("ro", "rou", "rov", "cxx", "cxy", "cyy")
+
+    def _tangent(self, dU):
+        t = Tangent()
+        assert lib().or_tangent_alloc(C.byref(t), self.d.np) == 0
+        arrs = [np.ascontiguousarray(dU[k], dtype=np.float64) for k in self.STATE_KEYS]
+        lib().or_tangent_set(C.byref(self.d), C.byref(self.st), C.byref(t),
+                             *[_pd(a) for a in arrs])
+        return t
+
+    def rhs_tlm(self, dU):
+        """Tangent of calc_all_rhs (DI) at the current state along dU."""
+        t = self._tangent(dU)
+        n = self.d.npfb
+        outs = [np.zeros(n) for _ in range(6)]
+        ptrs = (PD * 6)(*[_pd(a) for a in outs])
+        lib().or_calc_all_rhs_di_tlm(C.byref(self.d), C.byref(self.p), C.byref(self.st),
+                                     C.byref(t), ptrs)
+        lib().or_tangent_free(C.byref(t))
+        return dict(zip(("ro", "rou", "rov", "xx", "xy", "yy"), outs))
+
+    def rollout_tlm(self, dU, nsteps):
+        """Advance nsteps (fixed dt) with the tangent: (loss, dloss, final tangent)."""
+        t = self._tangent(dU)
+        L, dL = C.c_double(), C.c_double()
+        rc = lib().or_rollout_tlm(C.byref(self.d), C.byref(self.p), C.byref(self.st),
+                                  C.byref(t), nsteps, C.byref(L), C.byref(dL))
+        assert rc == 0
+        n = self.d.npfb
+        fin = {k: _view(getattr(t, k), n) for k in self.STATE_KEYS}
+        lib().or_tangent_free(C.byref(t))
+        return L.value, dL.value, fin
+
+    def loss_trace(self):
+        return lib().or_loss_trace(C.byref(self.d), C.byref(self.st))

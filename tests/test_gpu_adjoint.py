@@ -1,0 +1,145 @@
+"""Device reverse mode (DI, fixed dt) through the C ABI, checked against the
+oracle's tangent-linear model (pinned by finite differences in
+test_oracle_tlm.py) and against central finite differences of the device's
+own forward rollout:
+  - right-hand-side VJP:   <rbar, J dU>_oracle == <J^T rbar, dU>_device
+  - final-state VJP:       <ubar, dU_T>_oracle == <grad0, dU>_device
+  - loss gradient:         dL_oracle == <grad0, dU>_device, and central FD
+  - the state after the VJP is the state after the forward rollout, bitwise
+  - torch.autograd.Function around the rollout (CPU tensors at the boundary)
+Tolerances: 1e-9 relative against the tangent-linear model (rounding of two
+different evaluation orders), 1e-6 against finite differences (step 1e-6)."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from sunset_amd.nodes import jittered_lattice, perturbed_rest_ic
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("ro", "rou", "rov", "cxx", "cxy", "cyy")
+PH = dict(Re=1.0, Wi=1.0, beta=0.5, eps=0.1, Ma=0.1, kol_amp=1.0, kol_k=4 * np.pi)
+DT = 2e-4
+
+
+def _ic(ns):
+    ic = perturbed_rest_ic(ns, amp=0.05, seed=3)
+    ic["cxx"] = 1.0 + 0.2 * np.sin(2 * np.pi * ns.x) ** 2
+    ic["cxy"] = 0.1 * np.cos(2 * np.pi * ns.y)
+    return ic
+
+
+def _state(ic):
+    return dict(ro=ic["ro"], rou=ic["ro"] * ic["u"], rov=ic["ro"] * ic["v"], cxx=ic["cxx"],
+                cxy=ic["cxy"], cyy=ic["cyy"])
+
+
+def _dir(n, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.standard_normal(n) * (0.01 if k == "ro" else 0.1) for k in KEYS}
+
+
+def _dot(a, b):
+    return sum(float(np.dot(a[k], b[k])) for k in KEYS)
+
+
+def _device(ns, ph, order=4):
+    import sunset_amd
+    s = sunset_amd.Solver(morder=order, form="di").setup(ns)
+    s.set_phys(**ph)
+    return s
+
+
+def _oracle(ns, ph, ic, order=4):
+    o = OL.OracleSim(ns, order).setup()
+    o.make_phys(form="di", dt_mode="fixed", **ph)
+    o.set_initial(ic, dt=DT)
+    return o
+
+
+@pytest.mark.parametrize("mdiff", [0.0, 2e-3])
+def test_rhs_vjp_dot_product_with_oracle_tangent(mdiff):
+    ns = jittered_lattice(16, seed=2, h_over_s=1.8)
+    ic = _ic(ns)
+    ph = dict(PH, Mdiff=mdiff)
+    print(f"[adjoint] rhs vjp mdiff={mdiff}", flush=True)
+    s = _device(ns, ph)
+    s.set_state(_state(ic), dt=DT)
+    s.adj_setup(5)
+    dU = _dir(ns.npfb, 7)
+    rbar = _dir(ns.npfb, 9)
+    ub = s.rhs_vjp(rbar)
+    tl = _oracle(ns, ph, ic).rhs_tlm(dU)
+    tl = dict(zip(KEYS, (tl["ro"], tl["rou"], tl["rov"], tl["xx"], tl["xy"], tl["yy"])))
+    lhs, rhs = _dot(rbar, tl), _dot(ub, dU)
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs), (lhs, rhs)
+    s.close()
+
+
+def test_rollout_vjp_matches_tangent_linear_model_and_fd():
+    ns = jittered_lattice(16, seed=4, h_over_s=1.8)
+    ic = _ic(ns)
+    U0 = _state(ic)
+    steps, every = 25, 7          # several checkpoint intervals, a short last one
+    dU = _dir(ns.npfb, 11)
+    print("[adjoint] rollout", flush=True)
+    s = _device(ns, PH)
+    s.set_state(U0, dt=DT)
+    s.adj_setup(every)
+    L = s.rollout_forward(steps)
+    end = s.fields()
+    g = s.rollout_vjp(1.0)
+    after = s.fields()
+    for k in ("ro", "rou", "rov", "cxx", "cxy", "cyy", "u", "v"):
+        assert np.array_equal(end[k], after[k]), k
+    Lo, dLo, _ = _oracle(ns, PH, ic).rollout_tlm(dU, steps)
+    assert abs(L - Lo) <= 1e-10 * abs(Lo) + 1e-15
+    dL = _dot(g, dU)
+    assert abs(dL - dLo) <= 1e-9 * abs(dLo), (dL, dLo)
+    # central differences of the device forward rollout
+    eps = 1e-6
+    Lpm = []
+    for sign in (1, -1):
+        s.set_state({k: U0[k] + sign * eps * dU[k] for k in KEYS}, dt=DT)
+        Lpm.append(s.rollout_forward(steps))
+    fd = (Lpm[0] - Lpm[1]) / (2 * eps)
+    assert abs(fd - dL) <= 1e-6 * abs(dL), (fd, dL)
+    s.close()
+
+
+def test_final_state_cotangent_matches_tangent():
+    ns = jittered_lattice(16, seed=5, h_over_s=1.8)
+    ic = _ic(ns)
+    steps = 9
+    dU = _dir(ns.npfb, 13)
+    ubar = _dir(ns.npfb, 17)
+    s = _device(ns, dict(PH, Mdiff=1e-3))
+    s.set_state(_state(ic), dt=DT)
+    s.adj_setup(4)
+    s.rollout_forward(steps)
+    g = s.rollout_vjp(0.0, ubar_final=ubar)
+    _, _, dUT = _oracle(ns, dict(PH, Mdiff=1e-3), ic).rollout_tlm(dU, steps)
+    lhs, rhs = _dot(ubar, dUT), _dot(g, dU)
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs), (lhs, rhs)
+    s.close()
+
+
+def test_torch_autograd_rollout():
+    torch = pytest.importorskip("torch")
+    from sunset_amd.autograd import rollout_loss
+    ns = jittered_lattice(16, seed=6, h_over_s=1.8)
+    ic = _ic(ns)
+    U0 = np.stack([_state(ic)[k] for k in KEYS])
+    d = np.stack([_dir(ns.npfb, 19)[k] for k in KEYS])
+    s = _device(ns, PH)
+    x = torch.tensor(U0, dtype=torch.float64, requires_grad=True)
+    L = rollout_loss(s, x, 12, DT, checkpoint_every=5)
+    L.backward()
+    dL = float((x.grad * torch.from_numpy(d)).sum())
+    eps = 1e-6
+    with torch.no_grad():
+        Lp = float(rollout_loss(s, torch.from_numpy(U0 + eps * d), 12, DT, 5))
+        Lm = float(rollout_loss(s, torch.from_numpy(U0 - eps * d), 12, DT, 5))
+    fd = (Lp - Lm) / (2 * eps)
+    assert abs(fd - dL) <= 1e-6 * abs(dL), (fd, dL)
+    s.close()
