@@ -425,19 +425,20 @@ static int zero_dev(sv_ctx *c, double *p, int64_t n) {
 // Transposed stencil of the folded SELL lists: for each j the entries (i, k)
 // with scol = j, in ascending i, stage weights zero beyond ij_count_small.
 static int build_transpose(sv_ctx *c, sv_adj *A) {
-  const int n = c->npfb, ns = c->nslices;
-  std::vector<int> cnt(n), cs(n), scol(c->sell_nnz);
+  const sv_loopl &LP = c->L;
+  const int n = c->npfb, ns = LP.nslices;
+  std::vector<int> cnt(n), cs(n), scol(LP.nnz);
   std::vector<int64_t> off(ns);
-  std::vector<double> wx(c->sell_nnz), wy(c->sell_nnz), wl(c->sell_nnz), wh(c->sell_nnz);
-  SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(cs.data(), c->cnt_small, sizeof(int) * n, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(off.data(), c->sell_off, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
-  const size_t L = sizeof(double) * c->sell_nnz;
-  SV_HIP(c, hipMemcpy(scol.data(), c->scol, sizeof(int) * c->sell_nnz, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(wx.data(), c->swx, L, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(wy.data(), c->swy, L, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(wl.data(), c->swl, L, hipMemcpyDeviceToHost));
-  SV_HIP(c, hipMemcpy(wh.data(), c->swh, L, hipMemcpyDeviceToHost));
+  std::vector<double> wx(LP.nnz), wy(LP.nnz), wl(LP.nnz), wh(LP.nnz);
+  SV_HIP(c, hipMemcpy(cnt.data(), LP.cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(cs.data(), LP.cnt_small, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(off.data(), LP.off, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+  const size_t L = sizeof(double) * LP.nnz;
+  SV_HIP(c, hipMemcpy(scol.data(), LP.col, sizeof(int) * LP.nnz, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wx.data(), LP.wx, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wy.data(), LP.wy, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wl.data(), LP.wl, L, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(wh.data(), LP.wh, L, hipMemcpyDeviceToHost));
   std::vector<int> tc(n, 0);
   for (int i = 0; i < n; i++)
     for (int k = 0; k < cnt[i]; k++) {
@@ -504,8 +505,8 @@ static int ensure_tape(sv_ctx *c, sv_adj *A) {
 template <int STAGE>
 static int adj_stage(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
   const int k = (STAGE == 0 ? 1 : STAGE) - 1;
-  PointArgs p{c->blk_seg, c->segk, tape[2 * k], tape[2 * k + 1], A->Rbar, c->gsx, c->gsy,
-              c->lsum, A->dbar, A->dbarm, A->own};
+  PointArgs p{c->blk_seg, c->segk, tape[2 * k], tape[2 * k + 1], A->Rbar, c->L.gsx, c->L.gsy,
+              c->L.lsum, A->dbar, A->dbarm, A->own};
   if (c->any_mdiff) k_adj_point<true><<<c->nblk, BS, 0, c->stream>>>(p);
   else k_adj_point<false><<<c->nblk, BS, 0, c->stream>>>(p);
   TransArgs t{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twx, A->twy, A->twl,
@@ -521,9 +522,9 @@ static int adj_stage(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
 // in psum) -> Ubar (before the step) + seed on cxx, cyy
 static int adj_step(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
   k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, A->sbar);
-  k_adj_mcorr<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->sbar, c->vol, A->Ubar);
+  k_adj_mcorr<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->sbar, c->L.vol, A->Ubar);
   k_adj_filter<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol,
-                                              A->twh, c->hsum, A->Ubar, c->ctrl, A->regbar,
+                                              A->twh, c->L.hsum, A->Ubar, c->ctrl, A->regbar,
                                               A->Rbar);
   SV_HIP(c, hipGetLastError());
   if (int rc = adj_stage<4>(c, A, tape, 0.0)) return rc;
@@ -539,21 +540,26 @@ static int loss_accumulate(sv_ctx *c, sv_adj *A) {
   return 0;
 }
 
+// caller-order arrays <-> loop-order records
 static int upload_records(sv_ctx *c, double *dst, const double *const f[6]) {
   const int n = c->npfb;
+  const std::vector<int> &P = c->L.hperm;
   std::vector<double> rec((size_t)SV_US * c->ntot, 0.0);
   for (int q = 0; q < 6; q++)
     if (f && f[q])
-      for (int i = 0; i < n; i++) rec[(size_t)SV_US * i + q] = f[q][i];
+      for (int k = 0; k < n; k++) rec[(size_t)SV_US * k + q] = f[q][P[k]];
   SV_HIP(c, hipMemcpy(dst, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
 static int download_records(sv_ctx *c, const double *src, double *const f[6]) {
+  const int n = c->npfb;
+  const std::vector<int> &P = c->L.hperm;
+  std::vector<double> rec((size_t)SV_US * n);
+  SV_HIP(c, hipMemcpy(rec.data(), src, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
   for (int q = 0; q < 6; q++)
     if (f[q])
-      SV_HIP(c, hipMemcpy2D(f[q], sizeof(double), src + q, sizeof(double) * SV_US,
-                            sizeof(double), c->npfb, hipMemcpyDeviceToHost));
+      for (int k = 0; k < n; k++) f[q][P[k]] = rec[(size_t)SV_US * k + q];
   return 0;
 }
 

@@ -13,6 +13,8 @@
 // The cell grid is this library's own: cells of side ss*max(h) over the box
 // padded by its size on each side, so a 3x3 block of cells covers every
 // stencil whatever h_i is.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "sv_internal.h"
@@ -359,6 +361,7 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   SV_HIP(c, hipFree(ip));
   c->have_lists = true;
   c->sell_dirty = true;
+  c->L.valid = false;
   return 0;
 }
 
@@ -407,5 +410,138 @@ int sv_dev_build_sell(sv_ctx *c) {
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
   c->sell_dirty = false;
+  return 0;
+}
+
+// ---- time-loop layout (Morton order) --------------------------------------
+namespace {
+
+__global__ void k_loop_pack(int npfb, const int64_t *off, const int *perm, const int *iperm,
+                            const int64_t *row_ptr, const int *cnt, const int *col_raw,
+                            const int *irel, const double *wx, const double *wy,
+                            const double *wl, const double *wh, int64_t nnz_max, int *lcol,
+                            double *lwx, double *lwy, double *lwl, double *lwh) {
+  // one block of 64 threads per (slice, link-chunk of 4); slot = s*64 + l
+  const int s = blockIdx.x;
+  const int l = threadIdx.x;
+  const int slot = s * SV_WAVE + l;
+  const int64_t o = off[s];
+  const int W = (int)((off[s + 1] - o) / SV_WAVE);
+  const int k0 = blockIdx.y * 4;
+  const int i = slot < npfb ? perm[slot] : -1;
+  for (int k = k0; k < W && k < k0 + 4; k++) {
+    const int64_t d = o + (int64_t)k * SV_WAVE + l;
+    if (d >= nnz_max) break;
+    if (i >= 0 && k < cnt[i]) {
+      const int64_t sidx = row_ptr[i] + k;
+      lcol[d] = iperm[irel[col_raw[sidx]]];
+      lwx[d] = wx[sidx];
+      lwy[d] = wy[sidx];
+      lwl[d] = wl[sidx];
+      lwh[d] = wh[sidx];
+    } else {
+      lcol[d] = slot < npfb ? slot : 0;
+      lwx[d] = 0.0;
+      lwy[d] = 0.0;
+      lwl[d] = 0.0;
+      lwh[d] = 0.0;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_permute(int n, const int *perm, const T *src, T *dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[perm[k]];
+}
+
+// interleave the low 16 bits of a and b (Morton key)
+uint32_t morton2(uint32_t a, uint32_t b) {
+  auto spread = [](uint32_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ff;
+    v = (v | (v << 4)) & 0x0f0f0f0f;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+  };
+  return spread(a) | (spread(b) << 1);
+}
+
+}  // namespace
+
+int sv_dev_build_loop(sv_ctx *c) {
+  const int n = c->npfb;
+  sv_loopl &L = c->L;
+  // permutation
+  std::vector<double> x(n), y(n);
+  SV_HIP(c, hipMemcpy(x.data(), c->x, sizeof(double) * n, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(y.data(), c->y, sizeof(double) * n, hipMemcpyDeviceToHost));
+  L.hperm.resize(n);
+  for (int i = 0; i < n; i++) L.hperm[i] = i;
+  if (c->loop_order) {
+    const double sx = 65535.0 / (c->xmax - c->xmin), sy = 65535.0 / (c->ymax - c->ymin);
+    std::vector<uint32_t> key(n);
+    for (int i = 0; i < n; i++) {
+      const double fx = std::min(std::max((x[i] - c->xmin) * sx, 0.0), 65535.0);
+      const double fy = std::min(std::max((y[i] - c->ymin) * sy, 0.0), 65535.0);
+      key[i] = morton2((uint32_t)fx, (uint32_t)fy);
+    }
+    std::stable_sort(L.hperm.begin(), L.hperm.end(),
+                     [&](int a, int b) { return key[a] < key[b]; });
+  }
+  std::vector<int> iperm(c->np);
+  for (int k = 0; k < n; k++) iperm[L.hperm[k]] = k;
+  for (int j = n; j < c->np; j++) iperm[j] = -1;   // ghosts are folded before lookup
+  std::vector<int> cnt(n);
+  SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  const int ns = (n + SV_WAVE - 1) / SV_WAVE;
+  std::vector<int64_t> off(ns + 1);
+  off[0] = 0;
+  int wmax = 0;
+  for (int s = 0; s < ns; s++) {
+    int m = 0;
+    for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) m = std::max(m, cnt[L.hperm[s * SV_WAVE + l]]);
+    wmax = std::max(wmax, m);
+    off[s + 1] = off[s] + (int64_t)m * SV_WAVE;
+  }
+  // (re)allocate
+  void *old[] = {L.off, L.col, L.wx, L.wy, L.wl, L.wh, L.cnt, L.cnt_small, L.gsx, L.gsy,
+                 L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm};
+  for (void *b : old)
+    if (b) sv_free(c, b);
+  std::vector<int> hp = std::move(L.hperm);
+  L = sv_loopl{};
+  L.nslices = ns;
+  L.nnz = off[ns];
+  L.hperm = std::move(hp);
+  const size_t E = (size_t)L.nnz + 1, D = sizeof(double), I = sizeof(int);
+  if (sv_alloc(c, (void **)&L.off, sizeof(int64_t) * (ns + 1)) ||
+      sv_alloc(c, (void **)&L.col, I * E) || sv_alloc(c, (void **)&L.wx, D * E) ||
+      sv_alloc(c, (void **)&L.wy, D * E) || sv_alloc(c, (void **)&L.wl, D * E) ||
+      sv_alloc(c, (void **)&L.wh, D * E) || sv_alloc(c, (void **)&L.cnt, I * n) ||
+      sv_alloc(c, (void **)&L.cnt_small, I * n) || sv_alloc(c, (void **)&L.gsx, D * n) ||
+      sv_alloc(c, (void **)&L.gsy, D * n) || sv_alloc(c, (void **)&L.lsum, D * n) ||
+      sv_alloc(c, (void **)&L.hsum, D * n) || sv_alloc(c, (void **)&L.vol, D * n) ||
+      sv_alloc(c, (void **)&L.s, D * n) || sv_alloc(c, (void **)&L.perm, I * n) ||
+      sv_alloc(c, (void **)&L.iperm, I * c->np))
+    return SV_E_NOMEM;
+  SV_HIP(c, hipMemcpy(L.off, off.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(L.perm, L.hperm.data(), I * n, hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(L.iperm, iperm.data(), I * c->np, hipMemcpyHostToDevice));
+  dim3 grid(ns, (wmax + 3) / 4);
+  k_loop_pack<<<grid, SV_WAVE, 0, c->stream>>>(n, L.off, L.perm, L.iperm, c->row_ptr, c->cnt,
+                                               c->col_raw, c->irel, c->wx, c->wy, c->wl, c->wh,
+                                               L.nnz, L.col, L.wx, L.wy, L.wl, L.wh);
+  SV_HIP(c, hipGetLastError());
+  const int B = 256, nb = (n + B - 1) / B;
+  k_permute<int><<<nb, B, 0, c->stream>>>(n, L.perm, c->cnt, L.cnt);
+  k_permute<int><<<nb, B, 0, c->stream>>>(n, L.perm, c->cnt_small, L.cnt_small);
+  const double *src[6] = {c->gsx, c->gsy, c->lsum, c->hsum, c->vol, c->s};
+  double *dst[6] = {L.gsx, L.gsy, L.lsum, L.hsum, L.vol, L.s};
+  for (int k = 0; k < 6; k++) k_permute<double><<<nb, B, 0, c->stream>>>(n, L.perm, src[k], dst[k]);
+  SV_HIP(c, hipGetLastError());
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  L.valid = true;
   return 0;
 }

@@ -564,12 +564,35 @@ struct FilterArgs {
   double *partial;   // per-block sum of (ro_f - rho_char)*vol
 };
 
+// Pointwise part of the filter for one node: filtered record, its gather
+// record (psi and c = exp(psi) for LC), and the mass-correction term.
+template <int FORM>
+__device__ __forceinline__ double filter_point(const FilterArgs &a, const NodeRef &nr,
+                                               const double t[6]) {
+  const int i = nr.i;
+  const double hs = a.hsum[i];
+  double f[6], g[6];
+  load_u(a.U, i, f);
+#pragma unroll
+  for (int q = 0; q < 6; q++) g[q] = f[q] + (t[q] - f[q] * hs);
+  store_u(a.Uf, i, g);
+  constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
+  double *G = a.Gout + (size_t)i * S;
+  double c1 = g[3], c2 = g[4], c3 = g[5];
+  if (FORM == SV_FORM_LC) {
+    G[SV_G_PXX] = g[3]; G[SV_G_PXY] = g[4]; G[SV_G_PYY] = g[5];
+    c_from_psi(g[3], g[4], g[5], c1, c2, c3);
+  }
+  G[SV_G_CXX] = c1; G[SV_G_CXY] = c2; G[SV_G_CYY] = c3;
+  return (g[0] - a.seg[nr.sg].pk.rho_char) * a.vol[i];
+}
+
 template <int FORM>
 __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
   __shared__ double red[BS];
   const int bid = sv_xcd_block();
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
-  const int i = nr.i, ii = nr.ii;
+  const int ii = nr.ii;
   const bool act = nr.act;
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
@@ -600,24 +623,70 @@ __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
         for (int q = 0; q < 6; q++) t[q] += f[u][q] * w[u];
       }
   }
-  double m = 0.0;
-  if (act) {
-    const double hs = a.hsum[i];
-    double f[6], g[6];
-    load_u(a.U, i, f);
-#pragma unroll
-    for (int q = 0; q < 6; q++) g[q] = f[q] + (t[q] - f[q] * hs);
-    store_u(a.Uf, i, g);
-    constexpr int S = (FORM == SV_FORM_LC) ? SV_GS_LC : SV_GS_DI;
-    double *G = a.Gout + (size_t)i * S;
-    double c1 = g[3], c2 = g[4], c3 = g[5];
-    if (FORM == SV_FORM_LC) {
-      G[SV_G_PXX] = g[3]; G[SV_G_PXY] = g[4]; G[SV_G_PYY] = g[5];
-      c_from_psi(g[3], g[4], g[5], c1, c2, c3);
-    }
-    G[SV_G_CXX] = c1; G[SV_G_CXY] = c2; G[SV_G_CYY] = c3;
-    m = (g[0] - a.seg[nr.sg].pk.rho_char) * a.vol[i];
+  const double m = act ? filter_point<FORM>(a, nr, t) : 0.0;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) a.partial[bid] = red[0];
+}
+
+// Cooperative-gather form of k_filter (see k_stage_coop): 4 lanes per node,
+// lanes 0-2 each load 16 bytes (two fields) of every neighbour's state record.
+template <int FORM>
+__global__ __launch_bounds__(BS) void k_filter_coop(FilterArgs a) {
+  constexpr int LPN = 4, NPR = SV_WAVE / LPN, NQ = 3;
+  __shared__ double sm[BS / SV_WAVE][SV_WAVE][6];
+  __shared__ double red[BS];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int lane = threadIdx.x & (SV_WAVE - 1), w = threadIdx.x / SV_WAVE;
+  const int g = lane / LPN, q = lane % LPN;
+  const sv_segk &SG = a.seg[nr.sg];
+  const int wlocal = (bid - SG.blk0) * BS + w * SV_WAVE;
+  const int wbase = SG.base + wlocal;
+  const int64_t soff = wlocal < SG.npfb ? a.off[wbase >> 6] : 0;
+  const double2 *U2 = reinterpret_cast<const double2 *>(a.U);
+  const bool qa = q < NQ;
+  for (int r = 0; r < LPN; r++) {
+    const int l = r * NPR + g;
+    const bool nact = wlocal + l < SG.npfb;
+    const int K = nact ? a.kcnt[wbase + l] : 0;
+    const int64_t b = soff + l;
+    double P0 = 0, P1 = 0;
+    constexpr int U = SV_GATHER_UNROLL;
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : wbase;
+    for (int k = 0; k < K; k += U) {
+      double2 d[U];
+      double wk[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k + u < K) {
+          d[u] = qa ? U2[(size_t)jn[u] * (SV_US / 2) + q] : double2{0.0, 0.0};
+          wk[u] = ld_stream(a.wh + b + (int64_t)(k + u) * SV_WAVE);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int kn = k + U + u;
+        jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : wbase;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (k + u < K) { P0 += d[u].x * wk[u]; P1 += d[u].y * wk[u]; }
+    }
+    if (nact && qa) { sm[w][l][2 * q] = P0; sm[w][l][2 * q + 1] = P1; }
+  }
+  __syncthreads();
+  double t[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) t[f] = nr.act ? sm[w][lane][f] : 0.0;
+  const double m = nr.act ? filter_point<FORM>(a, nr, t) : 0.0;
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = BS / 2; s > 0; s >>= 1) {
@@ -787,11 +856,11 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   a.rhs_stride = c->npfb;
   a.blk_seg = c->blk_seg;
   a.seg = c->segk;
-  a.off = c->sell_off;
-  a.kcnt = c->cnt_small;
-  a.col = c->scol;
-  a.wx = c->swx; a.wy = c->swy; a.wl = c->swl;
-  a.gsx = c->gsx; a.gsy = c->gsy; a.lsum = c->lsum;
+  a.off = c->L.off;
+  a.kcnt = c->L.cnt_small;
+  a.col = c->L.col;
+  a.wx = c->L.wx; a.wy = c->L.wy; a.wl = c->L.wl;
+  a.gsx = c->L.gsx; a.gsy = c->L.gsy; a.lsum = c->L.lsum;
   a.G = Gin;
   a.reg_in = (stage == 1) ? c->U : c->reg;
   a.reg_out = c->reg;
@@ -849,17 +918,18 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   FilterArgs f{};
   f.blk_seg = c->blk_seg;
   f.seg = c->segk;
-  f.off = c->sell_off;
-  f.kcnt = c->cnt;
-  f.col = c->scol;
-  f.wh = c->swh;
-  f.hsum = c->hsum;
-  f.vol = c->vol;
+  f.off = c->L.off;
+  f.kcnt = c->L.cnt;
+  f.col = c->L.col;
+  f.wh = c->L.wh;
+  f.hsum = c->L.hsum;
+  f.vol = c->L.vol;
   f.U = c->U;
   f.Uf = c->Uf;
   f.Gout = c->GA;
   f.partial = pm;
-  k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
+  if (SV_COOP_GATHER) k_filter_coop<FORM><<<nb, BS, 0, c->stream>>>(f);
+  else k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
   mark();
   k_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pm, pe, c->ctrl);
   mark();
@@ -870,7 +940,7 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   ap.Uf = c->Uf;
   ap.U = c->U;
   ap.G = c->GA;
-  ap.s = c->s;
+  ap.s = c->L.s;
   ap.ctrl = c->ctrl;
   ap.pcfl = pc;
   ap.pvisc = pv;
@@ -892,7 +962,7 @@ int sv_dev_init_partials(sv_ctx *c) {
   ap.Uf = c->U;
   ap.U = c->U;
   ap.G = c->GA;
-  ap.s = c->s;
+  ap.s = c->L.s;
   ap.ctrl = nullptr;
   ap.pcfl = c->partial + 2 * nb;
   ap.pvisc = c->partial + 3 * nb;

@@ -585,6 +585,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   c->have_sums = false;
   c->have_filter = false;
   c->sell_dirty = true;
+  c->L.valid = false;
   c->n_singular = nsing;
   return 0;
 }
@@ -597,6 +598,7 @@ int sv_dev_sums(sv_ctx *c) {
   SV_HIP(c, hipStreamSynchronize(c->stream));
   c->have_sums = true;
   c->sell_dirty = true;
+  c->L.valid = false;
   return 0;
 }
 
@@ -608,5 +610,6 @@ int sv_dev_filter_coeffs(sv_ctx *c) {
   SV_HIP(c, hipStreamSynchronize(c->stream));
   c->have_filter = true;
   c->sell_dirty = true;
+  c->L.valid = false;
   return 0;
 }

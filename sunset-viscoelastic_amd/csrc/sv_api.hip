@@ -49,6 +49,11 @@ int sv_ensure_sell(sv_ctx *c) {
   return sv_dev_build_sell(c);
 }
 
+int sv_ensure_loop(sv_ctx *c) {
+  if (c->L.valid) return 0;
+  return sv_dev_build_loop(c);
+}
+
 static int set_device(sv_ctx *c) {
   SV_HIP(c, hipSetDevice(c->device));
   return 0;
@@ -143,6 +148,7 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   c->U = c->Uf = c->reg = c->eacc = nullptr;
   c->GA = c->GB = nullptr;
   c->segk = nullptr; c->blk_seg = nullptr; c->hseg.clear();
+  c->L = sv_loopl{};
   c->tmp_phi = c->tmp_out = nullptr; c->tmp_cap = 0;
   if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
   c->graph_valid = false;
@@ -307,6 +313,11 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
 int sv_set_option(sv_ctx *c, const char *key, double value) {
   SV_CHECK(c, c && key, "null argument");
   if (!strcmp(key, "weights_persistent")) c->weights_persistent = value != 0.0;
+  else if (!strcmp(key, "loop_order")) {
+    SV_CHECK(c, !c->have_state, "loop_order must be set before sv_set_initial");
+    c->loop_order = value != 0.0 ? 1 : 0;
+    c->L.valid = false;
+  }
   else return sv_fail(c, SV_E_ARG, "unknown option '%s'", key);
   return 0;
 }
@@ -433,7 +444,8 @@ static int upload_segment(sv_ctx *c) {
   double tv = 0.0;
   for (int i = 0; i < n; i++) tv += vol[i];
   c->volsum = tv;
-  for (int i = 0; i < n; i++) fx[i] = c->pk.kol_amp * sin(c->pk.kol_k * y[i]);
+  const std::vector<int> &P = c->L.hperm;   // loop order
+  for (int k = 0; k < n; k++) fx[k] = c->pk.kol_amp * sin(c->pk.kol_k * y[P[k]]);
   SV_HIP(c, hipMemcpy(c->fx, fx.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   c->hseg.assign(1, sv_segk{});
   sv_segk &S = c->hseg[0];
@@ -492,7 +504,7 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
   SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
   if (int rc = set_device(c)) return rc;
-  if (int rc = ensure_sell(c)) return rc;
+  if (int rc = sv_ensure_loop(c)) return rc;
   if (int rc = ensure_state_buffers(c)) return rc;
   const int n = c->npfb;
   std::vector<double> a(n), b(n), d(n), e(n), f(n), g(n);
@@ -510,8 +522,10 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
     }
   }
   std::vector<double> rec((size_t)SV_US * n, 0.0);
-  for (int i = 0; i < n; i++) {
-    double *r = rec.data() + (size_t)SV_US * i;
+  const std::vector<int> &P = c->L.hperm;   // record k holds the caller's node P[k]
+  for (int k = 0; k < n; k++) {
+    double *r = rec.data() + (size_t)SV_US * k;
+    const int i = P[k];
     r[0] = a[i]; r[1] = b[i]; r[2] = d[i]; r[3] = e[i]; r[4] = f[i]; r[5] = g[i];
   }
   SV_HIP(c, hipMemcpy(c->U, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
@@ -580,10 +594,15 @@ int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *
   int rc = sv_dev_rhs(c, iRKstep, out);
   if (!rc) {
     double *dst[6] = {r0, r1, r2, r3, r4, r5};
+    const int n = c->npfb;
+    const std::vector<int> &P = c->L.hperm;   // loop order -> caller's
+    std::vector<double> tmp(n);
     for (int f = 0; f < 6; f++)
-      if (dst[f])
-        SV_HIP(c, hipMemcpy(dst[f], out + (size_t)f * c->npfb, sizeof(double) * c->npfb,
+      if (dst[f]) {
+        SV_HIP(c, hipMemcpy(tmp.data(), out + (size_t)f * n, sizeof(double) * n,
                             hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; k++) dst[f][P[k]] = tmp[k];
+      }
   }
   (void)hipFree(out);
   return rc;
@@ -663,7 +682,7 @@ int sv_get_stencil_stats(sv_ctx *c, double *stats) {
   stats[0] = a / c->npfb;          // mean ij_count
   stats[1] = b / c->npfb;          // mean ij_count_small
   stats[2] = mx;                   // max ij_count
-  stats[3] = (double)c->sell_nnz;  // SELL entries incl. padding (0 before the first use)
+  stats[3] = (double)(c->L.valid ? c->L.nnz : c->sell_nnz);  // SELL entries incl. padding
   stats[4] = c->np - c->npfb;      // ghosts
   return 0;
 }
@@ -687,6 +706,8 @@ int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, do
   const sv_segk &S = c->hseg[member];
   const double *U = c->U + (size_t)S.base * SV_US;
   const double *GA = c->GA + (size_t)S.base * c->gs;
+  const std::vector<int> &P = c->is_ensemble ? c->member_perm[member] : c->L.hperm;
+  std::vector<double> tmp(S.npfb);
   SV_HIP(c, hipStreamSynchronize(c->stream));
   // strided copies out of the records
   const bool lc = c->cfg.form == SV_FORM_LC;
@@ -698,9 +719,11 @@ int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, do
       {cxx, GA + SV_G_CXX, c->gs}, {cxy, GA + SV_G_CXY, c->gs},
       {cyy, GA + SV_G_CYY, c->gs}};
   for (auto &e : m)
-    if (e.dst && e.src)
-      SV_HIP(c, hipMemcpy2D(e.dst, sizeof(double), e.src, sizeof(double) * e.stride,
+    if (e.dst && e.src) {
+      SV_HIP(c, hipMemcpy2D(tmp.data(), sizeof(double), e.src, sizeof(double) * e.stride,
                             sizeof(double), S.npfb, hipMemcpyDeviceToHost));
+      for (int k = 0; k < S.npfb; k++) e.dst[P[k]] = tmp[k];   // loop order -> caller's
+    }
   return 0;
 }
 

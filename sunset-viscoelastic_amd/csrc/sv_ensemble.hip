@@ -55,7 +55,7 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   for (int m = 0; m < nmembers; m++) {
     sv_ctx *c = members[m];
     if (hipSetDevice(c->device) != hipSuccess) return fail(SV_E_HIP);
-    if (int rc = sv_ensure_sell(c)) return fail(rc);
+    if (int rc = sv_ensure_loop(c)) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SV_E_HIP);
     sv_segk &S = E->hseg[m];
     S.base = (int)nodes;
@@ -67,7 +67,7 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     sell_base[m] = sell;
     blocks += S.nblk;
     nodes += (int64_t)S.nblk * SV_BLOCK;
-    sell += c->sell_nnz;
+    sell += c->L.nnz;
     E->any_forcing |= c->pk.kol_amp != 0.0;
     E->any_mdiff |= c->pk.Mdiff != 0.0;
   }
@@ -79,17 +79,18 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   E->ntot = (int)nodes;
   E->nblk = blocks;
   E->npfb = E->ntot;
-  E->nslices = E->ntot / SV_WAVE;
-  E->sell_nnz = sell;
+  E->L.nslices = E->ntot / SV_WAVE;
+  E->L.nnz = sell;
   const size_t N = (size_t)E->ntot, D = sizeof(double), I = sizeof(int);
-  void **bufs[] = {(void **)&E->sell_off, (void **)&E->cnt, (void **)&E->cnt_small,
-                   (void **)&E->scol, (void **)&E->swx, (void **)&E->swy, (void **)&E->swl,
-                   (void **)&E->swh, (void **)&E->gsx, (void **)&E->gsy, (void **)&E->lsum,
-                   (void **)&E->hsum, (void **)&E->vol, (void **)&E->s, (void **)&E->fx,
+  sv_loopl &EL = E->L;
+  void **bufs[] = {(void **)&EL.off, (void **)&EL.cnt, (void **)&EL.cnt_small,
+                   (void **)&EL.col, (void **)&EL.wx, (void **)&EL.wy, (void **)&EL.wl,
+                   (void **)&EL.wh, (void **)&EL.gsx, (void **)&EL.gsy, (void **)&EL.lsum,
+                   (void **)&EL.hsum, (void **)&EL.vol, (void **)&EL.s, (void **)&E->fx,
                    (void **)&E->U, (void **)&E->Uf, (void **)&E->reg, (void **)&E->eacc,
                    (void **)&E->GA, (void **)&E->GB, (void **)&E->partial, (void **)&E->ctrl,
                    (void **)&E->segk, (void **)&E->blk_seg};
-  const size_t sizes[] = {sizeof(int64_t) * (E->nslices + 1), I * N, I * N,
+  const size_t sizes[] = {sizeof(int64_t) * (EL.nslices + 1), I * N, I * N,
                           I * (size_t)sell, D * (size_t)sell, D * (size_t)sell,
                           D * (size_t)sell, D * (size_t)sell, D * N, D * N, D * N, D * N, D * N,
                           D * N, D * N, D * SV_US * N, D * SV_US * N, D * SV_US * N,
@@ -99,11 +100,11 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   for (size_t k = 0; k < sizeof(sizes) / sizeof(sizes[0]); k++)
     if (sv_alloc(E, bufs[k], sizes[k])) return fail(SV_E_NOMEM);
   E->npartial = 4 * blocks;
-  if (hipMemsetAsync(E->sell_off, 0, sizes[0], E->stream) != hipSuccess ||
-      hipMemsetAsync(E->cnt, 0, I * N, E->stream) != hipSuccess ||
-      hipMemsetAsync(E->cnt_small, 0, I * N, E->stream) != hipSuccess ||
+  if (hipMemsetAsync(EL.off, 0, sizes[0], E->stream) != hipSuccess ||
+      hipMemsetAsync(EL.cnt, 0, I * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(EL.cnt_small, 0, I * N, E->stream) != hipSuccess ||
       hipMemsetAsync(E->fx, 0, D * N, E->stream) != hipSuccess ||
-      hipMemsetAsync(E->s, 0, D * N, E->stream) != hipSuccess ||
+      hipMemsetAsync(EL.s, 0, D * N, E->stream) != hipSuccess ||
       hipMemsetAsync(E->U, 0, D * SV_US * N, E->stream) != hipSuccess ||
       hipMemsetAsync(E->GA, 0, D * E->gs * N, E->stream) != hipSuccess ||
       hipMemsetAsync(E->GB, 0, D * E->gs * N, E->stream) != hipSuccess)
@@ -115,19 +116,20 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     const size_t n = (size_t)S.npfb, b = (size_t)S.base;
     const int64_t sb = sell_base[m];
     for (int k = 0; k < S.nblk; k++) bseg[S.blk0 + k] = m;
-    int rc = copy_d2d(E, E->sell_off + b / SV_WAVE, c->sell_off,
-                      sizeof(int64_t) * (size_t)c->nslices);
-    if (!rc) rc = sv_dev_offset_i64(E, E->sell_off + b / SV_WAVE, c->nslices, sb);
-    if (!rc) rc = copy_d2d(E, E->scol + sb, c->scol, I * (size_t)c->sell_nnz);
-    if (!rc) rc = sv_dev_offset_int(E, E->scol + sb, c->sell_nnz, S.base);
-    const double *wsrc[4] = {c->swx, c->swy, c->swl, c->swh};
-    double *wdst[4] = {E->swx, E->swy, E->swl, E->swh};
+    const sv_loopl &CL = c->L;
+    E->member_perm.push_back(CL.hperm);
+    int rc = copy_d2d(E, EL.off + b / SV_WAVE, CL.off, sizeof(int64_t) * (size_t)CL.nslices);
+    if (!rc) rc = sv_dev_offset_i64(E, EL.off + b / SV_WAVE, CL.nslices, sb);
+    if (!rc) rc = copy_d2d(E, EL.col + sb, CL.col, I * (size_t)CL.nnz);
+    if (!rc) rc = sv_dev_offset_int(E, EL.col + sb, CL.nnz, S.base);
+    const double *wsrc[4] = {CL.wx, CL.wy, CL.wl, CL.wh};
+    double *wdst[4] = {EL.wx, EL.wy, EL.wl, EL.wh};
     for (int k = 0; k < 4 && !rc; k++)
-      rc = copy_d2d(E, wdst[k] + sb, wsrc[k], D * (size_t)c->sell_nnz);
-    if (!rc) rc = copy_d2d(E, E->cnt + b, c->cnt, I * n);
-    if (!rc) rc = copy_d2d(E, E->cnt_small + b, c->cnt_small, I * n);
-    const double *nsrc[7] = {c->gsx, c->gsy, c->lsum, c->hsum, c->vol, c->s, c->fx};
-    double *ndst[7] = {E->gsx, E->gsy, E->lsum, E->hsum, E->vol, E->s, E->fx};
+      rc = copy_d2d(E, wdst[k] + sb, wsrc[k], D * (size_t)CL.nnz);
+    if (!rc) rc = copy_d2d(E, EL.cnt + b, CL.cnt, I * n);
+    if (!rc) rc = copy_d2d(E, EL.cnt_small + b, CL.cnt_small, I * n);
+    const double *nsrc[7] = {CL.gsx, CL.gsy, CL.lsum, CL.hsum, CL.vol, CL.s, c->fx};
+    double *ndst[7] = {EL.gsx, EL.gsy, EL.lsum, EL.hsum, EL.vol, EL.s, E->fx};
     for (int k = 0; k < 7 && !rc; k++) rc = copy_d2d(E, ndst[k] + b, nsrc[k], D * n);
     if (!rc) rc = copy_d2d(E, E->U + b * SV_US, c->U, D * SV_US * n);
     if (!rc) rc = copy_d2d(E, E->GA + b * E->gs, c->GA, D * E->gs * n);
@@ -141,6 +143,7 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
       hipStreamSynchronize(E->stream) != hipSuccess)
     return fail(SV_E_HIP);
   E->sell_dirty = false;
+  EL.valid = true;
   E->have_phys = true;
   E->have_state = true;
   if (int rc = sv_dev_init_partials(E)) return fail(rc);

@@ -100,6 +100,27 @@ struct sv_segk {
   sv_physk pk;
 };
 
+// Time-loop layout: the arrays the RK sweeps read, with the real nodes in
+// "loop order" -- a Morton (Z-order) sort of the positions, so a block of 256
+// consecutive loop slots is a compact patch of the domain and the neighbour
+// records it gathers are shared across its nodes in the caches.  perm[k] is
+// the caller's node in loop slot k, iperm its inverse.  SELL-64 over loop
+// slots; columns are loop slots of the (ghost-folded) neighbours.  Per-node
+// arrays are in loop order; the drop-in operators keep the caller's order.
+struct sv_loopl {
+  bool valid = false;
+  int nslices = 0;
+  int64_t nnz = 0;
+  int64_t *off = nullptr;
+  int *col = nullptr;
+  double *wx = nullptr, *wy = nullptr, *wl = nullptr, *wh = nullptr;
+  int *cnt = nullptr, *cnt_small = nullptr;
+  double *gsx = nullptr, *gsy = nullptr, *lsum = nullptr, *hsum = nullptr, *vol = nullptr;
+  double *s = nullptr;
+  int *perm = nullptr, *iperm = nullptr;
+  std::vector<int> hperm;
+};
+
 struct sv_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -153,6 +174,10 @@ struct sv_ctx {
   int *blk_seg = nullptr;   // [nblk] segment of each logical block
   bool is_ensemble = false;
   sv_adj *adj = nullptr;
+  sv_loopl L;
+  int loop_order = 1;       // 1: Morton order for the time loop, 0: the caller's order
+  // ensemble: loop-slot -> member-node permutation of each member
+  std::vector<std::vector<int>> member_perm;
   bool any_forcing = false, any_mdiff = false;  // over segments
   std::vector<sv_segk> hseg;
   // scratch for drop-in ops
@@ -183,6 +208,9 @@ void sv_free(sv_ctx *c, void *p);
 int sv_dev_create_ghosts(sv_ctx *c);
 int sv_dev_find_neighbours(sv_ctx *c);
 int sv_dev_build_sell(sv_ctx *c);
+// time-loop layout from the final weights (after sums and filter coefficients)
+int sv_dev_build_loop(sv_ctx *c);
+int sv_ensure_loop(sv_ctx *c);
 // ---- kernels: weights (k_weights.hip) ----
 int sv_dev_weights(sv_ctx *c, const int *order_map_host);
 int sv_dev_sums(sv_ctx *c);

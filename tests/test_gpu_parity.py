@@ -211,3 +211,29 @@ def test_ipart_fixture_lists_and_weights(sunset):
     ptr = np.concatenate([[0], np.cumsum(cnt)])
     pack = lambda w, c: np.concatenate([w[i, :c[i]] for i in range(len(c))])
     assert per_node_rel(ptr, pack(W["w_lap"], cnt), pack(Wo["w_lap"], oc)) < WEIGHT_RTOL
+
+
+def test_loop_order_does_not_change_results(sunset):
+    """The time loop runs its nodes in Morton order (cache locality); the
+    caller's order gives the same fields up to the summation order of the mass
+    correction (one reduction per step)."""
+    g = load_golden("m6_20")
+    ns = golden_nodeset(g)
+    out = []
+    for order in (0, 1):
+        s = sunset.Solver(morder=6, form="lc")
+        assert lib_set_option(s, "loop_order", order) == 0
+        s.setup(ns)
+        s.set_phys(**golden_phys(g))
+        s.set_initial(taylor_green_ic(ns))
+        s.run(10)
+        out.append(s.fields())
+        s.close()
+    for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy", "cxx"):
+        assert field_rel(out[0][k], out[1][k]) < 1e-12, k
+    assert out[0]["dt"] == out[1]["dt"]
+
+
+def lib_set_option(s, key, value):
+    import sunset_amd
+    return sunset_amd.lib().sv_set_option(s.h, key.encode(), float(value))
