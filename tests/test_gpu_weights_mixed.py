@@ -3,10 +3,11 @@ interior, 4 within three local spacings of any wall), through the C ABI.
 
 Small instance (1.5k nodes): neighbour lists bit-exact vs the oracle; weights
 within 1e-10 per node (max_k |dw| / max_k |w|) of the oracle's, except on
-ill-conditioned moment systems (one-sided order-8 stencils near walls, cond(A)
-up to ~1e13), where the oracle's own error against a 40-digit solve of the same
-system reaches ~1e-8: there both are compared with the 40-digit solution and
-the device must be within max(1e-10, 2x the oracle's error) of it.  The
+ill-conditioned moment systems (one-sided stencils near walls, cond(A) up to
+~1e13; ~9% of the nodes here), where the oracle's own error against a 40-digit
+solve of the same system reaches ~1e-8: for the 12 largest discrepancies and 12
+more drawn at random, both are compared with the 40-digit solution and the
+device must be within max(1e-10, 2x the oracle's error) of it.  The
 one-block-per-system and the persistent work-queue launches must agree bit for
 bit.  Full instance (s = 0.004 .. 0.016, ~126k nodes): size-independent checks
 -- no singular system, persistent == plain bitwise, every sampled node
@@ -99,9 +100,13 @@ def test_small_channel_lists_and_weights_vs_oracle():
             hard = np.where(rel >= WEIGHT_RTOL)[0]
             print(f"persistent={persistent} weights {name}: max per-node rel {rel.max():.3e}, "
                   f"{len(hard)} nodes above {WEIGHT_RTOL:g}")
-            assert len(hard) <= 0.05 * ns.npfb, (persistent, name, len(hard))
-            if name == "x":
-                for i in hard[np.argsort(rel[hard])[::-1][:12]]:
+            assert rel.max() < 1e-7, (persistent, name)
+            if name == "x" and len(hard):
+                # the 12 largest discrepancies and 12 more drawn at random
+                worst = hard[np.argsort(rel[hard])[::-1][:12]]
+                rest = np.setdiff1d(hard, worst)
+                pick = np.random.default_rng(0).choice(rest, min(12, len(rest)), replace=False)
+                for i in np.concatenate([worst, pick]):
                     wex = _exact_dx_weights(ns, x, y, link, cs, i, int(om[i]))
                     e_dev = np.abs(a[i, : cs[i]] - wex).max() / np.abs(wex).max()
                     e_ora = np.abs(b[i, : cs[i]] - wex).max() / np.abs(wex).max()
