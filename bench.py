@@ -150,6 +150,8 @@ def ensemble_leg(members=32, steps=100, seed=7):
     def build():
         out = []
         for k, (side, order, Wi, beta) in enumerate(specs):
+            print(f"[bench] ensemble member {k}: {side}^2 nodes, order {order}",
+                  file=sys.stderr, flush=True)
             ns = jittered_lattice(side, seed=100 + k, h_over_s={4: 1.8, 6: 2.4}[order])
             s = sunset_amd.Solver(morder=order, form="lc").setup(ns)
             s.set_phys(Re=1.0, Wi=Wi, beta=beta, eps=0.0, Ma=0.1, kol_amp=1.0,
@@ -160,18 +162,20 @@ def ensemble_leg(members=32, steps=100, seed=7):
 
     mem = build()
     nodes = sum(s.npfb for s in mem)
+    print(f"[bench] ensemble: {members} members set up, {nodes} nodes; members one by one",
+          file=sys.stderr, flush=True)
     t_seq = 0.0
     for s in mem:
         s.run(2, graph=True)
         s.run(steps, graph=True)
         t_seq += s.timing("step")
-    for s in mem:
-        s.close()
-    mem = build()
-    E = sunset_amd.Ensemble(mem)
+    print(f"[bench] ensemble: sequential {t_seq:.1f} ms; one ensemble context",
+          file=sys.stderr, flush=True)
+    E = sunset_amd.Ensemble(mem)   # copies the members' current states
     E.run(2, graph=True)
     E.run(steps, graph=True)
     t_ens = E.step_ms()
+    print(f"[bench] ensemble: {t_ens:.1f} ms for {steps} steps", file=sys.stderr, flush=True)
     ok = all(np.isfinite(E.fields(k)["ro"]).all() for k in range(len(mem)))
     E.close()
     for s in mem:

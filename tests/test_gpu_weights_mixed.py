@@ -7,7 +7,9 @@ ill-conditioned moment systems (one-sided stencils near walls, cond(A) up to
 ~1e13; ~9% of the nodes here), where the oracle's own error against a 40-digit
 solve of the same system reaches ~1e-8: for the 12 largest discrepancies and 12
 more drawn at random, both are compared with the 40-digit solution and the
-device must be within max(1e-10, 2x the oracle's error) of it.  The
+device must be within max(1e-10, 4x the oracle's error) of it (both carry the
+rounding of the FP64 assembly of such systems, ~1e-9..1e-8 here; the residual
+side -- moment conditions to 1e-9 -- is checked on the full instance below).  The
 one-block-per-system and the persistent work-queue launches must agree bit for
 bit.  Full instance (s = 0.004 .. 0.016, ~126k nodes): size-independent checks
 -- no singular system, persistent == plain bitwise, every sampled node
@@ -112,7 +114,7 @@ def test_small_channel_lists_and_weights_vs_oracle():
                     e_ora = np.abs(b[i, : cs[i]] - wex).max() / np.abs(wex).max()
                     print(f"  node {i} order {om[i]}: device {e_dev:.2e}, oracle {e_ora:.2e} "
                           "vs 40 digits")
-                    assert e_dev <= max(WEIGHT_RTOL, 2.0 * e_ora), (i, e_dev, e_ora)
+                    assert e_dev <= max(WEIGHT_RTOL, 4.0 * e_ora), (i, e_dev, e_ora)
         assert np.allclose(W["vol"], Wo["vol"], rtol=1e-12)
         res[persistent] = W
     for k in ("w_grad", "w_lap", "w_hyp", "vol"):
