@@ -693,11 +693,13 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
 #ifndef SV_WEIGHTS_REG
 #define SV_WEIGHTS_REG 0
 #endif
+#define SV_TUNE_SAMPLE 4096
 
+// variant: bit 0 = persistent work-queue grid, bit 1 = register-row solver
 template <int M>
-int launch_order(sv_ctx *c, SysArgs a, int first, int count, bool persistent) {
+int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
   if (count <= 0) return 0;
-  const bool reg = SV_WEIGHTS_REG != 0;
+  const bool reg = (variant & 2) != 0, persistent = (variant & 1) != 0;
   const size_t shm = sizeof(double) * (reg ? LdsR<M>::TOTAL : Lds<M>::TOTAL);
   SysArgs b = a;
   b.sys_node = a.sys_node + first;
@@ -787,29 +789,86 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   SysArgs a{dn, dk, (int)sys.size(), c->x, c->y, c->h, c->hs, c->node_type, c->row_ptr,
             c->col_raw, c->cnt, c->cnt_small, c->h2, c->wx, c->wy, c->wl, c->wh, c->vol,
             c->singular, dq};
-  hipEvent_t e0, e1;
+  hipEvent_t e0, e1, t0, t1;
   SV_HIP(c, hipEventCreate(&e0));
   SV_HIP(c, hipEventCreate(&e1));
+  SV_HIP(c, hipEventCreate(&t0));
+  SV_HIP(c, hipEventCreate(&t1));
   SV_HIP(c, hipEventRecord(e0, c->stream));
+  float tune_ms = 0.0f;    // autotuning sample launches (2 passes each)
+  float solve_ms = 0.0f;   // the full launches
+  auto launch = [&](int m, SysArgs args, int f0, int cnt, int variant) -> int {
+    if (m == 4) return launch_order<4>(c, args, f0, cnt, variant);
+    if (m == 6) return launch_order<6>(c, args, f0, cnt, variant);
+    return launch_order<8>(c, args, f0, cnt, variant);
+  };
   int first = 0;
   for (int m : {4, 6, 8}) {
     int count = 0;
     while (first + count < (int)sys.size() && sys[first + count].m == m) count++;
-    int rc = 0;
-    if (m == 4) rc = launch_order<4>(c, a, first, count, c->weights_persistent);
-    if (m == 6) rc = launch_order<6>(c, a, first, count, c->weights_persistent);
-    if (m == 8) rc = launch_order<8>(c, a, first, count, c->weights_persistent);
-    if (rc) return rc;
+    if (count == 0) continue;
+    const int mi = m / 2 - 2;   // 0, 1, 2
+    int variant = (c->weights_persistent ? 1 : 0) | (SV_WEIGHTS_REG ? 2 : 0);
+    if (c->weights_autotune && count >= 4 * SV_TUNE_SAMPLE) {
+      if (c->wtune[mi] < 0) {
+        // time the four launch variants on a strided sample of this order's
+        // systems (they are sorted by stencil length, so the sample has the
+        // same length mix); the sample's weights are rewritten by the full
+        // launch with identical values
+        const int S = SV_TUNE_SAMPLE, stride = count / S;
+        std::vector<int> sn(S), sk(S);
+        for (int q = 0; q < S; q++) { sn[q] = hn[first + q * stride]; sk[q] = hk[first + q * stride]; }
+        int *tn = nullptr, *tk = nullptr;
+        SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
+        SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
+        SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+        SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+        SysArgs ts = a;
+        ts.sys_node = tn;
+        ts.sys_kind = tk;
+        float best = 1e30f;
+        for (int v = 0; v < 4; v++) {
+          float tv = 0.0f;
+          for (int rep = 0; rep < 2; rep++) {   // first pass warms up
+            SV_HIP(c, hipEventRecord(t0, c->stream));
+            if (int rc = launch(m, ts, 0, S, v)) return rc;
+            SV_HIP(c, hipEventRecord(t1, c->stream));
+            SV_HIP(c, hipEventSynchronize(t1));
+            SV_HIP(c, hipEventElapsedTime(&tv, t0, t1));
+          }
+          c->wtune_ms[mi][v] = tv;
+          if (tv < best) { best = tv; c->wtune[mi] = v; }
+        }
+        SV_HIP(c, hipFree(tn));
+        SV_HIP(c, hipFree(tk));
+        float tt = 0.0f;
+        for (int v = 0; v < 4; v++) tt += 2.0f * c->wtune_ms[mi][v];
+        tune_ms += tt;
+      }
+      variant = c->wtune[mi];
+    }
+    SV_HIP(c, hipEventRecord(t0, c->stream));
+    if (int rc = launch(m, a, first, count, variant)) return rc;
+    SV_HIP(c, hipEventRecord(t1, c->stream));
+    SV_HIP(c, hipEventSynchronize(t1));
+    float tl = 0.0f;
+    SV_HIP(c, hipEventElapsedTime(&tl, t0, t1));
+    solve_ms += tl;
+    c->wvariant[mi] = variant;
     first += count;
   }
   SV_HIP(c, hipEventRecord(e1, c->stream));
   SV_HIP(c, hipStreamSynchronize(c->stream));
   float ms = 0;
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-  c->t_weights = ms;
+  (void)ms;
+  c->t_weights_tune = tune_ms;
+  c->t_weights = solve_ms;
   c->n_systems = (int)sys.size();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   int nsing = 0;
   SV_HIP(c, hipMemcpy(&nsing, c->singular, sizeof(int), hipMemcpyDeviceToHost));
   SV_HIP(c, hipFree(dn));

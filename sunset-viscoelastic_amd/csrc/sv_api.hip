@@ -304,15 +304,29 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
   SV_CHECK(c, c && what && ms, "null argument");
   if (!strcmp(what, "neighbours")) *ms = c->t_neigh;
   else if (!strcmp(what, "weights")) *ms = c->t_weights;
+  else if (!strcmp(what, "weights_tune")) *ms = c->t_weights_tune;
   else if (!strcmp(what, "step")) *ms = c->t_step;
   else if (!strcmp(what, "systems")) *ms = c->n_systems;
+  else if (!strncmp(what, "weights_variant_m", 17) && (what[17] == '4' || what[17] == '6' ||
+                                                        what[17] == '8'))
+    *ms = c->wvariant[(what[17] - '0') / 2 - 2];
+  else if (!strncmp(what, "weights_tune_m", 14) && strlen(what) == 17 && what[15] == '_' &&
+           (what[14] == '4' || what[14] == '6' || what[14] == '8') && what[16] >= '0' &&
+           what[16] <= '3')
+    *ms = c->wtune_ms[(what[14] - '0') / 2 - 2][what[16] - '0'];
   else return sv_fail(c, SV_E_ARG, "unknown timing '%s'", what);
   return 0;
 }
 
 int sv_set_option(sv_ctx *c, const char *key, double value) {
   SV_CHECK(c, c && key, "null argument");
-  if (!strcmp(key, "weights_persistent")) c->weights_persistent = value != 0.0;
+  if (!strcmp(key, "weights_persistent")) {
+    c->weights_persistent = value != 0.0;
+    c->weights_autotune = false;   // an explicit choice turns the autotuning off
+  } else if (!strcmp(key, "weights_autotune")) {
+    c->weights_autotune = value != 0.0;
+    for (int k = 0; k < 3; k++) c->wtune[k] = -1;
+  }
   else if (!strcmp(key, "loop_order")) {
     SV_CHECK(c, !c->have_state, "loop_order must be set before sv_set_initial");
     c->loop_order = value != 0.0 ? 1 : 0;
@@ -707,23 +721,24 @@ int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, do
   const double *U = c->U + (size_t)S.base * SV_US;
   const double *GA = c->GA + (size_t)S.base * c->gs;
   const std::vector<int> &P = c->is_ensemble ? c->member_perm[member] : c->L.hperm;
-  std::vector<double> tmp(S.npfb);
+  const size_t n = (size_t)S.npfb;
   SV_HIP(c, hipStreamSynchronize(c->stream));
-  // strided copies out of the records
+  // two contiguous copies of the member's records, fields picked on the host
+  // (a strided copy of 8-byte rows is far slower than the whole record array)
+  std::vector<double> hu(n * SV_US), hg(n * c->gs);
+  SV_HIP(c, hipMemcpy(hu.data(), U, sizeof(double) * hu.size(), hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(hg.data(), GA, sizeof(double) * hg.size(), hipMemcpyDeviceToHost));
   const bool lc = c->cfg.form == SV_FORM_LC;
   struct { double *dst; const double *src; int stride; } m[] = {
-      {ro, U + 0, SV_US}, {rou, U + 1, SV_US}, {rov, U + 2, SV_US},
-      {u, GA + SV_G_U, c->gs}, {v, GA + SV_G_V, c->gs},
-      {pxx, lc ? U + 3 : nullptr, SV_US}, {pxy, lc ? U + 4 : nullptr, SV_US},
-      {pyy, lc ? U + 5 : nullptr, SV_US},
-      {cxx, GA + SV_G_CXX, c->gs}, {cxy, GA + SV_G_CXY, c->gs},
-      {cyy, GA + SV_G_CYY, c->gs}};
+      {ro, hu.data() + 0, SV_US}, {rou, hu.data() + 1, SV_US}, {rov, hu.data() + 2, SV_US},
+      {u, hg.data() + SV_G_U, c->gs}, {v, hg.data() + SV_G_V, c->gs},
+      {pxx, lc ? hu.data() + 3 : nullptr, SV_US}, {pxy, lc ? hu.data() + 4 : nullptr, SV_US},
+      {pyy, lc ? hu.data() + 5 : nullptr, SV_US},
+      {cxx, hg.data() + SV_G_CXX, c->gs}, {cxy, hg.data() + SV_G_CXY, c->gs},
+      {cyy, hg.data() + SV_G_CYY, c->gs}};
   for (auto &e : m)
-    if (e.dst && e.src) {
-      SV_HIP(c, hipMemcpy2D(tmp.data(), sizeof(double), e.src, sizeof(double) * e.stride,
-                            sizeof(double), S.npfb, hipMemcpyDeviceToHost));
-      for (int k = 0; k < S.npfb; k++) e.dst[P[k]] = tmp[k];   // loop order -> caller's
-    }
+    if (e.dst && e.src)
+      for (size_t k = 0; k < n; k++) e.dst[P[k]] = e.src[k * e.stride];   // loop order -> caller's
   return 0;
 }
 

@@ -157,6 +157,12 @@ struct sv_ctx {
   int *singular = nullptr;  // count of zero pivots in the last solve
   int n_singular = 0, n_systems = 0;
   bool weights_persistent = false;
+  // weight-solve launch autotuning per order (4, 6, 8): chosen variant (bit 0
+  // persistent grid, bit 1 register-row solver), -1 until tuned; sample times
+  bool weights_autotune = true;
+  int wtune[3] = {-1, -1, -1};
+  float wtune_ms[3][4] = {};
+  int wvariant[3] = {-1, -1, -1};   // variant of the last launch per order
   // physics/state
   bool have_phys = false, have_state = false;
   sv_physk pk{};
@@ -186,7 +192,7 @@ struct sv_ctx {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
   // timings (ms)
-  double t_neigh = 0, t_weights = 0, t_step = 0;
+  double t_neigh = 0, t_weights = 0, t_weights_tune = 0, t_step = 0;
   // device buffer registry for cleanup
   std::vector<void *> allocs;
 };

@@ -202,8 +202,13 @@ class Solver:
         self._ck(lib().sv_find_neighbours(self.h, L, None, None, _i(link)))
         return cnt, cs, link - 1
 
-    def calc_weights(self, order_map=None, persistent: bool = False):
-        self._ck(lib().sv_set_option(self.h, b"weights_persistent", 1.0 if persistent else 0.0))
+    def calc_weights(self, order_map=None, persistent: Optional[bool] = None):
+        """persistent=None: autotuned launch (default); True/False: fixed launch."""
+        if persistent is not None:
+            self._ck(lib().sv_set_option(self.h, b"weights_persistent",
+                                         1.0 if persistent else 0.0))
+        else:
+            self._ck(lib().sv_set_option(self.h, b"weights_autotune", 1.0))
         om = None if order_map is None else np.ascontiguousarray(order_map, dtype=np.int32)
         self._ck(lib().sv_calc_labf_weights(self.h, _i(om)))
 

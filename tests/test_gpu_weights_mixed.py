@@ -127,11 +127,12 @@ def test_full_channel_moment_conditions_and_launch_equivalence():
     ns, om = cylinder_channel()
     print(f"[mixed] full channel {ns.npfb} nodes", flush=True)
     out = {}
-    for persistent in (False, True):
+    for persistent in (False, True, None):      # None: autotuned launch
         out[persistent] = _device_weights(ns, om, persistent)
     cnt, cs, link, W, x, y = out[True]
     for k in ("w_grad", "w_lap", "w_hyp"):
         assert np.array_equal(out[False][3][k], W[k]), k
+        assert np.array_equal(out[None][3][k], W[k]), k
     rng = np.random.default_rng(5)
     worst = {4: 0.0, 8: 0.0}
     for i in rng.choice(ns.npfb, 400, replace=False):
