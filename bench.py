@@ -99,7 +99,8 @@ def c3_weights(smin, smax):
         out[f"{name}_ms"] = round(t, 2)
         out["systems"] = int(s.timing("systems"))
     out["autotune_ms"] = round(s.timing("weights_tune"), 2)
-    out["autotuned_variants"] = {f"order{m}": int(s.timing(f"weights_variant_m{m}"))
+    out["autotuned_variants"] = {f"order{m}": ("persistent" if s.timing(f"weights_variant_m{m}")
+                                               == 1 else "one block per system")
                                  for m in (4, 8)}
     s.calc_sums()
     s.filter_coefficients()

@@ -38,7 +38,7 @@ constexpr int BS = SV_BLOCK;   // threads per block of the node kernels
 #define SV_GATHER_UNROLL 2
 #endif
 #ifndef SV_STREAM_NT
-#define SV_STREAM_NT 0
+#define SV_STREAM_NT 1
 #endif
 
 template <typename T>
@@ -438,119 +438,6 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   }
 }
 
-// Cooperative-gather form of k_stage: in the neighbour sweep each node is
-// served by LPN lanes (8 for the 128-byte LC record, 4 for the 64-byte DI
-// record), lane q loading 16 bytes (slot q) of every neighbour record, so one
-// gather instruction of a wavefront touches 64/LPN cache lines instead of 64.
-// Each lane accumulates the six products of its two fields with (wx, wy, wl);
-// the sums a node needs go through LDS to one lane per node for the pointwise
-// part.  Summation order per sum is sequential in k as in k_stage, so the
-// results are bitwise the same.
-//   LC slots: (ro,u) (v,cxx) (cxy,cyy) (pxx,pxy) (pyy,-);  DI: (ro,u) (v,cxx) (cxy,cyy)
-template <int FORM>
-struct Coop {
-  static constexpr int LPN = FORM == SV_FORM_LC ? 8 : 4;   // lanes per node
-  static constexpr int NPR = SV_WAVE / LPN;                // nodes per round
-  static constexpr int NQ = FORM == SV_FORM_LC ? 5 : 3;    // record slots used
-  static constexpr int NSUM = FORM == SV_FORM_LC ? 18 : 17;
-  static constexpr int RS2 = (FORM == SV_FORM_LC ? SV_GS_LC : SV_GS_DI) / 2;
-};
-
-template <int FORM, int STAGE, bool MDIFF>
-__global__ __launch_bounds__(BS) void k_stage_coop(StageArgs a) {
-  using CP = Coop<FORM>;
-  __shared__ double sm[BS / SV_WAVE][SV_WAVE][CP::NSUM];
-  __shared__ double red[BS / SV_WAVE];
-  const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
-  const int lane = threadIdx.x & (SV_WAVE - 1), w = threadIdx.x / SV_WAVE;
-  const int g = lane / CP::LPN, q = lane % CP::LPN;
-  const sv_segk &SG = a.seg[nr.sg];
-  const int wlocal = (bid - SG.blk0) * BS + w * SV_WAVE;   // wave's first node, local
-  const int wbase = SG.base + wlocal;
-  const int64_t soff = wlocal < SG.npfb ? a.off[wbase >> 6] : 0;
-  const double2 *G2 = reinterpret_cast<const double2 *>(a.G);
-  const bool qa = q < CP::NQ;
-  for (int r = 0; r < CP::LPN; r++) {
-    const int l = r * CP::NPR + g;
-    const bool nact = wlocal + l < SG.npfb;
-    const int K = nact ? a.kcnt[wbase + l] : 0;
-    const int64_t b = soff + l;
-    double P0 = 0, P1 = 0, P2 = 0, P3 = 0, P4 = 0, P5 = 0;
-    constexpr int U = SV_GATHER_UNROLL;
-    int jn[U];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : wbase;
-    for (int k = 0; k < K; k += U) {
-      double2 d[U];
-      double wxk[U], wyk[U], wlk[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (k + u < K) {
-          const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
-          d[u] = qa ? G2[(size_t)jn[u] * CP::RS2 + q] : double2{0.0, 0.0};
-          wxk[u] = ld_stream(a.wx + e); wyk[u] = ld_stream(a.wy + e);
-          wlk[u] = ld_stream(a.wl + e);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int kn = k + U + u;
-        jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : wbase;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (k + u < K) {
-          P0 += d[u].x * wxk[u]; P1 += d[u].x * wyk[u]; P2 += d[u].x * wlk[u];
-          P3 += d[u].y * wxk[u]; P4 += d[u].y * wyk[u]; P5 += d[u].y * wlk[u];
-        }
-    }
-    if (nact && qa) {
-      double *o = sm[w][l];
-      if (FORM == SV_FORM_LC) {
-        if (q == 0) { o[0] = P0; o[1] = P1; o[2] = P3; o[3] = P4; o[4] = P5; }
-        else if (q == 1) { o[5] = P0; o[6] = P1; o[7] = P2; o[8] = P3; }
-        else if (q == 2) { o[9] = P0; o[10] = P1; o[11] = P4; }
-        else if (q == 3) { o[12] = P0; o[13] = P1; o[14] = P3; o[15] = P4; }
-        else { o[16] = P0; o[17] = P1; }
-      } else {
-        if (q == 0) { o[0] = P0; o[1] = P1; o[2] = P3; o[3] = P4; o[4] = P5; }
-        else if (q == 1) { o[5] = P0; o[6] = P1; o[7] = P2; o[8] = P3; o[9] = P4; o[10] = P5; }
-        else { o[11] = P0; o[12] = P1; o[13] = P2; o[14] = P3; o[15] = P4; o[16] = P5; }
-      }
-    }
-  }
-  __syncthreads();
-  double emax_loc = 0.0;
-  if (nr.act) {
-    const double *o = sm[w][lane];
-    Sums S{};
-    S.rx = o[0]; S.ry = o[1]; S.ux = o[2]; S.uy = o[3]; S.lu = o[4];
-    S.vx = o[5]; S.vy = o[6]; S.lv = o[7]; S.axx = o[8];
-    if (FORM == SV_FORM_LC) {
-      S.axy = o[9]; S.ayy = o[10]; S.byy = o[11];
-      S.q1x = o[12]; S.q1y = o[13]; S.q2x = o[14]; S.q2y = o[15]; S.q3x = o[16]; S.q3y = o[17];
-    } else {
-      S.ayx = o[9]; S.cxx_l = o[10];
-      S.axy = o[11]; S.ayy = o[12]; S.cxy_l = o[13]; S.bxy = o[14]; S.byy = o[15];
-      S.cyy_l = o[16];
-    }
-    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
-  }
-  if (STAGE == 4) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) emax_loc = fmax(emax_loc, __shfl_xor(emax_loc, o));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emax_loc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double m = red[0];
-      for (int v = 1; v < BS / SV_WAVE; v++) m = fmax(m, red[v]);
-      a.partial[bid] = m;
-    }
-  }
-}
-
 struct FilterArgs {
   const int *blk_seg;
   const sv_segk *seg;
@@ -624,69 +511,6 @@ __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
       }
   }
   const double m = act ? filter_point<FORM>(a, nr, t) : 0.0;
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = BS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) a.partial[bid] = red[0];
-}
-
-// Cooperative-gather form of k_filter (see k_stage_coop): 4 lanes per node,
-// lanes 0-2 each load 16 bytes (two fields) of every neighbour's state record.
-template <int FORM>
-__global__ __launch_bounds__(BS) void k_filter_coop(FilterArgs a) {
-  constexpr int LPN = 4, NPR = SV_WAVE / LPN, NQ = 3;
-  __shared__ double sm[BS / SV_WAVE][SV_WAVE][6];
-  __shared__ double red[BS];
-  const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
-  const int lane = threadIdx.x & (SV_WAVE - 1), w = threadIdx.x / SV_WAVE;
-  const int g = lane / LPN, q = lane % LPN;
-  const sv_segk &SG = a.seg[nr.sg];
-  const int wlocal = (bid - SG.blk0) * BS + w * SV_WAVE;
-  const int wbase = SG.base + wlocal;
-  const int64_t soff = wlocal < SG.npfb ? a.off[wbase >> 6] : 0;
-  const double2 *U2 = reinterpret_cast<const double2 *>(a.U);
-  const bool qa = q < NQ;
-  for (int r = 0; r < LPN; r++) {
-    const int l = r * NPR + g;
-    const bool nact = wlocal + l < SG.npfb;
-    const int K = nact ? a.kcnt[wbase + l] : 0;
-    const int64_t b = soff + l;
-    double P0 = 0, P1 = 0;
-    constexpr int U = SV_GATHER_UNROLL;
-    int jn[U];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      jn[u] = (u < K) ? ld_stream(a.col + b + (int64_t)u * SV_WAVE) : wbase;
-    for (int k = 0; k < K; k += U) {
-      double2 d[U];
-      double wk[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (k + u < K) {
-          d[u] = qa ? U2[(size_t)jn[u] * (SV_US / 2) + q] : double2{0.0, 0.0};
-          wk[u] = ld_stream(a.wh + b + (int64_t)(k + u) * SV_WAVE);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int kn = k + U + u;
-        jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : wbase;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (k + u < K) { P0 += d[u].x * wk[u]; P1 += d[u].y * wk[u]; }
-    }
-    if (nact && qa) { sm[w][l][2 * q] = P0; sm[w][l][2 * q + 1] = P1; }
-  }
-  __syncthreads();
-  double t[6];
-#pragma unroll
-  for (int f = 0; f < 6; f++) t[f] = nr.act ? sm[w][lane][f] : 0.0;
-  const double m = nr.act ? filter_point<FORM>(a, nr, t) : 0.0;
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = BS / 2; s > 0; s >>= 1) {
@@ -873,20 +697,12 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   return a;
 }
 
-#ifndef SV_COOP_GATHER
-#define SV_COOP_GATHER 0
-#endif
-
 template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
-  const bool coop = SV_COOP_GATHER != 0;
-  if (FORM == SV_FORM_DI && c->any_mdiff) {
-    if (coop) k_stage_coop<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
-    else k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
-  } else {
-    if (coop) k_stage_coop<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
-    else k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
-  }
+  if (FORM == SV_FORM_DI && c->any_mdiff)
+    k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
+  else
+    k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
 }
 
 // ev (nullable): SV_NKERNELS+1 events recorded between the launches of one step
@@ -928,8 +744,7 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   f.Uf = c->Uf;
   f.Gout = c->GA;
   f.partial = pm;
-  if (SV_COOP_GATHER) k_filter_coop<FORM><<<nb, BS, 0, c->stream>>>(f);
-  else k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
+  k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
   mark();
   k_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pm, pe, c->ctrl);
   mark();

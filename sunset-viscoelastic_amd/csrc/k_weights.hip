@@ -449,194 +449,6 @@ __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(Sy
   }
 }
 
-// ---- register-resident variant --------------------------------------------
-// The same factorisation and solves as solve_system, with the matrix held one
-// row per lane in VGPRs instead of LDS: partial pivoting keeps every row in its
-// lane and tracks positions (no row moves), the pivot row is broadcast with
-// readlane, and the triangular solves walk the pivot order.  Each update is
-// the same FP64 operation on the same operands in the same order as the LDS
-// version, so the results are identical; LDS holds only the assembly chunks
-// (then the original matrix for the residuals) and the solution, which lets
-// several times more systems run per CU.
-template <int M>
-struct LdsR {
-  static constexpr int N = nsize(M), NP = npad(M), LDA = NP + 1, CH = 32;
-  static constexpr int CHUNK = 2 * CH * NP, A0SZ = NP * LDA;
-  static constexpr int U0 = CHUNK > A0SZ ? CHUNK : A0SZ;
-  static constexpr int TOTAL = U0 + NP * 4;   // doubles
-};
-
-__device__ __forceinline__ double rdlane(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-template <int M>
-__device__ void solve_system_reg(const SysArgs &S_, int sys, double *lds, int *pl) {
-  using L = LdsR<M>;
-  constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH;
-  double *Xc = lds;                 // [CH][NP]
-  double *Gc = lds + CH * NP;       // [CH][NP]
-  double *A0 = lds;                 // [NP][LDA] after assembly
-  double *Xs = lds + L::U0;         // [NP][4] solution
-  const int l = threadIdx.x;
-  const int i = S_.sys_node[sys];
-  const int kind = S_.sys_kind[sys];
-  const bool sysD = kind & SYS_D, sysF = kind & SYS_F;
-  const double hh = sysD ? S_.hs[i] : S_.h[i];
-  const int Kuse = sysD ? S_.cnt_small[i] : S_.cnt[i];
-  const int Kfull = S_.cnt[i];
-  const int64_t base = S_.row_ptr[i];
-
-  assemble<M>(S_, i, hh, Kuse, base, Xc, Gc, A0);
-  // order reduction near walls by identity padding (labf.F90:126-148, :311-350)
-  const int nt = S_.node_type[i];
-  int nkeep = N, mf = M;
-  if (M >= 7 && (nt == 998 || nt < 0)) { nkeep = 27; mf = 6; }
-  if (M >= 5 && sysD && (nt == -1 || nt == -2)) nkeep = 14;
-  if (M >= 5 && sysF && nt == -1) { nkeep = 14; mf = 4; }
-  if (nkeep < N && l < N) {
-    for (int r = 0; r < N; r++)
-      if (r >= nkeep || l >= nkeep) A0[r * LDA + l] = (r == l) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  // this lane's row (row l, l < N) and right-hand sides:
-  // q0 d/dx, q1 d/dy, q2 Laplacian (e3+e5), q3 filter
-  double a[NP];
-#pragma unroll
-  for (int c = 0; c < NP; c++) a[c] = (l < N) ? A0[l * LDA + c] : 0.0;
-  double bq[4];
-  bq[0] = (l < N && sysD && l == 0) ? 1.0 : 0.0;
-  bq[1] = (l < N && sysD && l == 1) ? 1.0 : 0.0;
-  bq[2] = (l < N && sysD && (l == 2 || l == 4)) ? 1.0 : 0.0;
-  bq[3] = (l < N && sysF) ? hyp_rhs(mf, l) : 0.0;
-
-  // ---- LU with partial pivoting; pos = the row's position in the pivoted
-  // order (rows never move), pl[k] = lane of the row at position k ----
-  int pos = l;
-  bool sing = false;
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    double v = (pos >= k && l < N) ? fabs(a[k]) : -1.0;
-    int pp = pos, pln = l;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const double vo = __shfl_xor(v, o);
-      const int po = __shfl_xor(pp, o);
-      const int lo = __shfl_xor(pln, o);
-      if (vo > v || (vo == v && po < pp)) { v = vo; pp = po; pln = lo; }
-    }
-    const int p = __builtin_amdgcn_readfirstlane(pln);
-    const int ppos = __builtin_amdgcn_readfirstlane(pp);
-    if (v == 0.0) sing = true;
-    if (l == 0) pl[k] = p;
-    // the row at position k takes the pivot row's position, the pivot row k
-    if (l == p) pos = k;
-    else if (pos == k) pos = ppos;
-    const double pv = rdlane(a[k], p);
-    const double pivot = (pv == 0.0) ? 1.0 : pv;
-    if (pos > k && l < N) {
-      a[k] = a[k] / pivot;
-      const double m = a[k];
-#pragma unroll
-      for (int j = k + 1; j < N; j++) a[j] -= m * rdlane(a[j], p);
-    }
-  }
-
-  // ---- solve, then two refinement steps with a double-double residual ----
-  for (int it = 0; it < 3; it++) {
-    double r[4];
-    __syncthreads();   // pl[] written
-    if (it == 0) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) r[q] = bq[q];
-    } else {
-      // residual of row l: b - A0 x (A0 row from LDS, x from LDS)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        double sh = 0.0, sl = 0.0;
-        if (l < N) {
-          for (int cc = 0; cc < N; cc++) {
-            const double av = A0[l * LDA + cc], xv = Xs[cc * 4 + q];
-            const double pr = av * xv;
-            const double pe = fma(av, xv, -pr);
-            double t;
-            two_sum(sh, pr, sh, t);
-            sl += t + pe;
-          }
-        }
-        double d, de;
-        two_sum(bq[q], -sh, d, de);
-        r[q] = (l < N) ? d + (de - sl) : 0.0;
-      }
-    }
-    // forward: L y = P r (positions 0..N-1 in order)
-#pragma unroll
-    for (int k = 0; k < N - 1; k++) {
-      const int p = __builtin_amdgcn_readfirstlane(pl[k]);
-      if (pos > k && l < N) {
-        const double m = a[k];
-#pragma unroll
-        for (int q = 0; q < 4; q++) r[q] -= m * rdlane(r[q], p);
-      }
-    }
-    // backward: U x = y
-#pragma unroll
-    for (int k = N - 1; k >= 0; k--) {
-      const int p = __builtin_amdgcn_readfirstlane(pl[k]);
-      if (l == p) {
-        const double d = a[k];
-#pragma unroll
-        for (int q = 0; q < 4; q++) r[q] = r[q] / (d == 0.0 ? 1.0 : d);
-      }
-      if (pos < k && l < N) {
-        const double u = a[k];
-#pragma unroll
-        for (int q = 0; q < 4; q++) r[q] -= u * rdlane(r[q], p);
-      }
-    }
-    __syncthreads();
-    if (l < N) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) Xs[pos * 4 + q] = (it == 0) ? r[q] : Xs[pos * 4 + q] + r[q];
-    } else if (l < NP) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) Xs[l * 4 + q] = 0.0;
-    }
-    __syncthreads();
-  }
-  eval_weights<M>(S_, i, sysD, sysF, hh, Kfull, base, Xs);
-  if (sing && l == 0) atomicAdd(S_.singular, 1);
-  __syncthreads();
-}
-
-
-template <int M>
-__global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_reg(SysArgs a) {
-  extern __shared__ double lds[];
-  __shared__ int pl[64];
-  const int sys = blockIdx.x;
-  if (sys >= a.nsys) return;
-  solve_system_reg<M>(a, sys, lds, pl);
-}
-
-template <int M>
-__global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_reg_persistent(SysArgs a) {
-  extern __shared__ double lds[];
-  __shared__ int pl[64];
-  __shared__ int next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
-    __syncthreads();
-    const int sys = next;
-    __syncthreads();
-    if (sys >= a.nsys) break;
-    solve_system_reg<M>(a, sys, lds, pl);
-  }
-}
-
 // calc_labf_sums (labf.F90:396-451), one thread per node, sequential in k
 __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double L, double *wx,
                        double *wy, double *wl, const double *wh, double *gsx, double *gsy,
@@ -690,39 +502,31 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
   hsum[i] = hs;
 }
 
-#ifndef SV_WEIGHTS_REG
-#define SV_WEIGHTS_REG 0
-#endif
 #define SV_TUNE_SAMPLE 4096
+#define SV_TUNE_VARIANTS 2
 
-// variant: bit 0 = persistent work-queue grid, bit 1 = register-row solver
+// variant: bit 0 = persistent work-queue grid
 template <int M>
 int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
   if (count <= 0) return 0;
-  const bool reg = (variant & 2) != 0, persistent = (variant & 1) != 0;
-  const size_t shm = sizeof(double) * (reg ? LdsR<M>::TOTAL : Lds<M>::TOTAL);
+  const bool persistent = (variant & 1) != 0;
+  const size_t shm = sizeof(double) * Lds<M>::TOTAL;
   SysArgs b = a;
   b.sys_node = a.sys_node + first;
   b.sys_kind = a.sys_kind + first;
   b.nsys = count;
   if (!persistent) {
-    if (reg) k_weights_reg<M><<<count, SV_WAVE, shm, c->stream>>>(b);
-    else k_weights<M><<<count, SV_WAVE, shm, c->stream>>>(b);
+    k_weights<M><<<count, SV_WAVE, shm, c->stream>>>(b);
   } else {
     int dev = c->device, ncu = 0, occ = 0;
     SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    if (reg)
-      SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_reg_persistent<M>,
-                                                              SV_WAVE, shm));
-    else
-      SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_persistent<M>,
-                                                              SV_WAVE, shm));
+    SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_persistent<M>,
+                                                            SV_WAVE, shm));
     if (occ < 1) occ = 1;
     int grid = ncu * occ;
     if (grid > count) grid = count;
     SV_HIP(c, hipMemsetAsync(a.queue, 0, sizeof(int), c->stream));
-    if (reg) k_weights_reg_persistent<M><<<grid, SV_WAVE, shm, c->stream>>>(b);
-    else k_weights_persistent<M><<<grid, SV_WAVE, shm, c->stream>>>(b);
+    k_weights_persistent<M><<<grid, SV_WAVE, shm, c->stream>>>(b);
   }
   SV_HIP(c, hipGetLastError());
   return 0;
@@ -808,10 +612,10 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     while (first + count < (int)sys.size() && sys[first + count].m == m) count++;
     if (count == 0) continue;
     const int mi = m / 2 - 2;   // 0, 1, 2
-    int variant = (c->weights_persistent ? 1 : 0) | (SV_WEIGHTS_REG ? 2 : 0);
+    int variant = c->weights_persistent ? 1 : 0;
     if (c->weights_autotune && count >= 4 * SV_TUNE_SAMPLE) {
       if (c->wtune[mi] < 0) {
-        // time the four launch variants on a strided sample of this order's
+        // time the launch variants on a strided sample of this order's
         // systems (they are sorted by stencil length, so the sample has the
         // same length mix); the sample's weights are rewritten by the full
         // launch with identical values
@@ -827,7 +631,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
         ts.sys_node = tn;
         ts.sys_kind = tk;
         float best = 1e30f;
-        for (int v = 0; v < 4; v++) {
+        for (int v = 0; v < SV_TUNE_VARIANTS; v++) {
           float tv = 0.0f;
           for (int rep = 0; rep < 2; rep++) {   // first pass warms up
             SV_HIP(c, hipEventRecord(t0, c->stream));
@@ -842,7 +646,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
         SV_HIP(c, hipFree(tn));
         SV_HIP(c, hipFree(tk));
         float tt = 0.0f;
-        for (int v = 0; v < 4; v++) tt += 2.0f * c->wtune_ms[mi][v];
+        for (int v = 0; v < SV_TUNE_VARIANTS; v++) tt += 2.0f * c->wtune_ms[mi][v];
         tune_ms += tt;
       }
       variant = c->wtune[mi];

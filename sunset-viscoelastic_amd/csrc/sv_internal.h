@@ -157,8 +157,8 @@ struct sv_ctx {
   int *singular = nullptr;  // count of zero pivots in the last solve
   int n_singular = 0, n_systems = 0;
   bool weights_persistent = false;
-  // weight-solve launch autotuning per order (4, 6, 8): chosen variant (bit 0
-  // persistent grid, bit 1 register-row solver), -1 until tuned; sample times
+  // weight-solve launch autotuning per order (4, 6, 8): chosen variant (0 one
+  // block per system, 1 persistent grid), -1 until tuned; sample times
   bool weights_autotune = true;
   int wtune[3] = {-1, -1, -1};
   float wtune_ms[3][4] = {};
