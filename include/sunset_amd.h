@@ -120,7 +120,11 @@ int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
  *   "weights_persistent" (0/1): fixes the launch to one-block-per-system (0) or
  *       the persistent grid (1) and turns the autotuning off.
  *   "loop_order" (default 1, before sv_set_initial): 1 runs the time loop in
- *       Morton order of the nodes, 0 in the caller's order. */
+ *       Morton order of the nodes, 0 in the caller's order.
+ *   "halo_sweeps" (default 1, before sv_set_initial): the RK sweeps copy each
+ *       256-node block's neighbour records into LDS once and read the links
+ *       from there (when the largest block halo fits 64 KB); 0 gathers every
+ *       link from global memory. */
 int sv_set_option(sv_ctx *ctx, const char *key, double value);
 
 /* ---- derivative operators: derivatives.F90 ------------------------------ */

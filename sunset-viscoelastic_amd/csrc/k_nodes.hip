@@ -470,6 +470,62 @@ uint32_t morton2(uint32_t a, uint32_t b) {
 
 }  // namespace
 
+
+// Per-block halos of the loop layout (see sv_loopl): host pass over the SELL
+// columns, O(nnz log K) at setup.
+static int build_halos(sv_ctx *c) {
+  sv_loopl &L = c->L;
+  const int n = c->npfb, ns = L.nslices;
+  const int nb = (n + SV_BLOCK - 1) / SV_BLOCK;
+  std::vector<int64_t> off(ns + 1);
+  std::vector<int> col(L.nnz), cnt(n);
+  SV_HIP(c, hipMemcpy(off.data(), L.off, sizeof(int64_t) * (ns + 1), hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(col.data(), L.col, sizeof(int) * L.nnz, hipMemcpyDeviceToHost));
+  SV_HIP(c, hipMemcpy(cnt.data(), L.cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<int64_t> hoff(nb + 1, 0);
+  std::vector<int> hidx;
+  std::vector<uint16_t> lcol(L.nnz, 0);
+  hidx.reserve((size_t)n * 3);
+  std::vector<int> h;
+  int hmax = 0;
+  bool fits = true;
+  constexpr int SPB = SV_BLOCK / SV_WAVE;   // slices per block
+  for (int b = 0; b < nb; b++) {
+    h.clear();
+    for (int s = b * SPB; s < std::min(ns, (b + 1) * SPB); s++)
+      for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
+        const int slot = s * SV_WAVE + l;
+        for (int k = 0; k < cnt[slot]; k++) h.push_back(col[off[s] + (int64_t)k * SV_WAVE + l]);
+      }
+    std::sort(h.begin(), h.end());
+    h.erase(std::unique(h.begin(), h.end()), h.end());
+    hmax = std::max(hmax, (int)h.size());
+    if (h.size() > 65535) fits = false;
+    for (int s = b * SPB; fits && s < std::min(ns, (b + 1) * SPB); s++)
+      for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
+        const int slot = s * SV_WAVE + l;
+        for (int k = 0; k < cnt[slot]; k++) {
+          const int64_t e = off[s] + (int64_t)k * SV_WAVE + l;
+          lcol[e] = (uint16_t)(std::lower_bound(h.begin(), h.end(), col[e]) - h.begin());
+        }
+      }
+    hidx.insert(hidx.end(), h.begin(), h.end());
+    hoff[b + 1] = (int64_t)hidx.size();
+  }
+  L.hmax = hmax;
+  // the LC sweep keeps 72 bytes per halo record in LDS
+  L.halo = c->halo_sweeps && fits && (size_t)hmax * 72 <= SV_HALO_LDS_MAX;
+  if (!L.halo) return 0;
+  if (sv_alloc(c, (void **)&L.hoff, sizeof(int64_t) * (nb + 1)) ||
+      sv_alloc(c, (void **)&L.hidx, sizeof(int) * (hidx.size() + 1)) ||
+      sv_alloc(c, (void **)&L.lcol, sizeof(uint16_t) * (L.nnz + 1)))
+    return SV_E_NOMEM;
+  SV_HIP(c, hipMemcpy(L.hoff, hoff.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(L.hidx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice));
+  SV_HIP(c, hipMemcpy(L.lcol, lcol.data(), sizeof(uint16_t) * L.nnz, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int sv_dev_build_loop(sv_ctx *c) {
   const int n = c->npfb;
   sv_loopl &L = c->L;
@@ -507,7 +563,7 @@ int sv_dev_build_loop(sv_ctx *c) {
   }
   // (re)allocate
   void *old[] = {L.off, L.col, L.wx, L.wy, L.wl, L.wh, L.cnt, L.cnt_small, L.gsx, L.gsy,
-                 L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm};
+                 L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm, L.hoff, L.hidx, L.lcol};
   for (void *b : old)
     if (b) sv_free(c, b);
   std::vector<int> hp = std::move(L.hperm);
@@ -542,6 +598,7 @@ int sv_dev_build_loop(sv_ctx *c) {
   for (int k = 0; k < 6; k++) k_permute<double><<<nb, B, 0, c->stream>>>(n, L.perm, src[k], dst[k]);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
+  if (int rc = build_halos(c)) return rc;
   L.valid = true;
   return 0;
 }

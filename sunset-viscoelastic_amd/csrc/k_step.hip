@@ -229,6 +229,11 @@ struct StageArgs {
   // derivative values [ntot][16] (rx ry ux uy lu vx vy lv, cxx_x cxx_y, cxy_x
   // cxy_y, cyy_x cyy_y, 0 0), the tape the adjoint sweep reads
   double *tapeG, *tapeD;
+  // per-block halos (k_stage_h)
+  const int64_t *hoff;
+  const int *hidx;
+  const uint16_t *lcol;
+  int hcap;             // records the dynamic LDS holds
 };
 
 // The neighbour sums of one node in a stage: gradients and Laplacians before
@@ -438,6 +443,133 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   }
 }
 
+// ---- halo-staged sweeps ---------------------------------------------------
+// A block first copies the gather records of its halo (the distinct neighbours
+// of its 256 nodes, ~2.6 per node on a Morton-ordered lattice) into LDS, then
+// every link reads its record from LDS by a 16-bit block-local index.  The
+// L1 then serves only the streamed arrays (indices, weights) and the halo copy,
+// instead of one scattered record per link.
+// LDS record layout: 8 doubles (ro u v cxx cxy cyy pxx pxy) + pyy in a
+// separate array for LC (72 bytes per record); 6 doubles for DI.
+template <int FORM>
+struct HaloL {
+  static constexpr int RS = FORM == SV_FORM_LC ? 8 : 6;   // doubles in the main record
+  static constexpr int BYTES = FORM == SV_FORM_LC ? 72 : 48;
+};
+
+template <int FORM>
+__device__ __forceinline__ void halo_stage(const double *G, const int *hidx, int64_t h0, int nh,
+                                           double *hl, double *hz) {
+  constexpr int RS = HaloL<FORM>::RS;
+  for (int h = threadIdx.x; h < nh; h += BS) {
+    const Rec9 r = load_g<FORM>(G, hidx[h0 + h]);
+    double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * RS);
+    d[0] = double2{r.ro, r.u};
+    d[1] = double2{r.v, r.cxx};
+    d[2] = double2{r.cxy, r.cyy};
+    if (FORM == SV_FORM_LC) {
+      d[3] = double2{r.pxx, r.pxy};
+      hz[h] = r.pyy;
+    }
+  }
+}
+
+template <int FORM>
+__device__ __forceinline__ Rec9 halo_rec(const double *hl, const double *hz, int h) {
+  constexpr int RS = HaloL<FORM>::RS;
+  const double2 *d = reinterpret_cast<const double2 *>(hl + (size_t)h * RS);
+  Rec9 r;
+  const double2 a = d[0], b = d[1], c = d[2];
+  r.ro = a.x; r.u = a.y; r.v = b.x; r.cxx = b.y; r.cxy = c.x; r.cyy = c.y;
+  if (FORM == SV_FORM_LC) {
+    const double2 e = d[3];
+    r.pxx = e.x; r.pxy = e.y; r.pyy = hz[h];
+  } else {
+    r.pxx = r.pxy = r.pyy = 0.0;
+  }
+  return r;
+}
+
+template <int FORM, int STAGE, bool MDIFF>
+__global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
+  extern __shared__ double hl[];
+  __shared__ double red[BS / SV_WAVE];
+  double *hz = hl + (size_t)a.hcap * HaloL<FORM>::RS;
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int ii = nr.ii;
+  const bool act = nr.act;
+  const int64_t h0 = a.hoff[bid];
+  halo_stage<FORM>(a.G, a.hidx, h0, (int)(a.hoff[bid + 1] - h0), hl, hz);
+  __syncthreads();
+  const int64_t b = a.off[ii >> 6] + (ii & 63);
+  const int K = act ? a.kcnt[ii] : 0;
+  double rx = 0, ry = 0, ux = 0, uy = 0, vx = 0, vy = 0, lu = 0, lv = 0;
+  double axx = 0, ayx = 0, axy = 0, ayy = 0, bxy = 0, byy = 0;
+  double cxx_l = 0, cxy_l = 0, cyy_l = 0;
+  double q1x = 0, q1y = 0, q2x = 0, q2y = 0, q3x = 0, q3y = 0;
+  constexpr int U = SV_GATHER_UNROLL;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? (int)ld_stream(a.lcol + b + (int64_t)u * SV_WAVE) : 0;
+  for (int k = 0; k < K; k += U) {
+    Rec9 r[U];
+    double wxk[U], wyk[U], wlk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k + u < K) {
+        const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+        r[u] = halo_rec<FORM>(hl, hz, jn[u]);
+        wxk[u] = ld_stream(a.wx + e); wyk[u] = ld_stream(a.wy + e); wlk[u] = ld_stream(a.wl + e);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kn = k + U + u;
+      jn[u] = (kn < K) ? (int)ld_stream(a.lcol + b + (int64_t)kn * SV_WAVE) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k + u < K) {
+        const Rec9 &q = r[u];
+        const double wx_ = wxk[u], wy_ = wyk[u], wl_ = wlk[u];
+        rx += q.ro * wx_; ry += q.ro * wy_;
+        ux += q.u * wx_; uy += q.u * wy_; lu += q.u * wl_;
+        vx += q.v * wx_; vy += q.v * wy_; lv += q.v * wl_;
+        if (FORM == SV_FORM_LC) {
+          axx += q.cxx * wx_;
+          axy += q.cxy * wx_; ayy += q.cxy * wy_;
+          byy += q.cyy * wy_;
+          q1x += q.pxx * wx_; q1y += q.pxx * wy_;
+          q2x += q.pxy * wx_; q2y += q.pxy * wy_;
+          q3x += q.pyy * wx_; q3y += q.pyy * wy_;
+        } else {
+          axx += q.cxx * wx_; ayx += q.cxx * wy_;
+          axy += q.cxy * wx_; ayy += q.cxy * wy_;
+          bxy += q.cyy * wx_; byy += q.cyy * wy_;
+          if (MDIFF) { cxx_l += q.cxx * wl_; cxy_l += q.cxy * wl_; cyy_l += q.cyy * wl_; }
+        }
+      }
+  }
+  double emax_loc = 0.0;
+  if (act) {
+    const Sums S{rx, ry, ux, uy, vx, vy, lu, lv, axx, ayx, axy, ayy, bxy, byy, cxx_l, cxy_l,
+                 cyy_l, q1x, q1y, q2x, q2y, q3x, q3y};
+    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
+  }
+  if (STAGE == 4) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) emax_loc = fmax(emax_loc, __shfl_xor(emax_loc, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emax_loc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int w = 1; w < BS / SV_WAVE; w++) m = fmax(m, red[w]);
+      a.partial[bid] = m;
+    }
+  }
+}
+
 struct FilterArgs {
   const int *blk_seg;
   const sv_segk *seg;
@@ -448,7 +580,10 @@ struct FilterArgs {
   const double *U;   // gathered state records
   double *Uf;        // filtered state records
   double *Gout;      // psi (LC) and c of the filtered state
-  double *partial;   // per-block sum of (ro_f - rho_char)*vol
+  double *partial;
+  const int64_t *hoff;   // per-block halos (k_filter_h)
+  const int *hidx;
+  const uint16_t *lcol;   // per-block sum of (ro_f - rho_char)*vol
 };
 
 // Pointwise part of the filter for one node: filtered record, its gather
@@ -502,6 +637,68 @@ __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
     for (int u = 0; u < U; u++) {
       const int kn = k + U + u;
       jn[u] = (kn < K) ? ld_stream(a.col + b + (int64_t)kn * SV_WAVE) : ii;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k + u < K) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) t[q] += f[u][q] * w[u];
+      }
+  }
+  const double m = act ? filter_point<FORM>(a, nr, t) : 0.0;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.partial[bid] = red[0];
+}
+
+// Halo-staged form of k_filter (see k_stage_h): the block's neighbour state
+// records (6 doubles) go to LDS once, links read them by block-local index.
+template <int FORM>
+__global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
+  extern __shared__ double hl[];
+  __shared__ double red[BS];
+  const int bid = sv_xcd_block();
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const int ii = nr.ii;
+  const bool act = nr.act;
+  const int64_t h0 = a.hoff[bid];
+  const int nh = (int)(a.hoff[bid + 1] - h0);
+  for (int h = threadIdx.x; h < nh; h += BS) {
+    double f[6];
+    load_u(a.U, a.hidx[h0 + h], f);
+    double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * 6);
+    d[0] = double2{f[0], f[1]};
+    d[1] = double2{f[2], f[3]};
+    d[2] = double2{f[4], f[5]};
+  }
+  __syncthreads();
+  const int64_t b = a.off[ii >> 6] + (ii & 63);
+  const int K = act ? a.kcnt[ii] : 0;
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  constexpr int U = SV_GATHER_UNROLL;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (u < K) ? (int)ld_stream(a.lcol + b + (int64_t)u * SV_WAVE) : 0;
+  for (int k = 0; k < K; k += U) {
+    double f[U][6], w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k + u < K) {
+        const double2 *d = reinterpret_cast<const double2 *>(hl + (size_t)jn[u] * 6);
+        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
+        f[u][0] = d0.x; f[u][1] = d0.y; f[u][2] = d1.x; f[u][3] = d1.y; f[u][4] = d2.x;
+        f[u][5] = d2.y;
+        w[u] = ld_stream(a.wh + b + (int64_t)(k + u) * SV_WAVE);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kn = k + U + u;
+      jn[u] = (kn < K) ? (int)ld_stream(a.lcol + b + (int64_t)kn * SV_WAVE) : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
@@ -693,16 +890,25 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   a.Gout = Gout;
   a.fx = c->any_forcing ? c->fx : nullptr;
   a.ctrl = c->ctrl;
+  a.hoff = c->L.hoff;
+  a.hidx = c->L.hidx;
+  a.lcol = c->L.lcol;
+  a.hcap = c->L.hmax;
   a.partial = c->partial;
   return a;
 }
 
 template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
-  if (FORM == SV_FORM_DI && c->any_mdiff)
-    k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
-  else
-    k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+  const bool md = FORM == SV_FORM_DI && c->any_mdiff;
+  if (c->L.halo) {
+    const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
+    if (md) k_stage_h<FORM, STAGE, true><<<nb, BS, shm, c->stream>>>(a);
+    else k_stage_h<FORM, STAGE, false><<<nb, BS, shm, c->stream>>>(a);
+  } else {
+    if (md) k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
+    else k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+  }
 }
 
 // ev (nullable): SV_NKERNELS+1 events recorded between the launches of one step
@@ -744,7 +950,13 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   f.Uf = c->Uf;
   f.Gout = c->GA;
   f.partial = pm;
-  k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
+  f.hoff = c->L.hoff;
+  f.hidx = c->L.hidx;
+  f.lcol = c->L.lcol;
+  if (c->L.halo)
+    k_filter_h<FORM><<<nb, BS, sizeof(double) * 6 * (size_t)c->L.hmax, c->stream>>>(f);
+  else
+    k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
   mark();
   k_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pm, pe, c->ctrl);
   mark();

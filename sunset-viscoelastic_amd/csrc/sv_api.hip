@@ -327,7 +327,11 @@ int sv_set_option(sv_ctx *c, const char *key, double value) {
     c->weights_autotune = value != 0.0;
     for (int k = 0; k < 3; k++) c->wtune[k] = -1;
   }
-  else if (!strcmp(key, "loop_order")) {
+  else if (!strcmp(key, "halo_sweeps")) {
+    SV_CHECK(c, !c->have_state, "halo_sweeps must be set before sv_set_initial");
+    c->halo_sweeps = value != 0.0 ? 1 : 0;
+    c->L.valid = false;
+  } else if (!strcmp(key, "loop_order")) {
     SV_CHECK(c, !c->have_state, "loop_order must be set before sv_set_initial");
     c->loop_order = value != 0.0 ? 1 : 0;
     c->L.valid = false;

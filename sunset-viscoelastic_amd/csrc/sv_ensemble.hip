@@ -12,6 +12,8 @@
 // and unchanged: the ensemble holds copies taken at creation.
 #include <string.h>
 
+#include <algorithm>
+
 #include "sv_internal.h"
 
 namespace {
@@ -109,6 +111,32 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
       hipMemsetAsync(E->GA, 0, D * E->gs * N, E->stream) != hipSuccess ||
       hipMemsetAsync(E->GB, 0, D * E->gs * N, E->stream) != hipSuccess)
     return fail(SV_E_HIP);
+  // per-block halos: used when every member has them (see sv_loopl)
+  bool halo = true;
+  int hmax = 0;
+  std::vector<int64_t> hbase(nmembers + 1, 0);
+  for (int m = 0; m < nmembers; m++) {
+    const sv_loopl &CL = members[m]->L;
+    halo = halo && CL.halo;
+    if (!halo) break;
+    hmax = std::max(hmax, CL.hmax);
+    const int nbm = E->hseg[m].nblk;
+    int64_t last = 0;
+    if (hipMemcpy(&last, CL.hoff + nbm, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(SV_E_HIP);
+    hbase[m + 1] = hbase[m] + last;
+  }
+  if (halo) {
+    if (sv_alloc(E, (void **)&EL.hoff, sizeof(int64_t) * ((size_t)blocks + 1)) ||
+        sv_alloc(E, (void **)&EL.hidx, I * ((size_t)hbase[nmembers] + 1)) ||
+        sv_alloc(E, (void **)&EL.lcol, sizeof(uint16_t) * ((size_t)sell + 1)))
+      return fail(SV_E_NOMEM);
+    const int64_t tot = hbase[nmembers];
+    if (hipMemcpy(EL.hoff + blocks, &tot, sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(SV_E_HIP);
+    EL.halo = true;
+    EL.hmax = hmax;
+  }
   std::vector<int> bseg(blocks);
   for (int m = 0; m < nmembers; m++) {
     const sv_ctx *c = members[m];
@@ -134,6 +162,14 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     if (!rc) rc = copy_d2d(E, E->U + b * SV_US, c->U, D * SV_US * n);
     if (!rc) rc = copy_d2d(E, E->GA + b * E->gs, c->GA, D * E->gs * n);
     if (!rc) rc = copy_d2d(E, E->ctrl + m, c->ctrl, sizeof(sv_ctrl));
+    if (!rc && halo) {
+      const int64_t hb = hbase[m], hn = hbase[m + 1] - hbase[m];
+      rc = copy_d2d(E, EL.hoff + S.blk0, CL.hoff, sizeof(int64_t) * (size_t)S.nblk);
+      if (!rc) rc = sv_dev_offset_i64(E, EL.hoff + S.blk0, S.nblk, hb);
+      if (!rc) rc = copy_d2d(E, EL.hidx + hb, CL.hidx, I * (size_t)hn);
+      if (!rc) rc = sv_dev_offset_int(E, EL.hidx + hb, hn, S.base);
+      if (!rc) rc = copy_d2d(E, EL.lcol + sb, CL.lcol, sizeof(uint16_t) * (size_t)CL.nnz);
+    }
     if (rc) return fail(rc);
   }
   if (hipMemcpyAsync(E->segk, E->hseg.data(), sizeof(sv_segk) * nmembers,

@@ -91,6 +91,9 @@ void sv_adj_free(sv_ctx *c);
 // SV_BLOCK nodes, so every block of the node kernels lies in one member and
 // per-block partial reductions are per-member partial reductions.
 #define SV_BLOCK 256
+// LDS budget of a block's staged halo records (bytes); above it the sweeps
+// gather from global memory instead
+#define SV_HALO_LDS_MAX (64 * 1024)
 struct sv_segk {
   int base;      // first node (concatenated numbering), a multiple of SV_BLOCK
   int npfb;      // real nodes
@@ -119,6 +122,15 @@ struct sv_loopl {
   double *s = nullptr;
   int *perm = nullptr, *iperm = nullptr;
   std::vector<int> hperm;
+  // Per-block neighbour halos: the distinct neighbour slots of the SV_BLOCK
+  // slots of block b are hidx[hoff[b] .. hoff[b+1]) (ascending); lcol[e] is
+  // the position of col[e] in its block's halo.  The sweeps stage a block's
+  // halo records in LDS once and read every link from there.
+  bool halo = false;
+  int hmax = 0;             // largest halo of a block
+  int64_t *hoff = nullptr;  // [nblocks + 1]
+  int *hidx = nullptr;
+  uint16_t *lcol = nullptr; // [nnz]
 };
 
 struct sv_ctx {
@@ -182,6 +194,7 @@ struct sv_ctx {
   sv_adj *adj = nullptr;
   sv_loopl L;
   int loop_order = 1;       // 1: Morton order for the time loop, 0: the caller's order
+  int halo_sweeps = 1;      // 1: stage neighbour halos in LDS when they fit
   // ensemble: loop-slot -> member-node permutation of each member
   std::vector<std::vector<int>> member_perm;
   bool any_forcing = false, any_mdiff = false;  // over segments
