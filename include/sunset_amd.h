@@ -123,7 +123,7 @@ int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
  *       Morton order of the nodes, 0 in the caller's order.
  *   "halo_sweeps" (default 1, before sv_set_initial): the RK sweeps copy each
  *       256-node block's neighbour records into LDS once and read the links
- *       from there (when the largest block halo fits 64 KB); 0 gathers every
+ *       from there (when the largest block halo fits 60 KB); 0 gathers every
  *       link from global memory. */
 int sv_set_option(sv_ctx *ctx, const char *key, double value);
 

@@ -40,6 +40,11 @@ constexpr int BS = SV_BLOCK;   // threads per block of the node kernels
 #ifndef SV_STREAM_NT
 #define SV_STREAM_NT 1
 #endif
+// links per batch in the halo-staged sweeps (global streams prefetched one
+// batch ahead)
+#ifndef SV_HALO_UNROLL
+#define SV_HALO_UNROLL 4
+#endif
 
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T *p) {
@@ -508,31 +513,34 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   double axx = 0, ayx = 0, axy = 0, ayy = 0, bxy = 0, byy = 0;
   double cxx_l = 0, cxy_l = 0, cyy_l = 0;
   double q1x = 0, q1y = 0, q2x = 0, q2y = 0, q3x = 0, q3y = 0;
-  constexpr int U = SV_GATHER_UNROLL;
+  // The global streams (block-local column, three weights) of the next batch
+  // of links are loaded before the current batch is consumed, so each lane
+  // keeps SV_HALO_UNROLL links (26 bytes each) in flight; records come from LDS.
+  constexpr int U = SV_HALO_UNROLL;
   int jn[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) jn[u] = (u < K) ? (int)ld_stream(a.lcol + b + (int64_t)u * SV_WAVE) : 0;
-  for (int k = 0; k < K; k += U) {
-    Rec9 r[U];
-    double wxk[U], wyk[U], wlk[U];
+  double wxn[U], wyn[U], wln[U];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (k + u < K) {
-        const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
-        r[u] = halo_rec<FORM>(hl, hz, jn[u]);
-        wxk[u] = ld_stream(a.wx + e); wyk[u] = ld_stream(a.wy + e); wlk[u] = ld_stream(a.wl + e);
+      if (k0 + u < K) {
+        const int64_t e = b + (int64_t)(k0 + u) * SV_WAVE;
+        jn[u] = (int)ld_stream(a.lcol + e);
+        wxn[u] = ld_stream(a.wx + e); wyn[u] = ld_stream(a.wy + e); wln[u] = ld_stream(a.wl + e);
       }
     }
+  };
+  fetch(0);
+  for (int k = 0; k < K; k += U) {
+    int jc[U];
+    double wxc[U], wyc[U], wlc[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int kn = k + U + u;
-      jn[u] = (kn < K) ? (int)ld_stream(a.lcol + b + (int64_t)kn * SV_WAVE) : 0;
-    }
+    for (int u = 0; u < U; u++) { jc[u] = jn[u]; wxc[u] = wxn[u]; wyc[u] = wyn[u]; wlc[u] = wln[u]; }
+    fetch(k + U);
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (k + u < K) {
-        const Rec9 &q = r[u];
-        const double wx_ = wxk[u], wy_ = wyk[u], wl_ = wlk[u];
+        const Rec9 q = halo_rec<FORM>(hl, hz, jc[u]);
+        const double wx_ = wxc[u], wy_ = wyc[u], wl_ = wlc[u];
         rx += q.ro * wx_; ry += q.ro * wy_;
         ux += q.u * wx_; uy += q.u * wy_; lu += q.u * wl_;
         vx += q.v * wx_; vy += q.v * wy_; lv += q.v * wl_;
@@ -679,32 +687,33 @@ __global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
   double t[6] = {0, 0, 0, 0, 0, 0};
-  constexpr int U = SV_GATHER_UNROLL;
+  constexpr int U = SV_HALO_UNROLL;
   int jn[U];
+  double wn[U];
+  auto fetch = [&](int k0) {
 #pragma unroll
-  for (int u = 0; u < U; u++) jn[u] = (u < K) ? (int)ld_stream(a.lcol + b + (int64_t)u * SV_WAVE) : 0;
-  for (int k = 0; k < K; k += U) {
-    double f[U][6], w[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (k + u < K) {
-        const double2 *d = reinterpret_cast<const double2 *>(hl + (size_t)jn[u] * 6);
-        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
-        f[u][0] = d0.x; f[u][1] = d0.y; f[u][2] = d1.x; f[u][3] = d1.y; f[u][4] = d2.x;
-        f[u][5] = d2.y;
-        w[u] = ld_stream(a.wh + b + (int64_t)(k + u) * SV_WAVE);
+    for (int u = 0; u < U; u++)
+      if (k0 + u < K) {
+        const int64_t e = b + (int64_t)(k0 + u) * SV_WAVE;
+        jn[u] = (int)ld_stream(a.lcol + e);
+        wn[u] = ld_stream(a.wh + e);
       }
-    }
+  };
+  fetch(0);
+  for (int k = 0; k < K; k += U) {
+    int jc[U];
+    double wc[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int kn = k + U + u;
-      jn[u] = (kn < K) ? (int)ld_stream(a.lcol + b + (int64_t)kn * SV_WAVE) : 0;
-    }
+    for (int u = 0; u < U; u++) { jc[u] = jn[u]; wc[u] = wn[u]; }
+    fetch(k + U);
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (k + u < K) {
-#pragma unroll
-        for (int q = 0; q < 6; q++) t[q] += f[u][q] * w[u];
+        const double2 *d = reinterpret_cast<const double2 *>(hl + (size_t)jc[u] * 6);
+        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
+        const double w = wc[u];
+        t[0] += d0.x * w; t[1] += d0.y * w; t[2] += d1.x * w;
+        t[3] += d1.y * w; t[4] += d2.x * w; t[5] += d2.y * w;
       }
   }
   const double m = act ? filter_point<FORM>(a, nr, t) : 0.0;

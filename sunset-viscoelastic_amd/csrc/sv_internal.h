@@ -93,7 +93,7 @@ void sv_adj_free(sv_ctx *c);
 #define SV_BLOCK 256
 // LDS budget of a block's staged halo records (bytes); above it the sweeps
 // gather from global memory instead
-#define SV_HALO_LDS_MAX (64 * 1024)
+#define SV_HALO_LDS_MAX (60 * 1024)
 struct sv_segk {
   int base;      // first node (concatenated numbering), a multiple of SV_BLOCK
   int npfb;      // real nodes
