@@ -30,16 +30,23 @@ HBM_PEAK_GBS = 8000.0
 PHYS = dict(Re=1.0, Wi=2.0, beta=0.1, eps=0.25, Ma=0.1, kol_amp=1.0, kol_k=4.0 * math.pi)
 
 
-def stage_bytes_per_node(ks_mean):
-    """Algorithmic HBM bytes per node of the four LC stage sweeps (mean):
-    28 B per link (int32 index + 3 FP64 weights) + own-node traffic
-    (stage 1: 316 B, stages 2-3: 364 B, stage 4: 244 B)."""
-    return 28.0 * ks_mean + (316.0 + 364.0 + 364.0 + 244.0) / 4.0
+def stage_bytes_per_node(ks_mean, halo):
+    """Algorithmic HBM bytes per node of the four LC stage sweeps (mean).
+    Per link: a 16-bit block-local index + 3 FP64 weights (26 B); per staged
+    halo record: its 4-byte index + the 9 gathered fields (76 B); own-node
+    traffic (stage 1: 316 B, stages 2-3: 364 B, stage 4: 244 B).  Without
+    halos (halo == 0): a 4-byte index per link and the gathers are not
+    counted."""
+    link = 26.0 if halo > 0 else 28.0
+    return link * ks_mean + 76.0 * halo + (316.0 + 364.0 + 364.0 + 244.0) / 4.0
 
 
-def step_bytes_per_node(ks_mean, k_mean):
-    """Whole step: 4 stages + filter (12 B per link + 212 B) + apply (128 B)."""
-    return 4 * 28.0 * ks_mean + 1288.0 + 12.0 * k_mean + 212.0 + 128.0
+def step_bytes_per_node(ks_mean, k_mean, halo):
+    """Whole step: 4 stages + filter (10 B per link + 52 B per halo record +
+    212 B) + apply (128 B)."""
+    flink = 10.0 if halo > 0 else 12.0
+    return (4 * stage_bytes_per_node(ks_mean, halo) + flink * k_mean + 52.0 * halo + 212.0
+            + 128.0)
 
 
 def cpu_baseline(n_sample, steps, threads):
@@ -287,13 +294,14 @@ def main():
     roof = None
     if kern is not None:
         t_stage = np.mean([kern[k] for k in ("stage1", "stage2", "stage3", "stage4")])
-        b_stage = stage_bytes_per_node(st["Ks_mean"]) * n
+        b_stage = stage_bytes_per_node(st["Ks_mean"], st["halo_per_node"]) * n
         ach = b_stage / (t_stage / 1e3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
                     kernel="k_stage (RK stage sweep, mean of stages 1-4)",
                     bytes_per_launch=int(b_stage), ms_per_launch=round(t_stage, 4))
-    step_gbs = step_bytes_per_node(st["Ks_mean"], st["K_mean"]) * n / (ms_step / 1e3) / 1e9
+    step_gbs = (step_bytes_per_node(st["Ks_mean"], st["K_mean"], st["halo_per_node"]) * n
+                / (ms_step / 1e3) / 1e9)
 
     cpu = None
     if not args.no_cpu:
@@ -330,6 +338,7 @@ def main():
                                "log-conformation sPTT eps=0.25 Wi=2 beta=0.1 Re=1 Ma=0.1, "
                                "Kolmogorov f_x=sin(4 pi y), PID dt",
                    "nodes": n, "K_mean": round(st["K_mean"], 2), "order": 6,
+                   "halo_records_per_node": round(st["halo_per_node"], 3),
                    "timing": "hipGraph replay" if args.graph else "eager, HIP events"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -170,8 +170,9 @@ int sv_run(sv_ctx *ctx, int32_t nsteps, int32_t use_graph);
  * of each launch, in launch order: dt control, stage 1, stage 2, stage 3,
  * stage 4, filter, mass-correction reduce, state apply.  total_ms: whole run. */
 int sv_run_profiled(sv_ctx *ctx, int32_t nsteps, double *kernel_ms, double *total_ms);
-/* stats[5]: mean ij_count, mean ij_count_small, max ij_count, SELL entries
- * (with padding), number of ghost nodes */
+/* stats[6]: mean ij_count, mean ij_count_small, max ij_count, SELL entries
+ * (with padding), number of ghost nodes, neighbour-halo records per node
+ * staged in LDS by the time-loop sweeps (0 when they gather from memory) */
 int sv_get_stencil_stats(sv_ctx *ctx, double *stats);
 /* Fields on real nodes (any pointer may be NULL): psi* only in the LC form. */
 int sv_get_state(sv_ctx *ctx, double *ro, double *rou, double *rov, double *u, double *v,

@@ -11,15 +11,18 @@
 //   Gbar_j += sum_{(i,k): j_k = i's link} w_ik Dbar_i
 // over the transposed list of j in a fixed order (ascending i): deterministic.
 //
-// Per RK stage the adjoint is two launches:
-//   k_adj_point  pointwise: Rbar_i and the stage tape (own gather record G_i and
+// Each RK stage's adjoint has two parts:
+//   pointwise    Rbar_i and the stage tape (own gather record G_i and
 //                derivative values D_i written by the forward stage) -> the
 //                adjoints of the 14 derivative values Dbar_i (+3 Laplacians of
 //                c with Mdiff) and the own-node adjoint of G_i
-//   k_adj_trans  transposed gather of Dbar, G -> U (u = rou/ro), and the RK
-//                register recurrences that give the previous stage's Rbar
-// and per step: the mass-correction sum, its pointwise correction and the
-// transposed filter gather.  Rollouts keep the state every `checkpoint`
+//   transposed   gather of Dbar, G -> U (u = rou/ro), and the RK register
+//                recurrences that give the previous stage's Rbar
+// The pointwise part of stage k-1 needs only node i's own Rbar, so it runs
+// inside the transposed gather of stage k (two buffer sets, ping-pong), and
+// the pointwise part of stage 4 inside the fused mass-correction + filter
+// adjoint: one step's adjoint is six launches (a per-segment sum, the filter,
+// four transposed gathers).  Rollouts keep the state every `checkpoint`
 // steps; the adjoint recomputes one checkpoint interval with the tape on and
 // walks it backwards.
 #include <string.h>
@@ -72,15 +75,14 @@ struct PointArgs {
   double *dbar, *dbarm, *own;
 };
 
+// the pointwise adjoint of one (active) node, Rbar given in registers
 template <bool MDIFF>
-__global__ __launch_bounds__(BS) void k_adj_point(PointArgs a) {
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
-  if (!nr.act) return;
+__device__ __forceinline__ void adj_point_node(const PointArgs &a, const NodeRef &nr,
+                                               const double rb[6]) {
   const int i = nr.i;
   const sv_physk &P = a.seg[nr.sg].pk;
-  double g[6], rb[6];
+  double g[6];
   ld6(a.tG + (size_t)i * SV_US, g);
-  ld6(a.Rbar + (size_t)i * SV_US, rb);
   const double ro = g[0], u = g[1], v = g[2], cxx = g[3], cxy = g[4], cyy = g[5];
   const double *D = a.tD + (size_t)i * SV_TAPE_D;
   const double rx = D[0], ry = D[1], ux = D[2], uy = D[3], vx = D[5], vy = D[6];
@@ -166,6 +168,15 @@ __global__ __launch_bounds__(BS) void k_adj_point(PointArgs a) {
   }
 }
 
+template <bool MDIFF>
+__global__ __launch_bounds__(BS) void k_adj_point(PointArgs a) {
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
+  if (!nr.act) return;
+  double rb[6];
+  ld6(a.Rbar + (size_t)nr.i * SV_US, rb);
+  adj_point_node<MDIFF>(a, nr, rb);
+}
+
 // ---- transposed gather + G->U + RK register recurrences ------------------
 struct TransArgs {
   const int *blk_seg;
@@ -181,6 +192,8 @@ struct TransArgs {
   double *regbar;            // STAGE 2..4: in/out
   double seed;               // STAGE 1: loss seed added to cxx and cyy
   double *psum;              // STAGE 1: per-block sum of Ubar_ro (mass correction)
+  PointArgs nxt;             // STAGE 2..4: the pointwise adjoint of stage STAGE-1,
+                             // fused (its Rbar stays in registers)
 };
 
 // STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar)
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
         rg[f] = Ub[f] + rg[f];
       }
       st6(a.regbar + (size_t)j * SV_US, rg);
-      st6(a.Rbar + (size_t)j * SV_US, Rn);
+      adj_point_node<MDIFF>(a.nxt, nr, Rn);
     }
   }
   if (STAGE == 1) {
@@ -297,48 +310,53 @@ __global__ __launch_bounds__(1024) void k_adj_sum_reduce(const sv_segk *seg, con
   if (threadIdx.x == 0) sbar[blockIdx.x] = s1[0] / S.volsum;
 }
 
-__global__ __launch_bounds__(BS) void k_adj_mcorr(const int *blk_seg, const sv_segk *seg,
-                                                  const double *sbar, const double *vol,
-                                                  double *Ubar) {
-  const NodeRef nr = node_ref(blk_seg, seg, sv_xcd_block());
-  if (!nr.act) return;
-  Ubar[(size_t)nr.i * SV_US] -= vol[nr.i] * sbar[nr.sg];
-}
+// mass-correction + filter adjoint, fused with the pointwise adjoint of
+// stage 4: Uf_i = Ubar_i - [ro] vol_i sbar (evaluated for every gathered i),
+// Ustar_j = Uf_j - hsum_j Uf_j + sum_t wh_t Uf_{i_t}; regbar_3 = Ustar,
+// Rbar_4 = b_4 Ustar (registers) -> stage-4 pointwise adjoint
+struct FiltArgs {
+  const int *blk_seg;
+  const sv_segk *seg;
+  const int64_t *toff;
+  const int *tcnt, *tcol;
+  const double *twh, *hsum, *vol, *sbar;
+  const double *Ubar;
+  const sv_ctrl *ctrl;
+  double *regbar;
+  PointArgs p4;
+};
 
-// filter adjoint: Ustar_j = Uf_j - hsum_j Uf_j + sum_t wh_t Uf_{i_t}; then
-// regbar_3 = Ustar, Rbar_4 = b_4 Ustar
-__global__ __launch_bounds__(BS) void k_adj_filter(const int *blk_seg, const sv_segk *seg,
-                                                   const int64_t *toff, const int *tcnt,
-                                                   const int *tcol, const double *twh,
-                                                   const double *hsum, const double *Ubar,
-                                                   const sv_ctrl *ctrl, double *regbar,
-                                                   double *Rbar) {
-  const NodeRef nr = node_ref(blk_seg, seg, sv_xcd_block());
+template <bool MDIFF>
+__global__ __launch_bounds__(BS) void k_adj_filter(FiltArgs a) {
+  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
   const int j = nr.i, jj = nr.ii;
-  const int64_t b = toff[jj >> 6] + (jj & 63);
-  const int K = nr.act ? tcnt[jj] : 0;
+  const int64_t b = a.toff[jj >> 6] + (jj & 63);
+  const int K = nr.act ? a.tcnt[jj] : 0;
+  const double sv = a.sbar[nr.sg];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int k = 0; k < K; k++) {
     const int64_t e = b + (int64_t)k * SV_WAVE;
-    const int i = tcol[e];
-    const double w = twh[e];
+    const int i = a.tcol[e];
+    const double w = a.twh[e];
     double f[6];
-    ld6(Ubar + (size_t)i * SV_US, f);
+    ld6(a.Ubar + (size_t)i * SV_US, f);
+    f[0] -= a.vol[i] * sv;
 #pragma unroll
     for (int q = 0; q < 6; q++) acc[q] += w * f[q];
   }
   if (!nr.act) return;
   double f[6], out[6], rb[6];
-  ld6(Ubar + (size_t)j * SV_US, f);
-  const double hs = hsum[j];
-  const double b4 = ctrl[nr.sg].RKb[3];
+  ld6(a.Ubar + (size_t)j * SV_US, f);
+  f[0] -= a.vol[j] * sv;
+  const double hs = a.hsum[j];
+  const double b4 = a.ctrl[nr.sg].RKb[3];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     out[q] = f[q] - hs * f[q] + acc[q];
     rb[q] = b4 * out[q];
   }
-  st6(regbar + (size_t)j * SV_US, out);
-  st6(Rbar + (size_t)j * SV_US, rb);
+  st6(a.regbar + (size_t)j * SV_US, out);
+  adj_point_node<MDIFF>(a.p4, nr, rb);
 }
 
 // trace loss of the state: per-block sums of (cxx + cyy - 2), then one block
@@ -393,8 +411,11 @@ struct sv_adj {
   double *twx = nullptr, *twy = nullptr, *twl = nullptr, *twh = nullptr;
   int64_t tnnz = 0;
   // adjoint records [ntot][8] and stencil-adjoint records
-  double *Ubar = nullptr, *Rbar = nullptr, *regbar = nullptr, *own = nullptr;
-  double *dbar = nullptr, *dbarm = nullptr;
+  double *Ubar = nullptr, *Rbar = nullptr, *regbar = nullptr;
+  // stencil-adjoint records of two consecutive stages (ping-pong: a fused
+  // kernel reads stage k's while writing stage k-1's)
+  double *own[2] = {nullptr, nullptr}, *dbar[2] = {nullptr, nullptr};
+  double *dbarm[2] = {nullptr, nullptr};
   double *psum = nullptr, *sbar = nullptr, *lpart = nullptr, *lacc = nullptr;
   // rollout: checkpoints of U records, the end state, the tape of one interval
   int T = 0;                 // steps of the last forward rollout
@@ -501,36 +522,49 @@ static int ensure_tape(sv_ctx *c, sv_adj *A) {
   return 0;
 }
 
-// Jt Rbar at one tape slot: pointwise + transposed gather for the given stage
+// pointwise-adjoint arguments for stage k (1..4) of a tape slot, buffer set q
+static PointArgs point_args(sv_ctx *c, sv_adj *A, double *const *tape, int k, int q) {
+  return PointArgs{c->blk_seg, c->segk, tape[2 * (k - 1)], tape[2 * (k - 1) + 1], A->Rbar,
+                   c->L.gsx, c->L.gsy, c->L.lsum, A->dbar[q], A->dbarm[q], A->own[q]};
+}
+
+// transposed gather of stage STAGE (its records in buffer set q); for STAGE
+// 2..4 fused with the pointwise adjoint of STAGE-1 into buffer set 1-q
 template <int STAGE>
-static int adj_stage(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
-  const int k = (STAGE == 0 ? 1 : STAGE) - 1;
-  PointArgs p{c->blk_seg, c->segk, tape[2 * k], tape[2 * k + 1], A->Rbar, c->L.gsx, c->L.gsy,
-              c->L.lsum, A->dbar, A->dbarm, A->own};
-  if (c->any_mdiff) k_adj_point<true><<<c->nblk, BS, 0, c->stream>>>(p);
-  else k_adj_point<false><<<c->nblk, BS, 0, c->stream>>>(p);
+static int adj_trans(sv_ctx *c, sv_adj *A, double *const *tape, int q, double seed) {
+  const int k = STAGE == 0 ? 1 : STAGE;
   TransArgs t{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twx, A->twy, A->twl,
-              A->dbar, A->dbarm, A->own, tape[2 * k], c->ctrl, A->Ubar, A->Rbar,
-              A->regbar, seed, A->psum};
+              A->dbar[q], A->dbarm[q], A->own[q], tape[2 * (k - 1)], c->ctrl, A->Ubar, A->Rbar,
+              A->regbar, seed, A->psum, PointArgs{}};
+  if (STAGE >= 2) t.nxt = point_args(c, A, tape, STAGE - 1, 1 - q);
   if (c->any_mdiff) k_adj_trans<STAGE, true><<<c->nblk, BS, 0, c->stream>>>(t);
   else k_adj_trans<STAGE, false><<<c->nblk, BS, 0, c->stream>>>(t);
   SV_HIP(c, hipGetLastError());
   return 0;
 }
 
-// adjoint of one step (tape slot s): Ubar (after the step, with its mass sums
-// in psum) -> Ubar (before the step) + seed on cxx, cyy
+// calc_all_rhs vector-Jacobian product at tape slot 0 (Rbar in A->Rbar)
+static int adj_rhs(sv_ctx *c, sv_adj *A, double *const *tape) {
+  const PointArgs p = point_args(c, A, tape, 1, 0);
+  if (c->any_mdiff) k_adj_point<true><<<c->nblk, BS, 0, c->stream>>>(p);
+  else k_adj_point<false><<<c->nblk, BS, 0, c->stream>>>(p);
+  SV_HIP(c, hipGetLastError());
+  return adj_trans<0>(c, A, tape, 0, 0.0);
+}
+
+// adjoint of one step (tape slot): Ubar (after the step, with its mass sums
+// in psum) -> Ubar (before the step) + seed on cxx, cyy.  Six launches.
 static int adj_step(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
   k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, A->sbar);
-  k_adj_mcorr<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->sbar, c->L.vol, A->Ubar);
-  k_adj_filter<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol,
-                                              A->twh, c->L.hsum, A->Ubar, c->ctrl, A->regbar,
-                                              A->Rbar);
+  FiltArgs f{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twh, c->L.hsum, c->L.vol,
+             A->sbar, A->Ubar, c->ctrl, A->regbar, point_args(c, A, tape, 4, 0)};
+  if (c->any_mdiff) k_adj_filter<true><<<c->nblk, BS, 0, c->stream>>>(f);
+  else k_adj_filter<false><<<c->nblk, BS, 0, c->stream>>>(f);
   SV_HIP(c, hipGetLastError());
-  if (int rc = adj_stage<4>(c, A, tape, 0.0)) return rc;
-  if (int rc = adj_stage<3>(c, A, tape, 0.0)) return rc;
-  if (int rc = adj_stage<2>(c, A, tape, 0.0)) return rc;
-  return adj_stage<1>(c, A, tape, seed);
+  if (int rc = adj_trans<4>(c, A, tape, 0, 0.0)) return rc;   // + point 3 -> set 1
+  if (int rc = adj_trans<3>(c, A, tape, 1, 0.0)) return rc;   // + point 2 -> set 0
+  if (int rc = adj_trans<2>(c, A, tape, 0, 0.0)) return rc;   // + point 1 -> set 1
+  return adj_trans<1>(c, A, tape, 1, seed);
 }
 
 static int loss_accumulate(sv_ctx *c, sv_adj *A) {
@@ -587,8 +621,11 @@ int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
     const size_t R = (size_t)SV_US * c->ntot;
     if (int rc = build_transpose(c, A)) return rc;
     if (adj_alloc(c, &A->Ubar, R) || adj_alloc(c, &A->Rbar, R) || adj_alloc(c, &A->regbar, R) ||
-        adj_alloc(c, &A->own, R) || adj_alloc(c, &A->dbar, (size_t)SV_TAPE_D * c->ntot) ||
-        adj_alloc(c, &A->dbarm, (size_t)4 * c->ntot) || adj_alloc(c, &A->psum, c->nblk) ||
+        adj_alloc(c, &A->own[0], R) || adj_alloc(c, &A->own[1], R) ||
+        adj_alloc(c, &A->dbar[0], (size_t)SV_TAPE_D * c->ntot) ||
+        adj_alloc(c, &A->dbar[1], (size_t)SV_TAPE_D * c->ntot) ||
+        adj_alloc(c, &A->dbarm[0], (size_t)4 * c->ntot) ||
+        adj_alloc(c, &A->dbarm[1], (size_t)4 * c->ntot) || adj_alloc(c, &A->psum, c->nblk) ||
         adj_alloc(c, &A->sbar, c->nseg) || adj_alloc(c, &A->lpart, c->nblk) ||
         adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R))
       return SV_E_NOMEM;
@@ -610,7 +647,7 @@ int sv_rhs_vjp(sv_ctx *c, const double *const rbar[6], double *const ubar[6]) {
   (void)hipFree(scratch);
   if (rc) return rc;
   if (int r2 = upload_records(c, A->Rbar, rbar)) return r2;
-  if (int r2 = adj_stage<0>(c, A, A->tape.data(), 0.0)) return r2;
+  if (int r2 = adj_rhs(c, A, A->tape.data())) return r2;
   SV_HIP(c, hipStreamSynchronize(c->stream));
   return download_records(c, A->Ubar, ubar);
 }

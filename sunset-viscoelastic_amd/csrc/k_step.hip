@@ -43,7 +43,7 @@ constexpr int BS = SV_BLOCK;   // threads per block of the node kernels
 // links per batch in the halo-staged sweeps (global streams prefetched one
 // batch ahead)
 #ifndef SV_HALO_UNROLL
-#define SV_HALO_UNROLL 4
+#define SV_HALO_UNROLL 2
 #endif
 
 template <typename T>

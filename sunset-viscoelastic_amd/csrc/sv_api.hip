@@ -702,6 +702,13 @@ int sv_get_stencil_stats(sv_ctx *c, double *stats) {
   stats[2] = mx;                   // max ij_count
   stats[3] = (double)(c->L.valid ? c->L.nnz : c->sell_nnz);  // SELL entries incl. padding
   stats[4] = c->np - c->npfb;      // ghosts
+  stats[5] = 0.0;                  // halo records per node staged by the sweeps
+  if (c->L.valid && c->L.halo) {
+    int64_t tot = 0;
+    const int nb = (c->npfb + SV_BLOCK - 1) / SV_BLOCK;
+    SV_HIP(c, hipMemcpy(&tot, c->L.hoff + nb, sizeof(int64_t), hipMemcpyDeviceToHost));
+    stats[5] = (double)tot / c->npfb;
+  }
   return 0;
 }
 

@@ -303,10 +303,10 @@ class Solver:
         return dict(zip(KERNELS, k)), tot.value
 
     def stencil_stats(self):
-        st = np.zeros(5)
+        st = np.zeros(6)
         self._ck(lib().sv_get_stencil_stats(self.h, _d(st)))
         return dict(K_mean=st[0], Ks_mean=st[1], K_max=int(st[2]), sell_entries=int(st[3]),
-                    ghosts=int(st[4]))
+                    ghosts=int(st[4]), halo_per_node=st[5])
 
     def rhs(self, iRKstep=1):
         outs = [np.zeros(self.npfb) for _ in range(6)]
