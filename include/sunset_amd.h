@@ -226,6 +226,10 @@ int sv_rollout_forward(sv_ctx *ctx, int32_t nsteps, double *loss);
  * the context's state is left at the end of the rollout */
 int sv_rollout_vjp(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
                    double *const grad0[6]);
+/* gradient of the same quantity with respect to the parameters, from the last
+ * sv_rollout_vjp (or, for <rbar, rhs>, the last sv_rhs_vjp):
+ * grad[0] = d/dWi, grad[1] = d/dbeta, grad[2] = d/d(epsPTT) */
+int sv_adj_param_grad(sv_ctx *ctx, double grad[3]);
 /* device time (ms) of the last sv_rollout_forward and sv_rollout_vjp */
 int sv_adj_timing(sv_ctx *ctx, double *forward_ms, double *vjp_ms);
 

@@ -135,6 +135,7 @@ void or_set_num_threads(int n);
 typedef struct {
   int np;
   double *ro, *rou, *rov, *u, *v, *cxx, *cxy, *cyy;   /* [np] tangents */
+  double dWi, dbeta, deps;   /* tangent of the parameters Wi, beta, epsPTT (0 = state only) */
 } or_tangent;
 int  or_tangent_alloc(or_tangent *t, int np);
 void or_tangent_free(or_tangent *t);

@@ -17,7 +17,11 @@
  *     with the oracle's full-gradient reading) by the product rule, term by
  *     term;
  *   - RK stages and registers (step.F90:281-409) are linear combinations;
- *   - the mass correction (rhs_di.F90:667-720, mcorr): dro -= sum(dro*vol)/sum(vol).
+ *   - the mass correction (rhs_di.F90:667-720, mcorr): dro -= sum(dro*vol)/sum(vol);
+ *   - parameters (dWi, dbeta, deps of or_tangent): through the derived
+ *     constants of load_data.F90:132-148, visc_solvent = beta*visc_total,
+ *     lambda = Wi*L/U, cp = (1-beta)*visc_total/lambda, and the sPTT factor
+ *     fr = -(1 - 2 eps + eps*(cxx + cyy + 1))/lambda of calc_rhs_c.
  * The loss is L = (1/T) sum_{t=1..T} mean_i (cxx_i + cyy_i - 2) on the state
  * after each step; or_rollout_tlm returns L and dL = <grad L, dU>.
  *
@@ -110,6 +114,10 @@ void or_calc_all_rhs_di_tlm(const or_domain *d, const or_phys *p, const or_state
   }
   const double csq = p->csq, cs_ = p->visc_solvent, cp = p->visc_polymeric / p->lambda;
   const double eps = p->epsPTT, lam = p->lambda;
+  /* parameter tangents of the derived constants */
+  const double dlam = t->dWi * p->L_char / p->U_char;
+  const double dcs = t->dbeta * p->visc_total;
+  const double dcp = -t->dbeta * p->visc_total / lam - cp * dlam / lam;
   const double bfx = p->grav[0] + p->drive[0], bfy = p->grav[1] + p->drive[1];
 #pragma omp parallel for
   for (int i = 0; i < n; i++) {
@@ -133,8 +141,8 @@ void or_calc_all_rhs_di_tlm(const or_domain *d, const or_phys *p, const or_state
                          - du * r_ro - u * d_ro;
     const double d_tsv = dro * (u * vx + v * vy) + ro * (du * vx + u * dvx + dv * vy + v * dvy)
                          - dv * r_ro - v * d_ro;
-    const double d_fvu = cs_ * dlu + cp * (daxx + dayy);
-    const double d_fvv = cs_ * dlv + cp * (daxy + dbyy);
+    const double d_fvu = cs_ * dlu + cp * (daxx + dayy) + dcs * b[LU][i] + dcp * (axx + ayy);
+    const double d_fvv = cs_ * dlv + cp * (daxy + dbyy) + dcs * b[LV][i] + dcp * (axy + byy);
     dr[0][i] = d_ro;
     dr[1][i] = -d_tsu - csq * drx + dro * bfx + d_fvu;
     dr[2][i] = -d_tsv - csq * dry + dro * bfy + d_fvv;
@@ -147,7 +155,8 @@ void or_calc_all_rhs_di_tlm(const or_domain *d, const or_phys *p, const or_state
                           + dvy * cxy + vy * dcxy;
     const double d_ucyy = 2.0 * (dvx * cxy + vx * dcxy) + 2.0 * (dvy * cyy + vy * dcyy);
     const double fr = -(1.0 - eps * 2.0 + eps * (cxx + cyy + 1.0)) / lam;
-    const double dfr = -eps * (dcxx + dcyy) / lam;
+    const double dfr = -eps * (dcxx + dcyy) / lam - (cxx + cyy - 1.0) * t->deps / lam
+                       - fr * dlam / lam;
     double mxx = 0, mxy = 0, myy = 0;
     if (Md != 0.0) { mxx = Md * g[LXX][i]; mxy = Md * g[LXY][i]; myy = Md * g[LYY][i]; }
     dr[3][i] = d_adxx + d_ucxx + dfr * (cxx - 1.0) + fr * dcxx + mxx;

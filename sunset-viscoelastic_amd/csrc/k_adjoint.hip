@@ -73,6 +73,8 @@ struct PointArgs {
   const double *Rbar;        // [ntot][8]
   const double *gsx, *gsy, *lsum;
   double *dbar, *dbarm, *own;
+  double *pacc;              // [ntot][4] parameter-gradient accumulators (or null):
+                             // d/d(visc_solvent), d/d(cp), d/d(lambda), d/d(eps)
 };
 
 // the pointwise adjoint of one (active) node, Rbar given in registers
@@ -165,6 +167,19 @@ __device__ __forceinline__ void adj_point_node(const PointArgs &a, const NodeRef
     double2 *M = reinterpret_cast<double2 *>(a.dbarm + (size_t)i * 4);
     M[0] = double2{blxx, blxy};
     M[1] = double2{blyy, 0.0};
+  }
+  if (a.pacc) {
+    // r_rou, r_rov hold visc_solvent*(lu, lv) and cp*(div c); the conformation
+    // source fr*(c - I) holds lambda and eps through fr
+    const double lu = D[4], lv = D[7];
+    double2 *Pq = reinterpret_cast<double2 *>(a.pacc + (size_t)i * 4);
+    double2 p0 = Pq[0], p1 = Pq[1];
+    p0.x += Bu * lu + Bv * lv;
+    p0.y += Bu * (gxx_x + gxy_y) + Bv * (gxy_x + gyy_y);
+    p1.x += bfr * (-fr / P.lambda);
+    p1.y += bfr * (-(cxx + cyy - 1.0) / P.lambda);
+    Pq[0] = p0;
+    Pq[1] = p1;
   }
 }
 
@@ -395,6 +410,23 @@ __global__ __launch_bounds__(1024) void k_loss_acc(const sv_segk *seg, const dou
   if (threadIdx.x == 0) lacc[blockIdx.x] += s1[0] / S.npfb;
 }
 
+// sums of the parameter-gradient accumulators over the real nodes (one block,
+// fixed order: deterministic)
+__global__ __launch_bounds__(1024) void k_param_reduce(const double *pacc, int n, double *out) {
+  __shared__ double s1[4][1024];
+  double a[4] = {0, 0, 0, 0};
+  for (int k = threadIdx.x; k < n; k += 1024)
+    for (int q = 0; q < 4; q++) a[q] += pacc[(size_t)k * 4 + q];
+  for (int q = 0; q < 4; q++) s1[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 4; q++) s1[q][threadIdx.x] += s1[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = s1[threadIdx.x][0];
+}
+
 __global__ void k_zero(double *p, int64_t n) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0.0;
@@ -417,6 +449,8 @@ struct sv_adj {
   double *own[2] = {nullptr, nullptr}, *dbar[2] = {nullptr, nullptr};
   double *dbarm[2] = {nullptr, nullptr};
   double *psum = nullptr, *sbar = nullptr, *lpart = nullptr, *lacc = nullptr;
+  double *pacc = nullptr, *pred = nullptr;   // parameter-gradient accumulators, sums
+  double pgrad[3] = {0, 0, 0};               // d/d(Wi, beta, epsPTT) of the last VJP
   // rollout: checkpoints of U records, the end state, the tape of one interval
   int T = 0;                 // steps of the last forward rollout
   std::vector<double *> ckpt;
@@ -525,7 +559,26 @@ static int ensure_tape(sv_ctx *c, sv_adj *A) {
 // pointwise-adjoint arguments for stage k (1..4) of a tape slot, buffer set q
 static PointArgs point_args(sv_ctx *c, sv_adj *A, double *const *tape, int k, int q) {
   return PointArgs{c->blk_seg, c->segk, tape[2 * (k - 1)], tape[2 * (k - 1) + 1], A->Rbar,
-                   c->L.gsx, c->L.gsy, c->L.lsum, A->dbar[q], A->dbarm[q], A->own[q]};
+                   c->L.gsx, c->L.gsy, c->L.lsum, A->dbar[q], A->dbarm[q], A->own[q],
+                   A->pacc};
+}
+
+// Sum the accumulators and map d/d(visc_solvent, cp, lambda, eps) to the
+// parameters through the derived constants (load_data.F90:132-148):
+// visc_solvent = beta*vt, lambda = Wi*L/U, cp = (1 - beta)*vt/lambda.
+static int param_grad_finish(sv_ctx *c, sv_adj *A) {
+  k_param_reduce<<<1, 1024, 0, c->stream>>>(A->pacc, c->npfb, A->pred);
+  SV_HIP(c, hipGetLastError());
+  double g[4];
+  SV_HIP(c, hipMemcpyAsync(g, A->pred, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  const sv_physk &P = c->pk;
+  const double vt = P.visc_total, lam = P.lambda, cp = P.coef_polymeric;
+  const double dlam_dWi = P.L_char / P.U_char;
+  A->pgrad[0] = g[2] * dlam_dWi - g[1] * cp / lam * dlam_dWi;
+  A->pgrad[1] = g[0] * vt - g[1] * vt / lam;
+  A->pgrad[2] = g[3];
+  return 0;
 }
 
 // transposed gather of stage STAGE (its records in buffer set q); for STAGE
@@ -627,7 +680,8 @@ int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
         adj_alloc(c, &A->dbarm[0], (size_t)4 * c->ntot) ||
         adj_alloc(c, &A->dbarm[1], (size_t)4 * c->ntot) || adj_alloc(c, &A->psum, c->nblk) ||
         adj_alloc(c, &A->sbar, c->nseg) || adj_alloc(c, &A->lpart, c->nblk) ||
-        adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R))
+        adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R) ||
+        adj_alloc(c, &A->pacc, (size_t)4 * c->ntot) || adj_alloc(c, &A->pred, 4))
       return SV_E_NOMEM;
   }
   c->adj->every = checkpoint_every;
@@ -647,8 +701,9 @@ int sv_rhs_vjp(sv_ctx *c, const double *const rbar[6], double *const ubar[6]) {
   (void)hipFree(scratch);
   if (rc) return rc;
   if (int r2 = upload_records(c, A->Rbar, rbar)) return r2;
+  if (int r2 = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return r2;
   if (int r2 = adj_rhs(c, A, A->tape.data())) return r2;
-  SV_HIP(c, hipStreamSynchronize(c->stream));
+  if (int r2 = param_grad_finish(c, A)) return r2;
   return download_records(c, A->Ubar, ubar);
 }
 
@@ -735,6 +790,7 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   SV_HIP(c, hipEventCreate(&e0));
   SV_HIP(c, hipEventCreate(&e1));
   SV_HIP(c, hipEventRecord(e0, c->stream));
+  if (int rc = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return rc;
   k_adj_sum<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
   SV_HIP(c, hipGetLastError());
   const int nck = (T + E - 1) / E;
@@ -759,11 +815,18 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   A->t_bwd = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (int rc = param_grad_finish(c, A)) return rc;
   if (int rc = download_records(c, A->Ubar, grad0)) return rc;
   // leave the state at the end of the rollout
   SV_HIP(c, hipMemcpy(c->U, A->Uend, sizeof(double) * R, hipMemcpyDeviceToDevice));
   SV_HIP(c, hipMemcpy(c->ctrl, &A->ctrl_end, sizeof(sv_ctrl), hipMemcpyHostToDevice));
   return sv_dev_init_state(c);
+}
+
+int sv_adj_param_grad(sv_ctx *c, double grad[3]) {
+  SV_ADJ_CHECK(c, c && c->adj && grad, "sv_adj_setup must come first");
+  for (int k = 0; k < 3; k++) grad[k] = c->adj->pgrad[k];
+  return 0;
 }
 
 int sv_adj_timing(sv_ctx *c, double *fwd_ms, double *vjp_ms) {

@@ -161,6 +161,11 @@ module sv_capi
        type(c_ptr), value :: ubar_final       ! c_null_ptr or c_loc of 6 c_ptr
        type(c_ptr), intent(in) :: grad0(6)    ! c_loc of the six output arrays
      end function sv_rollout_vjp
+     integer(c_int) function sv_adj_param_grad(ctx, grad) bind(C, name='sv_adj_param_grad')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: grad(3)  ! d/dWi, d/dbeta, d/d(epsPTT)
+     end function sv_adj_param_grad
   end interface
 
   type(c_ptr), save :: sv_ctx = c_null_ptr

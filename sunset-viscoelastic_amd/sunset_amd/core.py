@@ -55,7 +55,7 @@ EXPORTS = [
     "sv_get_control", "sv_calc_all_rhs", "sv_run", "sv_get_state", "sv_sync",
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
-    "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_timing",
+    "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -110,6 +110,7 @@ def lib(path: Optional[str] = None):
     L.sv_rhs_vjp.argtypes = [C.c_void_p, C.POINTER(PD), C.POINTER(PD)]
     L.sv_rollout_forward.argtypes = [C.c_void_p, C.c_int32, PD]
     L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
+    L.sv_adj_param_grad.argtypes = [C.c_void_p, PD]
     L.sv_adj_timing.argtypes = [C.c_void_p, PD, PD]
     L.sv_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.sv_ensemble_size.argtypes = [C.c_void_p]
@@ -283,6 +284,9 @@ class Solver:
             p.driving_force[k] = drive[k]
         p.kol_amp, p.kol_k = kol_amp, kol_k
         self._ck(lib().sv_set_phys(self.h, C.byref(p)))
+        self.phys = dict(Re=Re, Wi=Wi, beta=beta, eps=eps, Ma=Ma, Mdiff=Mdiff, kol_amp=kol_amp,
+                         kol_k=kol_k, U_char=U_char, L_char=L_char, grav=tuple(grav),
+                         drive=tuple(drive))
 
     def set_initial(self, ic, dt=None, itime=0, dt_mode="pid", time=0.0,
                     emax=(1e-5, 1e-5, 1e-5)):
@@ -344,6 +348,13 @@ class Solver:
         pout = (PD * 6)(*[_d(a) for a in out])
         self._ck(lib().sv_rollout_vjp(self.h, loss_bar, pin, pout))
         return dict(zip(self.STATE_KEYS, out))
+
+    def adj_param_grad(self):
+        """d/d(Wi, beta, eps) of the quantity of the last rollout_vjp (or of
+        <rbar, rhs> for the last rhs_vjp)."""
+        g = np.zeros(3)
+        self._ck(lib().sv_adj_param_grad(self.h, _d(g)))
+        return dict(Wi=g[0], beta=g[1], eps=g[2])
 
     def adj_timing(self):
         f, b = C.c_double(), C.c_double()

@@ -50,7 +50,8 @@ class State(C.Structure):
 
 class Tangent(C.Structure):
     _fields_ = [("np", C.c_int), ("ro", PD), ("rou", PD), ("rov", PD), ("u", PD), ("v", PD),
-                ("cxx", PD), ("cxy", PD), ("cyy", PD)]
+                ("cxx", PD), ("cxy", PD), ("cyy", PD), ("dWi", C.c_double),
+                ("dbeta", C.c_double), ("deps", C.c_double)]
 
 
 _lib = None
@@ -284,17 +285,20 @@ class OracleSim:
     # ---- tangent-linear model (DI, fixed dt) ----
     STATE_KEYS = ("ro", "rou", "rov", "cxx", "cxy", "cyy")
 
-    def _tangent(self, dU):
+    def _tangent(self, dU, dp=None):
         t = Tangent()
         assert lib().or_tangent_alloc(C.byref(t), self.d.np) == 0
         arrs = [np.ascontiguousarray(dU[k], dtype=np.float64) for k in self.STATE_KEYS]
         lib().or_tangent_set(C.byref(self.d), C.byref(self.st), C.byref(t),
                              *[_pd(a) for a in arrs])
+        dp = dp or {}
+        t.dWi, t.dbeta, t.deps = dp.get("Wi", 0.0), dp.get("beta", 0.0), dp.get("eps", 0.0)
         return t
 
-    def rhs_tlm(self, dU):
-        """Tangent of calc_all_rhs (DI) at the current state along dU."""
-        t = self._tangent(dU)
+    def rhs_tlm(self, dU, dp=None):
+        """Tangent of calc_all_rhs (DI) at the current state along dU and the
+        parameter direction dp = {"Wi", "beta", "eps"}."""
+        t = self._tangent(dU, dp)
         n = self.d.npfb
         outs = [np.zeros(n) for _ in range(6)]
         ptrs = (PD * 6)(*[_pd(a) for a in outs])
@@ -303,9 +307,10 @@ class OracleSim:
         lib().or_tangent_free(C.byref(t))
         return dict(zip(("ro", "rou", "rov", "xx", "xy", "yy"), outs))
 
-    def rollout_tlm(self, dU, nsteps):
-        """Advance nsteps (fixed dt) with the tangent: (loss, dloss, final tangent)."""
-        t = self._tangent(dU)
+    def rollout_tlm(self, dU, nsteps, dp=None):
+        """Advance nsteps (fixed dt) with the tangent along dU and the parameter
+        direction dp: (loss, dloss, final tangent)."""
+        t = self._tangent(dU, dp)
         L, dL = C.c_double(), C.c_double()
         rc = lib().or_rollout_tlm(C.byref(self.d), C.byref(self.p), C.byref(self.st),
                                   C.byref(t), nsteps, C.byref(L), C.byref(dL))

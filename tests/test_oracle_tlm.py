@@ -92,3 +92,61 @@ def test_rollout_tangent_matches_central_differences_of_the_loss():
     fd = (loss(+1) - loss(-1)) / (2 * eps)
     assert abs(fd - dL) <= 1e-6 * abs(dL), (fd, dL)
     assert abs(dL) > 1e-8
+
+
+def test_parameter_tangent_of_the_rhs_matches_central_differences():
+    """Wi, beta and eps enter through lambda, the solvent and polymeric
+    viscosities and the sPTT factor; each direction alone, and a mix."""
+    ns = jittered_lattice(16, seed=5, h_over_s=1.8)
+    ic = _ic(ns)
+    zero = {k: np.zeros(ns.npfb) for k in KEYS}
+    ph = dict(PH, eps=0.25, Wi=1.3, beta=0.4)
+    for dp in ({"Wi": 1.0}, {"beta": 1.0}, {"eps": 1.0}, {"Wi": 0.3, "beta": -0.7, "eps": 0.5}):
+        tl = _sim(ns, ic, ph).rhs_tlm(zero, dp)
+        # the parameter terms are small beside the pressure and advection
+        # terms of the same right-hand side: a step of 1e-4 keeps the
+        # differences' rounding (~1e-16 |r| / h) well under the tolerance,
+        # and the truncation error is O(h^2)
+        h = 1e-4
+
+        def shifted(sign):
+            return dict(ph, **{k: ph[k] + sign * h * v for k, v in dp.items()})
+
+        rp = _sim(ns, ic, shifted(+1)).rhs(1)
+        rm = _sim(ns, ic, shifted(-1)).rhs(1)
+        for k in tl:
+            fd = (rp[k] - rm[k]) / (2 * h)
+            scale = max(np.abs(tl[k]).max(), np.abs(fd).max())
+            if k == "ro":       # the continuity equation does not see the parameters
+                assert scale == 0.0, (dp, k)
+                continue
+            if scale == 0.0:    # e.g. eps does not reach the momentum equations
+                continue
+            err = np.abs(fd - tl[k]).max() / scale
+            assert err < 1e-6, (dp, k, err)
+
+
+def test_parameter_tangent_of_the_rollout_matches_central_differences_of_the_loss():
+    ns = jittered_lattice(16, seed=6, h_over_s=1.8)
+    ic = _ic(ns)
+    zero = {k: np.zeros(ns.npfb) for k in KEYS}
+    ph = dict(PH, eps=0.1)
+    steps = 25
+    # dL/dbeta is ~1e-4 of L here: with a step of 1e-6 the rounding of the two
+    # 25-step losses (~1e-16 L / h) is ~1e-5 of the derivative; 1e-4 brings
+    # it under 1e-7 while the O(h^2) truncation stays ~1e-8
+    h = 1e-4
+
+    def loss(p):
+        o = _sim(ns, ic, p)
+        tot = 0.0
+        for _ in range(steps):
+            o.run(1)
+            tot += o.loss_trace()
+        return tot / steps
+
+    for name in ("Wi", "beta", "eps"):
+        _, dL, _ = _sim(ns, ic, ph).rollout_tlm(zero, steps, {name: 1.0})
+        fd = (loss(dict(ph, **{name: ph[name] + h})) - loss(dict(ph, **{name: ph[name] - h}))) / (2 * h)
+        assert abs(fd - dL) <= 1e-6 * abs(dL), (name, fd, dL)
+        assert abs(dL) > 1e-10, name
