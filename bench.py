@@ -240,11 +240,16 @@ def adjoint_leg(n=128, steps=400, every=20):
     s.set_state(U0, dt=2e-5)
     L = s.rollout_forward(steps)
     g = s.rollout_vjp(1.0)
+    pg = s.adj_param_grad()
     fwd, bwd = s.adj_timing()
     s.close()
     import numpy as np
     gn = math.sqrt(sum(float(np.dot(v, v)) for v in g.values()))
+    # initial velocity at fixed density: rou = ro*u, so dL/du = ro dL/drou
+    gu = math.sqrt(float(np.dot(ic["ro"] * g["rou"], ic["ro"] * g["rou"])
+                         + np.dot(ic["ro"] * g["rov"], ic["ro"] * g["rov"])))
     return dict(nodes=ns.npfb, steps=steps, checkpoint_every=every, loss=L, grad_norm=gn,
+                grad_velocity_norm=gu, dL_dWi=pg["Wi"], dL_dbeta=pg["beta"],
                 forward_ms=round(fwd, 2), vjp_ms=round(bwd, 2), ratio=round(bwd / fwd, 3))
 
 

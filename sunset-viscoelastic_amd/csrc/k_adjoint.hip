@@ -33,6 +33,10 @@ namespace {
 
 constexpr int BS = SV_BLOCK;
 
+#ifndef SV_ADJ_BATCH
+#define SV_ADJ_BATCH 4     // transposed links per batch of requests
+#endif
+
 __device__ __forceinline__ void ld6(const double *p, double f[6]) {
   const double2 *q = reinterpret_cast<const double2 *>(p);
   const double2 a = q[0], b = q[1], c = q[2];
@@ -211,6 +215,24 @@ struct TransArgs {
                              // fused (its Rbar stays in registers)
 };
 
+// one transposed link: the stencil-adjoint record d (+ m with Mdiff) of node i
+// weighted by the link's gradient and Laplacian weights
+template <bool MDIFF>
+__device__ __forceinline__ void trans_link(double acc[6], double wx, double wy, double wl,
+                                           const double2 d[7], const double2 m[2]) {
+  acc[0] += wx * d[0].x + wy * d[0].y;
+  acc[1] += wx * d[1].x + wy * d[1].y + wl * d[2].x;
+  acc[2] += wx * d[2].y + wy * d[3].x + wl * d[3].y;
+  acc[3] += wx * d[4].x + wy * d[4].y;
+  acc[4] += wx * d[5].x + wy * d[5].y;
+  acc[5] += wx * d[6].x + wy * d[6].y;
+  if (MDIFF) {
+    acc[3] += wl * m[0].x;
+    acc[4] += wl * m[0].y;
+    acc[5] += wl * m[1].x;
+  }
+}
+
 // STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar)
 template <int STAGE, bool MDIFF>
 __global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
@@ -221,25 +243,51 @@ __global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
   const int64_t b = a.toff[jj >> 6] + (jj & 63);
   const int K = nr.act ? a.tcnt[jj] : 0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = 0; k < K; k++) {
+  // links in batches of SV_ADJ_BATCH: every index and weight of a batch, then
+  // every gathered record, are requested before the first sum, so the
+  // dependent index -> record loads cost one memory round trip per batch
+  // rather than per link (the sums keep the link order)
+  int k = 0;
+  for (; k + SV_ADJ_BATCH <= K; k += SV_ADJ_BATCH) {
+    int ix[SV_ADJ_BATCH];
+    double wx[SV_ADJ_BATCH], wy[SV_ADJ_BATCH], wl[SV_ADJ_BATCH];
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++) {
+      const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+      ix[u] = a.tcol[e];
+      wx[u] = a.twx[e];
+      wy[u] = a.twy[e];
+      wl[u] = a.twl[e];
+    }
+    double2 d[SV_ADJ_BATCH][7], m[SV_ADJ_BATCH][2];
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++) {
+      const double2 *r = reinterpret_cast<const double2 *>(a.dbar + (size_t)ix[u] * SV_TAPE_D);
+#pragma unroll
+      for (int q = 0; q < 7; q++) d[u][q] = r[q];
+      if (MDIFF) {
+        const double2 *rm = reinterpret_cast<const double2 *>(a.dbarm + (size_t)ix[u] * 4);
+        m[u][0] = rm[0];
+        m[u][1] = rm[1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++)
+      trans_link<MDIFF>(acc, wx[u], wy[u], wl[u], d[u], m[u]);
+  }
+  for (; k < K; k++) {
     const int64_t e = b + (int64_t)k * SV_WAVE;
     const int i = a.tcol[e];
-    const double wx = a.twx[e], wy = a.twy[e], wl = a.twl[e];
-    const double2 *d = reinterpret_cast<const double2 *>(a.dbar + (size_t)i * SV_TAPE_D);
-    const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
-    acc[0] += wx * d0.x + wy * d0.y;
-    acc[1] += wx * d1.x + wy * d1.y + wl * d2.x;
-    acc[2] += wx * d2.y + wy * d3.x + wl * d3.y;
-    acc[3] += wx * d4.x + wy * d4.y;
-    acc[4] += wx * d5.x + wy * d5.y;
-    acc[5] += wx * d6.x + wy * d6.y;
+    const double2 *r = reinterpret_cast<const double2 *>(a.dbar + (size_t)i * SV_TAPE_D);
+    double2 d[7], m[2];
+#pragma unroll
+    for (int q = 0; q < 7; q++) d[q] = r[q];
     if (MDIFF) {
-      const double2 *m = reinterpret_cast<const double2 *>(a.dbarm + (size_t)i * 4);
-      const double2 m0 = m[0], m1 = m[1];
-      acc[3] += wl * m0.x;
-      acc[4] += wl * m0.y;
-      acc[5] += wl * m1.x;
+      const double2 *rm = reinterpret_cast<const double2 *>(a.dbarm + (size_t)i * 4);
+      m[0] = rm[0];
+      m[1] = rm[1];
     }
+    trans_link<MDIFF>(acc, a.twx[e], a.twy[e], a.twl[e], d, m);
   }
   double part = 0.0;
   if (nr.act) {
@@ -349,7 +397,29 @@ __global__ __launch_bounds__(BS) void k_adj_filter(FiltArgs a) {
   const int K = nr.act ? a.tcnt[jj] : 0;
   const double sv = a.sbar[nr.sg];
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = 0; k < K; k++) {
+  int k = 0;
+  for (; k + SV_ADJ_BATCH <= K; k += SV_ADJ_BATCH) {   // batched as in k_adj_trans
+    int ix[SV_ADJ_BATCH];
+    double w[SV_ADJ_BATCH], f[SV_ADJ_BATCH][6], vo[SV_ADJ_BATCH];
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++) {
+      const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+      ix[u] = a.tcol[e];
+      w[u] = a.twh[e];
+    }
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++) {
+      ld6(a.Ubar + (size_t)ix[u] * SV_US, f[u]);
+      vo[u] = a.vol[ix[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < SV_ADJ_BATCH; u++) {
+      f[u][0] -= vo[u] * sv;
+#pragma unroll
+      for (int q = 0; q < 6; q++) acc[q] += w[u] * f[u][q];
+    }
+  }
+  for (; k < K; k++) {
     const int64_t e = b + (int64_t)k * SV_WAVE;
     const int i = a.tcol[e];
     const double w = a.twh[e];
