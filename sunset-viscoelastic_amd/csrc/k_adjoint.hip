@@ -36,6 +36,15 @@ constexpr int BS = SV_BLOCK;
 #ifndef SV_ADJ_BATCH
 #define SV_ADJ_BATCH 4     // transposed links per batch of requests
 #endif
+// The reverse sweeps run blocks of AB threads, BS/AB of them per 256-node
+// block of the node layout, so that small node sets (a 128x128 lattice is 64
+// such blocks) still occupy every CU.
+#ifndef SV_ADJ_THREADS
+#define SV_ADJ_THREADS 64
+#endif
+constexpr int AB = SV_ADJ_THREADS;
+constexpr int SUBS = BS / AB;
+static_assert(BS % AB == 0 && AB % 64 == 0, "adjoint blocks are whole waves dividing SV_BLOCK");
 
 __device__ __forceinline__ void ld6(const double *p, double f[6]) {
   const double2 *q = reinterpret_cast<const double2 *>(p);
@@ -54,6 +63,19 @@ struct NodeRef {
   int sg, i, ii;
   bool act;
 };
+
+// node of this thread in sub-block sb (AB threads) of the node layout
+__device__ __forceinline__ NodeRef node_ref_sub(const int *blk_seg, const sv_segk *seg, int sb) {
+  NodeRef r;
+  const int bid = sb / SUBS;
+  r.sg = blk_seg[bid];
+  const sv_segk &S = seg[r.sg];
+  const int li = (bid - S.blk0) * BS + (sb % SUBS) * AB + (int)threadIdx.x;
+  r.act = li < S.npfb;
+  r.i = S.base + li;
+  r.ii = r.act ? r.i : S.base;
+  return r;
+}
 
 __device__ __forceinline__ NodeRef node_ref(const int *blk_seg, const sv_segk *seg, int bid) {
   NodeRef r;
@@ -188,8 +210,8 @@ __device__ __forceinline__ void adj_point_node(const PointArgs &a, const NodeRef
 }
 
 template <bool MDIFF>
-__global__ __launch_bounds__(BS) void k_adj_point(PointArgs a) {
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
+__global__ __launch_bounds__(AB) void k_adj_point(PointArgs a) {
+  const NodeRef nr = node_ref_sub(a.blk_seg, a.seg, sv_xcd_block());
   if (!nr.act) return;
   double rb[6];
   ld6(a.Rbar + (size_t)nr.i * SV_US, rb);
@@ -210,7 +232,7 @@ struct TransArgs {
   double *Rbar;              // STAGE 2..4: next (earlier) stage's Rbar
   double *regbar;            // STAGE 2..4: in/out
   double seed;               // STAGE 1: loss seed added to cxx and cyy
-  double *psum;              // STAGE 1: per-block sum of Ubar_ro (mass correction)
+  double *psum;              // STAGE 1: per-sub-block sum of Ubar_ro (mass correction)
   PointArgs nxt;             // STAGE 2..4: the pointwise adjoint of stage STAGE-1,
                              // fused (its Rbar stays in registers)
 };
@@ -235,10 +257,10 @@ __device__ __forceinline__ void trans_link(double acc[6], double wx, double wy, 
 
 // STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar)
 template <int STAGE, bool MDIFF>
-__global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
-  __shared__ double red[BS];
+__global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
+  __shared__ double red[AB];
   const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
+  const NodeRef nr = node_ref_sub(a.blk_seg, a.seg, bid);
   const int j = nr.i, jj = nr.ii;
   const int64_t b = a.toff[jj >> 6] + (jj & 63);
   const int K = nr.act ? a.tcnt[jj] : 0;
@@ -333,7 +355,7 @@ __global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
   if (STAGE == 1) {
     red[threadIdx.x] = part;
     __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
+    for (int s = AB / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
@@ -342,14 +364,14 @@ __global__ __launch_bounds__(BS) void k_adj_trans(TransArgs a) {
 }
 
 // per-block sums of Ubar_ro (the start of a backward sweep)
-__global__ __launch_bounds__(BS) void k_adj_sum(const int *blk_seg, const sv_segk *seg,
+__global__ __launch_bounds__(AB) void k_adj_sum(const int *blk_seg, const sv_segk *seg,
                                                 const double *Ubar, double *psum) {
-  __shared__ double red[BS];
+  __shared__ double red[AB];
   const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref(blk_seg, seg, bid);
+  const NodeRef nr = node_ref_sub(blk_seg, seg, bid);
   red[threadIdx.x] = nr.act ? Ubar[(size_t)nr.i * SV_US] : 0.0;
   __syncthreads();
-  for (int s = BS / 2; s > 0; s >>= 1) {
+  for (int s = AB / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
@@ -357,13 +379,13 @@ __global__ __launch_bounds__(BS) void k_adj_sum(const int *blk_seg, const sv_seg
 }
 
 // mass-correction adjoint: sbar[seg] = sum_j Ubar_ro,j / sum_j vol_j (one block
-// per segment), then Ubar_ro,i -= vol_i * sbar
+// per segment, over the segment's sub-block sums), then Ubar_ro,i -= vol_i * sbar
 __global__ __launch_bounds__(1024) void k_adj_sum_reduce(const sv_segk *seg, const double *psum,
                                                          double *sbar) {
   __shared__ double s1[1024];
   const sv_segk &S = seg[blockIdx.x];
   double a = 0.0;
-  for (int k = threadIdx.x; k < S.nblk; k += 1024) a += psum[S.blk0 + k];
+  for (int k = threadIdx.x; k < S.nblk * SUBS; k += 1024) a += psum[S.blk0 * SUBS + k];
   s1[threadIdx.x] = a;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {
@@ -390,8 +412,8 @@ struct FiltArgs {
 };
 
 template <bool MDIFF>
-__global__ __launch_bounds__(BS) void k_adj_filter(FiltArgs a) {
-  const NodeRef nr = node_ref(a.blk_seg, a.seg, sv_xcd_block());
+__global__ __launch_bounds__(AB) void k_adj_filter(FiltArgs a) {
+  const NodeRef nr = node_ref_sub(a.blk_seg, a.seg, sv_xcd_block());
   const int j = nr.i, jj = nr.ii;
   const int64_t b = a.toff[jj >> 6] + (jj & 63);
   const int K = nr.act ? a.tcnt[jj] : 0;
@@ -660,8 +682,8 @@ static int adj_trans(sv_ctx *c, sv_adj *A, double *const *tape, int q, double se
               A->dbar[q], A->dbarm[q], A->own[q], tape[2 * (k - 1)], c->ctrl, A->Ubar, A->Rbar,
               A->regbar, seed, A->psum, PointArgs{}};
   if (STAGE >= 2) t.nxt = point_args(c, A, tape, STAGE - 1, 1 - q);
-  if (c->any_mdiff) k_adj_trans<STAGE, true><<<c->nblk, BS, 0, c->stream>>>(t);
-  else k_adj_trans<STAGE, false><<<c->nblk, BS, 0, c->stream>>>(t);
+  if (c->any_mdiff) k_adj_trans<STAGE, true><<<c->nblk * SUBS, AB, 0, c->stream>>>(t);
+  else k_adj_trans<STAGE, false><<<c->nblk * SUBS, AB, 0, c->stream>>>(t);
   SV_HIP(c, hipGetLastError());
   return 0;
 }
@@ -669,8 +691,8 @@ static int adj_trans(sv_ctx *c, sv_adj *A, double *const *tape, int q, double se
 // calc_all_rhs vector-Jacobian product at tape slot 0 (Rbar in A->Rbar)
 static int adj_rhs(sv_ctx *c, sv_adj *A, double *const *tape) {
   const PointArgs p = point_args(c, A, tape, 1, 0);
-  if (c->any_mdiff) k_adj_point<true><<<c->nblk, BS, 0, c->stream>>>(p);
-  else k_adj_point<false><<<c->nblk, BS, 0, c->stream>>>(p);
+  if (c->any_mdiff) k_adj_point<true><<<c->nblk * SUBS, AB, 0, c->stream>>>(p);
+  else k_adj_point<false><<<c->nblk * SUBS, AB, 0, c->stream>>>(p);
   SV_HIP(c, hipGetLastError());
   return adj_trans<0>(c, A, tape, 0, 0.0);
 }
@@ -681,8 +703,8 @@ static int adj_step(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
   k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, A->sbar);
   FiltArgs f{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twh, c->L.hsum, c->L.vol,
              A->sbar, A->Ubar, c->ctrl, A->regbar, point_args(c, A, tape, 4, 0)};
-  if (c->any_mdiff) k_adj_filter<true><<<c->nblk, BS, 0, c->stream>>>(f);
-  else k_adj_filter<false><<<c->nblk, BS, 0, c->stream>>>(f);
+  if (c->any_mdiff) k_adj_filter<true><<<c->nblk * SUBS, AB, 0, c->stream>>>(f);
+  else k_adj_filter<false><<<c->nblk * SUBS, AB, 0, c->stream>>>(f);
   SV_HIP(c, hipGetLastError());
   if (int rc = adj_trans<4>(c, A, tape, 0, 0.0)) return rc;   // + point 3 -> set 1
   if (int rc = adj_trans<3>(c, A, tape, 1, 0.0)) return rc;   // + point 2 -> set 0
@@ -748,7 +770,8 @@ int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
         adj_alloc(c, &A->dbar[0], (size_t)SV_TAPE_D * c->ntot) ||
         adj_alloc(c, &A->dbar[1], (size_t)SV_TAPE_D * c->ntot) ||
         adj_alloc(c, &A->dbarm[0], (size_t)4 * c->ntot) ||
-        adj_alloc(c, &A->dbarm[1], (size_t)4 * c->ntot) || adj_alloc(c, &A->psum, c->nblk) ||
+        adj_alloc(c, &A->dbarm[1], (size_t)4 * c->ntot) ||
+        adj_alloc(c, &A->psum, (size_t)c->nblk * SUBS) ||
         adj_alloc(c, &A->sbar, c->nseg) || adj_alloc(c, &A->lpart, c->nblk) ||
         adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R) ||
         adj_alloc(c, &A->pacc, (size_t)4 * c->ntot) || adj_alloc(c, &A->pred, 4))
@@ -861,7 +884,7 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   SV_HIP(c, hipEventCreate(&e1));
   SV_HIP(c, hipEventRecord(e0, c->stream));
   if (int rc = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return rc;
-  k_adj_sum<<<c->nblk, BS, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
+  k_adj_sum<<<c->nblk * SUBS, AB, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
   SV_HIP(c, hipGetLastError());
   const int nck = (T + E - 1) / E;
   for (int k = nck - 1; k >= 0; k--) {
