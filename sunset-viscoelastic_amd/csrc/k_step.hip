@@ -22,8 +22,9 @@
 // reproducible run to run.
 //
 // Memory layout of the state (records, see sv_internal.h): what a neighbour
-// gather needs of node j sits in one 128-byte record (LC) or 64-byte record
-// (DI), so a link costs one cache line instead of one line per field.
+// gather needs of node j sits in one packed 80-byte record (LC) or 48-byte
+// record (DI), so a link costs at most two cache lines instead of one line
+// per field, and records of Morton-adjacent nodes share lines.
 #include "sv_internal.h"
 
 namespace {
@@ -504,9 +505,6 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
-  const int64_t h0 = a.hoff[bid];
-  halo_stage<FORM>(a.G, a.hidx, h0, (int)(a.hoff[bid + 1] - h0), hl, hz);
-  __syncthreads();
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
   double rx = 0, ry = 0, ux = 0, uy = 0, vx = 0, vy = 0, lu = 0, lv = 0;
@@ -516,6 +514,7 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   // The global streams (block-local column, three weights) of the next batch
   // of links are loaded before the current batch is consumed, so each lane
   // keeps SV_HALO_UNROLL links (26 bytes each) in flight; records come from LDS.
+  // The first batch is requested before the halo is staged.
   constexpr int U = SV_HALO_UNROLL;
   int jn[U];
   double wxn[U], wyn[U], wln[U];
@@ -530,6 +529,9 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
     }
   };
   fetch(0);
+  const int64_t h0 = a.hoff[bid];
+  halo_stage<FORM>(a.G, a.hidx, h0, (int)(a.hoff[bid + 1] - h0), hl, hz);
+  __syncthreads();
   for (int k = 0; k < K; k += U) {
     int jc[U];
     double wxc[U], wyc[U], wlc[U];
@@ -673,17 +675,6 @@ __global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
-  const int64_t h0 = a.hoff[bid];
-  const int nh = (int)(a.hoff[bid + 1] - h0);
-  for (int h = threadIdx.x; h < nh; h += BS) {
-    double f[6];
-    load_u(a.U, a.hidx[h0 + h], f);
-    double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * 6);
-    d[0] = double2{f[0], f[1]};
-    d[1] = double2{f[2], f[3]};
-    d[2] = double2{f[4], f[5]};
-  }
-  __syncthreads();
   const int64_t b = a.off[ii >> 6] + (ii & 63);
   const int K = act ? a.kcnt[ii] : 0;
   double t[6] = {0, 0, 0, 0, 0, 0};
@@ -699,7 +690,18 @@ __global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
         wn[u] = ld_stream(a.wh + e);
       }
   };
-  fetch(0);
+  fetch(0);   // the first batch is in flight while the halo is staged
+  const int64_t h0 = a.hoff[bid];
+  const int nh = (int)(a.hoff[bid + 1] - h0);
+  for (int h = threadIdx.x; h < nh; h += BS) {
+    double f[6];
+    load_u(a.U, a.hidx[h0 + h], f);
+    double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * 6);
+    d[0] = double2{f[0], f[1]};
+    d[1] = double2{f[2], f[3]};
+    d[2] = double2{f[4], f[5]};
+  }
+  __syncthreads();
   for (int k = 0; k < K; k += U) {
     int jc[U];
     double wc[U];

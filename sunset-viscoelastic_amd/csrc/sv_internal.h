@@ -66,12 +66,14 @@ struct sv_physk {
 
 // State records (real nodes only; ghosts are resolved through the folded
 // stencil columns).  Gather record of node j, what a neighbour sweep reads:
-//   [ro, u, v, cxx, cxy, cyy, psixx, psixy, psiyy]  (LC: 16 doubles = 128 B)
-//   [ro, u, v, cxx, cxy, cyy]                        (DI:  8 doubles =  64 B)
+//   [ro, u, v, cxx, cxy, cyy, psixx, psixy, psiyy]  (LC: 10 doubles =  80 B)
+//   [ro, u, v, cxx, cxy, cyy]                        (DI:  6 doubles =  48 B)
+// packed (16-byte aligned, no padding to a cache line): a sweep's gathers of
+// Morton-neighbouring records then read whole lines of useful fields.
 // State / register / error records: [ro, rou, rov, f1, f2, f3] in 8 doubles,
 // f = psi (LC) or c (DI).
-#define SV_GS_LC 16
-#define SV_GS_DI 8
+#define SV_GS_LC 10
+#define SV_GS_DI 6
 #define SV_US 8
 struct sv_adj;   // adjoint work space (k_adjoint.hip)
 struct sv_ctx;
