@@ -96,7 +96,7 @@ struct PointArgs {
   const int *blk_seg;
   const sv_segk *seg;
   const double *tG, *tD;     // stage tape
-  const double *Rbar;        // [ntot][8]
+  const double *Rbar;        // [ntot][SV_US]
   const double *gsx, *gsy, *lsum;
   double *dbar, *dbarm, *own;
   double *pacc;              // [ntot][4] parameter-gradient accumulators (or null):
@@ -534,7 +534,7 @@ struct sv_adj {
   int *tcnt = nullptr, *tcol = nullptr;
   double *twx = nullptr, *twy = nullptr, *twl = nullptr, *twh = nullptr;
   int64_t tnnz = 0;
-  // adjoint records [ntot][8] and stencil-adjoint records
+  // adjoint records [ntot][SV_US] and stencil-adjoint records
   double *Ubar = nullptr, *Rbar = nullptr, *regbar = nullptr;
   // stencil-adjoint records of two consecutive stages (ping-pong: a fused
   // kernel reads stage k's while writing stage k-1's)

@@ -231,7 +231,7 @@ struct StageArgs {
   const sv_ctrl *ctrl;  // [nseg]
   double *partial;      // stage 4: per-block max error
   int rk1;              // STAGE 0: reference stage counter is 1
-  // DI only, nullable: the stage's own gather record [ntot][8] and its
+  // DI only, nullable: the stage's own gather record [ntot][SV_US] and its
   // derivative values [ntot][16] (rx ry ux uy lu vx vy lv, cxx_x cxx_y, cxy_x
   // cxy_y, cyy_x cyy_y, 0 0), the tape the adjoint sweep reads
   double *tapeG, *tapeD;

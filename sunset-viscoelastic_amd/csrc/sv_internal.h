@@ -70,11 +70,11 @@ struct sv_physk {
 //   [ro, u, v, cxx, cxy, cyy]                        (DI:  6 doubles =  48 B)
 // packed (16-byte aligned, no padding to a cache line): a sweep's gathers of
 // Morton-neighbouring records then read whole lines of useful fields.
-// State / register / error records: [ro, rou, rov, f1, f2, f3] in 8 doubles,
-// f = psi (LC) or c (DI).
+// State / register / error records: [ro, rou, rov, f1, f2, f3], 6 doubles
+// (48 B, packed), f = psi (LC) or c (DI).
 #define SV_GS_LC 10
 #define SV_GS_DI 6
-#define SV_US 8
+#define SV_US 6
 struct sv_adj;   // adjoint work space (k_adjoint.hip)
 struct sv_ctx;
 void sv_adj_free(sv_ctx *c);
