@@ -10,10 +10,12 @@
 //      32-neighbour chunks of X and g staged in LDS;
 //   2. A is factorised in LDS by LU with partial pivoting (one column per lane);
 //   3. the right-hand sides (d/dx, d/dy, Laplacian, hyperviscous filter) are
-//      solved and improved by two steps of iterative refinement with the
-//      residual accumulated in double-double, which brings the solution to the
-//      exact solution of the FP64 matrix (the reference's Golub-Reinsch SVD,
-//      svdlib.F90, is only within cond(A)*eps of it);
+//      solved, one matrix row per lane with the substitution's values
+//      broadcast through the scalar unit, and improved by two steps of
+//      iterative refinement with the residual accumulated in double-double,
+//      which brings the solution to the exact solution of the FP64 matrix (the
+//      reference's Golub-Reinsch SVD, svdlib.F90, is only within cond(A)*eps
+//      of it);
 //   4. weights w_k = (psi . g_k)/hh are evaluated one neighbour per lane and
 //      written in CSR order.
 // The derivative system (h_small, small stencil) and the filter system (h, full
@@ -177,40 +179,45 @@ struct Lds {
   static constexpr int CHUNK = 2 * CH * NP;          // X and G chunks
   static constexpr int A0SZ = NP * LDA;
   static constexpr int U0 = CHUNK > A0SZ ? CHUNK : A0SZ;
-  static constexpr int TOTAL = U0 + NP * LDA + 3 * NP * 4;   // doubles
+  static constexpr int TOTAL = U0 + NP * LDA + 2 * NP * 4;   // doubles
 };
 
-// In-place solve of LU x = P b for the 4 columns of S ([NP][4]).
+// Value of v in lane k (k wave-uniform), through the scalar unit.
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
+// Solve L U x = s for 4 right-hand sides held one row per lane (lane r < N
+// holds row r of the permuted right-hand side in s[0..3]); L (unit lower) and
+// U are the LU factors in A ([NP][LDA], LDS), dinv (lane r) = 1/U[r][r].
+// Row k's value is broadcast from lane k through the scalar unit, so the
+// substitution has no LDS round trip and no barrier in its dependency chain;
+// the factor entries each lane needs are plain LDS reads off that chain.  On
+// return s holds row r of x.
 template <int N, int LDA>
-__device__ __forceinline__ void lu_solve4(const double *A, const int *piv, double *S, int l) {
-  if (l < 4)
-    for (int k = 0; k < N; k++) {
-      const int p = piv[k];
-      if (p != k) {
-        const double t = S[k * 4 + l];
-        S[k * 4 + l] = S[p * 4 + l];
-        S[p * 4 + l] = t;
-      }
-    }
-  __syncthreads();
+__device__ __forceinline__ void lu_solve_rows(const double *A, int r, double dinv,
+                                              double s[4]) {
+#pragma unroll 4
   for (int k = 0; k < N - 1; k++) {
-    for (int e = l; e < (N - k - 1) * 4; e += SV_WAVE) {
-      const int r = k + 1 + (e >> 2), q = e & 3;
-      S[r * 4 + q] -= A[r * LDA + k] * S[k * 4 + q];
+    const double lrk = (r > k && r < N) ? A[r * LDA + k] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double y = lane_bcast(s[q], k);
+      if (r > k) s[q] -= lrk * y;
     }
-    __syncthreads();
   }
+#pragma unroll 4
   for (int k = N - 1; k >= 0; k--) {
-    if (l < 4) {
-      const double d = A[k * LDA + k];
-      S[k * 4 + l] = S[k * 4 + l] / (d == 0.0 ? 1.0 : d);
+    const double dk = lane_bcast(dinv, k);
+    const double urk = (r < k) ? A[r * LDA + k] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double x = lane_bcast(s[q], k) * dk;
+      if (r == k) s[q] = x;
+      else if (r < k) s[q] -= urk * x;
     }
-    __syncthreads();
-    for (int e = l; e < k * 4; e += SV_WAVE) {
-      const int r = e >> 2, q = e & 3;
-      S[r * 4 + q] -= A[r * LDA + k] * S[k * 4 + q];
-    }
-    __syncthreads();
   }
 }
 
@@ -316,7 +323,7 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD
 
 // Solve one system; the block is one wavefront.
 template <int M>
-__device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) {
+__device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   using L = Lds<M>;
   constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH;
   double *Xc = lds;                 // [CH][NP]
@@ -324,8 +331,7 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
   double *A0 = lds;                 // [NP][LDA] (reuses the chunks after assembly)
   double *A = lds + L::U0;          // [NP][LDA] LU workspace
   double *Bm = A + NP * LDA;        // [NP][4] right-hand sides
-  double *Rm = Bm + NP * 4;         // [NP][4] residual / correction
-  double *Xs = Rm + NP * 4;         // [NP][4] solution
+  double *Xs = Bm + NP * 4;         // [NP][4] solution
   const int l = threadIdx.x;
   const int i = S_.sys_node[sys];
   const int kind = S_.sys_kind[sys];
@@ -359,13 +365,14 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
       else v = sysF ? hyp_rhs(mf, r) : 0.0;
     }
     Bm[e] = v;
-    Rm[e] = v;
     Xs[e] = 0.0;
   }
   __syncthreads();
 
   // ---- 2. LU with partial pivoting, one column per lane ----
+  // idx: the row permutation, one row per lane (row l of P A is row idx of A)
   bool sing = false;
+  int idx = l;
   for (int k = 0; k < N; k++) {
     double v = (l >= k && l < N) ? fabs(A[l * LDA + k]) : -1.0;
     int p = l;
@@ -376,7 +383,12 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
       if (vo > v || (vo == v && po < p)) { v = vo; p = po; }
     }
     if (v == 0.0) sing = true;
-    if (l == 0) piv[k] = p;
+    p = __builtin_amdgcn_readfirstlane(p);   // equal in every lane after the reduction
+    {
+      const int ik = __builtin_amdgcn_readlane(idx, k), ip = __builtin_amdgcn_readlane(idx, p);
+      if (l == k) idx = ip;
+      else if (l == p) idx = ik;
+    }
     if (p != k && l < N) {
       const double t = A[k * LDA + l];
       A[k * LDA + l] = A[p * LDA + l];
@@ -387,35 +399,80 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
     const double pivot = (pv == 0.0) ? 1.0 : pv;
     if (l > k && l < N) A[l * LDA + k] = A[l * LDA + k] / pivot;
     __syncthreads();
+    // rank-1 update of column l, rows in batches of 4 so that each lane has
+    // 8 independent LDS reads in flight
     if (l > k && l < N) {
       const double akc = A[k * LDA + l];
-      for (int r = k + 1; r < N; r++) A[r * LDA + l] -= A[r * LDA + k] * akc;
+      int r = k + 1;
+      for (; r + 4 <= N; r += 4) {
+        double m[4], a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          m[u] = A[(r + u) * LDA + k];
+          a[u] = A[(r + u) * LDA + l];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) A[(r + u) * LDA + l] = a[u] - m[u] * akc;
+      }
+      for (; r < N; r++) A[r * LDA + l] -= A[r * LDA + k] * akc;
     }
     __syncthreads();
   }
 
   // ---- 3. solve, then two refinement steps with a double-double residual ----
+  // One row per lane: lane r < N keeps row r of the solution x (xs) and of the
+  // residual (res) in registers; Xs (LDS) holds x for the residual's
+  // broadcast reads and for the weight evaluation.
+  const int r = l;
+  double dinv = 1.0;
+  if (r < N) {
+    const double d = A[r * LDA + r];
+    dinv = 1.0 / (d == 0.0 ? 1.0 : d);
+  }
+  double s4[4], xs[4], res[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    s4[q] = (r < N) ? Bm[idx * 4 + q] : 0.0;
+    xs[q] = 0.0;
+  }
   for (int it = 0; it < 3; it++) {
     if (it > 0) {
-      for (int e = l; e < N * 4; e += SV_WAVE) {
-        const int r = e >> 2, q = e & 3;
-        double sh = 0.0, sl = 0.0;
+      // res_r = b_r - sum_c A0[r][c] x_c, accumulated in double-double
+      double sh[4] = {0, 0, 0, 0}, sl[4] = {0, 0, 0, 0};
+      if (r < N)
         for (int cc = 0; cc < N; cc++) {
-          const double a = A0[r * LDA + cc], xv = Xs[cc * 4 + q];
-          const double p = a * xv;
-          const double pe = fma(a, xv, -p);
-          double t;
-          two_sum(sh, p, sh, t);
-          sl += t + pe;
+          const double a = A0[r * LDA + cc];
+          const double2 x01 = reinterpret_cast<const double2 *>(Xs + cc * 4)[0];
+          const double2 x23 = reinterpret_cast<const double2 *>(Xs + cc * 4)[1];
+          const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const double pr = a * xv[q];
+            const double pe = fma(a, xv[q], -pr);
+            double t;
+            two_sum(sh[q], pr, sh[q], t);
+            sl[q] += t + pe;
+          }
         }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
         double d, de;
-        two_sum(Bm[e], -sh, d, de);
-        Rm[e] = d + (de - sl);
+        two_sum(r < N ? Bm[r * 4 + q] : 0.0, -sh[q], d, de);
+        res[q] = d + (de - sl[q]);
       }
-      __syncthreads();
+      // permuted residual: row r of P res is row idx of res (lane idx)
+#pragma unroll
+      for (int q = 0; q < 4; q++) s4[q] = __shfl(res[q], idx);
     }
-    lu_solve4<N, LDA>(A, piv, Rm, l);
-    for (int e = l; e < N * 4; e += SV_WAVE) Xs[e] = (it == 0) ? Rm[e] : Xs[e] + Rm[e];
+    lu_solve_rows<N, LDA>(A, r, dinv, s4);
+#pragma unroll
+    for (int q = 0; q < 4; q++) xs[q] = (it == 0) ? s4[q] : xs[q] + s4[q];
+    __syncthreads();   // every lane's residual reads of Xs are done
+    if (r < N) {
+      double2 *X2 = reinterpret_cast<double2 *>(Xs + r * 4);
+      X2[0] = double2{xs[0], xs[1]};
+      X2[1] = double2{xs[2], xs[3]};
+    }
     __syncthreads();
   }
 
@@ -427,10 +484,9 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds, int *piv) 
 template <int M>
 __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights(SysArgs a) {
   extern __shared__ double lds[];
-  __shared__ int piv[64];
   const int sys = blockIdx.x;
   if (sys >= a.nsys) return;
-  solve_system<M>(a, sys, lds, piv);
+  solve_system<M>(a, sys, lds);
 }
 
 // Persistent variant: a fixed grid takes systems from a queue; systems are
@@ -439,7 +495,6 @@ __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights(SysArgs a) {
 template <int M>
 __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(SysArgs a) {
   extern __shared__ double lds[];
-  __shared__ int piv[64];
   __shared__ int next;
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
@@ -447,7 +502,7 @@ __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(Sy
     const int sys = next;
     __syncthreads();
     if (sys >= a.nsys) break;
-    solve_system<M>(a, sys, lds, piv);
+    solve_system<M>(a, sys, lds);
   }
 }
 
