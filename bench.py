@@ -102,6 +102,23 @@ def cpu_baseline(n_sample, steps, threads):
                        f"{med * 1e3:.1f} ms, OMP_NUM_THREADS={threads}")
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def weight_flops(order, k_mean):
+    """Algorithmic FP64 flops of one moment system (n unknowns, K links):
+    assembly 2 K n^2 (MFMA), LU 2/3 n^3, three solves of 4 right-hand sides
+    24 n^2, two residuals 16 n^2, weights 8 K n."""
+    n = (order * order + 3 * order) // 2
+    return 2.0 * k_mean * n * n + 2.0 / 3.0 * n ** 3 + 40.0 * n * n + 8.0 * k_mean * n
+
+
+def fp64_figure(flops, ms):
+    tf = flops / (ms / 1e3) / 1e12
+    return dict(bound="fp64", achieved=round(tf, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(tf / FP64_PEAK_TFLOPS, 4), flops=round(flops))
+
+
 def c3_weights(smin, smax):
     """Weight setup on the cylinder-in-channel node set: spacing smin at the
     cylinder growing linearly to smax over 1.5, +/-0.3 s jitter, LABFM order 8
@@ -132,6 +149,9 @@ def c3_weights(smin, smax):
     s.close()
     out.update(K_mean=round(st["K_mean"], 2), K_max=int(st["K_max"]),
                weight_solves_per_s=round(ns.npfb / (out["autotuned_ms"] / 1e3), 1))
+    fl = (out["order8"] * weight_flops(8, st["K_mean"])
+          + out["order4"] * weight_flops(4, st["K_mean"]))
+    out["fp64"] = fp64_figure(fl, out["autotuned_ms"])
     return out
 
 
@@ -368,6 +388,7 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "step_hbm_gbs": round(step_gbs, 1),
         "weight_solves_per_s": round(n / (t_w / 1e3), 1),
+        "weights_fp64": fp64_figure(n_sys * weight_flops(6, st["Ks_mean"]), t_w),
         "cylinder_channel_weights": c3,
         "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
