@@ -167,19 +167,27 @@ struct SysArgs {
 #define SV_WEIGHTS_WPE 0
 #endif
 // Measured on MI355X (1M order-6 systems): a 2-wave budget is 13% faster than
-// the compiler's default (1 wave, ~290 registers); order 8 is not helped.
+// the compiler's default (1 wave, ~290 registers).  Order 8 keeps the full
+// budget: its LDS allows one wave per SIMD, and 2 waves would spill.
 template <int M>
 struct RegOcc {
   static constexpr int W = SV_WEIGHTS_WPE > 0 ? SV_WEIGHTS_WPE : (M <= 6 ? 2 : 1);
 };
 
+// LDS of one system (doubles): the assembly's 16-neighbour chunks of X and g
+// rows ([CH][NP] each), then, in the same space, the LU workspace A
+// ([N][LDA], LDA odd against bank conflicts of column reads); after them the
+// original matrix A0 ([N][LDA], for the refinement residuals) and the
+// solution rows Xs ([NP][4]).  The right-hand sides live in registers, one
+// row per lane.  Order 8: 33 KB (4 waves per CU), order 6: 14 KB.
 template <int M>
 struct Lds {
-  static constexpr int N = nsize(M), NP = npad(M), LDA = NP + 1, CH = 32;
-  static constexpr int CHUNK = 2 * CH * NP;          // X and G chunks
-  static constexpr int A0SZ = NP * LDA;
-  static constexpr int U0 = CHUNK > A0SZ ? CHUNK : A0SZ;
-  static constexpr int TOTAL = U0 + NP * LDA + 2 * NP * 4;   // doubles
+  static constexpr int N = nsize(M), NP = npad(M), LDA = (N % 2 == 1) ? N : N + 1;
+  static constexpr int CH = 16;
+  static constexpr int CHUNK = 2 * CH * NP;
+  static constexpr int ASZ = N * LDA;
+  static constexpr int U0 = CHUNK > ASZ ? CHUNK : ASZ;
+  static constexpr int TOTAL = U0 + ASZ + NP * 4;
 };
 
 // Value of v in lane k (k wave-uniform), through the scalar unit.
@@ -237,14 +245,15 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
   for (int a = 0; a < T; a++)
 #pragma unroll
     for (int b = 0; b < T; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int role = l / CH;   // 0: monomial row, 1: ABF row, 2-3 (CH = 16): idle
   for (int c0 = 0; c0 < Kuse; c0 += CH) {
-    const int kk = c0 + (l & (CH - 1));
-    double *row = ((l >> 5) == 0 ? Xc : Gc) + (l & (CH - 1)) * NP;
-    if (kk < Kuse) {
+    const int kk = c0 + (l % CH);
+    double *row = (role == 0 ? Xc : Gc) + (l % CH) * NP;
+    if (role < 2 && kk < Kuse) {
       const int j = S_.col[base + kk];
       const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
       const double xs = x / hh, ys = y / hh;
-      if ((l >> 5) == 0) {
+      if (role == 0) {
         mono_row<M>(xs, ys, row);
       } else {
         const double rad = sqrt(x * x + y * y) / hh;
@@ -252,7 +261,7 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
       }
 #pragma unroll
       for (int a = N; a < NP; a++) row[a] = 0.0;
-    } else {
+    } else if (role < 2) {
 #pragma unroll
       for (int a = 0; a < NP; a++) row[a] = 0.0;
     }
@@ -273,7 +282,9 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
     __syncthreads();
   }
   // D layout of a 16x16 f64 tile (v_mfma_f64_16x16x4_f64): lane l holds rows
-  // (l>>4) + 4v, v = 0..3, of column l&15
+  // (l>>4) + 4v, v = 0..3, of column l&15; the padding rows and columns
+  // (>= N) are dropped.  Adst may alias the chunks: every lane is past its
+  // last chunk read (the barrier above).
 #pragma unroll
   for (int ti = 0; ti < T; ti++)
 #pragma unroll
@@ -281,7 +292,7 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int r = ti * 16 + (l >> 4) + 4 * v, c = tj * 16 + (l & 15);
-        Adst[r * LDA + c] = acc[ti][tj][v];
+        if (r < N && c < N) Adst[r * LDA + c] = acc[ti][tj][v];
       }
   __syncthreads();
 }
@@ -328,10 +339,9 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH;
   double *Xc = lds;                 // [CH][NP]
   double *Gc = lds + CH * NP;       // [CH][NP]
-  double *A0 = lds;                 // [NP][LDA] (reuses the chunks after assembly)
-  double *A = lds + L::U0;          // [NP][LDA] LU workspace
-  double *Bm = A + NP * LDA;        // [NP][4] right-hand sides
-  double *Xs = Bm + NP * 4;         // [NP][4] solution
+  double *A = lds;                  // [N][LDA] LU workspace (the chunks' space)
+  double *A0 = lds + L::U0;         // [N][LDA] original matrix
+  double *Xs = A0 + L::ASZ;         // [NP][4] solution
   const int l = threadIdx.x;
   const int i = S_.sys_node[sys];
   const int kind = S_.sys_kind[sys];
@@ -354,19 +364,15 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   }
   __syncthreads();
   for (int e = l; e < N * LDA; e += SV_WAVE) A0[e] = A[e];
-  // right-hand sides: q0 d/dx, q1 d/dy, q2 Laplacian (e3+e5), q3 filter
-  for (int e = l; e < NP * 4; e += SV_WAVE) {
-    const int r = e >> 2, q = e & 3;
-    double v = 0.0;
-    if (r < N) {
-      if (q == 0) v = (sysD && r == 0) ? 1.0 : 0.0;
-      else if (q == 1) v = (sysD && r == 1) ? 1.0 : 0.0;
-      else if (q == 2) v = (sysD && (r == 2 || r == 4)) ? 1.0 : 0.0;
-      else v = sysF ? hyp_rhs(mf, r) : 0.0;
-    }
-    Bm[e] = v;
-    Xs[e] = 0.0;
+  // right-hand sides of row l: q0 d/dx, q1 d/dy, q2 Laplacian (e3+e5), q3 filter
+  double b[4] = {0.0, 0.0, 0.0, 0.0};
+  if (l < N) {
+    b[0] = (sysD && l == 0) ? 1.0 : 0.0;
+    b[1] = (sysD && l == 1) ? 1.0 : 0.0;
+    b[2] = (sysD && (l == 2 || l == 4)) ? 1.0 : 0.0;
+    b[3] = sysF ? hyp_rhs(mf, l) : 0.0;
   }
+  for (int e = l; e < NP * 4; e += SV_WAVE) Xs[e] = 0.0;
   __syncthreads();
 
   // ---- 2. LU with partial pivoting, one column per lane ----
@@ -432,7 +438,7 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   double s4[4], xs[4], res[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    s4[q] = (r < N) ? Bm[idx * 4 + q] : 0.0;
+    s4[q] = __shfl(b[q], idx);   // row r of P b is row idx of b (lane idx)
     xs[q] = 0.0;
   }
   for (int it = 0; it < 3; it++) {
@@ -457,7 +463,7 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         double d, de;
-        two_sum(r < N ? Bm[r * 4 + q] : 0.0, -sh[q], d, de);
+        two_sum(b[q], -sh[q], d, de);
         res[q] = d + (de - sl[q]);
       }
       // permuted residual: row r of P res is row idx of res (lane idx)
