@@ -204,6 +204,23 @@ int sv_ensemble_get_control(sv_ctx *ensemble, int32_t member, double *dt, double
                             int32_t *itime, double *emax3, double *dt_cfl,
                             double *dt_parabolic);
 
+/* ---- Diagnostics of the current state ------------------------------------
+ * The reductions of statistics.F90 on the device, returned instead of written
+ * to the reference's statistics files.  member = 0 for a single simulation,
+ * the member index on an ensemble context (MASS_ENERGY and CONF only).
+ *   SV_DIAG_MASS_ENERGY (mass_and_energy_check, statistics.F90:305-355):
+ *     out[0] = sum (ro - rho_char) vol / sum vol, out[1] = sum vol (times
+ *     L_char^2), out[2] = sum ro |u|^2 / 2 vol / sum vol
+ *   SV_DIAG_CONF (conf_check, :357-442): volume means of cxx, cxy, cyy, czz,
+ *     rms of cxy, mean trace (czz = 1 in 2D)
+ *   SV_DIAG_KOLMOGOROV (kolmogorov_l2norm, :552-612): volume L2 errors of u,
+ *     v, cxx, cxy, cyy against u = cos 2y, cxx = 1 + 8 lambda^2 sin^2 2y,
+ *     cxy = -2 lambda sin 2y, cyy = 1
+ *   SV_DIAG_TG (error_TG, :614-654): relative L2 error of |u| against the
+ *     decaying Taylor-Green vortex at the context's time */
+enum { SV_DIAG_MASS_ENERGY = 0, SV_DIAG_CONF = 1, SV_DIAG_KOLMOGOROV = 2, SV_DIAG_TG = 3 };
+int sv_diagnostics(sv_ctx *ctx, int32_t kind, int32_t member, double *out);
+
 /* ---- Reverse mode (DI form, fixed time step) ----------------------------
  * No counterpart in the reference (it has no adjoint); these are the
  * vector-Jacobian products of the reference's own operators: calc_all_rhs

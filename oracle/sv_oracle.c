@@ -1102,3 +1102,78 @@ void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps) {
     st->itime++;
   }
 }
+
+/* ---- diagnostics (statistics.F90:305-654), serial loops in node order ---- */
+
+/* mass_and_energy_check (statistics.F90:305-355): out = {mass/vol, vol, ke/vol} */
+void or_diag_mass_energy(const or_domain *d, const or_phys *p, const or_state *st,
+                         double out[3]) {
+  double m = 0.0, vt = 0.0, ke = 0.0;
+  for (int i = 0; i < d->npfb; i++) {
+    const double ro = st->ro[i], dV = d->vol[i];
+    m += (ro - p->rho_char) * dV;
+    vt += dV;
+    ke += 0.5 * ro * (st->u[i] * st->u[i] + st->v[i] * st->v[i]) * dV;
+  }
+  const double L2 = p->L_char * p->L_char;
+  out[0] = (m * L2) / (vt * L2);
+  out[1] = vt * L2;
+  out[2] = (ke * L2) / (vt * L2);
+}
+
+/* conf_check (:357-442): means of cxx, cxy, cyy, czz (= 1 in 2D), rms cxy,
+ * mean trace */
+void or_diag_conf(const or_domain *d, const or_state *st, double out[6]) {
+  double vt = 0, sxx = 0, sxy = 0, syy = 0, szz = 0, sxy2 = 0, str = 0;
+  for (int i = 0; i < d->npfb; i++) {
+    const double dV = d->vol[i], czz = 1.0;
+    vt += dV;
+    sxx += st->cxx[i] * dV;
+    sxy += st->cxy[i] * dV;
+    syy += st->cyy[i] * dV;
+    szz += czz * dV;
+    sxy2 += st->cxy[i] * st->cxy[i] * dV;
+    str += (st->cxx[i] + st->cyy[i] + czz) * dV;
+  }
+  out[0] = sxx / vt; out[1] = sxy / vt; out[2] = syy / vt; out[3] = szz / vt;
+  out[4] = sqrt(sxy2 / vt);
+  out[5] = str / vt;
+}
+
+/* kolmogorov_l2norm (:552-612): L2 errors of u, v, cxx, cxy, cyy */
+void or_diag_kolmogorov(const or_domain *d, const or_phys *p, const or_state *st,
+                        double out[5]) {
+  double e[5] = {0, 0, 0, 0, 0}, vt = 0.0;
+  const double lam = p->lambda;
+  for (int i = 0; i < d->npfb; i++) {
+    const double y = d->y[i], dV = d->vol[i];
+    const double ue = cos(2.0 * y);
+    const double cxxe = 1.0 + 2.0 * lam * lam * 4.0 * pow(sin(2.0 * y), 2.0);
+    const double cxye = -lam * 2.0 * sin(2.0 * y);
+    vt += dV;
+    e[0] += dV * pow(st->u[i] - ue, 2.0);
+    e[1] += dV * pow(st->v[i], 2.0);
+    e[2] += dV * pow(st->cxx[i] - cxxe, 2.0);
+    e[3] += dV * pow(st->cxy[i] - cxye, 2.0);
+    e[4] += dV * pow(st->cyy[i] - 1.0, 2.0);
+  }
+  for (int k = 0; k < 5; k++) out[k] = sqrt(e[k] / vt);
+}
+
+/* error_TG (:614-654): relative L2 error of |u| against the decaying vortex */
+double or_diag_tg(const or_domain *d, const or_phys *p, const or_state *st) {
+  const double pi = 3.14159265358979323846;
+  const double Tc = p->L_char / p->U_char;
+  const double expo = exp(-8.0 * pi * pi * st->time / Tc / p->Re);
+  double es = 0.0, ex = 0.0;
+  for (int i = 0; i < d->npfb; i++) {
+    const double x = d->x[i], y = d->y[i];
+    const double ue = -expo * cos(2.0 * pi * x) * sin(2.0 * pi * y);
+    const double ve = expo * sin(2.0 * pi * x) * cos(2.0 * pi * y);
+    const double Uex = sqrt(ue * ue + ve * ve);
+    const double Un = sqrt(st->u[i] * st->u[i] + st->v[i] * st->v[i]);
+    es += (Un - Uex) * (Un - Uex);
+    ex += Uex * Uex;
+  }
+  return sqrt(es / ex);
+}

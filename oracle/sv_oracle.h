@@ -131,6 +131,14 @@ void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps);
 int  or_num_threads(void);
 void or_set_num_threads(int n);
 
+/* ---- diagnostics (statistics.F90:305-654) ---- */
+void   or_diag_mass_energy(const or_domain *d, const or_phys *p, const or_state *st,
+                           double out[3]);
+void   or_diag_conf(const or_domain *d, const or_state *st, double out[6]);
+void   or_diag_kolmogorov(const or_domain *d, const or_phys *p, const or_state *st,
+                          double out[5]);
+double or_diag_tg(const or_domain *d, const or_phys *p, const or_state *st);
+
 /* ---- tangent-linear model of the DI step at a fixed dt (sv_oracle_tlm.c) ---- */
 typedef struct {
   int np;

@@ -161,6 +161,13 @@ module sv_capi
        type(c_ptr), value :: ubar_final       ! c_null_ptr or c_loc of 6 c_ptr
        type(c_ptr), intent(in) :: grad0(6)    ! c_loc of the six output arrays
      end function sv_rollout_vjp
+     integer(c_int) function sv_diagnostics(ctx, kind, member, out) &
+          bind(C, name='sv_diagnostics')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: kind, member    ! kind: 0 mass/energy, 1 conf, 2 Kolmogorov, 3 TG
+       real(c_double), intent(out) :: out(*)
+     end function sv_diagnostics
      integer(c_int) function sv_adj_param_grad(ctx, grad) bind(C, name='sv_adj_param_grad')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -56,6 +56,7 @@ EXPORTS = [
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
+    "sv_diagnostics",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -111,6 +112,7 @@ def lib(path: Optional[str] = None):
     L.sv_rollout_forward.argtypes = [C.c_void_p, C.c_int32, PD]
     L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
     L.sv_adj_param_grad.argtypes = [C.c_void_p, PD]
+    L.sv_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, PD]
     L.sv_adj_timing.argtypes = [C.c_void_p, PD, PD]
     L.sv_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.sv_ensemble_size.argtypes = [C.c_void_p]
@@ -349,6 +351,13 @@ class Solver:
         self._ck(lib().sv_rollout_vjp(self.h, loss_bar, pin, pout))
         return dict(zip(self.STATE_KEYS, out))
 
+    def diagnostics(self, kind: str):
+        """On-device statistics.F90 reductions of the current state:
+        "mass_energy" (mass/vol, vol, ke/vol), "conf" (means of cxx, cxy,
+        cyy, czz, rms cxy, mean trace), "kolmogorov" (L2 errors of u, v, cxx,
+        cxy, cyy), "tg" (relative L2 error of |u|)."""
+        return _diagnostics(self.h, self._ck, kind, 0)
+
     def adj_param_grad(self):
         """d/d(Wi, beta, eps) of the quantity of the last rollout_vjp (or of
         <rbar, rhs> for the last rhs_vjp)."""
@@ -384,6 +393,16 @@ class Solver:
 
 
 _FIELD_KEYS = ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy")
+
+
+DIAG = {"mass_energy": (0, 3), "conf": (1, 6), "kolmogorov": (2, 5), "tg": (3, 1)}
+
+
+def _diagnostics(h, ck, kind, member):
+    k, n = DIAG[kind]
+    out = np.zeros(n)
+    ck(lib().sv_diagnostics(h, k, member, _d(out)))
+    return out
 
 
 class Ensemble:
@@ -422,6 +441,10 @@ class Ensemble:
     def _ck(self, rc):
         if rc != 0:
             raise SunsetError(f"error {rc}: {lib().sv_last_error(self.h).decode()}")
+
+    def diagnostics(self, member: int, kind: str):
+        """Member's "mass_energy" or "conf" reductions (see Solver.diagnostics)."""
+        return _diagnostics(self.h, self._ck, kind, member)
 
     def run(self, nsteps: int, graph: bool = False):
         self._ck(lib().sv_run(self.h, nsteps, 1 if graph else 0))

@@ -94,6 +94,13 @@ def lib():
                                      C.POINTER(Tangent), C.c_int, PD, PD]
         L.or_loss_trace.argtypes = [C.POINTER(Domain), C.POINTER(State)]
         L.or_loss_trace.restype = C.c_double
+        L.or_diag_mass_energy.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State),
+                                          PD]
+        L.or_diag_conf.argtypes = [C.POINTER(Domain), C.POINTER(State), PD]
+        L.or_diag_kolmogorov.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State),
+                                         PD]
+        L.or_diag_tg.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State)]
+        L.or_diag_tg.restype = C.c_double
         L.or_set_num_threads.argtypes = [C.c_int]
         L.or_num_threads.restype = C.c_int
         _lib = L
@@ -322,3 +329,20 @@ class OracleSim:
 
     def loss_trace(self):
         return lib().or_loss_trace(C.byref(self.d), C.byref(self.st))
+
+    def diagnostics(self, kind):
+        """statistics.F90 reductions: "mass_energy" (3), "conf" (6),
+        "kolmogorov" (5), "tg" (1 value)."""
+        d, p, st = C.byref(self.d), C.byref(self.p), C.byref(self.st)
+        if kind == "mass_energy":
+            out = np.zeros(3)
+            lib().or_diag_mass_energy(d, p, st, _pd(out))
+        elif kind == "conf":
+            out = np.zeros(6)
+            lib().or_diag_conf(d, st, _pd(out))
+        elif kind == "kolmogorov":
+            out = np.zeros(5)
+            lib().or_diag_kolmogorov(d, p, st, _pd(out))
+        else:
+            out = np.array([lib().or_diag_tg(d, p, st)])
+        return out
