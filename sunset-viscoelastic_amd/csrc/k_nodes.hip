@@ -144,68 +144,95 @@ __device__ __forceinline__ double dist2(double xi, double yi, double xj, double 
   return t;
 }
 
-// pass 0: count; pass 1: write (col, r2) then sort the row by (r2, col)
+// Wave-scope ordering of LDS traffic between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int NW = 4;   // nodes (wavefronts) per block of the neighbour search
+
+// One wavefront per node.  The cells within the node's own search radius
+// 2 h_i (one cell of margin) are scanned row by row: a row of cells is one
+// contiguous range of the cell-sorted node list, taken 64 candidates at a
+// time, and matches are compacted with a ballot.  Pass 0 counts; pass 1
+// collects (j, r^2) in LDS, ranks every entry by (r^2, j) -- the order of the
+// reference's sorted lists (neighbours.F90:247-341) -- and writes the row in
+// that order, with the count of entries inside 2 h_small.  The cell size is
+// 2 h_min, so nodes of every size scan a few candidates per neighbour.
 template <int PASS>
-__global__ void k_neigh(int npfb, Grid G, const double *x, const double *y, const double *h,
-                        const double *hs, const int *cellof, const int *cstart, const int *ip,
-                        int *cnt, int *cnt_small, const int64_t *row_ptr, int *col, double *r2a) {
+__global__ __launch_bounds__(NW * SV_WAVE) void k_neigh(
+    int npfb, Grid G, const double *x, const double *y, const double *h, const double *hs,
+    const int *cstart, const int *ip, int *cnt, int *cnt_small, const int64_t *row_ptr,
+    int *col, int cap) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npfb) return;
+  extern __shared__ double nb_lds[];
+  const int wave = (int)threadIdx.x / SV_WAVE, lane = (int)threadIdx.x % SV_WAVE;
+  const int i = (int)blockIdx.x * NW + wave;
+  if (i >= npfb) return;   // whole wavefronts; no block-wide barrier follows
+  double *r2s = nb_lds + (size_t)wave * cap;
+  int *js = reinterpret_cast<int *>(nb_lds + (size_t)NW * cap) + (size_t)wave * cap;
   const double xi = x[i], yi = y[i];
-  const double hh = h[i] * SS;
-  const double st2 = hh * hh;
-  const int c = cellof[i];
-  const int cx = c % G.ncx, cy = c / G.ncx;
+  const double R = h[i] * SS;
+  const double st2 = R * R;
+  int ix0 = (int)((xi - R - G.x0) * G.inv) - 1, ix1 = (int)((xi + R - G.x0) * G.inv) + 1;
+  int iy0 = (int)((yi - R - G.y0) * G.inv) - 1, iy1 = (int)((yi + R - G.y0) * G.inv) + 1;
+  ix0 = ix0 < 0 ? 0 : ix0;
+  iy0 = iy0 < 0 ? 0 : iy0;
+  ix1 = ix1 >= G.ncx ? G.ncx - 1 : ix1;
+  iy1 = iy1 >= G.ncy ? G.ncy - 1 : iy1;
   int n = 0;
-  int64_t base = 0;
-  if (PASS == 1) base = row_ptr[i];
-  for (int oy = -1; oy <= 1; oy++) {
-    const int yy = cy + oy;
-    if (yy < 0 || yy >= G.ncy) continue;
-    for (int ox = -1; ox <= 1; ox++) {
-      const int xx = cx + ox;
-      if (xx < 0 || xx >= G.ncx) continue;
-      const int cc = yy * G.ncx + xx;
-      const int e = cstart[cc + 1];
-      for (int jj = cstart[cc]; jj < e; jj++) {
-        const int j = ip[jj];
-        const double r2 = dist2(xi, yi, x[j], y[j]);
-        if (r2 <= st2) {
-          if (PASS == 1) {
-            col[base + n] = j;
-            r2a[base + n] = r2;
-          }
-          n++;
+  for (int cy = iy0; cy <= iy1; cy++) {
+    const int lo = cstart[cy * G.ncx + ix0], hi = cstart[cy * G.ncx + ix1 + 1];
+    for (int b0 = lo; b0 < hi; b0 += SV_WAVE) {
+      const int jj = b0 + lane;
+      bool m = false;
+      double r2 = 0.0;
+      int j = 0;
+      if (jj < hi) {
+        j = ip[jj];
+        r2 = dist2(xi, yi, x[j], y[j]);
+        m = r2 <= st2;
+      }
+      const unsigned long long bal = __ballot(m);
+      if (PASS == 1 && m) {
+        const int pos = n + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < cap) {
+          r2s[pos] = r2;
+          js[pos] = j;
         }
       }
+      n += __popcll(bal);
     }
   }
   if (PASS == 0) {
-    cnt[i] = n;
+    if (lane == 0) cnt[i] = n;
     return;
   }
-  // insertion sort of the row by (r2, j); rows are short (tens to ~200)
-  for (int a = 1; a < n; a++) {
-    const double rv = r2a[base + a];
-    const int jv = col[base + a];
-    int b = a - 1;
-    while (b >= 0) {
-      const double rb = r2a[base + b];
-      const int jb = col[base + b];
-      if (rb < rv || (rb == rv && jb < jv)) break;
-      r2a[base + b + 1] = rb;
-      col[base + b + 1] = jb;
-      b--;
-    }
-    r2a[base + b + 1] = rv;
-    col[base + b + 1] = jv;
-  }
+  wave_sync();
+  // the same scan as pass 0, so n == cnt[i] <= cap; clamped all the same so
+  // that a row can never spill into the next one
+  n = n < cnt[i] ? n : cnt[i];
+  const int64_t base = row_ptr[i];
   const double hsm = hs[i] * SS;
   const double st2s = hsm * hsm;
   int ks = 0;
-  for (int a = 0; a < n; a++) ks += (r2a[base + a] <= st2s) ? 1 : 0;
-  cnt_small[i] = ks;
+  for (int a = lane; a < n; a += SV_WAVE) {
+    const double ra = r2s[a];
+    const int ja = js[a];
+    int rank = 0;
+    for (int b = 0; b < n; b++) {
+      const double rb = r2s[b];
+      const int jb = js[b];
+      rank += (rb < ra || (rb == ra && jb < ja)) ? 1 : 0;
+    }
+    col[base + rank] = ja;
+    ks += (ra <= st2s) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) ks += __shfl_xor(ks, o);
+  if (lane == 0) cnt_small[i] = ks;
 }
 
 __global__ void k_sell_pack(int npfb, const int64_t *sell_off, const int *sell_w,
@@ -291,7 +318,7 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   SV_HIP(c, hipEventRecord(e0, c->stream));
   const int np = c->np, npfb = c->npfb;
   const double Lx = c->xmax - c->xmin, Ly = c->ymax - c->ymin;
-  double csz = SS * c->hmax;
+  double csz = SS * (c->hmin > 0.0 ? c->hmin : c->hmax);
   // keep the padded grid below ~64M cells
   const double area = 9.0 * Lx * Ly;
   if (area / (csz * csz) > 6.4e7) csz = sqrt(area / 6.4e7);
@@ -321,9 +348,9 @@ int sv_dev_find_neighbours(sv_ctx *c) {
     if (sv_alloc(c, (void **)&c->cnt_small, sizeof(int) * npfb)) return SV_E_NOMEM;
     if (sv_alloc(c, (void **)&c->row_ptr, sizeof(int64_t) * (npfb + 1))) return SV_E_NOMEM;
   }
-  const int nb = (npfb + B - 1) / B;
-  k_neigh<0><<<nb, B, 0, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cellof, cstart, ip,
-                                      c->cnt, c->cnt_small, nullptr, nullptr, nullptr);
+  const int nb = (npfb + NW - 1) / NW;
+  k_neigh<0><<<nb, NW * SV_WAVE, 0, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cstart, ip,
+                                                 c->cnt, c->cnt_small, nullptr, nullptr, 0);
   SV_HIP(c, hipGetLastError());
   std::vector<int> hc(npfb);
   SV_HIP(c, hipMemcpyAsync(hc.data(), c->cnt, sizeof(int) * npfb, hipMemcpyDeviceToHost,
@@ -342,10 +369,13 @@ int sv_dev_find_neighbours(sv_ctx *c) {
                            hipMemcpyHostToDevice, c->stream));
   if (c->col_raw) sv_free(c, c->col_raw);
   if (sv_alloc(c, (void **)&c->col_raw, sizeof(int) * (c->nnz + 1))) return SV_E_NOMEM;
-  double *r2a = nullptr;
-  SV_HIP(c, hipMalloc(&r2a, sizeof(double) * (c->nnz + 1)));
-  k_neigh<1><<<nb, B, 0, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cellof, cstart, ip,
-                                      c->cnt, c->cnt_small, c->row_ptr, c->col_raw, r2a);
+  const int cap = maxk > 0 ? maxk : 1;
+  const size_t shm = (size_t)NW * cap * (sizeof(double) + sizeof(int));
+  if (shm > 64 * 1024)
+    return sv_fail(c, SV_E_OVERFLOW, "neighbour list of %d entries too long for the search", maxk);
+  k_neigh<1><<<nb, NW * SV_WAVE, shm, c->stream>>>(npfb, G, c->x, c->y, c->h, c->hs, cstart, ip,
+                                                   c->cnt, c->cnt_small, c->row_ptr, c->col_raw,
+                                                   cap);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipEventRecord(e1, c->stream));
   SV_HIP(c, hipStreamSynchronize(c->stream));
@@ -354,7 +384,6 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   c->t_neigh = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  SV_HIP(c, hipFree(r2a));
   SV_HIP(c, hipFree(cellof));
   SV_HIP(c, hipFree(ccount));
   SV_HIP(c, hipFree(cstart));

@@ -174,9 +174,13 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
     std::vector<int> nt(npfb, 999);
     SV_HIP(c, hipMemcpy(c->node_type, nt.data(), sizeof(int) * npfb, hipMemcpyHostToDevice));
   }
-  double hm = 0;
-  for (int i = 0; i < npfb; i++) hm = h[i] > hm ? h[i] : hm;
+  double hm = 0, hn = 0;
+  for (int i = 0; i < npfb; i++) {
+    hm = h[i] > hm ? h[i] : hm;
+    hn = (i == 0 || h[i] < hn) ? h[i] : hn;
+  }
   c->hmax = hm;
+  c->hmin = hn;
   return sv_dev_create_ghosts(c);
 }
 

@@ -148,7 +148,7 @@ struct sv_ctx {
   int npfb = 0, np = 0, cap = 0;
   double *x = nullptr, *y = nullptr, *h = nullptr, *hs = nullptr, *s = nullptr;
   int *node_type = nullptr, *irel = nullptr, *vrel = nullptr;
-  double hmax = 0;
+  double hmax = 0, hmin = 0;
   // neighbours
   bool have_lists = false;
   int nplink = 0;
