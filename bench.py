@@ -140,9 +140,14 @@ def c3_weights(smin, smax):
         out[f"{name}_ms"] = round(t, 2)
         out["systems"] = int(s.timing("systems"))
     out["autotune_ms"] = round(s.timing("weights_tune"), 2)
-    out["autotuned_variants"] = {f"order{m}": ("persistent" if s.timing(f"weights_variant_m{m}")
-                                               == 1 else "one block per system")
+    def variant_name(v):
+        v = int(v)
+        return (("persistent grid" if v & 1 else "one block per system")
+                + (", 32-neighbour tiles" if v & 2 else ", 16-neighbour tiles"))
+    out["autotuned_variants"] = {f"order{m}": variant_name(s.timing(f"weights_variant_m{m}"))
                                  for m in (4, 8)}
+    out["autotune_sample_ms"] = {f"order{m}": [round(s.timing(f"weights_tune_m{m}_{v}"), 3)
+                                               for v in range(4)] for m in (8,)}
     s.calc_sums()
     s.filter_coefficients()
     st = s.stencil_stats()

@@ -108,17 +108,22 @@ int sv_get_weights(sv_ctx *ctx, int32_t nplink, double *ij_w_grad, double *ij_w_
  * call, if any), "step" (the last sv_run);
  * "systems" gives the number of moment systems solved by the last
  * sv_calc_labf_weights; "weights_variant_m4|6|8" the launch variant used for
- * that order (0: one block per system, 1: persistent grid; -1 none);
+ * that order (bit 0: persistent grid, else one block per system; bit 1:
+ * 32-neighbour assembly tiles, else 16; -1 none);
  * "weights_tune_m<order>_<variant>" the autotuning sample time of a variant. */
 int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
 /* Tuning switches:
  *   "weights_autotune" (default 1): for an order with >= 16384 systems, the
- *       first sv_calc_labf_weights times the launch variants (one block per
- *       system / a persistent grid fed from a work queue) on a strided
- *       4096-system sample and uses the faster; the choice is kept for later
- *       calls (setting the option again re-tunes).
+ *       first sv_calc_labf_weights times the four launch variants (one block
+ *       per system or a persistent grid fed from a work queue, times assembly
+ *       tiles of 16 or 32 neighbours) on a strided 4096-system sample and uses
+ *       the fastest; the choice is kept for later calls (setting the option
+ *       again re-tunes).
  *   "weights_persistent" (0/1): fixes the launch to one-block-per-system (0) or
- *       the persistent grid (1) and turns the autotuning off.
+ *       the persistent grid (1), with 16-neighbour tiles, and turns the
+ *       autotuning off.
+ *   "weights_variant" (0-3): fixes the launch variant (bits as in
+ *       "weights_variant_m<order>") and turns the autotuning off.
  *   "loop_order" (default 1, before sv_set_initial): 1 runs the time loop in
  *       Morton order of the nodes, 0 in the caller's order.
  *   "halo_sweeps" (default 1, before sv_set_initial): the RK sweeps copy each

@@ -174,16 +174,17 @@ struct RegOcc {
   static constexpr int W = SV_WEIGHTS_WPE > 0 ? SV_WEIGHTS_WPE : (M <= 6 ? 2 : 1);
 };
 
-// LDS of one system (doubles): the assembly's 16-neighbour chunks of X and g
+// LDS of one system (doubles): the assembly's CH-neighbour chunks of X and g
 // rows ([CH][NP] each), then, in the same space, the LU workspace A
 // ([N][LDA], LDA odd against bank conflicts of column reads); after them the
 // original matrix A0 ([N][LDA], for the refinement residuals) and the
 // solution rows Xs ([NP][4]).  The right-hand sides live in registers, one
-// row per lane.  Order 8: 33 KB (4 waves per CU), order 6: 14 KB.
-template <int M>
+// row per lane.  CH = 16 or 32 is the launch autotuner's tile choice (16:
+// order 8 in 33 KB, 4 waves per CU; 32: fewer chunk rounds, 42 KB).
+template <int M, int CH_>
 struct Lds {
   static constexpr int N = nsize(M), NP = npad(M), LDA = (N % 2 == 1) ? N : N + 1;
-  static constexpr int CH = 16;
+  static constexpr int CH = CH_;
   static constexpr int CHUNK = 2 * CH * NP;
   static constexpr int ASZ = N * LDA;
   static constexpr int U0 = CHUNK > ASZ ? CHUNK : ASZ;
@@ -233,10 +234,10 @@ __device__ __forceinline__ void lu_solve_rows(const double *A, int r, double din
 // cores: 32-neighbour chunks of Taylor monomial rows X and ABF rows g are
 // staged in LDS (Xc, Gc: [CH][NP]) and contracted with
 // v_mfma_f64_16x16x4_f64; the result is written to Adst ([NP][LDA], LDS).
-template <int M>
+template <int M, int CH_>
 __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, int Kuse,
                                          int64_t base, double *Xc, double *Gc, double *Adst) {
-  using L = Lds<M>;
+  using L = Lds<M, CH_>;
   constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH, T = NP / 16;
   const int l = threadIdx.x;
   const double xi = S_.x[i], yi = S_.y[i];
@@ -333,9 +334,9 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD
 }
 
 // Solve one system; the block is one wavefront.
-template <int M>
+template <int M, int CH_>
 __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
-  using L = Lds<M>;
+  using L = Lds<M, CH_>;
   constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH;
   double *Xc = lds;                 // [CH][NP]
   double *Gc = lds + CH * NP;       // [CH][NP]
@@ -351,7 +352,7 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   const int Kfull = S_.cnt[i];
   const int64_t base = S_.row_ptr[i];
 
-  assemble<M>(S_, i, hh, Kuse, base, Xc, Gc, A);
+  assemble<M, CH_>(S_, i, hh, Kuse, base, Xc, Gc, A);
   // order reduction near walls by identity padding (labf.F90:126-148, :311-350)
   const int nt = S_.node_type[i];
   int nkeep = N, mf = M;
@@ -487,18 +488,18 @@ __device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
   __syncthreads();
 }
 
-template <int M>
+template <int M, int CH_>
 __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights(SysArgs a) {
   extern __shared__ double lds[];
   const int sys = blockIdx.x;
   if (sys >= a.nsys) return;
-  solve_system<M>(a, sys, lds);
+  solve_system<M, CH_>(a, sys, lds);
 }
 
 // Persistent variant: a fixed grid takes systems from a queue; systems are
 // sorted by (order, stencil length) so every wave works on the longest
 // remaining systems first.
-template <int M>
+template <int M, int CH_>
 __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(SysArgs a) {
   extern __shared__ double lds[];
   __shared__ int next;
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(Sy
     const int sys = next;
     __syncthreads();
     if (sys >= a.nsys) break;
-    solve_system<M>(a, sys, lds);
+    solve_system<M, CH_>(a, sys, lds);
   }
 }
 
@@ -566,33 +567,40 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
 }
 
 #define SV_TUNE_SAMPLE 4096
-#define SV_TUNE_VARIANTS 2
+#define SV_TUNE_VARIANTS 4
 
-// variant: bit 0 = persistent work-queue grid
-template <int M>
-int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
-  if (count <= 0) return 0;
-  const bool persistent = (variant & 1) != 0;
-  const size_t shm = sizeof(double) * Lds<M>::TOTAL;
-  SysArgs b = a;
-  b.sys_node = a.sys_node + first;
-  b.sys_kind = a.sys_kind + first;
-  b.nsys = count;
+template <int M, int CH>
+int launch_tiles(sv_ctx *c, const SysArgs &b, int count, bool persistent) {
+  const size_t shm = sizeof(double) * Lds<M, CH>::TOTAL;
   if (!persistent) {
-    k_weights<M><<<count, SV_WAVE, shm, c->stream>>>(b);
+    k_weights<M, CH><<<count, SV_WAVE, shm, c->stream>>>(b);
   } else {
     int dev = c->device, ncu = 0, occ = 0;
     SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_persistent<M>,
+    SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_persistent<M, CH>,
                                                             SV_WAVE, shm));
     if (occ < 1) occ = 1;
     int grid = ncu * occ;
     if (grid > count) grid = count;
-    SV_HIP(c, hipMemsetAsync(a.queue, 0, sizeof(int), c->stream));
-    k_weights_persistent<M><<<grid, SV_WAVE, shm, c->stream>>>(b);
+    SV_HIP(c, hipMemsetAsync(b.queue, 0, sizeof(int), c->stream));
+    k_weights_persistent<M, CH><<<grid, SV_WAVE, shm, c->stream>>>(b);
   }
   SV_HIP(c, hipGetLastError());
   return 0;
+}
+
+// variant: bit 0 = persistent work-queue grid, bit 1 = 32-neighbour assembly
+// chunks (else 16)
+template <int M>
+int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
+  if (count <= 0) return 0;
+  SysArgs b = a;
+  b.sys_node = a.sys_node + first;
+  b.sys_kind = a.sys_kind + first;
+  b.nsys = count;
+  const bool persistent = (variant & 1) != 0;
+  return (variant & 2) ? launch_tiles<M, 32>(c, b, count, persistent)
+                       : launch_tiles<M, 16>(c, b, count, persistent);
 }
 
 }  // namespace
@@ -675,7 +683,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     while (first + count < (int)sys.size() && sys[first + count].m == m) count++;
     if (count == 0) continue;
     const int mi = m / 2 - 2;   // 0, 1, 2
-    int variant = c->weights_persistent ? 1 : 0;
+    int variant = c->weights_fixed;
     if (c->weights_autotune && count >= 4 * SV_TUNE_SAMPLE) {
       if (c->wtune[mi] < 0) {
         // time the launch variants on a strided sample of this order's

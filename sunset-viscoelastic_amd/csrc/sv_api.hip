@@ -325,8 +325,13 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
 int sv_set_option(sv_ctx *c, const char *key, double value) {
   SV_CHECK(c, c && key, "null argument");
   if (!strcmp(key, "weights_persistent")) {
-    c->weights_persistent = value != 0.0;
+    c->weights_fixed = value != 0.0 ? 1 : 0;
     c->weights_autotune = false;   // an explicit choice turns the autotuning off
+  } else if (!strcmp(key, "weights_variant")) {
+    SV_CHECK(c, value >= 0.0 && value <= 3.0 && value == (int)value,
+             "weights_variant must be 0, 1, 2 or 3");
+    c->weights_fixed = (int)value;
+    c->weights_autotune = false;
   } else if (!strcmp(key, "weights_autotune")) {
     c->weights_autotune = value != 0.0;
     for (int k = 0; k < 3; k++) c->wtune[k] = -1;

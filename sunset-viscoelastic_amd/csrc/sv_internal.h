@@ -170,7 +170,7 @@ struct sv_ctx {
   bool sell_dirty = true;
   int *singular = nullptr;  // count of zero pivots in the last solve
   int n_singular = 0, n_systems = 0;
-  bool weights_persistent = false;
+  int weights_fixed = 0;    // launch variant when not autotuning (see launch_order)
   // weight-solve launch autotuning per order (4, 6, 8): chosen variant (0 one
   // block per system, 1 persistent grid), -1 until tuned; sample times
   bool weights_autotune = true;

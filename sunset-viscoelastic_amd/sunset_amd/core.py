@@ -205,9 +205,15 @@ class Solver:
         self._ck(lib().sv_find_neighbours(self.h, L, None, None, _i(link)))
         return cnt, cs, link - 1
 
-    def calc_weights(self, order_map=None, persistent: Optional[bool] = None):
-        """persistent=None: autotuned launch (default); True/False: fixed launch."""
-        if persistent is not None:
+    def calc_weights(self, order_map=None, persistent: Optional[bool] = None,
+                     variant: Optional[int] = None):
+        """persistent=None and variant=None: autotuned launch (default);
+        persistent True/False: fixed grid kind with 16-neighbour tiles;
+        variant 0-3: fixed launch variant (bit 0 persistent, bit 1 32-neighbour
+        tiles)."""
+        if variant is not None:
+            self._ck(lib().sv_set_option(self.h, b"weights_variant", float(variant)))
+        elif persistent is not None:
             self._ck(lib().sv_set_option(self.h, b"weights_persistent",
                                          1.0 if persistent else 0.0))
         else:

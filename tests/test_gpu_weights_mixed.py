@@ -64,12 +64,12 @@ def _exact_dx_weights(ns, x, y, link, cs, i, m):
                      for k in range(len(js))])
 
 
-def _device_weights(ns, om, persistent):
+def _device_weights(ns, om, persistent, variant=None):
     import sunset_amd
     s = sunset_amd.Solver(morder=8, form="lc")
     s.set_nodes(ns)
     cnt, cs, link = s.find_neighbours()
-    s.calc_weights(order_map=om, persistent=persistent)   # raises on a zero pivot
+    s.calc_weights(order_map=om, persistent=persistent, variant=variant)   # raises on a zero pivot
     s.calc_sums()
     s.filter_coefficients()
     W = s.weights(link.shape[1])
@@ -119,6 +119,12 @@ def test_small_channel_lists_and_weights_vs_oracle():
         res[persistent] = W
     for k in ("w_grad", "w_lap", "w_hyp", "vol"):
         assert np.array_equal(res[False][k], res[True][k]), k
+    # the 32-neighbour assembly tiles (launch variants 2, 3) accumulate the
+    # same products in the same order: bitwise the same weights
+    for v in (2, 3):
+        W = _device_weights(ns, om, None, variant=v)[3]
+        for k in ("w_grad", "w_lap", "w_hyp", "vol"):
+            assert np.array_equal(res[False][k], W[k]), (v, k)
 
 
 @pytest.mark.slow
