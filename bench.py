@@ -134,11 +134,17 @@ def c3_weights(smin, smax):
     t_n = s.timing("neighbours")
     out = dict(nodes=ns.npfb, order8=int((om == 8).sum()), order4=int((om == 4).sum()),
                spacing=[smin, smax], neighbour_ms=round(t_n, 2))
-    for name, pers in (("plain", False), ("persistent", True), ("autotuned", None)):
-        s.calc_weights(order_map=om, persistent=pers)
+    for name, kw in (("plain", dict(persistent=False)), ("persistent", dict(persistent=True)),
+                     ("single_launch", dict(mixed=1)), ("autotuned", {})):
+        s.calc_weights(order_map=om, **kw)
         t = s.timing("weights")
         out[f"{name}_ms"] = round(t, 2)
         out["systems"] = int(s.timing("systems"))
+    mode = int(s.timing("weights_mixed"))
+    out["autotuned_mode"] = ("per-order launches" if mode == 0 else
+                             f"one persistent launch over all orders, {16 * mode}-neighbour tiles")
+    out["autotune_mode_sample_ms"] = [round(s.timing(f"weights_mixed_tune_{k}"), 3)
+                                      for k in range(3)]
     out["autotune_ms"] = round(s.timing("weights_tune"), 2)
     def variant_name(v):
         v = int(v)

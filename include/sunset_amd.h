@@ -124,6 +124,14 @@ int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
  *       autotuning off.
  *   "weights_variant" (0-3): fixes the launch variant (bits as in
  *       "weights_variant_m<order>") and turns the autotuning off.
+ *   "weights_mixed" (0-2): with several orders (an order map), 0 launches each
+ *       order separately, 1 and 2 run every system in one persistent launch
+ *       (queue: order 8 first, longest stencils first; 16- or 32-neighbour
+ *       tiles); turns the autotuning off.  Autotuned (the default), the
+ *       single launch is timed against the per-order launches on a strided
+ *       4096-system sample of the whole queue.  sv_get_timing "weights_mixed"
+ *       reports the mode of the last call, "weights_mixed_tune_<mode>" the
+ *       sample times.
  *   "loop_order" (default 1, before sv_set_initial): 1 runs the time loop in
  *       Morton order of the nodes, 0 in the caller's order.
  *   "halo_sweeps" (default 1, before sv_set_initial): the RK sweeps copy each

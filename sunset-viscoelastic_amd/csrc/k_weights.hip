@@ -159,6 +159,7 @@ struct SysArgs {
   double *wx, *wy, *wl, *wh, *vol;
   int *singular;
   int *queue;                // persistent launch: next system to take
+  const int *sys_order;      // mixed-order launch: LABFM order of each system
 };
 
 // Waves per SIMD the register budget of the solve kernels is sized for
@@ -513,6 +514,27 @@ __global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(Sy
   }
 }
 
+// One persistent launch over the systems of every order: the queue holds
+// them order 8 first, then 6, then 4, each by decreasing stencil length, so
+// the grid drains the most expensive systems first and the cheap ones fill
+// the tail.  LDS is sized for the largest order present.
+template <int CH_>
+__global__ __launch_bounds__(SV_WAVE, 1) void k_weights_mixed(SysArgs a) {
+  extern __shared__ double lds[];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int sys = next;
+    __syncthreads();
+    if (sys >= a.nsys) break;
+    const int m = a.sys_order[sys];
+    if (m == 8) solve_system<8, CH_>(a, sys, lds);
+    else if (m == 6) solve_system<6, CH_>(a, sys, lds);
+    else solve_system<4, CH_>(a, sys, lds);
+  }
+}
+
 // calc_labf_sums (labf.F90:396-451), one thread per node, sequential in k
 __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double L, double *wx,
                        double *wy, double *wl, const double *wh, double *gsx, double *gsy,
@@ -603,6 +625,32 @@ int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
                        : launch_tiles<M, 16>(c, b, count, persistent);
 }
 
+template <int CH>
+int launch_mixed_tiles(sv_ctx *c, const SysArgs &b, int count, int mmax) {
+  const int tot = mmax == 8 ? Lds<8, CH>::TOTAL : mmax == 6 ? Lds<6, CH>::TOTAL : Lds<4, CH>::TOTAL;
+  const size_t shm = sizeof(double) * tot;
+  int dev = c->device, ncu = 0, occ = 0;
+  SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  SV_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_weights_mixed<CH>, SV_WAVE,
+                                                          shm));
+  if (occ < 1) occ = 1;
+  int grid = ncu * occ;
+  if (grid > count) grid = count;
+  SV_HIP(c, hipMemsetAsync(b.queue, 0, sizeof(int), c->stream));
+  k_weights_mixed<CH><<<grid, SV_WAVE, shm, c->stream>>>(b);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// mode 1: one mixed-order persistent launch with 16-neighbour tiles, 2: with 32
+int launch_mixed(sv_ctx *c, SysArgs a, int count, int mmax, int mode) {
+  if (count <= 0) return 0;
+  SysArgs b = a;
+  b.nsys = count;
+  return mode == 2 ? launch_mixed_tiles<32>(c, b, count, mmax)
+                   : launch_mixed_tiles<16>(c, b, count, mmax);
+}
+
 }  // namespace
 
 // Build the system lists (one per order), launch, and check for zero pivots.
@@ -663,7 +711,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   SV_HIP(c, hipMemsetAsync(c->singular, 0, sizeof(int) * 4, c->stream));
   SysArgs a{dn, dk, (int)sys.size(), c->x, c->y, c->h, c->hs, c->node_type, c->row_ptr,
             c->col_raw, c->cnt, c->cnt_small, c->h2, c->wx, c->wy, c->wl, c->wh, c->vol,
-            c->singular, dq};
+            c->singular, dq, nullptr};
   hipEvent_t e0, e1, t0, t1;
   SV_HIP(c, hipEventCreate(&e0));
   SV_HIP(c, hipEventCreate(&e1));
@@ -677,60 +725,164 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     if (m == 6) return launch_order<6>(c, args, f0, cnt, variant);
     return launch_order<8>(c, args, f0, cnt, variant);
   };
-  int first = 0;
-  for (int m : {4, 6, 8}) {
-    int count = 0;
-    while (first + count < (int)sys.size() && sys[first + count].m == m) count++;
-    if (count == 0) continue;
-    const int mi = m / 2 - 2;   // 0, 1, 2
-    int variant = c->weights_fixed;
-    if (c->weights_autotune && count >= 4 * SV_TUNE_SAMPLE) {
-      if (c->wtune[mi] < 0) {
-        // time the launch variants on a strided sample of this order's
-        // systems (they are sorted by stencil length, so the sample has the
-        // same length mix); the sample's weights are rewritten by the full
-        // launch with identical values
-        const int S = SV_TUNE_SAMPLE, stride = count / S;
-        std::vector<int> sn(S), sk(S);
-        for (int q = 0; q < S; q++) { sn[q] = hn[first + q * stride]; sk[q] = hk[first + q * stride]; }
-        int *tn = nullptr, *tk = nullptr;
-        SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
-        SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
-        SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-        SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-        SysArgs ts = a;
-        ts.sys_node = tn;
-        ts.sys_kind = tk;
-        float best = 1e30f;
-        for (int v = 0; v < SV_TUNE_VARIANTS; v++) {
-          float tv = 0.0f;
-          for (int rep = 0; rep < 2; rep++) {   // first pass warms up
-            SV_HIP(c, hipEventRecord(t0, c->stream));
-            if (int rc = launch(m, ts, 0, S, v)) return rc;
-            SV_HIP(c, hipEventRecord(t1, c->stream));
-            SV_HIP(c, hipEventSynchronize(t1));
-            SV_HIP(c, hipEventElapsedTime(&tv, t0, t1));
-          }
-          c->wtune_ms[mi][v] = tv;
-          if (tv < best) { best = tv; c->wtune[mi] = v; }
-        }
-        SV_HIP(c, hipFree(tn));
-        SV_HIP(c, hipFree(tk));
-        float tt = 0.0f;
-        for (int v = 0; v < SV_TUNE_VARIANTS; v++) tt += 2.0f * c->wtune_ms[mi][v];
-        tune_ms += tt;
-      }
-      variant = c->wtune[mi];
+  // time fn() twice (the first pass warms up), return the second
+  auto timed = [&](auto fn, float &ms) -> int {
+    for (int rep = 0; rep < 2; rep++) {
+      SV_HIP(c, hipEventRecord(t0, c->stream));
+      if (int rc = fn()) return rc;
+      SV_HIP(c, hipEventRecord(t1, c->stream));
+      SV_HIP(c, hipEventSynchronize(t1));
+      SV_HIP(c, hipEventElapsedTime(&ms, t0, t1));
     }
+    return 0;
+  };
+  // order slices of the (order, length)-sorted list
+  struct Slice { int m, first, count; };
+  std::vector<Slice> slices;
+  for (int m : {4, 6, 8}) {
+    int f = 0;
+    while (f < (int)sys.size() && sys[f].m != m) f++;
+    int cnt = 0;
+    while (f + cnt < (int)sys.size() && sys[f + cnt].m == m) cnt++;
+    if (cnt) slices.push_back({m, f, cnt});
+  }
+  const int S = SV_TUNE_SAMPLE;
+  // 1. per order: the launch variant (grid kind x assembly tile), timed on a
+  //    strided sample of that order's systems (sorted by stencil length, so
+  //    the sample has the same length mix); the sample's weights are
+  //    rewritten by the full launch with identical values
+  for (const Slice &sl : slices) {
+    const int mi = sl.m / 2 - 2;
+    if (!c->weights_autotune || sl.count < 4 * S || c->wtune[mi] >= 0) continue;
+    const int stride = sl.count / S;
+    std::vector<int> sn(S), sk(S);
+    for (int q = 0; q < S; q++) {
+      sn[q] = hn[sl.first + q * stride];
+      sk[q] = hk[sl.first + q * stride];
+    }
+    int *tn = nullptr, *tk = nullptr;
+    SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
+    SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
+    SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    SysArgs ts = a;
+    ts.sys_node = tn;
+    ts.sys_kind = tk;
+    float best = 1e30f;
+    for (int v = 0; v < SV_TUNE_VARIANTS; v++) {
+      float tv = 0.0f;
+      if (int rc = timed([&] { return launch(sl.m, ts, 0, S, v); }, tv)) return rc;
+      c->wtune_ms[mi][v] = tv;
+      tune_ms += 2.0f * tv;
+      if (tv < best) { best = tv; c->wtune[mi] = v; }
+    }
+    SV_HIP(c, hipFree(tn));
+    SV_HIP(c, hipFree(tk));
+  }
+  auto order_variant = [&](int m) {
+    const int mi = m / 2 - 2;
+    return (c->weights_autotune && c->wtune[mi] >= 0) ? c->wtune[mi] : c->weights_fixed;
+  };
+  // 2. several orders: one persistent launch over all of them (queue order 8
+  //    first) against the per-order launches, on a strided sample of that queue
+  int mode = c->weights_autotune ? 0 : c->weights_mixed;
+  std::vector<int> qn, qk, qm;   // the mixed queue
+  if (slices.size() >= 2) {
+    for (auto it = slices.rbegin(); it != slices.rend(); ++it)
+      for (int q = it->first; q < it->first + it->count; q++) {
+        qn.push_back(hn[q]);
+        qk.push_back(hk[q]);
+        qm.push_back(it->m);
+      }
+    const int total = (int)qn.size();
+    if (c->weights_autotune && total >= 4 * S && c->wmixed < 0) {
+      const int stride = total / S;
+      std::vector<int> sn(S), sk(S), sm(S);
+      for (int q = 0; q < S; q++) {
+        sn[q] = qn[q * stride];
+        sk[q] = qk[q * stride];
+        sm[q] = qm[q * stride];
+      }
+      int *tn = nullptr, *tk = nullptr, *tm = nullptr;
+      SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
+      SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
+      SV_HIP(c, hipMalloc(&tm, sizeof(int) * S));
+      SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+      SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+      SV_HIP(c, hipMemcpy(tm, sm.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+      SysArgs ts = a;
+      ts.sys_node = tn;
+      ts.sys_kind = tk;
+      ts.sys_order = tm;
+      float tv[3] = {0.0f, 0.0f, 0.0f};
+      // per-order launches over the sample's contiguous order groups
+      if (int rc = timed([&] {
+            int f = 0;
+            while (f < S) {
+              int n2 = 0;
+              while (f + n2 < S && sm[f + n2] == sm[f]) n2++;
+              if (int r2 = launch(sm[f], ts, f, n2, order_variant(sm[f]))) return r2;
+              f += n2;
+            }
+            return 0;
+          }, tv[0]))
+        return rc;
+      const int mmax = qm[0];
+      for (int md = 1; md <= 2; md++)
+        if (int rc = timed([&] { return launch_mixed(c, ts, S, mmax, md); }, tv[md])) return rc;
+      int bestm = 0;
+      for (int md = 0; md < 3; md++) {
+        c->wmixed_ms[md] = tv[md];
+        tune_ms += 2.0f * tv[md];
+        if (tv[md] < tv[bestm]) bestm = md;
+      }
+      c->wmixed = bestm;
+      SV_HIP(c, hipFree(tn));
+      SV_HIP(c, hipFree(tk));
+      SV_HIP(c, hipFree(tm));
+    }
+    if (c->weights_autotune && c->wmixed >= 0) mode = c->wmixed;
+  } else {
+    mode = 0;
+  }
+  // 3. the launches
+  c->wmixed_used = mode;
+  for (int k = 0; k < 3; k++) c->wvariant[k] = -1;
+  if (mode == 0) {
+    for (const Slice &sl : slices) {
+      const int variant = order_variant(sl.m);
+      float tl = 0.0f;
+      SV_HIP(c, hipEventRecord(t0, c->stream));
+      if (int rc = launch(sl.m, a, sl.first, sl.count, variant)) return rc;
+      SV_HIP(c, hipEventRecord(t1, c->stream));
+      SV_HIP(c, hipEventSynchronize(t1));
+      SV_HIP(c, hipEventElapsedTime(&tl, t0, t1));
+      solve_ms += tl;
+      c->wvariant[sl.m / 2 - 2] = variant;
+    }
+  } else {
+    const int total = (int)qn.size();
+    int *mn = nullptr, *mk = nullptr, *mm = nullptr;
+    SV_HIP(c, hipMalloc(&mn, sizeof(int) * total));
+    SV_HIP(c, hipMalloc(&mk, sizeof(int) * total));
+    SV_HIP(c, hipMalloc(&mm, sizeof(int) * total));
+    SV_HIP(c, hipMemcpy(mn, qn.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    SV_HIP(c, hipMemcpy(mk, qk.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    SV_HIP(c, hipMemcpy(mm, qm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    SysArgs am = a;
+    am.sys_node = mn;
+    am.sys_kind = mk;
+    am.sys_order = mm;
+    float tl = 0.0f;
     SV_HIP(c, hipEventRecord(t0, c->stream));
-    if (int rc = launch(m, a, first, count, variant)) return rc;
+    if (int rc = launch_mixed(c, am, total, qm[0], mode)) return rc;
     SV_HIP(c, hipEventRecord(t1, c->stream));
     SV_HIP(c, hipEventSynchronize(t1));
-    float tl = 0.0f;
     SV_HIP(c, hipEventElapsedTime(&tl, t0, t1));
     solve_ms += tl;
-    c->wvariant[mi] = variant;
-    first += count;
+    SV_HIP(c, hipFree(mn));
+    SV_HIP(c, hipFree(mk));
+    SV_HIP(c, hipFree(mm));
   }
   SV_HIP(c, hipEventRecord(e1, c->stream));
   SV_HIP(c, hipStreamSynchronize(c->stream));

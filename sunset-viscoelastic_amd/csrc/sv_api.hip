@@ -314,6 +314,10 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
   else if (!strncmp(what, "weights_variant_m", 17) && (what[17] == '4' || what[17] == '6' ||
                                                         what[17] == '8'))
     *ms = c->wvariant[(what[17] - '0') / 2 - 2];
+  else if (!strcmp(what, "weights_mixed")) *ms = c->wmixed_used;
+  else if (!strncmp(what, "weights_mixed_tune_", 19) && strlen(what) == 20 && what[19] >= '0' &&
+           what[19] <= '2')
+    *ms = c->wmixed_ms[what[19] - '0'];
   else if (!strncmp(what, "weights_tune_m", 14) && strlen(what) == 17 && what[15] == '_' &&
            (what[14] == '4' || what[14] == '6' || what[14] == '8') && what[16] >= '0' &&
            what[16] <= '3')
@@ -332,9 +336,14 @@ int sv_set_option(sv_ctx *c, const char *key, double value) {
              "weights_variant must be 0, 1, 2 or 3");
     c->weights_fixed = (int)value;
     c->weights_autotune = false;
+  } else if (!strcmp(key, "weights_mixed")) {
+    SV_CHECK(c, value == 0.0 || value == 1.0 || value == 2.0, "weights_mixed must be 0, 1 or 2");
+    c->weights_mixed = (int)value;
+    c->weights_autotune = false;
   } else if (!strcmp(key, "weights_autotune")) {
     c->weights_autotune = value != 0.0;
     for (int k = 0; k < 3; k++) c->wtune[k] = -1;
+    c->wmixed = -1;
   }
   else if (!strcmp(key, "halo_sweeps")) {
     SV_CHECK(c, !c->have_state, "halo_sweeps must be set before sv_set_initial");

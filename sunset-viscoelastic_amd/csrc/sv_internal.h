@@ -177,6 +177,10 @@ struct sv_ctx {
   int wtune[3] = {-1, -1, -1};
   float wtune_ms[3][4] = {};
   int wvariant[3] = {-1, -1, -1};   // variant of the last launch per order
+  // several orders: 0 per-order launches, 1/2 one mixed-order persistent launch
+  // with 16/32-neighbour tiles; chosen by the autotuner (-1 untuned) or fixed
+  int wmixed = -1, weights_mixed = 0, wmixed_used = 0;
+  float wmixed_ms[3] = {};
   // physics/state
   bool have_phys = false, have_state = false;
   sv_physk pk{};

@@ -206,12 +206,15 @@ class Solver:
         return cnt, cs, link - 1
 
     def calc_weights(self, order_map=None, persistent: Optional[bool] = None,
-                     variant: Optional[int] = None):
-        """persistent=None and variant=None: autotuned launch (default);
-        persistent True/False: fixed grid kind with 16-neighbour tiles;
-        variant 0-3: fixed launch variant (bit 0 persistent, bit 1 32-neighbour
-        tiles)."""
-        if variant is not None:
+                     variant: Optional[int] = None, mixed: Optional[int] = None):
+        """persistent=None, variant=None, mixed=None: autotuned launch
+        (default); persistent True/False: fixed grid kind with 16-neighbour
+        tiles; variant 0-3: fixed launch variant (bit 0 persistent, bit 1
+        32-neighbour tiles); mixed 0-2: per-order launches (0) or one
+        mixed-order persistent launch with 16/32-neighbour tiles."""
+        if mixed is not None:
+            self._ck(lib().sv_set_option(self.h, b"weights_mixed", float(mixed)))
+        elif variant is not None:
             self._ck(lib().sv_set_option(self.h, b"weights_variant", float(variant)))
         elif persistent is not None:
             self._ck(lib().sv_set_option(self.h, b"weights_persistent",

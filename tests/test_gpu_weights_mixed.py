@@ -64,12 +64,13 @@ def _exact_dx_weights(ns, x, y, link, cs, i, m):
                      for k in range(len(js))])
 
 
-def _device_weights(ns, om, persistent, variant=None):
+def _device_weights(ns, om, persistent, variant=None, mixed=None):
     import sunset_amd
     s = sunset_amd.Solver(morder=8, form="lc")
     s.set_nodes(ns)
     cnt, cs, link = s.find_neighbours()
-    s.calc_weights(order_map=om, persistent=persistent, variant=variant)   # raises on a zero pivot
+    s.calc_weights(order_map=om, persistent=persistent, variant=variant,
+                   mixed=mixed)   # raises on a zero pivot
     s.calc_sums()
     s.filter_coefficients()
     W = s.weights(link.shape[1])
@@ -125,6 +126,11 @@ def test_small_channel_lists_and_weights_vs_oracle():
         W = _device_weights(ns, om, None, variant=v)[3]
         for k in ("w_grad", "w_lap", "w_hyp", "vol"):
             assert np.array_equal(res[False][k], W[k]), (v, k)
+    # one persistent launch over both orders runs the same per-system solve
+    for md in (1, 2):
+        W = _device_weights(ns, om, None, mixed=md)[3]
+        for k in ("w_grad", "w_lap", "w_hyp", "vol"):
+            assert np.array_equal(res[False][k], W[k]), ("mixed", md, k)
 
 
 @pytest.mark.slow
