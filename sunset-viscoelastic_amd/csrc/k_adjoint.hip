@@ -788,11 +788,11 @@ int sv_rhs_vjp(sv_ctx *c, const double *const rbar[6], double *const ubar[6]) {
   sv_adj *A = c->adj;
   SV_HIP(c, hipSetDevice(c->device));
   if (int rc = ensure_tape(c, A)) return rc;
-  double *scratch = nullptr;
-  SV_HIP(c, hipMalloc(&scratch, sizeof(double) * 6 * (size_t)c->npfb));
-  int rc = sv_dev_rhs(c, 1, scratch, A->tape[0], A->tape[1]);
-  (void)hipFree(scratch);
-  if (rc) return rc;
+  {
+    DevTmp<double> scratch;
+    SV_HIP(c, scratch.alloc(6 * (size_t)c->npfb));
+    if (int rc = sv_dev_rhs(c, 1, scratch, A->tape[0], A->tape[1])) return rc;
+  }
   if (int r2 = upload_records(c, A->Rbar, rbar)) return r2;
   if (int r2 = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return r2;
   if (int r2 = adj_rhs(c, A, A->tape.data())) return r2;
@@ -821,9 +821,9 @@ int sv_rollout_forward(sv_ctx *c, int32_t nsteps, double *loss) {
   }
   A->ckpt_ctrl.assign(nck, sv_ctrl{});
   if (int rc = zero_dev(c, A->lacc, c->nseg)) return rc;
-  hipEvent_t e0, e1;
-  SV_HIP(c, hipEventCreate(&e0));
-  SV_HIP(c, hipEventCreate(&e1));
+  EventTmp e0, e1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   for (int s = 0; s < nsteps; s++) {
     if (s % A->every == 0) {
@@ -847,8 +847,6 @@ int sv_rollout_forward(sv_ctx *c, int32_t nsteps, double *loss) {
   float ms = 0;
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
   A->t_fwd = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   A->T = nsteps;
   if (loss) *loss = acc / nsteps;
   return 0;
@@ -879,9 +877,9 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
     }
     SV_HIP(c, hipMemcpy(A->Ubar, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice));
   }
-  hipEvent_t e0, e1;
-  SV_HIP(c, hipEventCreate(&e0));
-  SV_HIP(c, hipEventCreate(&e1));
+  EventTmp e0, e1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   if (int rc = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return rc;
   k_adj_sum<<<c->nblk * SUBS, AB, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
@@ -906,8 +904,6 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   float ms = 0;
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
   A->t_bwd = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (int rc = param_grad_finish(c, A)) return rc;
   if (int rc = download_records(c, A->Ubar, grad0)) return rc;
   // leave the state at the end of the rollout

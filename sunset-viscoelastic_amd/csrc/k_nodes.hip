@@ -275,20 +275,19 @@ __global__ void k_sell_pack(int npfb, const int64_t *sell_off, const int *sell_w
 static int scan_int(sv_ctx *c, const int *in, int *out, int n) {
   size_t tb = 0;
   SV_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, c->stream));
-  void *tmp = nullptr;
-  SV_HIP(c, hipMalloc(&tmp, tb > 0 ? tb : 16));
-  SV_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, c->stream));
+  DevTmp<char> tmp;
+  SV_HIP(c, tmp.alloc(tb > 0 ? tb : 16));
+  SV_HIP(c, hipcub::DeviceScan::ExclusiveSum((void *)tmp, tb, in, out, n, c->stream));
   SV_HIP(c, hipStreamSynchronize(c->stream));
-  SV_HIP(c, hipFree(tmp));
   return 0;
 }
 
 int sv_dev_create_ghosts(sv_ctx *c) {
   const int n = c->npfb;
   GhostDom g{c->xmin, c->xmax, c->ymin, c->ymax, c->bc[0], c->bc[1], c->bc[2], c->bc[3]};
-  int *gcount = nullptr, *goff = nullptr;
-  SV_HIP(c, hipMalloc(&gcount, sizeof(int) * (n + 1)));
-  SV_HIP(c, hipMalloc(&goff, sizeof(int) * (n + 1)));
+  DevTmp<int> gcount, goff;
+  SV_HIP(c, gcount.alloc(n + 1));
+  SV_HIP(c, goff.alloc(n + 1));
   SV_HIP(c, hipMemsetAsync(gcount, 0, sizeof(int) * (n + 1), c->stream));
   const int B = 256, nb = (n + B - 1) / B;
   k_ghost_count<<<nb, B, 0, c->stream>>>(n, g, c->x, c->y, c->h, gcount);
@@ -297,24 +296,20 @@ int sv_dev_create_ghosts(sv_ctx *c) {
   int ng = 0;
   SV_HIP(c, hipMemcpy(&ng, goff + n, sizeof(int), hipMemcpyDeviceToHost));
   if (n + ng > c->cap) {
-    (void)hipFree(gcount);
-    (void)hipFree(goff);
     return sv_fail(c, SV_E_ARG, "ghost count %d exceeds node capacity %d", ng, c->cap - n);
   }
   k_ghost_write<<<nb, B, 0, c->stream>>>(n, g, c->x, c->y, c->h, c->hs, c->s, c->node_type,
                                          c->irel, c->vrel, goff);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
-  SV_HIP(c, hipFree(gcount));
-  SV_HIP(c, hipFree(goff));
   c->np = n + ng;
   return 0;
 }
 
 int sv_dev_find_neighbours(sv_ctx *c) {
-  hipEvent_t e0, e1;
-  SV_HIP(c, hipEventCreate(&e0));
-  SV_HIP(c, hipEventCreate(&e1));
+  EventTmp e0, e1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   const int np = c->np, npfb = c->npfb;
   const double Lx = c->xmax - c->xmin, Ly = c->ymax - c->ymin;
@@ -329,11 +324,11 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   G.ncx = (int)(3.0 * Lx / csz) + 1;
   G.ncy = (int)(3.0 * Ly / csz) + 1;
   const int nct = G.ncx * G.ncy;
-  int *cellof = nullptr, *ccount = nullptr, *cstart = nullptr, *ip = nullptr;
-  SV_HIP(c, hipMalloc(&cellof, sizeof(int) * np));
-  SV_HIP(c, hipMalloc(&ccount, sizeof(int) * (nct + 1)));
-  SV_HIP(c, hipMalloc(&cstart, sizeof(int) * (nct + 1)));
-  SV_HIP(c, hipMalloc(&ip, sizeof(int) * np));
+  DevTmp<int> cellof, ccount, cstart, ip;
+  SV_HIP(c, cellof.alloc(np));
+  SV_HIP(c, ccount.alloc(nct + 1));
+  SV_HIP(c, cstart.alloc(nct + 1));
+  SV_HIP(c, ip.alloc(np));
   SV_HIP(c, hipMemsetAsync(ccount, 0, sizeof(int) * (nct + 1), c->stream));
   const int B = 256;
   k_cell_count<<<(np + B - 1) / B, B, 0, c->stream>>>(np, G, c->x, c->y, cellof, ccount);
@@ -382,12 +377,6 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   float ms = 0;
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
   c->t_neigh = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  SV_HIP(c, hipFree(cellof));
-  SV_HIP(c, hipFree(ccount));
-  SV_HIP(c, hipFree(cstart));
-  SV_HIP(c, hipFree(ip));
   c->have_lists = true;
   c->sell_dirty = true;
   c->L.valid = false;

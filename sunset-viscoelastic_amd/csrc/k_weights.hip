@@ -684,10 +684,10 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   });
   std::vector<int> hn(sys.size()), hk(sys.size());
   for (size_t s = 0; s < sys.size(); s++) { hn[s] = sys[s].node; hk[s] = sys[s].kind; }
-  int *dn = nullptr, *dk = nullptr, *dq = nullptr;
-  SV_HIP(c, hipMalloc(&dn, sizeof(int) * (sys.size() + 1)));
-  SV_HIP(c, hipMalloc(&dk, sizeof(int) * (sys.size() + 1)));
-  SV_HIP(c, hipMalloc(&dq, sizeof(int) * 4));
+  DevTmp<int> dn, dk, dq;
+  SV_HIP(c, dn.alloc(sys.size() + 1));
+  SV_HIP(c, dk.alloc(sys.size() + 1));
+  SV_HIP(c, dq.alloc(4));
   SV_HIP(c, hipMemcpy(dn, hn.data(), sizeof(int) * sys.size(), hipMemcpyHostToDevice));
   SV_HIP(c, hipMemcpy(dk, hk.data(), sizeof(int) * sys.size(), hipMemcpyHostToDevice));
   const size_t L = (size_t)c->nnz + 1;
@@ -712,11 +712,11 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   SysArgs a{dn, dk, (int)sys.size(), c->x, c->y, c->h, c->hs, c->node_type, c->row_ptr,
             c->col_raw, c->cnt, c->cnt_small, c->h2, c->wx, c->wy, c->wl, c->wh, c->vol,
             c->singular, dq, nullptr};
-  hipEvent_t e0, e1, t0, t1;
-  SV_HIP(c, hipEventCreate(&e0));
-  SV_HIP(c, hipEventCreate(&e1));
-  SV_HIP(c, hipEventCreate(&t0));
-  SV_HIP(c, hipEventCreate(&t1));
+  EventTmp e0, e1, t0, t1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
+  SV_HIP(c, t0.create());
+  SV_HIP(c, t1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   float tune_ms = 0.0f;    // autotuning sample launches (2 passes each)
   float solve_ms = 0.0f;   // the full launches
@@ -760,9 +760,9 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
       sn[q] = hn[sl.first + q * stride];
       sk[q] = hk[sl.first + q * stride];
     }
-    int *tn = nullptr, *tk = nullptr;
-    SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
-    SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
+    DevTmp<int> tn, tk;
+    SV_HIP(c, tn.alloc(S));
+    SV_HIP(c, tk.alloc(S));
     SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
     SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
     SysArgs ts = a;
@@ -776,8 +776,6 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
       tune_ms += 2.0f * tv;
       if (tv < best) { best = tv; c->wtune[mi] = v; }
     }
-    SV_HIP(c, hipFree(tn));
-    SV_HIP(c, hipFree(tk));
   }
   auto order_variant = [&](int m) {
     const int mi = m / 2 - 2;
@@ -803,10 +801,10 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
         sk[q] = qk[q * stride];
         sm[q] = qm[q * stride];
       }
-      int *tn = nullptr, *tk = nullptr, *tm = nullptr;
-      SV_HIP(c, hipMalloc(&tn, sizeof(int) * S));
-      SV_HIP(c, hipMalloc(&tk, sizeof(int) * S));
-      SV_HIP(c, hipMalloc(&tm, sizeof(int) * S));
+      DevTmp<int> tn, tk, tm;
+      SV_HIP(c, tn.alloc(S));
+      SV_HIP(c, tk.alloc(S));
+      SV_HIP(c, tm.alloc(S));
       SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
       SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
       SV_HIP(c, hipMemcpy(tm, sm.data(), sizeof(int) * S, hipMemcpyHostToDevice));
@@ -837,9 +835,6 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
         if (tv[md] < tv[bestm]) bestm = md;
       }
       c->wmixed = bestm;
-      SV_HIP(c, hipFree(tn));
-      SV_HIP(c, hipFree(tk));
-      SV_HIP(c, hipFree(tm));
     }
     if (c->weights_autotune && c->wmixed >= 0) mode = c->wmixed;
   } else {
@@ -862,10 +857,10 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     }
   } else {
     const int total = (int)qn.size();
-    int *mn = nullptr, *mk = nullptr, *mm = nullptr;
-    SV_HIP(c, hipMalloc(&mn, sizeof(int) * total));
-    SV_HIP(c, hipMalloc(&mk, sizeof(int) * total));
-    SV_HIP(c, hipMalloc(&mm, sizeof(int) * total));
+    DevTmp<int> mn, mk, mm;
+    SV_HIP(c, mn.alloc(total));
+    SV_HIP(c, mk.alloc(total));
+    SV_HIP(c, mm.alloc(total));
     SV_HIP(c, hipMemcpy(mn, qn.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     SV_HIP(c, hipMemcpy(mk, qk.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     SV_HIP(c, hipMemcpy(mm, qm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
@@ -880,9 +875,6 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     SV_HIP(c, hipEventSynchronize(t1));
     SV_HIP(c, hipEventElapsedTime(&tl, t0, t1));
     solve_ms += tl;
-    SV_HIP(c, hipFree(mn));
-    SV_HIP(c, hipFree(mk));
-    SV_HIP(c, hipFree(mm));
   }
   SV_HIP(c, hipEventRecord(e1, c->stream));
   SV_HIP(c, hipStreamSynchronize(c->stream));
@@ -892,15 +884,8 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   c->t_weights_tune = tune_ms;
   c->t_weights = solve_ms;
   c->n_systems = (int)sys.size();
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   int nsing = 0;
   SV_HIP(c, hipMemcpy(&nsing, c->singular, sizeof(int), hipMemcpyDeviceToHost));
-  SV_HIP(c, hipFree(dn));
-  SV_HIP(c, hipFree(dk));
-  SV_HIP(c, hipFree(dq));
   c->have_weights = true;
   c->have_sums = false;
   c->have_filter = false;

@@ -625,8 +625,8 @@ int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   if (int rc = set_device(c)) return rc;
-  double *out = nullptr;
-  SV_HIP(c, hipMalloc(&out, sizeof(double) * 6 * c->npfb));
+  DevTmp<double> out;
+  SV_HIP(c, out.alloc(6 * (size_t)c->npfb));
   int rc = sv_dev_rhs(c, iRKstep, out);
   if (!rc) {
     double *dst[6] = {r0, r1, r2, r3, r4, r5};
@@ -635,12 +635,11 @@ int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *
     std::vector<double> tmp(n);
     for (int f = 0; f < 6; f++)
       if (dst[f]) {
-        SV_HIP(c, hipMemcpy(tmp.data(), out + (size_t)f * n, sizeof(double) * n,
+        SV_HIP(c, hipMemcpy(tmp.data(), out.p + (size_t)f * n, sizeof(double) * n,
                             hipMemcpyDeviceToHost));
         for (int k = 0; k < n; k++) dst[f][P[k]] = tmp[k];
       }
   }
-  (void)hipFree(out);
   return rc;
 }
 
@@ -648,9 +647,9 @@ int sv_run(sv_ctx *c, int32_t nsteps, int32_t use_graph) {
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
   SV_CHECK(c, nsteps >= 0, "nsteps must be >= 0");
   if (int rc = set_device(c)) return rc;
-  hipEvent_t e0, e1;
-  SV_HIP(c, hipEventCreate(&e0));
-  SV_HIP(c, hipEventCreate(&e1));
+  EventTmp e0, e1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   if (use_graph && nsteps > 0) {
     if (!c->graph_valid) {
@@ -675,8 +674,6 @@ int sv_run(sv_ctx *c, int32_t nsteps, int32_t use_graph) {
   float ms = 0;
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
   c->t_step = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return 0;
 }
 
@@ -685,8 +682,10 @@ int sv_run_profiled(sv_ctx *c, int32_t nsteps, double *kernel_ms, double *total_
   SV_CHECK(c, nsteps > 0, "nsteps must be > 0");
   if (int rc = set_device(c)) return rc;
   const int NE = SV_NKERNELS + 1;
-  std::vector<hipEvent_t> ev((size_t)nsteps * NE);
-  for (auto &e : ev) SV_HIP(c, hipEventCreate(&e));
+  std::vector<EventTmp> evs((size_t)nsteps * NE);
+  for (auto &e : evs) SV_HIP(c, e.create());
+  std::vector<hipEvent_t> ev(evs.size());
+  for (size_t k = 0; k < evs.size(); k++) ev[k] = evs[k];
   for (int s = 0; s < nsteps; s++)
     if (int rc = sv_dev_step(c, ev.data() + (size_t)s * NE)) return rc;
   SV_HIP(c, hipEventSynchronize(ev.back()));
@@ -703,7 +702,6 @@ int sv_run_profiled(sv_ctx *c, int32_t nsteps, double *kernel_ms, double *total_
     if (kernel_ms) kernel_ms[k] = acc[k] / nsteps;
   if (total_ms) *total_ms = tot;
   c->t_step = tot;
-  for (auto &e : ev) (void)hipEventDestroy(e);
   return 0;
 }
 

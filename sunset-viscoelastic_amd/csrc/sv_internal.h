@@ -226,6 +226,34 @@ int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
                      hipGetErrorString(e_), __FILE__, __LINE__);               \
   } while (0)
 int sv_alloc(sv_ctx *c, void **p, size_t bytes);
+
+// A temporary device buffer owned by the enclosing scope: freed on every exit
+// path, error returns included (hipFree waits for work that still uses it).
+template <class T>
+struct DevTmp {
+  T *p = nullptr;
+  DevTmp() = default;
+  DevTmp(const DevTmp &) = delete;
+  DevTmp &operator=(const DevTmp &) = delete;
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) { return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)); }
+  operator T *() const { return p; }
+};
+
+// A HIP event owned by the enclosing scope.
+struct EventTmp {
+  hipEvent_t e = nullptr;
+  EventTmp() = default;
+  EventTmp(const EventTmp &) = delete;
+  EventTmp &operator=(const EventTmp &) = delete;
+  ~EventTmp() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t create() { return hipEventCreate(&e); }
+  operator hipEvent_t() const { return e; }
+};
 int sv_ensure_sell(sv_ctx *c);
 void sv_free(sv_ctx *c, void *p);
 
