@@ -544,6 +544,9 @@ __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double 
   const int64_t b = row_ptr[i];
   const int K = cnt[i];
   double sh = 0.0, sx = 0.0, sy = 0.0, sl = 0.0;
+  // unrolled so that a batch of loads is in flight; the additions keep the
+  // reference's sequential order
+#pragma unroll 8
   for (int k = 0; k < K; k++) {
     sh += wh[b + k];
     sx += wx[b + k];
@@ -554,6 +557,7 @@ __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double 
   gsx[i] = sx / L;
   gsy[i] = sy / L;
   lsum[i] = sl / L / L;
+#pragma unroll 8
   for (int k = 0; k < K; k++) {
     wx[b + k] = wx[b + k] / L;
     wy[b + k] = wy[b + k] / L;
@@ -572,6 +576,7 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
   const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
   const double tmp = (2.0 / 3.0) * SV_PI / lscal;
   double ls = 0.0;
+#pragma unroll 4
   for (int k = 0; k < K; k++) {
     const int j = col[b + k];
     const double dx = x[j] - x[i], dy = y[j] - y[i];
@@ -580,6 +585,7 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
   }
   const double fc = (2.0 / 3.0) / ls;
   double hs = 0.0;
+#pragma unroll 8
   for (int k = 0; k < K; k++) {
     const double w = wh[b + k] * fc;
     wh[b + k] = w;
