@@ -185,6 +185,30 @@ def cpu_weights_baseline(n_sample):
                        f"calc_labf_weights {t:.2f} s (serial loop)")
 
 
+def cpu_weights_omp(n_sample, threads):
+    """The OpenMP-parallel CPU restatement of the weight solve (the oracle's
+    calc_labf_weights with a per-node order, one node per thread, one-sided
+    Jacobi SVDs) on the same order-8 lattice; node weight sets per second.
+    The serial reference loop above is the x-baseline; this one shows what
+    the host's cores give in parallel."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as OL
+    from sunset_amd.nodes import jittered_lattice
+    ns = jittered_lattice(n_sample, seed=2, h_over_s=2.7)
+    o = OL.OracleSim(ns, 8)
+    o.ghosts()
+    o.neighbours()
+    om = np.full(ns.npfb, 8, dtype=np.int32)
+    t0 = time.perf_counter()
+    o.weights_ordermap(om, nthreads=threads)
+    t = time.perf_counter() - t0
+    return dict(value=round(ns.npfb / t, 1), unit="node weight sets/s", cores=threads,
+                kind="port",
+                sample=f"{n_sample}x{n_sample} lattice ({ns.npfb} nodes), order 8, h=2.7s, "
+                       f"{t:.2f} s with {threads} OpenMP threads")
+
+
 def ensemble_leg(members=32, steps=100, seed=7):
     """32 simulations of different sizes in one context: N_m = round(exp(U(ln 16k,
     ln 256k))) nodes (square jittered lattices), LABFM order 4 or 6, Wi =
@@ -373,6 +397,16 @@ def main():
             print(f"[bench] cpu weights baseline: reference on {args.cpu_weights_n}^2 order 8",
                   file=sys.stderr, flush=True)
             c3["cpu_baseline"] = cpu_weights_baseline(args.cpu_weights_n)
+            threads = min(len(os.sched_getaffinity(0)), 16)
+            print(f"[bench] cpu weights baseline: OpenMP port on {2 * args.cpu_weights_n}^2, "
+                  f"{threads} threads", file=sys.stderr, flush=True)
+            c3["cpu_baseline_omp"] = cpu_weights_omp(2 * args.cpu_weights_n, threads)
+            c3["x_baseline"] = "cpu_baseline (serial reference loop)"
+            if c3["cpu_baseline"]:
+                c3["x_vs_cpu_serial"] = round(c3["weight_solves_per_s"]
+                                              / c3["cpu_baseline"]["value"], 1)
+            c3["x_vs_cpu_omp"] = round(c3["weight_solves_per_s"]
+                                       / c3["cpu_baseline_omp"]["value"], 1)
 
     ens = None
     if not args.no_ensemble:
