@@ -25,6 +25,11 @@
 !!          ij_w_lap(nplink,npfb), ij_w_hyp(nplink,npfb), ij_w_grad_sum(2,npfb),
 !!          ij_w_lap_sum(npfb), ij_w_hyp_sum(npfb), vol(npfb)
 !!          bit 2 (4) -- write only the fields after the steps (no lists/weights)
+!!          bit 3 (8) -- boundary block after the fields: real64 rnx(npfb),
+!!          rny(npfb); int32 inflow_type, inflow_velocity_control; real64
+!!          u_inflow_start, u_inflow_end, ramp_time.  Nodes of type 0/1/2 are
+!!          boundary nodes, each followed by its FD rows (types -1..-4) as the
+!!          IPART reader lays them out (setup_domain.F90:179-193)
 !!   output: see write section below (mirrored by tests/refdrv.py)
 program sunset_ref
   use kind_parameters
@@ -44,6 +49,8 @@ program sunset_ref
   real(rkind),allocatable :: tmp(:),tro(:),tu(:),tv(:),tcxx(:),tcxy(:),tcyy(:)
   real(rkind),allocatable :: g2(:,:),l1(:),f1(:)
   integer(ikind),allocatable :: itmp(:)
+  integer(ikind) :: ib2(2)
+  real(rkind) :: rb3(3)
   real(rkind) :: t0,t1
 
   call get_command_argument(1,fin)
@@ -84,13 +91,40 @@ program sunset_ref
   end do
   allocate(tro(npfb),tu(npfb),tv(npfb),tcxx(npfb),tcxy(npfb),tcyy(npfb))
   read(30) tro;read(30) tu;read(30) tv;read(30) tcxx;read(30) tcxy;read(30) tcyy
+  if(iand(flags,8).ne.0) then
+     read(30) tmp;rnorm(1:npfb,1) = tmp
+     read(30) tmp;rnorm(1:npfb,2) = tmp
+     read(30) ib2
+     read(30) rb3
+     inflow_type = ib2(1);inflow_velocity_control = ib2(2)
+     u_inflow_start = rb3(1);u_inflow_end = rb3(2);ramp_time = rb3(3)
+     do i=1,npfb
+        if(node_type(i).lt.0) then
+           fd_parent(i) = i + node_type(i)
+           rnorm(i,:) = rnorm(fd_parent(i),:)
+        end if
+     end do
+  end if
   close(30)
   npfb_layer = npfb;npfb_global = npfb
 
   call create_mirror_particles
-  allocate(internal_list(npfb))
+  !! boundary and internal lists as build_domain (setup_domain.F90:314-346)
+  nb = 0
   do i=1,npfb
-     internal_list(i) = i
+     if(node_type(i).ge.0.and.node_type(i).le.2) nb = nb + 1
+  end do
+  allocate(internal_list(npfb-nb))
+  if(nb.ne.0) allocate(boundary_list(nb),znf_vdiff(nb),znf_vtdiff(nb))
+  k = 0;nm = 0
+  do i=1,npfb
+     if(node_type(i).lt.0.or.node_type(i).eq.999.or.node_type(i).eq.998) then
+        k = k + 1;internal_list(k) = i
+     else
+        nm = nm + 1;boundary_list(nm) = i
+        znf_vdiff(nm) = node_type(i).eq.1
+        znf_vtdiff(nm) = node_type(i).eq.2
+     end if
   end do
 
   if(iand(flags,2).ne.0) then
@@ -111,6 +145,17 @@ program sunset_ref
      call calc_labf_weights
      t1 = omp_get_wtime()
      write(6,*) "REFTIME calc_labf_weights",t1-t0
+     if(nb.ne.0) then
+        call calc_boundary_weights
+        !! calc_labf_weights skips boundary nodes and calc_boundary_weights
+        !! writes ij_w_lap only for walls: the Laplacian weights of inflow and
+        !! outflow nodes are whatever the allocation held.  Zero them, as a
+        !! fresh zero-filled allocation would give.
+        do k=1,nb
+           i = boundary_list(k)
+           if(node_type(i).ne.0) ij_w_lap(:,i) = zero
+        end do
+     end if
      call calc_labf_sums
      call filter_coefficients
   end if
@@ -122,6 +167,8 @@ program sunset_ref
   visc_polymeric = (one-beta)*visc_total
   lambda = Wi*L_char/U_char
   csq = (u_char/Ma)**two
+  p_inflow = rho_char*csq;p_outflow = rho_char*csq      !! load_data.F90:147-148
+  L_domain_x = (xmax - xmin)*L_char;L_domain_y = (ymax - ymin)*L_char   !! setup_domain.F90:98-99
   fenep_l2 = 1.0d30
 
   !! initial_solution (setup_flow.F90:24-173), fields from the input file
@@ -153,6 +200,13 @@ program sunset_ref
   do i=npfb+1,np
      u(i)=rou(i)/ro(i);v(i)=rov(i)/ro(i);w(i)=row(i)/ro(i)
   end do
+  if(nb.ne.0) then   !! setup_flow.F90:48-51, :130-137
+     allocate(u_inflow_local(nb));u_inflow_local = u_char
+     do k=1,nb
+        i = boundary_list(k)
+        if(node_type(i).eq.1) u_inflow_local(k) = u(i)
+     end do
+  end if
   dt = rh(20);time = rh(21)
   emax_np1 = rh(22);emax_n = rh(23);emax_nm1 = rh(24)
   ero_norm = one;erou_norm = one/(one*u_char);exx_norm = one

@@ -116,6 +116,7 @@ void or_domain_free(or_domain *d) {
   free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
   free(d->w_grad); free(d->w_lap); free(d->w_hyp);
   free(d->gsum); free(d->lsum); free(d->hsum); free(d->vol);
+  free(d->blist); free(d->fdp); free(d->rnx); free(d->rny);
   memset(d, 0, sizeof(*d));
 }
 
@@ -435,6 +436,21 @@ static void reduce_order(double *A, int n, int nkeep) {
 }
 
 static void node_weights(or_domain *d, int i, int morder, int forder) {
+  {
+    /* boundary nodes are not on internal_list (labf.F90:82, :268, :463): their
+     * operators come from calc_boundary_weights, their volume is s^2
+     * (:490-494) */
+    const int t = d->node_type[i];
+    if (t >= 0 && t <= 2) {
+      const size_t b0 = (size_t)i * d->nplink;
+      for (int k = 0; k < d->nplink; k++) {
+        d->w_grad[(b0 + k) * 2 + 0] = 0; d->w_grad[(b0 + k) * 2 + 1] = 0;
+        d->w_lap[b0 + k] = 0; d->w_hyp[b0 + k] = 0;
+      }
+      d->vol[i] = d->s[i] * d->s[i];
+      return;
+    }
+  }
   const int nG = nsize_of(morder);
   const int nF = nsize_of(forder);
   const int nL = nF > nG ? nF : nG;
@@ -601,23 +617,47 @@ void or_calc_labf_sums(or_domain *d) {
   }
 }
 
-/* filter_coefficients, labf.F90:1723-1811 (interior nodes) */
+/* filter_coefficients, labf.F90:1723-1811 */
 void or_filter_coefficients(or_domain *d) {
+  const double oosqrt2 = 1.0 / sqrt(2.0);
 #pragma omp parallel for
   for (int i = 0; i < d->npfb; i++) {
     const size_t base = (size_t)i * d->nplink;
-    double lscal = 2.0 * d->h[i] * sqrt(PI / (double)d->ij_count[i]);
-    double tmp = (2.0 / 3.0) * PI / lscal;
-    double lsum = 0;
-    for (int k = 0; k < d->ij_count[i]; k++) {
-      int j = d->ij_link[base + k];
-      double x = d->x[j] - d->x[i], y = d->y[j] - d->y[i];
-      double fji = 1.0 - cos(tmp * x) * cos(tmp * y);
-      lsum += fji * d->w_hyp[base + k];
+    const int nt = d->node_type[i];
+    double fc;
+    if (nt >= 0 && nt <= 2) {
+      /* boundary nodes: tangential filter only, radial test function
+       * (:1770-1792) */
+      double lscal = d->s[i];
+      double tmp = (2.0 / 3.0) * PI / lscal;
+      double lsum = 0;
+      for (int k = 0; k < d->ij_count[i]; k++) {
+        int j = d->ij_link[base + k];
+        double x = d->x[j] - d->x[i], y = d->y[j] - d->y[i];
+        double fji = 1.0 - cos(tmp * sqrt(x * x + y * y));
+        lsum += fji * d->w_hyp[base + k];
+      }
+      fc = (1.0 / 3.0) / lsum;
+      fc = fc * 0.5;
+    } else {
+      double lscal = 2.0 * d->h[i] * sqrt(PI / (double)d->ij_count[i]);
+      double tmp = (2.0 / 3.0) * PI / lscal;
+      double lsum = 0;
+      for (int k = 0; k < d->ij_count[i]; k++) {
+        int j = d->ij_link[base + k];
+        double x = d->x[j] - d->x[i], y = d->y[j] - d->y[i];
+        double fji = 1.0 - cos(tmp * x) * cos(tmp * y);
+        lsum += fji * d->w_hyp[base + k];
+      }
+      fc = (2.0 / 3.0) / lsum;
+      /* weaker filtering on the FD rows of walls (:1756-1763) */
+      if (nt < 0 && d->fdp && d->fdp[i] >= 0 && d->node_type[d->fdp[i]] == 0) {
+        if (nt == -1) fc = fc * 0.5 * oosqrt2;
+        if (nt == -2) fc = fc * 0.5;
+        if (nt == -3) fc = fc * oosqrt2;
+        if (nt == -4) fc = fc * oosqrt2;
+      }
     }
-    double fc = (2.0 / 3.0) / lsum;
-    int nt = d->node_type[i];
-    (void)nt;  /* wall-row reductions need fd_parent (boundary nodes): not in scope */
     double hs = 0;
     for (int k = 0; k < d->ij_count[i]; k++) {
       d->w_hyp[base + k] = d->w_hyp[base + k] * fc;
@@ -762,7 +802,7 @@ int or_state_alloc(or_state *st, int np) {
   memset(st, 0, sizeof(*st));
   st->np = np;
   double **f[] = {&st->ro, &st->rou, &st->rov, &st->u, &st->v, &st->pxx, &st->pxy,
-                  &st->pyy, &st->cxx, &st->cxy, &st->cyy};
+                  &st->pyy, &st->cxx, &st->cxy, &st->cyy, &st->u_in};
   for (unsigned k = 0; k < sizeof(f) / sizeof(f[0]); k++) {
     *f[k] = calloc(np, sizeof(double));
     if (!*f[k]) return -1;
@@ -773,6 +813,7 @@ int or_state_alloc(or_state *st, int np) {
 void or_state_free(or_state *st) {
   free(st->ro); free(st->rou); free(st->rov); free(st->u); free(st->v);
   free(st->pxx); free(st->pxy); free(st->pyy); free(st->cxx); free(st->cxy); free(st->cyy);
+  free(st->u_in);
   memset(st, 0, sizeof(*st));
 }
 
@@ -832,9 +873,190 @@ void or_set_initial(const or_domain *d, const or_phys *p, or_state *st, const do
     st->u[i] = st->rou[i] / st->ro[i];
     st->v[i] = st->rov[i] / st->ro[i];
   }
+  for (int jj = 0; jj < d->nb; jj++) {          /* setup_flow.F90:48-51, :130-137 */
+    const int i = d->blist[jj];
+    st->u_in[i] = (d->node_type[i] == 1) ? st->u[i] : p->U_char;
+  }
   st->dt = 1.0e-6;                              /* setup_flow.F90:149-151 */
   st->emax_np1 = PID_TOL; st->emax_n = PID_TOL; st->emax_nm1 = PID_TOL;
   st->time = 0.0; st->itime = 0;
+}
+
+/* calc_rhs_logconf pointwise part (rhs_lc.F90:416-557) for one node: adv is
+ * the advection term, G the velocity gradient d u_a / d x_b */
+static void lc_point(const double adv[3], double G11, double G12, double G21, double G22,
+                 double P11, double P12, double P22, int first, double eps, double lam,
+                 double out[3]) {
+  double L[2], R[4];
+  or_eigens2(P11, P12, P22, L, R);
+  L[0] = exp(L[0]); L[1] = exp(L[1]);
+  double tr_c = L[0] + L[1];
+  /* M = R^T (G R) */
+  double GR11 = G11 * R[0] + G12 * R[1], GR12 = G11 * R[2] + G12 * R[3];
+  double GR21 = G21 * R[0] + G22 * R[1], GR22 = G21 * R[2] + G22 * R[3];
+  double M11 = R[0] * GR11 + R[1] * GR21, M12 = R[0] * GR12 + R[1] * GR22;
+  double M21 = R[2] * GR11 + R[3] * GR21, M22 = R[2] * GR12 + R[3] * GR22;
+  double om = L[1] * M12 + L[0] * M21;
+  if (fabs(L[1] - L[0]) <= VERYSMALL) om = 0.0;
+  else om = om / (L[1] - L[0]);
+  /* B = R diag(M11,M22) R^T ; Omega = R [[0,om],[-om,0]] R^T */
+  double B11, B12, B21, B22, O11, O12, O21, O22;
+  {
+    double i11 = M11 * R[0], i12 = M11 * R[1], i21 = M22 * R[2], i22 = M22 * R[3];
+    B11 = R[0] * i11 + R[2] * i21; B12 = R[0] * i12 + R[2] * i22;
+    B21 = R[1] * i11 + R[3] * i21; B22 = R[1] * i12 + R[3] * i22;
+  }
+  {
+    /* inner = Mom * R^T, Mom = [[0,om],[-om,0]] */
+    double i11 = om * R[2], i12 = om * R[3];
+    double i21 = -om * R[0], i22 = -om * R[1];
+    O11 = R[0] * i11 + R[2] * i21; O12 = R[0] * i12 + R[2] * i22;
+    O21 = R[1] * i11 + R[3] * i21; O22 = R[1] * i12 + R[3] * i22;
+  }
+  if (first) {                               /* rhs_lc.F90:503-506 */
+    O11 = O12 = O21 = O22 = 0.0;
+    B11 = 0.5 * (G11 + G11); B12 = 0.5 * (G12 + G21);
+    B21 = 0.5 * (G21 + G12); B22 = 0.5 * (G22 + G22);
+  }
+  /* UC = Om*Psi - Psi*Om + 2B */
+  double UC11 = (O11 * P11 + O12 * P12) - (P11 * O11 + P12 * O21) + 2.0 * B11;
+  double UC12 = (O11 * P12 + O12 * P22) - (P11 * O12 + P12 * O22) + 2.0 * B12;
+  double UC22 = (O21 * P12 + O22 * P22) - (P12 * O12 + P22 * O22) + 2.0 * B22;
+  /* sPTT relaxation, rhs_lc.F90:538-545 */
+  double fR = 1.0 - 2.0 * eps + eps * tr_c;
+  double C1 = fR / L[0] - fR, C2 = fR / L[1] - fR;
+  double RX11, RX12, RX22;
+  {
+    double i11 = C1 * R[0], i12 = C1 * R[1], i21 = C2 * R[2], i22 = C2 * R[3];
+    RX11 = (R[0] * i11 + R[2] * i21) / lam;
+    RX12 = (R[0] * i12 + R[2] * i22) / lam;
+    RX22 = (R[1] * i12 + R[3] * i22) / lam;
+  }
+  (void)B21;
+  out[0] = adv[0] + UC11 + RX11;
+  out[1] = adv[1] + UC12 + RX12;
+  out[2] = adv[2] + UC22 + RX22;
+}
+
+/* Right-hand sides of boundary nodes, log-conformation form: the boundary
+ * branches of calc_rhs_ro (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and
+ * calc_rhs_logconf (:564-745), then the characteristic waves of
+ * characteristic_boundaries.F90:27-175 applied by calc_rhs_nscbc (:752-813).
+ * Wall derivatives are in the normal-tangent frame (component 1 = d/dn),
+ * inflow/outflow ones in the x-y frame. */
+static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, int iRKstep,
+                      const double *rox, const double *roy, const double *ux, const double *uy,
+                      const double *vx, const double *vy, const double *lapu,
+                      const double *lapv, const double *cxxx, const double *cxyx,
+                      const double *cxyy, const double *cyyy, double *r_ro, double *r_rou,
+                      double *r_rov, double *r_xx, double *r_xy, double *r_yy) {
+  const double csq = p->csq, c = sqrt(csq);
+  const double coef_solvent = p->visc_solvent;
+  const double coef_polymeric = p->visc_polymeric / p->lambda;
+  const double nscbc = 2.87e-1;                       /* common_parameter.F90:71 */
+  const double Lx = (d->xmax - d->xmin) * p->L_char;  /* setup_domain.F90:99 */
+  const double p_in = p->rho_char * csq, p_out = p->rho_char * csq;   /* load_data.F90:147-148 */
+  const int first = (st->itime == 1 && iRKstep == 1);
+  for (int jj = 0; jj < d->nb; jj++) {
+    const int i = d->blist[jj], t = d->node_type[i];
+    const double ro = st->ro[i], u = st->u[i], v = st->v[i];
+    const double pr = csq * ro;
+    const double dpdn = csq * rox[i];
+    double L1, L2, L3, L5;
+    if (t == 0) {
+      const double xn = d->rnx[i], yn = d->rny[i];
+      const double dutdt = -yn * uy[i] + xn * vy[i];
+      r_ro[i] = -ro * dutdt - ro * 0.0;
+      const double un = u * xn + v * yn;
+      const double dundn = xn * ux[i] + yn * vx[i];
+      const double dutdn = -yn * ux[i] + xn * vx[i];
+      L1 = 0.5 * (un - c) * (dpdn - ro * c * dundn);
+      L2 = un * (rox[i] - csq * rox[i] / c / c);
+      L3 = un * dutdn;
+      L5 = 0.5 * (un + c) * (dpdn + ro * c * dundn);
+      r_rou[i] = 0.0;
+      r_rov[i] = 0.0;
+      /* conformation: velocity gradient rotated to x-y, no advection */
+      const double G11 = xn * ux[i] - yn * uy[i], G12 = yn * ux[i] + xn * uy[i];
+      const double G21 = xn * vx[i] - yn * vy[i], G22 = yn * vx[i] + xn * vy[i];
+      const double adv[3] = {0.0, 0.0, 0.0};
+      double o[3];
+      lc_point(adv, G11, G12, G21, G22, st->pxx[i], st->pxy[i], st->pyy[i], first, p->epsPTT,
+               p->lambda, o);
+      r_xx[i] = o[0]; r_xy[i] = o[1]; r_yy[i] = o[2];
+      L5 = L1;                                       /* isothermal wall: reflection */
+    } else {
+      r_ro[i] = -v * roy[i] - ro * vy[i] - 0.0 * 0.0 - ro * 0.0;
+      L1 = 0.5 * (u - c) * (dpdn - ro * c * ux[i]);
+      L2 = u * (rox[i] - csq * rox[i] / c / c);
+      L3 = u * vx[i];
+      L5 = 0.5 * (u + c) * (dpdn + ro * c * ux[i]);
+      double fvu = coef_solvent * lapu[i];
+      double fvv = coef_solvent * lapv[i];
+      fvu = fvu + coef_polymeric * (cxxx[i] + cxyy[i] + 0.0);
+      fvv = fvv + coef_polymeric * (cxyx[i] + cyyy[i] + 0.0);
+      const double bfu = ro * (p->grav[0] + p->drive[0]);
+      const double bfv = ro * (p->grav[1] + p->drive[1]);
+      r_rou[i] = -v * uy[i] - 0.0 * 0.0 + fvu + bfu;
+      r_rov[i] = -v * vy[i] - 0.0 * 0.0 - csq * roy[i] + fvv + bfv;
+      /* the conformation right-hand side is the interior one (x-y frame) */
+      if (t == 1) {
+        const double uin = st->u_in[i];
+        if (p->inflow_type == 1) {                   /* hard inflow */
+          L2 = 0.0; L3 = 0.0; L5 = L1;
+        } else if (p->inflow_type == 2) {            /* inflow-outflow */
+          if (u > 0.0) {
+            L2 = 0.0;
+            L3 = v * nscbc * c / Lx;
+            L5 = (u - uin) * nscbc * (1.0 - u / c) * c * c * 1.0 / Lx;
+          } else {
+            L2 = 0.0;
+            L5 = (pr - p_in) * nscbc * c * 1.0 / 2.0 / Lx;
+          }
+        } else {                                     /* soft inflow */
+          const double gp2 = csq * roy[i];
+          L2 = 0.0;
+          L3 = v * nscbc * c / Lx - v * vy[i] - 0.0 * 0.0 - gp2 / ro;
+          L5 = (u - uin) * nscbc * (1.0 - u / c) * c * c * 1.0 / Lx -
+               0.5 * (v * gp2 + pr * vy[i] + ro * c * v * uy[i]) -
+               0.5 * (0.0 * 0.0 + pr * 0.0 + ro * c * 0.0 * 0.0);
+        }
+      } else {                                       /* outflow */
+        if (u < c) L1 = (pr - p_out) * nscbc * c * 1.0 / 2.0 / Lx;
+        L2 = 0.0;
+        if (u <= 0.0) { L2 = 0.0; L3 = 0.0; }
+      }
+    }
+    /* calc_rhs_nscbc (rhs_lc.F90:787-806) */
+    const double ts = (c * c * L2 + L5 + L1) / c / c;
+    r_ro[i] = r_ro[i] - ts;
+    r_rou[i] = r_rou[i] - u * ts - (L5 - L1) / c;
+    r_rov[i] = r_rov[i] - v * ts - ro * L3;
+  }
+}
+
+/* apply_time_dependent_bounds + update_u_inflow
+ * (characteristic_boundaries.F90:176-289): walls hold zero velocity, hard
+ * inflows the inflow velocity; the ramped inflow velocity follows time while
+ * time <= ramp_time. */
+static void apply_bounds(const or_domain *d, const or_phys *p, or_state *st) {
+  if (p->u_control == 1 && st->time <= p->ramp_time) {
+    const double um = p->u_start + (p->u_end - p->u_start) * st->time / p->ramp_time;
+    for (int jj = 0; jj < d->nb; jj++) {
+      const int i = d->blist[jj];
+      if (d->node_type[i] == 1) st->u_in[i] = um;
+    }
+  }
+  for (int jj = 0; jj < d->nb; jj++) {
+    const int i = d->blist[jj], t = d->node_type[i];
+    if (t == 0) {
+      st->rou[i] = 0.0;
+      st->rov[i] = 0.0;
+    } else if (t == 1 && p->inflow_type == 1) {
+      st->rou[i] = st->u_in[i] * st->ro[i];
+      st->rov[i] = 0.0;
+    }
+  }
 }
 
 /* calc_all_rhs + calc_rhs_ro + calc_rhs_rovel + calc_rhs_logconf/calc_rhs_c,
@@ -897,61 +1119,14 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
 #pragma omp parallel for
     for (int i = 0; i < n; i++) {               /* rhs_lc.F90:414-558 */
       double u = st->u[i], v = st->v[i];
-      double adxx = -(u * pxxx[i] + v * pxxy[i] + 0.0);
-      double adxy = -(u * pxyx[i] + v * pxyy[i] + 0.0);
-      double adyy = -(u * pyyx[i] + v * pyyy[i] + 0.0);
-      double L[2], R[4];
-      or_eigens2(st->pxx[i], st->pxy[i], st->pyy[i], L, R);
-      L[0] = exp(L[0]); L[1] = exp(L[1]);
-      double tr_c = L[0] + L[1];
-      /* gradu_local(a,b) = d u_a / d x_b */
-      double G11 = ux[i], G12 = uy[i], G21 = vx[i], G22 = vy[i];
-      /* M = R^T (G R) */
-      double GR11 = G11 * R[0] + G12 * R[1], GR12 = G11 * R[2] + G12 * R[3];
-      double GR21 = G21 * R[0] + G22 * R[1], GR22 = G21 * R[2] + G22 * R[3];
-      double M11 = R[0] * GR11 + R[1] * GR21, M12 = R[0] * GR12 + R[1] * GR22;
-      double M21 = R[2] * GR11 + R[3] * GR21, M22 = R[2] * GR12 + R[3] * GR22;
-      double om = L[1] * M12 + L[0] * M21;
-      if (fabs(L[1] - L[0]) <= VERYSMALL) om = 0.0;
-      else om = om / (L[1] - L[0]);
-      /* B = R diag(M11,M22) R^T ; Omega = R [[0,om],[-om,0]] R^T */
-      double B11, B12, B21, B22, O11, O12, O21, O22;
-      {
-        double i11 = M11 * R[0], i12 = M11 * R[1], i21 = M22 * R[2], i22 = M22 * R[3];
-        B11 = R[0] * i11 + R[2] * i21; B12 = R[0] * i12 + R[2] * i22;
-        B21 = R[1] * i11 + R[3] * i21; B22 = R[1] * i12 + R[3] * i22;
-      }
-      {
-        /* inner = Mom * R^T, Mom = [[0,om],[-om,0]] */
-        double i11 = om * R[2], i12 = om * R[3];
-        double i21 = -om * R[0], i22 = -om * R[1];
-        O11 = R[0] * i11 + R[2] * i21; O12 = R[0] * i12 + R[2] * i22;
-        O21 = R[1] * i11 + R[3] * i21; O22 = R[1] * i12 + R[3] * i22;
-      }
-      if (first) {                               /* rhs_lc.F90:503-506 */
-        O11 = O12 = O21 = O22 = 0.0;
-        B11 = 0.5 * (G11 + G11); B12 = 0.5 * (G12 + G21);
-        B21 = 0.5 * (G21 + G12); B22 = 0.5 * (G22 + G22);
-      }
-      double P11 = st->pxx[i], P12 = st->pxy[i], P22 = st->pyy[i];
-      /* UC = Om*Psi - Psi*Om + 2B */
-      double UC11 = (O11 * P11 + O12 * P12) - (P11 * O11 + P12 * O21) + 2.0 * B11;
-      double UC12 = (O11 * P12 + O12 * P22) - (P11 * O12 + P12 * O22) + 2.0 * B12;
-      double UC22 = (O21 * P12 + O22 * P22) - (P12 * O12 + P22 * O22) + 2.0 * B22;
-      /* sPTT relaxation, rhs_lc.F90:538-545 */
-      double fR = 1.0 - 2.0 * eps + eps * tr_c;
-      double C1 = fR / L[0] - fR, C2 = fR / L[1] - fR;
-      double RX11, RX12, RX22;
-      {
-        double i11 = C1 * R[0], i12 = C1 * R[1], i21 = C2 * R[2], i22 = C2 * R[3];
-        RX11 = (R[0] * i11 + R[2] * i21) / lam;
-        RX12 = (R[0] * i12 + R[2] * i22) / lam;
-        RX22 = (R[1] * i12 + R[3] * i22) / lam;
-      }
-      (void)B21;
-      r_xx[i] = adxx + UC11 + RX11;
-      r_xy[i] = adxy + UC12 + RX12;
-      r_yy[i] = adyy + UC22 + RX22;
+      double adv[3] = {-(u * pxxx[i] + v * pxxy[i] + 0.0), -(u * pxyx[i] + v * pxyy[i] + 0.0),
+                       -(u * pyyx[i] + v * pyyy[i] + 0.0)};
+      double out[3];
+      lc_point(adv, ux[i], uy[i], vx[i], vy[i], st->pxx[i], st->pxy[i], st->pyy[i], first, eps,
+               lam, out);
+      r_xx[i] = out[0];
+      r_xy[i] = out[1];
+      r_yy[i] = out[2];
     }
   } else {
     const double eps = p->epsPTT, lam = p->lambda, Md = p->Mdiff;
@@ -982,6 +1157,9 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
     }
     free(lcxy); free(lcyy);
   }
+  if (d->nb > 0 && p->form == OR_FORM_LC)
+    bound_rhs(d, p, st, iRKstep, rox, roy, ux, uy, vx, vy, lapu, lapv, cxxx, cxyx, cxyy, cyyy,
+              r_ro, r_rou, r_rov, r_xx, r_xy, r_yy);
   for (int k = 0; k < 22; k++) free(g[k]);
   (void)tmpl;
 }
@@ -1070,6 +1248,7 @@ void or_step_rk3(const or_domain *d, const or_phys *p, or_state *st) {
         reg[f][i] = reg[f][i] + RKb[k] * rhs[f][i];
         eacc[f][i] = eacc[f][i] + RKbmbh[k] * rhs[f][i];
       }
+    if (d->nb) apply_bounds(d, p, st);                         /* :346 */
     or_reapply_mirror_bcs(d, p, st);
     velocity_from_momentum(d, st);
     if (p->form == OR_FORM_LC) c_from_psi_all(d, st);
@@ -1087,8 +1266,10 @@ void or_step_rk3(const or_domain *d, const or_phys *p, or_state *st) {
   double emax_conf = fmax(en[3], fmax(en[4], en[5]));
   st->emax_np1 = fmax(emax_conf, fmax(en[0], fmax(en[1], fmax(en[2], DOUBLESMALL))));
   for (int f = 0; f < 6; f++) { free(reg[f]); free(rhs[f]); free(eacc[f]); }
+  if (d->nb) apply_bounds(d, p, st);                           /* :488 */
   or_reapply_mirror_bcs(d, p, st);                             /* :489 */
   or_filter_variables(d, p, st);                               /* :495 */
+  if (d->nb) apply_bounds(d, p, st);                           /* :498 */
   or_reapply_mirror_bcs(d, p, st);                             /* :499 */
   velocity_from_momentum(d, st);
   if (p->form == OR_FORM_LC) c_from_psi_all(d, st);

@@ -52,6 +52,13 @@ typedef struct {
   double *w_lap, *w_hyp;         /* [npfb][nplink] */
   double *gsum, *lsum, *hsum, *vol; /* [npfb][2], [npfb], [npfb], [npfb] */
   double L_char;
+  /* Boundaries (setup_domain.F90:158-202, :314-326): node types 0 wall,
+   * 1 inflow, 2 outflow, each followed in index order by its four FD rows
+   * (types -1..-4) along the normal.  Set by or_set_boundaries. */
+  int nb;
+  int *blist;                    /* [nb] boundary nodes, ascending */
+  int *fdp;                      /* [cap] FD parent of a row node, -1 otherwise */
+  double *rnx, *rny;             /* [cap] unit normal into the fluid (ghosts: parent's) */
 } or_domain;
 
 typedef struct {
@@ -61,6 +68,10 @@ typedef struct {
   int form, dt_mode;
   /* derived as load_data.F90:132-148 */
   double rho_char, visc_total, visc_solvent, visc_polymeric, lambda, csq;
+  /* inflow boundaries (load_data.F90:152): type 0 soft, 1 hard, 2 inflow-outflow;
+   * velocity control 0 fixed (the initial u), 1 ramp u_start -> u_end over ramp_time */
+  int inflow_type, u_control;
+  double u_start, u_end, ramp_time;
 } or_phys;
 
 typedef struct {
@@ -71,6 +82,7 @@ typedef struct {
   double dt, time, dt_cfl, dt_par;
   double emax_np1, emax_n, emax_nm1;
   int itime;
+  double *u_in;                  /* [np] u_inflow_local of inflow nodes (setup_flow.F90:48-51, :130-137) */
 } or_state;
 
 /* ---- domain ---- */
@@ -85,6 +97,13 @@ int  or_find_neighbours(or_domain *d, int nplink);
 int  or_calc_labf_weights(or_domain *d);     /* grad, lap, hyp(raw), vol */
 void or_calc_labf_sums(or_domain *d);
 void or_filter_coefficients(or_domain *d);
+/* Boundary nodes (after or_create_ghosts): normals for the npfb real nodes;
+ * checks the FD-row layout, builds the boundary list and FD parents.  -1 on a
+ * malformed layout. */
+int  or_set_boundaries(or_domain *d, const double *rnx, const double *rny);
+/* calc_boundary_weights (labf.F90:505-1039): between or_calc_labf_weights and
+ * or_calc_labf_sums */
+int  or_boundary_weights(or_domain *d);
 /* per-node order map (nullable): n-system size chosen per node (config C3) */
 int  or_calc_labf_weights_ordermap(or_domain *d, const int *order_of_node,
                                    int nthreads);

@@ -23,6 +23,8 @@ import numpy as np
 
 # stencil-size multipliers h = hovs*s per order, common_parameter.F90:45-56
 HOVS = {4: 1.8, 6: 2.4, 8: 2.7, 10: 3.1}
+# h = hovs_bound*s on boundary nodes and their FD rows (setup_domain.F90:182)
+HOVS_BOUND = {4: 2.1, 6: 2.4, 8: 2.4, 10: 2.4}
 
 
 @dataclasses.dataclass
@@ -40,6 +42,15 @@ class NodeSet:
     bc: tuple = (1, 1, 1, 1)
     smax: float = 0.0
     meta: dict = dataclasses.field(default_factory=dict)
+    # unit normals into the fluid of boundary nodes (types 0 wall, 1 inflow,
+    # 2 outflow) and their FD rows (types -1..-4, which follow their boundary
+    # node in index order); None when the set has no boundary nodes
+    rnx: Optional[np.ndarray] = None
+    rny: Optional[np.ndarray] = None
+
+    @property
+    def nb(self) -> int:
+        return int(np.count_nonzero((self.node_type >= 0) & (self.node_type <= 2)))
 
     @property
     def npfb(self) -> int:
@@ -228,3 +239,81 @@ def cylinder_channel(smin: float = 0.004, smax: float = 0.016, grow: float = 1.5
                  meta=dict(generator="advancing front (Fornberg-Flyer 2015) + PCG64 jitter",
                            seed=seed, smin=smin, smax=smax, grow=grow))
     return ns, order
+
+
+def _fd_column(xb, yb, nx_, ny_, s, hb, tb):
+    """A boundary node and its four FD rows along the normal
+    (setup_domain.F90:179-193)."""
+    xs = [xb + nx_ * j * s for j in range(5)]
+    ys = [yb + ny_ * j * s for j in range(5)]
+    return xs, ys, [tb, -1, -2, -3, -4]
+
+
+def wall_channel(nx: int, ny: int, *, order: int = 6, seed: int = 0, jitter: float = 0.3,
+                 box=(0.0, 1.0, 0.0, 1.0)) -> NodeSet:
+    """Channel periodic in x with no-slip walls (node type 0) at y = ymin and
+    y = ymax.  Spacing s = (xmax-xmin)/nx; ny = (ymax-ymin)/s layers.  Wall
+    nodes sit at x = (i+0.5)s with their FD rows at 1..4 s along the normal;
+    the interior is a jittered lattice on the layers 5 .. ny-5 (PCG64, seed).
+    Nodes are written column by column, a boundary node followed by its rows.
+    h = hovs*s inside, hovs_bound*s on walls and rows (setup_domain.F90:180-190)."""
+    x0, x1, y0, y1 = box
+    s = (x1 - x0) / nx
+    if abs((y1 - y0) / s - ny) > 1e-9 or ny < 12:
+        raise ValueError("ny must equal (ymax-ymin)/s and be >= 12")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    X, Y, T, NX, NY = [], [], [], [], []
+    for i in range(nx):
+        xc = x0 + (i + 0.5) * s
+        xs, ys, ts = _fd_column(xc, y0, 0.0, 1.0, s, 0, 0)
+        X += xs; Y += ys; T += ts; NX += [0.0] * 5; NY += [1.0] * 5
+        for m in range(5, ny - 4):
+            X.append(xc + rng.uniform(-jitter, jitter) * s)
+            Y.append(y0 + m * s + rng.uniform(-jitter, jitter) * s)
+            T.append(999); NX.append(0.0); NY.append(0.0)
+        xs, ys, ts = _fd_column(xc, y1, 0.0, -1.0, s, 0, 0)
+        X += xs; Y += ys; T += ts; NX += [0.0] * 5; NY += [-1.0] * 5
+    x, y = np.array(X), np.array(Y)
+    t = np.array(T, dtype=np.int32)
+    sv = np.full(x.shape[0], s)
+    h = np.where(t == 999, HOVS[order], HOVS_BOUND[order]) * s
+    return NodeSet(x=x, y=y, s=sv, h=h, h_small=h.copy(), node_type=t, xmin=x0, xmax=x1,
+                   ymin=y0, ymax=y1, bc=(1, 1, 0, 0), smax=s,
+                   meta=dict(generator="PCG64", seed=seed, nx=nx, ny=ny, jitter=jitter,
+                             kind="wall_channel"),
+                   rnx=np.array(NX), rny=np.array(NY))
+
+
+def inflow_outflow_channel(nx: int, ny: int, *, order: int = 6, seed: int = 0,
+                           jitter: float = 0.3, box=(0.0, 1.0, 0.0, 1.0)) -> NodeSet:
+    """Channel periodic in y with an inflow (node type 1, normal +x) at
+    x = xmin and an outflow (type 2, normal -x) at x = xmax; spacing
+    s = (ymax-ymin)/ny, nx = (xmax-xmin)/s layers; layout as wall_channel."""
+    x0, x1, y0, y1 = box
+    s = (y1 - y0) / ny
+    if abs((x1 - x0) / s - nx) > 1e-9 or nx < 12:
+        raise ValueError("nx must equal (xmax-xmin)/s and be >= 12")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    X, Y, T, NX, NY = [], [], [], [], []
+    for j in range(ny):
+        yc = y0 + (j + 0.5) * s
+        xs, ys, ts = _fd_column(x0, yc, 1.0, 0.0, s, 1, 1)
+        X += xs; Y += ys; T += [1, -1, -2, -3, -4]; NX += [1.0] * 5; NY += [0.0] * 5
+    for m in range(5, nx - 4):
+        for j in range(ny):
+            X.append(x0 + m * s + rng.uniform(-jitter, jitter) * s)
+            Y.append(y0 + (j + 0.5) * s + rng.uniform(-jitter, jitter) * s)
+            T.append(999); NX.append(0.0); NY.append(0.0)
+    for j in range(ny):
+        yc = y0 + (j + 0.5) * s
+        xs, ys, ts = _fd_column(x1, yc, -1.0, 0.0, s, 2, 2)
+        X += xs; Y += ys; T += [2, -1, -2, -3, -4]; NX += [-1.0] * 5; NY += [0.0] * 5
+    x, y = np.array(X), np.array(Y)
+    t = np.array(T, dtype=np.int32)
+    sv = np.full(x.shape[0], s)
+    h = np.where(t == 999, HOVS[order], HOVS_BOUND[order]) * s
+    return NodeSet(x=x, y=y, s=sv, h=h, h_small=h.copy(), node_type=t, xmin=x0, xmax=x1,
+                   ymin=y0, ymax=y1, bc=(0, 0, 1, 1), smax=s,
+                   meta=dict(generator="PCG64", seed=seed, nx=nx, ny=ny, jitter=jitter,
+                             kind="inflow_outflow_channel"),
+                   rnx=np.array(NX), rny=np.array(NY))

@@ -27,7 +27,8 @@ class Domain(C.Structure):
                 ("forder", C.c_int), ("ij_count", PI), ("ij_count_small", PI),
                 ("ij_link", PI), ("w_grad", PD), ("w_lap", PD), ("w_hyp", PD),
                 ("gsum", PD), ("lsum", PD), ("hsum", PD), ("vol", PD),
-                ("L_char", C.c_double)]
+                ("L_char", C.c_double), ("nb", C.c_int), ("blist", PI), ("fdp", PI),
+                ("rnx", PD), ("rny", PD)]
 
 
 class Phys(C.Structure):
@@ -37,7 +38,9 @@ class Phys(C.Structure):
                 ("drive", C.c_double * 3), ("kol_amp", C.c_double), ("kol_k", C.c_double),
                 ("form", C.c_int), ("dt_mode", C.c_int), ("rho_char", C.c_double),
                 ("visc_total", C.c_double), ("visc_solvent", C.c_double),
-                ("visc_polymeric", C.c_double), ("lam", C.c_double), ("csq", C.c_double)]
+                ("visc_polymeric", C.c_double), ("lam", C.c_double), ("csq", C.c_double),
+                ("inflow_type", C.c_int), ("u_control", C.c_int), ("u_start", C.c_double),
+                ("u_end", C.c_double), ("ramp_time", C.c_double)]
 
 
 class State(C.Structure):
@@ -45,7 +48,7 @@ class State(C.Structure):
                 ("pxx", PD), ("pxy", PD), ("pyy", PD), ("cxx", PD), ("cxy", PD), ("cyy", PD),
                 ("dt", C.c_double), ("time", C.c_double), ("dt_cfl", C.c_double),
                 ("dt_par", C.c_double), ("emax_np1", C.c_double), ("emax_n", C.c_double),
-                ("emax_nm1", C.c_double), ("itime", C.c_int)]
+                ("emax_nm1", C.c_double), ("itime", C.c_int), ("u_in", PD)]
 
 
 class Tangent(C.Structure):
@@ -101,6 +104,8 @@ def lib():
                                          PD]
         L.or_diag_tg.argtypes = [C.POINTER(Domain), C.POINTER(Phys), C.POINTER(State)]
         L.or_diag_tg.restype = C.c_double
+        L.or_set_boundaries.argtypes = [C.POINTER(Domain), PD, PD]
+        L.or_boundary_weights.argtypes = [C.POINTER(Domain)]
         L.or_set_num_threads.argtypes = [C.c_int]
         L.or_num_threads.restype = C.c_int
         _lib = L
@@ -154,6 +159,15 @@ class OracleSim:
     def ghosts(self):
         rc = lib().or_create_ghosts(C.byref(self.d))
         assert rc == 0, rc
+        if getattr(self.ns, "rnx", None) is not None:
+            self._rn = [np.ascontiguousarray(self.ns.rnx, dtype=np.float64),
+                        np.ascontiguousarray(self.ns.rny, dtype=np.float64)]
+            rc = lib().or_set_boundaries(C.byref(self.d), _pd(self._rn[0]), _pd(self._rn[1]))
+            assert rc == 0, "malformed boundary layout"
+
+    @property
+    def nb(self):
+        return self.d.nb
 
     def neighbours(self):
         rc = lib().or_find_neighbours(C.byref(self.d), self.nplink)
@@ -161,6 +175,8 @@ class OracleSim:
 
     def weights(self):
         assert lib().or_calc_labf_weights(C.byref(self.d)) == 0
+        if self.d.nb:
+            assert lib().or_boundary_weights(C.byref(self.d)) == 0
         lib().or_calc_labf_sums(C.byref(self.d))
         lib().or_filter_coefficients(C.byref(self.d))
 
@@ -247,8 +263,12 @@ class OracleSim:
 
     # ---- physics ----
     def make_phys(self, form="lc", dt_mode="pid", Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
-                  Mdiff=0.0, kol_amp=0.0, kol_k=0.0, U_char=1.0, L_char=1.0):
+                  Mdiff=0.0, kol_amp=0.0, kol_k=0.0, U_char=1.0, L_char=1.0, drive=(0.0, 0.0),
+                  inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0):
         p = Phys()
+        p.drive[0], p.drive[1] = drive
+        p.inflow_type, p.u_control = inflow_type, u_control
+        p.u_start, p.u_end, p.ramp_time = u_start, u_end, ramp_time
         p.L_char, p.U_char, p.Re, p.Wi, p.beta, p.epsPTT, p.Ma, p.Mdiff = (
             L_char, U_char, Re, Wi, beta, eps, Ma, Mdiff)
         p.kol_amp, p.kol_k = kol_amp, kol_k

@@ -30,14 +30,17 @@ def dropin_binary(order: int) -> str:
 
 def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e-6,
                   time0=0.0, emax=(1e-5, 1e-5, 1e-5), itime0=0, nplink=0, threads=4,
-                  timeout=600, stencil=None, fields_only=False, binary=None):
+                  timeout=600, stencil=None, fields_only=False, binary=None, inflow=None):
     """stencil (optional): dict(nplink, ij_count, ij_count_small, ij_link (0-based,
     [npfb, nplink]), w_grad [npfb, nplink, 2], w_lap, w_hyp, gsum [npfb, 2], lsum,
     hsum, vol) loaded instead of the reference's own neighbour/weight setup."""
     phys = dict(L_char=1.0, U_char=1.0, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
                 Mdiff=0.0, grav=(0, 0, 0), drive=(0, 0, 0)) | (phys or {})
     n = ns.npfb
-    flags = (2 if stencil is not None else 0) | (4 if fields_only else 0)
+    has_b = getattr(ns, "rnx", None) is not None
+    flags = (2 if stencil is not None else 0) | (4 if fields_only else 0) | (8 if has_b else 0)
+    inflow = dict(inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0) | (
+        inflow or {})
     hdr = np.array([n, nsteps, 0 if dt_mode == "pid" else 1, nplink, flags, *ns.bc, itime0, 0,
                     0], dtype=np.int32)
     rh = np.array([ns.xmin, ns.xmax, ns.ymin, ns.ymax, ns.smax, phys["L_char"], phys["U_char"],
@@ -54,6 +57,13 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
             f.write(np.ascontiguousarray(ns.node_type, dtype=np.int32).tobytes())
             for k in ("ro", "u", "v", "cxx", "cxy", "cyy"):
                 f.write(np.ascontiguousarray(ic[k], dtype=np.float64).tobytes())
+            if has_b:
+                f.write(np.ascontiguousarray(ns.rnx, dtype=np.float64).tobytes())
+                f.write(np.ascontiguousarray(ns.rny, dtype=np.float64).tobytes())
+                f.write(np.array([inflow["inflow_type"], inflow["u_control"]],
+                                 dtype=np.int32).tobytes())
+                f.write(np.array([inflow["u_start"], inflow["u_end"], inflow["ramp_time"]],
+                                 dtype=np.float64).tobytes())
         args = [binary or ref_binary(order), fin, fout]
         if stencil is not None:
             fst = os.path.join(td, "stencil.bin")
