@@ -80,6 +80,23 @@ int sv_set_domain(sv_ctx *ctx, double xmin, double xmax, double ymin, double yma
 int sv_set_nodes(sv_ctx *ctx, int32_t npfb, const double *x, const double *y,
                  const double *s, const double *h, const double *h_small,
                  const int32_t *node_type);
+/* Boundary nodes.  node_type 0 (wall), 1 (inflow) or 2 (outflow) marks a
+ * boundary node; it must be followed in node order by its four FD rows
+ * (types -1..-4) at s, 2s, 3s, 4s along its normal, as the IPART reader
+ * generates them (setup_domain.F90:179-193); sv_set_nodes checks this layout.
+ * sv_set_boundaries gives the unit normals into the fluid (rnorm,
+ * setup_domain.F90:177), sized npfb; entries of FD rows are taken from their
+ * boundary node, other non-boundary entries are ignored.  Required before
+ * sv_calc_boundary_weights when the node set has boundary nodes.  The time
+ * loop supports boundary nodes in the log-conformation form. */
+int sv_set_boundaries(sv_ctx *ctx, const double *rnx, const double *rny);
+/* The inflow line of control.in (load_data.F90:152): inflow_type 0 soft
+ * (partially non-reflecting), 1 hard, 2 inflow-outflow
+ * (characteristic_boundaries.F90:50-175); velocity_control 0 keeps each
+ * inflow node's initial u, 1 ramps it from u_start to u_end over ramp_time
+ * (update_u_inflow, :256-289).  Default 0 0 0 1 1. */
+int sv_set_inflow(sv_ctx *ctx, int32_t inflow_type, int32_t velocity_control, double u_start,
+                  double u_end, double ramp_time);
 /* np = npfb + ghosts; x,y sized np; irelation/vrelation sized np-npfb
  * (1-based parents, as common_vars.F90:101) */
 int sv_get_ghosts(sv_ctx *ctx, int32_t *np, double *x, double *y, int32_t *irelation,
@@ -97,6 +114,17 @@ int sv_find_neighbours(sv_ctx *ctx, int32_t nplink, int32_t *ij_count,
  * LABFM order of that node's derivative and filter systems, overriding
  * morder/forder (variable-order node sets). */
 int sv_calc_labf_weights(sv_ctx *ctx, const int32_t *order_map);
+/* calc_boundary_weights (labf.F90:505-1039), after sv_calc_labf_weights and
+ * before sv_calc_labf_sums, as sunset.F90:54-55 (a no-op without boundary
+ * nodes): boundary nodes keep their same-type neighbours and FD rows; their
+ * operators are 5-point one-sided differences along the normal and a 6th-order
+ * 1D LABFM along the tangent (walls: in the normal-tangent frame, with the
+ * curvature term in the Laplacian); rows 1 and 2 get normal differences plus a
+ * 1D LABFM along a tangent turned by the tangential gradient of s, mapped to
+ * x-y.  The Laplacian weights of inflow/outflow nodes are zero (the reference
+ * never writes them).  sv_calc_labf_weights skips boundary nodes and the
+ * derivative systems of rows 1 and 2, which this call replaces. */
+int sv_calc_boundary_weights(sv_ctx *ctx);
 int sv_calc_labf_sums(sv_ctx *ctx);           /* labf.F90:396-451 */
 int sv_filter_coefficients(sv_ctx *ctx);      /* labf.F90:1723-1811 */
 /* Host copies in the reference layouts (see top); any pointer may be NULL */
@@ -199,7 +227,7 @@ int sv_sync(sv_ctx *ctx);
  * process (sunset.F90:36-106); an ensemble is N such runs.  Each member is a
  * context taken through the whole setup and sv_set_initial (its own nodes,
  * orders, stencils, physics and control state); all members must have the
- * same conformation form and device.  The ensemble context holds copies of the
+ * same conformation form and device and no boundary nodes.  The ensemble context holds copies of the
  * members' device data in one concatenated layout, so every launch of a step,
  * and one captured hipGraph (sv_run with use_graph), advances all members;
  * each keeps its own dt / PID history / itime (set_tstep and set_tstep_PID per

@@ -106,6 +106,11 @@ __global__ void k_ghost_write(int npfb, GhostDom g, double *x, double *y, double
 struct Grid {
   double x0, y0, inv;
   int ncx, ncy;
+  // the reference's cells (cell_generator, neighbours.F90:75-93): size
+  // ss*h0 over the domain padded by its size on every side; a node sees the
+  // 3x3 cells around its own (neighbour_cells, :95-157), which cuts the disc
+  // of a node with h > h0
+  double rx0, ry0, rinv, h0;
 };
 
 __device__ __forceinline__ int cell_of(const Grid &G, double x, double y, int *cx, int *cy) {
@@ -160,7 +165,10 @@ constexpr int NW = 4;   // nodes (wavefronts) per block of the neighbour search
 // collects (j, r^2) in LDS, ranks every entry by (r^2, j) -- the order of the
 // reference's sorted lists (neighbours.F90:247-341) -- and writes the row in
 // that order, with the count of entries inside 2 h_small.  The cell size is
-// 2 h_min, so nodes of every size scan a few candidates per neighbour.
+// 2 h_min, so nodes of every size scan a few candidates per neighbour.  A node
+// with h > h0 (boundary nodes at order 4: hovs_bound 2.1 > hovs 1.8,
+// common_parameter.F90:46-47) keeps, as the reference does, only the
+// candidates inside the 3x3 reference cells of size 2 h0 around it.
 template <int PASS>
 __global__ __launch_bounds__(NW * SV_WAVE) void k_neigh(
     int npfb, Grid G, const double *x, const double *y, const double *h, const double *hs,
@@ -176,6 +184,9 @@ __global__ __launch_bounds__(NW * SV_WAVE) void k_neigh(
   const double xi = x[i], yi = y[i];
   const double R = h[i] * SS;
   const double st2 = R * R;
+  // only a node with h > h0 reaches past the reference's 3x3 cells
+  const bool clip = h[i] > G.h0;
+  const int rcx = (int)((xi - G.rx0) * G.rinv), rcy = (int)((yi - G.ry0) * G.rinv);
   int ix0 = (int)((xi - R - G.x0) * G.inv) - 1, ix1 = (int)((xi + R - G.x0) * G.inv) + 1;
   int iy0 = (int)((yi - R - G.y0) * G.inv) - 1, iy1 = (int)((yi + R - G.y0) * G.inv) + 1;
   ix0 = ix0 < 0 ? 0 : ix0;
@@ -194,6 +205,11 @@ __global__ __launch_bounds__(NW * SV_WAVE) void k_neigh(
         j = ip[jj];
         r2 = dist2(xi, yi, x[j], y[j]);
         m = r2 <= st2;
+        if (clip && m) {
+          const int dcx = (int)((x[j] - G.rx0) * G.rinv) - rcx;
+          const int dcy = (int)((y[j] - G.ry0) * G.rinv) - rcy;
+          m = dcx >= -1 && dcx <= 1 && dcy >= -1 && dcy <= 1;
+        }
       }
       const unsigned long long bal = __ballot(m);
       if (PASS == 1 && m) {
@@ -323,6 +339,10 @@ int sv_dev_find_neighbours(sv_ctx *c) {
   G.inv = 1.0 / csz;
   G.ncx = (int)(3.0 * Lx / csz) + 1;
   G.ncy = (int)(3.0 * Ly / csz) + 1;
+  G.rx0 = c->xmin - Lx;
+  G.ry0 = c->ymin - Ly;
+  G.rinv = 1.0 / (1.0 * (SS * c->h0));
+  G.h0 = c->h0;
   const int nct = G.ncx * G.ncy;
   DevTmp<int> cellof, ccount, cstart, ip;
   SV_HIP(c, cellof.alloc(np));
@@ -581,7 +601,8 @@ int sv_dev_build_loop(sv_ctx *c) {
   }
   // (re)allocate
   void *old[] = {L.off, L.col, L.wx, L.wy, L.wl, L.wh, L.cnt, L.cnt_small, L.gsx, L.gsy,
-                 L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm, L.hoff, L.hidx, L.lcol};
+                 L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm, L.hoff, L.hidx, L.lcol,
+                 L.bidx, L.brec};
   for (void *b : old)
     if (b) sv_free(c, b);
   std::vector<int> hp = std::move(L.hperm);

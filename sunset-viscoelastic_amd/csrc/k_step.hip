@@ -240,7 +240,125 @@ struct StageArgs {
   const int *hidx;
   const uint16_t *lcol;
   int hcap;             // records the dynamic LDS holds
+  // boundary nodes (nullable): per loop slot the record index or -1, records
+  const int *bidx;
+  const double *brec;
 };
+
+// u_inflow_local of an inflow node as the last apply_time_dependent_bounds
+// before stage s of the current step left it (update_u_inflow,
+// characteristic_boundaries.F90:256-289): while time <= ramp_time every apply
+// sets the ramp value; the applies of this step ran at time0 + RKc[q-1],
+// q < s.  s = 1: the value at the start of the step.
+__device__ __forceinline__ double uin_now(const sv_physk &P, const sv_ctrl *C, double uin0,
+                                          int s) {
+  if (P.u_control != 1) return uin0;
+  bool set = C->uin_set != 0;
+  double v = C->uin_ramp;
+  for (int q = 1; q < s; q++) {
+    const double t = C->time0 + C->RKc[q - 1];
+    if (t <= P.ramp_time) {
+      set = true;
+      v = P.u_start + (P.u_end - P.u_start) * t / P.ramp_time;
+    }
+  }
+  return set ? v : uin0;
+}
+
+// apply_time_dependent_bounds on one state record (characteristic_boundaries.F90:
+// 176-254): walls hold zero velocity, hard inflows the inflow velocity
+__device__ __forceinline__ void bound_apply(const sv_physk &P, int kind, double uin, double f[6]) {
+  if (kind == 0) {
+    f[1] = 0.0;
+    f[2] = 0.0;
+  } else if (kind == 1 && P.inflow_type == 1) {
+    f[1] = uin * f[0];
+    f[2] = 0.0;
+  }
+}
+
+// Right-hand side of a boundary node, log-conformation form: the boundary
+// branches of calc_rhs_ro (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and
+// calc_rhs_logconf (:564-745), then the characteristic waves
+// (characteristic_boundaries.F90:27-175) applied as calc_rhs_nscbc
+// (rhs_lc.F90:752-813).  Wall derivatives are in the normal-tangent frame
+// (x component = d/dn, y = d/dt), inflow/outflow ones in x-y.  The interior
+// conformation right-hand side (r_xx..r_yy on entry) is kept on inflow and
+// outflow nodes.
+__device__ __forceinline__ void bound_rhs(const sv_physk &P, int kind, double nx, double ny,
+                                          double uin, bool first, double ro, double u, double v,
+                                          double rx, double ry, double ux, double uy, double vx,
+                                          double vy, double lu, double lv, double dcxx_x,
+                                          double dcxy_y, double dcxy_x, double dcyy_y,
+                                          double pxx, double pxy, double pyy, double &r_ro,
+                                          double &r_rou, double &r_rov, double &r_xx,
+                                          double &r_xy, double &r_yy) {
+  const double csq = P.csq, c = P.cs;
+  const double nscbc = 2.87e-1;                   // common_parameter.F90:71
+  const double pr = csq * ro, dpdn = csq * rx;
+  double L1, L2, L3, L5;
+  if (kind == 0) {
+    const double dutdt = -ny * uy + nx * vy;
+    r_ro = -ro * dutdt - ro * 0.0;
+    const double un = u * nx + v * ny;
+    const double dundn = nx * ux + ny * vx;
+    const double dutdn = -ny * ux + nx * vx;
+    L1 = 0.5 * (un - c) * (dpdn - ro * c * dundn);
+    L2 = un * (rx - csq * rx / c / c);
+    L3 = un * dutdn;
+    L5 = 0.5 * (un + c) * (dpdn + ro * c * dundn);
+    r_rou = 0.0;
+    r_rov = 0.0;
+    // velocity gradient rotated to x-y; no advection on a wall
+    const double G11 = nx * ux - ny * uy, G12 = ny * ux + nx * uy;
+    const double G21 = nx * vx - ny * vy, G22 = ny * vx + nx * vy;
+    rhs_logconf(0.0, 0.0, G11, G12, G21, G22, pxx, pxy, pyy, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                first, P.eps, P.lambda, r_xx, r_xy, r_yy);
+    L5 = L1;                                      // isothermal wall
+  } else {
+    r_ro = -v * ry - ro * vy - 0.0 * 0.0 - ro * 0.0;
+    L1 = 0.5 * (u - c) * (dpdn - ro * c * ux);
+    L2 = u * (rx - csq * rx / c / c);
+    L3 = u * vx;
+    L5 = 0.5 * (u + c) * (dpdn + ro * c * ux);
+    double fvu = P.coef_solvent * lu;
+    double fvv = P.coef_solvent * lv;
+    fvu = fvu + P.coef_polymeric * (dcxx_x + dcxy_y + 0.0);
+    fvv = fvv + P.coef_polymeric * (dcxy_x + dcyy_y + 0.0);
+    r_rou = -v * uy - 0.0 * 0.0 + fvu + ro * P.bfx;
+    r_rov = -v * vy - 0.0 * 0.0 - csq * ry + fvv + ro * P.bfy;
+    const double Lx = P.Lx;
+    if (kind == 1) {
+      if (P.inflow_type == 1) {                   // hard inflow
+        L2 = 0.0; L3 = 0.0; L5 = L1;
+      } else if (P.inflow_type == 2) {            // inflow-outflow
+        if (u > 0.0) {
+          L2 = 0.0;
+          L3 = v * nscbc * c / Lx;
+          L5 = (u - uin) * nscbc * (1.0 - u / c) * c * c * 1.0 / Lx;
+        } else {
+          L2 = 0.0;
+          L5 = (pr - P.p_ref) * nscbc * c * 1.0 / 2.0 / Lx;
+        }
+      } else {                                    // soft inflow
+        const double gp2 = csq * ry;
+        L2 = 0.0;
+        L3 = v * nscbc * c / Lx - v * vy - 0.0 * 0.0 - gp2 / ro;
+        L5 = (u - uin) * nscbc * (1.0 - u / c) * c * c * 1.0 / Lx -
+             0.5 * (v * gp2 + pr * vy + ro * c * v * uy) -
+             0.5 * (0.0 * 0.0 + pr * 0.0 + ro * c * 0.0 * 0.0);
+      }
+    } else {                                      // outflow
+      if (u < c) L1 = (pr - P.p_ref) * nscbc * c * 1.0 / 2.0 / Lx;
+      L2 = 0.0;
+      if (u <= 0.0) { L2 = 0.0; L3 = 0.0; }
+    }
+  }
+  const double ts = (c * c * L2 + L5 + L1) / c / c;
+  r_ro = r_ro - ts;
+  r_rou = r_rou - u * ts - (L5 - L1) / c;
+  r_rov = r_rov - v * ts - ro * L3;
+}
 
 // The neighbour sums of one node in a stage: gradients and Laplacians before
 // the own-node term (sum_k w_k f_j), as named in the right-hand side.
@@ -251,7 +369,7 @@ struct Sums {
 // Pointwise part of a stage for one node: the right-hand side from the
 // gathered sums and the own record, then the RK update.  Returns the node's
 // error norm contribution (stage 4).
-template <int FORM, int STAGE, bool MDIFF>
+template <int FORM, int STAGE, bool MDIFF, bool BND>
 __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef &nr,
                                               const Sums &S) {
   const int i = nr.i;
@@ -272,6 +390,8 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
     const sv_ctrl *C = a.ctrl + nr.sg;
     // calc_rhs_ro (rhs_lc.F90:127-141)
     const double r_ro = -ro * (ux + vy + 0.0) - (u * rx + v * ry + 0.0 * 0.0);
+    int bkind = -1;        // boundary node kind (BND), -1 interior
+    double buin0 = 0.0;
     // calc_rhs_rovel (rhs_lc.F90:217-272)
     const double tsu = ro * (u * ux + v * uy + 0.0 * 0.0) - u * r_ro;
     const double tsv = ro * (u * vx + v * vy + 0.0 * 0.0) - v * r_ro;
@@ -286,14 +406,27 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
     double bfu = ro * P.bfx;
     const double bfv = ro * P.bfy;
     if (a.fx) bfu = bfu + a.fx[i];
-    const double r_rou = -tsu - P.csq * rx + bfu + fvu;
-    const double r_rov = -tsv - P.csq * ry + bfv + fvv;
+    double r_ro_ = r_ro;
+    double r_rou = -tsu - P.csq * rx + bfu + fvu;
+    double r_rov = -tsv - P.csq * ry + bfv + fvv;
     double r_xx, r_xy, r_yy;
     if (FORM == SV_FORM_LC) {
       const bool first = (STAGE == 1 || (STAGE == 0 && a.rk1)) && (C->itime == 1);
       rhs_logconf(u, v, ux, uy, vx, vy, s.pxx, s.pxy, s.pyy, q1x - s.pxx * sx,
                   q1y - s.pxx * sy, q2x - s.pxy * sx, q2y - s.pxy * sy, q3x - s.pyy * sx,
                   q3y - s.pyy * sy, first, P.eps, P.lambda, r_xx, r_xy, r_yy);
+      if (BND) {
+        const int jb = a.bidx[i];
+        if (jb >= 0) {
+          const double *br = a.brec + (size_t)SV_BREC * jb;
+          bkind = (int)br[0];
+          buin0 = br[3];
+          const int s_rhs = STAGE == 0 ? 1 : STAGE;
+          bound_rhs(P, bkind, br[1], br[2], uin_now(P, C, buin0, s_rhs), first, ro, u, v, rx,
+                    ry, ux, uy, vx, vy, lu, lv, dcxx_x, dcxy_y, dcxy_x, dcyy_y, s.pxx, s.pxy,
+                    s.pyy, r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy);
+        }
+      }
     } else {
       // calc_rhs_c (rhs_di.F90:417-475) with full gradients of cxx, cyy
       const double gcxx_y = ayx - cxx * sy, gcyy_x = bxy - cyy * sx;
@@ -323,7 +456,7 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
       }
     }
     // ---- RK update (step.F90:291-343, :375-409) ----
-    const double rhs[6] = {r_ro, r_rou, r_rov, r_xx, r_xy, r_yy};
+    const double rhs[6] = {r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy};
     if (STAGE == 0) {
 #pragma unroll
       for (int f = 0; f < 6; f++) a.rhs_out[(size_t)f * a.rhs_stride + i] = rhs[f];
@@ -340,6 +473,7 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
       }
       store_u(a.reg_out, i, ro_);
       store_u(a.e, i, eo);
+      if (BND && bkind >= 0) bound_apply(P, bkind, uin_now(P, C, buin0, STAGE + 1), Un);
       // derived fields for the next stage (step.F90:350-356)
       Rec9 g;
       const double ooro = 1.0 / Un[0];
@@ -366,6 +500,7 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
         const double ef = eo[f] + rm * rhs[f];
         emax_loc = fmax(emax_loc, fabs(ef) * nrm[f]);
       }
+      if (BND && bkind >= 0) bound_apply(P, bkind, uin_now(P, C, buin0, 5), uo);
       store_u(a.Uout, i, uo);
     }
   }
@@ -373,7 +508,7 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
 }
 
 // STAGE 1..4: RK stages; STAGE 0: write the right-hand side into rhs_out.
-template <int FORM, int STAGE, bool MDIFF>
+template <int FORM, int STAGE, bool MDIFF, bool BND>
 __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   __shared__ double red[BS / SV_WAVE];
   const int bid = sv_xcd_block();
@@ -434,7 +569,7 @@ __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   double emax_loc = 0.0;
   if (act) {
     const Sums S{rx, ry, ux, uy, vx, vy, lu, lv, axx, ayx, axy, ayy, bxy, byy, cxx_l, cxy_l, cyy_l, q1x, q1y, q2x, q2y, q3x, q3y};
-    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
+    emax_loc = stage_point<FORM, STAGE, MDIFF, BND>(a, nr, S);
   }
   if (STAGE == 4) {
 #pragma unroll
@@ -496,7 +631,7 @@ __device__ __forceinline__ Rec9 halo_rec(const double *hl, const double *hz, int
   return r;
 }
 
-template <int FORM, int STAGE, bool MDIFF>
+template <int FORM, int STAGE, bool MDIFF, bool BND>
 __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   extern __shared__ double hl[];
   __shared__ double red[BS / SV_WAVE];
@@ -565,7 +700,7 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   if (act) {
     const Sums S{rx, ry, ux, uy, vx, vy, lu, lv, axx, ayx, axy, ayy, bxy, byy, cxx_l, cxy_l,
                  cyy_l, q1x, q1y, q2x, q2y, q3x, q3y};
-    emax_loc = stage_point<FORM, STAGE, MDIFF>(a, nr, S);
+    emax_loc = stage_point<FORM, STAGE, MDIFF, BND>(a, nr, S);
   }
   if (STAGE == 4) {
 #pragma unroll
@@ -755,6 +890,17 @@ __global__ __launch_bounds__(1024) void k_reduce(const sv_segk *seg, const doubl
     ctrl->dro = s1[0] / volsum;
     ctrl->emax_np1 = fmax(s2[0], 1.0e-16);       // step.F90:480-485
     if (!(s2[0] == s2[0]) || !(s1[0] == s1[0])) ctrl->flags |= 1;
+    // the step's four applies of the boundary conditions ran at
+    // time0 + RKc[0..3]; the ramped inflow velocity follows the last one at
+    // time <= ramp_time (characteristic_boundaries.F90:268-283)
+    if (S.pk.u_control == 1)
+      for (int q = 0; q < 4; q++) {
+        const double t = ctrl->time0 + ctrl->RKc[q];
+        if (t <= S.pk.ramp_time) {
+          ctrl->uin_set = 1;
+          ctrl->uin_ramp = S.pk.u_start + (S.pk.u_end - S.pk.u_start) * t / S.pk.ramp_time;
+        }
+      }
     ctrl->time = ctrl->time0 + ctrl->RKc[3];
     ctrl->itime = ctrl->itime + 1;               // sunset.F90:99
   }
@@ -770,6 +916,8 @@ struct ApplyArgs {
   const double *s;
   const sv_ctrl *ctrl;   // ctrl[sg].dro is subtracted; nullptr for no correction
   double *pcfl, *pvisc;  // per-block minima for set_tstep
+  const int *bidx;       // boundary nodes (nullable), as StageArgs
+  const double *brec;
 };
 
 // rho -= drho (rhs_lc.F90:887-889); u = rou/rho (step.F90:514-529); set_tstep
@@ -786,6 +934,16 @@ __global__ __launch_bounds__(BS) void k_apply_state(ApplyArgs a) {
     double f[6];
     load_u(a.Uf, i, f);
     f[0] = f[0] - dro;
+    int bk = -1;
+    if (a.bidx) {
+      const int jb = a.bidx[i];
+      if (jb >= 0) {
+        const double *br = a.brec + (size_t)SV_BREC * jb;
+        bk = (int)br[0];
+        // the apply after filter_variables (step.F90:498)
+        if (a.ctrl) bound_apply(P, bk, uin_now(P, a.ctrl + nr.sg, br[3], 1), f);
+      }
+    }
     store_u(a.U, i, f);
     const double ooro = 1.0 / f[0];
     const double u = f[1] * ooro, v = f[2] * ooro;
@@ -794,7 +952,8 @@ __global__ __launch_bounds__(BS) void k_apply_state(ApplyArgs a) {
     G[SV_G_V] = v;
     const double um = sqrt(u * u + v * v + 0.0);
     const double si = a.s[i];
-    mc = si / (um + P.cs);
+    // slightly smaller on outflows (step.F90:591-595)
+    mc = bk == 2 ? 0.8 * si / (um + P.cs) : si / (um + P.cs);
     mv = si * si * f[0] / P.visc_total;
   }
   r1[threadIdx.x] = mc;
@@ -906,19 +1065,34 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
   a.lcol = c->L.lcol;
   a.hcap = c->L.hmax;
   a.partial = c->partial;
+  a.bidx = c->L.bidx;
+  a.brec = c->L.brec;
   return a;
 }
 
 template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
   const bool md = FORM == SV_FORM_DI && c->any_mdiff;
-  if (c->L.halo) {
-    const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
-    if (md) k_stage_h<FORM, STAGE, true><<<nb, BS, shm, c->stream>>>(a);
-    else k_stage_h<FORM, STAGE, false><<<nb, BS, shm, c->stream>>>(a);
+  if constexpr (FORM == SV_FORM_LC) {
+    // boundary nodes exist in the log-conformation form only
+    const bool bnd = a.bidx != nullptr;
+    if (c->L.halo) {
+      const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
+      if (bnd) k_stage_h<FORM, STAGE, false, true><<<nb, BS, shm, c->stream>>>(a);
+      else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
+    } else {
+      if (bnd) k_stage<FORM, STAGE, false, true><<<nb, BS, 0, c->stream>>>(a);
+      else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
+    }
   } else {
-    if (md) k_stage<FORM, STAGE, true><<<nb, BS, 0, c->stream>>>(a);
-    else k_stage<FORM, STAGE, false><<<nb, BS, 0, c->stream>>>(a);
+    if (c->L.halo) {
+      const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
+      if (md) k_stage_h<FORM, STAGE, true, false><<<nb, BS, shm, c->stream>>>(a);
+      else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
+    } else {
+      if (md) k_stage<FORM, STAGE, true, false><<<nb, BS, 0, c->stream>>>(a);
+      else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
+    }
   }
 }
 
@@ -982,6 +1156,8 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   ap.ctrl = c->ctrl;
   ap.pcfl = pc;
   ap.pvisc = pv;
+  ap.bidx = c->L.bidx;
+  ap.brec = c->L.brec;
   k_apply_state<<<nb, BS, 0, c->stream>>>(ap);
   mark();
   SV_HIP(c, hipGetLastError());
@@ -1004,6 +1180,8 @@ int sv_dev_init_partials(sv_ctx *c) {
   ap.ctrl = nullptr;
   ap.pcfl = c->partial + 2 * nb;
   ap.pvisc = c->partial + 3 * nb;
+  ap.bidx = c->L.bidx;
+  ap.brec = c->L.brec;
   k_apply_state<<<nb, BS, 0, c->stream>>>(ap);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));

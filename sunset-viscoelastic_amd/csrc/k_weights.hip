@@ -565,25 +565,51 @@ __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double 
   }
 }
 
-// filter_coefficients (labf.F90:1723-1811), interior nodes
+// filter_coefficients (labf.F90:1723-1811)
 __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt,
                                 const int *col, const double *x, const double *y,
-                                const double *h, double *wh, double *hsum) {
+                                const double *h, const double *s, const int *nt, int nb,
+                                double *wh, double *hsum) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npfb) return;
   const int64_t b = row_ptr[i];
   const int K = cnt[i];
-  const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
-  const double tmp = (2.0 / 3.0) * SV_PI / lscal;
-  double ls = 0.0;
+  const int t = nb > 0 ? nt[i] : 999;
+  double fc;
+  if (t >= 0 && t <= 2) {
+    // boundary nodes filter along the boundary only: radial test function,
+    // length scale s (:1770-1792)
+    const double tmp = (2.0 / 3.0) * SV_PI / s[i];
+    double ls = 0.0;
+    for (int k = 0; k < K; k++) {
+      const int j = col[b + k];
+      const double dx = x[j] - x[i], dy = y[j] - y[i];
+      const double f = 1.0 - cos(tmp * sqrt(dx * dx + dy * dy));
+      ls = ls + f * wh[b + k];
+    }
+    fc = (1.0 / 3.0) / ls;
+    fc = fc * 0.5;
+  } else {
+    const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
+    const double tmp = (2.0 / 3.0) * SV_PI / lscal;
+    double ls = 0.0;
 #pragma unroll 4
-  for (int k = 0; k < K; k++) {
-    const int j = col[b + k];
-    const double dx = x[j] - x[i], dy = y[j] - y[i];
-    const double f = 1.0 - cos(tmp * dx) * cos(tmp * dy);
-    ls = ls + f * wh[b + k];
+    for (int k = 0; k < K; k++) {
+      const int j = col[b + k];
+      const double dx = x[j] - x[i], dy = y[j] - y[i];
+      const double f = 1.0 - cos(tmp * dx) * cos(tmp * dy);
+      ls = ls + f * wh[b + k];
+    }
+    fc = (2.0 / 3.0) / ls;
+    // weaker filtering on the FD rows of walls (:1756-1763)
+    if (t < 0 && nt[i + t] == 0) {
+      const double oosqrt2 = 1.0 / sqrt(2.0);
+      if (t == -1) fc = fc * 0.5 * oosqrt2;
+      if (t == -2) fc = fc * 0.5;
+      if (t == -3) fc = fc * oosqrt2;
+      if (t == -4) fc = fc * oosqrt2;
+    }
   }
-  const double fc = (2.0 / 3.0) / ls;
   double hs = 0.0;
 #pragma unroll 8
   for (int k = 0; k < K; k++) {
@@ -664,6 +690,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   const int n = c->npfb;
   std::vector<int> cnt(n), cs(n);
   std::vector<double> h(n), hs(n);
+  const std::vector<int> &nt = c->hnt;
   SV_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
   SV_HIP(c, hipMemcpy(cs.data(), c->cnt_small, sizeof(int) * n, hipMemcpyDeviceToHost));
   SV_HIP(c, hipMemcpy(h.data(), c->h, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -678,6 +705,14 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     for (int m : {md, mf})
       if (m != 4 && m != 6 && m != 8)
         return sv_fail(c, SV_E_ARG, "LABFM order %d not supported on the device (4, 6, 8)", m);
+    // boundary nodes are not on internal_list (labf.F90:82, :268): their
+    // operators come from calc_boundary_weights, which also replaces the
+    // derivative weights of FD rows 1 and 2 (only their filter system is kept)
+    if (c->nb > 0 && nt[i] >= 0 && nt[i] <= 2) continue;
+    if (c->nb > 0 && (nt[i] == -1 || nt[i] == -2)) {
+      sys.push_back({mf, cnt[i], i, SYS_F});
+      continue;
+    }
     if (md == mf && h[i] == hs[i] && cnt[i] == cs[i]) {
       sys.push_back({md, cnt[i], i, SYS_D | SYS_F});
     } else {
@@ -916,7 +951,8 @@ int sv_dev_sums(sv_ctx *c) {
 int sv_dev_filter_coeffs(sv_ctx *c) {
   const int n = c->npfb, B = 256;
   k_filter_coeffs<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->col_raw, c->x,
-                                                        c->y, c->h, c->wh, c->hsum);
+                                                        c->y, c->h, c->s, c->node_type, c->nb,
+                                                        c->wh, c->hsum);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
   c->have_filter = true;

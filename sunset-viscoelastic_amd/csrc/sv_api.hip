@@ -63,6 +63,9 @@ extern "C" {
 
 const char *sv_version(void) { return "sunset_amd 0.1 (gfx950)"; }
 
+static void derive_phys(sv_ctx *c);
+static int upload_segment(sv_ctx *c);
+
 int sv_create(const sv_config *cfg, sv_ctx **out) {
   if (!cfg || !out) return SV_E_ARG;
   *out = nullptr;
@@ -154,6 +157,29 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   c->graph_valid = false;
   c->have_lists = c->have_weights = c->have_sums = c->have_filter = false;
   c->have_state = false;
+  c->blist = nullptr; c->rnx = c->rny = nullptr; c->ooR = nullptr;
+  c->have_normals = c->have_bweights = false;
+  // boundary nodes and their FD rows (setup_domain.F90:179-193, :314-326)
+  c->hnt.assign(npfb, 999);
+  if (node_type) c->hnt.assign(node_type, node_type + npfb);
+  c->hblist.clear();
+  for (int i = 0; i < npfb; i++) {
+    const int t = c->hnt[i];
+    if (t >= 0 && t <= 2) {
+      SV_CHECK(c, i + 4 < npfb, "boundary node %d: its four FD rows must follow it", i);
+      for (int j = 1; j <= 4; j++)
+        SV_CHECK(c, c->hnt[i + j] == -j,
+                 "boundary node %d: node %d must be its FD row %d (node type %d)", i, i + j, j,
+                 -j);
+      c->hblist.push_back(i);
+    } else if (t < 0) {
+      SV_CHECK(c, t >= -4 && i + t >= 0 && c->hnt[i + t] >= 0 && c->hnt[i + t] <= 2,
+               "node %d: FD row type %d without its boundary node", i, t);
+    } else {
+      SV_CHECK(c, t == 999 || t == 998, "node %d: unknown node type %d", i, t);
+    }
+  }
+  c->nb = (int)c->hblist.size();
   c->npfb = npfb;
   c->cap = 9 * npfb + 64;      // at most 8 ghosts per node
   const size_t D = sizeof(double) * c->cap, I = sizeof(int) * c->cap;
@@ -181,7 +207,54 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   }
   c->hmax = hm;
   c->hmin = hn;
+  if (c->nb > 0) {
+    if (sv_alloc(c, (void **)&c->blist, sizeof(int) * c->nb)) return SV_E_NOMEM;
+    SV_HIP(c, hipMemcpy(c->blist, c->hblist.data(), sizeof(int) * c->nb,
+                        hipMemcpyHostToDevice));
+  }
   return sv_dev_create_ghosts(c);
+}
+
+int sv_set_boundaries(sv_ctx *c, const double *rnx, const double *rny) {
+  SV_CHECK(c, c && rnx && rny, "null argument");
+  SV_CHECK(c, c->npfb > 0, "sv_set_nodes must come first");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  if (int rc = set_device(c)) return rc;
+  return sv_dev_boundary_normals(c, rnx, rny);
+}
+
+int sv_set_inflow(sv_ctx *c, int32_t inflow_type, int32_t velocity_control, double u_start,
+                  double u_end, double ramp_time) {
+  SV_CHECK(c, c, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  SV_CHECK(c, inflow_type >= 0 && inflow_type <= 2, "inflow type %d: 0 soft, 1 hard, 2 "
+           "inflow-outflow", inflow_type);
+  SV_CHECK(c, velocity_control == 0 || velocity_control == 1,
+           "inflow velocity control %d: 0 fixed, 1 ramp", velocity_control);
+  SV_CHECK(c, velocity_control == 0 || ramp_time > 0, "ramp_time must be positive");
+  c->inflow_type = inflow_type;
+  c->u_control = velocity_control;
+  c->u_start = u_start;
+  c->u_end = u_end;
+  c->ramp_time = ramp_time;
+  if (c->have_phys) {
+    derive_phys(c);
+    c->graph_valid = false;
+    if (c->have_state) {
+      if (int rc = set_device(c)) return rc;
+      return upload_segment(c);
+    }
+  }
+  return 0;
+}
+
+int sv_calc_boundary_weights(sv_ctx *c) {
+  SV_CHECK(c, c && c->have_weights, "sv_calc_labf_weights must come first");
+  SV_CHECK(c, !c->have_sums, "boundary weights come before sv_calc_labf_sums "
+           "(sunset.F90:54-55)");
+  SV_CHECK(c, c->nb == 0 || c->have_normals, "sv_set_boundaries must come first");
+  if (int rc = set_device(c)) return rc;
+  return sv_dev_boundary_weights(c);
 }
 
 int sv_get_ghosts(sv_ctx *c, int32_t *np, double *x, double *y, int32_t *irelation,
@@ -236,6 +309,7 @@ int sv_find_neighbours(sv_ctx *c, int32_t nplink, int32_t *ij_count, int32_t *ij
 int sv_calc_labf_weights(sv_ctx *c, const int32_t *order_map) {
   SV_CHECK(c, c && c->have_lists, "sv_find_neighbours must come first");
   if (int rc = set_device(c)) return rc;
+  c->have_bweights = false;
   if (int rc = sv_dev_weights(c, order_map)) return rc;
   if (c->n_singular > 0)
     return sv_fail(c, SV_E_SINGULAR, "%d moment systems had a zero pivot", c->n_singular);
@@ -244,6 +318,8 @@ int sv_calc_labf_weights(sv_ctx *c, const int32_t *order_map) {
 
 int sv_calc_labf_sums(sv_ctx *c) {
   SV_CHECK(c, c && c->have_weights, "sv_calc_labf_weights must come first");
+  SV_CHECK(c, c->nb == 0 || c->have_bweights,
+           "%d boundary nodes: sv_calc_boundary_weights must come first", c->nb);
   if (int rc = set_device(c)) return rc;
   return sv_dev_sums(c);
 }
@@ -442,6 +518,13 @@ static void derive_phys(sv_ctx *c) {
   k.U_char = p.U_char;
   k.L_char = p.L_char;
   k.err_norm_rou = 1.0 / (1.0 * p.U_char);
+  k.inflow_type = c->inflow_type;
+  k.u_control = c->u_control;
+  k.u_start = c->u_start;
+  k.u_end = c->u_end;
+  k.ramp_time = c->ramp_time;
+  k.Lx = (c->xmax - c->xmin) * p.L_char;
+  k.p_ref = k.rho_char * k.csq;
 }
 
 static int ensure_state_buffers(sv_ctx *c) {
@@ -539,6 +622,10 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
   SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
+  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC,
+           "boundary nodes: the time loop supports them in the log-conformation form only "
+           "(the reference's direct form leaves inflow/outflow right-hand sides unset, "
+           "rhs_di.F90:579-590)");
   if (int rc = set_device(c)) return rc;
   if (int rc = sv_ensure_loop(c)) return rc;
   if (int rc = ensure_state_buffers(c)) return rc;
@@ -565,6 +652,36 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
     r[0] = a[i]; r[1] = b[i]; r[2] = d[i]; r[3] = e[i]; r[4] = f[i]; r[5] = g[i];
   }
   SV_HIP(c, hipMemcpy(c->U, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+  if (c->nb > 0) {
+    // boundary records and the loop slot -> record map; u_inflow_local starts
+    // at U_char, on inflow nodes at the initial u (setup_flow.F90:48-51, :130-137)
+    std::vector<double> nx(n), ny(n);
+    SV_HIP(c, hipMemcpy(nx.data(), c->rnx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    SV_HIP(c, hipMemcpy(ny.data(), c->rny, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<double> br((size_t)SV_BREC * c->nb);
+    std::vector<int> jbof(n, -1), bidx(n, -1);
+    for (int jb = 0; jb < c->nb; jb++) {
+      const int i = c->hblist[jb];
+      jbof[i] = jb;
+      double *r = br.data() + (size_t)SV_BREC * jb;
+      r[0] = c->hnt[i];
+      r[1] = nx[i];
+      r[2] = ny[i];
+      r[3] = c->hnt[i] == 1 ? u[i] : c->phys.U_char;
+    }
+    for (int k = 0; k < n; k++) bidx[k] = jbof[P[k]];
+    sv_free(c, c->L.bidx);
+    sv_free(c, c->L.brec);
+    c->L.bidx = nullptr;
+    c->L.brec = nullptr;
+    if (sv_alloc(c, (void **)&c->L.bidx, sizeof(int) * c->ntot) ||
+        sv_alloc(c, (void **)&c->L.brec, sizeof(double) * br.size()))
+      return SV_E_NOMEM;
+    SV_HIP(c, hipMemset(c->L.bidx, 0xff, sizeof(int) * c->ntot));
+    SV_HIP(c, hipMemcpy(c->L.bidx, bidx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    SV_HIP(c, hipMemcpy(c->L.brec, br.data(), sizeof(double) * br.size(),
+                        hipMemcpyHostToDevice));
+  }
   if (int rc = upload_segment(c)) return rc;
   sv_ctrl h{};
   h.dt = 1.0e-6;                         // setup_flow.F90:149-151

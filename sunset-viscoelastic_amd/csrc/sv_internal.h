@@ -55,6 +55,12 @@ struct sv_ctrl {
   double dro;
   int itime, dt_mode;
   int flags;   // bit 0: non-finite value met in a reduction
+  // ramped inflow velocity (update_u_inflow, characteristic_boundaries.F90:
+  // 256-289): the value the last apply_time_dependent_bounds left, once one
+  // has run at time <= ramp_time (uin_set); before that each inflow node
+  // keeps its initial velocity
+  double uin_ramp;
+  int uin_set;
 };
 
 // Physical constants passed by value to the stage kernels
@@ -62,7 +68,21 @@ struct sv_physk {
   double csq, cs, visc_total, coef_solvent, coef_polymeric, lambda, eps, Mdiff;
   double bfx, bfy, kol_amp, kol_k, rho_char, U_char, L_char;
   double err_norm_rou;   // 1/U_char (setup_flow.F90:156)
+  // boundaries: characteristic conditions (characteristic_boundaries.F90),
+  // inflow_type 0 soft, 1 hard, 2 inflow-outflow; u_control 1 ramps the
+  // inflow velocity u_start -> u_end over ramp_time (load_data.F90:152)
+  int inflow_type, u_control;
+  double u_start, u_end, ramp_time;
+  double Lx;             // L_domain_x (setup_domain.F90:98)
+  double p_ref;          // p_inflow = p_outflow = rho_char*csq (load_data.F90:147-148)
 };
+
+// Boundary nodes (node types 0 wall, 1 inflow, 2 outflow), each followed in
+// node order by its four FD rows (types -1..-4) along the unit normal
+// (setup_domain.F90:179-193).  Per boundary node jb the time loop reads one
+// record of SV_BREC doubles: kind (0 wall, 1 inflow, 2 outflow), nx, ny and
+// the initial velocity u of the node (u_inflow_local, setup_flow.F90:130-137).
+#define SV_BREC 4
 
 // State records (real nodes only; ghosts are resolved through the folded
 // stencil columns).  Gather record of node j, what a neighbour sweep reads:
@@ -133,6 +153,10 @@ struct sv_loopl {
   int64_t *hoff = nullptr;  // [nblocks + 1]
   int *hidx = nullptr;
   uint16_t *lcol = nullptr; // [nnz]
+  // boundary nodes: per loop slot the boundary record index or -1, and the
+  // records [nb][SV_BREC]; nullptr when the node set has no boundary nodes
+  int *bidx = nullptr;
+  double *brec = nullptr;
 };
 
 struct sv_ctx {
@@ -149,6 +173,17 @@ struct sv_ctx {
   double *x = nullptr, *y = nullptr, *h = nullptr, *hs = nullptr, *s = nullptr;
   int *node_type = nullptr, *irel = nullptr, *vrel = nullptr;
   double hmax = 0, hmin = 0;
+  // boundary nodes (types 0..2) in node order, node types on the host, unit
+  // normals [cap] (FD rows and ghosts carry their parent's)
+  int nb = 0;
+  std::vector<int> hblist, hnt;
+  int *blist = nullptr;
+  double *rnx = nullptr, *rny = nullptr;
+  double *ooR = nullptr;     // [nb] wall curvature 1/R (calc_boundary_weights)
+  bool have_normals = false, have_bweights = false;
+  // inflow settings (sv_set_inflow)
+  int inflow_type = 0, u_control = 0;
+  double u_start = 0.0, u_end = 1.0, ramp_time = 1.0;
   // neighbours
   bool have_lists = false;
   int nplink = 0;
@@ -268,6 +303,9 @@ int sv_ensure_loop(sv_ctx *c);
 int sv_dev_weights(sv_ctx *c, const int *order_map_host);
 int sv_dev_sums(sv_ctx *c);
 int sv_dev_filter_coeffs(sv_ctx *c);
+// ---- kernels: boundary nodes (k_bound.hip) ----
+int sv_dev_boundary_normals(sv_ctx *c, const double *rnx_host, const double *rny_host);
+int sv_dev_boundary_weights(sv_ctx *c);
 // ---- kernels: stencil operators (k_stencil.hip) ----
 int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_cols,
                  hipStream_t st);

@@ -56,7 +56,7 @@ EXPORTS = [
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
-    "sv_diagnostics",
+    "sv_diagnostics", "sv_set_boundaries", "sv_set_inflow", "sv_calc_boundary_weights",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -113,6 +113,10 @@ def lib(path: Optional[str] = None):
     L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
     L.sv_adj_param_grad.argtypes = [C.c_void_p, PD]
     L.sv_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, PD]
+    L.sv_set_boundaries.argtypes = [C.c_void_p, PD, PD]
+    L.sv_set_inflow.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                C.c_double]
+    L.sv_calc_boundary_weights.argtypes = [C.c_void_p]
     L.sv_adj_timing.argtypes = [C.c_void_p, PD, PD]
     L.sv_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.sv_ensemble_size.argtypes = [C.c_void_p]
@@ -179,6 +183,10 @@ class Solver:
         nt = np.ascontiguousarray(ns.node_type, dtype=np.int32)
         self._ck(lib().sv_set_nodes(self.h, ns.npfb, *[_d(a) for a in self._arr], _i(nt)))
         self.npfb = ns.npfb
+        self.nb = int(np.count_nonzero((nt >= 0) & (nt <= 2)))
+        if getattr(ns, "rnx", None) is not None:
+            self._rn = [_f64(ns.rnx), _f64(ns.rny)]
+            self._ck(lib().sv_set_boundaries(self.h, _d(self._rn[0]), _d(self._rn[1])))
         npv = C.c_int32()
         self._ck(lib().sv_get_ghosts(self.h, C.byref(npv), None, None, None, None, 0))
         self.np = npv.value
@@ -224,6 +232,10 @@ class Solver:
         om = None if order_map is None else np.ascontiguousarray(order_map, dtype=np.int32)
         self._ck(lib().sv_calc_labf_weights(self.h, _i(om)))
 
+    def calc_boundary_weights(self):
+        """calc_boundary_weights (labf.F90:505-1039); a no-op without boundary nodes."""
+        self._ck(lib().sv_calc_boundary_weights(self.h))
+
     def calc_sums(self):
         self._ck(lib().sv_calc_labf_sums(self.h))
 
@@ -234,9 +246,15 @@ class Solver:
         self.set_nodes(ns)
         self.find_neighbours(copy=False)
         self.calc_weights(order_map)
+        self.calc_boundary_weights()
         self.calc_sums()
         self.filter_coefficients()
         return self
+
+    def set_inflow(self, inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0):
+        """The inflow line of control.in (load_data.F90:152): type 0 soft,
+        1 hard, 2 inflow-outflow; u_control 1 ramps the inflow velocity."""
+        self._ck(lib().sv_set_inflow(self.h, inflow_type, u_control, u_start, u_end, ramp_time))
 
     def timing(self, what: str) -> float:
         v = C.c_double()
