@@ -4,9 +4,11 @@
 !! calc_node_volumes :453-503), calc_labf_sums (:396-451) and
 !! filter_coefficients (:1723-1811) run on the device and leave the same arrays
 !! in common_vars as the reference does after each call (ij_w_grad, ij_w_lap,
-!! ij_w_hyp, their sums, vol).  adapt_stencils and calc_boundary_weights have no
-!! device implementation: they stop with a message (run with -Drestart and
-!! supplied h, h_small; boundary nodes nb > 0 are not supported).
+!! ij_w_hyp, their sums, vol).  calc_boundary_weights (:505-1039) runs on the
+!! device too and prunes the host lists of boundary nodes as the reference does
+!! (ij_wb_grad2, used only by the compressible rhs.F90, is not kept).
+!! adapt_stencils has no device implementation: it stops with a message (run
+!! with -Drestart and supplied h, h_small).
 module labf
   use iso_c_binding
   use kind_parameters
@@ -75,9 +77,30 @@ contains
   end subroutine adapt_stencils
 
   subroutine calc_boundary_weights
-    if (nb /= 0) then
-       write(6,*) "sunset_amd: boundary weights (nb>0) have no device implementation"
-       stop
-    end if
+    integer(ikind) :: jj, i, k, j, n, t
+    integer(ikind), allocatable :: keep(:)
+    call sv_check(sv_calc_boundary_weights(sv_ctx), "sv_calc_boundary_weights")
+    !! the device pruned the lists of boundary nodes to their same-type
+    !! neighbours and FD rows (labf.F90:541-566); the host lists follow it
+    allocate(keep(nplink))
+    do jj = 1, nb
+       i = boundary_list(jj)
+       t = node_type(i)
+       n = 0
+       do k = 1, ij_count(i)
+          j = ij_link(k, i)
+          if (node_type(j) == t .or. (j <= npfb .and. node_type(j) < 0 .and. &
+              j + node_type(j) == i)) then
+             n = n + 1
+             keep(n) = j
+          end if
+       end do
+       ij_link(:, i) = 0
+       ij_link(1:n, i) = keep(1:n)
+       ij_count(i) = n
+       ij_count_small(i) = n
+    end do
+    call fetch(.false.)
+    write(6,*) "finished boundary weights"
   end subroutine calc_boundary_weights
 end module labf

@@ -168,6 +168,23 @@ module sv_capi
        integer(c_int32_t), value :: kind, member    ! kind: 0 mass/energy, 1 conf, 2 Kolmogorov, 3 TG
        real(c_double), intent(out) :: out(*)
      end function sv_diagnostics
+     integer(c_int) function sv_set_boundaries(ctx, rnx, rny) bind(C, name='sv_set_boundaries')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rnx(*), rny(*)     ! rnorm(1:npfb,1), rnorm(1:npfb,2)
+     end function sv_set_boundaries
+     integer(c_int) function sv_set_inflow(ctx, inflow_type, velocity_control, u_start, u_end, &
+          ramp_time) bind(C, name='sv_set_inflow')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: inflow_type, velocity_control
+       real(c_double), value :: u_start, u_end, ramp_time
+     end function sv_set_inflow
+     integer(c_int) function sv_calc_boundary_weights(ctx) &
+          bind(C, name='sv_calc_boundary_weights')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function sv_calc_boundary_weights
      integer(c_int) function sv_adj_param_grad(ctx, grad) bind(C, name='sv_adj_param_grad')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
@@ -233,15 +250,13 @@ contains
           stop
        end if
     end if
-    if (nb /= 0) then
-       write(6,*) "sunset_amd: boundary nodes (nb>0) are not supported by the device library"
-       stop
-    end if
     bc = (/ xbcond_L, xbcond_U, ybcond_L, ybcond_U /)
     call sv_check(sv_set_domain(sv_ctx, xmin, xmax, ymin, ymax, bc, smax, L_char), &
                   "sv_set_domain")
     call sv_check(sv_set_nodes(sv_ctx, npfb, rp(1:npfb,1), rp(1:npfb,2), s(1:npfb), &
                   h(1:npfb), h_small(1:npfb), node_type(1:npfb)), "sv_set_nodes")
+    if (nb /= 0) call sv_check(sv_set_boundaries(sv_ctx, rnorm(1:npfb,1), rnorm(1:npfb,2)), &
+                               "sv_set_boundaries")
     sv_ctx_npfb = npfb
   end subroutine sv_upload_nodes
 

@@ -14,6 +14,10 @@ Fixtures (numpy .npz, no pickles):
                        derivative operators of the initial density, and the
                        fields after `nsteps` RK3 steps of the log-conformation
                        solver
+  ref_b<name>.npz      the same on node sets with boundary nodes (walls, or an
+                       inflow and an outflow), with the node arrays and normals:
+                       the reference's calc_boundary_weights and characteristic
+                       boundary conditions (only the fields and the CSR weights)
 """
 import os
 import sys
@@ -25,7 +29,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "sunset-viscoelastic_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from sunset_amd.nodes import jittered_lattice, read_ipart, taylor_green_ic  # noqa: E402
+from sunset_amd.nodes import (inflow_outflow_channel, jittered_lattice, read_ipart,  # noqa: E402
+                              taylor_green_ic, wall_channel)
 import refdrv  # noqa: E402
 
 CASES = [
@@ -37,6 +42,33 @@ CASES = [
     ("m4_h18", 4, 20, 1.8, 303, 100, dict(Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1)),
     ("m6_20", 6, 20, 2.4, 202, 20, dict(Re=1.0, Wi=2.0, beta=0.1, eps=0.25, Ma=0.1)),
 ]
+
+
+# name, node set, order, nsteps, physics, driving force, inflow settings
+BPHYS = dict(Re=10.0, Wi=0.5, beta=0.5, eps=0.1, Ma=0.1)
+BCASES = [
+    ("wall_m6", "wall", 6, 40, BPHYS, (8.0, 0.0), {}),
+    ("io_m6_soft", "io", 6, 40, BPHYS, (0.0, 0.0), dict(inflow_type=0)),
+    ("io_m4_ramp", "io", 4, 40, BPHYS, (0.0, 0.0),
+     dict(inflow_type=1, u_control=1, u_start=0.5, u_end=1.0, ramp_time=1e-4)),
+]
+
+
+def boundary_case(kind, order):
+    """The seeded channel of a boundary case and its initial state."""
+    if kind == "wall":
+        ns = wall_channel(20, 20, order=order, seed=1)
+        n = ns.npfb
+        ic = dict(ro=np.ones(n), u=4 * ns.y * (1 - ns.y),
+                  v=0.01 * np.sin(2 * np.pi * ns.x) * np.sin(np.pi * ns.y) ** 2,
+                  cxx=np.ones(n), cxy=np.zeros(n), cyy=np.ones(n))
+    else:
+        ns = inflow_outflow_channel(20, 20, order=order, seed=2)
+        n = ns.npfb
+        ic = dict(ro=np.ones(n), u=1.0 + 0.01 * np.sin(2 * np.pi * ns.y),
+                  v=0.01 * np.cos(2 * np.pi * ns.y) * np.sin(np.pi * ns.x),
+                  cxx=np.ones(n), cxy=np.zeros(n), cyy=np.ones(n))
+    return ns, ic
 
 
 def to_csr(out, n):
@@ -85,6 +117,25 @@ def main():
             lap_ro=out["lap_ro"], filt_ro=out["filt_ro"],
             **{f"f_{k}": out[k] for k in ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy",
                                          "cxx", "cxy", "cyy")},
+            ctl=np.array([out[k] for k in ("dt", "time", "emax_np1", "emax_n", "emax_nm1",
+                                          "dt_cfl", "dt_par")]),
+            itime=out["itime"])
+        print("wrote", name, ns.npfb, "nodes", int(ptr[-1]), "links")
+    for name, kind, order, nsteps, phys, drive, inflow in BCASES:
+        ns, ic = boundary_case(kind, order)
+        out, _ = refdrv.run_reference(ns, order, ic, nsteps=nsteps,
+                                      phys=dict(phys, drive=(*drive, 0.0)), inflow=inflow)
+        ptr, col, wx, wy, wl, wh = to_csr(out, ns.npfb)
+        inf = dict(inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0) | inflow
+        np.savez_compressed(
+            os.path.join(HERE, f"ref_b{name}.npz"),
+            kind=kind, order=order, nsteps=nsteps,
+            phys=np.array([phys["Re"], phys["Wi"], phys["beta"], phys["eps"], phys["Ma"]]),
+            drive=np.array(drive), inflow_i=np.array([inf["inflow_type"], inf["u_control"]]),
+            inflow_r=np.array([inf["u_start"], inf["u_end"], inf["ramp_time"]]),
+            x=ns.x, y=ns.y, node_type=ns.node_type, rnx=ns.rnx, rny=ns.rny,
+            ij_count=out["ij_count"], ptr=ptr, col=col, wx=wx, wy=wy, wl=wl, wh=wh,
+            **{f"f_{k}": out[k] for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy")},
             ctl=np.array([out[k] for k in ("dt", "time", "emax_np1", "emax_n", "emax_nm1",
                                           "dt_cfl", "dt_par")]),
             itime=out["itime"])

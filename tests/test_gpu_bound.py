@@ -13,7 +13,11 @@ cond ~1e9 (1e-6); fields after 20 steps to 1e-9 of max(|field|, 1e-2)."""
 import numpy as np
 import pytest
 
+import os
+
 import oracle_lib as OL
+import refdrv
+from parity_util import FIELD_RTOL, field_rel, golden_bcase
 from sunset_amd.nodes import inflow_outflow_channel, wall_channel
 
 pytestmark = pytest.mark.gpu
@@ -224,3 +228,55 @@ def test_boundary_errors():
     with pytest.raises(sv.SunsetError):
         sv.Ensemble([s])
     s.close()
+
+
+@pytest.mark.parametrize("name", ["wall_m6", "io_m6_soft", "io_m4_ramp"])
+def test_boundary_device_vs_golden(name):
+    """The committed vectors of the compiled reference: weights (CSR, canonical
+    order) and the fields after 40 RK3 steps, PID time steps.  psi tolerance
+    as in test_oracle_bound.test_boundary_oracle_vs_golden (near-isotropic
+    outflow nodes)."""
+    sv = _sv()
+    g, ns, ic, phys, drive, inflow = golden_bcase(name)
+    order = int(g["order"])
+    print(f"[bound] golden {name}", flush=True)
+    s = sv.Solver(morder=order, form="lc").setup(ns)
+    W = s.weights(2 * int(g["ij_count"].max()))
+    ptr = g["ptr"]
+    for i in range(ns.npfb):
+        K = int(g["ij_count"][i])
+        sl = slice(ptr[i], ptr[i + 1])
+        for a, gk in ((W["w_grad"][i, :K, 0], "wx"), (W["w_grad"][i, :K, 1], "wy"),
+                      (W["w_lap"][i, :K], "wl"), (W["w_hyp"][i, :K], "wh")):
+            b = g[gk][sl]
+            assert np.abs(a - b).max() <= 1e-9 * max(np.abs(b).max(), 1e-300), (i, gk)
+    s.set_inflow(**inflow)
+    s.set_phys(drive=(*drive, 0.0), **phys)
+    s.set_initial(ic)
+    s.run(int(g["nsteps"]), graph=True)
+    F = s.fields()
+    s.close()
+    assert F["dt"] == pytest.approx(float(g["ctl"][0]), rel=1e-12)
+    for k in ("ro", "rou", "rov"):
+        assert field_rel(F[k], g[f"f_{k}"]) < FIELD_RTOL, k
+    for k in ("pxx", "pxy", "pyy"):
+        assert field_rel(F[k], g[f"f_{k}"]) < 1e-8, k
+
+
+@pytest.mark.parametrize("name", ["wall_m6", "io_m6_soft", "io_m4_ramp"])
+def test_boundary_dropin_vs_golden(name):
+    """The reference's own time loop with this library's Fortran drop-in
+    neighbours / labf (calc_boundary_weights included) / derivatives modules."""
+    g, ns, ic, phys, drive, inflow = golden_bcase(name)
+    order = int(g["order"])
+    binary = refdrv.dropin_binary(order)
+    if not os.path.exists(binary):
+        pytest.skip("drop-in binary not built (make -C oracle dropin)")
+    print(f"[bound] drop-in {name}", flush=True)
+    out, _ = refdrv.run_reference(ns, order, ic, nsteps=int(g["nsteps"]),
+                                  phys=dict(phys, drive=(*drive, 0.0)), inflow=inflow,
+                                  binary=binary, timeout=300, fields_only=True)
+    for k in ("ro", "rou", "rov"):
+        assert field_rel(out[k], g[f"f_{k}"]) < FIELD_RTOL, k
+    for k in ("pxx", "pxy", "pyy"):
+        assert field_rel(out[k], g[f"f_{k}"]) < 1e-8, k

@@ -13,6 +13,7 @@ import pytest
 
 import oracle_lib as OL
 import refdrv
+from parity_util import FIELD_RTOL, field_rel, golden_bcase
 from sunset_amd.nodes import inflow_outflow_channel, wall_channel
 
 PH = dict(Re=10.0, Wi=0.5, beta=0.5, eps=0.1, Ma=0.1)
@@ -133,3 +134,40 @@ def test_boundary_operators_closed_forms():
     wy = 4 * (ns.y[rows] - 0.2) ** 3
     assert np.abs(gx[rows] - wx).max() < 1e-7 * max(1.0, np.abs(wx).max())
     assert np.abs(gy[rows] - wy).max() < 1e-9 * max(1.0, np.abs(wy).max())
+
+
+@pytest.mark.parametrize("name", ["wall_m6", "io_m6_soft", "io_m4_ramp"])
+def test_boundary_oracle_vs_golden(name):
+    """The committed reference vectors (tests/golden/make_golden.py): CSR weights
+    and the fields after 40 RK3 steps with PID time steps."""
+    g, ns, ic, phys, drive, inflow = golden_bcase(name)
+    order = int(g["order"])
+    o = OL.OracleSim(ns, order).setup()
+    cnt, _, link = o.lists()
+    assert np.array_equal(cnt, g["ij_count"])
+    W = o.weight_arrays()
+    ptr = g["ptr"]
+    for i in range(ns.npfb):
+        s_ = slice(ptr[i], ptr[i + 1])
+        assert np.array_equal(link[i, :cnt[i]], g["col"][s_]), i
+        for key, comp, gk in (("w_grad", 0, "wx"), ("w_grad", 1, "wy"), ("w_lap", None, "wl"),
+                              ("w_hyp", None, "wh")):
+            a = W[key][i, :cnt[i]] if comp is None else W[key][i, :cnt[i], comp]
+            b = g[gk][s_]
+            assert np.abs(a - b).max() <= 1e-9 * max(np.abs(b).max(), 1e-300), (i, gk)
+    o.make_phys(form="lc", drive=drive, **phys, **inflow)
+    o.set_initial(ic)
+    o.run(int(g["nsteps"]))
+    F = o.fields()
+    assert F["dt"] == pytest.approx(float(g["ctl"][0]), rel=1e-12)
+    assert F["itime"] == int(g["itime"])
+    # scale max(|f|, 1) as the periodic golden tests.  psi = log c is nearly
+    # isotropic (|psi| ~ 1e-7) on the outflow nodes of the inflow/outflow
+    # cases, where the reference's 2x2 eigenvectors (conf_transforms.F90:
+    # 399-444, direction from (lambda1 - d, b) with |b| ~ 1e-9) amplify
+    # rounding: two FP64 evaluations drift apart by ~2e-9 over 40 steps there,
+    # while rho and the momentum agree to 1e-13
+    for k in ("ro", "rou", "rov"):
+        assert field_rel(F[k], g[f"f_{k}"]) < FIELD_RTOL, k
+    for k in ("pxx", "pxy", "pyy"):
+        assert field_rel(F[k], g[f"f_{k}"]) < 1e-8, k
