@@ -87,25 +87,33 @@ def jittered_lattice(nx: int, ny: Optional[int] = None, *, seed: int = 0,
 
 
 def write_ipart(path: str, ns: NodeSet, nprocs_schedule: bool = False) -> None:
-    """Write the IPART text file (datclass.F90:734-745 layout)."""
-    nb = int(np.count_nonzero((ns.node_type >= 0) & (ns.node_type <= 2)))
+    """Write the IPART text file (datclass.F90:734-745 layout).  FD rows
+    (types -1..-4) are not written: the reader generates them from their
+    boundary node's position, normal and spacing (setup_domain.F90:179-193)."""
+    nb = ns.nb
+    keep = np.where(ns.node_type >= 0)[0]
+    nx = ns.rnx if ns.rnx is not None else np.zeros(ns.npfb)
+    ny = ns.rny if ns.rny is not None else np.zeros(ns.npfb)
     with open(path, "w") as f:
-        f.write(f"{nb} {ns.npfb} {ns.smax!r} {float(ns.s.min())!r}\n")
-        f.write(f"{ns.xmin!r} {ns.xmax!r} {ns.ymin!r} {ns.ymax!r}\n")
+        f.write(f"{nb} {keep.shape[0]} {float(ns.smax)!r} {float(ns.s.min())!r}\n")
+        f.write(f"{float(ns.xmin)!r} {float(ns.xmax)!r} {float(ns.ymin)!r} "
+                f"{float(ns.ymax)!r}\n")
         f.write(" ".join(str(int(b)) for b in ns.bc) + "\n")
         if nprocs_schedule:
             f.write("1 1 1\n1\n")
-            f.write(f"1 {ns.npfb}\n")
-        for i in range(ns.npfb):
-            f.write(f"{i + 1} {ns.x[i]!r} {ns.y[i]!r} {int(ns.node_type[i])} "
-                    f"0.0 0.0 {ns.s[i]!r}\n")
+            f.write(f"1 {keep.shape[0]}\n")
+        for g, i in enumerate(keep):
+            f.write(f"{g + 1} {float(ns.x[i])!r} {float(ns.y[i])!r} {int(ns.node_type[i])} "
+                    f"{float(nx[i])!r} {float(ny[i])!r} {float(ns.s[i])!r}\n")
 
 
 def read_ipart(path: str, order: int = 8) -> NodeSet:
     """Read an IPART file (with or without a decomposition schedule).
 
     h = hovs*s as setup_domain.F90:180 (hovs for the given order), h_small = h.
-    Boundary nodes (types 0-2) are read but their FD rows are not generated.
+    A boundary node (type 0 wall, 1 inflow, 2 outflow) gets h = hovs_bound*s
+    and is followed by its four FD rows at s, 2s, 3s, 4s along its normal
+    (types -1..-4, same normal, h and s), as setup_domain.F90:179-193.
     """
     with open(path) as f:
         lines = [ln.split() for ln in f if ln.strip()]
@@ -119,12 +127,27 @@ def read_ipart(path: str, order: int = 8) -> NodeSet:
         nprocs = int(lines[k + 1][0])
         k += 2 + nprocs
     rows = lines[k:k + npfb]
-    arr = np.array([[float(v) for v in r[:7]] for r in rows])
-    x, y, typ, s = arr[:, 1], arr[:, 2], arr[:, 3].astype(np.int32), arr[:, 6]
-    h = s * HOVS.get(order, 2.7)
-    return NodeSet(x=x, y=y, s=s, h=h, h_small=h.copy(), node_type=typ,
-                   xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, bc=bc, smax=smax,
-                   meta=dict(source=path, nb=nb))
+    arr = np.array([[float(v.replace("D", "E").replace("d", "e")) for v in r[:7]]
+                    for r in rows])
+    X, Y, T, S, H, NX, NY = [], [], [], [], [], [], []
+    for r in arr:
+        t = int(r[3])
+        X.append(r[1]); Y.append(r[2]); T.append(t); S.append(r[6]); NX.append(r[4]); NY.append(r[5])
+        if 0 <= t <= 2:
+            hb = r[6] * HOVS_BOUND.get(order, 2.4)
+            H.append(hb)
+            for j in range(1, 5):
+                X.append(r[1] + r[4] * float(j) * r[6]); Y.append(r[2] + r[5] * float(j) * r[6])
+                T.append(-j); S.append(r[6]); H.append(hb); NX.append(r[4]); NY.append(r[5])
+        else:
+            H.append(r[6] * HOVS.get(order, 2.7))
+    typ = np.array(T, dtype=np.int32)
+    h = np.array(H)
+    has_b = bool(np.any(typ != 999) and np.any((typ >= 0) & (typ <= 2)))
+    return NodeSet(x=np.array(X), y=np.array(Y), s=np.array(S), h=h, h_small=h.copy(),
+                   node_type=typ, xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, bc=bc, smax=smax,
+                   meta=dict(source=path, nb=nb),
+                   rnx=np.array(NX) if has_b else None, rny=np.array(NY) if has_b else None)
 
 
 def _isint(v: str) -> bool:

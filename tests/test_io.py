@@ -58,3 +58,22 @@ def test_nodes_dump_round_trips_through_the_restart_reader(tmp_path):
     bad.node_type[5] = 0
     with pytest.raises(ValueError):
         io.read_nodes_restart(p, bad)
+
+
+def test_ipart_round_trip_with_boundaries(tmp_path):
+    """write_ipart writes boundary nodes with their normals and leaves the FD
+    rows out; read_ipart regenerates the rows along the normal
+    (setup_domain.F90:179-193) and gives the same node set, bit for bit."""
+    from sunset_amd.nodes import inflow_outflow_channel, read_ipart, wall_channel, write_ipart
+    for ns in (wall_channel(16, 16, order=6, seed=4), inflow_outflow_channel(16, 16, order=4,
+                                                                              seed=5)):
+        p = str(tmp_path / "IPART")
+        write_ipart(p, ns)
+        with open(p) as f:
+            head = f.readline().split()
+        assert int(head[0]) == ns.nb and int(head[1]) == ns.npfb - 4 * ns.nb
+        order = 6 if ns.meta["kind"] == "wall_channel" else 4
+        r = read_ipart(p, order=order)
+        for k in ("x", "y", "s", "h", "h_small", "node_type", "rnx", "rny"):
+            assert np.array_equal(getattr(r, k), getattr(ns, k)), k
+        assert r.bc == ns.bc
