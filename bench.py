@@ -209,6 +209,38 @@ def cpu_weights_omp(n_sample, threads):
                        f"{t:.2f} s with {threads} OpenMP threads")
 
 
+def boundary_leg(n=1024, steps=50, warmup=5):
+    """A wall-bounded channel of the headline's size and physics: periodic in
+    x, no-slip walls (characteristic wall conditions) at y = 0 and 1, FD rows
+    along the normals, order 6, log-conformation sPTT, driven by a body force;
+    the stage kernels take their boundary branch.  Node-steps/s over `steps`
+    eager steps (HIP events) and the stage sweep's time per node."""
+    import numpy as np
+    import sunset_amd
+    from sunset_amd.nodes import wall_channel
+    ns = wall_channel(n, n, order=6, seed=2)
+    s = sunset_amd.Solver(morder=6, form="lc").setup(ns)
+    phys = {k: v for k, v in PHYS.items() if not k.startswith("kol")}
+    s.set_phys(drive=(1.0, 0.0, 0.0), **phys)
+    m = ns.npfb
+    ic = dict(ro=np.ones(m), u=4 * ns.y * (1 - ns.y),
+              v=0.01 * np.sin(2 * np.pi * ns.x) * np.sin(np.pi * ns.y) ** 2,
+              cxx=np.ones(m), cxy=np.zeros(m), cyy=np.ones(m))
+    s.set_initial(ic)
+    s.run(warmup)
+    kern, total_ms = s.run_profiled(steps)
+    F = s.fields()
+    s.close()
+    if not all(np.isfinite(F[k]).all() for k in ("ro", "u", "v", "pxx", "pxy", "pyy")):
+        raise SystemExit("bench.py: non-finite state in the wall-channel leg")
+    t_stage = float(np.mean([kern[k] for k in ("stage1", "stage2", "stage3", "stage4")]))
+    return dict(workload=f"{n}x{n} wall channel (walls at y=0,1, periodic in x), order 6, "
+                         "log-conformation sPTT eps=0.25 Wi=2 beta=0.1, body force, PID dt",
+                nodes=m, boundary_nodes=ns.nb, steps=steps,
+                node_steps_per_s=round(m * steps / (total_ms / 1e3), 1),
+                ms_per_step=round(total_ms / steps, 4), stage_ms=round(t_stage, 4))
+
+
 def ensemble_leg(members=32, steps=100, seed=7):
     """32 simulations of different sizes in one context: N_m = round(exp(U(ln 16k,
     ln 256k))) nodes (square jittered lattices), LABFM order 4 or 6, Wi =
@@ -323,6 +355,7 @@ def main():
                          "(~1.2M nodes), spec = 0.004..0.016 (~126k nodes)")
     ap.add_argument("--cpu-weights-n", type=int, default=64)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-boundary", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -408,6 +441,12 @@ def main():
             c3["x_vs_cpu_omp"] = round(c3["weight_solves_per_s"]
                                        / c3["cpu_baseline_omp"]["value"], 1)
 
+    bnd = None
+    if not args.no_boundary:
+        print("[bench] wall channel (boundary nodes)", file=sys.stderr, flush=True)
+        bnd = boundary_leg()
+        bnd["vs_periodic"] = round(bnd["node_steps_per_s"] / value, 3)
+
     ens = None
     if not args.no_ensemble:
         print("[bench] ensemble: 32 members", file=sys.stderr, flush=True)
@@ -437,6 +476,7 @@ def main():
         "cylinder_channel_weights": c3,
         "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
+        "wall_channel": bnd,
         "ensemble": ens,
         "adjoint": adj,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
