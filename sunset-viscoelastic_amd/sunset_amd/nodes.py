@@ -340,3 +340,37 @@ def inflow_outflow_channel(nx: int, ny: int, *, order: int = 6, seed: int = 0,
                    meta=dict(generator="PCG64", seed=seed, nx=nx, ny=ny, jitter=jitter,
                              kind="inflow_outflow_channel"),
                    rnx=np.array(NX), rny=np.array(NY))
+
+
+def cylinder_in_box(n: int, *, radius: float = 0.2, order: int = 6, seed: int = 0,
+                    jitter: float = 0.3, box=(0.0, 1.0, 0.0, 1.0)) -> NodeSet:
+    """A periodic box with a no-slip cylinder (curved wall, node type 0) at its
+    centre.  Spacing s = (xmax-xmin)/n; wall nodes at angles 2 pi k/m on the
+    circle (m = round(2 pi r / s)) with outward normals and their FD rows at
+    r + s .. r + 4s; the rest a jittered lattice (PCG64, seed) outside
+    r + 4.5 s.  Walls first, each followed by its rows, then the lattice in
+    x-major order."""
+    x0, x1, y0, y1 = box
+    s = (x1 - x0) / n
+    cx, cy = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    m = int(round(2 * np.pi * radius / s))
+    X, Y, T, NX, NY = [], [], [], [], []
+    for k in range(m):
+        th = 2 * np.pi * k / m
+        nx_, ny_ = float(np.cos(th)), float(np.sin(th))
+        xs, ys, ts = _fd_column(cx + radius * nx_, cy + radius * ny_, nx_, ny_, s, 0, 0)
+        X += xs; Y += ys; T += ts; NX += [nx_] * 5; NY += [ny_] * 5
+    lat = jittered_lattice(n, seed=seed, jitter=jitter, box=box)
+    keep = np.hypot(lat.x - cx, lat.y - cy) > radius + 4.5 * s
+    X += list(lat.x[keep]); Y += list(lat.y[keep])
+    nk = int(keep.sum())
+    T += [999] * nk; NX += [0.0] * nk; NY += [0.0] * nk
+    x, y = np.array(X), np.array(Y)
+    t = np.array(T, dtype=np.int32)
+    sv = np.full(x.shape[0], s)
+    h = np.where(t == 999, HOVS[order], HOVS_BOUND[order]) * s
+    return NodeSet(x=x, y=y, s=sv, h=h, h_small=h.copy(), node_type=t, xmin=x0, xmax=x1,
+                   ymin=y0, ymax=y1, bc=(1, 1, 1, 1), smax=s,
+                   meta=dict(generator="PCG64", seed=seed, n=n, radius=radius, jitter=jitter,
+                             kind="cylinder_in_box"),
+                   rnx=np.array(NX), rny=np.array(NY))

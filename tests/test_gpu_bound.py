@@ -280,3 +280,39 @@ def test_boundary_dropin_vs_golden(name):
         assert field_rel(out[k], g[f"f_{k}"]) < FIELD_RTOL, k
     for k in ("pxx", "pxy", "pyy"):
         assert field_rel(out[k], g[f"f_{k}"]) < 1e-8, k
+
+
+def test_curved_wall_vs_oracle():
+    """A no-slip cylinder in a periodic box, order 6 (curvature terms, wall
+    frame rotation): weights and 20 RK3 steps, device against the oracle
+    (the oracle is checked against the compiled reference on this case in
+    test_oracle_bound.test_curved_wall_vs_reference)."""
+    sv = _sv()
+    from sunset_amd.nodes import cylinder_in_box
+    ns = cylinder_in_box(32, order=6, seed=3)
+    n = ns.npfb
+    ic = dict(ro=np.ones(n), u=np.where(ns.node_type == 0, 0.0, 0.1 * np.sin(2 * np.pi * ns.y)),
+              v=np.zeros(n), cxx=np.ones(n), cxy=np.zeros(n), cyy=np.ones(n))
+    print(f"[bound] cylinder in a box, {n} nodes", flush=True)
+    s = sv.Solver(morder=6, form="lc").setup(ns)
+    o = OL.OracleSim(ns, 6).setup()
+    cnt, _, _ = o.lists()
+    Wo = o.weight_arrays()
+    Wd = s.weights(2 * int(cnt.max()))
+    for i in range(n):
+        K = cnt[i]
+        for key in ("w_grad", "w_lap", "w_hyp"):
+            b = Wo[key][i, :K]
+            assert np.abs(Wd[key][i, :K] - b).max() <= 1e-9 * max(np.abs(b).max(), 1e-300), (i, key)
+    s.set_phys(drive=(1.0, 0.0, 0.0), **PH)
+    s.set_initial(ic)
+    s.run(20)
+    F = s.fields()
+    s.close()
+    o.make_phys(form="lc", drive=(1.0, 0.0), **PH)
+    o.set_initial(ic)
+    o.run(20)
+    R = o.fields()
+    assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
+    for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy"):
+        assert field_rel(F[k], R[k]) < FIELD_RTOL, k
