@@ -87,8 +87,9 @@ int sv_set_nodes(sv_ctx *ctx, int32_t npfb, const double *x, const double *y,
  * sv_set_boundaries gives the unit normals into the fluid (rnorm,
  * setup_domain.F90:177), sized npfb; entries of FD rows are taken from their
  * boundary node, other non-boundary entries are ignored.  Required before
- * sv_calc_boundary_weights when the node set has boundary nodes.  The time
- * loop supports boundary nodes in the log-conformation form. */
+ * sv_calc_boundary_weights when the node set has boundary nodes.  In the
+ * direct form, boundary nodes need Mdiff = 0 (sv_set_phys refuses otherwise);
+ * ensembles and the adjoint take node sets without boundary nodes. */
 int sv_set_boundaries(sv_ctx *ctx, const double *rnx, const double *rny);
 /* The inflow line of control.in (load_data.F90:152): inflow_type 0 soft
  * (partially non-reflecting), 1 hard, 2 inflow-outflow

@@ -938,12 +938,14 @@ static void lc_point(const double adv[3], double G11, double G12, double G21, do
   out[2] = adv[2] + UC22 + RX22;
 }
 
-/* Right-hand sides of boundary nodes, log-conformation form: the boundary
- * branches of calc_rhs_ro (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and
- * calc_rhs_logconf (:564-745), then the characteristic waves of
- * characteristic_boundaries.F90:27-175 applied by calc_rhs_nscbc (:752-813).
- * Wall derivatives are in the normal-tangent frame (component 1 = d/dn),
- * inflow/outflow ones in the x-y frame. */
+/* Right-hand sides of boundary nodes: the boundary branches of calc_rhs_ro
+ * (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and calc_rhs_logconf
+ * (:564-745), or for the direct form calc_rhs_c (rhs_di.F90:477-594, Mdiff = 0:
+ * walls the upper-convected and relaxation terms with the rotated velocity
+ * gradient, inflow/outflow zero), then the characteristic waves of
+ * characteristic_boundaries.F90:27-175 applied by calc_rhs_nscbc
+ * (rhs_lc.F90:752-813, rhs_di.F90:600-662).  Wall derivatives are in the
+ * normal-tangent frame (component 1 = d/dn), inflow/outflow ones in x-y. */
 static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, int iRKstep,
                       const double *rox, const double *roy, const double *ux, const double *uy,
                       const double *vx, const double *vy, const double *lapu,
@@ -979,11 +981,22 @@ static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, 
       /* conformation: velocity gradient rotated to x-y, no advection */
       const double G11 = xn * ux[i] - yn * uy[i], G12 = yn * ux[i] + xn * uy[i];
       const double G21 = xn * vx[i] - yn * vy[i], G22 = yn * vx[i] + xn * vy[i];
-      const double adv[3] = {0.0, 0.0, 0.0};
-      double o[3];
-      lc_point(adv, G11, G12, G21, G22, st->pxx[i], st->pxy[i], st->pyy[i], first, p->epsPTT,
-               p->lambda, o);
-      r_xx[i] = o[0]; r_xy[i] = o[1]; r_yy[i] = o[2];
+      if (p->form == OR_FORM_LC) {
+        const double adv[3] = {0.0, 0.0, 0.0};
+        double o[3];
+        lc_point(adv, G11, G12, G21, G22, st->pxx[i], st->pxy[i], st->pyy[i], first, p->epsPTT,
+                 p->lambda, o);
+        r_xx[i] = o[0]; r_xy[i] = o[1]; r_yy[i] = o[2];
+      } else {                                       /* rhs_di.F90:501-560 */
+        const double cxx = st->cxx[i], cxy = st->cxy[i], cyy = st->cyy[i], czz = 1.0;
+        const double ucxx = 2.0 * G11 * cxx + 2.0 * G12 * cxy + 0.0;
+        const double ucxy = G11 * cxy + G12 * cyy + 0.0 + G21 * cxx + G22 * cxy + 0.0;
+        const double ucyy = 2.0 * G21 * cxy + 2.0 * G22 * cyy + 0.0;
+        const double fr = -(1.0 - p->epsPTT * 2.0 + p->epsPTT * (cxx + cyy + czz)) / p->lambda;
+        r_xx[i] = ucxx + fr * (cxx - 1.0) + 0.0;
+        r_xy[i] = ucxy + fr * cxy + 0.0;
+        r_yy[i] = ucyy + fr * (cyy - 1.0) + 0.0;
+      }
       L5 = L1;                                       /* isothermal wall: reflection */
     } else {
       r_ro[i] = -v * roy[i] - ro * vy[i] - 0.0 * 0.0 - ro * 0.0;
@@ -999,6 +1012,9 @@ static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, 
       const double bfv = ro * (p->grav[1] + p->drive[1]);
       r_rou[i] = -v * uy[i] - 0.0 * 0.0 + fvu + bfu;
       r_rov[i] = -v * vy[i] - 0.0 * 0.0 - csq * roy[i] + fvv + bfv;
+      if (p->form == OR_FORM_DI) {                   /* "T.B.C.", rhs_di.F90:579-590 */
+        r_xx[i] = 0.0; r_xy[i] = 0.0; r_yy[i] = 0.0;
+      }
       /* the conformation right-hand side is the interior one (x-y frame) */
       if (t == 1) {
         const double uin = st->u_in[i];
@@ -1157,7 +1173,7 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
     }
     free(lcxy); free(lcyy);
   }
-  if (d->nb > 0 && p->form == OR_FORM_LC)
+  if (d->nb > 0)
     bound_rhs(d, p, st, iRKstep, rox, roy, ux, uy, vx, vy, lapu, lapv, cxxx, cxyx, cxyy, cyyy,
               r_ro, r_rou, r_rov, r_xx, r_xy, r_yy);
   for (int k = 0; k < 22; k++) free(g[k]);

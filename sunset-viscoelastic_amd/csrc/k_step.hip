@@ -277,20 +277,23 @@ __device__ __forceinline__ void bound_apply(const sv_physk &P, int kind, double 
   }
 }
 
-// Right-hand side of a boundary node, log-conformation form: the boundary
-// branches of calc_rhs_ro (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and
-// calc_rhs_logconf (:564-745), then the characteristic waves
+// Right-hand side of a boundary node: the boundary branches of calc_rhs_ro
+// (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and, for the conformation,
+// calc_rhs_logconf (:564-745) or calc_rhs_c (rhs_di.F90:477-594 with Mdiff = 0:
+// on walls the upper-convected and relaxation terms with the rotated velocity
+// gradient, on inflows/outflows zero), then the characteristic waves
 // (characteristic_boundaries.F90:27-175) applied as calc_rhs_nscbc
 // (rhs_lc.F90:752-813).  Wall derivatives are in the normal-tangent frame
-// (x component = d/dn, y = d/dt), inflow/outflow ones in x-y.  The interior
-// conformation right-hand side (r_xx..r_yy on entry) is kept on inflow and
-// outflow nodes.
+// (x component = d/dn, y = d/dt), inflow/outflow ones in x-y.  In the
+// log-conformation form the interior conformation right-hand side (r_xx..r_yy
+// on entry) is kept on inflow and outflow nodes.  f1..f3: psi (LC) or c (DI).
+template <int FORM>
 __device__ __forceinline__ void bound_rhs(const sv_physk &P, int kind, double nx, double ny,
                                           double uin, bool first, double ro, double u, double v,
                                           double rx, double ry, double ux, double uy, double vx,
                                           double vy, double lu, double lv, double dcxx_x,
                                           double dcxy_y, double dcxy_x, double dcyy_y,
-                                          double pxx, double pxy, double pyy, double &r_ro,
+                                          double f1, double f2, double f3, double &r_ro,
                                           double &r_rou, double &r_rov, double &r_xx,
                                           double &r_xy, double &r_yy) {
   const double csq = P.csq, c = P.cs;
@@ -312,8 +315,19 @@ __device__ __forceinline__ void bound_rhs(const sv_physk &P, int kind, double nx
     // velocity gradient rotated to x-y; no advection on a wall
     const double G11 = nx * ux - ny * uy, G12 = ny * ux + nx * uy;
     const double G21 = nx * vx - ny * vy, G22 = ny * vx + nx * vy;
-    rhs_logconf(0.0, 0.0, G11, G12, G21, G22, pxx, pxy, pyy, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
-                first, P.eps, P.lambda, r_xx, r_xy, r_yy);
+    if (FORM == SV_FORM_LC) {
+      rhs_logconf(0.0, 0.0, G11, G12, G21, G22, f1, f2, f3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                  first, P.eps, P.lambda, r_xx, r_xy, r_yy);
+    } else {
+      const double cxx = f1, cxy = f2, cyy = f3;
+      const double ucxx = 2.0 * G11 * cxx + 2.0 * G12 * cxy + 0.0;
+      const double ucxy = G11 * cxy + G12 * cyy + 0.0 + G21 * cxx + G22 * cxy + 0.0;
+      const double ucyy = 2.0 * G21 * cxy + 2.0 * G22 * cyy + 0.0;
+      const double fr = -(1.0 - P.eps * 2.0 + P.eps * (cxx + cyy + 1.0)) / P.lambda;
+      r_xx = ucxx + fr * (cxx - 1.0) + 0.0;
+      r_xy = ucxy + fr * cxy + 0.0;
+      r_yy = ucyy + fr * (cyy - 1.0) + 0.0;
+    }
     L5 = L1;                                      // isothermal wall
   } else {
     r_ro = -v * ry - ro * vy - 0.0 * 0.0 - ro * 0.0;
@@ -327,6 +341,7 @@ __device__ __forceinline__ void bound_rhs(const sv_physk &P, int kind, double nx
     fvv = fvv + P.coef_polymeric * (dcxy_x + dcyy_y + 0.0);
     r_rou = -v * uy - 0.0 * 0.0 + fvu + ro * P.bfx;
     r_rov = -v * vy - 0.0 * 0.0 - csq * ry + fvv + ro * P.bfy;
+    if (FORM == SV_FORM_DI) { r_xx = 0.0; r_xy = 0.0; r_yy = 0.0; }   // rhs_di.F90:579-590
     const double Lx = P.Lx;
     if (kind == 1) {
       if (P.inflow_type == 1) {                   // hard inflow
@@ -415,18 +430,6 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
       rhs_logconf(u, v, ux, uy, vx, vy, s.pxx, s.pxy, s.pyy, q1x - s.pxx * sx,
                   q1y - s.pxx * sy, q2x - s.pxy * sx, q2y - s.pxy * sy, q3x - s.pyy * sx,
                   q3y - s.pyy * sy, first, P.eps, P.lambda, r_xx, r_xy, r_yy);
-      if (BND) {
-        const int jb = a.bidx[i];
-        if (jb >= 0) {
-          const double *br = a.brec + (size_t)SV_BREC * jb;
-          bkind = (int)br[0];
-          buin0 = br[3];
-          const int s_rhs = STAGE == 0 ? 1 : STAGE;
-          bound_rhs(P, bkind, br[1], br[2], uin_now(P, C, buin0, s_rhs), first, ro, u, v, rx,
-                    ry, ux, uy, vx, vy, lu, lv, dcxx_x, dcxy_y, dcxy_x, dcyy_y, s.pxx, s.pxy,
-                    s.pyy, r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy);
-        }
-      }
     } else {
       // calc_rhs_c (rhs_di.F90:417-475) with full gradients of cxx, cyy
       const double gcxx_y = ayx - cxx * sy, gcyy_x = bxy - cyy * sx;
@@ -453,6 +456,22 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
         T[6] = double2{gcyy_x, dcyy_y}; T[7] = double2{0.0, 0.0};
         double2 *Q = reinterpret_cast<double2 *>(a.tapeG + (size_t)i * SV_US);
         Q[0] = double2{ro, u}; Q[1] = double2{v, cxx}; Q[2] = double2{cxy, cyy};
+      }
+    }
+    if (BND) {
+      const int jb = a.bidx[i];
+      if (jb >= 0) {
+        const double *br = a.brec + (size_t)SV_BREC * jb;
+        bkind = (int)br[0];
+        buin0 = br[3];
+        const int s_rhs = STAGE == 0 ? 1 : STAGE;
+        const bool first = (STAGE == 1 || (STAGE == 0 && a.rk1)) && (C->itime == 1);
+        const double f1 = FORM == SV_FORM_LC ? s.pxx : cxx;
+        const double f2 = FORM == SV_FORM_LC ? s.pxy : cxy;
+        const double f3 = FORM == SV_FORM_LC ? s.pyy : cyy;
+        bound_rhs<FORM>(P, bkind, br[1], br[2], uin_now(P, C, buin0, s_rhs), first, ro, u, v,
+                        rx, ry, ux, uy, vx, vy, lu, lv, dcxx_x, dcxy_y, dcxy_x, dcyy_y, f1, f2,
+                        f3, r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy);
       }
     }
     // ---- RK update (step.F90:291-343, :375-409) ----
@@ -1074,7 +1093,6 @@ template <int FORM, int STAGE>
 void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
   const bool md = FORM == SV_FORM_DI && c->any_mdiff;
   if constexpr (FORM == SV_FORM_LC) {
-    // boundary nodes exist in the log-conformation form only
     const bool bnd = a.bidx != nullptr;
     if (c->L.halo) {
       const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
@@ -1085,12 +1103,16 @@ void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
       else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
     }
   } else {
+    // boundary nodes in the direct form come without polymer diffusion
+    const bool bnd = a.bidx != nullptr;
     if (c->L.halo) {
       const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
       if (md) k_stage_h<FORM, STAGE, true, false><<<nb, BS, shm, c->stream>>>(a);
+      else if (bnd) k_stage_h<FORM, STAGE, false, true><<<nb, BS, shm, c->stream>>>(a);
       else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
     } else {
       if (md) k_stage<FORM, STAGE, true, false><<<nb, BS, 0, c->stream>>>(a);
+      else if (bnd) k_stage<FORM, STAGE, false, true><<<nb, BS, 0, c->stream>>>(a);
       else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
     }
   }

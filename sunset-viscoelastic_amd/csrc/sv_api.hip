@@ -581,6 +581,9 @@ int sv_set_phys(sv_ctx *c, const sv_phys *p) {
            "Re, Wi, Ma, U_char and L_char must be positive");
   SV_CHECK(c, !c->is_ensemble, "ensemble context: physics is set per member before "
                                  "sv_ensemble_create");
+  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC || p->Mdiff == 0.0,
+           "boundary nodes in the direct form: polymer diffusion (Mdiff != 0) needs the "
+           "no-flux wall Laplacian of rhs_di.F90:528-545, which is not built");
   c->phys = *p;
   derive_phys(c);
   c->have_phys = true;
@@ -622,10 +625,9 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
   SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
-  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC,
-           "boundary nodes: the time loop supports them in the log-conformation form only "
-           "(the reference's direct form leaves inflow/outflow right-hand sides unset, "
-           "rhs_di.F90:579-590)");
+  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC || c->phys.Mdiff == 0.0,
+           "boundary nodes in the direct form: polymer diffusion (Mdiff != 0) needs the "
+           "no-flux wall Laplacian of rhs_di.F90:528-545, which is not built");
   if (int rc = set_device(c)) return rc;
   if (int rc = sv_ensure_loop(c)) return rc;
   if (int rc = ensure_state_buffers(c)) return rc;

@@ -192,11 +192,15 @@ def test_boundary_rhs_vs_oracle(kind):
 def test_boundary_errors():
     sv = _sv()
     ns = _node_set("wall", 6, 16)
-    # the direct form has no boundary time loop
+    # the direct form takes boundary nodes without polymer diffusion only, and
+    # its adjoint no boundary nodes
     s = sv.Solver(morder=6, form="di").setup(ns)
+    with pytest.raises(sv.SunsetError, match="Mdiff"):
+        s.set_phys(Mdiff=1e-3, **PH)
     s.set_phys(**PH)
-    with pytest.raises(sv.SunsetError, match="log-conformation"):
-        s.set_initial(wall_ic(ns))
+    s.set_initial(wall_ic(ns), dt=1e-6, dt_mode="fixed")
+    with pytest.raises(sv.SunsetError, match="boundary"):
+        s.adj_setup(5)
     s.close()
     # sums before the boundary operators
     s = sv.Solver(morder=6, form="lc")
@@ -315,4 +319,34 @@ def test_curved_wall_vs_oracle():
     R = o.fields()
     assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
     for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy"):
+        assert field_rel(F[k], R[k]) < FIELD_RTOL, k
+
+
+@pytest.mark.parametrize("kind,inflow", [("wall", {}), ("io", dict(inflow_type=0)),
+                                         ("io", dict(inflow_type=1))])
+def test_boundary_direct_form_vs_oracle(kind, inflow):
+    """The direct (DI) form with boundary nodes: walls with the rotated
+    upper-convected terms, inflow/outflow conformation right-hand sides zero as
+    the reference's DI branch (rhs_di.F90:477-594); 20 RK3 steps against the
+    oracle (whose DI boundary branch is pinned to the reference-pinned LC one
+    in test_oracle_bound.test_di_boundary_rhs_pinned_by_lc)."""
+    sv = _sv()
+    ns = _node_set(kind, 6)
+    ic = wall_ic(ns) if kind == "wall" else io_ic(ns)
+    drive = (8.0, 0.0) if kind == "wall" else (0.0, 0.0)
+    print(f"[bound] DI {kind} {inflow}", flush=True)
+    s = sv.Solver(morder=6, form="di").setup(ns)
+    s.set_inflow(**inflow)
+    s.set_phys(drive=(*drive, 0.0), **PH)
+    s.set_initial(ic)
+    s.run(20)
+    F = s.fields()
+    s.close()
+    o = OL.OracleSim(ns, 6).setup()
+    o.make_phys(form="di", drive=drive, **PH, **inflow)
+    o.set_initial(ic)
+    o.run(20)
+    R = o.fields()
+    assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
+    for k in ("ro", "rou", "rov", "cxx", "cxy", "cyy"):
         assert field_rel(F[k], R[k]) < FIELD_RTOL, k

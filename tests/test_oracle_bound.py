@@ -209,3 +209,37 @@ def test_curved_wall_vs_reference():
     assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
     for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy"):
         assert field_rel(F[k], R[k]) < FIELD_RTOL, k
+
+
+@pytest.mark.parametrize("kind", ["wall", "io"])
+def test_di_boundary_rhs_pinned_by_lc(kind):
+    """The direct form's boundary branch (rhs_di.F90:477-594) against the
+    reference-pinned log-conformation one: at c = I (psi = 0) dc/dt = d psi/dt,
+    so on walls the DI conformation right-hand side (upper-convected terms with
+    the rotated velocity gradient, relaxation) equals the LC one, and rho, rho u,
+    rho v are the same code in both forms; on inflow/outflow nodes the
+    reference's DI branch sets the conformation right-hand side to zero
+    ("T.B.C.", :579-590).  The LC side is taken on the first stage of the first
+    step (rhs_lc.F90:503-506: Omega = 0, B = sym(grad u)), which is the LC
+    formula's value at psi = 0; the general branch sets omega = 0 for equal
+    eigenvalues and drops the shear there."""
+    ns = wall_channel(16, 16, order=6, seed=1) if kind == "wall" else \
+        inflow_outflow_channel(16, 16, order=6, seed=2)
+    ic = wall_ic(ns) if kind == "wall" else io_ic(ns)
+    out = {}
+    for form in ("lc", "di"):
+        o = OL.OracleSim(ns, 6).setup()
+        o.make_phys(form=form, **PH)
+        o.set_initial(ic, itime=1)
+        out[form] = o.rhs(1)
+    b = (ns.node_type >= 0) & (ns.node_type <= 2)
+    for k in ("ro", "rou", "rov"):
+        assert np.array_equal(out["lc"][k][b], out["di"][k][b]), k
+    walls = ns.node_type == 0
+    inout = b & ~walls
+    for k in ("xx", "xy", "yy"):
+        a, c = out["di"][k], out["lc"][k]
+        if walls.any():
+            assert np.abs(a[walls] - c[walls]).max() <= 1e-12 * max(np.abs(c[walls]).max(),
+                                                                   1.0), k
+        assert np.all(a[inout] == 0.0), k
