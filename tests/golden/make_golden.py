@@ -142,5 +142,44 @@ def main():
         print("wrote", name, ns.npfb, "nodes", int(ptr[-1]), "links")
 
 
+# convergence of the reference's derivative operators on an analytic field:
+# rho = 1 + 0.1 sin(2 pi x) cos(2 pi y) on periodic jittered lattices
+CONV = [(4, 1.8, (16, 32, 64)), (6, 2.4, (16, 32, 64))]
+
+
+def conv_field(x, y):
+    return 1.0 + 0.1 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+
+
+def conv_exact(x, y):
+    k = 2 * np.pi
+    gx = 0.1 * k * np.cos(k * x) * np.cos(k * y)
+    gy = -0.1 * k * np.sin(k * x) * np.sin(k * y)
+    lap = -2 * k * k * 0.1 * np.sin(k * x) * np.cos(k * y)
+    return gx, gy, lap
+
+
+def write_convergence():
+    for order, hos, sizes in CONV:
+        errs = []
+        for n in sizes:
+            ns = jittered_lattice(n, seed=11, h_over_s=hos)
+            ic = taylor_green_ic(ns)
+            ic["ro"] = conv_field(ns.x, ns.y)
+            out, _ = refdrv.run_reference(ns, order, ic, nsteps=0)
+            gx, gy, lap = conv_exact(ns.x, ns.y)
+            errs.append([np.abs(out["grad_ro_x"] - gx).max(), np.abs(out["grad_ro_y"] - gy).max(),
+                         np.abs(out["lap_ro"] - lap).max()])
+        errs = np.array(errs)
+        rates = np.log2(errs[:-1] / errs[1:])
+        np.savez_compressed(os.path.join(HERE, f"ref_conv_m{order}.npz"), order=order,
+                            h_over_s=hos, sizes=np.array(sizes), seed=11, errs=errs, rates=rates)
+        print("wrote convergence", order, rates.tolist())
+
+
 if __name__ == "__main__":
-    main()
+    if "--convergence" in sys.argv:
+        write_convergence()
+    else:
+        main()
+        write_convergence()

@@ -363,3 +363,29 @@ def test_mixed_order_weights_follow_the_order_map():
             / (np.abs(W["w_grad"][i, : cnt[i], 0]).sum()
                * np.abs(x[link[i, : cnt[i]]] - x[i]).max() ** 5) for i in i4]
     assert np.median(deg5) > 1e-6
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_convergence_order_equals_reference_vectors(order):
+    """The oracle's operators reproduce the reference's errors on an analytic
+    field at three resolutions (tests/golden/ref_conv_m*.npz), hence its
+    observed convergence orders."""
+    import os
+    from parity_util import GOLDEN
+    g = np.load(os.path.join(GOLDEN, f"ref_conv_m{order}.npz"))
+    k = 2 * np.pi
+    errs = []
+    for n in g["sizes"]:
+        ns = jittered_lattice(int(n), seed=int(g["seed"]), h_over_s=float(g["h_over_s"]))
+        o = OL.OracleSim(ns, order).setup()
+        x, y = o.xy()
+        f = 1.0 + 0.1 * np.sin(k * x) * np.cos(k * y)
+        gx, gy = o.gradient(f)
+        lap = o.laplacian(f)
+        ex = 0.1 * k * np.cos(k * ns.x) * np.cos(k * ns.y)
+        ey = -0.1 * k * np.sin(k * ns.x) * np.sin(k * ns.y)
+        el = -2 * k * k * 0.1 * np.sin(k * ns.x) * np.cos(k * ns.y)
+        errs.append([np.abs(gx - ex).max(), np.abs(gy - ey).max(), np.abs(lap - el).max()])
+    # the two sides' weights differ by ~1e-11 relative (SVD vs SVD), which
+    # shows in the smallest errors as ~1e-10 absolute
+    np.testing.assert_allclose(np.array(errs), g["errs"], rtol=1e-6, atol=1e-9)
