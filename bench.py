@@ -289,7 +289,10 @@ def ensemble_leg(members=32, steps=100, seed=7):
     E.run(2, graph=True)
     E.run(steps, graph=True)
     t_ens = E.step_ms()
-    print(f"[bench] ensemble: {t_ens:.1f} ms for {steps} steps", file=sys.stderr, flush=True)
+    mode = int(E.timing("ensemble_groups"))
+    tune = [round(E.timing(f"ensemble_groups_tune_{k}"), 3) for k in (0, 1)]
+    print(f"[bench] ensemble: {t_ens:.1f} ms for {steps} steps (launch mode {mode}, "
+          f"tuning samples {tune} ms)", file=sys.stderr, flush=True)
     ok = all(np.isfinite(E.fields(k)["ro"]).all() for k in range(len(mem)))
     E.close()
     for s in mem:
@@ -300,6 +303,9 @@ def ensemble_leg(members=32, steps=100, seed=7):
                 sizes=[side * side for side, *_ in specs],
                 orders=[o for _, o, _, _ in specs],
                 ensemble_ms=round(t_ens, 2), sequential_ms=round(t_seq, 2),
+                launch_mode=("one launch per order group" if mode == 1
+                             else "one launch over all members"),
+                launch_tune_ms=tune,
                 node_steps_per_s=round(nodes * steps / (t_ens / 1e3), 1),
                 sequential_node_steps_per_s=round(nodes * steps / (t_seq / 1e3), 1),
                 speedup=round(t_seq / t_ens, 2))

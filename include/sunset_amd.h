@@ -166,7 +166,14 @@ int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
  *   "halo_sweeps" (default 1, before sv_set_initial): the RK sweeps copy each
  *       256-node block's neighbour records into LDS once and read the links
  *       from there (when the largest block halo fits 60 KB); 0 gathers every
- *       link from global memory. */
+ *       link from global memory.
+ *   "ensemble_groups" (ensemble contexts): with members of several LABFM
+ *       orders, the stage and filter sweeps run either as one launch over all
+ *       members (0) or as one launch per order group (1), each group with the
+ *       LDS halo budget of its own largest halo.  sv_ensemble_create times the
+ *       right-hand-side sweep both ways and keeps the faster; -1 re-tunes.
+ *       sv_get_timing "ensemble_groups" reports the mode,
+ *       "ensemble_groups_tune_<mode>" the sample times. */
 int sv_set_option(sv_ctx *ctx, const char *key, double value);
 
 /* ---- derivative operators: derivatives.F90 ------------------------------ */

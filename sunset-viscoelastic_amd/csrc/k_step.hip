@@ -243,6 +243,7 @@ struct StageArgs {
   // boundary nodes (nullable): per loop slot the record index or -1, records
   const int *bidx;
   const double *brec;
+  const int *blist;     // launch group's logical blocks (nullable: all blocks)
 };
 
 // u_inflow_local of an inflow node as the last apply_time_dependent_bounds
@@ -530,7 +531,7 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
 template <int FORM, int STAGE, bool MDIFF, bool BND>
 __global__ __launch_bounds__(BS) void k_stage(StageArgs a) {
   __shared__ double red[BS / SV_WAVE];
-  const int bid = sv_xcd_block();
+  const int bid = sv_logical_block(a.blist);
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(BS) void k_stage_h(StageArgs a) {
   extern __shared__ double hl[];
   __shared__ double red[BS / SV_WAVE];
   double *hz = hl + (size_t)a.hcap * HaloL<FORM>::RS;
-  const int bid = sv_xcd_block();
+  const int bid = sv_logical_block(a.blist);
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
@@ -748,6 +749,7 @@ struct FilterArgs {
   const int64_t *hoff;   // per-block halos (k_filter_h)
   const int *hidx;
   const uint16_t *lcol;   // per-block sum of (ro_f - rho_char)*vol
+  const int *blist;       // launch group's logical blocks (nullable: all blocks)
 };
 
 // Pointwise part of the filter for one node: filtered record, its gather
@@ -776,7 +778,7 @@ __device__ __forceinline__ double filter_point(const FilterArgs &a, const NodeRe
 template <int FORM>
 __global__ __launch_bounds__(BS) void k_filter(FilterArgs a) {
   __shared__ double red[BS];
-  const int bid = sv_xcd_block();
+  const int bid = sv_logical_block(a.blist);
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
@@ -825,7 +827,7 @@ template <int FORM>
 __global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
   extern __shared__ double hl[];
   __shared__ double red[BS];
-  const int bid = sv_xcd_block();
+  const int bid = sv_logical_block(a.blist);
   const NodeRef nr = node_ref(a.blk_seg, a.seg, bid);
   const int ii = nr.ii;
   const bool act = nr.act;
@@ -1090,12 +1092,12 @@ StageArgs stage_args(sv_ctx *c, int stage, const double *Gin, double *Gout) {
 }
 
 template <int FORM, int STAGE>
-void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
+void launch_stage_grid(sv_ctx *c, const StageArgs &a, int nb, int hmax) {
   const bool md = FORM == SV_FORM_DI && c->any_mdiff;
   if constexpr (FORM == SV_FORM_LC) {
     const bool bnd = a.bidx != nullptr;
     if (c->L.halo) {
-      const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
+      const size_t shm = (size_t)hmax * HaloL<FORM>::BYTES;
       if (bnd) k_stage_h<FORM, STAGE, false, true><<<nb, BS, shm, c->stream>>>(a);
       else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
     } else {
@@ -1106,7 +1108,7 @@ void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
     // boundary nodes in the direct form come without polymer diffusion
     const bool bnd = a.bidx != nullptr;
     if (c->L.halo) {
-      const size_t shm = (size_t)c->L.hmax * HaloL<FORM>::BYTES;
+      const size_t shm = (size_t)hmax * HaloL<FORM>::BYTES;
       if (md) k_stage_h<FORM, STAGE, true, false><<<nb, BS, shm, c->stream>>>(a);
       else if (bnd) k_stage_h<FORM, STAGE, false, true><<<nb, BS, shm, c->stream>>>(a);
       else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
@@ -1116,6 +1118,39 @@ void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
       else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
     }
   }
+}
+
+// One stage over every block: one launch, or one launch per ensemble launch
+// group, each with its own halo budget (its blocks' largest halo).
+template <int FORM, int STAGE>
+void launch_stage(sv_ctx *c, const StageArgs &a, int nb) {
+  if (c->lgroup_mode == 1 && !c->lgroups.empty()) {
+    for (const sv_ctx::lgroup &g : c->lgroups) {
+      StageArgs b = a;
+      b.blist = g.blist;
+      b.hcap = g.hmax;
+      launch_stage_grid<FORM, STAGE>(c, b, g.nblk, g.hmax);
+    }
+    return;
+  }
+  launch_stage_grid<FORM, STAGE>(c, a, nb, c->L.hmax);
+}
+
+template <int FORM>
+void launch_filter(sv_ctx *c, const FilterArgs &f, int nb) {
+  if (!c->L.halo) {
+    k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
+    return;
+  }
+  if (c->lgroup_mode == 1 && !c->lgroups.empty()) {
+    for (const sv_ctx::lgroup &g : c->lgroups) {
+      FilterArgs b = f;
+      b.blist = g.blist;
+      k_filter_h<FORM><<<g.nblk, BS, sizeof(double) * 6 * (size_t)g.hmax, c->stream>>>(b);
+    }
+    return;
+  }
+  k_filter_h<FORM><<<nb, BS, sizeof(double) * 6 * (size_t)c->L.hmax, c->stream>>>(f);
 }
 
 // ev (nullable): SV_NKERNELS+1 events recorded between the launches of one step
@@ -1160,10 +1195,7 @@ int step_form(sv_ctx *c, hipEvent_t *ev, double *const *tape) {
   f.hoff = c->L.hoff;
   f.hidx = c->L.hidx;
   f.lcol = c->L.lcol;
-  if (c->L.halo)
-    k_filter_h<FORM><<<nb, BS, sizeof(double) * 6 * (size_t)c->L.hmax, c->stream>>>(f);
-  else
-    k_filter<FORM><<<nb, BS, 0, c->stream>>>(f);
+  launch_filter<FORM>(c, f, nb);
   mark();
   k_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, pm, pe, c->ctrl);
   mark();
@@ -1243,6 +1275,39 @@ int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6, double *tapeG, double *tape
   else launch_stage<SV_FORM_DI, 0>(c, a, nb);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Launch-group autotuning of an ensemble: the right-hand-side-only form of the
+// stage sweep (it leaves the state untouched) timed over every block, one
+// launch against one launch per group, reps times each after a warm-up; the
+// faster mode is kept.
+int sv_dev_tune_groups(sv_ctx *c, int reps) {
+  if (c->lgroups.size() < 2) { c->lgroup_mode = 0; return 0; }
+  DevTmp<double> out;
+  SV_HIP(c, out.alloc(6 * (size_t)c->ntot));
+  EventTmp e0, e1;
+  SV_HIP(c, e0.create());
+  SV_HIP(c, e1.create());
+  float ms[2] = {0.0f, 0.0f};
+  for (int mode = 0; mode < 2; mode++) {
+    c->lgroup_mode = mode;
+    StageArgs a = stage_args(c, 1, c->GA, c->GB);
+    a.rhs_stride = c->ntot;
+    a.rhs_out = out;
+    for (int r = -1; r < reps; r++) {
+      if (r == 0) SV_HIP(c, hipEventRecord(e0, c->stream));
+      if (c->cfg.form == SV_FORM_LC) launch_stage<SV_FORM_LC, 0>(c, a, c->nblk);
+      else launch_stage<SV_FORM_DI, 0>(c, a, c->nblk);
+    }
+    SV_HIP(c, hipEventRecord(e1, c->stream));
+    SV_HIP(c, hipGetLastError());
+    SV_HIP(c, hipEventSynchronize(e1));
+    SV_HIP(c, hipEventElapsedTime(&ms[mode], e0, e1));
+  }
+  c->lgroup_ms[0] = ms[0] / reps;
+  c->lgroup_ms[1] = ms[1] / reps;
+  c->lgroup_mode = ms[1] < ms[0] ? 1 : 0;
   return 0;
 }
 

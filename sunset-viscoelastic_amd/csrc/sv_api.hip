@@ -391,6 +391,9 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
                                                         what[17] == '8'))
     *ms = c->wvariant[(what[17] - '0') / 2 - 2];
   else if (!strcmp(what, "weights_mixed")) *ms = c->wmixed_used;
+  else if (!strcmp(what, "ensemble_groups")) *ms = c->lgroup_mode;
+  else if (!strcmp(what, "ensemble_groups_tune_0")) *ms = c->lgroup_ms[0];
+  else if (!strcmp(what, "ensemble_groups_tune_1")) *ms = c->lgroup_ms[1];
   else if (!strncmp(what, "weights_mixed_tune_", 19) && strlen(what) == 20 && what[19] >= '0' &&
            what[19] <= '2')
     *ms = c->wmixed_ms[what[19] - '0'];
@@ -404,6 +407,16 @@ int sv_get_timing(sv_ctx *c, const char *what, double *ms) {
 
 int sv_set_option(sv_ctx *c, const char *key, double value) {
   SV_CHECK(c, c && key, "null argument");
+  if (!strcmp(key, "ensemble_groups")) {
+    SV_CHECK(c, c->is_ensemble, "ensemble_groups: an ensemble context option");
+    SV_CHECK(c, value == -1.0 || value == 0.0 || value == 1.0,
+             "ensemble_groups: -1 (retune), 0 (one launch) or 1 (one launch per group)");
+    if (int rc = set_device(c)) return rc;
+    c->graph_valid = false;
+    if (value < 0.0) return sv_dev_tune_groups(c, 5);
+    c->lgroup_mode = (value > 0.0 && c->lgroups.size() >= 2) ? 1 : 0;
+    return 0;
+  }
   if (!strcmp(key, "weights_persistent")) {
     c->weights_fixed = value != 0.0 ? 1 : 0;
     c->weights_autotune = false;   // an explicit choice turns the autotuning off

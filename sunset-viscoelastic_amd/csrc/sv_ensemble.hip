@@ -184,6 +184,34 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   E->have_phys = true;
   E->have_state = true;
   if (int rc = sv_dev_init_partials(E)) return fail(rc);
+  // launch groups: the blocks of the members of each LABFM order, with the
+  // largest halo among them; the autotuner keeps one launch or one per group
+  if (halo) {
+    std::vector<int> orders;
+    for (int m = 0; m < nmembers; m++) {
+      const int o = members[m]->cfg.morder;
+      if (std::find(orders.begin(), orders.end(), o) == orders.end()) orders.push_back(o);
+    }
+    std::sort(orders.begin(), orders.end());
+    if (orders.size() >= 2) {
+      for (int o : orders) {
+        sv_ctx::lgroup g;
+        std::vector<int> bl;
+        for (int m = 0; m < nmembers; m++) {
+          if (members[m]->cfg.morder != o) continue;
+          const sv_segk &S = E->hseg[m];
+          for (int k = 0; k < S.nblk; k++) bl.push_back(S.blk0 + k);
+          g.hmax = std::max(g.hmax, members[m]->L.hmax);
+        }
+        g.nblk = (int)bl.size();
+        if (sv_alloc(E, (void **)&g.blist, I * bl.size()) ||
+            hipMemcpy(g.blist, bl.data(), I * bl.size(), hipMemcpyHostToDevice) != hipSuccess)
+          return fail(SV_E_NOMEM);
+        E->lgroups.push_back(g);
+      }
+      if (int rc = sv_dev_tune_groups(E, 5)) return fail(rc);
+    }
+  }
   *out = E;
   return 0;
 }

@@ -238,6 +238,14 @@ struct sv_ctx {
   int halo_sweeps = 1;      // 1: stage neighbour halos in LDS when they fit
   // ensemble: loop-slot -> member-node permutation of each member
   std::vector<std::vector<int>> member_perm;
+  // ensemble launch groups: the blocks of the members of one LABFM order,
+  // launched together with their own LDS halo budget (blist: logical blocks,
+  // device).  lgroup_mode: -1 untuned, 0 one launch over all blocks, 1 one
+  // launch per group (chosen by timing both at sv_ensemble_create)
+  struct lgroup { int nblk = 0, hmax = 0; int *blist = nullptr; };
+  std::vector<lgroup> lgroups;
+  int lgroup_mode = 0;
+  float lgroup_ms[2] = {0.0f, 0.0f};
   bool any_forcing = false, any_mdiff = false;  // over segments
   std::vector<sv_segk> hseg;
   // scratch for drop-in ops
@@ -322,9 +330,19 @@ int sv_dev_rhs(sv_ctx *c, int iRKstep, double *out6, double *tapeG = nullptr,
 int sv_dev_step_tape(sv_ctx *c, double *const tape[8]);
 // per-block partial minima of set_tstep for the current state (no correction)
 int sv_dev_init_partials(sv_ctx *c);
+// ensembles: time one launch against per-group launches, keep the faster
+int sv_dev_tune_groups(sv_ctx *c, int reps);
 // dst[k] += add for k < n (index rebasing when members are concatenated)
 int sv_dev_offset_int(sv_ctx *c, int *dst, int64_t n, int add);
 int sv_dev_offset_i64(sv_ctx *c, int64_t *dst, int64_t n, int64_t add);
+
+// Logical block of this workgroup: the XCD-aware index, through a launch
+// group's block list when the launch covers a subset of the blocks.
+__device__ __forceinline__ int sv_xcd_block();
+__device__ __forceinline__ int sv_logical_block(const int *blist) {
+  const int b = sv_xcd_block();
+  return blist ? blist[b] : b;
+}
 
 // XCD-aware block index: consecutive logical blocks land on the same XCD (one
 // L2), so neighbouring slices share cached field lines.  Blocks are dispatched

@@ -483,9 +483,19 @@ class Ensemble:
         return dict(zip(KERNELS, k)), tot.value
 
     def step_ms(self) -> float:
+        return self.timing("step")
+
+    def timing(self, what: str) -> float:
+        """sv_get_timing: "step", "ensemble_groups" (the launch mode in use: 0
+        one launch over all members, 1 one launch per LABFM-order group),
+        "ensemble_groups_tune_0|1" (the autotuner's sample times, ms)."""
         v = C.c_double()
-        self._ck(lib().sv_get_timing(self.h, b"step", C.byref(v)))
+        self._ck(lib().sv_get_timing(self.h, what.encode(), C.byref(v)))
         return v.value
+
+    def set_option(self, key: str, value: float):
+        """sv_set_option; "ensemble_groups": -1 retune, 0 / 1 fix the launch mode."""
+        self._ck(lib().sv_set_option(self.h, key.encode(), float(value)))
 
     def fields(self, member: int):
         n = self.sizes[member]
