@@ -754,6 +754,8 @@ int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
   SV_ADJ_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_ADJ_CHECK(c, c->cfg.form == SV_FORM_DI, "the adjoint is implemented for the DI form");
   SV_ADJ_CHECK(c, c->nb == 0, "the adjoint is implemented for node sets without boundary nodes");
+  SV_ADJ_CHECK(c, !c->has_mirror, "the adjoint is implemented for periodic node sets (no "
+               "symmetric/no-slip mirror ghosts)");
   SV_ADJ_CHECK(c, checkpoint_every >= 1 && checkpoint_every <= 1000,
                "checkpoint interval must be in [1, 1000]");
   SV_HIP(c, hipSetDevice(c->device));

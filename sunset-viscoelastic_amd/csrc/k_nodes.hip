@@ -318,6 +318,9 @@ int sv_dev_create_ghosts(sv_ctx *c) {
                                          c->irel, c->vrel, goff);
   SV_HIP(c, hipGetLastError());
   SV_HIP(c, hipStreamSynchronize(c->stream));
+  // symmetric (2) or no-slip (3) edges make mirror ghosts with sign flips
+  c->has_mirror = false;
+  for (int k = 0; k < 4; k++) c->has_mirror |= (c->bc[k] == 2 || c->bc[k] == 3) && ng > 0;
   c->np = n + ng;
   return 0;
 }
@@ -456,9 +459,10 @@ namespace {
 
 __global__ void k_loop_pack(int npfb, const int64_t *off, const int *perm, const int *iperm,
                             const int64_t *row_ptr, const int *cnt, const int *col_raw,
-                            const int *irel, const double *wx, const double *wy,
-                            const double *wl, const double *wh, int64_t nnz_max, int *lcol,
-                            double *lwx, double *lwy, double *lwl, double *lwh) {
+                            const int *irel, const int *vrel, const double *wx,
+                            const double *wy, const double *wl, const double *wh,
+                            int64_t nnz_max, int *lcol, double *lwx, double *lwy, double *lwl,
+                            double *lwh, uint8_t *lflip) {
   // one block of 64 threads per (slice, link-chunk of 4); slot = s*64 + l
   const int s = blockIdx.x;
   const int l = threadIdx.x;
@@ -472,17 +476,23 @@ __global__ void k_loop_pack(int npfb, const int64_t *off, const int *perm, const
     if (d >= nnz_max) break;
     if (i >= 0 && k < cnt[i]) {
       const int64_t sidx = row_ptr[i] + k;
-      lcol[d] = iperm[irel[col_raw[sidx]]];
+      const int raw = col_raw[sidx];
+      lcol[d] = iperm[irel[raw]];
       lwx[d] = wx[sidx];
       lwy[d] = wy[sidx];
       lwl[d] = wl[sidx];
       lwh[d] = wh[sidx];
+      if (lflip) {
+        const int vr = raw >= npfb ? vrel[raw] : 0;
+        lflip[d] = (uint8_t)(vr >= 2 ? vr - 1 : 0);
+      }
     } else {
       lcol[d] = slot < npfb ? slot : 0;
       lwx[d] = 0.0;
       lwy[d] = 0.0;
       lwl[d] = 0.0;
       lwh[d] = 0.0;
+      if (lflip) lflip[d] = 0;
     }
   }
 }
@@ -520,6 +530,13 @@ static int build_halos(sv_ctx *c) {
   SV_HIP(c, hipMemcpy(off.data(), L.off, sizeof(int64_t) * (ns + 1), hipMemcpyDeviceToHost));
   SV_HIP(c, hipMemcpy(col.data(), L.col, sizeof(int) * L.nnz, hipMemcpyDeviceToHost));
   SV_HIP(c, hipMemcpy(cnt.data(), L.cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (L.lflip) {
+    // a mirror link and a plain link to the same parent are different halo
+    // records (the mirror one carries the sign flips)
+    std::vector<uint8_t> fl(L.nnz);
+    SV_HIP(c, hipMemcpy(fl.data(), L.lflip, L.nnz, hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < L.nnz; e++) col[e] |= (int)fl[e] << SV_FLIP_SHIFT;
+  }
   std::vector<int64_t> hoff(nb + 1, 0);
   std::vector<int> hidx;
   std::vector<uint16_t> lcol(L.nnz, 0);
@@ -602,7 +619,7 @@ int sv_dev_build_loop(sv_ctx *c) {
   // (re)allocate
   void *old[] = {L.off, L.col, L.wx, L.wy, L.wl, L.wh, L.cnt, L.cnt_small, L.gsx, L.gsy,
                  L.lsum, L.hsum, L.vol, L.s, L.perm, L.iperm, L.hoff, L.hidx, L.lcol,
-                 L.bidx, L.brec};
+                 L.bidx, L.brec, L.lflip};
   for (void *b : old)
     if (b) sv_free(c, b);
   std::vector<int> hp = std::move(L.hperm);
@@ -625,9 +642,11 @@ int sv_dev_build_loop(sv_ctx *c) {
   SV_HIP(c, hipMemcpy(L.perm, L.hperm.data(), I * n, hipMemcpyHostToDevice));
   SV_HIP(c, hipMemcpy(L.iperm, iperm.data(), I * c->np, hipMemcpyHostToDevice));
   dim3 grid(ns, (wmax + 3) / 4);
+  if (c->has_mirror && sv_alloc(c, (void **)&L.lflip, (size_t)L.nnz + 1)) return SV_E_NOMEM;
   k_loop_pack<<<grid, SV_WAVE, 0, c->stream>>>(n, L.off, L.perm, L.iperm, c->row_ptr, c->cnt,
-                                               c->col_raw, c->irel, c->wx, c->wy, c->wl, c->wh,
-                                               L.nnz, L.col, L.wx, L.wy, L.wl, L.wh);
+                                               c->col_raw, c->irel, c->vrel, c->wx, c->wy,
+                                               c->wl, c->wh, L.nnz, L.col, L.wx, L.wy, L.wl,
+                                               L.wh, L.lflip);
   SV_HIP(c, hipGetLastError());
   const int B = 256, nb = (n + B - 1) / B;
   k_permute<int><<<nb, B, 0, c->stream>>>(n, L.perm, c->cnt, L.cnt);

@@ -623,7 +623,14 @@ __device__ __forceinline__ void halo_stage(const double *G, const int *hidx, int
                                            double *hl, double *hz) {
   constexpr int RS = HaloL<FORM>::RS;
   for (int h = threadIdx.x; h < nh; h += BS) {
-    const Rec9 r = load_g<FORM>(G, hidx[h0 + h]);
+    const int key = hidx[h0 + h];
+    Rec9 r = load_g<FORM>(G, key & SV_FLIP_MASK);
+    const int fl = key >> SV_FLIP_SHIFT;
+    if (fl) {   // mirror ghost (reapply_mirror_bcs, mirror_boundaries.F90:285-340)
+      if (fl != 2) r.u = -r.u;
+      if (fl != 1) r.v = -r.v;
+      if (fl != 3) { r.cxy = -r.cxy; r.pxy = -r.pxy; }
+    }
     double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * RS);
     d[0] = double2{r.ro, r.u};
     d[1] = double2{r.v, r.cxx};
@@ -851,7 +858,14 @@ __global__ __launch_bounds__(BS) void k_filter_h(FilterArgs a) {
   const int nh = (int)(a.hoff[bid + 1] - h0);
   for (int h = threadIdx.x; h < nh; h += BS) {
     double f[6];
-    load_u(a.U, a.hidx[h0 + h], f);
+    const int key = a.hidx[h0 + h];
+    load_u(a.U, key & SV_FLIP_MASK, f);
+    const int fl = key >> SV_FLIP_SHIFT;
+    if (fl) {   // mirror ghost: rho u, rho v and the xy component
+      if (fl != 2) f[1] = -f[1];
+      if (fl != 1) f[2] = -f[2];
+      if (fl != 3) f[4] = -f[4];
+    }
     double2 *d = reinterpret_cast<double2 *>(hl + (size_t)h * 6);
     d[0] = double2{f[0], f[1]};
     d[1] = double2{f[2], f[3]};

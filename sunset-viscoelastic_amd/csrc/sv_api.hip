@@ -643,6 +643,9 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
            "no-flux wall Laplacian of rhs_di.F90:528-545, which is not built");
   if (int rc = set_device(c)) return rc;
   if (int rc = sv_ensure_loop(c)) return rc;
+  SV_CHECK(c, !c->has_mirror || c->L.halo,
+           "symmetric/no-slip mirror ghosts need the LDS halo sweeps (halo_sweeps = 1 and "
+           "block halos within %d bytes)", SV_HALO_LDS_MAX);
   if (int rc = ensure_state_buffers(c)) return rc;
   const int n = c->npfb;
   std::vector<double> a(n), b(n), d(n), e(n), f(n), g(n);

@@ -35,9 +35,10 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   if (!m0) return SV_E_ARG;
   for (int m = 0; m < nmembers; m++) {
     const sv_ctx *c = members[m];
-    // members are periodic node sets: boundary nodes are single-context only
+    // members are periodic node sets: boundary nodes and mirror ghosts are
+    // single-context only
     if (!c || c->is_ensemble || !c->have_state || c->cfg.form != m0->cfg.form ||
-        c->device != m0->device || c->nb > 0)
+        c->device != m0->device || c->nb > 0 || c->has_mirror)
       return SV_E_ARG;
   }
   sv_config cfg = m0->cfg;

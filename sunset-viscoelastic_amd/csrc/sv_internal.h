@@ -157,7 +157,18 @@ struct sv_loopl {
   // records [nb][SV_BREC]; nullptr when the node set has no boundary nodes
   int *bidx = nullptr;
   double *brec = nullptr;
+  // mirror ghosts: per link the sign-flip code of its column (SV_FLIP_*),
+  // only when the node set has symmetric / no-slip mirror ghosts
+  uint8_t *lflip = nullptr;
 };
+
+// Mirror ghosts (mirror_boundaries.F90:263-352): a link to a ghost of vrelation
+// 2 (x mirror) reads its parent's record with u and the conformation's xy
+// component negated, 3 (y mirror) with v and xy negated, 4 (no-slip, or a
+// corner of two mirrors) with u and v negated.  In the halo index of a block
+// the code sits above the loop slot: key = slot | code << SV_FLIP_SHIFT.
+#define SV_FLIP_SHIFT 29
+#define SV_FLIP_MASK ((1 << SV_FLIP_SHIFT) - 1)
 
 struct sv_ctx {
   int device = 0;
@@ -247,6 +258,7 @@ struct sv_ctx {
   int lgroup_mode = 0;
   float lgroup_ms[2] = {0.0f, 0.0f};
   bool any_forcing = false, any_mdiff = false;  // over segments
+  bool has_mirror = false;  // symmetric / no-slip mirror ghosts exist
   std::vector<sv_segk> hseg;
   // scratch for drop-in ops
   double *tmp_phi = nullptr, *tmp_out = nullptr; int tmp_cap = 0;

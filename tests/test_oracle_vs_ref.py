@@ -146,3 +146,35 @@ def test_order8_oracle_vs_extended_precision_and_reference():
                         for k in range(len(js))])
         wo = W["w_grad"][i, :len(js), 0]
         assert np.abs(wo - wex).max() / np.abs(wex).max() < WEIGHT_RTOL
+
+
+@pytest.mark.skipif(not refdrv.available(6), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bc", [(2, 2, 1, 1), (1, 1, 2, 2), (2, 2, 2, 2)])
+def test_mirror_edges_vs_reference(bc):
+    """Symmetric edges and symmetric-symmetric corners (vrelation 2, 3, 4):
+    ghosts, lists, weights and 20 LC steps against the compiled reference.
+    (No-slip edges, bcond 3, abort inside the reference's own run in this
+    harness -- "double free or corruption" -- so they are pinned only through
+    the corner case, which uses the same vrelation 4 sign flips.)"""
+    import numpy as np
+    from sunset_amd.nodes import jittered_lattice
+    ns = jittered_lattice(20, seed=5, h_over_s=2.4, bc=bc)
+    n = ns.npfb
+    x, y = ns.x, ns.y
+    ic = dict(ro=1 + 0.01 * np.cos(2 * np.pi * x), u=0.1 * np.sin(np.pi * x) * np.cos(2 * np.pi * y),
+              v=0.1 * np.sin(np.pi * y) * np.cos(np.pi * x),
+              cxx=1 + 0.05 * np.cos(np.pi * x) ** 2, cxy=0.02 * np.sin(np.pi * x) * np.sin(np.pi * y),
+              cyy=np.ones(n))
+    ph = dict(Re=10.0, Wi=0.5, beta=0.5, eps=0.1, Ma=0.1)
+    R, _ = refdrv.run_reference(ns, 6, ic, nsteps=20, phys=ph)
+    o = OL.OracleSim(ns, 6).setup()
+    gx, gy = o.xy()
+    assert np.array_equal(gx, R["x"]) and np.array_equal(gy, R["y"])
+    assert np.array_equal(o.irel()[n:], R["irel"])
+    o.make_phys(form="lc", **ph)
+    o.set_initial(ic)
+    o.run(20)
+    F = o.fields()
+    assert F["dt"] == R["dt"]
+    for k in ("ro", "rou", "rov", "pxx", "pxy", "pyy"):
+        assert np.abs(F[k] - R[k]).max() <= 1e-9 * max(np.abs(R[k]).max(), 1.0), k
