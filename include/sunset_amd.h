@@ -88,8 +88,9 @@ int sv_set_nodes(sv_ctx *ctx, int32_t npfb, const double *x, const double *y,
  * setup_domain.F90:177), sized npfb; entries of FD rows are taken from their
  * boundary node, other non-boundary entries are ignored.  Required before
  * sv_calc_boundary_weights when the node set has boundary nodes.  In the
- * direct form, boundary nodes need Mdiff = 0 (sv_set_phys refuses otherwise);
- * ensembles and the adjoint take node sets without boundary nodes. */
+ * direct form with polymer diffusion (Mdiff != 0) walls add the no-flux term
+ * of rhs_di.F90:528-545 over their FD rows; the adjoint takes node sets
+ * without boundary nodes. */
 int sv_set_boundaries(sv_ctx *ctx, const double *rnx, const double *rny);
 /* The inflow line of control.in (load_data.F90:152): inflow_type 0 soft
  * (partially non-reflecting), 1 hard, 2 inflow-outflow
@@ -235,7 +236,10 @@ int sv_sync(sv_ctx *ctx);
  * process (sunset.F90:36-106); an ensemble is N such runs.  Each member is a
  * context taken through the whole setup and sv_set_initial (its own nodes,
  * orders, stencils, physics and control state); all members must have the
- * same conformation form and device and no boundary nodes.  The ensemble context holds copies of the
+ * same conformation form and device.  Members may have boundary nodes (walls,
+ * inflows with their own inflow settings and ramp, outflows) and symmetric /
+ * no-slip mirror ghosts; an ensemble with mirror members holds at most 2^29
+ * nodes.  The ensemble context holds copies of the
  * members' device data in one concatenated layout, so every launch of a step,
  * and one captured hipGraph (sv_run with use_graph), advances all members;
  * each keeps its own dt / PID history / itime (set_tstep and set_tstep_PID per

@@ -940,9 +940,10 @@ static void lc_point(const double adv[3], double G11, double G12, double G21, do
 
 /* Right-hand sides of boundary nodes: the boundary branches of calc_rhs_ro
  * (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and calc_rhs_logconf
- * (:564-745), or for the direct form calc_rhs_c (rhs_di.F90:477-594, Mdiff = 0:
- * walls the upper-convected and relaxation terms with the rotated velocity
- * gradient, inflow/outflow zero), then the characteristic waves of
+ * (:564-745), or for the direct form calc_rhs_c (rhs_di.F90:477-594: walls the
+ * upper-convected and relaxation terms with the rotated velocity gradient and,
+ * with polymer diffusion, the Laplacian of c plus the one-sided no-flux term of
+ * :528-545 over the wall's four FD rows; inflow/outflow zero), then the characteristic waves of
  * characteristic_boundaries.F90:27-175 applied by calc_rhs_nscbc
  * (rhs_lc.F90:752-813, rhs_di.F90:600-662).  Wall derivatives are in the
  * normal-tangent frame (component 1 = d/dn), inflow/outflow ones in x-y. */
@@ -950,7 +951,8 @@ static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, 
                       const double *rox, const double *roy, const double *ux, const double *uy,
                       const double *vx, const double *vy, const double *lapu,
                       const double *lapv, const double *cxxx, const double *cxyx,
-                      const double *cxyy, const double *cyyy, double *r_ro, double *r_rou,
+                      const double *cxyy, const double *cyyy, const double *lcxx,
+                      const double *lcxy, const double *lcyy, double *r_ro, double *r_rou,
                       double *r_rov, double *r_xx, double *r_xy, double *r_yy) {
   const double csq = p->csq, c = sqrt(csq);
   const double coef_solvent = p->visc_solvent;
@@ -993,9 +995,23 @@ static void bound_rhs(const or_domain *d, const or_phys *p, const or_state *st, 
         const double ucxy = G11 * cxy + G12 * cyy + 0.0 + G21 * cxx + G22 * cxy + 0.0;
         const double ucyy = 2.0 * G21 * cxy + 2.0 * G22 * cyy + 0.0;
         const double fr = -(1.0 - p->epsPTT * 2.0 + p->epsPTT * (cxx + cyy + czz)) / p->lambda;
-        r_xx[i] = ucxx + fr * (cxx - 1.0) + 0.0;
-        r_xy[i] = ucxy + fr * cxy + 0.0;
-        r_yy[i] = ucyy + fr * (cyy - 1.0) + 0.0;
+        double mxx = 0.0, mxy = 0.0, myy = 0.0;
+        if (lcxx) {
+          /* no diffusive flux through the wall: the row values i+1..i+4 */
+          const double Md = p->Mdiff;
+          const double den = 72.0 * d->s[i] * d->s[i] * d->L_char * d->L_char;
+          const double *f[3] = {st->cxx, st->cxy, st->cyy};
+          double m[3];
+          for (int q = 0; q < 3; q++)
+            m[q] = (-415.0 * f[q][i] + 576.0 * f[q][i + 1] - 216.0 * f[q][i + 2] +
+                    64.0 * f[q][i + 3] - 9.0 * f[q][i + 4]) / den;
+          mxx = Md * (lcxx[i] + m[0]);
+          mxy = Md * (lcxy[i] + m[1]);
+          myy = Md * (lcyy[i] + m[2]);
+        }
+        r_xx[i] = ucxx + fr * (cxx - 1.0) + mxx;
+        r_xy[i] = ucxy + fr * cxy + mxy;
+        r_yy[i] = ucyy + fr * (cyy - 1.0) + myy;
       }
       L5 = L1;                                       /* isothermal wall: reflection */
     } else {
@@ -1088,6 +1104,7 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
   double *cyyx = g[12], *cyyy = g[13];
   double *pxxx = g[14], *pxxy = g[15], *pxyx = g[16], *pxyy = g[17], *pyyx = g[18], *pyyy = g[19];
   double *lcxx = g[20], *tmpl = g[21];
+  double *lcxy = NULL, *lcyy = NULL;
   /* gradients of primary variables (rhs_lc.F90:64-66) */
   or_calc_gradient(d, st->ro, rox, roy);
   or_calc_gradient(d, st->u, ux, uy);
@@ -1146,7 +1163,6 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
     }
   } else {
     const double eps = p->epsPTT, lam = p->lambda, Md = p->Mdiff;
-    double *lcxy = NULL, *lcyy = NULL;
     if (Md != 0.0) {
       lcxy = malloc(sizeof(double) * n); lcyy = malloc(sizeof(double) * n);
       or_calc_laplacian(d, st->cxx, lcxx);        /* rhs_di.F90:398-401 */
@@ -1171,11 +1187,13 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
       r_xy[i] = adxy + ucxy + sxy + mxy;
       r_yy[i] = adyy + ucyy + syy + myy;
     }
-    free(lcxy); free(lcyy);
   }
-  if (d->nb > 0)
+  if (d->nb > 0) {
+    const int md = p->form == OR_FORM_DI && p->Mdiff != 0.0;
     bound_rhs(d, p, st, iRKstep, rox, roy, ux, uy, vx, vy, lapu, lapv, cxxx, cxyx, cxyy, cyyy,
-              r_ro, r_rou, r_rov, r_xx, r_xy, r_yy);
+              md ? lcxx : NULL, lcxy, lcyy, r_ro, r_rou, r_rov, r_xx, r_xy, r_yy);
+  }
+  free(lcxy); free(lcyy);
   for (int k = 0; k < 22; k++) free(g[k]);
   (void)tmpl;
 }

@@ -280,9 +280,10 @@ __device__ __forceinline__ void bound_apply(const sv_physk &P, int kind, double 
 
 // Right-hand side of a boundary node: the boundary branches of calc_rhs_ro
 // (rhs_lc.F90:144-160), calc_rhs_rovel (:279-367) and, for the conformation,
-// calc_rhs_logconf (:564-745) or calc_rhs_c (rhs_di.F90:477-594 with Mdiff = 0:
-// on walls the upper-convected and relaxation terms with the rotated velocity
-// gradient, on inflows/outflows zero), then the characteristic waves
+// calc_rhs_logconf (:564-745) or calc_rhs_c (rhs_di.F90:477-594: on walls the
+// upper-convected and relaxation terms with the rotated velocity gradient, the
+// polymer diffusion added by the caller; on inflows/outflows zero), then the
+// characteristic waves
 // (characteristic_boundaries.F90:27-175) applied as calc_rhs_nscbc
 // (rhs_lc.F90:752-813).  Wall derivatives are in the normal-tangent frame
 // (x component = d/dn, y = d/dt), inflow/outflow ones in x-y.  In the
@@ -473,6 +474,28 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
         bound_rhs<FORM>(P, bkind, br[1], br[2], uin_now(P, C, buin0, s_rhs), first, ro, u, v,
                         rx, ry, ux, uy, vx, vy, lu, lv, dcxx_x, dcxy_y, dcxy_x, dcyy_y, f1, f2,
                         f3, r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy);
+        if (FORM == SV_FORM_DI && MDIFF && bkind == 0) {
+          // polymer diffusion on a wall (rhs_di.F90:528-556): the wall
+          // Laplacian plus the one-sided term that removes the diffusive flux
+          // through the wall, over the four FD rows along the normal
+          double fr[4][3];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const Rec9 q = load_g<FORM>(a.G, (int)br[4 + k]);
+            fr[k][0] = q.cxx; fr[k][1] = q.cxy; fr[k][2] = q.cyy;
+          }
+          const double den = br[8];
+          const double c0[3] = {cxx, cxy, cyy};
+          const double lap[3] = {cxx_l - cxx * sl, cxy_l - cxy * sl, cyy_l - cyy * sl};
+          double m[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++)
+            m[q] = lap[q] + (-415.0 * c0[q] + 576.0 * fr[0][q] - 216.0 * fr[1][q] +
+                             64.0 * fr[2][q] - 9.0 * fr[3][q]) / den;
+          r_xx = r_xx + P.Mdiff * m[0];
+          r_xy = r_xy + P.Mdiff * m[1];
+          r_yy = r_yy + P.Mdiff * m[2];
+        }
       }
     }
     // ---- RK update (step.F90:291-343, :375-409) ----
@@ -1119,15 +1142,16 @@ void launch_stage_grid(sv_ctx *c, const StageArgs &a, int nb, int hmax) {
       else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
     }
   } else {
-    // boundary nodes in the direct form come without polymer diffusion
     const bool bnd = a.bidx != nullptr;
     if (c->L.halo) {
       const size_t shm = (size_t)hmax * HaloL<FORM>::BYTES;
-      if (md) k_stage_h<FORM, STAGE, true, false><<<nb, BS, shm, c->stream>>>(a);
+      if (md && bnd) k_stage_h<FORM, STAGE, true, true><<<nb, BS, shm, c->stream>>>(a);
+      else if (md) k_stage_h<FORM, STAGE, true, false><<<nb, BS, shm, c->stream>>>(a);
       else if (bnd) k_stage_h<FORM, STAGE, false, true><<<nb, BS, shm, c->stream>>>(a);
       else k_stage_h<FORM, STAGE, false, false><<<nb, BS, shm, c->stream>>>(a);
     } else {
-      if (md) k_stage<FORM, STAGE, true, false><<<nb, BS, 0, c->stream>>>(a);
+      if (md && bnd) k_stage<FORM, STAGE, true, true><<<nb, BS, 0, c->stream>>>(a);
+      else if (md) k_stage<FORM, STAGE, true, false><<<nb, BS, 0, c->stream>>>(a);
       else if (bnd) k_stage<FORM, STAGE, false, true><<<nb, BS, 0, c->stream>>>(a);
       else k_stage<FORM, STAGE, false, false><<<nb, BS, 0, c->stream>>>(a);
     }

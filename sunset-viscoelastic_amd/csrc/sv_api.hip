@@ -594,9 +594,6 @@ int sv_set_phys(sv_ctx *c, const sv_phys *p) {
            "Re, Wi, Ma, U_char and L_char must be positive");
   SV_CHECK(c, !c->is_ensemble, "ensemble context: physics is set per member before "
                                  "sv_ensemble_create");
-  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC || p->Mdiff == 0.0,
-           "boundary nodes in the direct form: polymer diffusion (Mdiff != 0) needs the "
-           "no-flux wall Laplacian of rhs_di.F90:528-545, which is not built");
   c->phys = *p;
   derive_phys(c);
   c->have_phys = true;
@@ -638,9 +635,6 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, c->have_phys, "sv_set_phys must come first");
   SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
-  SV_CHECK(c, c->nb == 0 || c->cfg.form == SV_FORM_LC || c->phys.Mdiff == 0.0,
-           "boundary nodes in the direct form: polymer diffusion (Mdiff != 0) needs the "
-           "no-flux wall Laplacian of rhs_di.F90:528-545, which is not built");
   if (int rc = set_device(c)) return rc;
   if (int rc = sv_ensure_loop(c)) return rc;
   SV_CHECK(c, !c->has_mirror || c->L.halo,
@@ -673,10 +667,13 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   if (c->nb > 0) {
     // boundary records and the loop slot -> record map; u_inflow_local starts
     // at U_char, on inflow nodes at the initial u (setup_flow.F90:48-51, :130-137)
-    std::vector<double> nx(n), ny(n);
+    std::vector<double> nx(n), ny(n), sp(n);
     SV_HIP(c, hipMemcpy(nx.data(), c->rnx, sizeof(double) * n, hipMemcpyDeviceToHost));
     SV_HIP(c, hipMemcpy(ny.data(), c->rny, sizeof(double) * n, hipMemcpyDeviceToHost));
-    std::vector<double> br((size_t)SV_BREC * c->nb);
+    SV_HIP(c, hipMemcpy(sp.data(), c->s, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<int> slot(n);
+    for (int k = 0; k < n; k++) slot[P[k]] = k;
+    std::vector<double> br((size_t)SV_BREC * c->nb, 0.0);
     std::vector<int> jbof(n, -1), bidx(n, -1);
     for (int jb = 0; jb < c->nb; jb++) {
       const int i = c->hblist[jb];
@@ -686,6 +683,8 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
       r[1] = nx[i];
       r[2] = ny[i];
       r[3] = c->hnt[i] == 1 ? u[i] : c->phys.U_char;
+      for (int k = 1; k <= 4; k++) r[3 + k] = slot[i + k];   // FD rows follow the node
+      r[8] = 72.0 * sp[i] * sp[i] * c->L_char * c->L_char;
     }
     for (int k = 0; k < n; k++) bidx[k] = jbof[P[k]];
     sv_free(c, c->L.bidx);

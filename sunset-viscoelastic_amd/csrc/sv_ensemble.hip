@@ -8,8 +8,10 @@
 // the node kernels therefore belongs to one member; the per-member reductions
 // (time step, error norm, mass correction) run one block per member over that
 // member's block range, and each member keeps its own control block (dt, PID
-// history, itime) and physics constants.  The members' own contexts stay valid
-// and unchanged: the ensemble holds copies taken at creation.
+// history, itime, inflow ramp) and physics constants.  Boundary records are
+// concatenated with their FD-row slots rebased; mirror ghosts keep their
+// sign-flip codes in the rebased halo keys.  The members' own contexts stay
+// valid and unchanged: the ensemble holds copies taken at creation.
 #include <string.h>
 
 #include <algorithm>
@@ -35,10 +37,8 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   if (!m0) return SV_E_ARG;
   for (int m = 0; m < nmembers; m++) {
     const sv_ctx *c = members[m];
-    // members are periodic node sets: boundary nodes and mirror ghosts are
-    // single-context only
     if (!c || c->is_ensemble || !c->have_state || c->cfg.form != m0->cfg.form ||
-        c->device != m0->device || c->nb > 0 || c->has_mirror)
+        c->device != m0->device)
       return SV_E_ARG;
   }
   sv_config cfg = m0->cfg;
@@ -74,8 +74,11 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     sell += c->L.nnz;
     E->any_forcing |= c->pk.kol_amp != 0.0;
     E->any_mdiff |= c->pk.Mdiff != 0.0;
+    E->has_mirror |= c->has_mirror;
+    E->nb += c->nb;
   }
-  if (nodes > INT32_MAX / 2) {
+  // mirror flip codes sit above the slot in the halo keys
+  if (nodes > INT32_MAX / 2 || (E->has_mirror && nodes > SV_FLIP_MASK)) {
     sv_destroy(E);
     return SV_E_ARG;
   }
@@ -120,7 +123,10 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   for (int m = 0; m < nmembers; m++) {
     const sv_loopl &CL = members[m]->L;
     halo = halo && CL.halo;
-    if (!halo) break;
+    if (!halo) {
+      if (E->has_mirror) return fail(SV_E_ARG);   // mirror flips live in the halo keys
+      break;
+    }
     hmax = std::max(hmax, CL.hmax);
     const int nbm = E->hseg[m].nblk;
     int64_t last = 0;
@@ -140,6 +146,12 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     EL.hmax = hmax;
   }
   std::vector<int> bseg(blocks);
+  std::vector<int> ebidx;
+  std::vector<double> ebrec;
+  if (E->nb > 0) {
+    ebidx.assign(N, -1);
+    ebrec.reserve((size_t)SV_BREC * E->nb);
+  }
   for (int m = 0; m < nmembers; m++) {
     const sv_ctx *c = members[m];
     const sv_segk &S = E->hseg[m];
@@ -164,6 +176,19 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     if (!rc) rc = copy_d2d(E, E->U + b * SV_US, c->U, D * SV_US * n);
     if (!rc) rc = copy_d2d(E, E->GA + b * E->gs, c->GA, D * E->gs * n);
     if (!rc) rc = copy_d2d(E, E->ctrl + m, c->ctrl, sizeof(sv_ctrl));
+    if (!rc && c->nb > 0) {
+      // boundary records: member record jb becomes jb0 + jb, FD-row slots move by base
+      std::vector<int> bi(n);
+      std::vector<double> br((size_t)SV_BREC * c->nb);
+      if (hipMemcpy(bi.data(), CL.bidx, I * n, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(br.data(), CL.brec, D * br.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SV_E_HIP);
+      const int jb0 = (int)(ebrec.size() / SV_BREC);
+      for (size_t k = 0; k < n; k++) ebidx[b + k] = bi[k] >= 0 ? jb0 + bi[k] : -1;
+      for (int jb = 0; jb < c->nb; jb++)
+        for (int k = 4; k < 8; k++) br[(size_t)SV_BREC * jb + k] += (double)S.base;
+      ebrec.insert(ebrec.end(), br.begin(), br.end());
+    }
     if (!rc && halo) {
       const int64_t hb = hbase[m], hn = hbase[m + 1] - hbase[m];
       rc = copy_d2d(E, EL.hoff + S.blk0, CL.hoff, sizeof(int64_t) * (size_t)S.nblk);
@@ -173,6 +198,15 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
       if (!rc) rc = copy_d2d(E, EL.lcol + sb, CL.lcol, sizeof(uint16_t) * (size_t)CL.nnz);
     }
     if (rc) return fail(rc);
+  }
+  if (E->nb > 0) {
+    if (sv_alloc(E, (void **)&EL.bidx, I * N) ||
+        sv_alloc(E, (void **)&EL.brec, D * ebrec.size()))
+      return fail(SV_E_NOMEM);
+    if (hipMemcpy(EL.bidx, ebidx.data(), I * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(EL.brec, ebrec.data(), D * ebrec.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return fail(SV_E_HIP);
   }
   if (hipMemcpyAsync(E->segk, E->hseg.data(), sizeof(sv_segk) * nmembers,
                      hipMemcpyHostToDevice, E->stream) != hipSuccess ||

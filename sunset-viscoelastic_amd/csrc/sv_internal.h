@@ -80,9 +80,11 @@ struct sv_physk {
 // Boundary nodes (node types 0 wall, 1 inflow, 2 outflow), each followed in
 // node order by its four FD rows (types -1..-4) along the unit normal
 // (setup_domain.F90:179-193).  Per boundary node jb the time loop reads one
-// record of SV_BREC doubles: kind (0 wall, 1 inflow, 2 outflow), nx, ny and
-// the initial velocity u of the node (u_inflow_local, setup_flow.F90:130-137).
-#define SV_BREC 4
+// record of SV_BREC doubles: kind (0 wall, 1 inflow, 2 outflow), nx, ny, the
+// initial velocity u of the node (u_inflow_local, setup_flow.F90:130-137), the
+// loop slots of its four FD rows, 72 s^2 L_char^2 (the denominator of the
+// direct form's no-flux wall term, rhs_di.F90:528-545) and a pad.
+#define SV_BREC 10
 
 // State records (real nodes only; ghosts are resolved through the folded
 // stencil columns).  Gather record of node j, what a neighbour sweep reads:

@@ -192,11 +192,8 @@ def test_boundary_rhs_vs_oracle(kind):
 def test_boundary_errors():
     sv = _sv()
     ns = _node_set("wall", 6, 16)
-    # the direct form takes boundary nodes without polymer diffusion only, and
-    # its adjoint no boundary nodes
+    # the adjoint takes no boundary nodes
     s = sv.Solver(morder=6, form="di").setup(ns)
-    with pytest.raises(sv.SunsetError, match="Mdiff"):
-        s.set_phys(Mdiff=1e-3, **PH)
     s.set_phys(**PH)
     s.set_initial(wall_ic(ns), dt=1e-6, dt_mode="fixed")
     with pytest.raises(sv.SunsetError, match="boundary"):
@@ -224,13 +221,6 @@ def test_boundary_errors():
     s = sv.Solver(morder=6, form="lc")
     with pytest.raises(sv.SunsetError, match="unit vector"):
         s.set_nodes(bad)
-    s.close()
-    # ensembles take periodic members only
-    s = sv.Solver(morder=6, form="lc").setup(ns)
-    s.set_phys(**PH)
-    s.set_initial(wall_ic(ns))
-    with pytest.raises(sv.SunsetError):
-        sv.Ensemble([s])
     s.close()
 
 
@@ -344,6 +334,72 @@ def test_boundary_direct_form_vs_oracle(kind, inflow):
     s.close()
     o = OL.OracleSim(ns, 6).setup()
     o.make_phys(form="di", drive=drive, **PH, **inflow)
+    o.set_initial(ic)
+    o.run(20)
+    R = o.fields()
+    assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
+    for k in ("ro", "rou", "rov", "cxx", "cxy", "cyy"):
+        assert field_rel(F[k], R[k]) < FIELD_RTOL, k
+
+
+def _varying_c_ic(ns, kind):
+    ic = wall_ic(ns) if kind == "wall" else io_ic(ns)
+    t = np.minimum(ns.y, 1.0 - ns.y) if kind == "wall" else ns.x
+    ic["cxx"] = 1.0 + 0.3 * t + 0.2 * np.sin(2 * np.pi * ns.x) * t ** 2
+    ic["cxy"] = 0.1 * t - 0.05 * t ** 3
+    ic["cyy"] = 1.0 + 0.2 * t ** 2
+    return ic
+
+
+@pytest.mark.parametrize("kind", ["wall", "io"])
+def test_boundary_direct_form_mdiff_rhs_vs_oracle(kind):
+    """Direct form with polymer diffusion: on walls the wall Laplacian of c
+    plus the one-sided no-flux term over the FD rows (rhs_di.F90:528-556); the
+    right-hand side of a conformation field varying along the normal, device
+    against the oracle (whose term is pinned by a closed form in
+    test_oracle_bound.test_di_wall_polymer_diffusion_closed_form)."""
+    sv = _sv()
+    ns = _node_set(kind, 6)
+    ic = _varying_c_ic(ns, kind)
+    print(f"[bound] DI Mdiff rhs {kind}", flush=True)
+    s = sv.Solver(morder=6, form="di").setup(ns)
+    s.set_phys(Mdiff=2e-3, **PH)
+    s.set_initial(ic)
+    o = OL.OracleSim(ns, 6).setup()
+    o.make_phys(form="di", Mdiff=2e-3, **PH)
+    o.set_initial(ic)
+    a = s.rhs(1)
+    b = o.rhs(1)
+    walls = ns.node_type == 0
+    for k in a:
+        err = np.abs(a[k] - b[k]).max() / max(np.abs(b[k]).max(), 1e-2)
+        assert err < 1e-9, (k, err)
+    if kind == "wall":
+        # the wall term is present: the walls' conformation rhs moves with Mdiff
+        s0 = sv.Solver(morder=6, form="di").setup(ns)
+        s0.set_phys(**PH)
+        s0.set_initial(ic)
+        assert np.abs(s0.rhs(1)["xx"][walls] - a["xx"][walls]).max() > 1e-3
+        s0.close()
+    s.close()
+
+
+@pytest.mark.parametrize("kind,inflow", [("wall", {}), ("io", dict(inflow_type=0))])
+def test_boundary_direct_form_mdiff_stepping_vs_oracle(kind, inflow):
+    sv = _sv()
+    ns = _node_set(kind, 6)
+    ic = _varying_c_ic(ns, kind)
+    drive = (8.0, 0.0) if kind == "wall" else (0.0, 0.0)
+    print(f"[bound] DI Mdiff steps {kind} {inflow}", flush=True)
+    s = sv.Solver(morder=6, form="di").setup(ns)
+    s.set_inflow(**inflow)
+    s.set_phys(drive=(*drive, 0.0), Mdiff=2e-3, **PH)
+    s.set_initial(ic)
+    s.run(20)
+    F = s.fields()
+    s.close()
+    o = OL.OracleSim(ns, 6).setup()
+    o.make_phys(form="di", drive=drive, Mdiff=2e-3, **PH, **inflow)
     o.set_initial(ic)
     o.run(20)
     R = o.fields()

@@ -243,3 +243,36 @@ def test_di_boundary_rhs_pinned_by_lc(kind):
             assert np.abs(a[walls] - c[walls]).max() <= 1e-12 * max(np.abs(c[walls]).max(),
                                                                    1.0), k
         assert np.all(a[inout] == 0.0), k
+
+
+def test_di_wall_polymer_diffusion_closed_form():
+    """Polymer diffusion on walls in the direct form (rhs_di.F90:528-556): the
+    wall Laplacian of c plus a one-sided term over the four FD rows at
+    s, 2s, 3s, 4s along the normal.  For c varying along the normal as a
+    quartic, the wall Laplacian is the exact second normal derivative and the
+    one-sided term (weights -415, 576, -216, 64, -9 over 72 s^2: zeroth, third
+    and fourth moments vanish) equals c'' + 25/(6 s) c'; the conformation
+    right-hand side therefore moves by Mdiff (2 c'' + 25/(6 s) c') between
+    Mdiff = M and Mdiff = 0 on every wall node, and nowhere else on walls."""
+    ns = wall_channel(16, 16, order=6, seed=4)
+    n = ns.npfb
+    x, y = ns.x, ns.y
+    a, b, c4 = 0.3, -0.2, 0.05
+    prof = lambda t: a * t + b * t ** 2 + c4 * t ** 4           # along the normal
+    # distance from the nearer wall (the channel is [0,1] in y)
+    t = np.minimum(y, 1.0 - y)
+    ic = dict(ro=np.ones(n), u=np.zeros(n), v=np.zeros(n),
+              cxx=1.0 + prof(t), cxy=0.5 * prof(t), cyy=1.0 + 2.0 * prof(t))
+    out = {}
+    for Md in (0.0, 2e-3):
+        o = OL.OracleSim(ns, 6).setup()
+        o.make_phys(form="di", Mdiff=Md, **PH)
+        o.set_initial(ic, itime=1)
+        out[Md] = o.rhs(1)
+    walls = np.where(ns.node_type == 0)[0]
+    assert len(walls) > 0
+    d1, d2 = a, 2.0 * b                        # c'(0), c''(0) along the normal
+    for k, scale in (("xx", 1.0), ("xy", 0.5), ("yy", 2.0)):
+        want = 2e-3 * scale * (2.0 * d2 + 25.0 / (6.0 * ns.s[walls]) * d1)
+        got = out[2e-3][k][walls] - out[0.0][k][walls]
+        assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max(), k
