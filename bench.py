@@ -474,6 +474,7 @@ def main():
                    "halo_records_per_node": round(st["halo_per_node"], 3),
                    "timing": "hipGraph replay" if args.graph else "eager, HIP events"},
         "node_stage_updates_per_s": round(4 * value, 1),
+        "cpu_node_stage_updates_per_s": round(4 * cpu["value"], 1) if cpu else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
