@@ -406,3 +406,30 @@ def test_boundary_direct_form_mdiff_stepping_vs_oracle(kind, inflow):
     assert F["dt"] == pytest.approx(R["dt"], rel=1e-12)
     for k in ("ro", "rou", "rov", "cxx", "cxy", "cyy"):
         assert field_rel(F[k], R[k]) < FIELD_RTOL, k
+
+
+@pytest.mark.parametrize("kind,form,mdiff", [("wall", "lc", 0.0), ("wall", "di", 2e-3),
+                                             ("io", "lc", 0.0), ("io", "di", 2e-3)])
+def test_boundary_halo_sweeps_match_global_gathers(kind, form, mdiff):
+    """The boundary variants of the stage sweep with LDS halos and with global
+    gathers (halo_sweeps 0) give identical fields."""
+    sv = _sv()
+    ns = _node_set(kind, 6)
+    ic = _varying_c_ic(ns, kind)
+    drive = (8.0, 0.0, 0.0) if kind == "wall" else (0.0, 0.0, 0.0)
+    print(f"[bound] halo vs global {kind} {form} Mdiff {mdiff}", flush=True)
+    out = []
+    for halo in (0, 1):
+        s = sv.Solver(morder=6, form=form)
+        assert sv.lib().sv_set_option(s.h, b"halo_sweeps", float(halo)) == 0
+        s.setup(ns)
+        if kind == "io":
+            s.set_inflow(inflow_type=1, u_control=1, u_start=0.5, u_end=1.0, ramp_time=1e-4)
+        s.set_phys(drive=drive, Mdiff=mdiff, **PH)
+        s.set_initial(ic)
+        s.run(8)
+        out.append(s.fields())
+        s.close()
+    keys = ("ro", "rou", "rov") + (("pxx", "pxy", "pyy") if form == "lc" else ("cxx", "cxy", "cyy"))
+    for k in keys:
+        assert np.array_equal(out[0][k], out[1][k]), k
