@@ -88,9 +88,10 @@ int sv_set_nodes(sv_ctx *ctx, int32_t npfb, const double *x, const double *y,
  * setup_domain.F90:177), sized npfb; entries of FD rows are taken from their
  * boundary node, other non-boundary entries are ignored.  Required before
  * sv_calc_boundary_weights when the node set has boundary nodes.  In the
- * direct form with polymer diffusion (Mdiff != 0) walls add the no-flux term
- * of rhs_di.F90:528-545 over their FD rows; the adjoint takes node sets
- * without boundary nodes. */
+ * direct form with polymer diffusion (Mdiff != 0) the Laplacian of c on walls
+ * is its tangential part (derivatives.F90:306-358) plus the no-flux term of
+ * rhs_di.F90:528-545 over the FD rows; the adjoint takes node sets without
+ * boundary nodes. */
 int sv_set_boundaries(sv_ctx *ctx, const double *rnx, const double *rny);
 /* The inflow line of control.in (load_data.F90:152): inflow_type 0 soft
  * (partially non-reflecting), 1 hard, 2 inflow-outflow

@@ -116,7 +116,7 @@ void or_domain_free(or_domain *d) {
   free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
   free(d->w_grad); free(d->w_lap); free(d->w_hyp);
   free(d->gsum); free(d->lsum); free(d->hsum); free(d->vol);
-  free(d->blist); free(d->fdp); free(d->rnx); free(d->rny);
+  free(d->blist); free(d->fdp); free(d->rnx); free(d->rny); free(d->wbt); free(d->wbtsum);
   memset(d, 0, sizeof(*d));
 }
 
@@ -614,6 +614,17 @@ void or_calc_labf_sums(or_domain *d) {
     }
     d->lsum[i] = sl / d->L_char / d->L_char;
     for (int k = 0; k < d->nplink; k++) d->w_lap[base + k] = d->w_lap[base + k] / d->L_char / d->L_char;
+  }
+  /* second derivatives on boundary nodes (:425-444) */
+  if (d->wbt) {
+    for (int jj = 0; jj < d->nb; jj++) {
+      const int i = d->blist[jj];
+      double *w = d->wbt + (size_t)jj * d->nplink;
+      double st = 0.0;
+      for (int k = 0; k < d->ij_count[i]; k++) st += w[k];
+      d->wbtsum[jj] = st / d->L_char / d->L_char;
+      for (int k = 0; k < d->nplink; k++) w[k] = w[k] / d->L_char / d->L_char;
+    }
   }
 }
 
@@ -1168,6 +1179,22 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
       or_calc_laplacian(d, st->cxx, lcxx);        /* rhs_di.F90:398-401 */
       or_calc_laplacian(d, st->cxy, lcxy);
       or_calc_laplacian(d, st->cyy, lcyy);
+      if (d->nb > 0) {
+        /* calc_laplacian_transverse_only_on_bound (derivatives.F90:329-339):
+         * on boundary nodes only the tangential second derivative */
+        double *lc[3] = {lcxx, lcxy, lcyy};
+        const double *f[3] = {st->cxx, st->cxy, st->cyy};
+        for (int jj = 0; jj < d->nb; jj++) {
+          const int i = d->blist[jj];
+          const double *w = d->wbt + (size_t)jj * d->nplink;
+          for (int q = 0; q < 3; q++) {
+            double t = 0.0;
+            for (int k = 0; k < d->ij_count_small[i]; k++)
+              t = t + f[q][d->ij_link[(size_t)i * d->nplink + k]] * w[k];
+            lc[q][i] = t - f[q][i] * d->wbtsum[jj];
+          }
+        }
+      }
     }
 #pragma omp parallel for
     for (int i = 0; i < n; i++) {               /* rhs_di.F90:417-475 */

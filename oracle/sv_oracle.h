@@ -59,6 +59,11 @@ typedef struct {
   int *blist;                    /* [nb] boundary nodes, ascending */
   int *fdp;                      /* [cap] FD parent of a row node, -1 otherwise */
   double *rnx, *rny;             /* [cap] unit normal into the fluid (ghosts: parent's) */
+  /* ij_wb_grad2(2,:,jj) and ij_wb_grad2_sum(2,jj): the tangential second
+   * derivative of boundary node jj (labf.F90:755, :428-443), divided by
+   * L_char^2 in or_calc_labf_sums; the Laplacian of c on boundary nodes in the
+   * direct form (calc_laplacian_transverse_only_on_bound, derivatives.F90:306-358) */
+  double *wbt, *wbtsum;          /* [nb][nplink], [nb] */
 } or_domain;
 
 typedef struct {

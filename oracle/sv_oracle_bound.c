@@ -22,8 +22,9 @@
 static const double PI_B = 3.141592653589793238462643383279502884197;
 
 int or_set_boundaries(or_domain *d, const double *rnx, const double *rny) {
-  free(d->blist); free(d->fdp); free(d->rnx); free(d->rny);
+  free(d->blist); free(d->fdp); free(d->rnx); free(d->rny); free(d->wbt); free(d->wbtsum);
   d->blist = NULL;
+  d->wbt = d->wbtsum = NULL;
   d->nb = 0;
   d->fdp = malloc(sizeof(int) * d->cap);
   d->rnx = calloc(d->cap, sizeof(double));
@@ -301,6 +302,13 @@ int or_boundary_weights(or_domain *d) {
       WL(i, k) = B2(jj, k, 0) + B2(jj, k, 1);
     }
   }
+  /* keep the tangential second-derivative weights of every boundary node */
+  free(d->wbt); free(d->wbtsum);
+  d->wbt = calloc((size_t)nb * NPL, sizeof(double));
+  d->wbtsum = calloc(nb, sizeof(double));
+  if (!d->wbt || !d->wbtsum) { free(wb2); free(ooR); free(tmp); return -1; }
+  for (int jj = 0; jj < nb; jj++)
+    for (int k = 0; k < NPL; k++) d->wbt[(size_t)jj * NPL + k] = B2(jj, k, 1);
   /* Rows 1 and 2: normal-tangent weights to the x-y frame (:984-1000) */
   for (int jj = 0; jj < nb; jj++) {
     const int ib = d->blist[jj];

@@ -475,23 +475,37 @@ __device__ __forceinline__ double stage_point(const StageArgs &a, const NodeRef 
                         rx, ry, ux, uy, vx, vy, lu, lv, dcxx_x, dcxy_y, dcxy_x, dcyy_y, f1, f2,
                         f3, r_ro_, r_rou, r_rov, r_xx, r_xy, r_yy);
         if (FORM == SV_FORM_DI && MDIFF && bkind == 0) {
-          // polymer diffusion on a wall (rhs_di.F90:528-556): the wall
-          // Laplacian plus the one-sided term that removes the diffusive flux
-          // through the wall, over the four FD rows along the normal
+          // polymer diffusion on a wall (rhs_di.F90:396-413, :528-556): the
+          // tangential second derivative of c only
+          // (calc_laplacian_transverse_only_on_bound, derivatives.F90:329-339),
+          // i.e. the wall Laplacian less its normal part ij_wb_grad2(1,:)
+          // (labf.F90:594-598 plus the curvature term of :1011), plus the
+          // one-sided term that removes the diffusive flux through the wall;
+          // both over the four FD rows along the normal
           double fr[4][3];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const Rec9 q = load_g<FORM>(a.G, (int)br[4 + k]);
             fr[k][0] = q.cxx; fr[k][1] = q.cxy; fr[k][2] = q.cyy;
           }
-          const double den = br[8];
+          const double dx = br[8], ooR = br[9], Lc = br[10];
+          const double dx2 = dx * dx;
+          const double wn[4] = {(-104.0 / 12.0 / dx2 + 4.0 / dx * ooR) / Lc / Lc,
+                                (114.0 / 12.0 / dx2 + -3.0 / dx * ooR) / Lc / Lc,
+                                (-56.0 / 12.0 / dx2 + (4.0 / 3.0) / dx * ooR) / Lc / Lc,
+                                (11.0 / 12.0 / dx2 + -0.25 / dx * ooR) / Lc / Lc};
+          const double wsum = wn[0] + wn[1] + wn[2] + wn[3];
+          const double den = 72.0 * dx * dx * Lc * Lc;
           const double c0[3] = {cxx, cxy, cyy};
           const double lap[3] = {cxx_l - cxx * sl, cxy_l - cxy * sl, cyy_l - cyy * sl};
           double m[3];
 #pragma unroll
-          for (int q = 0; q < 3; q++)
-            m[q] = lap[q] + (-415.0 * c0[q] + 576.0 * fr[0][q] - 216.0 * fr[1][q] +
-                             64.0 * fr[2][q] - 9.0 * fr[3][q]) / den;
+          for (int q = 0; q < 3; q++) {
+            const double n2 = wn[0] * fr[0][q] + wn[1] * fr[1][q] + wn[2] * fr[2][q] +
+                              wn[3] * fr[3][q] - c0[q] * wsum;
+            m[q] = (lap[q] - n2) + (-415.0 * c0[q] + 576.0 * fr[0][q] - 216.0 * fr[1][q] +
+                                    64.0 * fr[2][q] - 9.0 * fr[3][q]) / den;
+          }
           r_xx = r_xx + P.Mdiff * m[0];
           r_xy = r_xy + P.Mdiff * m[1];
           r_yy = r_yy + P.Mdiff * m[2];

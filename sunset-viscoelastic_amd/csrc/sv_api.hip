@@ -671,6 +671,9 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
     SV_HIP(c, hipMemcpy(nx.data(), c->rnx, sizeof(double) * n, hipMemcpyDeviceToHost));
     SV_HIP(c, hipMemcpy(ny.data(), c->rny, sizeof(double) * n, hipMemcpyDeviceToHost));
     SV_HIP(c, hipMemcpy(sp.data(), c->s, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<double> ooR(c->nb, 0.0);
+    if (c->ooR)
+      SV_HIP(c, hipMemcpy(ooR.data(), c->ooR, sizeof(double) * c->nb, hipMemcpyDeviceToHost));
     std::vector<int> slot(n);
     for (int k = 0; k < n; k++) slot[P[k]] = k;
     std::vector<double> br((size_t)SV_BREC * c->nb, 0.0);
@@ -684,7 +687,9 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
       r[2] = ny[i];
       r[3] = c->hnt[i] == 1 ? u[i] : c->phys.U_char;
       for (int k = 1; k <= 4; k++) r[3 + k] = slot[i + k];   // FD rows follow the node
-      r[8] = 72.0 * sp[i] * sp[i] * c->L_char * c->L_char;
+      r[8] = sp[i];
+      r[9] = ooR[jb];
+      r[10] = c->L_char;
     }
     for (int k = 0; k < n; k++) bidx[k] = jbof[P[k]];
     sv_free(c, c->L.bidx);

@@ -82,9 +82,10 @@ struct sv_physk {
 // (setup_domain.F90:179-193).  Per boundary node jb the time loop reads one
 // record of SV_BREC doubles: kind (0 wall, 1 inflow, 2 outflow), nx, ny, the
 // initial velocity u of the node (u_inflow_local, setup_flow.F90:130-137), the
-// loop slots of its four FD rows, 72 s^2 L_char^2 (the denominator of the
-// direct form's no-flux wall term, rhs_di.F90:528-545) and a pad.
-#define SV_BREC 10
+// loop slots of its four FD rows, its spacing s, the wall curvature 1/R and
+// L_char (what the direct form's wall diffusion needs, rhs_di.F90:528-545), and
+// a pad.
+#define SV_BREC 12
 
 // State records (real nodes only; ghosts are resolved through the folded
 // stencil columns).  Gather record of node j, what a neighbour sweep reads:

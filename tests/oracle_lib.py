@@ -28,7 +28,7 @@ class Domain(C.Structure):
                 ("ij_link", PI), ("w_grad", PD), ("w_lap", PD), ("w_hyp", PD),
                 ("gsum", PD), ("lsum", PD), ("hsum", PD), ("vol", PD),
                 ("L_char", C.c_double), ("nb", C.c_int), ("blist", PI), ("fdp", PI),
-                ("rnx", PD), ("rny", PD)]
+                ("rnx", PD), ("rny", PD), ("wbt", PD), ("wbtsum", PD)]
 
 
 class Phys(C.Structure):

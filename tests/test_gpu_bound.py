@@ -433,3 +433,31 @@ def test_boundary_halo_sweeps_match_global_gathers(kind, form, mdiff):
     keys = ("ro", "rou", "rov") + (("pxx", "pxy", "pyy") if form == "lc" else ("cxx", "cxy", "cyy"))
     for k in keys:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_curved_wall_direct_form_mdiff_rhs_vs_oracle():
+    """The direct form's wall diffusion on a curved wall (the normal part of
+    the wall Laplacian carries the curvature term, labf.F90:1011): the
+    right-hand side of a conformation field varying in x and y, device against
+    the oracle."""
+    sv = _sv()
+    from sunset_amd.nodes import cylinder_in_box
+    ns = cylinder_in_box(32, order=6, seed=3)
+    n = ns.npfb
+    x, y = ns.x, ns.y
+    ic = dict(ro=np.ones(n), u=np.where(ns.node_type == 0, 0.0, 0.1 * np.sin(2 * np.pi * y)),
+              v=np.zeros(n), cxx=1.0 + 0.2 * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * y) ** 2,
+              cxy=0.1 * np.sin(2 * np.pi * (x + y)), cyy=1.0 + 0.1 * np.cos(2 * np.pi * y))
+    print(f"[bound] cylinder DI Mdiff rhs, {n} nodes", flush=True)
+    s = sv.Solver(morder=6, form="di").setup(ns)
+    s.set_phys(Mdiff=2e-3, **PH)
+    s.set_initial(ic)
+    o = OL.OracleSim(ns, 6).setup()
+    o.make_phys(form="di", Mdiff=2e-3, **PH)
+    o.set_initial(ic)
+    a = s.rhs(1)
+    b = o.rhs(1)
+    s.close()
+    for k in a:
+        err = np.abs(a[k] - b[k]).max() / max(np.abs(b[k]).max(), 1e-2)
+        assert err < 1e-9, (k, err)

@@ -246,14 +246,18 @@ def test_di_boundary_rhs_pinned_by_lc(kind):
 
 
 def test_di_wall_polymer_diffusion_closed_form():
-    """Polymer diffusion on walls in the direct form (rhs_di.F90:528-556): the
-    wall Laplacian of c plus a one-sided term over the four FD rows at
-    s, 2s, 3s, 4s along the normal.  For c varying along the normal as a
-    quartic, the wall Laplacian is the exact second normal derivative and the
-    one-sided term (weights -415, 576, -216, 64, -9 over 72 s^2: zeroth, third
-    and fourth moments vanish) equals c'' + 25/(6 s) c'; the conformation
-    right-hand side therefore moves by Mdiff (2 c'' + 25/(6 s) c') between
-    Mdiff = M and Mdiff = 0 on every wall node, and nowhere else on walls."""
+    """Polymer diffusion on walls in the direct form (rhs_di.F90:396-413,
+    :528-556): on boundary nodes the Laplacian of c is the tangential second
+    derivative only (calc_laplacian_transverse_only_on_bound,
+    derivatives.F90:306-358), to which walls add a one-sided term over the four
+    FD rows at s, 2s, 3s, 4s along the normal.  For c varying along the normal
+    only (a quartic), the tangential part vanishes and the one-sided term
+    (weights -415, 576, -216, 64, -9 over 72 s^2: zeroth, third and fourth
+    moments vanish) equals c'' + 25/(6 s) c'; the conformation right-hand side
+    therefore moves by Mdiff (c'' + 25/(6 s) c') between Mdiff = M and
+    Mdiff = 0 on every wall node.  A profile along the wall adds its exact
+    tangential second derivative (a sextic in x is resolved exactly by the
+    6th-order 1D operator)."""
     ns = wall_channel(16, 16, order=6, seed=4)
     n = ns.npfb
     x, y = ns.x, ns.y
@@ -273,6 +277,23 @@ def test_di_wall_polymer_diffusion_closed_form():
     assert len(walls) > 0
     d1, d2 = a, 2.0 * b                        # c'(0), c''(0) along the normal
     for k, scale in (("xx", 1.0), ("xy", 0.5), ("yy", 2.0)):
-        want = 2e-3 * scale * (2.0 * d2 + 25.0 / (6.0 * ns.s[walls]) * d1)
+        want = 2e-3 * scale * (d2 + 25.0 / (6.0 * ns.s[walls]) * d1)
         got = out[2e-3][k][walls] - out[0.0][k][walls]
         assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max(), k
+
+    # along the wall: c = 1 + 0.1 cos(2 pi x) adds Mdiff times its tangential
+    # second derivative, -0.1 (2 pi)^2 cos(2 pi x), to 6th-order accuracy
+    ns = wall_channel(32, 32, order=6, seed=4)
+    x, n = ns.x, ns.npfb
+    ic2 = dict(ro=np.ones(n), u=np.zeros(n), v=np.zeros(n), cxx=1.0 + 0.1 * np.cos(2 * np.pi * x),
+               cxy=np.zeros(n), cyy=np.ones(n))
+    res = {}
+    for Md in (0.0, 2e-3):
+        o = OL.OracleSim(ns, 6).setup()
+        o.make_phys(form="di", Mdiff=Md, **PH)
+        o.set_initial(ic2, itime=1)
+        res[Md] = o.rhs(1)
+    walls = np.where(ns.node_type == 0)[0]
+    want = 2e-3 * (-0.1) * (2 * np.pi) ** 2 * np.cos(2 * np.pi * x[walls])
+    got = res[2e-3]["xx"][walls] - res[0.0]["xx"][walls]
+    assert np.abs(got - want).max() <= 1e-3 * np.abs(want).max()
