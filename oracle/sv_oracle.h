@@ -24,6 +24,10 @@
  *                           memory, rhs_di.F90:188-190,421-424); parity for the
  *                           DI form is pinned by relaxation/shear closed forms
  *                           and LC<->DI consistency, not by the reference.
+ *   or_boundary_weights, boundary branches (LC) -- oracle/_ref (weights, steps;
+ *                           walls, inflow types, ramp, curved walls)
+ *   boundary branches (DI) -- the LC branch at c = I; wall polymer diffusion by
+ *                           closed forms (normal quartic, tangential cosine)
  *   or_step_rk3          -- oracle/_ref (LC); Butcher order conditions
  *   or_*_tlm (DI, fixed dt) -- base trajectory bit-identical to or_run; tangent
  *                           vs central finite differences of or_run
