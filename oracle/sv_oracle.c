@@ -19,6 +19,10 @@
  *   - optional Kolmogorov forcing f_x = kol_amp*sin(kol_k*y) added as a momentum
  *     source, as the forcing line in rhs.F90:275 (commented out in
  *     rhs_lc.F90:263 / rhs_di.F90:264).
+ *
+ * Planted mistakes (or_set_plant, default 0 = none): each nonzero code switches
+ * one plausible slip into the right-hand side, so a CPU test can show that a
+ * pin fails when the slip is there (OR_PLANT_* in sv_oracle.h).
  */
 #include "sv_oracle.h"
 #include <math.h>
@@ -68,6 +72,9 @@ void or_set_num_threads(int n) {
 }
 
 static int nsize_of(int m) { return (m * m + 3 * m) / 2; }
+
+static int g_plant = OR_PLANT_NONE;
+void or_set_plant(int code) { g_plant = code; }
 
 /* ------------------------------------------------------------------------ */
 /* Domain                                                                    */
@@ -1131,6 +1138,8 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
     or_calc_gradient(d, st->cxx, cxxx, cxxy);
     or_calc_gradient(d, st->cxy, cxyx, cxyy);
     or_calc_gradient(d, st->cyy, cyyx, cyyy);
+    if (g_plant == OR_PLANT_DI_XONLY_GRAD)   /* the reference's calc_gradient_only use */
+      for (int i = 0; i < n; i++) { cxxy[i] = 0.0; cyyx[i] = 0.0; }
   }
   const double csq = p->csq;
   const double coef_solvent = p->visc_solvent;
@@ -1146,11 +1155,23 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
     double tsv = st->ro[i] * (st->u[i] * vx[i] + st->v[i] * vy[i] + 0.0 * 0.0) - st->v[i] * r_ro[i];
     double fvu = coef_solvent * lapu[i];
     double fvv = coef_solvent * lapv[i];
-    fvu = fvu + coef_polymeric * (cxxx[i] + cxyy[i] + 0.0);
-    fvv = fvv + coef_polymeric * (cxyx[i] + cyyy[i] + 0.0);
+    if (g_plant == OR_PLANT_DIVC_TRANSPOSED) {
+      fvu = fvu + coef_polymeric * (cxxy[i] + cxyx[i] + 0.0);
+      fvv = fvv + coef_polymeric * (cxyy[i] + cyyx[i] + 0.0);
+    } else {
+      fvu = fvu + coef_polymeric * (cxxx[i] + cxyy[i] + 0.0);
+      fvv = fvv + coef_polymeric * (cxyx[i] + cyyy[i] + 0.0);
+    }
     double bfu = st->ro[i] * (p->grav[0] + p->drive[0]);
     double bfv = st->ro[i] * (p->grav[1] + p->drive[1]);
-    if (p->kol_amp != 0.0) bfu = bfu + p->kol_amp * sin(p->kol_k * d->y[i]);
+    if (p->kol_amp != 0.0) {
+      double fk = p->kol_amp * sin(p->kol_k * d->y[i]);
+      if (g_plant == OR_PLANT_KOL_RHO) fk = st->ro[i] * fk;
+      if (g_plant == OR_PLANT_KOL_SIGN) fk = -fk;
+      if (g_plant == OR_PLANT_KOL_X) fk = p->kol_amp * sin(p->kol_k * d->x[i]);
+      if (g_plant == OR_PLANT_KOL_V) bfv = bfv + fk;
+      else bfu = bfu + fk;
+    }
     r_rou[i] = -tsu - csq * rox[i] + bfu + fvu;
     r_rov[i] = -tsv - csq * roy[i] + bfv + fvv;
   }
@@ -1203,6 +1224,13 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
       double adxx = -(u * cxxx[i] + v * cxxy[i] + 0.0);
       double adxy = -(u * cxyx[i] + v * cxyy[i] + 0.0);
       double adyy = -(u * cyyx[i] + v * cyyy[i] + 0.0);
+      if (g_plant == OR_PLANT_DI_ADV_SWAP) {
+        adxx = -(u * cxxy[i] + v * cxxx[i] + 0.0);
+        adxy = -(u * cxyy[i] + v * cxyx[i] + 0.0);
+        adyy = -(u * cyyy[i] + v * cyyx[i] + 0.0);
+      } else if (g_plant == OR_PLANT_DI_ADV_SIGN) {
+        adxx = -adxx; adxy = -adxy; adyy = -adyy;
+      }
       double ucxx = 2.0 * ux[i] * cxx + 2.0 * uy[i] * cxy + 0.0;
       double ucxy = ux[i] * cxy + uy[i] * cyy + 0.0 + vx[i] * cxx + vy[i] * cxy + 0.0;
       double ucyy = 2.0 * vx[i] * cxy + 2.0 * vy[i] * cyy + 0.0;
@@ -1210,6 +1238,8 @@ void or_calc_all_rhs(const or_domain *d, const or_phys *p, const or_state *st, i
       double sxx = fr * (cxx - 1.0), sxy = fr * cxy, syy = fr * (cyy - 1.0);
       double mxx = 0, mxy = 0, myy = 0;
       if (Md != 0.0) { mxx = Md * lcxx[i]; mxy = Md * lcxy[i]; myy = Md * lcyy[i]; }
+      if (g_plant == OR_PLANT_MDIFF_DROP) { mxx = 0.0; mxy = 0.0; myy = 0.0; }
+      if (g_plant == OR_PLANT_MDIFF_XX && Md != 0.0) { mxy = Md * lcxx[i]; myy = Md * lcxx[i]; }
       r_xx[i] = adxx + ucxx + sxx + mxx;
       r_xy[i] = adxy + ucxy + sxy + mxy;
       r_yy[i] = adyy + ucyy + syy + myy;

@@ -185,6 +185,24 @@ double or_loss_trace(const or_domain *d, const or_state *st);
 int  or_rollout_tlm(const or_domain *d, const or_phys *p, or_state *st, or_tangent *t,
                     int nsteps, double *loss, double *dloss);
 
+/* ---- planted mistakes (CPU tests only): or_set_plant(code) switches one slip
+ *      into or_calc_all_rhs until it is set back to OR_PLANT_NONE ---------- */
+enum {
+  OR_PLANT_NONE = 0,
+  OR_PLANT_DI_ADV_SWAP = 1,       /* DI advection u d/dy c + v d/dx c          */
+  OR_PLANT_DI_XONLY_GRAD = 2,     /* DI: d/dy cxx = d/dx cyy = 0 (the reference's
+                                     calc_gradient_only use, rhs_di.F90:188-190) */
+  OR_PLANT_DI_ADV_SIGN = 3,       /* DI advection +u.grad c                    */
+  OR_PLANT_MDIFF_DROP = 4,        /* DI polymer diffusion left out             */
+  OR_PLANT_MDIFF_XX = 5,          /* DI diffusion of cxx used for every component */
+  OR_PLANT_KOL_RHO = 6,           /* Kolmogorov forcing times rho              */
+  OR_PLANT_KOL_SIGN = 7,          /* Kolmogorov forcing with the opposite sign */
+  OR_PLANT_KOL_X = 8,             /* Kolmogorov forcing sin(k x)               */
+  OR_PLANT_KOL_V = 9,             /* Kolmogorov forcing on the v momentum      */
+  OR_PLANT_DIVC_TRANSPOSED = 10   /* polymer stress divergence d/dy cxx + d/dx cxy */
+};
+void or_set_plant(int code);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,7 @@ def lib():
         L.or_set_boundaries.argtypes = [C.POINTER(Domain), PD, PD]
         L.or_boundary_weights.argtypes = [C.POINTER(Domain)]
         L.or_set_num_threads.argtypes = [C.c_int]
+        L.or_set_plant.argtypes = [C.c_int]
         L.or_num_threads.restype = C.c_int
         _lib = L
     return _lib
