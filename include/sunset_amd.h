@@ -36,7 +36,10 @@ enum sv_status {
 };
 
 enum sv_form { SV_FORM_DI = 0, SV_FORM_LC = 1 };      /* -Ddi / -Dlc (Makefile:73-83) */
-enum sv_dt_mode { SV_DT_PID = 0, SV_DT_FIXED = 1 };
+/* PID control (set_tstep_PID, step.F90:624-682); a fixed dt; or the CFL-limited
+ * dt = min(dt_parabolic, dt_cfl) of every step (set_tstep's non-PID line,
+ * step.F90:610-611) */
+enum sv_dt_mode { SV_DT_PID = 0, SV_DT_FIXED = 1, SV_DT_CFL = 2 };
 enum sv_op {                                          /* derivatives.F90 routines */
   SV_OP_GRAD = 0,        /* calc_gradient          :26-71   */
   SV_OP_LAP = 1,         /* calc_laplacian         :73-116  */
@@ -63,8 +66,12 @@ typedef struct {       /* control.in quantities (load_data.F90:21-159) */
 } sv_phys;
 
 /* ---- context --------------------------------------------------------- */
+/* Refuses forms other than DI/LC and orders other than 4, 6, 8 (the orders
+ * the device weight kernels are built for). */
 int  sv_create(const sv_config *cfg, sv_ctx **out);
 void sv_destroy(sv_ctx *ctx);
+/* The context's last message; with ctx == NULL, the calling thread's last
+ * sv_create / sv_ensemble_create failure (no context exists to hold it). */
 const char *sv_last_error(const sv_ctx *ctx);
 const char *sv_version(void);
 

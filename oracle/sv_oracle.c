@@ -600,10 +600,74 @@ int or_calc_labf_weights_ordermap(or_domain *d, const int *order_of_node, int nt
   return 0;
 }
 
-/* calc_labf_sums, labf.F90:396-451 (interior part) */
-void or_calc_labf_sums(or_domain *d) {
-#pragma omp parallel for
-  for (int i = 0; i < d->npfb; i++) {
+static void sums_node(or_domain *d, int i);
+static void filter_coeff_node(or_domain *d, int i);
+
+/* Checker set-up for full-size node sets, where the oracle's own neighbour
+ * search or weight setup would take minutes: load lists (0-based, [npfb][nplink],
+ * ghost ids as or_create_ghosts numbers them) or finished weights (after sums
+ * and filter coefficients) computed elsewhere, and solve only chosen systems.
+ * The loaded data are what the reference's find_neighbours /
+ * calc_labf_weights / calc_labf_sums / filter_coefficients would leave in
+ * common_vars (common_vars.F90:87-97). */
+int or_set_stencil(or_domain *d, int nplink, const int *cnt, const int *cnt_small,
+                   const int *link) {
+  free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
+  const size_t L = (size_t)d->npfb * nplink;
+  d->nplink = nplink;
+  d->ij_count = malloc(sizeof(int) * d->npfb);
+  d->ij_count_small = malloc(sizeof(int) * d->npfb);
+  d->ij_link = malloc(sizeof(int) * L);
+  if (!d->ij_count || !d->ij_count_small || !d->ij_link) return -1;
+  memcpy(d->ij_count, cnt, sizeof(int) * d->npfb);
+  memcpy(d->ij_count_small, cnt_small, sizeof(int) * d->npfb);
+  memcpy(d->ij_link, link, sizeof(int) * L);
+  for (int i = 0; i < d->npfb; i++)
+    if (cnt[i] < 0 || cnt[i] > nplink || cnt_small[i] > cnt[i]) return -2;
+  for (size_t k = 0; k < L; k++)
+    if (link[k] < 0 || link[k] >= d->np) return -2;
+  return 0;
+}
+
+int or_set_weights(or_domain *d, const double *w_grad, const double *w_lap,
+                   const double *w_hyp, const double *gsum, const double *lsum,
+                   const double *hsum, const double *vol) {
+  if (!d->ij_link || alloc_weights(d)) return -1;
+  const size_t L = (size_t)d->npfb * d->nplink, n = (size_t)d->npfb;
+  memcpy(d->w_grad, w_grad, sizeof(double) * 2 * L);
+  memcpy(d->w_lap, w_lap, sizeof(double) * L);
+  memcpy(d->w_hyp, w_hyp, sizeof(double) * L);
+  memcpy(d->gsum, gsum, sizeof(double) * 2 * n);
+  memcpy(d->lsum, lsum, sizeof(double) * n);
+  memcpy(d->hsum, hsum, sizeof(double) * n);
+  memcpy(d->vol, vol, sizeof(double) * n);
+  return 0;
+}
+
+/* calc_labf_weights (labf.F90:34-394), then the node's calc_labf_sums and
+ * filter_coefficients (interior nodes), for the listed nodes only, with a per
+ * node order (nullable: morder/forder).  The weight arrays are allocated
+ * zeroed at full size; pages of nodes not listed are never written. */
+int or_weights_subset(or_domain *d, const int *nodes, int count, const int *order_of_node) {
+  if (!d->ij_link) return -1;
+  if (!d->w_grad && alloc_weights(d)) return -1;
+  for (int q = 0; q < count; q++)
+    if (nodes[q] < 0 || nodes[q] >= d->npfb) return -2;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int q = 0; q < count; q++) {
+    const int i = nodes[q];
+    const int md = order_of_node ? order_of_node[i] : d->morder;
+    const int mf = order_of_node ? order_of_node[i] : d->forder;
+    node_weights(d, i, md, mf);
+    sums_node(d, i);
+    filter_coeff_node(d, i);
+  }
+  return 0;
+}
+
+/* calc_labf_sums, labf.F90:396-451 (interior part), for node i */
+static void sums_node(or_domain *d, int i) {
+  {
     const size_t base = (size_t)i * d->nplink;
     double sh = 0, sx = 0, sy = 0, sl = 0;
     for (int k = 0; k < d->ij_count[i]; k++) {
@@ -622,6 +686,11 @@ void or_calc_labf_sums(or_domain *d) {
     d->lsum[i] = sl / d->L_char / d->L_char;
     for (int k = 0; k < d->nplink; k++) d->w_lap[base + k] = d->w_lap[base + k] / d->L_char / d->L_char;
   }
+}
+
+void or_calc_labf_sums(or_domain *d) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) sums_node(d, i);
   /* second derivatives on boundary nodes (:425-444) */
   if (d->wbt) {
     for (int jj = 0; jj < d->nb; jj++) {
@@ -635,11 +704,10 @@ void or_calc_labf_sums(or_domain *d) {
   }
 }
 
-/* filter_coefficients, labf.F90:1723-1811 */
-void or_filter_coefficients(or_domain *d) {
+/* filter_coefficients, labf.F90:1723-1811, for node i */
+static void filter_coeff_node(or_domain *d, int i) {
   const double oosqrt2 = 1.0 / sqrt(2.0);
-#pragma omp parallel for
-  for (int i = 0; i < d->npfb; i++) {
+  {
     const size_t base = (size_t)i * d->nplink;
     const int nt = d->node_type[i];
     double fc;
@@ -683,6 +751,11 @@ void or_filter_coefficients(or_domain *d) {
     }
     d->hsum[i] = hs;
   }
+}
+
+void or_filter_coefficients(or_domain *d) {
+#pragma omp parallel for
+  for (int i = 0; i < d->npfb; i++) filter_coeff_node(d, i);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1370,6 +1443,7 @@ void or_run(const or_domain *d, const or_phys *p, or_state *st, int nsteps) {
   for (int s = 0; s < nsteps; s++) {                           /* sunset.F90:75-106 */
     or_set_tstep(d, p, st);
     if (p->dt_mode == OR_DT_PID) or_set_tstep_pid(st);
+    else if (p->dt_mode == OR_DT_CFL) st->dt = fmin(st->dt_par, st->dt_cfl);
     or_step_rk3(d, p, st);
     st->itime++;
   }

@@ -41,7 +41,10 @@ extern "C" {
 #endif
 
 enum { OR_FORM_DI = 0, OR_FORM_LC = 1 };
-enum { OR_DT_PID = 0, OR_DT_FIXED = 1 };
+/* time-step modes: PID control (set_tstep_PID, step.F90:624-682), a fixed dt,
+ * or dt = min(dt_parabolic, dt_cfl) each step (the non-PID line of set_tstep,
+ * step.F90:610-611) */
+enum { OR_DT_PID = 0, OR_DT_FIXED = 1, OR_DT_CFL = 2 };
 
 /* Geometry + stencils: the state the reference keeps in common_vars.F90:51-95 */
 typedef struct {
@@ -116,6 +119,17 @@ int  or_boundary_weights(or_domain *d);
 /* per-node order map (nullable): n-system size chosen per node (config C3) */
 int  or_calc_labf_weights_ordermap(or_domain *d, const int *order_of_node,
                                    int nthreads);
+
+/* Checker set-up for full-size node sets (lists and weights computed
+ * elsewhere, e.g. downloaded from the device): 0-based lists [npfb][nplink]
+ * (after or_create_ghosts), finished weights in the reference layout, and a
+ * weight solve of listed nodes only (per-node order nullable). */
+int  or_set_stencil(or_domain *d, int nplink, const int *cnt, const int *cnt_small,
+                    const int *link);
+int  or_set_weights(or_domain *d, const double *w_grad, const double *w_lap,
+                    const double *w_hyp, const double *gsum, const double *lsum,
+                    const double *hsum, const double *vol);
+int  or_weights_subset(or_domain *d, const int *nodes, int count, const int *order_of_node);
 
 /* ---- kernels / bases ---- */
 double or_wab(double q, double h2);

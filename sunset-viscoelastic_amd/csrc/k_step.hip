@@ -1079,6 +1079,8 @@ __global__ __launch_bounds__(1024) void k_dt(const sv_segk *seg, const double *p
     const double dmax = fmin(c->dt_cfl, c->dt_par);
     if (dt > dmax) dt = dmax;
     c->dt = dt;
+  } else if (c->dt_mode == SV_DT_CFL) {
+    c->dt = fmin(c->dt_par, c->dt_cfl);
   }
   c->emax_nm1 = c->emax_n;
   c->emax_n = c->emax_np1;

@@ -703,7 +703,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     const int md = order_map_host ? order_map_host[i] : c->cfg.morder;
     const int mf = order_map_host ? order_map_host[i] : c->cfg.forder;
     for (int m : {md, mf})
-      if (m != 4 && m != 6 && m != 8)
+      if (!sv_order_supported(m))
         return sv_fail(c, SV_E_ARG, "LABFM order %d not supported on the device (4, 6, 8)", m);
     // boundary nodes are not on internal_list (labf.F90:82, :268): their
     // operators come from calc_boundary_weights, which also replaces the

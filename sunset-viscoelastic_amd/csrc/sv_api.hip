@@ -11,6 +11,10 @@
 
 #include "sv_internal.h"
 
+// Message of the last failure that had no context to hold it (sv_create,
+// sv_ensemble_create), per host thread; sv_last_error(NULL) returns it.
+static thread_local std::string g_noctx_err;
+
 int sv_fail(sv_ctx *c, int code, const char *fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -18,6 +22,7 @@ int sv_fail(sv_ctx *c, int code, const char *fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   if (c) c->err = buf;
+  else g_noctx_err = buf;
   return code;
 }
 
@@ -67,20 +72,26 @@ static void derive_phys(sv_ctx *c);
 static int upload_segment(sv_ctx *c);
 
 int sv_create(const sv_config *cfg, sv_ctx **out) {
-  if (!cfg || !out) return SV_E_ARG;
+  if (!cfg || !out) return sv_fail(nullptr, SV_E_ARG, "sv_create: null argument");
   *out = nullptr;
+  if (cfg->form != SV_FORM_DI && cfg->form != SV_FORM_LC)
+    return sv_fail(nullptr, SV_E_ARG, "sv_create: form %d is neither DI (0) nor LC (1)",
+                   cfg->form);
+  // the orders the weight kernels are built for (common_parameter.F90:45-66
+  // also allows 10 and 12)
+  for (int m : {cfg->morder, cfg->forder})
+    if (!sv_order_supported(m))
+      return sv_fail(nullptr, SV_E_ARG,
+                     "sv_create: LABFM order %d not supported on the device (4, 6, 8)", m);
   sv_ctx *c = new sv_ctx();
   c->cfg = *cfg;
   c->device = cfg->device;
-  if (cfg->form != SV_FORM_DI && cfg->form != SV_FORM_LC) {
-    delete c;
-    return SV_E_ARG;
-  }
   hipError_t e = hipSetDevice(c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
-    return SV_E_HIP;
+    return sv_fail(nullptr, SV_E_HIP, "sv_create: device %d: %s", cfg->device,
+                   hipGetErrorString(e));
   }
   *out = c;
   return 0;
@@ -97,7 +108,7 @@ void sv_destroy(sv_ctx *c) {
   delete c;
 }
 
-const char *sv_last_error(const sv_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *sv_last_error(const sv_ctx *c) { return c ? c->err.c_str() : g_noctx_err.c_str(); }
 
 int sv_sync(sv_ctx *c) {
   SV_CHECK(c, c, "null context");
@@ -721,7 +732,8 @@ int sv_set_control(sv_ctx *c, double dt, double time, int32_t itime, double e1, 
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   SV_CHECK(c, dt > 0 && e1 > 0 && e2 > 0 && e3 > 0, "dt and error norms must be positive");
-  SV_CHECK(c, dt_mode == SV_DT_PID || dt_mode == SV_DT_FIXED, "bad dt mode");
+  SV_CHECK(c, dt_mode == SV_DT_PID || dt_mode == SV_DT_FIXED || dt_mode == SV_DT_CFL,
+           "bad dt mode %d (0 PID, 1 fixed, 2 CFL-limited)", dt_mode);
   if (int rc = set_device(c)) return rc;
   sv_ctrl h{};
   SV_HIP(c, hipMemcpy(&h, c->ctrl, sizeof(h), hipMemcpyDeviceToHost));

@@ -31,22 +31,41 @@ int copy_d2d(sv_ctx *E, void *dst, const void *src, size_t bytes) {
 extern "C" {
 
 int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
-  if (!members || !out || nmembers < 1) return SV_E_ARG;
+  // The ensemble context is not returned on failure, so every refusal is
+  // reported as the context-free message (sv_last_error(NULL)) and copied to
+  // members[0]'s message when there is a first member.
+  sv_ctx *m0 = members && nmembers >= 1 ? members[0] : nullptr;
+  auto refuse = [&](int rc, const char *msg, int m) {
+    if (m >= 0) sv_fail(nullptr, rc, "sv_ensemble_create: member %d: %s", m, msg);
+    else sv_fail(nullptr, rc, "sv_ensemble_create: %s", msg);
+    if (m0) m0->err = sv_last_error(nullptr);
+    return rc;
+  };
+  if (!out) return refuse(SV_E_ARG, "null output pointer", -1);
   *out = nullptr;
-  const sv_ctx *m0 = members[0];
-  if (!m0) return SV_E_ARG;
+  if (!members || nmembers < 1) return refuse(SV_E_ARG, "no members", -1);
+  if (!m0) return refuse(SV_E_ARG, "null member", 0);
   for (int m = 0; m < nmembers; m++) {
     const sv_ctx *c = members[m];
-    if (!c || c->is_ensemble || !c->have_state || c->cfg.form != m0->cfg.form ||
-        c->device != m0->device)
-      return SV_E_ARG;
+    if (!c) return refuse(SV_E_ARG, "null member", m);
+    if (c->is_ensemble) return refuse(SV_E_ARG, "is an ensemble context", m);
+    if (!c->have_state) return refuse(SV_E_ARG, "has no state (sv_set_initial first)", m);
+    if (c->cfg.form != m0->cfg.form)
+      return refuse(SV_E_ARG, "conformation form differs from member 0's", m);
+    if (c->device != m0->device) return refuse(SV_E_ARG, "device differs from member 0's", m);
   }
   sv_config cfg = m0->cfg;
   sv_ctx *E = nullptr;
-  if (int rc = sv_create(&cfg, &E)) return rc;
+  if (int rc = sv_create(&cfg, &E)) return refuse(rc, sv_last_error(nullptr), -1);
+  // E's own message (allocation or HIP failures) goes to the caller
   auto fail = [&](int rc) {
+    std::string msg = E->err.empty() ? std::string("failed") : E->err;
     sv_destroy(E);
-    return rc;
+    return refuse(rc, msg.c_str(), -1);
+  };
+  auto fail_msg = [&](int rc, const char *msg) {
+    E->err = msg;
+    return fail(rc);
   };
   E->is_ensemble = true;
   E->nseg = nmembers;
@@ -59,7 +78,7 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
   for (int m = 0; m < nmembers; m++) {
     sv_ctx *c = members[m];
     if (hipSetDevice(c->device) != hipSuccess) return fail(SV_E_HIP);
-    if (int rc = sv_ensure_loop(c)) return fail(rc);
+    if (int rc = sv_ensure_loop(c)) return fail_msg(rc, c->err.c_str());
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SV_E_HIP);
     sv_segk &S = E->hseg[m];
     S.base = (int)nodes;
@@ -78,10 +97,11 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     E->nb += c->nb;
   }
   // mirror flip codes sit above the slot in the halo keys
-  if (nodes > INT32_MAX / 2 || (E->has_mirror && nodes > SV_FLIP_MASK)) {
-    sv_destroy(E);
-    return SV_E_ARG;
-  }
+  if (nodes > INT32_MAX / 2)
+    return fail_msg(SV_E_ARG, "the members' nodes exceed 2^30 in total");
+  if (E->has_mirror && nodes > SV_FLIP_MASK)
+    return fail_msg(SV_E_ARG, "members with mirror ghosts: the members' block-padded nodes "
+                              "exceed the halo key's slot field");
   (void)hipSetDevice(E->device);
   E->ntot = (int)nodes;
   E->nblk = blocks;
@@ -124,7 +144,11 @@ int sv_ensemble_create(sv_ctx *const *members, int32_t nmembers, sv_ctx **out) {
     const sv_loopl &CL = members[m]->L;
     halo = halo && CL.halo;
     if (!halo) {
-      if (E->has_mirror) return fail(SV_E_ARG);   // mirror flips live in the halo keys
+      // mirror flips live in the halo keys
+      if (E->has_mirror)
+        return fail_msg(SV_E_ARG, "a member has mirror ghosts, so every member needs "
+                                  "per-block neighbour halos (a member has none: its halo "
+                                  "exceeds the LDS budget or halo_sweeps is off)");
       break;
     }
     hmax = std::max(hmax, CL.hmax);

@@ -275,7 +275,11 @@ struct sv_ctx {
 };
 
 // ---- helpers (sv_api.cpp) ----
+// Sets the context's message (or, for c == NULL, the calling thread's
+// context-free message that sv_last_error(NULL) returns) and returns code.
 int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
+// LABFM orders the weight kernels are instantiated for
+static inline bool sv_order_supported(int m) { return m == 4 || m == 6 || m == 8; }
 #define SV_HIP(c, call)                                                        \
   do {                                                                         \
     hipError_t e_ = (call);                                                    \

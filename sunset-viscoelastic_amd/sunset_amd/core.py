@@ -16,11 +16,13 @@ from typing import Optional
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-# SUNSET_AMD_LIB selects another build of the library (e.g. a tuning variant)
-LIB_PATH = os.environ.get("SUNSET_AMD_LIB") or os.path.join(PKG_DIR, "lib", "libsunset_amd.so")
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libsunset_amd.so")
 
 FORM_DI, FORM_LC = 0, 1
-DT_PID, DT_FIXED = 0, 1
+DT_PID, DT_FIXED, DT_CFL = 0, 1, 2
+# "pid": set_tstep_PID (step.F90:624-682); "fixed": the given dt; "cfl": dt =
+# min(dt_parabolic, dt_cfl) every step (step.F90:610-611)
+DT_MODES = {"pid": DT_PID, "fixed": DT_FIXED, "cfl": DT_CFL}
 OP_GRAD, OP_LAP, OP_GRAD_X, OP_GRAD_Y, OP_FILTER, OP_FILTER_TERM = range(6)
 
 PD = C.POINTER(C.c_double)
@@ -62,12 +64,12 @@ EXPORTS = [
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
 
 
-def lib(path: Optional[str] = None):
+def lib():
     """Load the library (no compute happens here, so this works without a GPU)."""
     global _lib
     if _lib is not None:
         return _lib
-    p = path or LIB_PATH
+    p = LIB_PATH
     if not os.path.exists(p):
         raise SunsetError(f"libsunset_amd.so not built: {p} is missing "
                           "(run __graft_entry__.build())")
@@ -153,7 +155,7 @@ class Solver:
         h = C.c_void_p()
         rc = L.sv_create(C.byref(cfg), C.byref(h))
         if rc != 0:
-            raise SunsetError(f"sv_create failed ({rc}): is a HIP device visible?")
+            raise SunsetError(f"sv_create failed ({rc}): {L.sv_last_error(None).decode()}")
         self.h = h
         self.npfb = 0
         self.np = 0
@@ -256,6 +258,10 @@ class Solver:
         1 hard, 2 inflow-outflow; u_control 1 ramps the inflow velocity."""
         self._ck(lib().sv_set_inflow(self.h, inflow_type, u_control, u_start, u_end, ramp_time))
 
+    def set_option(self, key: str, value: float):
+        """sv_set_option (e.g. "halo_sweeps" 0/1 before set_initial)."""
+        self._ck(lib().sv_set_option(self.h, key.encode(), float(value)))
+
     def timing(self, what: str) -> float:
         v = C.c_double()
         self._ck(lib().sv_get_timing(self.h, what.encode(), C.byref(v)))
@@ -323,7 +329,7 @@ class Solver:
         self._ck(lib().sv_set_initial(self.h, *[_d(a) for a in arrs]))
         self._ck(lib().sv_set_control(self.h, 1.0e-6 if dt is None else dt, time, itime,
                                       emax[0], emax[1], emax[2],
-                                      DT_PID if dt_mode == "pid" else DT_FIXED))
+                                      DT_MODES[dt_mode]))
 
     def run(self, nsteps: int, graph: bool = False):
         self._ck(lib().sv_run(self.h, nsteps, 1 if graph else 0))
@@ -446,8 +452,8 @@ class Ensemble:
         h = C.c_void_p()
         rc = L.sv_ensemble_create(arr, len(members), C.byref(h))
         if rc != 0:
-            msgs = "; ".join(L.sv_last_error(m.h).decode() for m in members)
-            raise SunsetError(f"sv_ensemble_create failed ({rc}) {msgs}")
+            raise SunsetError(f"sv_ensemble_create failed ({rc}): "
+                              f"{L.sv_last_error(None).decode()}")
         self.h = h
         self.sizes = [m.npfb for m in members]
 

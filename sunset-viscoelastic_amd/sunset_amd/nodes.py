@@ -196,6 +196,23 @@ def perturbed_rest_ic(ns: NodeSet, amp: float = 0.01, seed: int = 0, kmax: int =
                 cyy=np.ones(n))
 
 
+def ensemble_specs(members: int = 32, seed: int = 7):
+    """Members of the batched-ensemble workload: (lattice side, LABFM order,
+    Wi, beta) with node counts N = round(exp(U(ln 16384, ln 262144))) (side =
+    round(sqrt N), at least 16), order 4 or 6, Wi = exp(U(ln 0.1, ln 10)) and
+    beta = U(0.1, 0.9), drawn in that order per member from numpy PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    specs = []
+    for _ in range(members):
+        N = int(round(math.exp(rng.uniform(math.log(16384), math.log(262144)))))
+        side = max(16, int(round(math.sqrt(N))))
+        order = int(rng.choice([4, 6]))
+        Wi = math.exp(rng.uniform(math.log(0.1), math.log(10.0)))
+        beta = rng.uniform(0.1, 0.9)
+        specs.append((side, order, Wi, beta))
+    return specs
+
+
 def cylinder_channel(smin: float = 0.004, smax: float = 0.016, grow: float = 1.5,
                      box=(0.0, 8.0, 0.0, 2.0), centre=(2.0, 1.0), radius: float = 0.5,
                      seed: int = 3, jitter: float = 0.3, h_over_s: float = 2.7,

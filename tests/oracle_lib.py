@@ -106,6 +106,9 @@ def lib():
         L.or_diag_tg.restype = C.c_double
         L.or_set_boundaries.argtypes = [C.POINTER(Domain), PD, PD]
         L.or_boundary_weights.argtypes = [C.POINTER(Domain)]
+        L.or_set_stencil.argtypes = [C.POINTER(Domain), C.c_int, PI, PI, PI]
+        L.or_set_weights.argtypes = [C.POINTER(Domain)] + [PD] * 7
+        L.or_weights_subset.argtypes = [C.POINTER(Domain), PI, C.c_int, PI]
         L.or_set_num_threads.argtypes = [C.c_int]
         L.or_set_plant.argtypes = [C.c_int]
         L.or_num_threads.restype = C.c_int
@@ -196,6 +199,34 @@ class OracleSim:
         self.weights()
         return self
 
+    # ---- checker set-up for full-size node sets ----
+    def load_lists(self, cnt, cnt_small, link):
+        """Neighbour lists computed elsewhere (0-based link [npfb, nplink])."""
+        link = np.ascontiguousarray(link, dtype=np.int32)
+        self._lists = [np.ascontiguousarray(cnt, dtype=np.int32),
+                       np.ascontiguousarray(cnt_small, dtype=np.int32), link]
+        self.nplink = link.shape[1]
+        rc = lib().or_set_stencil(C.byref(self.d), self.nplink, *[_pi(a) for a in self._lists])
+        assert rc == 0, rc
+
+    def load_weights(self, W):
+        """Finished weights (after sums and filter coefficients) in the
+        reference layout, as Solver.weights(nplink) returns them."""
+        arrs = [np.ascontiguousarray(W[k], dtype=np.float64)
+                for k in ("w_grad", "w_lap", "w_hyp", "gsum", "lsum", "hsum", "vol")]
+        assert arrs[1].shape == (self.d.npfb, self.d.nplink)
+        assert lib().or_set_weights(C.byref(self.d), *[_pd(a) for a in arrs]) == 0
+
+    def weights_subset(self, nodes, order_map=None):
+        """The weight solve (derivative and filter systems, volumes) of the
+        listed nodes only; the other nodes' weights stay zero."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        om = None if order_map is None else np.ascontiguousarray(order_map, dtype=np.int32)
+        self._om_subset = om
+        rc = lib().or_weights_subset(C.byref(self.d), _pi(nodes), nodes.size,
+                                     None if om is None else _pi(om))
+        assert rc == 0, rc
+
     # ---- array views ----
     @property
     def np_(self):
@@ -274,7 +305,7 @@ class OracleSim:
             L_char, U_char, Re, Wi, beta, eps, Ma, Mdiff)
         p.kol_amp, p.kol_k = kol_amp, kol_k
         p.form = 1 if form == "lc" else 0
-        p.dt_mode = 0 if dt_mode == "pid" else 1
+        p.dt_mode = {"pid": 0, "fixed": 1, "cfl": 2}[dt_mode]
         lib().or_phys_derive(C.byref(p))
         self.p = p
         return p

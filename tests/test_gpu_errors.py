@@ -95,7 +95,28 @@ def test_ensemble_rejects_mixed_forms():
         s.set_phys(**PH)
         s.set_initial(taylor_green_ic(ns))
         members.append(s)
-    with pytest.raises(sv.SunsetError):
+    with pytest.raises(sv.SunsetError, match="member 1: conformation form"):
+        sv.Ensemble(members)
+    # the message is also left on the first member
+    assert "conformation form" in sv.lib().sv_last_error(members[0].h).decode()
+    for s in members:
+        s.close()
+
+
+def test_ensemble_refuses_mirror_member_beside_member_without_halos():
+    """Mirror flips live in the per-block halo keys, so an ensemble with a
+    mirror-ghost member needs halos in every member; a member built with
+    halo_sweeps off is refused with a message."""
+    sv = _sv()
+    members = []
+    for k, (bc, halo) in enumerate((((2, 2, 1, 1), 1), ((1, 1, 1, 1), 0))):
+        ns = jittered_lattice(12, seed=7 + k, h_over_s=1.8, bc=bc)
+        s = sv.Solver(morder=4, form="lc").setup(ns)
+        s.set_option("halo_sweeps", halo)
+        s.set_phys(**PH)
+        s.set_initial(taylor_green_ic(ns))
+        members.append(s)
+    with pytest.raises(sv.SunsetError, match="mirror ghosts"):
         sv.Ensemble(members)
     for s in members:
         s.close()
