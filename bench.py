@@ -13,12 +13,15 @@ Printed line (one JSON object): value = node-steps/s (nodes x steps / time of
 the timed steps); roofline of the dominant kernel (the RK stage sweep)
 measured live with HIP events on the library's stream; weight solves/s of the
 device weight setup; cpu_baseline = the reference's own OpenMP RK3 loop
-(flang build in oracle/_ref) on a bounded sample of the same workload.
+(flang build in oracle/_ref) on the same 1M-node workload (3 timed steps),
+on the CPUs this job may use; error_vs_oracle of every leg against the FP64
+CPU oracle (and the reference's own fields for the headline run).
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -66,40 +69,55 @@ def step_bytes_per_node(ks_mean, k_mean, halo):
             + 128.0)
 
 
-def cpu_baseline(n_sample, steps, threads):
-    """The reference's RK3 loop (oracle/_ref, flang/OpenMP) on an n_sample^2
-    lattice of the same workload, its neighbour lists and weights loaded from
-    this library's parity-checked ones (the reference's serial weight setup
-    would take minutes).  Returns node-steps/s from the median step time."""
+def host_cpus():
+    """CPUs this process may use (the job's share on a shared GPU host: the
+    OpenMP thread count the host sets, else the affinity mask) and the
+    host's topology from lscpu."""
+    avail = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(int(omp), avail) if omp.isdigit() and int(omp) > 0 else avail
+    topo = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                     "Model name"):
+                topo[k] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    desc = (f"lscpu: {topo.get('Socket(s)', '?')} sockets x {topo.get('Core(s) per socket', '?')}"
+            f" cores x {topo.get('Thread(s) per core', '?')} threads "
+            f"({topo.get('CPU(s)', '?')} CPUs, {topo.get('Model name', '?')}); "
+            f"{avail} CPUs in this process's affinity mask, OMP_NUM_THREADS={omp or 'unset'}")
+    return threads, desc
+
+
+def cpu_baseline(ns, ic, stencil, steps, threads, host):
+    """The reference's own RK3 loop (oracle/_ref, flang/OpenMP, unmodified
+    sources) on the full workload's node set, its neighbour lists and weights
+    loaded from this library's (the reference's serial weight setup would
+    take hours at this size).  The reference's LC right-hand side has no
+    Kolmogorov line (rhs_lc.F90:263), so it runs the unforced flow; the
+    forcing is one multiply-add per node.  Returns the baseline record and
+    the reference's fields after 1 + steps steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
     import refdrv
-    import sunset_amd
-    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
     if not refdrv.available(6):
-        return None
-    ns = jittered_lattice(n_sample, seed=2, h_over_s=2.4)
-    s = sunset_amd.Solver(morder=6, form="lc")
-    s.set_nodes(ns)
-    cnt, cs, link = s.find_neighbours()
-    s.calc_weights()
-    s.calc_sums()
-    s.filter_coefficients()
-    W = s.weights(link.shape[1])
-    s.close()
-    st = dict(nplink=link.shape[1], ij_count=cnt, ij_count_small=cs, ij_link=link, **W)
-    ic = taylor_green_ic(ns)
-    # the reference's LC/DI right-hand sides have no Kolmogorov line (rhs_lc.F90:263);
-    # the forcing is one multiply-add per node and does not change the timing
+        return None, None
     phys = {k: v for k, v in PHYS.items() if not k.startswith("kol")}
-    _, times = refdrv.run_reference(ns, 6, ic, nsteps=steps + 1, phys=phys, stencil=st,
-                                    fields_only=True, threads=threads, timeout=900)
+    out, times = refdrv.run_reference(ns, 6, ic, nsteps=steps + 1, phys=phys, stencil=stencil,
+                                      fields_only=True, threads=threads, timeout=900)
     ts = sorted(times["steps"][1:])
     med = ts[len(ts) // 2]
-    return dict(value=ns.npfb / med, unit="node-steps/s", cores=threads, kind="reference",
-                sample=f"{n_sample}x{n_sample} lattice ({ns.npfb} nodes), same order/physics, "
-                       f"{steps} timed steps after 1 warm-up, median step "
-                       f"{med * 1e3:.1f} ms, OMP_NUM_THREADS={threads}")
+    rec = dict(value=ns.npfb / med, unit="node-steps/s", cores=threads, kind="reference",
+               sample=f"the full workload's {ns.npfb} nodes, same order/physics without the "
+                      f"forcing line, {steps} timed steps after 1 warm-up, median step "
+                      f"{med * 1e3:.1f} ms, OMP_NUM_THREADS={threads} OMP_PROC_BIND=close "
+                      f"OMP_PLACES=cores",
+               host=host)
+    return rec, out
 
 
 FP64_PEAK_TFLOPS = 78.6
@@ -130,7 +148,7 @@ def c3_weights(smin, smax):
     ns, om = cylinder_channel(smin=smin, smax=smax)
     s = sunset_amd.Solver(morder=8, form="lc")
     s.set_nodes(ns)
-    s.find_neighbours(copy=False)
+    lists = s.find_neighbours()     # host copies for the oracle check below
     t_n = s.timing("neighbours")
     out = dict(nodes=ns.npfb, order8=int((om == 8).sum()), order4=int((om == 4).sum()),
                spacing=[smin, smax], neighbour_ms=round(t_n, 2))
@@ -157,6 +175,7 @@ def c3_weights(smin, smax):
     s.calc_sums()
     s.filter_coefficients()
     st = s.stencil_stats()
+    out["error_vs_oracle"] = weights_error_vs_oracle(s, ns, om, lists)
     s.close()
     out.update(K_mean=round(st["K_mean"], 2), K_max=int(st["K_max"]),
                weight_solves_per_s=round(ns.npfb / (out["autotuned_ms"] / 1e3), 1))
@@ -164,6 +183,37 @@ def c3_weights(smin, smax):
           + out["order4"] * weight_flops(4, st["K_mean"]))
     out["fp64"] = fp64_figure(fl, out["autotuned_ms"])
     return out
+
+
+def weights_error_vs_oracle(s, ns, order_map, lists, samples=1000):
+    """Largest per-node relative weight error (max_k |w - w_oracle| / max_k
+    |w_oracle| over the derivative, Laplacian and filter weights) of a strided
+    sample of nodes against the oracle's solve of those nodes on the same
+    neighbour lists (one-sided Jacobi SVD; the reference's own Golub-Reinsch
+    SVD is farther from the exact solution at order 8)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as OL
+    cnt, cs, link = lists
+    nplink = link.shape[1]
+    W = s.weights(nplink)
+    o = OL.OracleSim(ns, 8, nplink=nplink)
+    o.ghosts()
+    o.load_lists(cnt, cs, link)
+    del link
+    n = ns.npfb
+    sample = np.arange(0, n, max(1, n // samples), dtype=np.int32)[:samples]
+    o.weights_subset(sample, order_map)
+    OW = o.weight_arrays()
+    worst = {}
+    for key in ("w_grad", "w_lap", "w_hyp"):
+        a = W[key][sample].reshape(sample.size, -1)
+        b = OW[key][sample].reshape(sample.size, -1)
+        den = np.abs(b).max(axis=1)
+        worst[key] = float((np.abs(a - b).max(axis=1) / den).max())
+    worst["vol"] = float(np.abs(W["vol"][sample] / OW["vol"][sample] - 1.0).max())
+    return dict(max_rel=max(worst.values()), per_array=worst, sample_nodes=int(sample.size),
+                how="strided node sample vs the oracle's solve on the device's lists")
 
 
 def cpu_weights_baseline(n_sample):
@@ -241,6 +291,12 @@ def boundary_leg(n=1024, steps=50, warmup=5):
                 ms_per_step=round(total_ms / steps, 4), stage_ms=round(t_stage, 4))
 
 
+def field_error(F, R, keys=("ro", "rou", "rov", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy")):
+    """max over the fields of max_i |F - R| / max(max_i |R|, 1)."""
+    import numpy as np
+    return max(float(np.abs(F[k] - R[k]).max() / max(np.abs(R[k]).max(), 1.0)) for k in keys)
+
+
 def ensemble_leg(members=32, steps=100, seed=7):
     """32 simulations of different sizes in one context: N_m = round(exp(U(ln 16k,
     ln 256k))) nodes (square jittered lattices), LABFM order 4 or 6, Wi =
@@ -286,11 +342,29 @@ def ensemble_leg(members=32, steps=100, seed=7):
     print(f"[bench] ensemble: {t_ens:.1f} ms for {steps} steps (launch mode {mode}, "
           f"tuning samples {tune} ms)", file=sys.stderr, flush=True)
     ok = all(np.isfinite(E.fields(k)["ro"]).all() for k in range(len(mem)))
+    # the smallest member against its own oracle run over all its steps (the
+    # one-by-one run above, then the ensemble's)
+    km = min(range(len(mem)), key=lambda k: mem[k].npfb)
+    side, order, Wi, beta = specs[km]
+    F = E.fields(km)
+    nsteps = int(F["itime"])
     E.close()
     for s in mem:
         s.close()
     if not ok:
         raise SystemExit("bench.py: non-finite ensemble state")
+    print(f"[bench] ensemble: member {km} ({side}^2, order {order}) vs the oracle, "
+          f"{nsteps} steps", file=sys.stderr, flush=True)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    ns = jittered_lattice(side, seed=100 + km, h_over_s={4: 1.8, 6: 2.4}[order])
+    o = OL.OracleSim(ns, order).setup()
+    o.make_phys(form="lc", Re=1.0, Wi=Wi, beta=beta, eps=0.0, Ma=0.1, kol_amp=1.0,
+                kol_k=4 * math.pi)
+    o.set_initial(taylor_green_ic(ns))
+    o.run(nsteps)
+    R = o.fields()
+    err = field_error(F, R)
     return dict(members=members, nodes=nodes, steps=steps,
                 sizes=[side * side for side, *_ in specs],
                 orders=[o for _, o, _, _ in specs],
@@ -300,7 +374,11 @@ def ensemble_leg(members=32, steps=100, seed=7):
                 launch_tune_ms=tune,
                 node_steps_per_s=round(nodes * steps / (t_ens / 1e3), 1),
                 sequential_node_steps_per_s=round(nodes * steps / (t_seq / 1e3), 1),
-                speedup=round(t_seq / t_ens, 2))
+                speedup=round(t_seq / t_ens, 2),
+                error_vs_oracle=dict(max_rel=err, member=km, nodes=ns.npfb, order=order,
+                                     steps=nsteps,
+                                     how="smallest member's fields vs its oracle run, "
+                                         "max over fields of max|f - f_oracle| / max(|f|, 1)"))
 
 
 def adjoint_leg(n=128, steps=400, every=20):
@@ -308,8 +386,9 @@ def adjoint_leg(n=128, steps=400, every=20):
     Oldroyd-B (Wi = 1, beta = 0.5, Re = 1, Ma = 0.1), fluid at rest plus a
     random-phase velocity perturbation of amplitude 0.01 (seed 5), Kolmogorov
     forcing sin(4 pi y), fixed dt = 2e-5; loss (1/T) sum_t mean(tr c - 2) over
-    400 steps, a checkpoint every 20 steps.  Ratio = device time of the
-    gradient (recompute + adjoint sweep) over the forward rollout."""
+    400 steps, a checkpoint every 20 steps.  Ratio = (forward rollout +
+    checkpoint recompute + adjoint sweep) / forward rollout, device times;
+    vjp_over_forward leaves the forward rollout out of the numerator."""
     import sunset_amd
     from sunset_amd.nodes import jittered_lattice, perturbed_rest_ic
     ns = jittered_lattice(n, seed=5, h_over_s=2.4)
@@ -333,9 +412,32 @@ def adjoint_leg(n=128, steps=400, every=20):
     # initial velocity at fixed density: rou = ro*u, so dL/du = ro dL/drou
     gu = math.sqrt(float(np.dot(ic["ro"] * g["rou"], ic["ro"] * g["rou"])
                          + np.dot(ic["ro"] * g["rov"], ic["ro"] * g["rov"])))
+    # the gradient along a random initial-velocity direction against the
+    # oracle's tangent-linear rollout (forward-mode derivative of its FP64
+    # step) over the same 400 steps
+    print("[bench] adjoint: oracle tangent-linear rollout", file=sys.stderr, flush=True)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    n = ns.npfb
+    rng = np.random.default_rng(21)
+    du, dv = rng.standard_normal(n) * 0.01, rng.standard_normal(n) * 0.01
+    dU = dict(ro=np.zeros(n), rou=ic["ro"] * du, rov=ic["ro"] * dv, cxx=np.zeros(n),
+              cxy=np.zeros(n), cyy=np.zeros(n))
+    dL = sum(float(np.dot(g[k], dU[k])) for k in dU)
+    o = OL.OracleSim(ns, 6).setup()
+    o.make_phys(form="di", dt_mode="fixed", Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
+                kol_amp=1.0, kol_k=4 * math.pi)
+    o.set_initial(ic, dt=2e-5)
+    Lo, dLo, _ = o.rollout_tlm(dU, steps)
     return dict(nodes=ns.npfb, steps=steps, checkpoint_every=every, loss=L, grad_norm=gn,
                 grad_velocity_norm=gu, dL_dWi=pg["Wi"], dL_dbeta=pg["beta"],
-                forward_ms=round(fwd, 2), vjp_ms=round(bwd, 2), ratio=round(bwd / fwd, 3))
+                forward_ms=round(fwd, 2), vjp_ms=round(bwd, 2),
+                ratio=round((fwd + bwd) / fwd, 3), vjp_over_forward=round(bwd / fwd, 3),
+                error_vs_oracle=dict(
+                    directional_derivative_rel=abs(dL - dLo) / abs(dLo),
+                    loss_rel=abs(L - Lo) / abs(Lo), device=dL, oracle_tlm=dLo,
+                    how="<grad, dU> along a random initial-velocity direction vs the "
+                        "oracle's tangent-linear rollout, same 400 steps"))
 
 
 def main():
@@ -344,9 +446,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024, help="lattice side (n^2 nodes)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU reference baseline and the oracle checks")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--graph", action="store_true", help="time hipGraph replay instead")
     ap.add_argument("--c3", choices=("full", "spec", "none"), default="full",
                     help="cylinder-channel weight setup: full = spacing 0.0013..0.0052 "
@@ -369,7 +471,7 @@ def main():
     print(f"[bench] {ns.npfb} nodes: setup", file=sys.stderr, flush=True)
     s = sunset_amd.Solver(morder=6, form="lc")
     s.set_nodes(ns)
-    s.find_neighbours(copy=False)
+    cnt, cs, link = s.find_neighbours()    # host copies: CPU baseline and oracle check
     t_neigh = s.timing("neighbours")
     s.calc_weights()
     t_w = s.timing("weights")
@@ -413,11 +515,48 @@ def main():
                 / (ms_step / 1e3) / 1e9)
 
     cpu = None
+    c2_err = None
+    threads, host = host_cpus()
     if not args.no_cpu:
-        threads = min(len(os.sched_getaffinity(0)), 16)
-        print(f"[bench] cpu baseline: reference on {args.cpu_n}^2, {threads} threads",
+        ic = taylor_green_ic(ns)
+        stencil = dict(nplink=link.shape[1], ij_count=cnt, ij_count_small=cs, ij_link=link,
+                       **s.weights(link.shape[1]))
+        # one RK3 step from the initial state on the device and in the oracle
+        # (run on the device's lists and weights), Kolmogorov forcing included
+        print("[bench] error vs oracle: one step of the workload", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as OL
+        OL.lib().or_set_num_threads(threads)
+        o = OL.OracleSim(ns, 6, nplink=link.shape[1])
+        o.ghosts()
+        o.load_lists(cnt, cs, link)
+        o.load_weights(stencil)
+        o.make_phys(form="lc", **PHYS)
+        o.set_initial(ic)
+        o.run(1)
+        s.set_initial(ic)
+        s.run(1)
+        c2_err = dict(max_rel_one_step=field_error(s.fields(), o.fields()))
+        del o
+        print(f"[bench] cpu baseline: the reference on all {n} nodes, {threads} threads",
               file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        try:
+            cpu, ref = cpu_baseline(ns, ic, stencil, args.cpu_steps, threads, host)
+        except (OSError, RuntimeError, subprocess.SubprocessError) as e:
+            # the device figures stand without the CPU reference; say why it is missing
+            cpu, ref = dict(value=None, error=str(e)[-400:], host=host), None
+        del stencil
+        if ref is not None:
+            # the device on the reference's (unforced) flow, same steps
+            s.set_phys(**{k: v for k, v in PHYS.items() if not k.startswith("kol")})
+            s.set_initial(ic)
+            s.run(args.cpu_steps + 1)
+            c2_err["max_rel_vs_reference"] = field_error(s.fields(), ref)
+            c2_err["reference_steps"] = args.cpu_steps + 1
+        c2_err["how"] = ("max over fields of max|f - f_ref| / max(|f|, 1): one step vs the "
+                         "oracle (forced flow); the reference's timed run (unforced) vs the "
+                         "device's run of the same steps")
+    del link
 
     c3 = None
     if args.c3 != "none":
@@ -428,7 +567,6 @@ def main():
             print(f"[bench] cpu weights baseline: reference on {args.cpu_weights_n}^2 order 8",
                   file=sys.stderr, flush=True)
             c3["cpu_baseline"] = cpu_weights_baseline(args.cpu_weights_n)
-            threads = min(len(os.sched_getaffinity(0)), 16)
             print(f"[bench] cpu weights baseline: OpenMP port on {2 * args.cpu_weights_n}^2, "
                   f"{threads} threads", file=sys.stderr, flush=True)
             c3["cpu_baseline_omp"] = cpu_weights_omp(2 * args.cpu_weights_n, threads)
@@ -466,10 +604,12 @@ def main():
                    "halo_records_per_node": round(st["halo_per_node"], 3),
                    "timing": "hipGraph replay" if args.graph else "eager, HIP events"},
         "node_stage_updates_per_s": round(4 * value, 1),
-        "cpu_node_stage_updates_per_s": round(4 * cpu["value"], 1) if cpu else None,
+        "cpu_node_stage_updates_per_s": (round(4 * cpu["value"], 1)
+                                         if cpu and cpu.get("value") else None),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "error_vs_oracle": c2_err,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu and cpu.get("value") else None,
         "step_hbm_gbs": round(step_gbs, 1),
         "weight_solves_per_s": round(n / (t_w / 1e3), 1),
         "weights_fp64": fp64_figure(n_sys * weight_flops(6, st["Ks_mean"]), t_w),
@@ -480,6 +620,7 @@ def main():
         "ensemble": ens,
         "adjoint": adj,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
+        "adjoint_vjp_over_forward": adj["vjp_over_forward"] if adj else None,
     }
     print(json.dumps(out))
 
