@@ -612,6 +612,7 @@ static void filter_coeff_node(or_domain *d, int i);
  * common_vars (common_vars.F90:87-97). */
 int or_set_stencil(or_domain *d, int nplink, const int *cnt, const int *cnt_small,
                    const int *link) {
+  /* entries past cnt[i] are padding and not read */
   free(d->ij_count); free(d->ij_count_small); free(d->ij_link);
   const size_t L = (size_t)d->npfb * nplink;
   d->nplink = nplink;
@@ -622,10 +623,13 @@ int or_set_stencil(or_domain *d, int nplink, const int *cnt, const int *cnt_smal
   memcpy(d->ij_count, cnt, sizeof(int) * d->npfb);
   memcpy(d->ij_count_small, cnt_small, sizeof(int) * d->npfb);
   memcpy(d->ij_link, link, sizeof(int) * L);
-  for (int i = 0; i < d->npfb; i++)
+  for (int i = 0; i < d->npfb; i++) {
     if (cnt[i] < 0 || cnt[i] > nplink || cnt_small[i] > cnt[i]) return -2;
-  for (size_t k = 0; k < L; k++)
-    if (link[k] < 0 || link[k] >= d->np) return -2;
+    for (int k = 0; k < cnt[i]; k++) {
+      const int j = link[(size_t)i * nplink + k];
+      if (j < 0 || j >= d->np) return -2;
+    }
+  }
   return 0;
 }
 

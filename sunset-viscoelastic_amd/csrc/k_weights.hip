@@ -2,16 +2,18 @@
 // calc_filter_weights :233-394; calc_node_volumes :453-503; calc_labf_sums
 // :396-451; filter_coefficients :1723-1811).
 //
-// One wavefront per moment system.  For node i with LABFM order m (n =
+// Systems are solved in units of G per wavefront (4 of order 4, 2 of order 6,
+// 1 of order 8: one matrix row per lane).  For node i with LABFM order m (n =
 // (m^2+3m)/2 unknowns) and stencil length K:
 //   1. the moment matrix A = sum_k X_k g_k^T (X_k: Taylor monomials of the
 //      scaled offset, g_k: Hermite ABFs times the Wendland C2 kernel) is
 //      contracted on the FP64 matrix cores, v_mfma_f64_16x16x4_f64, with
-//      32-neighbour chunks of X and g staged in LDS;
-//   2. A is factorised in LDS by LU with partial pivoting (one column per lane);
+//      16- or 32-neighbour chunks of X and g staged in LDS;
+//   2. A is factorised by Gaussian elimination with partial pivoting with
+//      each row in one lane's registers (pivot search by DPP reductions,
+//      pivot row broadcast through LDS);
 //   3. the right-hand sides (d/dx, d/dy, Laplacian, hyperviscous filter) are
-//      solved, one matrix row per lane with the substitution's values
-//      broadcast through the scalar unit, and improved by two steps of
+//      solved by substitution in pivot order and improved by one step of
 //      iterative refinement with the residual accumulated in double-double,
 //      which brings the solution to the exact solution of the FP64 matrix (the
 //      reference's Golub-Reinsch SVD, svdlib.F90, is only within cond(A)*eps
@@ -106,14 +108,14 @@ __device__ __forceinline__ void abf_row(double x, double y, double w, double *ds
     }
 }
 
-// psi . g for 4 right-hand sides, g evaluated on the fly (psi row-major [n][4])
+// psi . g for one right-hand side, g evaluated on the fly: sum over the
+// unknowns in order of psi[a] * (He_p(x) He_q(y) w)
 template <int M>
-__device__ __forceinline__ void abf_dot4(double x, double y, double w, const double *psi,
-                                         double *out) {
+__device__ __forceinline__ double abf_dot(double x, double y, double w, const double *psi) {
   double hx[M + 1], hy[M + 1];
   hermite_all<M>(x, hx);
   hermite_all<M>(y, hy);
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  double acc = 0.0;
   int idx = 0;
 #pragma unroll
   for (int d = 1; d <= M; d++)
@@ -121,13 +123,10 @@ __device__ __forceinline__ void abf_dot4(double x, double y, double w, const dou
     for (int b = 0; b <= d; b++) {
       const int a = d - b;
       const double v = ((b == 0) ? hx[a] : (a == 0) ? hy[b] : hx[a] * hy[b]) * w;
-      a0 += psi[idx * 4 + 0] * v;
-      a1 += psi[idx * 4 + 1] * v;
-      a2 += psi[idx * 4 + 2] * v;
-      a3 += psi[idx * 4 + 3] * v;
+      acc += psi[idx] * v;
       idx++;
     }
-  out[0] = a0; out[1] = a1; out[2] = a2; out[3] = a3;
+  return acc;
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
@@ -158,78 +157,49 @@ struct SysArgs {
   double h2;
   double *wx, *wy, *wl, *wh, *vol;
   int *singular;
-  int *queue;                // persistent launch: next system to take
+  int *queue;                // persistent launches: next unit to take
   const int *sys_order;      // mixed-order launch: LABFM order of each system
+  const int *unit_first;     // mixed-order launch: first system of each unit
+  int nunits;                //   (a unit: up to G_m systems of one order m)
 };
 
-// Waves per SIMD the register budget of the solve kernels is sized for
-// (compile-time tuning; 0 = the per-order default below).
-#ifndef SV_WEIGHTS_WPE
-#define SV_WEIGHTS_WPE 0
-#endif
-// Measured on MI355X (1M order-6 systems): a 2-wave budget is 13% faster than
-// the compiler's default (1 wave, ~290 registers).  Order 8 keeps the full
-// budget: its LDS allows one wave per SIMD, and 2 waves would spill.
+// Packing of systems into a wavefront.  The factorisation and the
+// substitutions hold one matrix row per lane, so a wavefront takes G = 64 / S
+// systems of the same order, each on a group of S lanes (the smallest of 16,
+// 32, 64 that holds the N rows): 4 order-4, 2 order-6 or 1 order-8 system.
 template <int M>
-struct RegOcc {
-  static constexpr int W = SV_WEIGHTS_WPE > 0 ? SV_WEIGHTS_WPE : (M <= 6 ? 2 : 1);
+struct Pack {
+  static constexpr int N = nsize(M);
+  static constexpr int S = N <= 16 ? 16 : N <= 32 ? 32 : 64;
+  static constexpr int G = SV_WAVE / S;
+  // the original matrix (for the refinement residual) stays in registers when
+  // one system fills the wavefront, else in LDS
+  static constexpr bool A0_REGS = (G == 1);
+  // register budget: waves per SIMD (order 8 holds two 44-double rows)
+  static constexpr int W = M <= 6 ? 2 : 1;
 };
 
-// LDS of one system (doubles): the assembly's CH-neighbour chunks of X and g
-// rows ([CH][NP] each), then, in the same space, the LU workspace A
-// ([N][LDA], LDA odd against bank conflicts of column reads); after them the
-// original matrix A0 ([N][LDA], for the refinement residuals) and the
-// solution rows Xs ([NP][4]).  The right-hand sides live in registers, one
-// row per lane.  CH = 16 or 32 is the launch autotuner's tile choice (16:
-// order 8 in 33 KB, 4 waves per CU; 32: fewer chunk rounds, 42 KB).
+// LDS of one wavefront (doubles):
+//   A0  [G][N][LDA]  each system's moment matrix as assembled (row-major; LDA
+//                    odd, so lanes reading different rows hit different banks);
+//                    with A0_REGS only one matrix, needed until the rows are
+//                    in registers
+//   C   the assembly's CH-neighbour chunks of monomial and ABF rows
+//       ([2][CH][NP]); after the assembly the same space holds the pivot-row
+//       broadcast ([G][NP]) during the factorisation and then the
+//       substitution values and the solution Xs ([G][NP][4])
 template <int M, int CH_>
 struct Lds {
   static constexpr int N = nsize(M), NP = npad(M), LDA = (N % 2 == 1) ? N : N + 1;
   static constexpr int CH = CH_;
-  static constexpr int CHUNK = 2 * CH * NP;
+  static constexpr int G = Pack<M>::G;
   static constexpr int ASZ = N * LDA;
-  static constexpr int U0 = CHUNK > ASZ ? CHUNK : ASZ;
-  static constexpr int TOTAL = U0 + ASZ + NP * 4;
+  static constexpr int NA = Pack<M>::A0_REGS ? 1 : G;
+  static constexpr int CHUNK = 2 * CH * NP;
+  static constexpr int XS = G * NP * 4;
+  static constexpr int CSZ = CHUNK > XS ? CHUNK : XS;
+  static constexpr int TOTAL = NA * ASZ + CSZ;
 };
-
-// Value of v in lane k (k wave-uniform), through the scalar unit.
-__device__ __forceinline__ double lane_bcast(double v, int k) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
-  return __hiloint2double(hi, lo);
-}
-
-// Solve L U x = s for 4 right-hand sides held one row per lane (lane r < N
-// holds row r of the permuted right-hand side in s[0..3]); L (unit lower) and
-// U are the LU factors in A ([NP][LDA], LDS), dinv (lane r) = 1/U[r][r].
-// Row k's value is broadcast from lane k through the scalar unit, so the
-// substitution has no LDS round trip and no barrier in its dependency chain;
-// the factor entries each lane needs are plain LDS reads off that chain.  On
-// return s holds row r of x.
-template <int N, int LDA>
-__device__ __forceinline__ void lu_solve_rows(const double *A, int r, double dinv,
-                                              double s[4]) {
-#pragma unroll 4
-  for (int k = 0; k < N - 1; k++) {
-    const double lrk = (r > k && r < N) ? A[r * LDA + k] : 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const double y = lane_bcast(s[q], k);
-      if (r > k) s[q] -= lrk * y;
-    }
-  }
-#pragma unroll 4
-  for (int k = N - 1; k >= 0; k--) {
-    const double dk = lane_bcast(dinv, k);
-    const double urk = (r < k) ? A[r * LDA + k] : 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const double x = lane_bcast(s[q], k) * dk;
-      if (r == k) s[q] = x;
-      else if (r < k) s[q] -= urk * x;
-    }
-  }
-}
 
 // Moment matrix A = sum_k X_k g_k^T (labf.F90:88-123) on the FP64 matrix
 // cores: 32-neighbour chunks of Taylor monomial rows X and ABF rows g are
@@ -285,8 +255,7 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
   }
   // D layout of a 16x16 f64 tile (v_mfma_f64_16x16x4_f64): lane l holds rows
   // (l>>4) + 4v, v = 0..3, of column l&15; the padding rows and columns
-  // (>= N) are dropped.  Adst may alias the chunks: every lane is past its
-  // last chunk read (the barrier above).
+  // (>= N) are dropped.
 #pragma unroll
   for (int ti = 0; ti < T; ti++)
 #pragma unroll
@@ -299,33 +268,32 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
   __syncthreads();
 }
 
-// Weights w_k = (psi . g_k)/hh, one neighbour per lane (labf.F90:183-215, k <=
-// ij_count), and the node volume (calc_node_volumes, labf.F90:453-503).
+// Weights w_k = (psi_q . g_k)/hh (labf.F90:183-215, k <= ij_count) and the
+// node volume (calc_node_volumes, labf.F90:453-503).  Four lanes per
+// neighbour, lane q = l & 3 for right-hand side q (d/dx, d/dy, Laplacian,
+// filter), 16 neighbours per pass: each lane keeps its column of psi in
+// registers for the whole stencil instead of re-reading 4n values from LDS
+// per neighbour.
 template <int M>
 __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD, bool sysF,
                                              double hh, int Kfull, int64_t base,
                                              const double *Xs) {
-  const int l = threadIdx.x;
+  constexpr int N = nsize(M);
+  const int l = threadIdx.x, q = l & 3, kl = l >> 2;
   const double xi = S_.x[i], yi = S_.y[i];
+  double ps[N];
+#pragma unroll
+  for (int a = 0; a < N; a++) ps[a] = Xs[a * 4 + q];
+  const bool want = q < 3 ? sysD : sysF;
+  double *dst = q == 0 ? S_.wx : q == 1 ? S_.wy : q == 2 ? S_.wl : S_.wh;
   double vsum = 0.0;
-  for (int k = l; k < Kfull; k += SV_WAVE) {
-    // psi is re-read from LDS each pass (1-3 passes): keeping all 4n values of
-    // it in registers across the loop would cost up to 350 VGPRs
-    __asm__ volatile("" ::: "memory");
+  for (int k = kl; k < Kfull; k += SV_WAVE / 4) {
     const int j = S_.col[base + k];
     const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
     const double rad = sqrt(x * x + y * y) / hh;
-    double o[4];
-    abf_dot4<M>(x / hh, y / hh, wab(rad, S_.h2), Xs, o);
-    if (sysD) {
-      S_.wx[base + k] = o[0] / hh;
-      S_.wy[base + k] = o[1] / hh;
-      S_.wl[base + k] = o[2] / hh / hh;
-    }
-    if (sysF) {
-      S_.wh[base + k] = o[3];
-      vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
-    }
+    const double o = abf_dot<M>(x / hh, y / hh, wab(rad, S_.h2), ps);
+    if (want) dst[base + k] = q < 2 ? o / hh : q == 2 ? o / hh / hh : o;
+    if (q == 3 && sysF) vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
   }
   if (sysF) {
 #pragma unroll
@@ -334,190 +302,297 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD
   }
 }
 
-// Solve one system; the block is one wavefront.
-template <int M, int CH_>
-__device__ void solve_system(const SysArgs &S_, int sys, double *lds) {
-  using L = Lds<M, CH_>;
-  constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH;
-  double *Xc = lds;                 // [CH][NP]
-  double *Gc = lds + CH * NP;       // [CH][NP]
-  double *A = lds;                  // [N][LDA] LU workspace (the chunks' space)
-  double *A0 = lds + L::U0;         // [N][LDA] original matrix
-  double *Xs = A0 + L::ASZ;         // [NP][4] solution
-  const int l = threadIdx.x;
-  const int i = S_.sys_node[sys];
-  const int kind = S_.sys_kind[sys];
-  const bool sysD = kind & SYS_D, sysF = kind & SYS_F;
-  const double hh = sysD ? S_.hs[i] : S_.h[i];
-  const int Kuse = sysD ? S_.cnt_small[i] : S_.cnt[i];
-  const int Kfull = S_.cnt[i];
-  const int64_t base = S_.row_ptr[i];
+// One lane's double from lane (l xor 1 / 2 ...) pattern CTRL within its row of
+// 16 (DPP; every source lane is inside the row)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
 
-  assemble<M, CH_>(S_, i, hh, Kuse, base, Xc, Gc, A);
+// Maximum of v over each group of S consecutive lanes, in every lane: four
+// DPP steps inside each row of 16 (quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror, row_mirror), then swaps across rows for S = 32, 64.
+template <int S>
+__device__ __forceinline__ double group_max(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));
+  v = fmax(v, dpp_d<0x4E>(v));
+  v = fmax(v, dpp_d<0x141>(v));
+  v = fmax(v, dpp_d<0x140>(v));
+  if (S >= 32) v = fmax(v, __shfl_xor(v, 16));
+  if (S >= 64) v = fmax(v, __shfl_xor(v, 32));
+  return v;
+}
+
+// Right-hand sides of row r (q0 d/dx, q1 d/dy, q2 Laplacian e3+e5, q3 filter)
+__device__ __forceinline__ void rhs_row(int r, bool sysD, bool sysF, int mf, double b[4]) {
+  b[0] = (sysD && r == 0) ? 1.0 : 0.0;
+  b[1] = (sysD && r == 1) ? 1.0 : 0.0;
+  b[2] = (sysD && (r == 2 || r == 4)) ? 1.0 : 0.0;
+  b[3] = sysF ? hyp_rhs(mf, r) : 0.0;
+}
+
+// The system of group g of a unit (G consecutive entries of the sorted list)
+struct SysOf {
+  int i, kind, Kuse, Kfull, nkeep, mf;
+  bool sysD, sysF;
+  double hh;
+  int64_t base;
+};
+
+template <int M>
+__device__ __forceinline__ SysOf sys_of(const SysArgs &S_, int sys) {
+  constexpr int N = nsize(M);
+  SysOf o;
+  o.i = S_.sys_node[sys];
+  o.kind = S_.sys_kind[sys];
+  o.sysD = o.kind & SYS_D;
+  o.sysF = o.kind & SYS_F;
+  o.hh = o.sysD ? S_.hs[o.i] : S_.h[o.i];
+  o.Kuse = o.sysD ? S_.cnt_small[o.i] : S_.cnt[o.i];
+  o.Kfull = S_.cnt[o.i];
+  o.base = S_.row_ptr[o.i];
   // order reduction near walls by identity padding (labf.F90:126-148, :311-350)
-  const int nt = S_.node_type[i];
-  int nkeep = N, mf = M;
-  if (M >= 7 && (nt == 998 || nt < 0)) { nkeep = 27; mf = 6; }
-  if (M >= 5 && sysD && (nt == -1 || nt == -2)) nkeep = 14;
-  if (M >= 5 && sysF && nt == -1) { nkeep = 14; mf = 4; }
-  if (nkeep < N && l < N) {
-    for (int r = 0; r < N; r++)
-      if (r >= nkeep || l >= nkeep) A[r * LDA + l] = (r == l) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int e = l; e < N * LDA; e += SV_WAVE) A0[e] = A[e];
-  // right-hand sides of row l: q0 d/dx, q1 d/dy, q2 Laplacian (e3+e5), q3 filter
-  double b[4] = {0.0, 0.0, 0.0, 0.0};
-  if (l < N) {
-    b[0] = (sysD && l == 0) ? 1.0 : 0.0;
-    b[1] = (sysD && l == 1) ? 1.0 : 0.0;
-    b[2] = (sysD && (l == 2 || l == 4)) ? 1.0 : 0.0;
-    b[3] = sysF ? hyp_rhs(mf, l) : 0.0;
-  }
-  for (int e = l; e < NP * 4; e += SV_WAVE) Xs[e] = 0.0;
-  __syncthreads();
+  const int nt = S_.node_type[o.i];
+  o.nkeep = N;
+  o.mf = M;
+  if (M >= 7 && (nt == 998 || nt < 0)) { o.nkeep = 27; o.mf = 6; }
+  if (M >= 5 && o.sysD && (nt == -1 || nt == -2)) o.nkeep = 14;
+  if (M >= 5 && o.sysF && nt == -1) { o.nkeep = 14; o.mf = 4; }
+  return o;
+}
 
-  // ---- 2. LU with partial pivoting, one column per lane ----
-  // idx: the row permutation, one row per lane (row l of P A is row idx of A)
+// Solve the `count` (<= G) systems first .. first+count-1 of one order; the
+// block is one wavefront.
+//  1. each moment matrix A = sum_k X_k g_k^T (labf.F90:88-123) is contracted on
+//     the matrix cores into LDS (assemble), rows and columns beyond nkeep set
+//     to the identity;
+//  2. lane r of group g takes row r of system g into registers and the group
+//     factorises by Gaussian elimination with partial pivoting, rows staying
+//     in their lanes: at step k the group's largest |a[k]| among the rows not
+//     yet chosen (lowest row on ties) is the pivot, its row is broadcast
+//     through LDS, and every other open row stores its multiplier in a[k] and
+//     updates a[k+1..N-1];
+//  3. the four right-hand sides (one row per lane) are solved by forward and
+//     backward substitution in pivot order, each step's value broadcast
+//     through LDS, then improved by one refinement step whose residual b - A x
+//     is accumulated in double-double (the reference's Golub-Reinsch SVD,
+//     svdlib.F90, is only within cond(A)*eps of the exact solution of the FP64
+//     matrix; the refined solution is at it to rounding);
+//  4. the weights of each system, one neighbour per lane (eval_weights).
+template <int M, int CH_>
+__device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int count, double *lds) {
+  using P = Pack<M>;
+  using L = Lds<M, CH_>;
+  constexpr int N = P::N, S = P::S, G = P::G, LDA = L::LDA, NP = L::NP, ASZ = L::ASZ;
+  double *A0 = lds;                     // [NA][N][LDA]
+  double *C = lds + L::NA * ASZ;        // chunks, then the broadcast / solution area
+  double *Ub = C;                       // [G][NP]: pivot row, its reciprocal at index N
+  double *Xs = C;                       // [G][NP][4]
+  const int l = threadIdx.x, g = l / S, r = l % S;
+  const bool active = g < count;
+  const bool row = active && r < N;
+
+  // ---- 1. assembly, one system after the other ----
+  // (A0_REGS: G == 1, one system; its matrix goes to registers twice: the
+  // row to factorise and the original for the residual)
+  const int nasm = P::A0_REGS ? 1 : count;
+  for (int gg = 0; gg < nasm; gg++) {
+    const SysOf o = sys_of<M>(S_, first + gg);
+    double *Ag = A0 + gg * ASZ;
+    assemble<M, CH_>(S_, o.i, o.hh, o.Kuse, o.base, C, C + CH_ * NP, Ag);
+    if (o.nkeep < N && l < N) {
+      // a rolled loop: unrolled, its N identity entries would be hoisted to
+      // the kernel entry and held in registers throughout
+#pragma unroll 1
+      for (int rr = 0; rr < N; rr++)
+        if (rr >= o.nkeep || l >= o.nkeep) Ag[rr * LDA + l] = (rr == l) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+  double a[N];
+  double a0[P::A0_REGS ? N : 1];
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    a[c] = row ? A0[g * ASZ + r * LDA + c] : 0.0;
+    if (P::A0_REGS) a0[c] = a[c];
+    if (c % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+  const SysOf o = sys_of<M>(S_, first + (active ? g : 0));
+  __syncthreads();   // the chunks' space becomes the broadcast area
+
+  // ---- 2. elimination with partial pivoting, rows in their lanes ----
+  int ps = row ? -1 : N;     // step at which this row was the pivot (N: padding lane)
+  double rinv = 1.0;         // 1 / pivot of this row
   bool sing = false;
-  int idx = l;
+  const int gbase = g * S;
+#pragma unroll
   for (int k = 0; k < N; k++) {
-    double v = (l >= k && l < N) ? fabs(A[l * LDA + k]) : -1.0;
-    int p = l;
+    const bool open = ps < 0;
+    double cand = open ? fabs(a[k]) : -1.0;
+    if (cand != cand) cand = 0.0;   // a NaN entry still yields a pivot (flagged below)
+    const double mx = group_max<S>(cand);
+    const unsigned long long bal = __ballot(open && cand == mx);
+    unsigned long long mg = bal >> gbase;
+    if (S < 64) mg &= (1ull << S) - 1ull;
+    const int p = mg ? __builtin_ctzll(mg) : S;
+    const bool piv = open && r == p;
+    if (piv) {
+      ps = k;
+      sing = !(a[k] != 0.0);   // zero or NaN
+      rinv = 1.0 / (a[k] == 0.0 ? 1.0 : a[k]);
+      double *u = Ub + g * NP;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const double vo = __shfl_xor(v, o);
-      const int po = __shfl_xor(p, o);
-      if (vo > v || (vo == v && po < p)) { v = vo; p = po; }
-    }
-    if (v == 0.0) sing = true;
-    p = __builtin_amdgcn_readfirstlane(p);   // equal in every lane after the reduction
-    {
-      const int ik = __builtin_amdgcn_readlane(idx, k), ip = __builtin_amdgcn_readlane(idx, p);
-      if (l == k) idx = ip;
-      else if (l == p) idx = ik;
-    }
-    if (p != k && l < N) {
-      const double t = A[k * LDA + l];
-      A[k * LDA + l] = A[p * LDA + l];
-      A[p * LDA + l] = t;
+      for (int j = k + 1; j < N; j++) u[j] = a[j];
+      u[N] = rinv;
     }
     __syncthreads();
-    const double pv = A[k * LDA + k];
-    const double pivot = (pv == 0.0) ? 1.0 : pv;
-    if (l > k && l < N) A[l * LDA + k] = A[l * LDA + k] / pivot;
-    __syncthreads();
-    // rank-1 update of column l, rows in batches of 4 so that each lane has
-    // 8 independent LDS reads in flight
-    if (l > k && l < N) {
-      const double akc = A[k * LDA + l];
-      int r = k + 1;
-      for (; r + 4 <= N; r += 4) {
-        double m[4], a[4];
+    if (open && !piv) {
+      const double *u = Ub + g * NP;
+      const double lk = a[k] * u[N];
+      a[k] = lk;
+      // batches of 8 loads in flight (a full row of them would cost registers)
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          m[u] = A[(r + u) * LDA + k];
-          a[u] = A[(r + u) * LDA + l];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) A[(r + u) * LDA + l] = a[u] - m[u] * akc;
+      for (int j = k + 1; j < N; j++) {
+        a[j] = fma(-lk, u[j], a[j]);
+        if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
       }
-      for (; r < N; r++) A[r * LDA + l] -= A[r * LDA + k] * akc;
     }
     __syncthreads();
   }
 
-  // ---- 3. solve, then two refinement steps with a double-double residual ----
-  // One row per lane: lane r < N keeps row r of the solution x (xs) and of the
-  // residual (res) in registers; Xs (LDS) holds x for the residual's
-  // broadcast reads and for the weight evaluation.
-  const int r = l;
-  double dinv = 1.0;
-  if (r < N) {
-    const double d = A[r * LDA + r];
-    dinv = 1.0 / (d == 0.0 ? 1.0 : d);
-  }
-  double s4[4], xs[4], res[4];
+  // ---- 3. substitutions in pivot order, one refinement ----
+  double b[4];
+  rhs_row(r, o.sysD, o.sysF, o.mf, b);
+  double *xg = Xs + g * NP * 4;
+  auto substitute = [&](double s[4]) {
+    // forward: L y = P s; step k's value y_k is row ps == k's entry
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    s4[q] = __shfl(b[q], idx);   // row r of P b is row idx of b (lane idx)
-    xs[q] = 0.0;
-  }
-  for (int it = 0; it < 3; it++) {
-    if (it > 0) {
-      // res_r = b_r - sum_c A0[r][c] x_c, accumulated in double-double
-      double sh[4] = {0, 0, 0, 0}, sl[4] = {0, 0, 0, 0};
-      if (r < N)
-        for (int cc = 0; cc < N; cc++) {
-          const double a = A0[r * LDA + cc];
-          const double2 x01 = reinterpret_cast<const double2 *>(Xs + cc * 4)[0];
-          const double2 x23 = reinterpret_cast<const double2 *>(Xs + cc * 4)[1];
-          const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const double pr = a * xv[q];
-            const double pe = fma(a, xv[q], -pr);
-            double t;
-            two_sum(sh[q], pr, sh[q], t);
-            sl[q] += t + pe;
-          }
-        }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        double d, de;
-        two_sum(b[q], -sh[q], d, de);
-        res[q] = d + (de - sl[q]);
+    for (int k = 0; k < N; k++) {
+      if (ps == k) {
+        double2 *d = reinterpret_cast<double2 *>(xg + k * 4);
+        d[0] = double2{s[0], s[1]};
+        d[1] = double2{s[2], s[3]};
       }
-      // permuted residual: row r of P res is row idx of res (lane idx)
-#pragma unroll
-      for (int q = 0; q < 4; q++) s4[q] = __shfl(res[q], idx);
+      __syncthreads();
+      if (ps > k && ps < N) {
+        const double2 *d = reinterpret_cast<const double2 *>(xg + k * 4);
+        const double2 y01 = d[0], y23 = d[1];
+        s[0] = fma(-a[k], y01.x, s[0]);
+        s[1] = fma(-a[k], y01.y, s[1]);
+        s[2] = fma(-a[k], y23.x, s[2]);
+        s[3] = fma(-a[k], y23.y, s[3]);
+      }
+      __syncthreads();
     }
-    lu_solve_rows<N, LDA>(A, r, dinv, s4);
+    // backward: U x = y; row ps == k finishes x_k (column k), the rows
+    // pivoted before it take its term
 #pragma unroll
-    for (int q = 0; q < 4; q++) xs[q] = (it == 0) ? s4[q] : xs[q] + s4[q];
-    __syncthreads();   // every lane's residual reads of Xs are done
-    if (r < N) {
-      double2 *X2 = reinterpret_cast<double2 *>(Xs + r * 4);
-      X2[0] = double2{xs[0], xs[1]};
-      X2[1] = double2{xs[2], xs[3]};
+    for (int k = N - 1; k >= 0; k--) {
+      if (ps == k) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) s[q] *= rinv;
+        double2 *d = reinterpret_cast<double2 *>(xg + k * 4);
+        d[0] = double2{s[0], s[1]};
+        d[1] = double2{s[2], s[3]};
+      }
+      __syncthreads();
+      if (ps < k) {
+        const double2 *d = reinterpret_cast<const double2 *>(xg + k * 4);
+        const double2 x01 = d[0], x23 = d[1];
+        s[0] = fma(-a[k], x01.x, s[0]);
+        s[1] = fma(-a[k], x01.y, s[1]);
+        s[2] = fma(-a[k], x23.x, s[2]);
+        s[3] = fma(-a[k], x23.y, s[3]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  };
+  double xk[4] = {b[0], b[1], b[2], b[3]};
+  substitute(xk);   // row ps == k now holds x_k; xg[k] = x_k for every column k
+  // residual of row r: b_r - sum_c A0[r][c] x_c in double-double
+  double res[4];
+  {
+    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row) {
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        const double av = P::A0_REGS ? a0[c] : A0[g * ASZ + r * LDA + c];
+        const double2 *d = reinterpret_cast<const double2 *>(xg + c * 4);
+        const double2 x01 = d[0], x23 = d[1];
+        const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double pr = av * xv[q];
+          const double pe = fma(av, xv[q], -pr);
+          double t;
+          two_sum(sh[q], pr, sh[q], t);
+          sl[q] += t + pe;
+        }
+        if (c % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      double d, de;
+      two_sum(b[q], -sh[q], d, de);
+      res[q] = d + (de - sl[q]);
+    }
   }
+  __syncthreads();   // every residual read of xg is done
+  substitute(res);   // row ps == k holds the correction of x_k
+  if (ps >= 0 && ps < N) {
+    double2 *d = reinterpret_cast<double2 *>(xg + ps * 4);
+    d[0] = double2{xk[0] + res[0], xk[1] + res[1]};
+    d[1] = double2{xk[2] + res[2], xk[3] + res[3]};
+  }
+  if (sing && row) atomicAdd(S_.singular, 1);
+  __syncthreads();
 
-  eval_weights<M>(S_, i, sysD, sysF, hh, Kfull, base, Xs);
-  if (sing && l == 0) atomicAdd(S_.singular, 1);
+  // ---- 4. weights, one system after the other ----
+  for (int gg = 0; gg < count; gg++) {
+    const SysOf og = sys_of<M>(S_, first + gg);
+    eval_weights<M>(S_, og.i, og.sysD, og.sysF, og.hh, og.Kfull, og.base, Xs + gg * NP * 4);
+  }
   __syncthreads();
 }
 
+// Systems [u*G, u*G + G) of this launch's list per block
 template <int M, int CH_>
-__global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights(SysArgs a) {
+__global__ __launch_bounds__(SV_WAVE, Pack<M>::W) void k_weights(SysArgs a) {
   extern __shared__ double lds[];
-  const int sys = blockIdx.x;
-  if (sys >= a.nsys) return;
-  solve_system<M, CH_>(a, sys, lds);
+  constexpr int G = Pack<M>::G;
+  const int first = blockIdx.x * G;
+  if (first >= a.nsys) return;
+  solve_unit<M, CH_>(a, first, min(G, a.nsys - first), lds);
 }
 
-// Persistent variant: a fixed grid takes systems from a queue; systems are
-// sorted by (order, stencil length) so every wave works on the longest
-// remaining systems first.
+// Persistent variant: a fixed grid takes units of G systems from a queue;
+// systems are sorted by (order, stencil length) so every wave works on the
+// longest remaining systems first.
+// (one wave per SIMD: with the unit loop around the solve, the two-wave
+// register budget of k_weights would spill)
 template <int M, int CH_>
-__global__ __launch_bounds__(SV_WAVE, RegOcc<M>::W) void k_weights_persistent(SysArgs a) {
+__global__ __launch_bounds__(SV_WAVE, 1) void k_weights_persistent(SysArgs a) {
   extern __shared__ double lds[];
   __shared__ int next;
+  constexpr int G = Pack<M>::G;
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
     __syncthreads();
-    const int sys = next;
+    const int first = next * G;
     __syncthreads();
-    if (sys >= a.nsys) break;
-    solve_system<M, CH_>(a, sys, lds);
+    if (first >= a.nsys) break;
+    solve_unit<M, CH_>(a, first, min(G, a.nsys - first), lds);
   }
 }
 
 // One persistent launch over the systems of every order: the queue holds
-// them order 8 first, then 6, then 4, each by decreasing stencil length, so
-// the grid drains the most expensive systems first and the cheap ones fill
-// the tail.  LDS is sized for the largest order present.
+// units (each G_m systems of one order m, unit_first[u] its first system) in
+// the order 8 first, then 6, then 4, each by decreasing stencil length, so the
+// grid drains the most expensive systems first and the cheap ones fill the
+// tail.  LDS is sized for the largest need of the orders present.
 template <int CH_>
 __global__ __launch_bounds__(SV_WAVE, 1) void k_weights_mixed(SysArgs a) {
   extern __shared__ double lds[];
@@ -525,13 +600,15 @@ __global__ __launch_bounds__(SV_WAVE, 1) void k_weights_mixed(SysArgs a) {
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
     __syncthreads();
-    const int sys = next;
+    const int u = next;
     __syncthreads();
-    if (sys >= a.nsys) break;
-    const int m = a.sys_order[sys];
-    if (m == 8) solve_system<8, CH_>(a, sys, lds);
-    else if (m == 6) solve_system<6, CH_>(a, sys, lds);
-    else solve_system<4, CH_>(a, sys, lds);
+    if (u >= a.nunits) break;
+    const int first = a.unit_first[u];
+    const int count = (u + 1 < a.nunits ? a.unit_first[u + 1] : a.nsys) - first;
+    const int m = a.sys_order[first];
+    if (m == 8) solve_unit<8, CH_>(a, first, count, lds);
+    else if (m == 6) solve_unit<6, CH_>(a, first, count, lds);
+    else solve_unit<4, CH_>(a, first, count, lds);
   }
 }
 
@@ -623,11 +700,31 @@ __global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt
 #define SV_TUNE_SAMPLE 4096
 #define SV_TUNE_VARIANTS 4
 
+// systems per wavefront of order m (Pack<m>::G)
+int pack_of(int m) { return m == 8 ? Pack<8>::G : m == 6 ? Pack<6>::G : Pack<4>::G; }
+
+// Units of a list sorted by order: a new unit at every change of order and
+// after every G_m systems of order m.
+std::vector<int> units_of(const std::vector<int> &order) {
+  std::vector<int> u;
+  int run = 0;
+  for (size_t q = 0; q < order.size(); q++) {
+    if (q == 0 || order[q] != order[q - 1] || run == pack_of(order[q])) {
+      u.push_back((int)q);
+      run = 0;
+    }
+    run++;
+  }
+  return u;
+}
+
 template <int M, int CH>
 int launch_tiles(sv_ctx *c, const SysArgs &b, int count, bool persistent) {
   const size_t shm = sizeof(double) * Lds<M, CH>::TOTAL;
+  constexpr int G = Pack<M>::G;
+  const int units = (count + G - 1) / G;
   if (!persistent) {
-    k_weights<M, CH><<<count, SV_WAVE, shm, c->stream>>>(b);
+    k_weights<M, CH><<<units, SV_WAVE, shm, c->stream>>>(b);
   } else {
     int dev = c->device, ncu = 0, occ = 0;
     SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -635,7 +732,7 @@ int launch_tiles(sv_ctx *c, const SysArgs &b, int count, bool persistent) {
                                                             SV_WAVE, shm));
     if (occ < 1) occ = 1;
     int grid = ncu * occ;
-    if (grid > count) grid = count;
+    if (grid > units) grid = units;
     SV_HIP(c, hipMemsetAsync(b.queue, 0, sizeof(int), c->stream));
     k_weights_persistent<M, CH><<<grid, SV_WAVE, shm, c->stream>>>(b);
   }
@@ -658,8 +755,11 @@ int launch_order(sv_ctx *c, SysArgs a, int first, int count, int variant) {
 }
 
 template <int CH>
-int launch_mixed_tiles(sv_ctx *c, const SysArgs &b, int count, int mmax) {
-  const int tot = mmax == 8 ? Lds<8, CH>::TOTAL : mmax == 6 ? Lds<6, CH>::TOTAL : Lds<4, CH>::TOTAL;
+int launch_mixed_tiles(sv_ctx *c, const SysArgs &b, int mmax) {
+  // LDS of the largest need among the orders up to mmax
+  int tot = Lds<4, CH>::TOTAL;
+  if (mmax >= 6) tot = std::max(tot, (int)Lds<6, CH>::TOTAL);
+  if (mmax >= 8) tot = std::max(tot, (int)Lds<8, CH>::TOTAL);
   const size_t shm = sizeof(double) * tot;
   int dev = c->device, ncu = 0, occ = 0;
   SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -667,20 +767,20 @@ int launch_mixed_tiles(sv_ctx *c, const SysArgs &b, int count, int mmax) {
                                                           shm));
   if (occ < 1) occ = 1;
   int grid = ncu * occ;
-  if (grid > count) grid = count;
+  if (grid > b.nunits) grid = b.nunits;
   SV_HIP(c, hipMemsetAsync(b.queue, 0, sizeof(int), c->stream));
   k_weights_mixed<CH><<<grid, SV_WAVE, shm, c->stream>>>(b);
   SV_HIP(c, hipGetLastError());
   return 0;
 }
 
-// mode 1: one mixed-order persistent launch with 16-neighbour tiles, 2: with 32
+// mode 1: one mixed-order persistent launch with 16-neighbour tiles, 2: with
+// 32; a.unit_first / a.nunits: the units of the count systems
 int launch_mixed(sv_ctx *c, SysArgs a, int count, int mmax, int mode) {
   if (count <= 0) return 0;
   SysArgs b = a;
   b.nsys = count;
-  return mode == 2 ? launch_mixed_tiles<32>(c, b, count, mmax)
-                   : launch_mixed_tiles<16>(c, b, count, mmax);
+  return mode == 2 ? launch_mixed_tiles<32>(c, b, mmax) : launch_mixed_tiles<16>(c, b, mmax);
 }
 
 }  // namespace
@@ -752,7 +852,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   SV_HIP(c, hipMemsetAsync(c->singular, 0, sizeof(int) * 4, c->stream));
   SysArgs a{dn, dk, (int)sys.size(), c->x, c->y, c->h, c->hs, c->node_type, c->row_ptr,
             c->col_raw, c->cnt, c->cnt_small, c->h2, c->wx, c->wy, c->wl, c->wh, c->vol,
-            c->singular, dq, nullptr};
+            c->singular, dq, nullptr, nullptr, 0};
   EventTmp e0, e1, t0, t1;
   SV_HIP(c, e0.create());
   SV_HIP(c, e1.create());
@@ -842,17 +942,22 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
         sk[q] = qk[q * stride];
         sm[q] = qm[q * stride];
       }
-      DevTmp<int> tn, tk, tm;
+      const std::vector<int> su = units_of(sm);
+      DevTmp<int> tn, tk, tm, tu;
       SV_HIP(c, tn.alloc(S));
       SV_HIP(c, tk.alloc(S));
       SV_HIP(c, tm.alloc(S));
+      SV_HIP(c, tu.alloc(su.size()));
       SV_HIP(c, hipMemcpy(tn, sn.data(), sizeof(int) * S, hipMemcpyHostToDevice));
       SV_HIP(c, hipMemcpy(tk, sk.data(), sizeof(int) * S, hipMemcpyHostToDevice));
       SV_HIP(c, hipMemcpy(tm, sm.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+      SV_HIP(c, hipMemcpy(tu, su.data(), sizeof(int) * su.size(), hipMemcpyHostToDevice));
       SysArgs ts = a;
       ts.sys_node = tn;
       ts.sys_kind = tk;
       ts.sys_order = tm;
+      ts.unit_first = tu;
+      ts.nunits = (int)su.size();
       float tv[3] = {0.0f, 0.0f, 0.0f};
       // per-order launches over the sample's contiguous order groups
       if (int rc = timed([&] {
@@ -898,17 +1003,22 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     }
   } else {
     const int total = (int)qn.size();
-    DevTmp<int> mn, mk, mm;
+    const std::vector<int> qu = units_of(qm);
+    DevTmp<int> mn, mk, mm, mu;
     SV_HIP(c, mn.alloc(total));
     SV_HIP(c, mk.alloc(total));
     SV_HIP(c, mm.alloc(total));
+    SV_HIP(c, mu.alloc(qu.size()));
     SV_HIP(c, hipMemcpy(mn, qn.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     SV_HIP(c, hipMemcpy(mk, qk.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     SV_HIP(c, hipMemcpy(mm, qm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    SV_HIP(c, hipMemcpy(mu, qu.data(), sizeof(int) * qu.size(), hipMemcpyHostToDevice));
     SysArgs am = a;
     am.sys_node = mn;
     am.sys_kind = mk;
     am.sys_order = mm;
+    am.unit_first = mu;
+    am.nunits = (int)qu.size();
     float tl = 0.0f;
     SV_HIP(c, hipEventRecord(t0, c->stream));
     if (int rc = launch_mixed(c, am, total, qm[0], mode)) return rc;

@@ -45,39 +45,66 @@ def _sv():
 C1_PH = dict(Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1)
 
 
-def _c1_pair(h_over_s, dt_mode):
+def _c1_pair(h_over_s, dt_mode, shared_weights=False):
+    """Device and oracle on the same node set and initial state.  With
+    shared_weights the oracle runs on the device's lists and weights (checked
+    against its own first) so that only the time stepping is compared."""
     sv = _sv()
     ns = jittered_lattice(64, seed=1, h_over_s=h_over_s)
     ic = taylor_green_ic(ns)
-    s = sv.Solver(morder=4, form="di").setup(ns)
+    s = sv.Solver(morder=4, form="di")
+    s.set_nodes(ns)
+    cnt, cs, link = s.find_neighbours()
+    s.calc_weights()
+    s.calc_sums()
+    s.filter_coefficients()
     s.set_phys(**C1_PH)
     s.set_initial(ic, dt_mode=dt_mode)
     o = OL.OracleSim(ns, 4).setup()
+    if shared_weights:
+        W = s.weights(link.shape[1])
+        OW = o.weight_arrays()
+        L = link.shape[1]
+        for key in ("w_grad", "w_lap", "w_hyp"):
+            a, b = W[key], OW[key][:, :L]
+            err = (np.abs(a - b).reshape(ns.npfb, -1).max(axis=1)
+                   / np.abs(b).reshape(ns.npfb, -1).max(axis=1)).max()
+            assert err < 1e-10, (key, err)
+        o = OL.OracleSim(ns, 4, nplink=L)
+        o.ghosts()
+        o.load_lists(cnt, cs, link)
+        o.load_weights(W)
     o.make_phys(form="di", dt_mode=dt_mode, **C1_PH)
     o.set_initial(ic)
     return s, o
 
 
-def _same(F, R, where):
-    assert F["dt"] == pytest.approx(R["dt"], rel=1e-12, abs=0.0), where
+def _same(F, R, where, dt_rtol=1e-12, rtol=FIELD_RTOL):
+    assert F["dt"] == pytest.approx(R["dt"], rel=dt_rtol, abs=0.0), where
     for k in FKEYS:
-        assert field_rel(F[k], R[k]) < FIELD_RTOL, (where, k, field_rel(F[k], R[k]))
+        assert field_rel(F[k], R[k]) < rtol, (where, k, field_rel(F[k], R[k]))
 
 
 def test_direct_form_order4_h13_vs_oracle():
     """64^2, order 4, h = 1.3 s (about 21 links for 14 unknowns): the run is
     unstable in the oracle as in the reference's own LC build.  With the
     CFL-limited step it leaves the bounded regime in its first step (|u|
-    from 1 to ~19), with the PID controller after ~15 steps; device = oracle
-    while |u| stays below 10."""
+    from 1 to ~19), with the PID controller within ~15 steps.  There a 1e-13
+    relative change of the weights moves the fields by ~5e-7 after one step,
+    so the device's weights are checked against the oracle's (1e-10) and the
+    oracle then steps on them; device = oracle while |u| stays below 10.  The
+    PID step follows the embedded error estimate, which this unstable run
+    makes sensitive to rounding: a 1-ulp change of the initial velocity moves
+    the oracle's own dt by 1.6e-9 after 5 steps and its fields by 1e-9 after
+    10, so the PID leg holds dt and fields to 1e-8."""
     print("[workloads] 64^2 DI order 4 h=1.3s, CFL-limited dt: step 1", flush=True)
-    s, o = _c1_pair(1.3, "cfl")
+    s, o = _c1_pair(1.3, "cfl", shared_weights=True)
     s.run(1)
     o.run(1)
     _same(s.fields(), o.fields(), "cfl step 1")
     s.close()
     print("[workloads] 64^2 DI order 4 h=1.3s, PID dt", flush=True)
-    s, o = _c1_pair(1.3, "pid")
+    s, o = _c1_pair(1.3, "pid", shared_weights=True)
     checked = 0
     for step in range(5, 101, 5):
         s.run(5)
@@ -85,7 +112,7 @@ def test_direct_form_order4_h13_vs_oracle():
         F, R = s.fields(), o.fields()
         if not (all(np.isfinite(R[k]).all() for k in FKEYS) and np.abs(R["u"]).max() < 10.0):
             break
-        _same(F, R, f"pid step {step}")
+        _same(F, R, f"pid step {step}", dt_rtol=1e-8, rtol=1e-8)
         checked = step
     print(f"[workloads] PID run: device = oracle through step {checked}", flush=True)
     assert checked >= 10
@@ -126,9 +153,10 @@ def test_lc_sptt_kolmogorov_1024sq_order6_vs_oracle():
     o.neighbours()
     oc, ocs, ol = o.lists()
     assert np.array_equal(oc, cnt) and np.array_equal(ocs, cs)
-    # rows in (r^2, j) order on both sides
-    assert np.array_equal(ol, link)
-    del ol
+    # rows in (r^2, j) order on both sides; entries past the count are padding
+    used = np.arange(link.shape[1])[None, :] < cnt[:, None]
+    assert np.array_equal(ol[used], link[used])
+    del ol, used
     sample = np.arange(0, n, 1031, dtype=np.int32)
     print(f"[workloads] oracle weights of {sample.size} strided nodes", flush=True)
     o.weights_subset(sample)
@@ -238,7 +266,8 @@ def test_full_ensemble_32_members_100_steps():
             for key in FKEYS:
                 assert field_rel(EF[k][key], R[key]) < FIELD_RTOL, (k, key)
             me, cf = o.diagnostics("mass_energy"), o.diagnostics("conf")
-            assert np.allclose(ED[k][0], me, rtol=1e-10, atol=0), (k, ED[k][0], me)
+            # the mass deficit is zero to rounding after the mass correction
+            assert np.allclose(ED[k][0], me, rtol=1e-10, atol=1e-14), (k, ED[k][0], me)
             assert np.allclose(ED[k][1], cf, rtol=1e-10, atol=1e-14), (k, ED[k][1], cf)
     for _, s in mem:
         s.close()
@@ -278,7 +307,8 @@ def test_adjoint_128sq_order6_400_steps_vs_tangent_linear_and_fd():
         print(f"[workloads] oracle tangent-linear rollout along {name}", flush=True)
         o.set_initial(ic, dt=dt)
         Lo, dLo, _ = o.rollout_tlm(d, steps, dp)
-        assert abs(L - Lo) <= 1e-10 * abs(Lo), (L, Lo)
+        # L ~ 6e-8 is a mean of tr c - 2 with tr c ~ 2: rounding ~1e-16 absolute
+        assert abs(L - Lo) <= 1e-8 * abs(Lo) + 1e-14, (L, Lo)
         assert abs(dev - dLo) <= 1e-6 * abs(dLo), (name, dev, dLo)
     # central difference of the device's forward rollout along the unit
     # gradient direction, where dL = |g| is largest against the rounding of L
