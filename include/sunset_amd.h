@@ -281,6 +281,20 @@ int sv_ensemble_get_control(sv_ctx *ensemble, int32_t member, double *dt, double
  *     decaying Taylor-Green vortex at the context's time */
 enum { SV_DIAG_MASS_ENERGY = 0, SV_DIAG_CONF = 1, SV_DIAG_KOLMOGOROV = 2, SV_DIAG_TG = 3 };
 int sv_diagnostics(sv_ctx *ctx, int32_t kind, int32_t member, double *out);
+/* sv_diagnostics without allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  The same reductions, enqueued on `stream` (a
+ * hipStream_t passed as void*; NULL = the context's stream) with no host
+ * synchronisation, the result written to out_dev (device memory, 6 doubles
+ * suffice for every kind).  Block partials and member sums live in a fixed
+ * 4 MiB array per device held until the process ends, so a context may have
+ * at most 7*(blocks + members) <= 2^19 (about 19M nodes); larger ones get
+ * SV_E_ARG.  Every call on one GPU uses the same array: calls that overlap on
+ * a GPU (several streams or host threads) race.  SV_DIAG_TG reads the time
+ * from the device control block, so its exp may differ from sv_diagnostics'
+ * host exp in the last bit. */
+int sv_diagnostics_noalloc(sv_ctx *ctx, int32_t kind, int32_t member, double *out_dev,
+                           void *stream);
 
 /* ---- Reverse mode (DI form, fixed time step) ----------------------------
  * No counterpart in the reference (it has no adjoint); these are the

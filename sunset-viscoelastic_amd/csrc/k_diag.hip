@@ -10,6 +10,8 @@
 // reference writes them to its statistics files; here they are returned.
 #include <math.h>
 
+#include <mutex>
+
 #include "sv_internal.h"
 
 namespace {
@@ -29,7 +31,18 @@ struct DiagArgs {
   double lambda;       // SV_DIAG_KOLMOGOROV
   double expo;         // SV_DIAG_TG: exp(-8 pi^2 t / T_char / Re)
   double *part;        // [nblk][DQ]
+  // SV_DIAG_TG without a host round trip: when set, expo is computed from the
+  // segment's control block (its time) and physics constants
+  const sv_ctrl *tg_ctrl;
 };
+
+// exp(-8 pi^2 t / T_char / Re) of the decaying Taylor-Green vortex
+// (statistics.F90:620-650), Re = rho U L / mu_total, T_char = L / U
+__host__ __device__ inline double tg_decay(const sv_physk &P, double time) {
+  const double Re = P.visc_total > 0 ? P.rho_char * P.U_char * P.L_char / P.visc_total : 1.0;
+  const double Tc = P.L_char / P.U_char;
+  return exp(-8.0 * SV_PI * SV_PI * time / Tc / Re);
+}
 
 template <int KIND>
 __global__ __launch_bounds__(BS) void k_diag_part(DiagArgs a) {
@@ -74,8 +87,9 @@ __global__ __launch_bounds__(BS) void k_diag_part(DiagArgs a) {
     } else {
       const int p = a.perm ? a.perm[li] : li;
       const double xx = a.x[p], yy = a.y[p];
-      const double ue = -a.expo * cos(2.0 * SV_PI * xx) * sin(2.0 * SV_PI * yy);
-      const double ve = a.expo * sin(2.0 * SV_PI * xx) * cos(2.0 * SV_PI * yy);
+      const double expo = a.tg_ctrl ? tg_decay(S.pk, a.tg_ctrl[sg].time) : a.expo;
+      const double ue = -expo * cos(2.0 * SV_PI * xx) * sin(2.0 * SV_PI * yy);
+      const double ve = expo * sin(2.0 * SV_PI * xx) * cos(2.0 * SV_PI * yy);
       const double Uex = sqrt(ue * ue + ve * ve);
       const double Unum = sqrt(u * u + v * v);
       q[0] = (Unum - Uex) * (Unum - Uex);
@@ -115,7 +129,66 @@ __global__ __launch_bounds__(256) void k_diag_sum(const sv_segk *seg, const doub
   if ((int)threadIdx.x < DQ) out[(size_t)blockIdx.x * DQ + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// The statistics.F90 quantities of one member from its DQ sums q (the
+// reductions above); L_char^2 scales 2D volumes.
+__host__ __device__ inline void diag_finish(int kind, const double *q, double L_char,
+                                            double *out) {
+  const double L2 = L_char * L_char;
+  switch (kind) {
+  case SV_DIAG_MASS_ENERGY:   // statistics.F90:309-343: mass/vol, vol, ke/vol
+    out[0] = (q[0] * L2) / (q[1] * L2);
+    out[1] = q[1] * L2;
+    out[2] = (q[2] * L2) / (q[1] * L2);
+    break;
+  case SV_DIAG_CONF: {        // :365-433: means of cxx cxy cyy czz, rms cxy, mean trace
+    const double V = q[0] * L2;
+    for (int k = 0; k < 4; k++) out[k] = (q[1 + k] * L2) / V;
+    out[4] = sqrt((q[5] * L2) / V);
+    out[5] = (q[6] * L2) / V;
+    break;
+  }
+  case SV_DIAG_KOLMOGOROV:    // :563-602: L2 of u, v, cxx, cxy, cyy
+    for (int k = 0; k < 5; k++) out[k] = sqrt(q[k] / q[5]);
+    break;
+  default:                    // :620-650
+    out[0] = sqrt(q[0] / q[1]);
+    break;
+  }
+}
+
+__global__ void k_diag_finish(int kind, const double *sums, int member, const sv_segk *seg,
+                              double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    diag_finish(kind, sums + (size_t)DQ * member, seg[member].pk.L_char, out);
+}
+
+// Scratch of sv_diagnostics_noalloc: the block partials ([nblk][DQ]) followed
+// by the per-member sums ([nseg][DQ]), 4 MiB per device (up to ~74,000
+// 256-node blocks, i.e. ~19M nodes).  The host side never passes the array's
+// name to a kernel: hipGetSymbolAddress gives each device's own address once,
+// kept in g_scratch below.
+#define SV_DIAG_SCRATCH_DOUBLES (1 << 19)
+static __device__ __align__(256) double scratch[SV_DIAG_SCRATCH_DOUBLES];
+
+#define SV_SCRATCH_MAX_DEVICES 64
+std::mutex g_scratch_mu;
+double *g_scratch[SV_SCRATCH_MAX_DEVICES];
+hipError_t g_scratch_err[SV_SCRATCH_MAX_DEVICES];
+
 }  // namespace
+
+// Look up this device's copy of `scratch` (the device must be current); once
+// per device, from sv_create.  A failure is kept and reported by
+// sv_diagnostics_noalloc, which is the only user.
+void sv_diag_scratch_init(int device) {
+  if (device < 0 || device >= SV_SCRATCH_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch[device] = static_cast<double *>(p);
+}
 
 static int diag_sums(sv_ctx *c, int kind, double lambda, double expo, std::vector<double> &sums) {
   double *part = nullptr, *out = nullptr;
@@ -123,7 +196,7 @@ static int diag_sums(sv_ctx *c, int kind, double lambda, double expo, std::vecto
       sv_alloc(c, (void **)&out, sizeof(double) * DQ * (size_t)c->nseg))
     return SV_E_NOMEM;
   DiagArgs a{c->blk_seg, c->segk, c->U, c->GA, c->gs, c->L.vol,
-             c->is_ensemble ? nullptr : c->L.perm, c->x, c->y, lambda, expo, part};
+             c->is_ensemble ? nullptr : c->L.perm, c->x, c->y, lambda, expo, part, nullptr};
   int rc = 0;
   switch (kind) {
   case SV_DIAG_MASS_ENERGY: k_diag_part<SV_DIAG_MASS_ENERGY><<<c->nblk, BS, 0, c->stream>>>(a); break;
@@ -166,34 +239,61 @@ int sv_diagnostics(sv_ctx *c, int32_t kind, int32_t member, double *out) {
     sv_ctrl h{};
     SV_HIP(c, hipStreamSynchronize(c->stream));
     SV_HIP(c, hipMemcpy(&h, c->ctrl + member, sizeof(h), hipMemcpyDeviceToHost));
-    const double Re = P.visc_total > 0 ? P.rho_char * P.U_char * P.L_char / P.visc_total : 1.0;
-    const double Tc = P.L_char / P.U_char;
-    expo = exp(-8.0 * SV_PI * SV_PI * h.time / Tc / Re);
+    expo = tg_decay(P, h.time);
   }
   std::vector<double> sums;
   if (int rc = diag_sums(c, kind, P.lambda, expo, sums)) return rc;
-  const double *q = sums.data() + (size_t)DQ * member;
-  const double L2 = P.L_char * P.L_char;   // 2D: volumes scale with L_char^2
+  diag_finish(kind, sums.data() + (size_t)DQ * member, P.L_char, out);
+  return 0;
+}
+
+int sv_diagnostics_noalloc(sv_ctx *c, int32_t kind, int32_t member, double *out_dev,
+                           void *stream) {
+  if (!c) return SV_E_ARG;
+  if (!out_dev) return sv_fail(c, SV_E_ARG, "null output");
+  if (!c->have_state) return sv_fail(c, SV_E_STATE, "no state (sv_set_initial first)");
+  if (kind < SV_DIAG_MASS_ENERGY || kind > SV_DIAG_TG)
+    return sv_fail(c, SV_E_ARG, "unknown diagnostic %d", kind);
+  if (member < 0 || member >= c->nseg)
+    return sv_fail(c, SV_E_ARG, "member %d out of range [0, %d)", member, c->nseg);
+  if (c->is_ensemble && (kind == SV_DIAG_KOLMOGOROV || kind == SV_DIAG_TG))
+    return sv_fail(c, SV_E_ARG, "position-based diagnostics are per simulation: call them on "
+                                "the member's own context");
+  const size_t need = (size_t)DQ * ((size_t)c->nblk + (size_t)c->nseg);
+  if (need > SV_DIAG_SCRATCH_DOUBLES)
+    return sv_fail(c, SV_E_ARG,
+                   "sv_diagnostics_noalloc: %d blocks of %d nodes and %d members need %zu "
+                   "scratch doubles, above its fixed %d (use sv_diagnostics)",
+                   c->nblk, SV_BLOCK, c->nseg, need, SV_DIAG_SCRATCH_DOUBLES);
+  SV_HIP(c, hipSetDevice(c->device));
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= 0 && c->device < SV_SCRATCH_MAX_DEVICES) base = g_scratch[c->device];
+    if (!base)
+      return sv_fail(c, SV_E_HIP, "sv_diagnostics_noalloc: no scratch address for device %d "
+                                  "(hipGetSymbolAddress: %s)", c->device,
+                     c->device >= 0 && c->device < SV_SCRATCH_MAX_DEVICES
+                         ? hipGetErrorString(g_scratch_err[c->device]) : "device out of range");
+  }
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  double *part = base, *sums = base + (size_t)DQ * c->nblk;
+  const sv_physk &P = c->hseg[member].pk;
+  DiagArgs a{c->blk_seg, c->segk, c->U, c->GA, c->gs, c->L.vol,
+             c->is_ensemble ? nullptr : c->L.perm, c->x, c->y, P.lambda, 0.0, part,
+             kind == SV_DIAG_TG ? c->ctrl : nullptr};
+  // every block writes its partials and every member its sums: nothing to clear
   switch (kind) {
-  case SV_DIAG_MASS_ENERGY:   // statistics.F90:309-343: mass/vol, vol, ke/vol
-    out[0] = (q[0] * L2) / (q[1] * L2);
-    out[1] = q[1] * L2;
-    out[2] = (q[2] * L2) / (q[1] * L2);
-    break;
-  case SV_DIAG_CONF: {        // :365-433: means of cxx cxy cyy czz, rms cxy, mean trace
-    const double V = q[0] * L2;
-    for (int k = 0; k < 4; k++) out[k] = (q[1 + k] * L2) / V;
-    out[4] = sqrt((q[5] * L2) / V);
-    out[5] = (q[6] * L2) / V;
-    break;
+  case SV_DIAG_MASS_ENERGY: k_diag_part<SV_DIAG_MASS_ENERGY><<<c->nblk, BS, 0, st>>>(a); break;
+  case SV_DIAG_CONF: k_diag_part<SV_DIAG_CONF><<<c->nblk, BS, 0, st>>>(a); break;
+  case SV_DIAG_KOLMOGOROV: k_diag_part<SV_DIAG_KOLMOGOROV><<<c->nblk, BS, 0, st>>>(a); break;
+  default: k_diag_part<SV_DIAG_TG><<<c->nblk, BS, 0, st>>>(a); break;
   }
-  case SV_DIAG_KOLMOGOROV:    // :563-602: L2 of u, v, cxx, cxy, cyy
-    for (int k = 0; k < 5; k++) out[k] = sqrt(q[k] / q[5]);
-    break;
-  default:                    // :620-650
-    out[0] = sqrt(q[0] / q[1]);
-    break;
-  }
+  SV_HIP(c, hipGetLastError());
+  k_diag_sum<<<c->nseg, 256, 0, st>>>(c->segk, part, sums);
+  SV_HIP(c, hipGetLastError());
+  k_diag_finish<<<1, 64, 0, st>>>(kind, sums, member, c->segk, out_dev);
+  SV_HIP(c, hipGetLastError());
   return 0;
 }
 

@@ -93,6 +93,8 @@ int sv_create(const sv_config *cfg, sv_ctx **out) {
     return sv_fail(nullptr, SV_E_HIP, "sv_create: device %d: %s", cfg->device,
                    hipGetErrorString(e));
   }
+  // this device's address of the allocation-free diagnostics' scratch array
+  sv_diag_scratch_init(c->device);
   *out = c;
   return 0;
 }

@@ -278,6 +278,9 @@ struct sv_ctx {
 // Sets the context's message (or, for c == NULL, the calling thread's
 // context-free message that sv_last_error(NULL) returns) and returns code.
 int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
+// Per-device address of the scratch array of sv_diagnostics_noalloc
+// (k_diag.hip), looked up once per device with that device current.
+void sv_diag_scratch_init(int device);
 // LABFM orders the weight kernels are instantiated for
 static inline bool sv_order_supported(int m) { return m == 4 || m == 6 || m == 8; }
 #define SV_HIP(c, call)                                                        \

@@ -58,7 +58,7 @@ EXPORTS = [
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
-    "sv_diagnostics", "sv_set_boundaries", "sv_set_inflow", "sv_calc_boundary_weights",
+    "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow", "sv_calc_boundary_weights",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -115,6 +115,8 @@ def lib():
     L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
     L.sv_adj_param_grad.argtypes = [C.c_void_p, PD]
     L.sv_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, PD]
+    L.sv_diagnostics_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                         C.c_void_p]
     L.sv_set_boundaries.argtypes = [C.c_void_p, PD, PD]
     L.sv_set_inflow.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                 C.c_double]
