@@ -182,7 +182,11 @@ int sv_get_timing(sv_ctx *ctx, const char *what, double *ms);
  *       LDS halo budget of its own largest halo.  sv_ensemble_create times the
  *       right-hand-side sweep both ways and keeps the faster; -1 re-tunes.
  *       sv_get_timing "ensemble_groups" reports the mode,
- *       "ensemble_groups_tune_<mode>" the sample times. */
+ *       "ensemble_groups_tune_<mode>" the sample times.
+ *   "adj_lanes" (0, 1, 4; read by sv_adj_setup): lanes per node of the
+ *       adjoint's transposed gathers; 4 splits each node's links over a quad
+ *       of lanes (node sets too small to fill the chip at one lane per node),
+ *       0 (default) picks 4 below 8 waves per CU. */
 int sv_set_option(sv_ctx *ctx, const char *key, double value);
 
 /* ---- derivative operators: derivatives.F90 ------------------------------ */

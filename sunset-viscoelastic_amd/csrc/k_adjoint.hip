@@ -77,6 +77,36 @@ __device__ __forceinline__ NodeRef node_ref_sub(const int *blk_seg, const sv_seg
   return r;
 }
 
+// node of this thread when LPN lanes share a node (LPN = 1 or 4): a block of
+// AB threads covers AB / LPN consecutive nodes, BS * LPN / AB blocks per
+// 256-node block of the node layout
+template <int LPN>
+__device__ __forceinline__ NodeRef node_ref_lanes(const int *blk_seg, const sv_segk *seg,
+                                                  int sb) {
+  if (LPN == 1) return node_ref_sub(blk_seg, seg, sb);
+  constexpr int NPB = AB / LPN, SUBL = BS / NPB;
+  NodeRef r;
+  const int bid = sb / SUBL;
+  r.sg = blk_seg[bid];
+  const sv_segk &S = seg[r.sg];
+  const int li = (bid - S.blk0) * BS + (sb % SUBL) * NPB + (int)threadIdx.x / LPN;
+  r.act = li < S.npfb;
+  r.i = S.base + li;
+  r.ii = r.act ? r.i : S.base;
+  return r;
+}
+
+// sum of v over the 4 lanes of a quad (DPP quad_perm [1,0,3,2] then
+// [2,3,0,1]); every lane of the quad ends with the same value
+__device__ __forceinline__ double quad_sum(double v) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, false);
+  v = v + __hiloint2double(hi, lo);
+  lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x4E, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x4E, 0xF, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ NodeRef node_ref(const int *blk_seg, const sv_segk *seg, int bid) {
   NodeRef r;
   r.sg = blk_seg[bid];
@@ -255,13 +285,17 @@ __device__ __forceinline__ void trans_link(double acc[6], double wx, double wy, 
   }
 }
 
-// STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar)
-template <int STAGE, bool MDIFF>
+// STAGE 0: plain vector-Jacobian product of calc_all_rhs (Ubar = J^T Rbar).
+// LPN lanes per node: lane r of a node sums its links r, r + LPN, ... and the
+// quad's partial sums are added in a fixed order (small node sets, where one
+// lane per node leaves most of the chip idle and each lane walks ~70 links).
+template <int STAGE, bool MDIFF, int LPN>
 __global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
   __shared__ double red[AB];
   const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref_sub(a.blk_seg, a.seg, bid);
+  const NodeRef nr = node_ref_lanes<LPN>(a.blk_seg, a.seg, bid);
   const int j = nr.i, jj = nr.ii;
+  const int lr = (int)threadIdx.x % LPN;
   const int64_t b = a.toff[jj >> 6] + (jj & 63);
   const int K = nr.act ? a.tcnt[jj] : 0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -269,13 +303,13 @@ __global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
   // every gathered record, are requested before the first sum, so the
   // dependent index -> record loads cost one memory round trip per batch
   // rather than per link (the sums keep the link order)
-  int k = 0;
-  for (; k + SV_ADJ_BATCH <= K; k += SV_ADJ_BATCH) {
+  int k = lr;
+  for (; k + (SV_ADJ_BATCH - 1) * LPN < K; k += SV_ADJ_BATCH * LPN) {
     int ix[SV_ADJ_BATCH];
     double wx[SV_ADJ_BATCH], wy[SV_ADJ_BATCH], wl[SV_ADJ_BATCH];
 #pragma unroll
     for (int u = 0; u < SV_ADJ_BATCH; u++) {
-      const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+      const int64_t e = b + (int64_t)(k + u * LPN) * SV_WAVE;
       ix[u] = a.tcol[e];
       wx[u] = a.twx[e];
       wy[u] = a.twy[e];
@@ -297,7 +331,7 @@ __global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
     for (int u = 0; u < SV_ADJ_BATCH; u++)
       trans_link<MDIFF>(acc, wx[u], wy[u], wl[u], d[u], m[u]);
   }
-  for (; k < K; k++) {
+  for (; k < K; k += LPN) {
     const int64_t e = b + (int64_t)k * SV_WAVE;
     const int i = a.tcol[e];
     const double2 *r = reinterpret_cast<const double2 *>(a.dbar + (size_t)i * SV_TAPE_D);
@@ -311,8 +345,12 @@ __global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
     }
     trans_link<MDIFF>(acc, a.twx[e], a.twy[e], a.twl[e], d, m);
   }
+  if (LPN == 4) {
+#pragma unroll
+    for (int f = 0; f < 6; f++) acc[f] = quad_sum(acc[f]);
+  }
   double part = 0.0;
-  if (nr.act) {
+  if (nr.act && lr == 0) {
     double o[6], g[6];
     ld6(a.own + (size_t)j * SV_US, o);
     ld6(a.tG + (size_t)j * SV_US, g);
@@ -363,13 +401,15 @@ __global__ __launch_bounds__(AB) void k_adj_trans(TransArgs a) {
   }
 }
 
-// per-block sums of Ubar_ro (the start of a backward sweep)
+// per-block sums of Ubar_ro (the start of a backward sweep), in the block
+// layout of the LPN sweeps that write psum afterwards
+template <int LPN>
 __global__ __launch_bounds__(AB) void k_adj_sum(const int *blk_seg, const sv_segk *seg,
                                                 const double *Ubar, double *psum) {
   __shared__ double red[AB];
   const int bid = sv_xcd_block();
-  const NodeRef nr = node_ref_sub(blk_seg, seg, bid);
-  red[threadIdx.x] = nr.act ? Ubar[(size_t)nr.i * SV_US] : 0.0;
+  const NodeRef nr = node_ref_lanes<LPN>(blk_seg, seg, bid);
+  red[threadIdx.x] = (nr.act && (int)threadIdx.x % LPN == 0) ? Ubar[(size_t)nr.i * SV_US] : 0.0;
   __syncthreads();
   for (int s = AB / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -381,11 +421,11 @@ __global__ __launch_bounds__(AB) void k_adj_sum(const int *blk_seg, const sv_seg
 // mass-correction adjoint: sbar[seg] = sum_j Ubar_ro,j / sum_j vol_j (one block
 // per segment, over the segment's sub-block sums), then Ubar_ro,i -= vol_i * sbar
 __global__ __launch_bounds__(1024) void k_adj_sum_reduce(const sv_segk *seg, const double *psum,
-                                                         double *sbar) {
+                                                         int subl, double *sbar) {
   __shared__ double s1[1024];
   const sv_segk &S = seg[blockIdx.x];
   double a = 0.0;
-  for (int k = threadIdx.x; k < S.nblk * SUBS; k += 1024) a += psum[S.blk0 * SUBS + k];
+  for (int k = threadIdx.x; k < S.nblk * subl; k += 1024) a += psum[S.blk0 * subl + k];
   s1[threadIdx.x] = a;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {
@@ -411,21 +451,23 @@ struct FiltArgs {
   PointArgs p4;
 };
 
-template <bool MDIFF>
+template <bool MDIFF, int LPN>
 __global__ __launch_bounds__(AB) void k_adj_filter(FiltArgs a) {
-  const NodeRef nr = node_ref_sub(a.blk_seg, a.seg, sv_xcd_block());
+  const NodeRef nr = node_ref_lanes<LPN>(a.blk_seg, a.seg, sv_xcd_block());
   const int j = nr.i, jj = nr.ii;
+  const int lr = (int)threadIdx.x % LPN;
   const int64_t b = a.toff[jj >> 6] + (jj & 63);
   const int K = nr.act ? a.tcnt[jj] : 0;
   const double sv = a.sbar[nr.sg];
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  int k = 0;
-  for (; k + SV_ADJ_BATCH <= K; k += SV_ADJ_BATCH) {   // batched as in k_adj_trans
+  int k = lr;
+  // batched as in k_adj_trans, lane lr of a node taking links lr, lr + LPN, ...
+  for (; k + (SV_ADJ_BATCH - 1) * LPN < K; k += SV_ADJ_BATCH * LPN) {
     int ix[SV_ADJ_BATCH];
     double w[SV_ADJ_BATCH], f[SV_ADJ_BATCH][6], vo[SV_ADJ_BATCH];
 #pragma unroll
     for (int u = 0; u < SV_ADJ_BATCH; u++) {
-      const int64_t e = b + (int64_t)(k + u) * SV_WAVE;
+      const int64_t e = b + (int64_t)(k + u * LPN) * SV_WAVE;
       ix[u] = a.tcol[e];
       w[u] = a.twh[e];
     }
@@ -441,7 +483,7 @@ __global__ __launch_bounds__(AB) void k_adj_filter(FiltArgs a) {
       for (int q = 0; q < 6; q++) acc[q] += w[u] * f[u][q];
     }
   }
-  for (; k < K; k++) {
+  for (; k < K; k += LPN) {
     const int64_t e = b + (int64_t)k * SV_WAVE;
     const int i = a.tcol[e];
     const double w = a.twh[e];
@@ -451,7 +493,11 @@ __global__ __launch_bounds__(AB) void k_adj_filter(FiltArgs a) {
 #pragma unroll
     for (int q = 0; q < 6; q++) acc[q] += w * f[q];
   }
-  if (!nr.act) return;
+  if (LPN == 4) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[q] = quad_sum(acc[q]);
+  }
+  if (!nr.act || lr != 0) return;
   double f[6], out[6], rb[6];
   ld6(a.Ubar + (size_t)j * SV_US, f);
   f[0] -= a.vol[j] * sv;
@@ -551,7 +597,11 @@ struct sv_adj {
   sv_ctrl ctrl_end{};
   std::vector<double *> tape;    // [every][8]
   double t_fwd = 0, t_bwd = 0;   // ms of the last rollout / vjp
+  int lpn = 1;                   // lanes per node of the transposed gathers (1 or 4)
 };
+
+// sub-blocks per 256-node block of the reverse sweeps with lpn lanes per node
+static int subl_of(int lpn) { return SUBS * lpn; }
 
 void sv_adj_free(sv_ctx *c) {
   delete c->adj;
@@ -682,8 +732,14 @@ static int adj_trans(sv_ctx *c, sv_adj *A, double *const *tape, int q, double se
               A->dbar[q], A->dbarm[q], A->own[q], tape[2 * (k - 1)], c->ctrl, A->Ubar, A->Rbar,
               A->regbar, seed, A->psum, PointArgs{}};
   if (STAGE >= 2) t.nxt = point_args(c, A, tape, STAGE - 1, 1 - q);
-  if (c->any_mdiff) k_adj_trans<STAGE, true><<<c->nblk * SUBS, AB, 0, c->stream>>>(t);
-  else k_adj_trans<STAGE, false><<<c->nblk * SUBS, AB, 0, c->stream>>>(t);
+  const int grid = c->nblk * subl_of(A->lpn);
+  if (A->lpn == 4) {
+    if (c->any_mdiff) k_adj_trans<STAGE, true, 4><<<grid, AB, 0, c->stream>>>(t);
+    else k_adj_trans<STAGE, false, 4><<<grid, AB, 0, c->stream>>>(t);
+  } else {
+    if (c->any_mdiff) k_adj_trans<STAGE, true, 1><<<grid, AB, 0, c->stream>>>(t);
+    else k_adj_trans<STAGE, false, 1><<<grid, AB, 0, c->stream>>>(t);
+  }
   SV_HIP(c, hipGetLastError());
   return 0;
 }
@@ -700,11 +756,17 @@ static int adj_rhs(sv_ctx *c, sv_adj *A, double *const *tape) {
 // adjoint of one step (tape slot): Ubar (after the step, with its mass sums
 // in psum) -> Ubar (before the step) + seed on cxx, cyy.  Six launches.
 static int adj_step(sv_ctx *c, sv_adj *A, double *const *tape, double seed) {
-  k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, A->sbar);
+  const int subl = subl_of(A->lpn), grid = c->nblk * subl;
+  k_adj_sum_reduce<<<c->nseg, 1024, 0, c->stream>>>(c->segk, A->psum, subl, A->sbar);
   FiltArgs f{c->blk_seg, c->segk, A->toff, A->tcnt, A->tcol, A->twh, c->L.hsum, c->L.vol,
              A->sbar, A->Ubar, c->ctrl, A->regbar, point_args(c, A, tape, 4, 0)};
-  if (c->any_mdiff) k_adj_filter<true><<<c->nblk * SUBS, AB, 0, c->stream>>>(f);
-  else k_adj_filter<false><<<c->nblk * SUBS, AB, 0, c->stream>>>(f);
+  if (A->lpn == 4) {
+    if (c->any_mdiff) k_adj_filter<true, 4><<<grid, AB, 0, c->stream>>>(f);
+    else k_adj_filter<false, 4><<<grid, AB, 0, c->stream>>>(f);
+  } else {
+    if (c->any_mdiff) k_adj_filter<true, 1><<<grid, AB, 0, c->stream>>>(f);
+    else k_adj_filter<false, 1><<<grid, AB, 0, c->stream>>>(f);
+  }
   SV_HIP(c, hipGetLastError());
   if (int rc = adj_trans<4>(c, A, tape, 0, 0.0)) return rc;   // + point 3 -> set 1
   if (int rc = adj_trans<3>(c, A, tape, 1, 0.0)) return rc;   // + point 2 -> set 0
@@ -774,13 +836,22 @@ int sv_adj_setup(sv_ctx *c, int32_t checkpoint_every) {
         adj_alloc(c, &A->dbar[1], (size_t)SV_TAPE_D * c->ntot) ||
         adj_alloc(c, &A->dbarm[0], (size_t)4 * c->ntot) ||
         adj_alloc(c, &A->dbarm[1], (size_t)4 * c->ntot) ||
-        adj_alloc(c, &A->psum, (size_t)c->nblk * SUBS) ||
+        adj_alloc(c, &A->psum, (size_t)c->nblk * subl_of(4)) ||
         adj_alloc(c, &A->sbar, c->nseg) || adj_alloc(c, &A->lpart, c->nblk) ||
         adj_alloc(c, &A->lacc, c->nseg) || adj_alloc(c, &A->Uend, R) ||
         adj_alloc(c, &A->pacc, (size_t)4 * c->ntot) || adj_alloc(c, &A->pred, 4))
       return SV_E_NOMEM;
   }
   c->adj->every = checkpoint_every;
+  // lanes per node of the transposed gathers: 4 when one lane per node would
+  // give fewer than 8 waves per CU (e.g. a 128x128 lattice: 256 waves)
+  {
+    int ncu = 0;
+    SV_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int64_t waves = (int64_t)c->nblk * SUBS;
+    c->adj->lpn = c->adj_lanes == 1 || c->adj_lanes == 4 ? c->adj_lanes
+                  : (waves < 8 * (int64_t)ncu ? 4 : 1);
+  }
   return 0;
 }
 
@@ -885,7 +956,11 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   SV_HIP(c, e1.create());
   SV_HIP(c, hipEventRecord(e0, c->stream));
   if (int rc = zero_dev(c, A->pacc, (int64_t)4 * c->ntot)) return rc;
-  k_adj_sum<<<c->nblk * SUBS, AB, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
+  if (A->lpn == 4)
+    k_adj_sum<4><<<c->nblk * subl_of(4), AB, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar,
+                                                             A->psum);
+  else
+    k_adj_sum<1><<<c->nblk * SUBS, AB, 0, c->stream>>>(c->blk_seg, c->segk, A->Ubar, A->psum);
   SV_HIP(c, hipGetLastError());
   const int nck = (T + E - 1) / E;
   for (int k = nck - 1; k >= 0; k--) {

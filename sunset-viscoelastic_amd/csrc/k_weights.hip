@@ -218,17 +218,20 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
 #pragma unroll
     for (int b = 0; b < T; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int role = l / CH;   // 0: monomial row, 1: ABF row, 2-3 (CH = 16): idle
+  // scaled offsets by one reciprocal (the weights move by rounding only: ~1
+  // ulp in the moment matrix, far inside the 1e-10 weight tolerance)
+  const double rhh = 1.0 / hh;
   for (int c0 = 0; c0 < Kuse; c0 += CH) {
     const int kk = c0 + (l % CH);
     double *row = (role == 0 ? Xc : Gc) + (l % CH) * NP;
     if (role < 2 && kk < Kuse) {
       const int j = S_.col[base + kk];
       const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
-      const double xs = x / hh, ys = y / hh;
+      const double xs = x * rhh, ys = y * rhh;
       if (role == 0) {
         mono_row<M>(xs, ys, row);
       } else {
-        const double rad = sqrt(x * x + y * y) / hh;
+        const double rad = sqrt(x * x + y * y) * rhh;
         abf_row<M>(xs, ys, wab(rad, S_.h2), row);
       }
 #pragma unroll
@@ -286,13 +289,14 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD
   for (int a = 0; a < N; a++) ps[a] = Xs[a * 4 + q];
   const bool want = q < 3 ? sysD : sysF;
   double *dst = q == 0 ? S_.wx : q == 1 ? S_.wy : q == 2 ? S_.wl : S_.wh;
+  const double rhh = 1.0 / hh;
   double vsum = 0.0;
   for (int k = kl; k < Kfull; k += SV_WAVE / 4) {
     const int j = S_.col[base + k];
     const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
-    const double rad = sqrt(x * x + y * y) / hh;
-    const double o = abf_dot<M>(x / hh, y / hh, wab(rad, S_.h2), ps);
-    if (want) dst[base + k] = q < 2 ? o / hh : q == 2 ? o / hh / hh : o;
+    const double rad = sqrt(x * x + y * y) * rhh;
+    const double o = abf_dot<M>(x * rhh, y * rhh, wab(rad, S_.h2), ps);
+    if (want) dst[base + k] = q < 2 ? o * rhh : q == 2 ? o * rhh * rhh : o;
     if (q == 3 && sysF) vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
   }
   if (sysF) {
@@ -385,7 +389,7 @@ template <int M, int CH_>
 __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int count, double *lds) {
   using P = Pack<M>;
   using L = Lds<M, CH_>;
-  constexpr int N = P::N, S = P::S, G = P::G, LDA = L::LDA, NP = L::NP, ASZ = L::ASZ;
+  constexpr int N = P::N, S = P::S, LDA = L::LDA, NP = L::NP, ASZ = L::ASZ;
   double *A0 = lds;                     // [NA][N][LDA]
   double *C = lds + L::NA * ASZ;        // chunks, then the broadcast / solution area
   double *Ub = C;                       // [G][NP]: pivot row, its reciprocal at index N

@@ -447,7 +447,11 @@ int sv_set_option(sv_ctx *c, const char *key, double value) {
     for (int k = 0; k < 3; k++) c->wtune[k] = -1;
     c->wmixed = -1;
   }
-  else if (!strcmp(key, "halo_sweeps")) {
+  else if (!strcmp(key, "adj_lanes")) {
+    SV_CHECK(c, value == 0.0 || value == 1.0 || value == 4.0,
+             "adj_lanes must be 0 (by node count), 1 or 4");
+    c->adj_lanes = (int)value;
+  } else if (!strcmp(key, "halo_sweeps")) {
     SV_CHECK(c, !c->have_state, "halo_sweeps must be set before sv_set_initial");
     c->halo_sweeps = value != 0.0 ? 1 : 0;
     c->L.valid = false;

@@ -250,6 +250,8 @@ struct sv_ctx {
   sv_loopl L;
   int loop_order = 1;       // 1: Morton order for the time loop, 0: the caller's order
   int halo_sweeps = 1;      // 1: stage neighbour halos in LDS when they fit
+  int adj_lanes = 0;        // lanes per node of the adjoint's transposed gathers
+                            // (1, 4; 0: by node count), read by sv_adj_setup
   // ensemble: loop-slot -> member-node permutation of each member
   std::vector<std::vector<int>> member_perm;
   // ensemble launch groups: the blocks of the members of one LABFM order,

@@ -76,14 +76,19 @@ def test_rhs_vjp_dot_product_with_oracle_tangent(mdiff):
     s.close()
 
 
-def test_rollout_vjp_matches_tangent_linear_model_and_fd():
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_rollout_vjp_matches_tangent_linear_model_and_fd(lanes):
+    """Both layouts of the transposed gathers: one lane per node, and four
+    lanes per node (links split over a quad, partial sums added in a fixed
+    order)."""
     ns = jittered_lattice(16, seed=4, h_over_s=1.8)
     ic = _ic(ns)
     U0 = _state(ic)
     steps, every = 25, 7          # several checkpoint intervals, a short last one
     dU = _dir(ns.npfb, 11)
-    print("[adjoint] rollout", flush=True)
+    print(f"[adjoint] rollout, {lanes} lanes per node", flush=True)
     s = _device(ns, PH)
+    s.set_option("adj_lanes", lanes)
     s.set_state(U0, dt=DT)
     s.adj_setup(every)
     L = s.rollout_forward(steps)
