@@ -30,6 +30,11 @@
 !!          u_inflow_start, u_inflow_end, ramp_time.  Nodes of type 0/1/2 are
 !!          boundary nodes, each followed by its FD rows (types -1..-4) as the
 !!          IPART reader lays them out (setup_domain.F90:179-193)
+!!          bit 4 (16) -- call adapt_stencils after the first find_neighbours
+!!          and find_neighbours again after it, as the main program does
+!!          without restart (sunset.F90:36-52); the output then starts with
+!!          int32 npfb; real64 h(npfb), h_small(npfb); int32 ij_count(npfb),
+!!          ij_count_small(npfb) as adapt_stencils left them
 !!   output: see write section below (mirrored by tests/refdrv.py)
 program sunset_ref
   use kind_parameters
@@ -48,7 +53,8 @@ program sunset_ref
   real(rkind) :: rh(24)
   real(rkind),allocatable :: tmp(:),tro(:),tu(:),tv(:),tcxx(:),tcxy(:),tcyy(:)
   real(rkind),allocatable :: g2(:,:),l1(:),f1(:)
-  integer(ikind),allocatable :: itmp(:)
+  real(rkind),allocatable :: ah(:),ahs(:)
+  integer(ikind),allocatable :: itmp(:),ac(:),acs(:)
   integer(ikind) :: ib2(2)
   real(rkind) :: rb3(3)
   real(rkind) :: t0,t1
@@ -141,6 +147,20 @@ program sunset_ref
      call find_neighbours
      t1 = omp_get_wtime()
      write(6,*) "REFTIME find_neighbours",t1-t0
+     if(iand(flags,16).ne.0) then
+        t0 = omp_get_wtime()
+        call adapt_stencils
+        t1 = omp_get_wtime()
+        write(6,*) "REFTIME adapt_stencils",t1-t0
+        allocate(ah(npfb),ahs(npfb),ac(npfb),acs(npfb))
+        ah = h(1:npfb);ahs = h_small(1:npfb)
+        ac = ij_count(1:npfb);acs = ij_count_small(1:npfb)
+        !! refine_and_finalise_domain (setup_domain.F90:373-386, non-MPI)
+        !! frees the lists, then they are rebuilt with the new h
+        !! (sunset.F90:49-52); the lists read only the real nodes' h
+        deallocate(ij_link,ij_count)
+        call find_neighbours
+     end if
      t0 = omp_get_wtime()
      call calc_labf_weights
      t1 = omp_get_wtime()
@@ -219,6 +239,11 @@ program sunset_ref
   call calc_filter_term(ro,f1)
 
   open(31,file=trim(fout),access='stream',form='unformatted',status='replace')
+  if(iand(flags,16).ne.0) then
+     write(31) npfb
+     write(31) ah,ahs
+     write(31) ac,acs
+  end if
   if(iand(flags,4).eq.0) then
   write(31) np,nplink
   write(31) rp(1:np,1),rp(1:np,2)

@@ -131,6 +131,18 @@ int  or_set_weights(or_domain *d, const double *w_grad, const double *w_lap,
                     const double *hsum, const double *vol);
 int  or_weights_subset(or_domain *d, const int *nodes, int count, const int *order_of_node);
 
+/* ---- stencil adaptation (sv_oracle_adapt.c) ---- */
+/* Golub-Reinsch SVD in svdlib.F90's operation order: U (n x n column-major,
+ * in: A) <- U, W singular values, V, rv1 work; -1 without convergence */
+int  or_svd_gr(int n, double *U, double *W, double *V, double *rv1);
+void or_svd_gr_backsub(int n, const double *U, const double *W, const double *V, double *b,
+                       double *tmp);
+int  or_svd_solve_gr(int n, const double *A, double *b);   /* svd_solve */
+/* adapt_stencils + adapt_stencils_main (labf.F90:1079-1721) after
+ * or_find_neighbours: h, h_small, the lists and counts of real nodes updated;
+ * reductions (nullable, 2*npfb) the accepted 2% steps per node and pass */
+int  or_adapt_stencils(or_domain *d, int *reductions);
+
 /* ---- kernels / bases ---- */
 double or_wab(double q, double h2);
 void   or_monomials(int order, double x, double y, double *out);

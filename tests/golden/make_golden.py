@@ -18,6 +18,11 @@ Fixtures (numpy .npz, no pickles):
                        inflow and an outflow), with the node arrays and normals:
                        the reference's calc_boundary_weights and characteristic
                        boundary conditions (only the fields and the CSR weights)
+  ref_adapt_<name>.npz adapt_stencils: h, h_small and the list counts as it
+                       leaves them, then the lists and weights rebuilt from
+                       the adapted h (CSR in canonical order)
+
+    python tests/golden/make_golden.py --adapt    (only the adapt fixtures)
 """
 import os
 import sys
@@ -142,6 +147,50 @@ def main():
         print("wrote", name, ns.npfb, "nodes", int(ptr[-1]), "links")
 
 
+# adapt_stencils: name, node set ("lattice" or "wall"), filter order,
+# derivative order, n, h/s, seed.  With equal orders the second pass (the
+# derivative system on the small stencil) retries the step the first pass
+# refused and stops at once; the mixed-order builds exercise it.
+ACASES = [
+    ("m4", "lattice", 4, 4, 24, 1.8, 41),
+    ("m6", "lattice", 6, 6, 24, 2.4, 42),
+    ("m8", "lattice", 8, 8, 20, 2.7, 43),
+    ("wall_m6", "wall", 6, 6, 20, 0.0, 1),
+    ("f8m6", "lattice", 8, 6, 20, 2.7, 44),
+    ("f6m4", "lattice", 6, 4, 24, 2.4, 45),
+]
+
+# cases whose fixture also holds the weights computed on the adapted stencils
+AWEIGHTS = ("f6m4",)
+
+
+def adapt_case(kind, order, n, hos, seed):
+    if kind == "wall":
+        return boundary_case("wall", order)[0]
+    return jittered_lattice(n, seed=seed, h_over_s=hos)
+
+
+def write_adapt():
+    for name, kind, forder, morder, n, hos, seed in ACASES:
+        ns = adapt_case(kind, forder, n, hos, seed)
+        out, _ = refdrv.run_reference(ns, forder, taylor_green_ic(ns), nsteps=0, adapt=True,
+                                      binary=refdrv.ref_binary_mixed(forder, morder))
+        extra = {}
+        if name in AWEIGHTS:
+            ptr, col, wx, wy, wl, wh = to_csr(out, ns.npfb)
+            extra = dict(ptr=ptr, col=col, wx=wx, wy=wy, wl=wl, wh=wh)
+        np.savez_compressed(
+            os.path.join(HERE, f"ref_adapt_{name}.npz"), kind=kind, forder=forder,
+            morder=morder, n=n,
+            h_over_s=hos, seed=seed, x=ns.x, y=ns.y, node_type=ns.node_type,
+            h=out["adapt_h"], h_small=out["adapt_h_small"], a_count=out["adapt_count"],
+            a_count_small=out["adapt_count_small"], ij_count=out["ij_count"],
+            ij_count_small=out["ij_count_small"], **extra)
+        print("wrote adapt", name, ns.npfb, "nodes, h reduced on",
+              int((out["adapt_h"] < ns.h).sum()), "h_small on",
+              int((out["adapt_h_small"] < out["adapt_h"]).sum()))
+
+
 # convergence of the reference's derivative operators on an analytic field:
 # rho = 1 + 0.1 sin(2 pi x) cos(2 pi y) on periodic jittered lattices
 CONV = [(4, 1.8, (16, 32, 64)), (6, 2.4, (16, 32, 64))]
@@ -180,6 +229,9 @@ def write_convergence():
 if __name__ == "__main__":
     if "--convergence" in sys.argv:
         write_convergence()
+    elif "--adapt" in sys.argv:
+        write_adapt()
     else:
         main()
         write_convergence()
+        write_adapt()

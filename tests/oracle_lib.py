@@ -106,6 +106,9 @@ def lib():
         L.or_diag_tg.restype = C.c_double
         L.or_set_boundaries.argtypes = [C.POINTER(Domain), PD, PD]
         L.or_boundary_weights.argtypes = [C.POINTER(Domain)]
+        L.or_adapt_stencils.argtypes = [C.POINTER(Domain), PI]
+        L.or_svd_solve_gr.argtypes = [C.c_int, PD, PD]
+        L.or_svd_gr.argtypes = [C.c_int, PD, PD, PD, PD]
         L.or_set_stencil.argtypes = [C.POINTER(Domain), C.c_int, PI, PI, PI]
         L.or_set_weights.argtypes = [C.POINTER(Domain)] + [PD] * 7
         L.or_weights_subset.argtypes = [C.POINTER(Domain), PI, C.c_int, PI]
@@ -216,6 +219,18 @@ class OracleSim:
                 for k in ("w_grad", "w_lap", "w_hyp", "gsum", "lsum", "hsum", "vol")]
         assert arrs[1].shape == (self.d.npfb, self.d.nplink)
         assert lib().or_set_weights(C.byref(self.d), *[_pd(a) for a in arrs]) == 0
+
+    def adapt_stencils(self):
+        """adapt_stencils + adapt_stencils_main (labf.F90:1079-1721) on the
+        current lists: h, h_small and the lists change; returns the accepted
+        2% reductions per node of the two passes ([2, npfb])."""
+        red = np.zeros(2 * self.d.npfb, dtype=np.int32)
+        assert lib().or_adapt_stencils(C.byref(self.d), _pi(red)) == 0, "SVD not converging"
+        return red.reshape(2, -1)
+
+    def h_hs(self):
+        n = self.d.npfb
+        return _view(self.d.h, n).copy(), _view(self.d.hs, n).copy()
 
     def weights_subset(self, nodes, order_map=None):
         """The weight solve (derivative and filter systems, volumes) of the

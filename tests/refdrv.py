@@ -22,6 +22,13 @@ def available(order: int) -> bool:
     return os.path.exists(ref_binary(order))
 
 
+def ref_binary_mixed(forder: int, morder: int) -> str:
+    """The reference built with filter order forder > derivative order morder."""
+    if forder == morder:
+        return ref_binary(forder)
+    return os.path.join(REF_DIR, f"sunset_ref_lc_f{forder}m{morder}")
+
+
 def dropin_binary(order: int) -> str:
     """The reference's time loop linked with this library's Fortran drop-in
     neighbours / labf / derivatives modules (oracle/Makefile target dropin)."""
@@ -30,15 +37,20 @@ def dropin_binary(order: int) -> str:
 
 def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e-6,
                   time0=0.0, emax=(1e-5, 1e-5, 1e-5), itime0=0, nplink=0, threads=4,
-                  timeout=600, stencil=None, fields_only=False, binary=None, inflow=None):
+                  timeout=600, stencil=None, fields_only=False, binary=None, inflow=None,
+                  adapt=False):
     """stencil (optional): dict(nplink, ij_count, ij_count_small, ij_link (0-based,
     [npfb, nplink]), w_grad [npfb, nplink, 2], w_lap, w_hyp, gsum [npfb, 2], lsum,
-    hsum, vol) loaded instead of the reference's own neighbour/weight setup."""
+    hsum, vol) loaded instead of the reference's own neighbour/weight setup.
+    adapt: run adapt_stencils after the first neighbour search (and the search
+    again after it); the output gains adapt_h, adapt_h_small, adapt_count,
+    adapt_count_small as adapt_stencils left them."""
     phys = dict(L_char=1.0, U_char=1.0, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
                 Mdiff=0.0, grav=(0, 0, 0), drive=(0, 0, 0)) | (phys or {})
     n = ns.npfb
     has_b = getattr(ns, "rnx", None) is not None
-    flags = (2 if stencil is not None else 0) | (4 if fields_only else 0) | (8 if has_b else 0)
+    flags = ((2 if stencil is not None else 0) | (4 if fields_only else 0) | (8 if has_b else 0)
+             | (16 if adapt else 0))
     inflow = dict(inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0) | (
         inflow or {})
     hdr = np.array([n, nsteps, 0 if dt_mode == "pid" else 1, nplink, flags, *ns.bc, itime0, 0,
@@ -89,10 +101,10 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
             times[parts[1]] = float(parts[-1])
         if "REFSTEP" in line:
             times["steps"].append(float(line.split()[-1]))
-    return _parse(raw, n, nsteps, fields_only), times
+    return _parse(raw, n, nsteps, fields_only, adapt), times
 
 
-def _parse(raw, n, nsteps, fields_only=False):
+def _parse(raw, n, nsteps, fields_only=False, adapt=False):
     off = 0
 
     def take(cnt, dt):
@@ -102,6 +114,12 @@ def _parse(raw, n, nsteps, fields_only=False):
         return a.copy()
 
     out = {}
+    if adapt:
+        assert int(take(1, np.int32)[0]) == n
+        out["adapt_h"] = take(n, np.float64)
+        out["adapt_h_small"] = take(n, np.float64)
+        out["adapt_count"] = take(n, np.int32)
+        out["adapt_count_small"] = take(n, np.int32)
     if fields_only:
         return _parse_fields(out, take, n)
     np_, nplink = take(2, np.int32)

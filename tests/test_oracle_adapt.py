@@ -1,0 +1,151 @@
+"""Stencil adaptation (adapt_stencils, labf.F90:1079-1721) in the CPU oracle.
+
+Pins: the Golub-Reinsch SVD against numpy's LAPACK SVD and solve (and the
+pseudo-inverse where a singular value is exactly zero); the adapted h,
+h_small and list counts bit for bit against the reference's own run
+(committed fixtures from the flang build, and the live build when present);
+the weights rebuilt on the adapted stencils against the reference's.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+import refdrv
+from parity_util import GOLDEN, WEIGHT_RTOL, csr_from_rows, per_node_rel
+
+sys.path.insert(0, GOLDEN)
+from make_golden import ACASES, adapt_case  # noqa: E402
+
+PD = C.POINTER(C.c_double)
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def svd_gr(A):
+    n = A.shape[0]
+    U = np.asfortranarray(A.copy())
+    W = np.zeros(n)
+    V = np.zeros((n, n), order="F")
+    rv1 = np.zeros(n)
+    assert OL.lib().or_svd_gr(n, _pd(U), _pd(W), _pd(V), _pd(rv1)) == 0
+    return U, W, V
+
+
+@pytest.mark.parametrize("n", [5, 9, 14, 20, 27, 44])
+def test_svd_gr_against_lapack(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    U, W, V = svd_gr(A)
+    # a decomposition: orthogonal factors, non-negative values, U diag(W) V^T = A
+    assert (W >= 0).all()
+    assert np.abs(U.T @ U - np.eye(n)).max() < 1e-13
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-13
+    assert np.abs((U * W) @ V.T - A).max() < 1e-13 * np.abs(A).max() * n
+    sv = np.linalg.svd(A, compute_uv=False)
+    assert np.allclose(np.sort(W)[::-1], sv, rtol=1e-12, atol=1e-14 * sv[0])
+    b = rng.standard_normal(n)
+    x = np.asfortranarray(b.copy())
+    assert OL.lib().or_svd_solve_gr(n, _pd(np.asfortranarray(A)), _pd(x)) == 0
+    ref = np.linalg.solve(A, b)
+    cond = sv[0] / sv[-1]
+    assert np.abs(x - ref).max() <= 1e-14 * cond * np.abs(ref).max() * n
+
+
+def test_svd_gr_skips_zero_singular_values():
+    # exactly zero singular values are skipped: the minimum-norm least-squares
+    # solution (svdlib.F90:282-303)
+    d = np.array([3.0, -2.0, 0.0, 0.5, 0.0, 7.0])
+    A = np.diag(d)
+    U, W, V = svd_gr(A)
+    assert np.count_nonzero(W == 0.0) == 2
+    b = np.arange(1.0, 7.0)
+    x = b.copy()
+    assert OL.lib().or_svd_solve_gr(6, _pd(np.asfortranarray(A)), _pd(x)) == 0
+    assert np.allclose(x, np.linalg.pinv(A) @ b, rtol=1e-15, atol=0)
+
+
+def _oracle_adapt(name):
+    case = {c[0]: c for c in ACASES}[name]
+    _, kind, forder, morder, n, hos, seed = case
+    g = dict(np.load(os.path.join(GOLDEN, f"ref_adapt_{name}.npz")))
+    ns = adapt_case(kind, forder, n, hos, seed)
+    assert np.array_equal(ns.x, g["x"]) and np.array_equal(ns.y, g["y"])
+    o = OL.OracleSim(ns, morder, forder)
+    o.ghosts()
+    o.neighbours()
+    red = o.adapt_stencils()
+    return g, ns, o, red
+
+
+@pytest.mark.parametrize("name", [c[0] for c in ACASES])
+def test_adapt_matches_reference_fixture(name):
+    g, ns, o, red = _oracle_adapt(name)
+    h, hs = o.h_hs()
+    assert np.array_equal(h, g["h"]), f"h differs on {np.count_nonzero(h != g['h'])} nodes"
+    assert np.array_equal(hs, g["h_small"]), \
+        f"h_small differs on {np.count_nonzero(hs != g['h_small'])} nodes"
+    cnt, cs, _ = o.lists()
+    assert np.array_equal(cnt, g["a_count"]) and np.array_equal(cs, g["a_count_small"])
+    # bookkeeping: h is the initial h times the single-precision 0.98 once per
+    # accepted reduction of pass 1, h_small that h times it once per pass-2
+    # reduction; boundary nodes and their first FD rows keep h
+    f = float(np.float32(0.98))
+    hh = ns.h.copy()
+    for r in range(int(red[0].max())):
+        hh = np.where(red[0] > r, hh * f, hh)
+    assert np.array_equal(hh, h)
+    for r in range(int(red[1].max())):
+        hh = np.where(red[1] > r, hh * f, hh)
+    assert np.array_equal(hh, hs)
+    fixed = (ns.node_type >= 0) & (ns.node_type <= 2)
+    assert (red[:, fixed] == 0).all()
+    assert (red[0] <= 26).all() and (red.sum() > 0)
+    # the lists rebuilt with the adapted h, h_small (the reference's counts are
+    # written after calc_boundary_weights, which shortens boundary nodes' lists)
+    o.neighbours()
+    cnt, cs, _ = o.lists()
+    live = ns.node_type != 0
+    assert np.array_equal(cnt[live], g["ij_count"][live])
+    assert np.array_equal(cs[live], g["ij_count_small"][live])
+
+
+def test_weights_on_adapted_stencils_match_reference():
+    g, ns, o, _ = _oracle_adapt("f6m4")
+    o.neighbours()
+    o.weights()
+    x, y = o.xy()
+    n = ns.npfb
+    cnt, _, link = o.lists()
+    W = o.weight_arrays()
+    ptr, col, (wx, wy, wl, wh) = csr_from_rows(
+        cnt, link, x, y, x[:n], y[:n],
+        [W["w_grad"][:, :, 0], W["w_grad"][:, :, 1], W["w_lap"], W["w_hyp"]])
+    assert np.array_equal(ptr, g["ptr"]) and np.array_equal(col, g["col"])
+    for a, b in ((wx, g["wx"]), (wy, g["wy"]), (wl, g["wl"]), (wh, g["wh"])):
+        assert per_node_rel(ptr, a, b) < WEIGHT_RTOL
+
+
+@pytest.mark.skipif(not os.path.exists(refdrv.ref_binary_mixed(8, 6)),
+                    reason="oracle/_ref not built")
+@pytest.mark.parametrize("forder,morder,n,hos", [(8, 6, 28, 2.7), (4, 4, 40, 1.8)])
+def test_adapt_vs_live_reference(forder, morder, n, hos):
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+    ns = jittered_lattice(n, seed=900 + n, h_over_s=hos)
+    o = OL.OracleSim(ns, morder, forder)
+    o.ghosts()
+    o.neighbours()
+    o.adapt_stencils()
+    h, hs = o.h_hs()
+    cnt, cs, _ = o.lists()
+    out, _ = refdrv.run_reference(ns, forder, taylor_green_ic(ns), nsteps=0, adapt=True,
+                                  binary=refdrv.ref_binary_mixed(forder, morder),
+                                  fields_only=True)
+    assert np.array_equal(h, out["adapt_h"]) and np.array_equal(hs, out["adapt_h_small"])
+    assert np.array_equal(cnt, out["adapt_count"])
+    assert np.array_equal(cs, out["adapt_count_small"])
