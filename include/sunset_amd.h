@@ -118,6 +118,23 @@ int sv_get_ghosts(sv_ctx *ctx, int32_t *np, double *x, double *y, int32_t *irela
 int sv_find_neighbours(sv_ctx *ctx, int32_t nplink, int32_t *ij_count,
                        int32_t *ij_count_small, int32_t *ij_link);
 
+/* adapt_stencils (labf.F90:1079-1416) + adapt_stencils_main (:1418-1721),
+ * called between sv_find_neighbours and the weights as sunset.F90:40-52 does
+ * without a restart.  Each interior node (node_type > 2, not 998) reduces h
+ * by 2% (the single-precision 0.98) at a time while the Laplacian moment
+ * system's SVD residual stays below res_tol/h (res_tol = c n^4 eps / m,
+ * c = 1e-3, 5e-3, 3e-2 for orders 4, 6, 8) and the d/dx, d/dy, Laplacian
+ * operators amplify none of the wavenumbers i pi/(32 s), i = 1..33, by more
+ * than 1.0001 (nor the Laplacian below 0), at most 26 times; pass 1 with the
+ * filter order sets h and h_small, pass 2 with the derivative order sets
+ * h_small.  The lists are then rebuilt from the new h, h_small (weights, sums
+ * and filter coefficients must be recomputed).  Needs morder <= forder.
+ * h, h_small (npfb, nullable): the adapted values; reductions (2*npfb,
+ * nullable): accepted reductions per node, pass 1 then pass 2.  Refused after
+ * sv_set_initial; SV_E_SINGULAR if an SVD does not converge (the reference
+ * stops, svdlib.F90:218-221). */
+int sv_adapt_stencils(sv_ctx *ctx, double *h, double *h_small, int32_t *reductions);
+
 /* ---- weights: labf.F90 -------------------------------------------------- */
 /* calc_labf_weights (labf.F90:34-231) incl. calc_filter_weights (:233-394)
  * and calc_node_volumes (:453-503).  order_map (nullable, per real node):

@@ -308,9 +308,24 @@ int or_find_neighbours(or_domain *d, int nplink) {
 /* ------------------------------------------------------------------------ */
 /* Wendland C2, rbfs.F90:48-57 (normalisation 7/(4*pi*h2), h2 from
  * setup_domain.F90:105) */
+/* (1 - q/2)^4 in the Wendland kernel: 0 = pow(t, 4.0), the libm call the
+ * reference's real power compiles to (glibc's pow is within 0.52 ulp, not
+ * always correctly rounded); 1 = t^4 correctly rounded: t^2 = p + e and
+ * p^2 = q1 + q2 exactly (fma), t^4 = q1 + (q2 + (2 p e + e^2)) rounded once
+ * (the dropped terms are below 2^-100 relative).  Process-wide; set it
+ * before a run, not during one. */
+static int g_wab_pow4 = 0;
+void or_set_wab_pow4(int mode) { g_wab_pow4 = mode; }
+
+static double pow4_cr(double t) {
+  const double p = t * t, e = fma(t, t, -p);
+  const double q1 = p * p, q2 = fma(p, p, -q1);
+  return q1 + (q2 + (2.0 * p * e + e * e));
+}
+
 double or_wab(double q, double h2) {
   double t = 1.0 - 0.5 * q;
-  t = pow(t, 4.0);
+  t = g_wab_pow4 ? pow4_cr(t) : pow(t, 4.0);
   if (q <= 2.0 && q > 0.0) return (2.0 * q + 1.0) * (7.0 / (4.0 * PI * h2)) * t;
   return 0.0;
 }

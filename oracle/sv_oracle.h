@@ -142,9 +142,13 @@ int  or_svd_solve_gr(int n, const double *A, double *b);   /* svd_solve */
  * or_find_neighbours: h, h_small, the lists and counts of real nodes updated;
  * reductions (nullable, 2*npfb) the accepted 2% steps per node and pass */
 int  or_adapt_stencils(or_domain *d, int *reductions);
+/* the acceptance checks of given stencils (h or h_small): residual measure
+ * over its tolerance and the first failing amplification wavenumber */
+int  or_adapt_check(or_domain *d, const double *h_sel, int pass, double *ratio, int *amp);
 
 /* ---- kernels / bases ---- */
 double or_wab(double q, double h2);
+void   or_set_wab_pow4(int mode);   /* 0: libm pow(t, 4.0); 1: correctly rounded */
 void   or_monomials(int order, double x, double y, double *out);
 void   or_abfs(int order, double x, double y, double *out);
 int    or_svd_solve(int n, const double *A, double *b);  /* A column-major n*n */

@@ -267,108 +267,138 @@ static double res_tol_of(int m) {
   return c * (double)(n * n * n * n) * 2.220446049250313080847263336181640625e-16 / (double)m;
 }
 
-/* One h-reduction sequence of node i (the body of the do-while loops of
- * labf.F90:1151-1336 and :1517-1702).  pass 1: the filter order's system on
- * the full list, accepted reductions shorten the list (and set h = h_small);
- * pass 2: the derivative order's system (the first nsize unknowns of the
- * filter order's functions) on the small list, accepted reductions set
- * h_small and the small count.  Returns the number of accepted reductions, or
- * -1 if an SVD did not converge. */
-static int adapt_node(or_domain *d, int i, int pass, double *work, int *list) {
-  const int mG = d->forder, nG = nsize_a(mG);
-  const int m = pass == 1 ? d->forder : d->morder, n = nsize_a(m);
-  const double reduction = (double)0.98f;
-  const double limit = (double)logf(0.6f) / log(reduction);
+/* The checks of one trial h (the body of the do-while loops of
+ * labf.F90:1151-1336 and :1517-1702 up to the decision): the list src[0..K0)
+ * cut to r <= 2 hh (into list, *K entries), the order-m moment system of the
+ * first nsize(m) of the filter order's functions, its SVD solves for the
+ * Laplacian and the two gradients, the Laplacian residual measure *hcs.
+ * amp_all 0: the reference's sequence -- the 33 amplification checks run
+ * while hcs < rtol/hh, the first failure setting hcs = 2 rtol/hh.  amp_all 1:
+ * *hcs is the residual measure alone and all 33 checks run; *amp_fail is the
+ * first failing wavenumber index (1..33) or 0.  Returns 0, or -1 if the SVD
+ * did not converge. */
+static int eval_hh(or_domain *d, int i, int m, double hh, const int *src, int K0, double *work,
+                   int *list, int *K_out, double *hcs_out, int amp_all, int *amp_fail) {
+  const int mG = d->forder, nG = nsize_a(mG), n = nsize_a(m);
   const double rtol = res_tol_of(m), amp_tol = 1.0001;
   const int nk = 32;
   double *A = work, *M = A + (size_t)n * n, *U = M + (size_t)n * n, *V = U + (size_t)n * n;
   double *W = V + (size_t)n * n, *rv1 = W + n, *tmp = rv1 + n;
   double *bL = tmp + n, *bX = bL + n, *bY = bX + n, *xv = bY + n;
   double *g = xv + nG, *mono = g + nG;
+  /* temporary list: current neighbours within the reduced support */
+  int K = 0;
+  for (int k = 0; k < K0; k++) {
+    const int j = src[k];
+    const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
+    const double rad = sqrt(rx * rx + ry * ry);
+    if (rad <= hh * 2.0) list[K++] = j;
+  }
+  *K_out = K;
+  memset(A, 0, sizeof(double) * (size_t)n * n);
+  for (int k = 0; k < K; k++) {
+    const int j = list[k];
+    const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
+    const double x = -rx, y = -ry;
+    const double rad = sqrt(rx * rx + ry * ry);
+    const double ff1 = or_wab(rad / hh, d->h2);
+    or_abfs(mG, x / hh, y / hh, g);
+    for (int a = 0; a < nG; a++) g[a] = g[a] * ff1;
+    or_monomials(mG, x / hh, y / hh, mono);
+    for (int r = 0; r < n; r++)
+      for (int c = 0; c < n; c++) A[(size_t)c * n + r] = A[(size_t)c * n + r] + mono[r] * g[c];
+  }
+  memcpy(M, A, sizeof(double) * (size_t)n * n);
+  /* the three solves share one decomposition: svd_solve recomputes the
+   * same one from the same matrix each time */
+  memcpy(U, A, sizeof(double) * (size_t)n * n);
+  if (or_svd_gr(n, U, W, V, rv1)) return -1;
+  for (int a = 0; a < n; a++) { bL[a] = 0.0; bX[a] = 0.0; bY[a] = 0.0; }
+  bL[2] = 1.0 / hh / hh;
+  bL[4] = 1.0 / hh / hh;
+  bX[0] = 1.0 / hh;
+  bY[1] = 1.0 / hh;
+  or_svd_gr_backsub(n, U, W, V, bL, tmp);
+  or_svd_gr_backsub(n, U, W, V, bX, tmp);
+  or_svd_gr_backsub(n, U, W, V, bY, tmp);
+  /* residual of the Laplacian solution */
+  for (int a = 0; a < n; a++) xv[a] = 0.0;
+  xv[2] = -1.0 / hh / hh;
+  xv[4] = -1.0 / hh / hh;
+  double hcs = 0.0;
+  for (int r = 0; r < n; r++) {
+    for (int c = 0; c < n; c++) xv[r] = xv[r] + M[(size_t)c * n + r] * bL[c];
+    hcs = hcs + xv[r] * xv[r];
+  }
+  hcs = hcs * hh * hh;
+  hcs = sqrt(hcs / (double)n);
+  const double hcs_raw = hcs;
+  if (amp_fail) *amp_fail = 0;
+  /* amplification at wavenumbers i1*pi/(s nk), i1 = 1..nk+1 */
+  int i1 = 0;
+  if (amp_all || hcs < rtol / hh) {
+    while (i1 <= nk && (amp_all || hcs < rtol / hh)) {
+      i1 = i1 + 1;
+      double aL = 0.0, aX = 0.0, aY = 0.0, qq = 0.0;
+      for (int k = 0; k < K; k++) {
+        const int j = list[k];
+        const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
+        const double x = -rx, y = -ry;
+        const double rad = sqrt(rx * rx + ry * ry);
+        const double ff1 = or_wab(rad / hh, d->h2);
+        or_abfs(mG, x / hh, y / hh, g);
+        for (int a = 0; a < nG; a++) g[a] = g[a] * ff1;
+        qq = (double)i1 * PI_A / d->s[i] / (double)nk;
+        double dL = 0.0, dX = 0.0, dY = 0.0;
+        for (int a = 0; a < n; a++) {
+          dL = dL + bL[a] * g[a];
+          dX = dX + bX[a] * g[a];
+          dY = dY + bY[a] * g[a];
+        }
+        aL = aL + (0.5 - 0.5 * cos(x * qq) * cos(y * qq)) * dL;
+        aX = aX + cos(y * qq) * sin(x * qq) * dX;
+        aY = aY + cos(x * qq) * sin(y * qq) * dY;
+      }
+      aL = aL / (qq * qq);
+      aX = aX / qq;
+      aY = aY / qq;
+      int bad = 0;
+      if (aL > amp_tol || aL < 0.0) bad = 1;
+      if (fabs(aX) > amp_tol) bad = 1;
+      if (fabs(aY) > amp_tol) bad = 1;
+      if (bad) hcs = 2.0 * rtol / hh;
+      if (isnan(hcs)) { hcs = 2.0 * rtol / hh; bad = 1; }
+      if (bad && amp_fail && *amp_fail == 0) *amp_fail = i1;
+    }
+  }
+  *hcs_out = amp_all ? hcs_raw : hcs;
+  return 0;
+}
+
+static size_t adapt_work_size(const or_domain *d) {
+  const int nG = nsize_a(d->forder > d->morder ? d->forder : d->morder);
+  return (size_t)4 * nG * nG + 12 * (size_t)nG + 16;
+}
+
+/* One h-reduction sequence of node i (labf.F90:1151-1336 and :1517-1702).
+ * pass 1: the filter order's system on the full list, accepted reductions
+ * shorten the list (and set h = h_small); pass 2: the derivative order's
+ * system (the first nsize unknowns of the filter order's functions) on the
+ * small list, accepted reductions set h_small and the small count.  Returns
+ * the number of accepted reductions, or -1 if an SVD did not converge. */
+static int adapt_node(or_domain *d, int i, int pass, double *work, int *list) {
+  const int m = pass == 1 ? d->forder : d->morder;
+  const double reduction = (double)0.98f;
+  const double limit = (double)logf(0.6f) / log(reduction);
+  const double rtol = res_tol_of(m);
   const size_t base = (size_t)i * d->nplink;
   int ii = 0;
   for (;;) {
     const double hh = (pass == 1 ? d->h[i] : d->hs[i]) * reduction;
-    /* temporary list: current neighbours within the reduced support */
     const int K0 = pass == 1 ? d->ij_count[i] : d->ij_count_small[i];
-    int K = 0;
-    for (int k = 0; k < K0; k++) {
-      const int j = d->ij_link[base + k];
-      const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
-      const double rad = sqrt(rx * rx + ry * ry);
-      if (rad <= hh * 2.0) list[K++] = j;
-    }
-    memset(A, 0, sizeof(double) * (size_t)n * n);
-    for (int k = 0; k < K; k++) {
-      const int j = list[k];
-      const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
-      const double x = -rx, y = -ry;
-      const double rad = sqrt(rx * rx + ry * ry);
-      const double ff1 = or_wab(rad / hh, d->h2);
-      or_abfs(mG, x / hh, y / hh, g);
-      for (int a = 0; a < nG; a++) g[a] = g[a] * ff1;
-      or_monomials(mG, x / hh, y / hh, mono);
-      for (int r = 0; r < n; r++)
-        for (int c = 0; c < n; c++) A[(size_t)c * n + r] = A[(size_t)c * n + r] + mono[r] * g[c];
-    }
-    memcpy(M, A, sizeof(double) * (size_t)n * n);
-    /* the three solves share one decomposition: svd_solve recomputes the
-     * same one from the same matrix each time */
-    memcpy(U, A, sizeof(double) * (size_t)n * n);
-    if (or_svd_gr(n, U, W, V, rv1)) return -1;
-    for (int a = 0; a < n; a++) { bL[a] = 0.0; bX[a] = 0.0; bY[a] = 0.0; }
-    bL[2] = 1.0 / hh / hh;
-    bL[4] = 1.0 / hh / hh;
-    bX[0] = 1.0 / hh;
-    bY[1] = 1.0 / hh;
-    or_svd_gr_backsub(n, U, W, V, bL, tmp);
-    or_svd_gr_backsub(n, U, W, V, bX, tmp);
-    or_svd_gr_backsub(n, U, W, V, bY, tmp);
-    /* residual of the Laplacian solution */
-    for (int a = 0; a < n; a++) xv[a] = 0.0;
-    xv[2] = -1.0 / hh / hh;
-    xv[4] = -1.0 / hh / hh;
-    double hcs = 0.0;
-    for (int r = 0; r < n; r++) {
-      for (int c = 0; c < n; c++) xv[r] = xv[r] + M[(size_t)c * n + r] * bL[c];
-      hcs = hcs + xv[r] * xv[r];
-    }
-    hcs = hcs * hh * hh;
-    hcs = sqrt(hcs / (double)n);
-    /* amplification at wavenumbers i1*pi/(s nk), i1 = 1..nk+1 */
-    int i1 = 0;
-    if (hcs < rtol / hh) {
-      while (i1 <= nk && hcs < rtol / hh) {
-        i1 = i1 + 1;
-        double aL = 0.0, aX = 0.0, aY = 0.0, qq = 0.0;
-        for (int k = 0; k < K; k++) {
-          const int j = list[k];
-          const double rx = d->x[i] - d->x[j], ry = d->y[i] - d->y[j];
-          const double x = -rx, y = -ry;
-          const double rad = sqrt(rx * rx + ry * ry);
-          const double ff1 = or_wab(rad / hh, d->h2);
-          or_abfs(mG, x / hh, y / hh, g);
-          for (int a = 0; a < nG; a++) g[a] = g[a] * ff1;
-          qq = (double)i1 * PI_A / d->s[i] / (double)nk;
-          double dL = 0.0, dX = 0.0, dY = 0.0;
-          for (int a = 0; a < n; a++) {
-            dL = dL + bL[a] * g[a];
-            dX = dX + bX[a] * g[a];
-            dY = dY + bY[a] * g[a];
-          }
-          aL = aL + (0.5 - 0.5 * cos(x * qq) * cos(y * qq)) * dL;
-          aX = aX + cos(y * qq) * sin(x * qq) * dX;
-          aY = aY + cos(x * qq) * sin(y * qq) * dY;
-        }
-        aL = aL / (qq * qq);
-        aX = aX / qq;
-        aY = aY / qq;
-        if (aL > amp_tol || aL < 0.0) hcs = 2.0 * rtol / hh;
-        if (fabs(aX) > amp_tol) hcs = 2.0 * rtol / hh;
-        if (fabs(aY) > amp_tol) hcs = 2.0 * rtol / hh;
-        if (isnan(hcs)) hcs = 2.0 * rtol / hh;
-      }
-    }
+    int K;
+    double hcs;
+    if (eval_hh(d, i, m, hh, d->ij_link + base, K0, work, list, &K, &hcs, 0, NULL)) return -1;
     if ((double)ii > limit) hcs = 2.0 * rtol / hh;   /* at most ~26 reductions */
     if (hcs >= rtol / hh) break;
     /* accept: the reduced h and list */
@@ -387,14 +417,46 @@ static int adapt_node(or_domain *d, int i, int pass, double *work, int *list) {
   return ii;
 }
 
+/* The reference's acceptance checks of the stencils h_sel (npfb values, h of
+ * pass 1 or h_small of pass 2) on the current lists, which must hold every
+ * neighbour within 2 h_sel (the lists before adaptation do): per node the
+ * residual measure over its tolerance, ratio[i] = hcs / (rtol / h_sel), and
+ * amp[i] = the first of the 33 wavenumbers whose amplification check fails
+ * (0: all pass).  Nodes that adaptation leaves alone (node_type <= 2, 998)
+ * get ratio 0, amp 0.  Returns 0, or -1 if an SVD did not converge. */
+int or_adapt_check(or_domain *d, const double *h_sel, int pass, double *ratio, int *amp) {
+  const size_t wsz = adapt_work_size(d);
+  const int m = pass == 1 ? d->forder : d->morder;
+  const double rtol = res_tol_of(m);
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(| : failed)
+  for (int i = 0; i < d->npfb; i++) {
+    const int nt = d->node_type[i];
+    ratio[i] = 0.0;
+    amp[i] = 0;
+    if (nt <= 2 || nt == 998) continue;
+    double *work = malloc(sizeof(double) * wsz);
+    int *list = malloc(sizeof(int) * (size_t)d->nplink);
+    int K, af = 0;
+    double hcs = 0.0;
+    if (eval_hh(d, i, m, h_sel[i], d->ij_link + (size_t)i * d->nplink, d->ij_count[i], work,
+                list, &K, &hcs, 1, &af))
+      failed = 1;
+    ratio[i] = hcs / (rtol / h_sel[i]);
+    amp[i] = af;
+    free(work);
+    free(list);
+  }
+  return failed ? -1 : 0;
+}
+
 /* adapt_stencils + adapt_stencils_main over the real nodes (2D: every node is
  * in the first layer), after or_find_neighbours.  Boundary nodes and their
  * first FD rows (node_type <= 2) and order-reduced nodes (998) keep h.
  * reductions (nullable, 2*npfb): accepted reductions per node, pass 1 then
  * pass 2.  Returns 0, or -1 if an SVD did not converge (the reference stops). */
 int or_adapt_stencils(or_domain *d, int *reductions) {
-  const int nG = nsize_a(d->forder > d->morder ? d->forder : d->morder);
-  const size_t wsz = (size_t)4 * nG * nG + 12 * (size_t)nG + 16;   /* see adapt_node */
+  const size_t wsz = adapt_work_size(d);
   int failed = 0;
   for (int pass = 1; pass <= 2; pass++) {
 #pragma omp parallel for schedule(dynamic, 8) reduction(| : failed)

@@ -319,6 +319,23 @@ int sv_find_neighbours(sv_ctx *c, int32_t nplink, int32_t *ij_count, int32_t *ij
   return 0;
 }
 
+int sv_adapt_stencils(sv_ctx *c, double *h, double *h_small, int32_t *reductions) {
+  SV_CHECK(c, c && c->have_lists, "sv_find_neighbours must come first");
+  SV_CHECK(c, !c->have_state, "stencils are adapted before sv_set_initial (sunset.F90:40-62)");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = sv_dev_adapt(c, reductions)) return rc;
+  // the lists are rebuilt from the new h and h_small, as after
+  // refine_and_finalise_domain (sunset.F90:45-52); everything built on the
+  // old lists is stale
+  c->have_lists = c->have_weights = c->have_sums = c->have_filter = false;
+  c->have_bweights = false;
+  if (int rc = sv_dev_find_neighbours(c)) return rc;
+  if (h) SV_HIP(c, hipMemcpy(h, c->h, sizeof(double) * c->npfb, hipMemcpyDeviceToHost));
+  if (h_small)
+    SV_HIP(c, hipMemcpy(h_small, c->hs, sizeof(double) * c->npfb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int sv_calc_labf_weights(sv_ctx *c, const int32_t *order_map) {
   SV_CHECK(c, c && c->have_lists, "sv_find_neighbours must come first");
   if (int rc = set_device(c)) return rc;

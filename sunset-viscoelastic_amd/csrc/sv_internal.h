@@ -327,6 +327,8 @@ void sv_free(sv_ctx *c, void *p);
 // ---- kernels: nodes/neighbours (k_nodes.hip) ----
 int sv_dev_create_ghosts(sv_ctx *c);
 int sv_dev_find_neighbours(sv_ctx *c);
+// adapt_stencils on the current lists: h, hs, cnt, cnt_small, col_raw change
+int sv_dev_adapt(sv_ctx *c, int *reductions_host);
 int sv_dev_build_sell(sv_ctx *c);
 // time-loop layout from the final weights (after sums and filter coefficients)
 int sv_dev_build_loop(sv_ctx *c);

@@ -7,8 +7,10 @@
 !! ij_w_hyp, their sums, vol).  calc_boundary_weights (:505-1039) runs on the
 !! device too and prunes the host lists of boundary nodes as the reference does
 !! (ij_wb_grad2, used only by the compressible rhs.F90, is not kept).
-!! adapt_stencils has no device implementation: it stops with a message (run
-!! with -Drestart and supplied h, h_small).
+!! adapt_stencils (:1079-1721) runs on the device after find_neighbours,
+!! writes the adapted h, h_small of the real nodes back to common_vars and
+!! leaves the device lists rebuilt on them (ghosts kept from build_domain),
+!! which the second find_neighbours (sunset.F90:52) fetches.
 module labf
   use iso_c_binding
   use kind_parameters
@@ -71,9 +73,18 @@ contains
   end subroutine filter_coefficients
 
   subroutine adapt_stencils
-    write(6,*) "sunset_amd: adapt_stencils has no device implementation; build with", &
-               " -Drestart and supply h, h_small"
-    stop
+    real(c_double), allocatable, target :: ha(:), hsa(:)
+    allocate(ha(npfb), hsa(npfb))
+    call sv_check(sv_adapt_stencils(sv_ctx, c_loc(ha), c_loc(hsa), c_null_ptr), &
+                  "sv_adapt_stencils")
+    h(1:npfb) = ha
+    h_small(1:npfb) = hsa
+    !! the device rebuilt its lists on the adapted h; the host lists follow
+    !! (refine_and_finalise_domain frees them and find_neighbours rebuilds them)
+    call sv_check(sv_find_neighbours(sv_ctx, nplink, ij_count, ij_count_small, ij_link), &
+                  "sv_find_neighbours")
+    sv_lists_adapted = .true.
+    write(6,*) iproc, "stencils adapted"
   end subroutine adapt_stencils
 
   subroutine calc_boundary_weights

@@ -18,7 +18,8 @@ module neighbours
 contains
   subroutine find_neighbours
     integer(c_int32_t) :: npdev
-    call sv_upload_nodes()
+    if (.not. sv_lists_adapted) call sv_upload_nodes()
+    sv_lists_adapted = .false.
     !! ghosts were made by create_mirror_particles; the device builds the same set
     call sv_check(sv_get_ghosts(sv_ctx, npdev, c_null_ptr, c_null_ptr, c_null_ptr, &
                   c_null_ptr, 0), "sv_get_ghosts")

@@ -63,6 +63,11 @@ module sv_capi
        integer(c_int32_t), value :: nplink
        integer(c_int32_t), intent(out) :: cnt(*), cnts(*), link(*)
      end function sv_find_neighbours
+     integer(c_int) function sv_adapt_stencils(ctx, h, hs, reductions) &
+          bind(C, name='sv_adapt_stencils')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, h, hs, reductions
+     end function sv_adapt_stencils
      integer(c_int) function sv_calc_labf_weights(ctx, order_map) &
           bind(C, name='sv_calc_labf_weights')
        import :: c_int, c_ptr
@@ -193,6 +198,11 @@ module sv_capi
   end interface
 
   type(c_ptr), save :: sv_ctx = c_null_ptr
+  !! set by adapt_stencils: the device holds the lists rebuilt on the adapted
+  !! h with the ghosts made from the original h, as the reference keeps the
+  !! mirror particles of build_domain; the next find_neighbours only fetches
+  !! them (re-uploading would remake the ghosts from the adapted h)
+  logical, save :: sv_lists_adapted = .false.
   integer(ikind), save :: sv_ctx_npfb = -1
 
 contains

@@ -58,7 +58,8 @@ EXPORTS = [
     "sv_run_profiled", "sv_get_stencil_stats", "sv_ensemble_create", "sv_ensemble_size",
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
-    "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow", "sv_calc_boundary_weights",
+    "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow",
+    "sv_calc_boundary_weights", "sv_adapt_stencils",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -84,6 +85,7 @@ def lib():
     L.sv_set_nodes.argtypes = [C.c_void_p, C.c_int32] + [PD] * 5 + [PI32]
     L.sv_get_ghosts.argtypes = [C.c_void_p, PI32, PD, PD, PI32, PI32, C.c_int32]
     L.sv_find_neighbours.argtypes = [C.c_void_p, C.c_int32, PI32, PI32, PI32]
+    L.sv_adapt_stencils.argtypes = [C.c_void_p, PD, PD, PI32]
     L.sv_calc_labf_weights.argtypes = [C.c_void_p, PI32]
     L.sv_calc_labf_sums.argtypes = [C.c_void_p]
     L.sv_filter_coefficients.argtypes = [C.c_void_p]
@@ -217,6 +219,16 @@ class Solver:
         self._ck(lib().sv_find_neighbours(self.h, L, None, None, _i(link)))
         return cnt, cs, link - 1
 
+    def adapt_stencils(self):
+        """adapt_stencils (labf.F90:1079-1721) after find_neighbours, as
+        sunset.F90:40-52 runs it without a restart; the lists are rebuilt
+        with the new h.  Returns (h, h_small, reductions [2, npfb])."""
+        h = np.zeros(self.npfb)
+        hs = np.zeros(self.npfb)
+        red = np.zeros(2 * self.npfb, dtype=np.int32)
+        self._ck(lib().sv_adapt_stencils(self.h, _d(h), _d(hs), _i(red)))
+        return h, hs, red.reshape(2, -1)
+
     def calc_weights(self, order_map=None, persistent: Optional[bool] = None,
                      variant: Optional[int] = None, mixed: Optional[int] = None):
         """persistent=None, variant=None, mixed=None: autotuned launch
@@ -246,9 +258,11 @@ class Solver:
     def filter_coefficients(self):
         self._ck(lib().sv_filter_coefficients(self.h))
 
-    def setup(self, ns, order_map=None):
+    def setup(self, ns, order_map=None, adapt: bool = False):
         self.set_nodes(ns)
         self.find_neighbours(copy=False)
+        if adapt:
+            self.adapt_stencils()
         self.calc_weights(order_map)
         self.calc_boundary_weights()
         self.calc_sums()

@@ -109,6 +109,8 @@ def lib():
         L.or_adapt_stencils.argtypes = [C.POINTER(Domain), PI]
         L.or_svd_solve_gr.argtypes = [C.c_int, PD, PD]
         L.or_svd_gr.argtypes = [C.c_int, PD, PD, PD, PD]
+        L.or_adapt_check.argtypes = [C.POINTER(Domain), PD, C.c_int, PD, PI]
+        L.or_set_wab_pow4.argtypes = [C.c_int]
         L.or_set_stencil.argtypes = [C.POINTER(Domain), C.c_int, PI, PI, PI]
         L.or_set_weights.argtypes = [C.POINTER(Domain)] + [PD] * 7
         L.or_weights_subset.argtypes = [C.POINTER(Domain), PI, C.c_int, PI]
@@ -131,6 +133,21 @@ def _view(ptr, n, dtype=np.float64):
     if n == 0:
         return np.zeros(0, dtype=dtype)
     return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+
+class wab_pow4:
+    """Context: the oracle's Wendland (1 - q/2)^4 as libm's pow(t, 4.0)
+    (mode 0, the reference's) or correctly rounded (mode 1, the device's)."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        lib().or_set_wab_pow4(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_wab_pow4(0)
 
 
 class OracleSim:
@@ -227,6 +244,17 @@ class OracleSim:
         red = np.zeros(2 * self.d.npfb, dtype=np.int32)
         assert lib().or_adapt_stencils(C.byref(self.d), _pi(red)) == 0, "SVD not converging"
         return red.reshape(2, -1)
+
+    def adapt_check(self, h_sel, pass_):
+        """The reference's acceptance checks of stencils h_sel (pass 1: h,
+        pass 2: h_small) on the current lists: (residual measure / tolerance,
+        first failing amplification wavenumber or 0) per node."""
+        h_sel = np.ascontiguousarray(h_sel, dtype=np.float64)
+        ratio = np.zeros(self.d.npfb)
+        amp = np.zeros(self.d.npfb, dtype=np.int32)
+        assert lib().or_adapt_check(C.byref(self.d), _pd(h_sel), pass_, _pd(ratio),
+                                    _pi(amp)) == 0
+        return ratio, amp
 
     def h_hs(self):
         n = self.d.npfb

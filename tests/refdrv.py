@@ -91,8 +91,10 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
                    OMP_PLACES="cores")
         res = subprocess.run(args, cwd=td, env=env, capture_output=True, text=True,
                              timeout=timeout)
-        if res.returncode != 0:
-            raise RuntimeError(f"reference failed rc={res.returncode}\n{res.stdout}\n{res.stderr}")
+        if res.returncode != 0 or not os.path.exists(fout):
+            # a Fortran STOP exits with status 0 and no output file
+            raise RuntimeError(f"reference failed rc={res.returncode}\n{res.stdout[-4000:]}\n"
+                               f"{res.stderr[-4000:]}")
         raw = open(fout, "rb").read()
     times = {"steps": []}
     for line in res.stdout.splitlines():

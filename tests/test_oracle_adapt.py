@@ -5,7 +5,19 @@ pseudo-inverse where a singular value is exactly zero); the adapted h,
 h_small and list counts bit for bit against the reference's own run
 (committed fixtures from the flang build, and the live build when present);
 the weights rebuilt on the adapted stencils against the reference's.
-"""
+
+The residual test accepts a reduction while a rounding-level residual stays
+below a rounding-level tolerance, so the outcome depends on the last bit of
+the Wendland kernel's (1 - q/2)^4: the reference gets it from libm's
+pow(t, 4.0) (glibc: within 0.52 ulp, misrounded on ~0.1% of arguments), the
+device computes it correctly rounded.  The oracle does either
+(oracle_lib.wab_pow4); the tests below pin the correctly rounded power
+against exact rational arithmetic and measure what the last bit does to the
+selected stencils: the selections of both arithmetics pass the reference's
+amplification checks everywhere, and each passes the other's residual check
+on >= 98% of the adapted nodes, failing it elsewhere by <= 3x, a factor the
+residual measure itself moves by under one-ulp changes of the kernel."""
+from fractions import Fraction
 import ctypes as C
 import os
 import sys
@@ -149,3 +161,69 @@ def test_adapt_vs_live_reference(forder, morder, n, hos):
     assert np.array_equal(h, out["adapt_h"]) and np.array_equal(hs, out["adapt_h_small"])
     assert np.array_equal(cnt, out["adapt_count"])
     assert np.array_equal(cs, out["adapt_count_small"])
+
+
+def test_pow4_correctly_rounded_against_exact_arithmetic():
+    rng = np.random.default_rng(4)
+    t = np.concatenate([rng.random(4000), 1.0 - rng.random(500) * 1e-3, rng.random(500) * 1e-3])
+    q = 2.0 * (1.0 - t)            # or_wab(q) = (2q + 1) C (1 - q/2)^4
+    h2 = 1.0
+    C0 = 7.0 / (4.0 * np.pi * h2)
+    ok = q > 0
+    t, q = t[ok], q[ok]
+    n_libm_diff = 0
+    for ti, qi in zip(t, q):
+        ti2 = 1.0 - 0.5 * qi       # the kernel's own t from q
+        exact = float(Fraction(ti2) ** 4)          # correctly rounded by float()
+        want = (2.0 * qi + 1.0) * C0 * exact
+        with OL.wab_pow4(1):
+            got = OL.lib().or_wab(qi, h2)
+        assert got == want
+        n_libm_diff += OL.lib().or_wab(qi, h2) != want
+    # libm's pow is not correctly rounded everywhere, but within an ulp
+    assert n_libm_diff < 0.01 * len(t)
+
+
+@pytest.mark.parametrize("name", ["m6", "f6m4"])
+def test_last_bit_of_the_kernel_and_the_selected_stencils(name):
+    """The reference's selections (oracle, libm pow) against the correctly
+    rounded arithmetic's (the device's): agreement fraction, and each side's
+    stencils under the other's acceptance checks."""
+    case = {c[0]: c for c in ACASES}[name]
+    _, kind, forder, morder, n, hos, seed = case
+    ns = adapt_case(kind, forder, n, hos, seed)
+    sel = {}
+    for mode in (0, 1):
+        with OL.wab_pow4(mode):
+            o = OL.OracleSim(ns, morder, forder)
+            o.ghosts()
+            o.neighbours()
+            red = o.adapt_stencils()
+            sel[mode] = (*o.h_hs(), red)
+    g = dict(np.load(os.path.join(GOLDEN, f"ref_adapt_{name}.npz")))
+    assert np.array_equal(sel[0][0], g["h"]) and np.array_equal(sel[0][1], g["h_small"])
+    eq = (sel[0][0] == sel[1][0]) & (sel[0][1] == sel[1][1])
+    assert eq.mean() > 0.9
+    for mine, other in ((0, 1), (1, 0)):
+        h, hs, red = sel[mine]
+        with OL.wab_pow4(other):
+            o = OL.OracleSim(ns, morder, forder)
+            o.ghosts()
+            o.neighbours()
+            for p, hsel in ((1, h), (2, hs)):
+                ratio, amp = o.adapt_check(hsel, p)
+                adapted = red[p - 1] > 0
+                if not adapted.any():
+                    continue
+                assert (amp[adapted] == 0).all()
+                assert (ratio[adapted] >= 1.0).mean() <= 0.02
+                assert ratio[adapted].max() < 3.0
+        # and under its own arithmetic every adapted stencil passes
+        with OL.wab_pow4(mine):
+            o = OL.OracleSim(ns, morder, forder)
+            o.ghosts()
+            o.neighbours()
+            for p, hsel in ((1, h), (2, hs)):
+                ratio, amp = o.adapt_check(hsel, p)
+                adapted = red[p - 1] > 0
+                assert (ratio[adapted] < 1.0).all() and (amp[adapted] == 0).all()
