@@ -229,6 +229,19 @@ int sv_set_phys(sv_ctx *ctx, const sv_phys *phys);
  * emax = pid_tol unless set by sv_set_control. */
 int sv_set_initial(sv_ctx *ctx, const double *ro, const double *u, const double *v,
                    const double *cxx, const double *cxy, const double *cyy);
+/* The evolved variables of the real nodes exactly as the reference's step
+ * registers hold them (step.F90:218-512): ro, rou, rov and psi (LC) or c (DI),
+ * replacing the state sv_set_initial derived from primitives (which must come
+ * first: it builds the boundary records and inflow velocities). */
+int sv_set_state(sv_ctx *ctx, const double *ro, const double *rou, const double *rov,
+                 const double *xx, const double *xy, const double *yy);
+/* filter_variables (rhs_lc.F90:818-893; rhs_di.F90 alike) on fields the
+ * caller holds: each of the six arrays (np entries, ghosts included as the
+ * reference keeps them; psi in LC, c in DI) filtered as calc_filtered_var
+ * does, then the mass correction of ro (-Dmcorr): ro -= sum((ro - 1) vol) /
+ * sum(vol) over the real nodes.  The real-node entries are overwritten. */
+int sv_filter_variables(sv_ctx *ctx, double *ro, double *rou, double *rov, double *xx,
+                        double *xy, double *yy);
 /* restart-style control state (setup_flow.F90:495-545) */
 int sv_set_control(sv_ctx *ctx, double dt, double time, int32_t itime, double emax_np1,
                    double emax_n, double emax_nm1, int32_t dt_mode);

@@ -50,6 +50,43 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
 
 }  // namespace
 
+namespace {
+
+// The mass correction of filter_variables (rhs_lc.F90:871-890, -Dmcorr):
+// ro -= sum_i (ro_i - rho_char) vol_i / sum_i vol_i over the real nodes, with
+// rho_char = 1 (load_data.F90:132).  One block: each thread sums a strided
+// slice in node order, then a fixed-shape tree, so the result does not depend
+// on scheduling.
+__global__ __launch_bounds__(1024) void k_mass_correct(int n, const double *vol, double *ro) {
+  __shared__ double sm[1024], sv[1024];
+  const int t = threadIdx.x;
+  double tm = 0.0, tv = 0.0;
+  for (int i = t; i < n; i += 1024) {
+    tm += (ro[i] - 1.0) * vol[i];
+    tv += vol[i];
+  }
+  sm[t] = tm;
+  sv[t] = tv;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) {
+      sm[t] += sm[t + o];
+      sv[t] += sv[t + o];
+    }
+    __syncthreads();
+  }
+  const double dro = sm[0] / sv[0];
+  for (int i = t; i < n; i += 1024) ro[i] = ro[i] - dro;
+}
+
+}  // namespace
+
+int sv_dev_mass_correct(sv_ctx *c, double *ro, hipStream_t st) {
+  k_mass_correct<<<1, 1024, 0, st>>>(c->npfb, c->vol, ro);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_cols,
                  hipStream_t st) {
   ApplyArgs a{};

@@ -750,6 +750,58 @@ int sv_set_initial(sv_ctx *c, const double *ro, const double *u, const double *v
   return 0;
 }
 
+int sv_set_state(sv_ctx *c, const double *ro, const double *rou, const double *rov,
+                 const double *xx, const double *xy, const double *yy) {
+  SV_CHECK(c, c && ro && rou && rov && xx && xy && yy, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  SV_CHECK(c, c->have_state, "sv_set_initial must come first (boundary records, inflow velocity)");
+  if (int rc = set_device(c)) return rc;
+  const int n = c->npfb;
+  for (int i = 0; i < n; i++) {
+    SV_CHECK(c, ro[i] > 0, "node %d: density must be positive", i);
+    if (c->cfg.form == SV_FORM_DI)
+      SV_CHECK(c, xx[i] * yy[i] - xy[i] * xy[i] > 0 && xx[i] > 0,
+               "node %d: conformation tensor not positive definite", i);
+  }
+  std::vector<double> rec((size_t)SV_US * n);
+  SV_HIP(c, hipMemcpy(rec.data(), c->U, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+  const std::vector<int> &P = c->L.hperm;   // record k holds the caller's node P[k]
+  for (int k = 0; k < n; k++) {
+    double *r = rec.data() + (size_t)SV_US * k;
+    const int i = P[k];
+    r[0] = ro[i]; r[1] = rou[i]; r[2] = rov[i]; r[3] = xx[i]; r[4] = xy[i]; r[5] = yy[i];
+  }
+  SV_HIP(c, hipMemcpy(c->U, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+  return sv_dev_init_state(c);
+}
+
+int sv_filter_variables(sv_ctx *c, double *ro, double *rou, double *rov, double *xx, double *xy,
+                        double *yy) {
+  SV_CHECK(c, c && ro && rou && rov && xx && xy && yy, "null argument");
+  SV_CHECK(c, c->have_filter, "weights, sums and filter coefficients must come first");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_sell(c)) return rc;
+  const int np = c->np, n = c->npfb;
+  DevTmp<double> in, out;
+  SV_HIP(c, in.alloc(6 * (size_t)np));
+  SV_HIP(c, out.alloc(6 * (size_t)n));
+  double *f[6] = {ro, rou, rov, xx, xy, yy};
+  for (int k = 0; k < 6; k++) {
+    SV_HIP(c, hipMemcpyAsync(in.p + (size_t)k * np, f[k], sizeof(double) * np,
+                             hipMemcpyHostToDevice, c->stream));
+    // calc_filtered_var on the caller's ghost values (raw ghost columns)
+    if (int rc = sv_dev_apply(c, SV_OP_FILTER, in.p + (size_t)k * np, out.p + (size_t)k * n,
+                              true, c->stream))
+      return rc;
+  }
+  if (int rc = sv_dev_mass_correct(c, out.p, c->stream)) return rc;
+  for (int k = 0; k < 6; k++)
+    SV_HIP(c, hipMemcpyAsync(f[k], out.p + (size_t)k * n, sizeof(double) * n,
+                             hipMemcpyDeviceToHost, c->stream));
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int sv_set_control(sv_ctx *c, double dt, double time, int32_t itime, double e1, double e2,
                    double e3, int32_t dt_mode) {
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");

@@ -343,6 +343,8 @@ int sv_dev_boundary_weights(sv_ctx *c);
 // ---- kernels: stencil operators (k_stencil.hip) ----
 int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_cols,
                  hipStream_t st);
+// ro (npfb, caller order) -= the volume-weighted mean deviation from 1
+int sv_dev_mass_correct(sv_ctx *c, double *ro, hipStream_t st);
 // ---- kernels: time stepping (k_step.hip) ----
 int sv_dev_init_state(sv_ctx *c);
 // one step; ev (nullable) gets SV_NKERNELS+1 events recorded between launches

@@ -122,6 +122,32 @@ module sv_capi
        real(c_double), value :: dt, time, e1, e2, e3
        integer(c_int32_t), value :: itime, dt_mode
      end function sv_set_control
+     integer(c_int) function sv_set_state(ctx, ro, rou, rov, xx, xy, yy) &
+          bind(C, name='sv_set_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: ro(*), rou(*), rov(*), xx(*), xy(*), yy(*)
+     end function sv_set_state
+     integer(c_int) function sv_get_control(ctx, dt, time, itime, emax3, dt_cfl, dt_par) &
+          bind(C, name='sv_get_control')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: dt, time, emax3(3), dt_cfl, dt_par
+       integer(c_int32_t), intent(out) :: itime
+     end function sv_get_control
+     integer(c_int) function sv_calc_all_rhs(ctx, irk, r0, r1, r2, r3, r4, r5) &
+          bind(C, name='sv_calc_all_rhs')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: irk
+       real(c_double), intent(out) :: r0(*), r1(*), r2(*), r3(*), r4(*), r5(*)
+     end function sv_calc_all_rhs
+     integer(c_int) function sv_filter_variables(ctx, ro, rou, rov, xx, xy, yy) &
+          bind(C, name='sv_filter_variables')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: ro(*), rou(*), rov(*), xx(*), xy(*), yy(*)
+     end function sv_filter_variables
      integer(c_int) function sv_run(ctx, nsteps, use_graph) bind(C, name='sv_run')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
@@ -270,4 +296,36 @@ contains
     sv_ctx_npfb = npfb
   end subroutine sv_upload_nodes
 
+
+  !! The physics of control.in as common_vars holds it (load_data.F90:21-159)
+  subroutine sv_upload_phys()
+    type(sv_phys_t) :: ph
+    ph%L_char = L_char;ph%U_char = U_char;ph%Re = Re;ph%Wi = Wi;ph%beta = beta
+    ph%epsPTT = epsPTT;ph%Ma = Ma;ph%Mdiff = Mdiff
+    ph%grav = grav;ph%driving_force = driving_force
+    ph%kol_amp = 0.0d0;ph%kol_k = 0.0d0
+    call sv_check(sv_set_phys(sv_ctx, ph), "sv_set_phys")
+  end subroutine sv_upload_phys
+
+  !! The host state of the real nodes onto the device: sv_set_initial builds
+  !! the boundary records and inflow velocities from the primitives, then the
+  !! registers are replaced by the host's own values bit for bit; the control
+  !! state (dt, time, itime, PID error history) follows.
+  subroutine sv_upload_state(dt_mode)
+    integer(c_int32_t), intent(in) :: dt_mode
+    call sv_upload_phys()
+    if (nb /= 0) call sv_check(sv_set_inflow(sv_ctx, inflow_type, inflow_velocity_control, &
+                  u_inflow_start, u_inflow_end, ramp_time), "sv_set_inflow")
+    call sv_check(sv_set_initial(sv_ctx, ro(1:npfb), u(1:npfb), v(1:npfb), cxx(1:npfb), &
+                  cxy(1:npfb), cyy(1:npfb)), "sv_set_initial")
+#ifdef di
+    call sv_check(sv_set_state(sv_ctx, ro(1:npfb), rou(1:npfb), rov(1:npfb), cxx(1:npfb), &
+                  cxy(1:npfb), cyy(1:npfb)), "sv_set_state")
+#else
+    call sv_check(sv_set_state(sv_ctx, ro(1:npfb), rou(1:npfb), rov(1:npfb), psixx(1:npfb), &
+                  psixy(1:npfb), psiyy(1:npfb)), "sv_set_state")
+#endif
+    call sv_check(sv_set_control(sv_ctx, dt, time, int(itime, c_int32_t), emax_np1, emax_n, &
+                  emax_nm1, dt_mode), "sv_set_control")
+  end subroutine sv_upload_state
 end module sv_capi

@@ -59,7 +59,7 @@ EXPORTS = [
     "sv_ensemble_get_state", "sv_ensemble_get_control", "sv_adj_setup", "sv_rhs_vjp",
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
     "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow",
-    "sv_calc_boundary_weights", "sv_adapt_stencils",
+    "sv_calc_boundary_weights", "sv_adapt_stencils", "sv_set_state", "sv_filter_variables",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -106,6 +106,8 @@ def lib():
                                  C.c_double, C.c_double, C.c_int32]
     L.sv_get_control.argtypes = [C.c_void_p, PD, PD, PI32, PD, PD, PD]
     L.sv_calc_all_rhs.argtypes = [C.c_void_p, C.c_int32] + [PD] * 6
+    L.sv_set_state.argtypes = [C.c_void_p] + [PD] * 6
+    L.sv_filter_variables.argtypes = [C.c_void_p] + [PD] * 6
     L.sv_run.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.sv_get_state.argtypes = [C.c_void_p] + [PD] * 11
     L.sv_sync.argtypes = [C.c_void_p]

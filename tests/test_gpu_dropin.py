@@ -73,3 +73,54 @@ def test_dropin_adapt_stencils_matches_golden(name):
     assert np.array_equal(out["adapt_h_small"], hs)
     assert np.array_equal(out["ij_count"], cnt)
     assert np.array_equal(out["ij_count_small"], cs)
+
+
+def _check_fields(out, g, keys=("ro", "rou", "rov", "pxx", "pxy", "pyy", "cxx", "cxy", "cyy"),
+                  conf_rtol=FIELD_RTOL):
+    for k in keys:
+        err = field_rel(out[k], g[f"f_{k}"])
+        print(f"  {k}: {err:.2e}")
+        assert err < (FIELD_RTOL if k in ("ro", "rou", "rov") else conf_rtol), k
+    for q, name in enumerate(("dt", "time")):
+        assert abs(out[name] - g["ctl"][q]) <= 1e-10 * abs(g["ctl"][q]), name
+    assert out["itime"] == int(g["itime"])
+
+
+@pytest.mark.parametrize("name", ["m4_h18", "m6_20"])
+@pytest.mark.parametrize("kind", ["dropstep", "droprhs"])
+def test_dropin_step_and_rhs_match_golden(kind, name):
+    """The reference's own time loop (ref_driver: set_tstep, set_tstep_PID,
+    step_rk3_4S_2R_EE) with the step module replaced (the device's fused
+    step, state device-resident) or the rhs module replaced (the reference's
+    step calling the device's calc_all_rhs and filter_variables)."""
+    g = load_golden(name)
+    order = int(g["order"])
+    binary = os.path.join(refdrv.REF_DIR, f"sunset_{kind}_lc_m{order}")
+    if not os.path.exists(binary):
+        pytest.skip("drop-in binary not built (make -C oracle dropin)")
+    print(f"[gpu-dropin] {kind} {name}", flush=True)
+    ns = golden_nodeset(g)
+    out, _ = refdrv.run_reference(ns, order, taylor_green_ic(ns), nsteps=int(g["nsteps"]),
+                                  phys=golden_phys(g), binary=binary, timeout=300,
+                                  fields_only=True)
+    _check_fields(out, g)
+
+
+@pytest.mark.parametrize("name", ["wall_m6", "io_m6_soft", "io_m4_ramp"])
+def test_dropin_step_with_boundaries_matches_golden(name):
+    """The step drop-in on node sets with walls, or an inflow and an outflow:
+    characteristic boundary conditions and the inflow ramp run in the device
+    step inside the reference's time loop."""
+    from parity_util import golden_bcase
+    g, ns, ic, phys, drive, inflow = golden_bcase(name)
+    order = int(g["order"])
+    binary = os.path.join(refdrv.REF_DIR, f"sunset_dropstep_lc_m{order}")
+    if not os.path.exists(binary):
+        pytest.skip("drop-in binary not built (make -C oracle dropin)")
+    print(f"[gpu-dropin] dropstep {name}", flush=True)
+    out, _ = refdrv.run_reference(ns, order, ic, nsteps=int(g["nsteps"]),
+                                  phys=dict(phys, drive=(*drive, 0.0)), inflow=inflow,
+                                  binary=binary, timeout=300, fields_only=True)
+    # the conformation at 1e-8 on these sets, as the device's own run
+    # (tests/test_gpu_bound.py)
+    _check_fields(out, g, keys=("ro", "rou", "rov", "pxx", "pxy", "pyy"), conf_rtol=1e-8)
