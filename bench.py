@@ -440,6 +440,64 @@ def adjoint_leg(n=128, steps=400, every=20):
                         "oracle's tangent-linear rollout, same 400 steps"))
 
 
+def adapt_leg(n=512, check_n=96, cpu_n=48, no_cpu=False):
+    """Stencil adaptation (adapt_stencils, labf.F90:1079-1721) of an n^2
+    order-6 lattice on the device, list rebuild included; checked against the
+    oracle in the device's arithmetic (correctly rounded Wendland power) on a
+    check_n^2 lattice, and timed beside the reference's own serial loop on a
+    cpu_n^2 lattice."""
+    import numpy as np
+    import sunset_amd
+    from sunset_amd.nodes import jittered_lattice, taylor_green_ic
+    ns = jittered_lattice(n, seed=3, h_over_s=2.4)
+    s = sunset_amd.Solver(morder=6, form="lc")
+    s.set_nodes(ns)
+    s.find_neighbours(copy=False)
+    t0 = time.perf_counter()
+    h, hs, red = s.adapt_stencils()
+    ms = (time.perf_counter() - t0) * 1e3
+    s.close()
+    out = {"nodes": ns.npfb, "order": 6, "ms": round(ms, 1),
+           "nodes_per_s": round(ns.npfb / (ms / 1e3), 1),
+           "mean_reductions": round(float(red[0].mean()), 3),
+           "h_reduced_fraction": round(float((red[0] > 0).mean()), 4)}
+    if no_cpu:
+        return out
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    import refdrv
+    nc = jittered_lattice(check_n, seed=77, h_over_s=2.4)
+    s = sunset_amd.Solver(morder=6, form="lc")
+    s.set_nodes(nc)
+    s.find_neighbours(copy=False)
+    dh, dhs, dred = s.adapt_stencils()
+    s.close()
+    with OL.wab_pow4(1):
+        o = OL.OracleSim(nc, 6)
+        o.ghosts()
+        o.neighbours()
+        ored = o.adapt_stencils()
+        oh, ohs = o.h_hs()
+    out["error_vs_oracle"] = {
+        "nodes": nc.npfb, "equal_fraction": float(((dh == oh) & (dhs == ohs)).mean()),
+        "max_reduction_difference": int(np.abs(dred - ored).max()),
+        "how": "h and h_small of every node vs the oracle's adapt_stencils in the same "
+               "arithmetic (correctly rounded (1-q/2)^4), 96^2 lattice"}
+    if refdrv.available(6):
+        ncp = jittered_lattice(cpu_n, seed=3, h_over_s=2.4)
+        _, t = refdrv.run_reference(ncp, 6, taylor_green_ic(ncp), nsteps=0, adapt=True,
+                                    threads=1, fields_only=True, timeout=600)
+        sec = t.get("adapt_stencils")
+        if sec:
+            out["cpu_baseline"] = {
+                "value": round(ncp.npfb / sec, 1), "unit": "nodes/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{cpu_n}^2 lattice ({ncp.npfb} nodes), order 6, adapt_stencils "
+                          f"{sec:.2f} s (serial in the reference, labf.F90:1145-1148)"}
+            out["x_vs_cpu"] = round(out["nodes_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +515,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-boundary", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-adapt", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -592,6 +651,11 @@ def main():
         print("[bench] adjoint: 128^2 DI rollout of 400 steps", file=sys.stderr, flush=True)
         adj = adjoint_leg()
 
+    adapt = None
+    if not args.no_adapt:
+        print("[bench] stencil adaptation: 512^2 order 6", file=sys.stderr, flush=True)
+        adapt = adapt_leg(no_cpu=args.no_cpu)
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "node-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -619,6 +683,7 @@ def main():
         "wall_channel": bnd,
         "ensemble": ens,
         "adjoint": adj,
+        "stencil_adaptation": adapt,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint_vjp_over_forward": adj["vjp_over_forward"] if adj else None,
     }
