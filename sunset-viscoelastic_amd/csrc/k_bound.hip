@@ -19,31 +19,38 @@ namespace {
 
 constexpr int BT = 64;
 
-// rbfs.F90:58-68: the Wendland C2 derivative kernel, h3 = h0^3
+// rbfs.F90:58-68: the Wendland C2 derivative kernel (7/(4 pi h0^3)) times
+// (5/8 q^4 - 15/4 q^3 + 15/2 q^2 - 5 q) on 0 < q <= 2, in Horner form
 __device__ __forceinline__ double fac_b(double qq, double h3) {
-  const double qq2 = qq * qq, qq3 = qq * qq2, qq4 = qq * qq3;
-  double f = 0.0;
-  if (qq <= 2.0 && qq > 0.0)
-    f = (7.0 / (4.0 * SV_PI * h3)) * (0.625 * qq4 - 3.75 * qq3 + 7.5 * qq2 - 5.0 * qq);
-  return f;
+  if (!(qq <= 2.0 && qq > 0.0)) return 0.0;
+  const double poly = qq * (-5.0 + qq * (7.5 + qq * (-3.75 + qq * 0.625)));
+  return poly * (7.0 / (4.0 * SV_PI * h3));
 }
 
-// monomials1D (labf.F90:1041-1048)
+// monomials1D (labf.F90:1041-1048): z^k / k!, k = 1..6, by the recurrence
+// t_k = t_{k-1} z / k
 __device__ __forceinline__ void mono1(double z, double c[6]) {
-  const double z2 = z * z;
-  c[0] = z; c[1] = 0.5 * z2; c[2] = (1.0 / 6.0) * z * z2; c[3] = z2 * z2 / 24.0;
-  c[4] = z2 * z2 * z / 120.0; c[5] = z2 * z2 * z2 / 720.0;
+  double t = z;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    c[k] = t;
+    t = t * z / (double)(k + 2);
+  }
 }
 
-// abfs1D with Hermite ABFs (labf.F90:1050-1077, Hermite1..6 :2329-2368)
+// abfs1D (labf.F90:1050-1077): Hermite ABFs He_1..He_6 (probabilists'), from
+// the three-term recurrence He_{k+1} = z He_k - k He_{k-1}, times ff1; the
+// reference scales He_3 by 1/sqrt(2) (:1060)
 __device__ __forceinline__ void abf1(double z, double ff1, double c[6]) {
-  const double z2 = z * z, z3 = z * z2;
-  c[0] = ff1 * z;
-  c[1] = ff1 * (z * z - 1.0);
-  c[2] = ff1 * (1.0 / sqrt(2.0)) * (z * z * z - 3.0 * z);
-  c[3] = ff1 * (z * z * z * z - 6.0 * z * z + 3.0);
-  c[4] = ff1 * (z2 * z3 - 10.0 * z3 + 15.0 * z);
-  c[5] = ff1 * (z3 * z3 - 15.0 * z2 * z2 + 45.0 * z2 - 15.0);
+  double hm = 1.0, hk = z;   // He_0, He_1
+#pragma unroll
+  for (int k = 1; k <= 6; k++) {
+    c[k - 1] = ff1 * hk;
+    const double hn = z * hk - (double)k * hm;
+    hm = hk;
+    hk = hn;
+  }
+  c[2] *= 0.70710678118654752440;
 }
 
 // 6x6 solve A x = b for NR right-hand sides (A column-major, A[c*6+r]):
