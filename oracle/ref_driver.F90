@@ -115,6 +115,16 @@ program sunset_ref
   npfb_layer = npfb;npfb_global = npfb
 
   call create_mirror_particles
+  !! A corner between a periodic and a no-slip edge makes a ghost whose
+  !! position and vrelation create_mirror_particles never writes
+  !! (mirror_boundaries.F90:149-240 set them only for the periodic/symmetric
+  !! pairs); the position then holds what the allocation held (zero here),
+  !! the vrelation any value, and reapply_mirror_bcs leaves such a ghost's
+  !! momentum unset.  Define it as a plain copy of the parent (vrelation 1),
+  !! as the oracle reads it.
+  do i=npfb+1,np
+     if(vrelation(i).lt.1.or.vrelation(i).gt.4) vrelation(i) = 1
+  end do
   !! boundary and internal lists as build_domain (setup_domain.F90:314-346)
   nb = 0
   do i=1,npfb

@@ -149,13 +149,19 @@ def test_order8_oracle_vs_extended_precision_and_reference():
 
 
 @pytest.mark.skipif(not refdrv.available(6), reason="oracle/_ref not built")
-@pytest.mark.parametrize("bc", [(2, 2, 1, 1), (1, 1, 2, 2), (2, 2, 2, 2)])
+@pytest.mark.parametrize("bc", [(2, 2, 1, 1), (1, 1, 2, 2), (2, 2, 2, 2), (1, 1, 3, 3),
+                                (2, 2, 3, 3)])
 def test_mirror_edges_vs_reference(bc):
-    """Symmetric edges and symmetric-symmetric corners (vrelation 2, 3, 4):
-    ghosts, lists, weights and 20 LC steps against the compiled reference.
-    (No-slip edges, bcond 3, abort inside the reference's own run in this
-    harness -- "double free or corruption" -- so they are pinned only through
-    the corner case, which uses the same vrelation 4 sign flips.)"""
+    """Symmetric edges and symmetric-symmetric corners (vrelation 2, 3, 4)
+    and no-slip edges (bcond 3, vrelation 4; the reference makes them only on
+    the y edges, mirror_boundaries.F90:111-147): ghosts, lists, weights and 20
+    LC steps against the compiled reference.  No-slip edges give the nodes
+    next to them ~360 neighbours (their mirror images and the corner ghosts),
+    so the reference runs with nplink 400 (its default, 200, overflows).  A
+    corner between a no-slip edge and a periodic or symmetric one makes ghosts
+    whose position and vrelation the reference never writes; the harness
+    defines them (position zero as allocated, vrelation 1: a copy of the
+    parent), as the oracle reads them."""
     import numpy as np
     from sunset_amd.nodes import jittered_lattice
     ns = jittered_lattice(20, seed=5, h_over_s=2.4, bc=bc)
@@ -166,8 +172,9 @@ def test_mirror_edges_vs_reference(bc):
               cxx=1 + 0.05 * np.cos(np.pi * x) ** 2, cxy=0.02 * np.sin(np.pi * x) * np.sin(np.pi * y),
               cyy=np.ones(n))
     ph = dict(Re=10.0, Wi=0.5, beta=0.5, eps=0.1, Ma=0.1)
-    R, _ = refdrv.run_reference(ns, 6, ic, nsteps=20, phys=ph)
-    o = OL.OracleSim(ns, 6).setup()
+    nplink = 400 if 3 in bc else 0
+    R, _ = refdrv.run_reference(ns, 6, ic, nsteps=20, phys=ph, nplink=nplink)
+    o = OL.OracleSim(ns, 6, nplink=nplink or None).setup()
     gx, gy = o.xy()
     assert np.array_equal(gx, R["x"]) and np.array_equal(gy, R["y"])
     assert np.array_equal(o.irel()[n:], R["irel"])
