@@ -33,7 +33,11 @@ namespace {
 
 constexpr int nsize(int m) { return (m * m + 3 * m) / 2; }
 constexpr int npad(int m) { return ((nsize(m) + 15) / 16) * 16; }
-constexpr double fact(int k) { return k <= 1 ? 1.0 : k * fact(k - 1); }
+// k! for k <= 10 as a table: a recursive constexpr function used outside a
+// constant expression is not folded, and the device code then calls it (a
+// call per monomial, with its stack frame and the register budget of a call)
+constexpr double kFact[11] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
+                              362880.0, 3628800.0};
 
 // Probabilists' Hermite polynomials He_0..He_M, written as labf.F90:2329-2391
 template <int M>
@@ -79,7 +83,7 @@ __device__ __forceinline__ void mono_row(double x, double y, double *dst) {
 #pragma unroll
     for (int b = 0; b <= d; b++) {
       const int a = d - b;
-      const double coef = 1.0 / (fact(a) * fact(b));
+      const double coef = 1.0 / (kFact[a] * kFact[b]);
       double v;
       if (b == 0) v = (a == 1) ? x : coef * xp[a];
       else if (a == 0) v = (b == 1) ? y : coef * yp[b];
@@ -288,7 +292,10 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD
 #pragma unroll
   for (int a = 0; a < N; a++) ps[a] = Xs[a * 4 + q];
   const bool want = q < 3 ? sysD : sysF;
-  double *dst = q == 0 ? S_.wx : q == 1 ? S_.wy : q == 2 ? S_.wl : S_.wh;
+  // (a two-level select: a four-way one can be lowered to a table in scratch)
+  double *const d01 = (q & 1) ? S_.wy : S_.wx;
+  double *const d23 = (q & 1) ? S_.wh : S_.wl;
+  double *const dst = (q & 2) ? d23 : d01;
   const double rhh = 1.0 / hh;
   double vsum = 0.0;
   for (int k = kl; k < Kfull; k += SV_WAVE / 4) {
