@@ -71,7 +71,7 @@ __device__ __forceinline__ double wab(double q, double h2) {
 }
 
 // Row of Taylor monomials x^a y^b/(a!b!) in graded order (labf.F90:1815-1920)
-template <int M>
+template <int M, int STRIDE = 1>
 __device__ __forceinline__ void mono_row(double x, double y, double *dst) {
   double xp[M + 1], yp[M + 1];
   xp[0] = 1.0; yp[0] = 1.0; xp[1] = x; yp[1] = y;
@@ -91,12 +91,12 @@ __device__ __forceinline__ void mono_row(double x, double y, double *dst) {
       else if (a == 1) v = (coef * x) * yp[b];
       else if (b == 1) v = (coef * xp[a]) * y;
       else v = (coef * xp[a]) * yp[b];
-      dst[idx++] = v;
+      dst[STRIDE * idx++] = v;
     }
 }
 
 // Row of Hermite ABFs He_a(x) He_b(y) * w (labf.F90:2017-2120)
-template <int M>
+template <int M, int STRIDE = 1>
 __device__ __forceinline__ void abf_row(double x, double y, double w, double *dst) {
   double hx[M + 1], hy[M + 1];
   hermite_all<M>(x, hx);
@@ -108,7 +108,7 @@ __device__ __forceinline__ void abf_row(double x, double y, double w, double *ds
     for (int b = 0; b <= d; b++) {
       const int a = d - b;
       const double v = (b == 0) ? hx[a] : (a == 0) ? hy[b] : hx[a] * hy[b];
-      dst[idx++] = v * w;
+      dst[STRIDE * idx++] = v * w;
     }
 }
 
@@ -199,21 +199,29 @@ struct Lds {
   static constexpr int G = Pack<M>::G;
   static constexpr int ASZ = N * LDA;
   static constexpr int NA = Pack<M>::A0_REGS ? 1 : G;
-  static constexpr int CHUNK = 2 * CH * NP;
+  // 16-neighbour chunks are stored column-major (function f of neighbour k
+  // at f * LDC + k, LDC = 17: the 16 lanes writing one function hit distinct
+  // banks; row-major, all 16 rows started on the same bank); 32-neighbour
+  // chunks row-major (measured faster for them)
+  static constexpr bool TR = (CH == 16);
+  static constexpr int LDC = TR ? CH + 1 : 1;
+  static constexpr int CHUNK = TR ? 2 * LDC * NP : 2 * CH * NP;
   static constexpr int XS = G * NP * 4;
   static constexpr int CSZ = CHUNK > XS ? CHUNK : XS;
   static constexpr int TOTAL = NA * ASZ + CSZ;
 };
 
 // Moment matrix A = sum_k X_k g_k^T (labf.F90:88-123) on the FP64 matrix
-// cores: 32-neighbour chunks of Taylor monomial rows X and ABF rows g are
-// staged in LDS (Xc, Gc: [CH][NP]) and contracted with
+// cores: 16- or 32-neighbour chunks of Taylor monomial rows X and ABF rows g
+// are staged in LDS (Xc, Gc) and contracted with
 // v_mfma_f64_16x16x4_f64; the result is written to Adst ([NP][LDA], LDS).
 template <int M, int CH_>
 __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, int Kuse,
                                          int64_t base, double *Xc, double *Gc, double *Adst) {
   using L = Lds<M, CH_>;
   constexpr int N = L::N, NP = L::NP, LDA = L::LDA, CH = L::CH, T = NP / 16;
+  constexpr bool TR = L::TR;
+  constexpr int LDC = L::LDC;
   const int l = threadIdx.x;
   const double xi = S_.x[i], yi = S_.y[i];
   dbl4 acc[T][T];
@@ -227,22 +235,22 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
   const double rhh = 1.0 / hh;
   for (int c0 = 0; c0 < Kuse; c0 += CH) {
     const int kk = c0 + (l % CH);
-    double *row = (role == 0 ? Xc : Gc) + (l % CH) * NP;
+    double *row = (role == 0 ? Xc : Gc) + (TR ? (l % CH) : (l % CH) * NP);
     if (role < 2 && kk < Kuse) {
       const int j = S_.col[base + kk];
       const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
       const double xs = x * rhh, ys = y * rhh;
       if (role == 0) {
-        mono_row<M>(xs, ys, row);
+        mono_row<M, TR ? LDC : 1>(xs, ys, row);
       } else {
         const double rad = sqrt(x * x + y * y) * rhh;
-        abf_row<M>(xs, ys, wab(rad, S_.h2), row);
+        abf_row<M, TR ? LDC : 1>(xs, ys, wab(rad, S_.h2), row);
       }
-#pragma unroll
-      for (int a = N; a < NP; a++) row[a] = 0.0;
+      // the padding functions N..NP-1 reach only the rows and columns of A
+      // that are dropped below, and are left unset
     } else if (role < 2) {
 #pragma unroll
-      for (int a = 0; a < NP; a++) row[a] = 0.0;
+      for (int a = 0; a < N; a++) row[TR ? a * LDC : a] = 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -250,10 +258,12 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
       const int kr = ks * 4 + (l >> 4);
 #pragma unroll
       for (int ti = 0; ti < T; ti++) {
-        const double av = Xc[kr * NP + ti * 16 + (l & 15)];
+        const int fa = ti * 16 + (l & 15);
+        const double av = TR ? Xc[fa * LDC + kr] : Xc[kr * NP + fa];
 #pragma unroll
         for (int tj = 0; tj < T; tj++) {
-          const double bv = Gc[kr * NP + tj * 16 + (l & 15)];
+          const int fb = tj * 16 + (l & 15);
+          const double bv = TR ? Gc[fb * LDC + kr] : Gc[kr * NP + fb];
           acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ti][tj], 0, 0, 0);
         }
       }
@@ -412,7 +422,7 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
   for (int gg = 0; gg < nasm; gg++) {
     const SysOf o = sys_of<M>(S_, first + gg);
     double *Ag = A0 + gg * ASZ;
-    assemble<M, CH_>(S_, o.i, o.hh, o.Kuse, o.base, C, C + CH_ * NP, Ag);
+    assemble<M, CH_>(S_, o.i, o.hh, o.Kuse, o.base, C, C + L::CHUNK / 2, Ag);
     if (o.nkeep < N && l < N) {
       // a rolled loop: unrolled, its N identity entries would be hoisted to
       // the kernel entry and held in registers throughout
