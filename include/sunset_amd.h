@@ -235,6 +235,16 @@ int sv_set_initial(sv_ctx *ctx, const double *ro, const double *u, const double 
  * first: it builds the boundary records and inflow velocities). */
 int sv_set_state(sv_ctx *ctx, const double *ro, const double *rou, const double *rov,
                  const double *xx, const double *xy, const double *yy);
+/* Device-resident forms of sv_set_state / the registers of sv_get_state:
+ * U[6] = (ro, rou, rov, xx, xy, yy) as device pointers (npfb each, the
+ * caller's node order) on the context's GPU in this library's HIP runtime
+ * (from a Python process, import torch first so that the library binds to
+ * torch's runtime and its tensors' data pointers qualify); the caller's
+ * stream (nullable) is synchronised first.  A pointer of another runtime or
+ * device is refused with SV_E_ARG.  No counterpart in the reference: these
+ * serve device-tensor bindings (no host copy). */
+int sv_set_state_device(sv_ctx *ctx, const double *const U[6], void *stream);
+int sv_get_state_device(sv_ctx *ctx, double *const U[6], void *stream);
 /* filter_variables (rhs_lc.F90:818-893; rhs_di.F90 alike) on fields the
  * caller holds: each of the six arrays (np entries, ghosts included as the
  * reference keeps them; psi in LC, c in DI) filtered as calc_filtered_var
@@ -352,6 +362,11 @@ int sv_rollout_forward(sv_ctx *ctx, int32_t nsteps, double *loss);
  * the context's state is left at the end of the rollout */
 int sv_rollout_vjp(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
                    double *const grad0[6]);
+/* As sv_rollout_vjp with ubar_final (nullable, or null fields) and grad0 as
+ * device pointers (see sv_set_state_device); the caller's stream (nullable)
+ * is synchronised first, the result is complete on return. */
+int sv_rollout_vjp_device(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
+                          double *const grad0[6], void *stream);
 /* gradient of the same quantity with respect to the parameters, from the last
  * sv_rollout_vjp (or, for <rbar, rhs>, the last sv_rhs_vjp):
  * grad[0] = d/dWi, grad[1] = d/dbeta, grad[2] = d/d(epsPTT) */

@@ -804,6 +804,49 @@ static int download_records(sv_ctx *c, const double *src, double *const f[6]) {
   return 0;
 }
 
+// caller-order device arrays <-> loop-order records, on the device
+struct Fields6 {
+  const double *f[6];
+};
+struct OutFields6 {
+  double *f[6];
+};
+
+__global__ void k_gather_records(int n, const int *perm, Fields6 in, double *rec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+#pragma unroll
+  for (int q = 0; q < 6; q++) rec[(size_t)SV_US * k + q] = in.f[q] ? in.f[q][i] : 0.0;
+}
+
+__global__ void k_scatter_records(int n, const int *perm, const double *rec, OutFields6 out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    if (out.f[q]) out.f[q][i] = rec[(size_t)SV_US * k + q];
+}
+
+int sv_dev_gather_records(sv_ctx *c, double *dst, const double *const f[6]) {
+  Fields6 in{};
+  for (int q = 0; q < 6; q++) in.f[q] = f ? f[q] : nullptr;
+  const int n = c->npfb;
+  k_gather_records<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->L.perm, in, dst);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int sv_dev_scatter_records(sv_ctx *c, const double *src, double *const f[6]) {
+  OutFields6 out{};
+  for (int q = 0; q < 6; q++) out.f[q] = f[q];
+  const int n = c->npfb;
+  k_scatter_records<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->L.perm, src, out);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 #define SV_ADJ_CHECK(c, cond, ...)                            \
   do {                                                        \
     if (!(cond)) return sv_fail((c), SV_E_ARG, __VA_ARGS__);  \
@@ -930,8 +973,16 @@ int sv_rollout_forward(sv_ctx *c, int32_t nsteps, double *loss) {
 // (ro, rou, rov, cxx, cxy, cyy) at the start of the last sv_rollout_forward.
 // ubar_final may be NULL (or hold NULL fields).  The context's state is left
 // at the end of the rollout.
-int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6],
-                   double *const grad0[6]) {
+// the loss term's seed on cxx and cyy of every real node's record
+__global__ void k_seed_loss(int n, double seed, double *rec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  rec[(size_t)SV_US * k + 3] += seed;
+  rec[(size_t)SV_US * k + 5] += seed;
+}
+
+static int rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6],
+                       double *const grad0[6], bool device_io) {
   SV_ADJ_CHECK(c, c && c->adj && c->adj->T > 0, "sv_rollout_forward must come first");
   SV_ADJ_CHECK(c, grad0, "null output");
   sv_adj *A = c->adj;
@@ -940,17 +991,14 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   const int T = A->T, E = A->every;
   const double seed = loss_bar / ((double)T * c->npfb);
   const size_t R = (size_t)SV_US * c->ntot;
-  if (int rc = upload_records(c, A->Ubar, ubar_final)) return rc;
-  // seed of the loss term at step T
-  {
-    std::vector<double> u((size_t)SV_US * c->ntot);
-    SV_HIP(c, hipMemcpy(u.data(), A->Ubar, sizeof(double) * u.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < c->npfb; i++) {
-      u[(size_t)SV_US * i + 3] += seed;
-      u[(size_t)SV_US * i + 5] += seed;
-    }
-    SV_HIP(c, hipMemcpy(A->Ubar, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice));
+  if (device_io) {
+    if (int rc = zero_dev(c, A->Ubar, (int64_t)R)) return rc;
+    if (int rc = sv_dev_gather_records(c, A->Ubar, ubar_final)) return rc;
+  } else if (int rc = upload_records(c, A->Ubar, ubar_final)) {
+    return rc;
   }
+  k_seed_loss<<<(c->npfb + 255) / 256, 256, 0, c->stream>>>(c->npfb, seed, A->Ubar);
+  SV_HIP(c, hipGetLastError());
   EventTmp e0, e1;
   SV_HIP(c, e0.create());
   SV_HIP(c, e1.create());
@@ -983,11 +1031,33 @@ int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6]
   SV_HIP(c, hipEventElapsedTime(&ms, e0, e1));
   A->t_bwd = ms;
   if (int rc = param_grad_finish(c, A)) return rc;
-  if (int rc = download_records(c, A->Ubar, grad0)) return rc;
+  if (device_io) {
+    if (int rc = sv_dev_scatter_records(c, A->Ubar, grad0)) return rc;
+    SV_HIP(c, hipStreamSynchronize(c->stream));
+  } else if (int rc = download_records(c, A->Ubar, grad0)) {
+    return rc;
+  }
   // leave the state at the end of the rollout
   SV_HIP(c, hipMemcpy(c->U, A->Uend, sizeof(double) * R, hipMemcpyDeviceToDevice));
   SV_HIP(c, hipMemcpy(c->ctrl, &A->ctrl_end, sizeof(sv_ctrl), hipMemcpyHostToDevice));
   return sv_dev_init_state(c);
+}
+
+int sv_rollout_vjp(sv_ctx *c, double loss_bar, const double *const ubar_final[6],
+                   double *const grad0[6]) {
+  return rollout_vjp(c, loss_bar, ubar_final, grad0, false);
+}
+
+int sv_rollout_vjp_device(sv_ctx *c, double loss_bar, const double *const ubar_final[6],
+                          double *const grad0[6], void *stream) {
+  SV_ADJ_CHECK(c, c && grad0, "null argument");
+  for (int q = 0; q < 6; q++) {
+    if (int rc = sv_check_device_ptr(c, grad0[q], "grad0")) return rc;
+    if (ubar_final && ubar_final[q])
+      if (int rc = sv_check_device_ptr(c, ubar_final[q], "ubar_final")) return rc;
+  }
+  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  return rollout_vjp(c, loss_bar, ubar_final, grad0, true);
 }
 
 int sv_adj_param_grad(sv_ctx *c, double grad[3]) {

@@ -775,6 +775,46 @@ int sv_set_state(sv_ctx *c, const double *ro, const double *rou, const double *r
   return sv_dev_init_state(c);
 }
 
+extern "C++" int sv_check_device_ptr(sv_ctx *c, const void *p, const char *what) {
+  if (!p) return sv_fail(c, SV_E_ARG, "%s: null device pointer", what);
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) (void)hipGetLastError();
+  if (e != hipSuccess || at.type != hipMemoryTypeDevice || at.device != c->device)
+    return sv_fail(c, SV_E_ARG,
+                   "%s: %p is not device memory of GPU %d in this library's HIP runtime (a "
+                   "Python process must import torch before loading the library for the two "
+                   "to share one runtime)", what, p, c->device);
+  return 0;
+}
+
+int sv_set_state_device(sv_ctx *c, const double *const U[6], void *stream) {
+  SV_CHECK(c, c && U, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  SV_CHECK(c, c->have_state, "sv_set_initial must come first (boundary records, inflow velocity)");
+  if (int rc = set_device(c)) return rc;
+  for (int q = 0; q < 6; q++)
+    if (int rc = sv_check_device_ptr(c, U[q], "sv_set_state_device")) return rc;
+  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  if (int rc = sv_dev_gather_records(c, c->U, U)) return rc;
+  if (int rc = sv_dev_init_state(c)) return rc;   // synchronises the context's stream
+  return 0;
+}
+
+int sv_get_state_device(sv_ctx *c, double *const U[6], void *stream) {
+  SV_CHECK(c, c && U, "null argument");
+  SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
+  SV_CHECK(c, c->have_state, "no state");
+  if (int rc = set_device(c)) return rc;
+  for (int q = 0; q < 6; q++)
+    if (U[q])
+      if (int rc = sv_check_device_ptr(c, U[q], "sv_get_state_device")) return rc;
+  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  if (int rc = sv_dev_scatter_records(c, c->U, U)) return rc;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int sv_filter_variables(sv_ctx *c, double *ro, double *rou, double *rov, double *xx, double *xy,
                         double *yy) {
   SV_CHECK(c, c && ro && rou && rov && xx && xy && yy, "null argument");

@@ -345,6 +345,13 @@ int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_col
                  hipStream_t st);
 // ro (npfb, caller order) -= the volume-weighted mean deviation from 1
 int sv_dev_mass_correct(sv_ctx *c, double *ro, hipStream_t st);
+// caller-order device arrays (six of npfb, null entries read as 0 / skipped)
+// <-> loop-order records on the context's stream
+int sv_dev_gather_records(sv_ctx *c, double *dst, const double *const f[6]);
+int sv_dev_scatter_records(sv_ctx *c, const double *src, double *const f[6]);
+// 0 if p is device memory of the context's device in this library's HIP
+// runtime, else an SV_E_ARG failure naming `what`
+int sv_check_device_ptr(sv_ctx *c, const void *p, const char *what);
 // ---- kernels: time stepping (k_step.hip) ----
 int sv_dev_init_state(sv_ctx *c);
 // one step; ev (nullable) gets SV_NKERNELS+1 events recorded between launches

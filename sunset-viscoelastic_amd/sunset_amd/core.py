@@ -60,6 +60,7 @@ EXPORTS = [
     "sv_rollout_forward", "sv_rollout_vjp", "sv_adj_param_grad", "sv_adj_timing",
     "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow",
     "sv_calc_boundary_weights", "sv_adapt_stencils", "sv_set_state", "sv_filter_variables",
+    "sv_set_state_device", "sv_get_state_device", "sv_rollout_vjp_device",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -117,6 +118,10 @@ def lib():
     L.sv_rhs_vjp.argtypes = [C.c_void_p, C.POINTER(PD), C.POINTER(PD)]
     L.sv_rollout_forward.argtypes = [C.c_void_p, C.c_int32, PD]
     L.sv_rollout_vjp.argtypes = [C.c_void_p, C.c_double, C.POINTER(PD), C.POINTER(PD)]
+    L.sv_set_state_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p]
+    L.sv_get_state_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p]
+    L.sv_rollout_vjp_device.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.c_void_p]
     L.sv_adj_param_grad.argtypes = [C.c_void_p, PD]
     L.sv_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, PD]
     L.sv_diagnostics_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
@@ -402,6 +407,27 @@ class Solver:
         self._ck(lib().sv_rollout_vjp(self.h, loss_bar, pin, pout))
         return dict(zip(self.STATE_KEYS, out))
 
+    # ---- device-pointer forms (integers: device addresses, e.g. a torch
+    # tensor's data_ptr(); the library must share the caller's HIP runtime) ----
+    def set_state_device(self, ptrs, stream: int = 0):
+        """The registers (ro, rou, rov, xx, xy, yy), six device arrays of npfb
+        float64 in the caller's node order; after set_initial."""
+        arr = (C.c_void_p * 6)(*[C.c_void_p(int(p)) for p in ptrs])
+        self._ck(lib().sv_set_state_device(self.h, arr, C.c_void_p(stream or None)))
+
+    def get_state_device(self, ptrs, stream: int = 0):
+        arr = (C.c_void_p * 6)(*[C.c_void_p(int(p)) if p else None for p in ptrs])
+        self._ck(lib().sv_get_state_device(self.h, arr, C.c_void_p(stream or None)))
+
+    def rollout_vjp_device(self, grad_ptrs, loss_bar: float = 1.0, ubar_ptrs=None,
+                           stream: int = 0):
+        out = (C.c_void_p * 6)(*[C.c_void_p(int(p)) for p in grad_ptrs])
+        inp = None
+        if ubar_ptrs is not None:
+            inp = (C.c_void_p * 6)(*[C.c_void_p(int(p)) if p else None for p in ubar_ptrs])
+        self._ck(lib().sv_rollout_vjp_device(self.h, loss_bar, inp, out,
+                                              C.c_void_p(stream or None)))
+
     def diagnostics(self, kind: str):
         """On-device statistics.F90 reductions of the current state:
         "mass_energy" (mass/vol, vol, ke/vol), "conf" (means of cxx, cxy,
@@ -422,11 +448,19 @@ class Solver:
         return f.value, b.value
 
     def set_state(self, U, dt=None, itime=0, time=0.0):
-        """Set (ro, rou, rov, cxx, cxy, cyy) directly (DI) with a fixed dt."""
+        """Set (ro, rou, rov, cxx, cxy, cyy) directly (DI) with a fixed dt: the
+        records of set_initial, then the registers exactly as given."""
         ro = _f64(U["ro"])
         ic = dict(ro=ro, u=_f64(U["rou"]) / ro, v=_f64(U["rov"]) / ro, cxx=U["cxx"],
                   cxy=U["cxy"], cyy=U["cyy"])
         self.set_initial(ic, dt=dt, itime=itime, dt_mode="fixed", time=time)
+        arrs = [_f64(U[k]) for k in ("ro", "rou", "rov", "cxx", "cxy", "cyy")]
+        self._ck(lib().sv_set_state(self.h, *[_d(a) for a in arrs]))
+
+    def set_state_control(self, dt, itime=0, time=0.0, emax=(1e-5, 1e-5, 1e-5)):
+        """A fixed dt, time and itime for the current state (restart-style)."""
+        self._ck(lib().sv_set_control(self.h, dt, time, itime, emax[0], emax[1], emax[2],
+                                      DT_FIXED))
 
     def fields(self):
         n = self.npfb

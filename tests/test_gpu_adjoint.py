@@ -6,7 +6,8 @@ own forward rollout:
   - final-state VJP:       <ubar, dU_T>_oracle == <grad0, dU>_device
   - loss gradient:         dL_oracle == <grad0, dU>_device, and central FD
   - the state after the VJP is the state after the forward rollout, bitwise
-  - torch.autograd.Function around the rollout (CPU tensors at the boundary)
+  - torch.autograd.Function around the rollout: CPU tensors at the boundary,
+    and CUDA tensors with no host copy (a process importing torch first)
 Tolerances: 1e-9 relative against the tangent-linear model (rounding of two
 different evaluation orders), 1e-6 against finite differences (step 1e-6)."""
 import numpy as np
@@ -212,16 +213,33 @@ def test_torch_autograd_parameters():
     s = _device(ns, PH)
     Wi = torch.tensor(PH["Wi"], dtype=torch.float64, requires_grad=True)
     beta = torch.tensor(PH["beta"], dtype=torch.float64, requires_grad=True)
-    L = rollout_loss(s, U0, 12, DT, checkpoint_every=5, Wi=Wi, beta=beta)
+    eps_ = torch.tensor(PH["eps"], dtype=torch.float64, requires_grad=True)
+    L = rollout_loss(s, U0, 12, DT, checkpoint_every=5, Wi=Wi, beta=beta, eps=eps_)
     L.backward()
     h = 1e-4
     with torch.no_grad():
-        for name, t in (("Wi", Wi), ("beta", beta)):
+        for name, t in (("Wi", Wi), ("beta", beta), ("eps", eps_)):
             def at(sign):
                 kw = {k: torch.tensor(PH[k] + (sign * h if k == name else 0.0),
-                                      dtype=torch.float64) for k in ("Wi", "beta")}
+                                      dtype=torch.float64) for k in ("Wi", "beta", "eps")}
                 return float(rollout_loss(s, U0, 12, DT, 5, **kw))
             Lp, Lm = at(+1), at(-1)
             fd = (Lp - Lm) / (2 * h)
             assert abs(fd - float(t.grad)) <= 1e-6 * abs(fd), (name, fd, float(t.grad))
     s.close()
+
+
+def test_torch_autograd_device_tensors():
+    """CUDA tensors at the boundary (sv_set_state_device, sv_rollout_vjp_device)
+    in a fresh process that imports torch before the library, so the two
+    share one HIP runtime; bit-for-bit equal to the CPU-tensor path
+    (tests/gpu_torch_device_check.py)."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    print("[gpu-adjoint] device tensors (subprocess)", flush=True)
+    r = subprocess.run([sys.executable, "-u", os.path.join(here, "gpu_torch_device_check.py")],
+                       capture_output=True, text=True, timeout=240)
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK")
