@@ -66,8 +66,9 @@ typedef struct {       /* control.in quantities (load_data.F90:21-159) */
 } sv_phys;
 
 /* ---- context --------------------------------------------------------- */
-/* Refuses forms other than DI/LC and orders other than 4, 6, 8 (the orders
- * the device weight kernels are built for). */
+/* Refuses forms other than DI/LC and orders other than 4, 6, 8, 10 (the
+ * orders the device weight kernels are built for; order 10 without boundary
+ * nodes and without stencil adaptation). */
 int  sv_create(const sv_config *cfg, sv_ctx **out);
 void sv_destroy(sv_ctx *ctx);
 /* The context's last message; with ctx == NULL, the calling thread's last

@@ -581,6 +581,9 @@ int sv_dev_adapt(sv_ctx *c, int *reductions_host) {
     return sv_fail(c, SV_E_ARG, "adapt_stencils needs morder <= forder (%d > %d): the "
                                 "derivative system takes the filter order's first functions",
                    md, mf);
+  if (mf > 8)
+    return sv_fail(c, SV_E_ARG, "adapt_stencils: LABFM order %d is not supported on the device "
+                                "(its SVD takes a lane per column, at most 64)", mf);
   DevTmp<int> red, fail;
   SV_HIP(c, red.alloc(2 * (size_t)n));
   SV_HIP(c, fail.alloc(1));

@@ -59,6 +59,14 @@ __device__ __forceinline__ void hermite_all(double z, double *H) {
       const double z4 = z2 * z2;
       H[8] = z4 * z4 - 28.0 * z4 * z2 + 210.0 * z4 - 420.0 * z2 + 105.0;
     }
+    if (M >= 9) {
+      const double z4 = z3 * z;
+      H[9] = z4 * z3 * z2 - 36.0 * z4 * z3 + 378.0 * z3 * z2 - 1260.0 * z3 + 945.0 * z;
+    }
+    if (M >= 10) {
+      const double z4 = z2 * z2;
+      H[10] = z4 * z4 * z2 - 45.0 * z4 * z4 + 630.0 * z4 * z2 - 3150.0 * z4 + 4725.0 * z2 - 945.0;
+    }
   }
 }
 
@@ -145,6 +153,9 @@ __device__ __forceinline__ double hyp_rhs(int m, int r) {
   if (m <= 7) return (r == 20 || r == 26) ? 1.0 : (r == 22 || r == 24) ? 3.0 : 0.0;
   if (m <= 9)
     return (r == 35 || r == 43) ? -1.0 : (r == 37 || r == 41) ? -4.0 : r == 39 ? -6.0 : 0.0;
+  if (m <= 11)
+    return (r == 54 || r == 64) ? 1.0 : (r == 56 || r == 62) ? 5.0
+           : (r == 58 || r == 60) ? 10.0 : 0.0;
   return 0.0;
 }
 
@@ -633,6 +644,247 @@ __global__ __launch_bounds__(SV_WAVE, 1) void k_weights_mixed(SysArgs a) {
   }
 }
 
+// ---- systems of more than 64 unknowns (order 10: n = 65) ----
+// One system per block of BW = 320 threads (five wavefronts); thread t < 4n
+// owns row r = t / 4 of the moment matrix and its columns c = q + 4m (q =
+// t % 4).  The entries are assembled in registers from 16-neighbour chunks of
+// monomial / ABF rows staged in LDS (a plain FP64 contraction: 65 rows do not
+// fit the one-row-per-lane layout of k_weights), copied to LDS for an LU with
+// partial pivoting and row swaps, and kept in registers for the refinement
+// residual, which each row's four threads sum in double-double and combine
+// with two lane exchanges.  Substitutions run one step per barrier with a row
+// per thread group; one refinement step; weights one neighbour per thread.
+template <int M>
+struct Big {
+  static constexpr int N = nsize(M), LDA = N + 1, MC = (N + 3) / 4, CH = 16, BW = 320;
+  // A [N][LDA], chunks X, G [CH][N], solution / right-hand sides [4][N],
+  // residuals [4][N], the permutation [N] and a few shared scalars
+  static constexpr int TOTAL = N * LDA + 2 * CH * N + 8 * N + N + 8;
+};
+
+template <int M>
+__global__ __launch_bounds__(320, 1) void k_weights_big(SysArgs S_) {
+  using B = Big<M>;
+  constexpr int N = B::N, LDA = B::LDA, MC = B::MC, CH = B::CH, BW = B::BW;
+  extern __shared__ double lds[];
+  double *A = lds;                       // [N][LDA]
+  double *Xc = A + N * LDA;              // [CH][N]
+  double *Gc = Xc + CH * N;              // [CH][N]
+  double *Xv = Gc + CH * N;              // [4][N]
+  double *Rv = Xv + 4 * N;               // [4][N]
+  int *perm = reinterpret_cast<int *>(Rv + 4 * N);
+  double *shd = Rv + 4 * N + N;          // shared scalars
+  const int t = threadIdx.x, r = t >> 2, q = t & 3;
+  const bool own = t < 4 * N;
+  const int sys = blockIdx.x;
+  if (sys >= S_.nsys) return;
+  const SysOf o = sys_of<M>(S_, sys);
+  const double xi = S_.x[o.i], yi = S_.y[o.i], rhh = 1.0 / o.hh;
+
+  // ---- assembly: A(r, c) = sum_k X_k[r] g_k[c] ----
+  double acc[MC];
+#pragma unroll
+  for (int m = 0; m < MC; m++) acc[m] = 0.0;
+  const int role = t / CH;   // 0: monomial row, 1: ABF row, others idle
+  for (int c0 = 0; c0 < o.Kuse; c0 += CH) {
+    const int kk = c0 + (t % CH);
+    if (role < 2) {
+      double *row = (role == 0 ? Xc : Gc) + (t % CH) * N;
+      if (kk < o.Kuse) {
+        const int j = S_.col[o.base + kk];
+        const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
+        const double xs = x * rhh, ys = y * rhh;
+        if (role == 0) {
+          mono_row<M>(xs, ys, row);
+        } else {
+          const double rad = sqrt(x * x + y * y) * rhh;
+          abf_row<M>(xs, ys, wab(rad, S_.h2), row);
+        }
+      } else {
+        for (int a = 0; a < N; a++) row[a] = 0.0;
+      }
+    }
+    __syncthreads();
+    if (own) {
+      const int nk = min(CH, o.Kuse - c0);
+      for (int k = 0; k < nk; k++) {
+        const double xr = Xc[k * N + r];
+#pragma unroll
+        for (int m = 0; m < MC; m++) {
+          const int c = q + 4 * m;
+          if (c < N) acc[m] = fma(xr, Gc[k * N + c], acc[m]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // order reduction near walls: rows and columns beyond nkeep become the
+  // identity (labf.F90:126-148, :311-350)
+  if (own) {
+#pragma unroll
+    for (int m = 0; m < MC; m++) {
+      const int c = q + 4 * m;
+      if (c < N) {
+        if (r >= o.nkeep || c >= o.nkeep) acc[m] = (r == c) ? 1.0 : 0.0;
+        A[r * LDA + c] = acc[m];
+      }
+    }
+  }
+  if (t < N) perm[t] = t;
+  __syncthreads();
+
+  // ---- LU with partial pivoting, rows swapped in LDS ----
+  int sing = 0;
+  for (int k = 0; k < N; k++) {
+    if (t < 64) {
+      // the largest |A(i, k)|, i >= k (lowest i on ties; NaN counts as 0)
+      double best = -1.0;
+      int bi = N;
+      for (int i = k + t; i < N; i += 64) {
+        double v = fabs(A[i * LDA + k]);
+        if (v != v) v = 0.0;
+        if (v > best) { best = v; bi = i; }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (t == 0) reinterpret_cast<int *>(shd)[0] = bi;
+    }
+    __syncthreads();
+    const int p = reinterpret_cast<int *>(shd)[0];
+    if (p != k) {
+      for (int c = t; c < N; c += BW) {
+        const double v = A[k * LDA + c];
+        A[k * LDA + c] = A[p * LDA + c];
+        A[p * LDA + c] = v;
+      }
+      if (t == 0) {
+        const int v = perm[k];
+        perm[k] = perm[p];
+        perm[p] = v;
+      }
+    }
+    __syncthreads();
+    const double piv = A[k * LDA + k];
+    if (!(piv != 0.0)) sing = 1;
+    const double rinv = 1.0 / (piv == 0.0 ? 1.0 : piv);
+    for (int i = k + 1 + t; i < N; i += BW) A[i * LDA + k] *= rinv;
+    __syncthreads();
+    const int w = N - k - 1;
+    for (int e = t; e < w * w; e += BW) {
+      const int i = k + 1 + e / w, j = k + 1 + e % w;
+      A[i * LDA + j] = fma(-A[i * LDA + k], A[k * LDA + j], A[i * LDA + j]);
+    }
+    __syncthreads();
+  }
+
+  // ---- substitutions: L U x = P v for the four right-hand sides in V ----
+  // (thread (r, q): row r, right-hand side q)
+  auto solve = [&](double *V) {
+    for (int k = 0; k < N; k++) {
+      if (own && r > k) V[q * N + r] = fma(-A[r * LDA + k], V[q * N + k], V[q * N + r]);
+      __syncthreads();
+    }
+    for (int k = N - 1; k >= 0; k--) {
+      if (own && r == k) V[q * N + k] /= A[k * LDA + k] == 0.0 ? 1.0 : A[k * LDA + k];
+      __syncthreads();
+      if (own && r < k) V[q * N + r] = fma(-A[r * LDA + k], V[q * N + k], V[q * N + r]);
+      __syncthreads();
+    }
+  };
+  double b4[4];
+  rhs_row(r, o.sysD, o.sysF, o.mf, b4);   // row r of the right-hand sides
+  if (own) {
+    double bp[4];
+    rhs_row(perm[r], o.sysD, o.sysF, o.mf, bp);
+    Xv[q * N + r] = bp[q];
+  }
+  __syncthreads();
+  solve(Xv);
+  // residual b - A0 x of row r in double-double: the row's four threads
+  // each sum their columns, then two lane exchanges combine them
+  {
+    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
+    if (own) {
+#pragma unroll
+      for (int m = 0; m < MC; m++) {
+        const int c = q + 4 * m;
+        if (c < N) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const double pr = acc[m] * Xv[v * N + c];
+            const double pe = fma(acc[m], Xv[v * N + c], -pr);
+            double e;
+            two_sum(sh[v], pr, sh[v], e);
+            sl[v] += e + pe;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 1; off <= 2; off <<= 1)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double oh = __shfl_xor(sh[v], off), ol = __shfl_xor(sl[v], off);
+        double e;
+        two_sum(sh[v], oh, sh[v], e);
+        sl[v] += e + ol;
+      }
+    if (own && q == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        double d, de;
+        two_sum(b4[v], -sh[v], d, de);
+        Rv[v * N + r] = d + (de - sl[v]);
+      }
+    }
+  }
+  __syncthreads();
+  // the correction: L U dx = P res, then x += dx
+  double rp = 0.0;
+  if (own) rp = Rv[q * N + perm[r]];
+  __syncthreads();
+  if (own) Rv[q * N + r] = rp;
+  __syncthreads();
+  solve(Rv);
+  if (own) Xv[q * N + r] += Rv[q * N + r];
+  if (sing && t == 0) atomicAdd(S_.singular, 1);
+  __syncthreads();
+
+  // ---- weights, one neighbour per thread; node volume ----
+  double vsum = 0.0;
+  for (int k = t; k < o.Kfull; k += BW) {
+    const int j = S_.col[o.base + k];
+    const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
+    const double rad = sqrt(x * x + y * y) * rhh;
+    const double w = wab(rad, S_.h2);
+    if (o.sysD) {
+      S_.wx[o.base + k] = abf_dot<M>(x * rhh, y * rhh, w, Xv) * rhh;
+      S_.wy[o.base + k] = abf_dot<M>(x * rhh, y * rhh, w, Xv + N) * rhh;
+      S_.wl[o.base + k] = abf_dot<M>(x * rhh, y * rhh, w, Xv + 2 * N) * rhh * rhh;
+    }
+    if (o.sysF) {
+      S_.wh[o.base + k] = abf_dot<M>(x * rhh, y * rhh, w, Xv + 3 * N);
+      vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (o.hh * o.hh);
+    }
+  }
+  if (o.sysF) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) vsum += __shfl_xor(vsum, off);
+    __syncthreads();
+    if ((t & 63) == 0) shd[1 + t / 64] = vsum;
+    __syncthreads();
+    if (t == 0) {
+      double s = 0.0;
+      for (int wv = 0; wv < BW / 64; wv++) s += shd[1 + wv];
+      S_.vol[o.i] = 1.0 / s;
+    }
+  }
+}
+
 // calc_labf_sums (labf.F90:396-451), one thread per node, sequential in k
 __global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double L, double *wx,
                        double *wy, double *wl, const double *wh, double *gsx, double *gsy,
@@ -795,6 +1047,20 @@ int launch_mixed_tiles(sv_ctx *c, const SysArgs &b, int mmax) {
   return 0;
 }
 
+// systems of more than 64 unknowns: one block each
+template <int M>
+int launch_big(sv_ctx *c, SysArgs a, int first, int count) {
+  if (count <= 0) return 0;
+  SysArgs b = a;
+  b.sys_node = a.sys_node + first;
+  b.sys_kind = a.sys_kind + first;
+  b.nsys = count;
+  const size_t shm = sizeof(double) * Big<M>::TOTAL;
+  k_weights_big<M><<<count, Big<M>::BW, shm, c->stream>>>(b);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 // mode 1: one mixed-order persistent launch with 16-neighbour tiles, 2: with
 // 32; a.unit_first / a.nunits: the units of the count systems
 int launch_mixed(sv_ctx *c, SysArgs a, int count, int mmax, int mode) {
@@ -825,7 +1091,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     const int mf = order_map_host ? order_map_host[i] : c->cfg.forder;
     for (int m : {md, mf})
       if (!sv_order_supported(m))
-        return sv_fail(c, SV_E_ARG, "LABFM order %d not supported on the device (4, 6, 8)", m);
+        return sv_fail(c, SV_E_ARG, "LABFM order %d not supported on the device (4, 6, 8, 10)", m);
     // boundary nodes are not on internal_list (labf.F90:82, :268): their
     // operators come from calc_boundary_weights, which also replaces the
     // derivative weights of FD rows 1 and 2 (only their filter system is kept)
@@ -883,6 +1149,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   float tune_ms = 0.0f;    // autotuning sample launches (2 passes each)
   float solve_ms = 0.0f;   // the full launches
   auto launch = [&](int m, SysArgs args, int f0, int cnt, int variant) -> int {
+    if (m == 10) return launch_big<10>(c, args, f0, cnt);
     if (m == 4) return launch_order<4>(c, args, f0, cnt, variant);
     if (m == 6) return launch_order<6>(c, args, f0, cnt, variant);
     return launch_order<8>(c, args, f0, cnt, variant);
@@ -901,12 +1168,14 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   // order slices of the (order, length)-sorted list
   struct Slice { int m, first, count; };
   std::vector<Slice> slices;
-  for (int m : {4, 6, 8}) {
+  bool has_big = false;
+  for (int m : {4, 6, 8, 10}) {
     int f = 0;
     while (f < (int)sys.size() && sys[f].m != m) f++;
     int cnt = 0;
     while (f + cnt < (int)sys.size() && sys[f + cnt].m == m) cnt++;
     if (cnt) slices.push_back({m, f, cnt});
+    if (cnt && m > 8) has_big = true;
   }
   const int S = SV_TUNE_SAMPLE;
   // 1. per order: the launch variant (grid kind x assembly tile), timed on a
@@ -914,6 +1183,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   //    the sample has the same length mix); the sample's weights are
   //    rewritten by the full launch with identical values
   for (const Slice &sl : slices) {
+    if (sl.m > 8) continue;   // one launch shape
     const int mi = sl.m / 2 - 2;
     if (!c->weights_autotune || sl.count < 4 * S || c->wtune[mi] >= 0) continue;
     const int stride = sl.count / S;
@@ -940,6 +1210,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
     }
   }
   auto order_variant = [&](int m) {
+    if (m > 8) return 0;
     const int mi = m / 2 - 2;
     return (c->weights_autotune && c->wtune[mi] >= 0) ? c->wtune[mi] : c->weights_fixed;
   };
@@ -947,7 +1218,9 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
   //    first) against the per-order launches, on a strided sample of that queue
   int mode = c->weights_autotune ? 0 : c->weights_mixed;
   std::vector<int> qn, qk, qm;   // the mixed queue
-  if (slices.size() >= 2) {
+  if (has_big) {
+    mode = 0;   // the mixed launch covers orders 4-8
+  } else if (slices.size() >= 2) {
     for (auto it = slices.rbegin(); it != slices.rend(); ++it)
       for (int q = it->first; q < it->first + it->count; q++) {
         qn.push_back(hn[q]);
@@ -1020,7 +1293,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
       SV_HIP(c, hipEventSynchronize(t1));
       SV_HIP(c, hipEventElapsedTime(&tl, t0, t1));
       solve_ms += tl;
-      c->wvariant[sl.m / 2 - 2] = variant;
+      if (sl.m <= 8) c->wvariant[sl.m / 2 - 2] = variant;
     }
   } else {
     const int total = (int)qn.size();

@@ -78,11 +78,11 @@ int sv_create(const sv_config *cfg, sv_ctx **out) {
     return sv_fail(nullptr, SV_E_ARG, "sv_create: form %d is neither DI (0) nor LC (1)",
                    cfg->form);
   // the orders the weight kernels are built for (common_parameter.F90:45-66
-  // also allows 10 and 12)
+  // also allows 12)
   for (int m : {cfg->morder, cfg->forder})
     if (!sv_order_supported(m))
       return sv_fail(nullptr, SV_E_ARG,
-                     "sv_create: LABFM order %d not supported on the device (4, 6, 8)", m);
+                     "sv_create: LABFM order %d not supported on the device (4, 6, 8, 10)", m);
   sv_ctx *c = new sv_ctx();
   c->cfg = *cfg;
   c->device = cfg->device;
@@ -266,6 +266,9 @@ int sv_calc_boundary_weights(sv_ctx *c) {
   SV_CHECK(c, !c->have_sums, "boundary weights come before sv_calc_labf_sums "
            "(sunset.F90:54-55)");
   SV_CHECK(c, c->nb == 0 || c->have_normals, "sv_set_boundaries must come first");
+  SV_CHECK(c, c->nb == 0 || (c->cfg.forder <= 8 && c->cfg.morder <= 8),
+           "boundary nodes with LABFM order 10 are not supported on the device (its "
+           "11-point FD stencils, common_parameter.F90:111-112)");
   if (int rc = set_device(c)) return rc;
   return sv_dev_boundary_weights(c);
 }

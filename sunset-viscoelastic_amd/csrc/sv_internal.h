@@ -284,7 +284,7 @@ int sv_fail(sv_ctx *c, int code, const char *fmt, ...);
 // (k_diag.hip), looked up once per device with that device current.
 void sv_diag_scratch_init(int device);
 // LABFM orders the weight kernels are instantiated for
-static inline bool sv_order_supported(int m) { return m == 4 || m == 6 || m == 8; }
+static inline bool sv_order_supported(int m) { return m == 4 || m == 6 || m == 8 || m == 10; }
 #define SV_HIP(c, call)                                                        \
   do {                                                                         \
     hipError_t e_ = (call);                                                    \

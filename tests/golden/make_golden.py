@@ -23,6 +23,7 @@ Fixtures (numpy .npz, no pickles):
                        the adapted h (CSR in canonical order)
 
     python tests/golden/make_golden.py --adapt    (only the adapt fixtures)
+    python tests/golden/make_golden.py --order10  (only the order-10 fixture)
 """
 import os
 import sys
@@ -47,6 +48,10 @@ CASES = [
     ("m4_h18", 4, 20, 1.8, 303, 100, dict(Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1)),
     ("m6_20", 6, 20, 2.4, 202, 20, dict(Re=1.0, Wi=2.0, beta=0.1, eps=0.25, Ma=0.1)),
 ]
+# order 10 (common_parameter.F90:51-53, hovs 3.1): lists and fields only; its
+# moment systems are ill-conditioned enough that the reference's SVD weights
+# carry ~1e-3 relative solver error (pinned instead against 40-digit solves)
+CASES10 = [("m10_20", 10, 20, 3.1, 404, 10, dict(Re=1.0, Wi=2.0, beta=0.1, eps=0.25, Ma=0.1))]
 
 
 # name, node set, order, nsteps, physics, driving force, inflow settings
@@ -191,6 +196,25 @@ def write_adapt():
               int((out["adapt_h_small"] < out["adapt_h"]).sum()))
 
 
+def write_order10():
+    for name, order, n, hos, seed, nsteps, phys in CASES10:
+        ns = jittered_lattice(n, seed=seed, h_over_s=hos)
+        out, _ = refdrv.run_reference(ns, order, taylor_green_ic(ns), nsteps=nsteps, phys=phys)
+        ptr, col, _, _, _, _ = to_csr(out, ns.npfb)
+        np.savez_compressed(
+            os.path.join(HERE, f"ref_{name}.npz"),
+            order=order, n=n, h_over_s=hos, seed=seed, nsteps=nsteps,
+            phys=np.array([phys["Re"], phys["Wi"], phys["beta"], phys["eps"], phys["Ma"]]),
+            x=ns.x, y=ns.y, ij_count=out["ij_count"], ij_count_small=out["ij_count_small"],
+            ptr=ptr, col=col, vol=out["vol"],
+            **{f"f_{k}": out[k] for k in ("ro", "rou", "rov", "u", "v", "pxx", "pxy", "pyy",
+                                         "cxx", "cxy", "cyy")},
+            ctl=np.array([out[k] for k in ("dt", "time", "emax_np1", "emax_n", "emax_nm1",
+                                          "dt_cfl", "dt_par")]),
+            itime=out["itime"])
+        print("wrote", name, ns.npfb, "nodes", int(ptr[-1]), "links")
+
+
 # convergence of the reference's derivative operators on an analytic field:
 # rho = 1 + 0.1 sin(2 pi x) cos(2 pi y) on periodic jittered lattices
 CONV = [(4, 1.8, (16, 32, 64)), (6, 2.4, (16, 32, 64))]
@@ -231,7 +255,10 @@ if __name__ == "__main__":
         write_convergence()
     elif "--adapt" in sys.argv:
         write_adapt()
+    elif "--order10" in sys.argv:
+        write_order10()
     else:
         main()
         write_convergence()
         write_adapt()
+        write_order10()

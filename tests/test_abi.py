@@ -61,7 +61,7 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
 
 
 @pytest.mark.parametrize("morder,forder,form,what", [
-    (10, 10, 1, "order 10"), (6, 12, 1, "order 12"), (5, 5, 0, "order 5"),
+    (12, 12, 1, "order 12"), (6, 12, 1, "order 12"), (5, 5, 0, "order 5"),
     (6, 6, 7, "form 7")])
 def test_create_refuses_unsupported_orders_and_forms(morder, forder, form, what):
     """sv_create checks the form and the LABFM orders before it touches a
