@@ -57,7 +57,7 @@ __device__ __forceinline__ void abf1(double z, double ff1, double c[6]) {
 // Gaussian elimination with partial pivoting, then one step of iterative
 // refinement with the residual of the original matrix.
 template <int NR>
-__device__ void solve6(const double A0[36], double x[NR][6]) {
+__device__ __forceinline__ void solve6(const double A0[36], double x[NR][6]) {
   double A[36];
   int piv[6];
 #pragma unroll
@@ -69,8 +69,13 @@ __device__ void solve6(const double A0[36], double x[NR][6]) {
     for (int r = k + 1; r < 6; r++)
       if (fabs(A[k * 6 + r]) > v) { v = fabs(A[k * 6 + r]); p = r; }
     piv[k] = p;
-    if (p != k)
-      for (int c = 0; c < 6; c++) { const double t = A[c * 6 + k]; A[c * 6 + k] = A[c * 6 + p]; A[c * 6 + p] = t; }
+    // the row swap through compile-time indexes only (a run-time index into
+    // A would place it in scratch memory)
+#pragma unroll
+    for (int r = k + 1; r < 6; r++)
+      if (r == p)
+#pragma unroll
+        for (int c = 0; c < 6; c++) { const double t = A[c * 6 + k]; A[c * 6 + k] = A[c * 6 + r]; A[c * 6 + r] = t; }
     const double d = A[k * 6 + k] == 0.0 ? 1.0 : A[k * 6 + k];
     for (int r = k + 1; r < 6; r++) {
       const double m = A[k * 6 + r] / d;
@@ -79,8 +84,16 @@ __device__ void solve6(const double A0[36], double x[NR][6]) {
     }
   }
   auto lu_solve = [&](double y[6]) {
+#pragma unroll
     for (int k = 0; k < 6; k++)
-      if (piv[k] != k) { const double t = y[k]; y[k] = y[piv[k]]; y[piv[k]] = t; }
+#pragma unroll
+      for (int r = k + 1; r < 6; r++)
+      {
+        const bool sw = piv[k] == r;
+        const double yk = y[k], yr = y[r];
+        y[k] = sw ? yr : yk;
+        y[r] = sw ? yk : yr;
+      }
     for (int r = 1; r < 6; r++)
       for (int c = 0; c < r; c++) y[r] -= A[c * 6 + r] * y[c];
     for (int r = 5; r >= 0; r--) {
@@ -89,6 +102,7 @@ __device__ void solve6(const double A0[36], double x[NR][6]) {
       y[r] = y[r] / d;
     }
   };
+#pragma unroll
   for (int q = 0; q < NR; q++) {
     double b[6], y[6];
     for (int r = 0; r < 6; r++) { b[r] = x[q][r]; y[r] = b[r]; }
@@ -133,7 +147,7 @@ __device__ __forceinline__ double tcoord(const BndArgs &a, int i, int j, double 
 }
 
 // 1D moment matrix of node i over its same-layer neighbours (:695-717)
-__device__ void assemble1d(const BndArgs &a, int i, int64_t base, int K, double xt, double yt,
+__device__ __forceinline__ void assemble1d(const BndArgs &a, int i, int64_t base, int K, double xt, double yt,
                            double hh, double A[36]) {
   for (int e = 0; e < 36; e++) A[e] = 0.0;
   const int ti = a.nt[i];

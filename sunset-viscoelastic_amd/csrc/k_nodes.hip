@@ -457,43 +457,78 @@ int sv_dev_build_sell(sv_ctx *c) {
 // ---- time-loop layout (Morton order) --------------------------------------
 namespace {
 
-__global__ void k_loop_pack(int npfb, const int64_t *off, const int *perm, const int *iperm,
-                            const int64_t *row_ptr, const int *cnt, const int *col_raw,
-                            const int *irel, const int *vrel, const double *wx,
-                            const double *wy, const double *wl, const double *wh,
-                            int64_t nnz_max, int *lcol, double *lwx, double *lwy, double *lwl,
-                            double *lwh, uint8_t *lflip) {
-  // one block of 64 threads per (slice, link-chunk of 4); slot = s*64 + l
+// Loop-order SELL packing: block (s, y) fills links [16y, 16y+16) of the 64
+// nodes of slice s.  The CSR rows are read 16 links at a time (one cache
+// line of each array per row), transposed through LDS, and written as the
+// slice's link-major columns (64 consecutive slots per link).  Links past a
+// node's count, and slots past npfb, get the padding entry: the node's own
+// slot (0 past npfb) with zero weights.
+constexpr int PK_K = 16;           // links per block
+constexpr int PK_T = 256;          // threads per block
+constexpr int PK_LD = PK_K + 1;    // LDS row stride of the tiles
+__global__ __launch_bounds__(PK_T) void k_loop_pack(
+    int npfb, const int64_t *off, const int *perm, const int *iperm, const int64_t *row_ptr,
+    const int *cnt, const int *col_raw, const int *irel, const int *vrel, const double *wx,
+    const double *wy, const double *wl, const double *wh, int64_t nnz_max, int *lcol,
+    double *lwx, double *lwy, double *lwl, double *lwh, uint8_t *lflip) {
+  __shared__ double tx[SV_WAVE * PK_LD], ty[SV_WAVE * PK_LD], tl[SV_WAVE * PK_LD],
+      th[SV_WAVE * PK_LD];
+  __shared__ int tc[SV_WAVE * PK_LD];
+  __shared__ uint8_t tf[SV_WAVE * PK_LD];
+  __shared__ int rnode[SV_WAVE], rcnt[SV_WAVE];
+  __shared__ int64_t rbase[SV_WAVE];
   const int s = blockIdx.x;
-  const int l = threadIdx.x;
-  const int slot = s * SV_WAVE + l;
+  const int t = threadIdx.x;
   const int64_t o = off[s];
   const int W = (int)((off[s + 1] - o) / SV_WAVE);
-  const int k0 = blockIdx.y * 4;
-  const int i = slot < npfb ? perm[slot] : -1;
-  for (int k = k0; k < W && k < k0 + 4; k++) {
+  const int k0 = blockIdx.y * PK_K;
+  if (k0 >= W) return;
+  if (t < SV_WAVE) {
+    const int slot = s * SV_WAVE + t;
+    const int i = slot < npfb ? perm[slot] : -1;
+    rnode[t] = i;
+    rcnt[t] = i >= 0 ? cnt[i] : 0;
+    rbase[t] = i >= 0 ? row_ptr[i] : 0;
+  }
+  __syncthreads();
+  // gather: 16 consecutive threads read 16 consecutive links of one row
+  for (int p = t; p < SV_WAVE * PK_K; p += PK_T) {
+    const int r = p / PK_K, kk = p % PK_K, k = k0 + kk;
+    const int slot = s * SV_WAVE + r;
+    const int e = r * PK_LD + kk;
+    if (rnode[r] >= 0 && k < rcnt[r]) {
+      const int64_t sidx = rbase[r] + k;
+      const int raw = col_raw[sidx];
+      tc[e] = iperm[irel[raw]];
+      tx[e] = wx[sidx];
+      ty[e] = wy[sidx];
+      tl[e] = wl[sidx];
+      th[e] = wh[sidx];
+      const int vr = (lflip && raw >= npfb) ? vrel[raw] : 0;
+      tf[e] = (uint8_t)(vr >= 2 ? vr - 1 : 0);
+    } else {
+      tc[e] = slot < npfb ? slot : 0;
+      tx[e] = 0.0;
+      ty[e] = 0.0;
+      tl[e] = 0.0;
+      th[e] = 0.0;
+      tf[e] = 0;
+    }
+  }
+  __syncthreads();
+  // scatter: 64 consecutive threads write one link's column of the slice
+  for (int p = t; p < SV_WAVE * PK_K; p += PK_T) {
+    const int kk = p / SV_WAVE, l = p % SV_WAVE, k = k0 + kk;
+    if (k >= W) break;
     const int64_t d = o + (int64_t)k * SV_WAVE + l;
     if (d >= nnz_max) break;
-    if (i >= 0 && k < cnt[i]) {
-      const int64_t sidx = row_ptr[i] + k;
-      const int raw = col_raw[sidx];
-      lcol[d] = iperm[irel[raw]];
-      lwx[d] = wx[sidx];
-      lwy[d] = wy[sidx];
-      lwl[d] = wl[sidx];
-      lwh[d] = wh[sidx];
-      if (lflip) {
-        const int vr = raw >= npfb ? vrel[raw] : 0;
-        lflip[d] = (uint8_t)(vr >= 2 ? vr - 1 : 0);
-      }
-    } else {
-      lcol[d] = slot < npfb ? slot : 0;
-      lwx[d] = 0.0;
-      lwy[d] = 0.0;
-      lwl[d] = 0.0;
-      lwh[d] = 0.0;
-      if (lflip) lflip[d] = 0;
-    }
+    const int e = l * PK_LD + kk;
+    lcol[d] = tc[e];
+    lwx[d] = tx[e];
+    lwy[d] = ty[e];
+    lwl[d] = tl[e];
+    lwh[d] = th[e];
+    if (lflip) lflip[d] = tf[e];
   }
 }
 
@@ -641,9 +676,9 @@ int sv_dev_build_loop(sv_ctx *c) {
   SV_HIP(c, hipMemcpy(L.off, off.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice));
   SV_HIP(c, hipMemcpy(L.perm, L.hperm.data(), I * n, hipMemcpyHostToDevice));
   SV_HIP(c, hipMemcpy(L.iperm, iperm.data(), I * c->np, hipMemcpyHostToDevice));
-  dim3 grid(ns, (wmax + 3) / 4);
+  dim3 grid(ns, (wmax + PK_K - 1) / PK_K);
   if (c->has_mirror && sv_alloc(c, (void **)&L.lflip, (size_t)L.nnz + 1)) return SV_E_NOMEM;
-  k_loop_pack<<<grid, SV_WAVE, 0, c->stream>>>(n, L.off, L.perm, L.iperm, c->row_ptr, c->cnt,
+  k_loop_pack<<<grid, PK_T, 0, c->stream>>>(n, L.off, L.perm, L.iperm, c->row_ptr, c->cnt,
                                                c->col_raw, c->irel, c->vrel, c->wx, c->wy,
                                                c->wl, c->wh, L.nnz, L.col, L.wx, L.wy, L.wl,
                                                L.wh, L.lflip);

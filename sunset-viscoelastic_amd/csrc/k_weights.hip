@@ -885,89 +885,145 @@ __global__ __launch_bounds__(320, 1) void k_weights_big(SysArgs S_) {
   }
 }
 
-// calc_labf_sums (labf.F90:396-451), one thread per node, sequential in k
-__global__ void k_sums(int npfb, const int64_t *row_ptr, const int *cnt, double L, double *wx,
-                       double *wy, double *wl, const double *wh, double *gsx, double *gsy,
-                       double *lsum, double *hsum) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npfb) return;
-  const int64_t b = row_ptr[i];
-  const int K = cnt[i];
+// The setup sums below run one thread per node over its CSR row, adding in
+// the reference's sequential order.  A block's rows are one contiguous range
+// of the CSR arrays (row_ptr is a prefix sum), which is staged through LDS in
+// coalesced chunks; each thread then adds the part of its row that lies in the
+// chunk.  Entries past cnt[i] (rows pruned after the lists were built) lie in
+// the range and are skipped by the per-row bounds.
+constexpr int SUM_B = 256;     // nodes (threads) per block
+constexpr int SUM_CH = 1024;   // CSR entries per staged chunk
+
+// calc_labf_sums (labf.F90:396-451): row sums of the four weight arrays, then
+// the gradient and Laplacian weights scaled by L_char (written back from the
+// staged chunk, so every CSR entry of the range is read and written once)
+__global__ __launch_bounds__(SUM_B) void k_sums(int npfb, const int64_t *row_ptr, const int *cnt,
+                                                double L, double *wx, double *wy, double *wl,
+                                                const double *wh, double *gsx, double *gsy,
+                                                double *lsum, double *hsum) {
+  __shared__ double cx[SUM_CH], cy[SUM_CH], cl[SUM_CH], ch[SUM_CH];
+  const int i0 = blockIdx.x * SUM_B;
+  const int i1 = min(i0 + SUM_B, npfb);
+  const int i = i0 + (int)threadIdx.x;
+  const int64_t R0 = row_ptr[i0], R1 = row_ptr[i1];
+  int64_t b = 0;
+  int K = 0;
+  if (i < npfb) { b = row_ptr[i]; K = cnt[i]; }
   double sh = 0.0, sx = 0.0, sy = 0.0, sl = 0.0;
-  // unrolled so that a batch of loads is in flight; the additions keep the
-  // reference's sequential order
-#pragma unroll 8
-  for (int k = 0; k < K; k++) {
-    sh += wh[b + k];
-    sx += wx[b + k];
-    sy += wy[b + k];
-    sl += wl[b + k];
+  for (int64_t cs = R0; cs < R1; cs += SUM_CH) {
+    const int n = (int)min((int64_t)SUM_CH, R1 - cs);
+    for (int e = threadIdx.x; e < n; e += SUM_B) {
+      cx[e] = wx[cs + e]; cy[e] = wy[cs + e]; cl[e] = wl[cs + e]; ch[e] = wh[cs + e];
+    }
+    __syncthreads();
+    const int lo = (int)(max(b, cs) - cs);
+    const int hi = (int)(min(b + K, cs + n) - cs);
+    for (int e = lo; e < hi; e++) {
+      sh += ch[e];
+      sx += cx[e];
+      sy += cy[e];
+      sl += cl[e];
+    }
+    for (int e = threadIdx.x; e < n; e += SUM_B) {
+      wx[cs + e] = cx[e] / L;
+      wy[cs + e] = cy[e] / L;
+      wl[cs + e] = cl[e] / L / L;
+    }
+    __syncthreads();
   }
+  if (i >= npfb) return;
   hsum[i] = sh;
   gsx[i] = sx / L;
   gsy[i] = sy / L;
   lsum[i] = sl / L / L;
-#pragma unroll 8
-  for (int k = 0; k < K; k++) {
-    wx[b + k] = wx[b + k] / L;
-    wy[b + k] = wy[b + k] / L;
-    wl[b + k] = wl[b + k] / L / L;
-  }
 }
 
-// filter_coefficients (labf.F90:1723-1811)
-__global__ void k_filter_coeffs(int npfb, const int64_t *row_ptr, const int *cnt,
-                                const int *col, const double *x, const double *y,
-                                const double *h, const double *s, const int *nt, int nb,
-                                double *wh, double *hsum) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npfb) return;
-  const int64_t b = row_ptr[i];
-  const int K = cnt[i];
-  const int t = nb > 0 ? nt[i] : 999;
-  double fc;
-  if (t >= 0 && t <= 2) {
-    // boundary nodes filter along the boundary only: radial test function,
-    // length scale s (:1770-1792)
-    const double tmp = (2.0 / 3.0) * SV_PI / s[i];
-    double ls = 0.0;
-    for (int k = 0; k < K; k++) {
-      const int j = col[b + k];
-      const double dx = x[j] - x[i], dy = y[j] - y[i];
-      const double f = 1.0 - cos(tmp * sqrt(dx * dx + dy * dy));
-      ls = ls + f * wh[b + k];
+// filter_coefficients (labf.F90:1723-1811).  Two passes over the block's
+// staged range: the first sums (test function) x w_hyp along each row, which
+// fixes the row's coefficient; the second scales the row by it, sums the
+// scaled weights and writes the chunk back.
+__global__ __launch_bounds__(SUM_B) void k_filter_coeffs(int npfb, const int64_t *row_ptr,
+                                                         const int *cnt, const int *col,
+                                                         const double *x, const double *y,
+                                                         const double *h, const double *s,
+                                                         const int *nt, int nb, double *wh,
+                                                         double *hsum) {
+  __shared__ double cw[SUM_CH];
+  __shared__ int cc[SUM_CH];
+  const int i0 = blockIdx.x * SUM_B;
+  const int i1 = min(i0 + SUM_B, npfb);
+  const int i = i0 + (int)threadIdx.x;
+  const int64_t R0 = row_ptr[i0], R1 = row_ptr[i1];
+  const bool act = i < npfb;
+  int64_t b = 0;
+  int K = 0;
+  if (act) { b = row_ptr[i]; K = cnt[i]; }
+  const int t = (act && nb > 0) ? nt[i] : 999;
+  // boundary nodes filter along the boundary only: radial test function,
+  // length scale s (:1770-1792); the others the product test function with
+  // length scale 2 h sqrt(pi / K) (:1738-1755)
+  const bool radial = t >= 0 && t <= 2;
+  double tmp = 0.0, xi = 0.0, yi = 0.0;
+  if (act) {
+    xi = x[i];
+    yi = y[i];
+    if (radial) {
+      tmp = (2.0 / 3.0) * SV_PI / s[i];
+    } else {
+      const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
+      tmp = (2.0 / 3.0) * SV_PI / lscal;
     }
-    fc = (1.0 / 3.0) / ls;
-    fc = fc * 0.5;
-  } else {
-    const double lscal = 2.0 * h[i] * sqrt(SV_PI / (double)K);
-    const double tmp = (2.0 / 3.0) * SV_PI / lscal;
-    double ls = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < K; k++) {
-      const int j = col[b + k];
-      const double dx = x[j] - x[i], dy = y[j] - y[i];
-      const double f = 1.0 - cos(tmp * dx) * cos(tmp * dy);
-      ls = ls + f * wh[b + k];
+  }
+  double ls = 0.0;
+  for (int64_t cs = R0; cs < R1; cs += SUM_CH) {
+    const int n = (int)min((int64_t)SUM_CH, R1 - cs);
+    for (int e = threadIdx.x; e < n; e += SUM_B) { cw[e] = wh[cs + e]; cc[e] = col[cs + e]; }
+    __syncthreads();
+    const int lo = (int)(max(b, cs) - cs);
+    const int hi = (int)(min(b + K, cs + n) - cs);
+    for (int e = lo; e < hi; e++) {
+      const int j = cc[e];
+      const double dx = x[j] - xi, dy = y[j] - yi;
+      const double f = radial ? 1.0 - cos(tmp * sqrt(dx * dx + dy * dy))
+                              : 1.0 - cos(tmp * dx) * cos(tmp * dy);
+      ls = ls + f * cw[e];
     }
-    fc = (2.0 / 3.0) / ls;
-    // weaker filtering on the FD rows of walls (:1756-1763)
-    if (t < 0 && nt[i + t] == 0) {
-      const double oosqrt2 = 1.0 / sqrt(2.0);
-      if (t == -1) fc = fc * 0.5 * oosqrt2;
-      if (t == -2) fc = fc * 0.5;
-      if (t == -3) fc = fc * oosqrt2;
-      if (t == -4) fc = fc * oosqrt2;
+    __syncthreads();
+  }
+  double fc = 0.0;
+  if (act) {
+    if (radial) {
+      fc = (1.0 / 3.0) / ls;
+      fc = fc * 0.5;
+    } else {
+      fc = (2.0 / 3.0) / ls;
+      // weaker filtering on the FD rows of walls (:1756-1763)
+      if (t < 0 && nt[i + t] == 0) {
+        const double oosqrt2 = 1.0 / sqrt(2.0);
+        if (t == -1) fc = fc * 0.5 * oosqrt2;
+        if (t == -2) fc = fc * 0.5;
+        if (t == -3) fc = fc * oosqrt2;
+        if (t == -4) fc = fc * oosqrt2;
+      }
     }
   }
   double hs = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < K; k++) {
-    const double w = wh[b + k] * fc;
-    wh[b + k] = w;
-    hs = hs + w;
+  for (int64_t cs = R0; cs < R1; cs += SUM_CH) {
+    const int n = (int)min((int64_t)SUM_CH, R1 - cs);
+    for (int e = threadIdx.x; e < n; e += SUM_B) cw[e] = wh[cs + e];
+    __syncthreads();
+    const int lo = (int)(max(b, cs) - cs);
+    const int hi = (int)(min(b + K, cs + n) - cs);
+    for (int e = lo; e < hi; e++) {
+      const double w = cw[e] * fc;
+      cw[e] = w;
+      hs = hs + w;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += SUM_B) wh[cs + e] = cw[e];
+    __syncthreads();
   }
-  hsum[i] = hs;
+  if (act) hsum[i] = hs;
 }
 
 #define SV_TUNE_SAMPLE 4096
@@ -1341,7 +1397,7 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
 }
 
 int sv_dev_sums(sv_ctx *c) {
-  const int n = c->npfb, B = 256;
+  const int n = c->npfb, B = SUM_B;
   k_sums<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->L_char, c->wx, c->wy,
                                                c->wl, c->wh, c->gsx, c->gsy, c->lsum, c->hsum);
   SV_HIP(c, hipGetLastError());
@@ -1353,7 +1409,7 @@ int sv_dev_sums(sv_ctx *c) {
 }
 
 int sv_dev_filter_coeffs(sv_ctx *c) {
-  const int n = c->npfb, B = 256;
+  const int n = c->npfb, B = SUM_B;
   k_filter_coeffs<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->col_raw, c->x,
                                                         c->y, c->h, c->s, c->node_type, c->nb,
                                                         c->wh, c->hsum);
