@@ -9,13 +9,14 @@
 //      scaled offset, g_k: Hermite ABFs times the Wendland C2 kernel) is
 //      contracted on the FP64 matrix cores, v_mfma_f64_16x16x4_f64, with
 //      16- or 32-neighbour chunks of X and g staged in LDS;
-//   2. A is factorised by Gaussian elimination with partial pivoting with
-//      each row in one lane's registers (pivot search by DPP reductions,
-//      pivot row broadcast through LDS);
-//   3. the right-hand sides (d/dx, d/dy, Laplacian, hyperviscous filter) are
-//      solved by substitution in pivot order and improved by one step of
-//      iterative refinement with the residual accumulated in double-double,
-//      which brings the solution to the exact solution of the FP64 matrix (the
+//   2. A x = b for the four right-hand sides (d/dx, d/dy, Laplacian,
+//      hyperviscous filter) by Gauss-Jordan elimination with partial pivoting,
+//      each row and its right-hand sides in one lane's registers (pivot search
+//      by DPP reductions, pivot row broadcast through LDS): N steps in
+//      sequence and no substitutions;
+//   3. one step of iterative refinement: the residual accumulated in
+//      double-double and the elimination replayed on it (N more steps), which
+//      brings the solution to the exact solution of the FP64 matrix (the
 //      reference's Golub-Reinsch SVD, svdlib.F90, is only within cond(A)*eps
 //      of it);
 //   4. weights w_k = (psi . g_k)/hh are evaluated one neighbour per lane and
@@ -187,29 +188,26 @@ struct Pack {
   static constexpr int N = nsize(M);
   static constexpr int S = N <= 16 ? 16 : N <= 32 ? 32 : 64;
   static constexpr int G = SV_WAVE / S;
-  // the original matrix (for the refinement residual) stays in registers when
-  // one system fills the wavefront, else in LDS
-  static constexpr bool A0_REGS = (G == 1);
   // register budget: waves per SIMD (order 8 holds two 44-double rows)
   static constexpr int W = M <= 6 ? 2 : 1;
 };
 
-// LDS of one wavefront (doubles):
-//   A0  [G][N][LDA]  each system's moment matrix as assembled (row-major; LDA
-//                    odd, so lanes reading different rows hit different banks);
-//                    with A0_REGS only one matrix, needed until the rows are
-//                    in registers
-//   C   the assembly's CH-neighbour chunks of monomial and ABF rows
-//       ([2][CH][NP]); after the assembly the same space holds the pivot-row
-//       broadcast ([G][NP]) during the factorisation and then the
-//       substitution values and the solution Xs ([G][NP][4])
+// LDS of one wavefront (doubles), one area reused by the phases:
+//   assembly     the CH-neighbour chunks of monomial and ABF rows ([2][CH][NP]),
+//                then the assembled matrix ([N][LDA], row-major; LDA odd, so
+//                lanes reading different rows hit different banks), which its
+//                group's lanes take into registers before the next assembly
+//   elimination  two pivot-row buffers ([2][G][UBS]: the row's entries past the
+//                pivot, 1 / pivot at N, its four right-hand sides at N+1..N+4),
+//                used by alternate steps;
+//                after them Xs [G][NP][4]: the solution for the residual, then
+//                the refinement's per-step values, then the refined solution
 template <int M, int CH_>
 struct Lds {
   static constexpr int N = nsize(M), NP = npad(M), LDA = (N % 2 == 1) ? N : N + 1;
   static constexpr int CH = CH_;
   static constexpr int G = Pack<M>::G;
   static constexpr int ASZ = N * LDA;
-  static constexpr int NA = Pack<M>::A0_REGS ? 1 : G;
   // 16-neighbour chunks are stored column-major (function f of neighbour k
   // at f * LDC + k, LDC = 17: the 16 lanes writing one function hit distinct
   // banks; row-major, all 16 rows started on the same bank); 32-neighbour
@@ -217,9 +215,11 @@ struct Lds {
   static constexpr bool TR = (CH == 16);
   static constexpr int LDC = TR ? CH + 1 : 1;
   static constexpr int CHUNK = TR ? 2 * LDC * NP : 2 * CH * NP;
-  static constexpr int XS = G * NP * 4;
-  static constexpr int CSZ = CHUNK > XS ? CHUNK : XS;
-  static constexpr int TOTAL = NA * ASZ + CSZ;
+  static constexpr int UBS = NP + 8;
+  static constexpr int XOFF = 2 * G * UBS;
+  static constexpr int XS = XOFF + G * NP * 4;
+  static constexpr int T1 = CHUNK > ASZ ? CHUNK : ASZ;
+  static constexpr int TOTAL = T1 > XS ? T1 : XS;
 };
 
 // Moment matrix A = sum_k X_k g_k^T (labf.F90:88-123) on the FP64 matrix
@@ -296,44 +296,6 @@ __device__ __forceinline__ void assemble(const SysArgs &S_, int i, double hh, in
   __syncthreads();
 }
 
-// Weights w_k = (psi_q . g_k)/hh (labf.F90:183-215, k <= ij_count) and the
-// node volume (calc_node_volumes, labf.F90:453-503).  Four lanes per
-// neighbour, lane q = l & 3 for right-hand side q (d/dx, d/dy, Laplacian,
-// filter), 16 neighbours per pass: each lane keeps its column of psi in
-// registers for the whole stencil instead of re-reading 4n values from LDS
-// per neighbour.
-template <int M>
-__device__ __forceinline__ void eval_weights(const SysArgs &S_, int i, bool sysD, bool sysF,
-                                             double hh, int Kfull, int64_t base,
-                                             const double *Xs) {
-  constexpr int N = nsize(M);
-  const int l = threadIdx.x, q = l & 3, kl = l >> 2;
-  const double xi = S_.x[i], yi = S_.y[i];
-  double ps[N];
-#pragma unroll
-  for (int a = 0; a < N; a++) ps[a] = Xs[a * 4 + q];
-  const bool want = q < 3 ? sysD : sysF;
-  // (a two-level select: a four-way one can be lowered to a table in scratch)
-  double *const d01 = (q & 1) ? S_.wy : S_.wx;
-  double *const d23 = (q & 1) ? S_.wh : S_.wl;
-  double *const dst = (q & 2) ? d23 : d01;
-  const double rhh = 1.0 / hh;
-  double vsum = 0.0;
-  for (int k = kl; k < Kfull; k += SV_WAVE / 4) {
-    const int j = S_.col[base + k];
-    const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
-    const double rad = sqrt(x * x + y * y) * rhh;
-    const double o = abf_dot<M>(x * rhh, y * rhh, wab(rad, S_.h2), ps);
-    if (want) dst[base + k] = q < 2 ? o * rhh : q == 2 ? o * rhh * rhh : o;
-    if (q == 3 && sysF) vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
-  }
-  if (sysF) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) vsum += __shfl_xor(vsum, o);
-    if (l == 0) S_.vol[i] = 1.0 / vsum;
-  }
-}
-
 // One lane's double from lane (l xor 1 / 2 ...) pattern CTRL within its row of
 // 16 (DPP; every source lane is inside the row)
 template <int CTRL>
@@ -395,66 +357,135 @@ __device__ __forceinline__ SysOf sys_of(const SysArgs &S_, int sys) {
   return o;
 }
 
+// Weights w_k = (psi_q . g_k)/hh (labf.F90:183-215, k <= ij_count) and the
+// node volume (calc_node_volumes, labf.F90:453-503) of the `count` (<= G)
+// systems of a unit.  Four lanes per neighbour, lane q = l & 3 for
+// right-hand side q (d/dx, d/dy, Laplacian, filter), 16 neighbours per pass
+// of each system; each lane keeps its column of every system's psi in
+// registers, and one pass evaluates a neighbour of each system (independent
+// chains that the compiler interleaves).  A lane past a system's stencil
+// evaluates the node itself and stores nothing.
+template <int M, int G>
+__device__ __forceinline__ void eval_weights(const SysArgs &S_, int first, int count,
+                                             const double *Xs) {
+  constexpr int N = nsize(M), NP = npad(M);
+  const int l = threadIdx.x, q = l & 3, kl = l >> 2;
+  int ii[G], K[G];
+  int64_t bs[G];
+  bool want[G], sf[G];
+  double hh[G], rhh[G], xi[G], yi[G], vsum[G];
+  double *dst[G];
+  double ps[G][N];
+  int kmax = 0;
+#pragma unroll
+  for (int gg = 0; gg < G; gg++) {
+    const bool on = gg < count;
+    const SysOf o = sys_of<M>(S_, first + (on ? gg : 0));
+    ii[gg] = o.i;
+    K[gg] = on ? o.Kfull : 0;
+    bs[gg] = o.base;
+    want[gg] = on && (q < 3 ? o.sysD : o.sysF);
+    sf[gg] = on && o.sysF;
+    hh[gg] = o.hh;
+    rhh[gg] = 1.0 / o.hh;
+    xi[gg] = S_.x[o.i];
+    yi[gg] = S_.y[o.i];
+    vsum[gg] = 0.0;
+    // (a two-level select: a four-way one can be lowered to a table in scratch)
+    double *const d01 = (q & 1) ? S_.wy : S_.wx;
+    double *const d23 = (q & 1) ? S_.wh : S_.wl;
+    dst[gg] = (q & 2) ? d23 : d01;
+#pragma unroll
+    for (int a = 0; a < N; a++) ps[gg][a] = Xs[gg * NP * 4 + a * 4 + q];
+    kmax = K[gg] > kmax ? K[gg] : kmax;
+  }
+  for (int k = kl; k < kmax; k += SV_WAVE / 4) {
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+      const bool in = k < K[gg];
+      // col has one int past the last row, so the clamped read stays inside it
+      const int jr = S_.col[bs[gg] + (in ? k : 0)];
+      const int j = in ? jr : ii[gg];
+      const double x = -(xi[gg] - S_.x[j]), y = -(yi[gg] - S_.y[j]);
+      const double rad = sqrt(x * x + y * y) * rhh[gg];
+      const double o = abf_dot<M>(x * rhh[gg], y * rhh[gg], wab(rad, S_.h2), ps[gg]);
+      if (in && want[gg])
+        dst[gg][bs[gg] + k] = q < 2 ? o * rhh[gg] : q == 2 ? o * rhh[gg] * rhh[gg] : o;
+      if (in && q == 3 && sf[gg])
+        vsum[gg] += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh[gg] * hh[gg]);
+    }
+  }
+#pragma unroll
+  for (int gg = 0; gg < G; gg++) {
+    if (sf[gg]) {
+      double v = vsum[gg];
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+      if (l == 0) S_.vol[ii[gg]] = 1.0 / v;
+    }
+  }
+}
+
 // Solve the `count` (<= G) systems first .. first+count-1 of one order; the
 // block is one wavefront.
 //  1. each moment matrix A = sum_k X_k g_k^T (labf.F90:88-123) is contracted on
 //     the matrix cores into LDS (assemble), rows and columns beyond nkeep set
-//     to the identity;
-//  2. lane r of group g takes row r of system g into registers and the group
-//     factorises by Gaussian elimination with partial pivoting, rows staying
-//     in their lanes: at step k the group's largest |a[k]| among the rows not
-//     yet chosen (lowest row on ties) is the pivot, its row is broadcast
-//     through LDS, and every other open row stores its multiplier in a[k] and
-//     updates a[k+1..N-1];
-//  3. the four right-hand sides (one row per lane) are solved by forward and
-//     backward substitution in pivot order, each step's value broadcast
-//     through LDS, then improved by one refinement step whose residual b - A x
-//     is accumulated in double-double (the reference's Golub-Reinsch SVD,
+//     to the identity, and lane r of group g takes row r of system g into
+//     registers (twice: the row to eliminate and the original for the
+//     residual);
+//  2. Gauss-Jordan elimination with partial pivoting, rows staying in their
+//     lanes and the four right-hand sides (d/dx, d/dy, Laplacian, filter)
+//     carried along: at step k the group's largest |a[k]| among the rows not
+//     yet chosen (lowest row on ties) is the pivot, its row and right-hand
+//     sides are broadcast through LDS, and every other row stores its
+//     multiplier in a[k] and updates a[k+1..N-1] and its right-hand sides; the
+//     pivot row of step k then holds pivot_k x_k;
+//  3. one refinement step: the residual b - A x accumulated in double-double,
+//     the elimination's multipliers replayed on it (one broadcast per step),
+//     and the correction added (the reference's Golub-Reinsch SVD,
 //     svdlib.F90, is only within cond(A)*eps of the exact solution of the FP64
 //     matrix; the refined solution is at it to rounding);
-//  4. the weights of each system, one neighbour per lane (eval_weights).
+//  4. the weights of the unit's systems, four lanes per neighbour (eval_weights).
 template <int M, int CH_>
 __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int count, double *lds) {
   using P = Pack<M>;
   using L = Lds<M, CH_>;
-  constexpr int N = P::N, S = P::S, LDA = L::LDA, NP = L::NP, ASZ = L::ASZ;
-  double *A0 = lds;                     // [NA][N][LDA]
-  double *C = lds + L::NA * ASZ;        // chunks, then the broadcast / solution area
-  double *Ub = C;                       // [G][NP]: pivot row, its reciprocal at index N
-  double *Xs = C;                       // [G][NP][4]
+  constexpr int N = P::N, S = P::S, LDA = L::LDA, NP = L::NP, UBS = L::UBS;
+  double *Xs = lds + L::XOFF;           // [G][NP][4]
   const int l = threadIdx.x, g = l / S, r = l % S;
   const bool active = g < count;
   const bool row = active && r < N;
 
   // ---- 1. assembly, one system after the other ----
-  // (A0_REGS: G == 1, one system; its matrix goes to registers twice: the
-  // row to factorise and the original for the residual)
-  const int nasm = P::A0_REGS ? 1 : count;
-  for (int gg = 0; gg < nasm; gg++) {
+  double a[N], a0[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) { a[c] = 0.0; a0[c] = 0.0; }
+  for (int gg = 0; gg < count; gg++) {
     const SysOf o = sys_of<M>(S_, first + gg);
-    double *Ag = A0 + gg * ASZ;
-    assemble<M, CH_>(S_, o.i, o.hh, o.Kuse, o.base, C, C + L::CHUNK / 2, Ag);
+    assemble<M, CH_>(S_, o.i, o.hh, o.Kuse, o.base, lds, lds + L::CHUNK / 2, lds);
     if (o.nkeep < N && l < N) {
       // a rolled loop: unrolled, its N identity entries would be hoisted to
       // the kernel entry and held in registers throughout
 #pragma unroll 1
       for (int rr = 0; rr < N; rr++)
-        if (rr >= o.nkeep || l >= o.nkeep) Ag[rr * LDA + l] = (rr == l) ? 1.0 : 0.0;
+        if (rr >= o.nkeep || l >= o.nkeep) lds[rr * LDA + l] = (rr == l) ? 1.0 : 0.0;
     }
     __syncthreads();
-  }
-  double a[N];
-  double a0[P::A0_REGS ? N : 1];
+    if (row && g == gg) {
 #pragma unroll
-  for (int c = 0; c < N; c++) {
-    a[c] = row ? A0[g * ASZ + r * LDA + c] : 0.0;
-    if (P::A0_REGS) a0[c] = a[c];
-    if (c % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+      for (int c = 0; c < N; c++) {
+        a[c] = lds[r * LDA + c];
+        a0[c] = a[c];
+        if (c % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();   // the next assembly reuses the area
   }
   const SysOf o = sys_of<M>(S_, first + (active ? g : 0));
-  __syncthreads();   // the chunks' space becomes the broadcast area
 
-  // ---- 2. elimination with partial pivoting, rows in their lanes ----
+  // ---- 2. Gauss-Jordan elimination with partial pivoting ----
+  double b[4];
+  rhs_row(r, o.sysD, o.sysF, o.mf, b);
   int ps = row ? -1 : N;     // step at which this row was the pivot (N: padding lane)
   double rinv = 1.0;         // 1 / pivot of this row
   bool sing = false;
@@ -470,18 +501,22 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
     if (S < 64) mg &= (1ull << S) - 1ull;
     const int p = mg ? __builtin_ctzll(mg) : S;
     const bool piv = open && r == p;
+    // step k's buffer; step k+1 uses the other one, and step k+2 writes this
+    // one again only after step k+1's barrier, which every read of step k
+    // precedes
+    double *u = lds + (k & 1) * (P::G * UBS) + g * UBS;
     if (piv) {
       ps = k;
       sing = !(a[k] != 0.0);   // zero or NaN
       rinv = 1.0 / (a[k] == 0.0 ? 1.0 : a[k]);
-      double *u = Ub + g * NP;
 #pragma unroll
       for (int j = k + 1; j < N; j++) u[j] = a[j];
       u[N] = rinv;
+#pragma unroll
+      for (int q = 0; q < 4; q++) u[N + 1 + q] = b[q];
     }
     __syncthreads();
-    if (open && !piv) {
-      const double *u = Ub + g * NP;
+    if (row && !piv) {
       const double lk = a[k] * u[N];
       a[k] = lk;
       // batches of 8 loads in flight (a full row of them would cost registers)
@@ -490,67 +525,32 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
         a[j] = fma(-lk, u[j], a[j]);
         if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
       }
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[q] = fma(-lk, u[N + 1 + q], b[q]);
     }
-    __syncthreads();
   }
 
-  // ---- 3. substitutions in pivot order, one refinement ----
-  double b[4];
-  rhs_row(r, o.sysD, o.sysF, o.mf, b);
+  // ---- 3. one refinement step ----
   double *xg = Xs + g * NP * 4;
-  auto substitute = [&](double s[4]) {
-    // forward: L y = P s; step k's value y_k is row ps == k's entry
+  double xk[4];
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      if (ps == k) {
-        double2 *d = reinterpret_cast<double2 *>(xg + k * 4);
-        d[0] = double2{s[0], s[1]};
-        d[1] = double2{s[2], s[3]};
-      }
-      __syncthreads();
-      if (ps > k && ps < N) {
-        const double2 *d = reinterpret_cast<const double2 *>(xg + k * 4);
-        const double2 y01 = d[0], y23 = d[1];
-        s[0] = fma(-a[k], y01.x, s[0]);
-        s[1] = fma(-a[k], y01.y, s[1]);
-        s[2] = fma(-a[k], y23.x, s[2]);
-        s[3] = fma(-a[k], y23.y, s[3]);
-      }
-      __syncthreads();
-    }
-    // backward: U x = y; row ps == k finishes x_k (column k), the rows
-    // pivoted before it take its term
-#pragma unroll
-    for (int k = N - 1; k >= 0; k--) {
-      if (ps == k) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) s[q] *= rinv;
-        double2 *d = reinterpret_cast<double2 *>(xg + k * 4);
-        d[0] = double2{s[0], s[1]};
-        d[1] = double2{s[2], s[3]};
-      }
-      __syncthreads();
-      if (ps < k) {
-        const double2 *d = reinterpret_cast<const double2 *>(xg + k * 4);
-        const double2 x01 = d[0], x23 = d[1];
-        s[0] = fma(-a[k], x01.x, s[0]);
-        s[1] = fma(-a[k], x01.y, s[1]);
-        s[2] = fma(-a[k], x23.x, s[2]);
-        s[3] = fma(-a[k], x23.y, s[3]);
-      }
-      __syncthreads();
-    }
-  };
-  double xk[4] = {b[0], b[1], b[2], b[3]};
-  substitute(xk);   // row ps == k now holds x_k; xg[k] = x_k for every column k
+  for (int q = 0; q < 4; q++) xk[q] = b[q] * rinv;
+  if (ps >= 0 && ps < N) {
+    double2 *d = reinterpret_cast<double2 *>(xg + ps * 4);
+    d[0] = double2{xk[0], xk[1]};
+    d[1] = double2{xk[2], xk[3]};
+  }
+  __syncthreads();
   // residual of row r: b_r - sum_c A0[r][c] x_c in double-double
   double res[4];
   {
+    double b0[4];
+    rhs_row(r, o.sysD, o.sysF, o.mf, b0);
     double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
     if (row) {
 #pragma unroll
       for (int c = 0; c < N; c++) {
-        const double av = P::A0_REGS ? a0[c] : A0[g * ASZ + r * LDA + c];
+        const double av = a0[c];
         const double2 *d = reinterpret_cast<const double2 *>(xg + c * 4);
         const double2 x01 = d[0], x23 = d[1];
         const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
@@ -568,25 +568,44 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       double d, de;
-      two_sum(b[q], -sh[q], d, de);
+      two_sum(b0[q], -sh[q], d, de);
       res[q] = d + (de - sl[q]);
     }
   }
   __syncthreads();   // every residual read of xg is done
-  substitute(res);   // row ps == k holds the correction of x_k
+  // the elimination replayed on the residual: step k's pivot row gives its
+  // value (slot k), every other row subtracts its multiplier times it
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    if (ps == k) {
+      double2 *d = reinterpret_cast<double2 *>(xg + k * 4);
+      d[0] = double2{res[0], res[1]};
+      d[1] = double2{res[2], res[3]};
+    }
+    __syncthreads();
+    if (ps != k && ps < N) {
+      const double2 *d = reinterpret_cast<const double2 *>(xg + k * 4);
+      const double2 y01 = d[0], y23 = d[1];
+      res[0] = fma(-a[k], y01.x, res[0]);
+      res[1] = fma(-a[k], y01.y, res[1]);
+      res[2] = fma(-a[k], y23.x, res[2]);
+      res[3] = fma(-a[k], y23.y, res[3]);
+    }
+  }
+  __syncthreads();   // every read of the slots is done
   if (ps >= 0 && ps < N) {
     double2 *d = reinterpret_cast<double2 *>(xg + ps * 4);
-    d[0] = double2{xk[0] + res[0], xk[1] + res[1]};
-    d[1] = double2{xk[2] + res[2], xk[3] + res[3]};
+    d[0] = double2{xk[0] + res[0] * rinv, xk[1] + res[1] * rinv};
+    d[1] = double2{xk[2] + res[2] * rinv, xk[3] + res[3] * rinv};
   }
   if (sing && row) atomicAdd(S_.singular, 1);
   __syncthreads();
 
-  // ---- 4. weights, one system after the other ----
-  for (int gg = 0; gg < count; gg++) {
-    const SysOf og = sys_of<M>(S_, first + gg);
-    eval_weights<M>(S_, og.i, og.sysD, og.sysF, og.hh, og.Kfull, og.base, Xs + gg * NP * 4);
-  }
+  // ---- 4. weights of the unit's systems, two at a time (four would cost
+  // the order-4 kernel half its occupancy in registers) ----
+  constexpr int EG = P::G < 2 ? P::G : 2;
+  for (int e0 = 0; e0 < count; e0 += EG)
+    eval_weights<M, EG>(S_, first + e0, min(EG, count - e0), Xs + e0 * NP * 4);
   __syncthreads();
 }
 
