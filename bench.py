@@ -124,9 +124,11 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def weight_flops(order, k_mean):
-    """Algorithmic FP64 flops of one moment system (n unknowns, K links):
-    assembly 2 K n^2 (MFMA), LU 2/3 n^3, three solves of 4 right-hand sides
-    24 n^2, two residuals 16 n^2, weights 8 K n."""
+    """Algorithmic FP64 flops of one moment system (n unknowns, K links),
+    counted for the textbook method whatever the device does: assembly
+    2 K n^2 (MFMA), LU 2/3 n^3, three solves of 4 right-hand sides 24 n^2,
+    two residuals 16 n^2, weights 8 K n.  (The device's Gauss-Jordan
+    elimination does n^3 + 8 n^2 in place of the LU and the first solve.)"""
     n = (order * order + 3 * order) // 2
     return 2.0 * k_mean * n * n + 2.0 / 3.0 * n ** 3 + 40.0 * n * n + 8.0 * k_mean * n
 

@@ -305,17 +305,38 @@ __device__ __forceinline__ double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// One lane's double from the partner row (S = 32: rows 0 <-> 1, 2 <-> 3) or
+// half (S = 64: lanes l <-> l ^ 32) at the same position, by
+// v_permlane16_swap / v_permlane32_swap with both operands v: the first
+// result holds the partner's value in the odd rows (upper half), the second
+// in the even rows (lower half)
+template <int S>
+__device__ __forceinline__ double swap_d(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  if (S == 32) {
+    const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const bool odd = (threadIdx.x >> 4) & 1;
+    return __hiloint2double(odd ? rh[0] : rh[1], odd ? rl[0] : rl[1]);
+  }
+  const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const bool up = threadIdx.x >= 32;
+  return __hiloint2double(up ? rh[0] : rh[1], up ? rl[0] : rl[1]);
+}
+
 // Maximum of v over each group of S consecutive lanes, in every lane: four
 // DPP steps inside each row of 16 (quad_perm [1,0,3,2], [2,3,0,1],
-// row_half_mirror, row_mirror), then swaps across rows for S = 32, 64.
+// row_half_mirror, row_mirror), then register swaps across rows for S = 32
+// and across halves for S = 64 (no trip through the LDS unit)
 template <int S>
 __device__ __forceinline__ double group_max(double v) {
   v = fmax(v, dpp_d<0xB1>(v));
   v = fmax(v, dpp_d<0x4E>(v));
   v = fmax(v, dpp_d<0x141>(v));
   v = fmax(v, dpp_d<0x140>(v));
-  if (S >= 32) v = fmax(v, __shfl_xor(v, 16));
-  if (S >= 64) v = fmax(v, __shfl_xor(v, 32));
+  if (S >= 32) v = fmax(v, swap_d<32>(v));
+  if (S >= 64) v = fmax(v, swap_d<64>(v));
   return v;
 }
 
