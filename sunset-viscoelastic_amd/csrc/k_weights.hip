@@ -511,12 +511,19 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
   double rinv = 1.0;         // 1 / pivot of this row
   bool sing = false;
   const int gbase = g * S;
+  // the group maximum of the next step's candidates; each step starts the
+  // next step's search as soon as its column k+1 is updated, so the DPP chain
+  // overlaps the rest of the row update
+  auto candidate = [&](int k) {
+    double c = ps < 0 ? fabs(a[k]) : -1.0;
+    if (c != c) c = 0.0;   // a NaN entry still yields a pivot (flagged below)
+    return c;
+  };
+  double mx = group_max<S>(candidate(0));
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const bool open = ps < 0;
-    double cand = open ? fabs(a[k]) : -1.0;
-    if (cand != cand) cand = 0.0;   // a NaN entry still yields a pivot (flagged below)
-    const double mx = group_max<S>(cand);
+    const double cand = candidate(k);
     const unsigned long long bal = __ballot(open && cand == mx);
     unsigned long long mg = bal >> gbase;
     if (S < 64) mg &= (1ull << S) - 1ull;
@@ -537,18 +544,21 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
       for (int q = 0; q < 4; q++) u[N + 1 + q] = b[q];
     }
     __syncthreads();
-    if (row && !piv) {
-      const double lk = a[k] * u[N];
-      a[k] = lk;
-      // batches of 8 loads in flight (a full row of them would cost registers)
-#pragma unroll
-      for (int j = k + 1; j < N; j++) {
-        a[j] = fma(-lk, u[j], a[j]);
-        if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; q++) b[q] = fma(-lk, u[N + 1 + q], b[q]);
+    // every row but the pivot takes multiplier lk = a[k] / pivot; the pivot
+    // row (and a padding lane) runs the same instructions with lk = 0, which
+    // leaves its entries as they are (up to the sign of an exact zero), so
+    // the update is one basic block with the next step's search
+    const bool upd = row && !piv;
+    const double lk = upd ? a[k] * u[N] : 0.0;
+    a[k] = upd ? lk : a[k];
+    if (k + 1 < N) {
+      a[k + 1] = fma(-lk, u[k + 1], a[k + 1]);
+      mx = group_max<S>(candidate(k + 1));
     }
+#pragma unroll
+    for (int j = k + 2; j < N; j++) a[j] = fma(-lk, u[j], a[j]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[q] = fma(-lk, u[N + 1 + q], b[q]);
   }
 
   // ---- 3. one refinement step ----
@@ -622,11 +632,9 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
   if (sing && row) atomicAdd(S_.singular, 1);
   __syncthreads();
 
-  // ---- 4. weights of the unit's systems, two at a time (four would cost
-  // the order-4 kernel half its occupancy in registers) ----
-  constexpr int EG = P::G < 2 ? P::G : 2;
-  for (int e0 = 0; e0 < count; e0 += EG)
-    eval_weights<M, EG>(S_, first + e0, min(EG, count - e0), Xs + e0 * NP * 4);
+  // ---- 4. weights, one system after the other (two per pass measured no
+  // faster at order 6 and cost the order-4 kernel a wave per SIMD) ----
+  for (int e0 = 0; e0 < count; e0++) eval_weights<M, 1>(S_, first + e0, 1, Xs + e0 * NP * 4);
   __syncthreads();
 }
 
