@@ -46,7 +46,8 @@ enum sv_op {                                          /* derivatives.F90 routine
   SV_OP_GRAD_X = 2,      /* calc_gradient_only(1)  :416-464 */
   SV_OP_GRAD_Y = 3,      /* calc_gradient_only(2)  :416-464 */
   SV_OP_FILTER = 4,      /* calc_filtered_var      :360-414 */
-  SV_OP_FILTER_TERM = 5  /* calc_filter_term       :466-511 */
+  SV_OP_FILTER_TERM = 5, /* calc_filter_term       :466-511 */
+  SV_OP_LAP_TRANSVERSE = 6 /* calc_laplacian_transverse_only_on_bound :306-358 */
 };
 
 typedef struct {
@@ -218,6 +219,13 @@ int sv_calc_gradient_only(sv_ctx *ctx, int32_t dim, const double *phi,
                           double *gradphi);                              /* :416-464 */
 int sv_calc_filtered_var(sv_ctx *ctx, double *phi);                      /* :360-414 */
 int sv_calc_filter_term(sv_ctx *ctx, const double *phi, double *filtphi);/* :466-511 */
+/* The Laplacian, except on boundary nodes (walls, inflow, outflow): there the
+ * tangential second derivative only, with the 1D LABFM weights of
+ * calc_boundary_weights (labf.F90:755) scaled by 1/L_char^2 in
+ * sv_calc_labf_sums (labf.F90:428-443).  With boundary nodes it needs
+ * sv_calc_boundary_weights before the sums. */
+int sv_calc_laplacian_transverse_only_on_bound(sv_ctx *ctx, const double *phi,
+                                               double *lapphi);          /* :306-358 */
 /* Device variant over fields stored on real nodes only (ghosts resolved to
  * their periodic parents): nfields fields, each npfb doubles. */
 int sv_apply_dev(sv_ctx *ctx, int32_t op, int32_t nfields, const double *const *phi,

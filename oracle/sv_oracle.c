@@ -806,6 +806,23 @@ void or_calc_laplacian(const or_domain *d, const double *phi, double *lap) {
   }
 }
 
+/* calc_laplacian_transverse_only_on_bound, derivatives.F90:306-358: the
+ * Laplacian on the internal nodes (:315-325); on boundary node jj the sum over
+ * k <= ij_count_small of phi_j ij_wb_grad2(2,k,jj) minus phi_i times its row
+ * sum (:327-337) */
+void or_calc_laplacian_transverse(const or_domain *d, const double *phi, double *lap) {
+  or_calc_laplacian(d, phi, lap);
+  if (!d->wbt) return;
+  for (int jj = 0; jj < d->nb; jj++) {
+    const int i = d->blist[jj];
+    const size_t base = (size_t)i * d->nplink;
+    const double *w = d->wbt + (size_t)jj * d->nplink;
+    double t = 0;
+    for (int k = 0; k < d->ij_count_small[i]; k++) t = t + phi[d->ij_link[base + k]] * w[k];
+    lap[i] = t - phi[i] * d->wbtsum[jj];
+  }
+}
+
 void or_calc_gradient_only(const or_domain *d, int dim, const double *phi, double *g) {
   if (dim < 1 || dim > 2) {                    /* 2D: third component is zero */
     for (int i = 0; i < d->npfb; i++) g[i] = 0.0;

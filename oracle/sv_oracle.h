@@ -161,6 +161,7 @@ void   or_psi_from_c(double cxx, double cxy, double cyy, double *pxx, double *px
 /* ---- stencil operators (phi sized np, outputs sized npfb) ---- */
 void or_calc_gradient(const or_domain *d, const double *phi, double *gx, double *gy);
 void or_calc_laplacian(const or_domain *d, const double *phi, double *lap);
+void or_calc_laplacian_transverse(const or_domain *d, const double *phi, double *lap);
 void or_calc_gradient_only(const or_domain *d, int dim, const double *phi, double *g);
 void or_calc_filtered_var(const or_domain *d, double *phi);
 void or_calc_filter_term(const or_domain *d, const double *phi, double *out);

@@ -128,6 +128,8 @@ struct BndArgs {
   int *cnt, *cnt_small;
   double *wx, *wy, *wl, *wh, *vol;
   double *ooR;                // [nb]
+  double *wtt;                // [nb][wtt_ld] tangential second-derivative weights
+  int wtt_ld;
   double h3;
 };
 
@@ -206,6 +208,8 @@ __global__ __launch_bounds__(BT) void k_bnd_nodes(BndArgs a) {
                                                                             : -0.25 / dx;
   }
   const double xt = -a.rny[i], yt = a.rnx[i], hh = a.h[i];
+  double *wt = a.wtt + (size_t)jb * a.wtt_ld;
+  for (int k = 0; k < a.wtt_ld; k++) wt[k] = 0.0;
   double A[36];
   assemble1d(a, i, base, K, xt, yt, hh, A);
   double b[3][6];
@@ -221,6 +225,7 @@ __global__ __launch_bounds__(BT) void k_bnd_nodes(BndArgs a) {
     if (a.nt[j] != ti) continue;
     a.wy[base + k] = w1d(a, i, j, xt, yt, hh, b[0]) / hh;
     const double g2t = w1d(a, i, j, xt, yt, hh, b[1]) / hh / hh;
+    wt[k] = g2t;                // ij_wb_grad2(2,k,jj) (:755)
     a.wl[base + k] = g2t;       // the tangential second derivative, for now
     a.wh[base + k] = 0.0 + w1d(a, i, j, xt, yt, hh, b[2]);
     if (ti == 0) tt = tt + g2t * ((a.rnx[j] * a.rnx[i] + a.rny[j] * a.rny[i]) - 1.0);
@@ -369,14 +374,87 @@ int sv_dev_boundary_normals(sv_ctx *c, const double *rnx_host, const double *rny
   return 0;
 }
 
+namespace {
+// calc_labf_sums' boundary block (labf.F90:428-443) for the tangential
+// second derivative: the row sum, then both divided by L_char^2
+__global__ __launch_bounds__(BT) void k_bnd_tt_sums(int nb, const int *blist, const int *cnt,
+                                                    double *wtt, int ld, double L,
+                                                    double *wtt_sum) {
+  const int jb = blockIdx.x * BT + threadIdx.x;
+  if (jb >= nb) return;
+  const int K = cnt[blist[jb]];
+  double *wt = wtt + (size_t)jb * ld;
+  double s = 0.0;
+  for (int k = 0; k < K; k++) s = s + wt[k];
+  wtt_sum[jb] = s / L / L;
+  for (int k = 0; k < K; k++) wt[k] = wt[k] / L / L;
+}
+
+// The boundary branch of calc_laplacian_transverse_only_on_bound
+// (derivatives.F90:327-337): sum over k <= ij_count_small of phi_j w_k, in
+// order, minus phi_i times the row sum
+__global__ __launch_bounds__(BT) void k_bnd_tt_apply(int nb, const int *blist,
+                                                     const int *cnt_small,
+                                                     const int64_t *row_ptr, const int *col,
+                                                     const int *irel, const double *wtt, int ld,
+                                                     const double *wtt_sum, const double *phi,
+                                                     double *out) {
+  const int jb = blockIdx.x * BT + threadIdx.x;
+  if (jb >= nb) return;
+  const int i = blist[jb];
+  const int K = cnt_small[i];
+  const int64_t base = row_ptr[i];
+  const double *wt = wtt + (size_t)jb * ld;
+  double s = 0.0;
+  for (int k = 0; k < K; k++) {
+    const int j = irel ? irel[col[base + k]] : col[base + k];
+    s = s + phi[j] * wt[k];
+  }
+  out[i] = s - phi[i] * wtt_sum[jb];
+}
+}  // namespace
+
+int sv_dev_boundary_tt_sums(sv_ctx *c) {
+  if (c->nb == 0 || !c->wtt) return 0;
+  k_bnd_tt_sums<<<(c->nb + BT - 1) / BT, BT, 0, c->stream>>>(c->nb, c->blist, c->cnt, c->wtt,
+                                                              c->wtt_ld, c->L_char, c->wtt_sum);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int sv_dev_boundary_tt_apply(sv_ctx *c, const double *phi, double *out, bool raw_cols,
+                             hipStream_t st) {
+  if (c->nb == 0) return 0;
+  if (!(c->have_bweights && c->wtt))
+    return sv_fail(c, SV_E_ARG, "calc_laplacian_transverse_only_on_bound: "
+                   "sv_calc_boundary_weights must come first");
+  k_bnd_tt_apply<<<(c->nb + BT - 1) / BT, BT, 0, st>>>(
+      c->nb, c->blist, c->cnt_small, c->row_ptr, c->col_raw, raw_cols ? nullptr : c->irel,
+      c->wtt, c->wtt_ld, c->wtt_sum, phi, out);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int sv_dev_boundary_weights(sv_ctx *c) {
   if (c->nb == 0) { c->have_bweights = true; return 0; }
   if (!c->ooR && sv_alloc(c, (void **)&c->ooR, sizeof(double) * c->nb)) return SV_E_NOMEM;
+  if (c->wtt && c->wtt_ld < c->maxK) {
+    sv_free(c, c->wtt);
+    c->wtt = nullptr;
+  }
+  if (!c->wtt) {
+    c->wtt_ld = c->maxK > 0 ? c->maxK : 1;
+    if (sv_alloc(c, (void **)&c->wtt, sizeof(double) * (size_t)c->nb * c->wtt_ld))
+      return SV_E_NOMEM;
+  }
+  if (!c->wtt_sum && sv_alloc(c, (void **)&c->wtt_sum, sizeof(double) * c->nb))
+    return SV_E_NOMEM;
   BndArgs a{};
   a.npfb = c->npfb; a.nb = c->nb; a.blist = c->blist; a.nt = c->node_type;
   a.x = c->x; a.y = c->y; a.s = c->s; a.h = c->h; a.rnx = c->rnx; a.rny = c->rny;
   a.row_ptr = c->row_ptr; a.col = c->col_raw; a.cnt = c->cnt; a.cnt_small = c->cnt_small;
   a.wx = c->wx; a.wy = c->wy; a.wl = c->wl; a.wh = c->wh; a.vol = c->vol; a.ooR = c->ooR;
+  a.wtt = c->wtt; a.wtt_ld = c->wtt_ld;
   a.h3 = c->h2 * c->h0;
   const int g1 = (c->nb + BT - 1) / BT, g2 = (2 * c->nb + BT - 1) / BT;
   k_bnd_nodes<<<g1, BT, 0, c->stream>>>(a);

@@ -98,7 +98,8 @@ int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_col
   a.o1 = out + c->npfb;
   switch (op) {
   case SV_OP_GRAD: a.kcnt = c->cnt_small; a.w0 = c->swx; a.w1 = c->swy; a.s0 = c->gsx; a.s1 = c->gsy; break;
-  case SV_OP_LAP: a.kcnt = c->cnt_small; a.w0 = c->swl; a.s0 = c->lsum; break;
+  case SV_OP_LAP:
+  case SV_OP_LAP_TRANSVERSE: a.kcnt = c->cnt_small; a.w0 = c->swl; a.s0 = c->lsum; break;
   case SV_OP_GRAD_X: a.kcnt = c->cnt_small; a.w0 = c->swx; a.s0 = c->gsx; break;
   case SV_OP_GRAD_Y: a.kcnt = c->cnt_small; a.w0 = c->swy; a.s0 = c->gsy; break;
   case SV_OP_FILTER:
@@ -112,5 +113,9 @@ int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_col
   default: k_apply<SV_OP_LAP><<<nb, B, 0, st>>>(a); break;
   }
   SV_HIP(c, hipGetLastError());
+  // calc_laplacian_transverse_only_on_bound: the Laplacian on the internal
+  // nodes (derivatives.F90:315-325), the boundary nodes overwritten with the
+  // tangential part (:327-337)
+  if (op == SV_OP_LAP_TRANSVERSE) return sv_dev_boundary_tt_apply(c, phi, out, raw_cols, st);
   return 0;
 }

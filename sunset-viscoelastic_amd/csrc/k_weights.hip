@@ -1449,6 +1449,7 @@ int sv_dev_sums(sv_ctx *c) {
   k_sums<<<(n + B - 1) / B, B, 0, c->stream>>>(n, c->row_ptr, c->cnt, c->L_char, c->wx, c->wy,
                                                c->wl, c->wh, c->gsx, c->gsy, c->lsum, c->hsum);
   SV_HIP(c, hipGetLastError());
+  if (int rc = sv_dev_boundary_tt_sums(c)) return rc;
   SV_HIP(c, hipStreamSynchronize(c->stream));
   c->have_sums = true;
   c->sell_dirty = true;

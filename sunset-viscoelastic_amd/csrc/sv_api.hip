@@ -171,6 +171,7 @@ int sv_set_nodes(sv_ctx *c, int32_t npfb, const double *x, const double *y, cons
   c->have_lists = c->have_weights = c->have_sums = c->have_filter = false;
   c->have_state = false;
   c->blist = nullptr; c->rnx = c->rny = nullptr; c->ooR = nullptr;
+  c->wtt = c->wtt_sum = nullptr; c->wtt_ld = 0;
   c->have_normals = c->have_bweights = false;
   // boundary nodes and their FD rows (setup_domain.F90:179-193, :314-326)
   c->hnt.assign(npfb, 999);
@@ -499,7 +500,7 @@ static int ensure_tmp(sv_ctx *c) {
 int sv_apply(sv_ctx *c, int32_t op, double *phi, double *out) {
   SV_CHECK(c, c && phi, "null argument");
   SV_CHECK(c, c->have_weights && c->have_sums, "weights and sums must be computed first");
-  SV_CHECK(c, op >= SV_OP_GRAD && op <= SV_OP_FILTER_TERM, "bad operator");
+  SV_CHECK(c, op >= SV_OP_GRAD && op <= SV_OP_LAP_TRANSVERSE, "bad operator");
   if ((op == SV_OP_FILTER || op == SV_OP_FILTER_TERM))
     SV_CHECK(c, c->have_filter, "sv_filter_coefficients must come first");
   if (op != SV_OP_FILTER) SV_CHECK(c, out, "null output");
@@ -532,6 +533,9 @@ int sv_calc_gradient_only(sv_ctx *c, int32_t dim, const double *phi, double *g) 
   return sv_apply(c, dim == 1 ? SV_OP_GRAD_X : SV_OP_GRAD_Y, const_cast<double *>(phi), g);
 }
 int sv_calc_filtered_var(sv_ctx *c, double *phi) { return sv_apply(c, SV_OP_FILTER, phi, nullptr); }
+int sv_calc_laplacian_transverse_only_on_bound(sv_ctx *c, const double *phi, double *l) {
+  return sv_apply(c, SV_OP_LAP_TRANSVERSE, const_cast<double *>(phi), l);
+}
 int sv_calc_filter_term(sv_ctx *c, const double *phi, double *f) {
   return sv_apply(c, SV_OP_FILTER_TERM, const_cast<double *>(phi), f);
 }

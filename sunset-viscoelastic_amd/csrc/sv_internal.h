@@ -194,6 +194,11 @@ struct sv_ctx {
   int *blist = nullptr;
   double *rnx = nullptr, *rny = nullptr;
   double *ooR = nullptr;     // [nb] wall curvature 1/R (calc_boundary_weights)
+  // [nb][wtt_ld] tangential second-derivative weights of the boundary nodes
+  // over their (pruned) rows (ij_wb_grad2(2,:,jj)) and [nb] their sums,
+  // both divided by L_char^2 once the sums are computed
+  double *wtt = nullptr, *wtt_sum = nullptr;
+  int wtt_ld = 0;
   bool have_normals = false, have_bweights = false;
   // inflow settings (sv_set_inflow)
   int inflow_type = 0, u_control = 0;
@@ -340,6 +345,14 @@ int sv_dev_filter_coeffs(sv_ctx *c);
 // ---- kernels: boundary nodes (k_bound.hip) ----
 int sv_dev_boundary_normals(sv_ctx *c, const double *rnx_host, const double *rny_host);
 int sv_dev_boundary_weights(sv_ctx *c);
+// the boundary nodes' tangential second-derivative weights: sums and the
+// 1/L_char^2 scaling (calc_labf_sums, labf.F90:428-443)
+int sv_dev_boundary_tt_sums(sv_ctx *c);
+// out[i] of every boundary node i = its tangential second derivative of phi
+// (calc_laplacian_transverse_only_on_bound, derivatives.F90:327-337); phi
+// sized np with raw (ghost) columns, else npfb with the ghosts' parents
+int sv_dev_boundary_tt_apply(sv_ctx *c, const double *phi, double *out, bool raw_cols,
+                             hipStream_t st);
 // ---- kernels: stencil operators (k_stencil.hip) ----
 int sv_dev_apply(sv_ctx *c, int op, const double *phi, double *out, bool raw_cols,
                  hipStream_t st);

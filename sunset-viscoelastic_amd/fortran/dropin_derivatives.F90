@@ -94,17 +94,19 @@ contains
     segment_time_local(7) = segment_time_local(7) + segment_dtend - segment_dtstart
   end subroutine calc_filter_term
 
-  !! derivatives.F90:306-358: without boundary nodes this is the Laplacian.  With
-  !! boundary nodes the tangential-only operator is not exported by the library
-  !! (its time loop applies it inside the stage sweep), so the drop-in stops.
+  !! derivatives.F90:306-358: the Laplacian on internal nodes, the tangential
+  !! second derivative only on boundary nodes (their 1D LABFM weights from
+  !! calc_boundary_weights, kept on the device)
   subroutine calc_laplacian_transverse_only_on_bound(phi, lapphi)
     real(rkind), dimension(:), intent(in) :: phi
     real(rkind), dimension(:), intent(inout) :: lapphi
-    if (nb /= 0) then
-       write(6,*) "sunset_amd: calc_laplacian_transverse_only_on_bound with nb>0 has no drop-in"
-       stop
-    end if
-    call calc_laplacian(phi, lapphi)
+    real(c_double), allocatable :: out(:)
+    segment_dtstart = omp_get_wtime()
+    allocate(out(npfb))
+    call apply_op(SV_OP_LAP_TRANSVERSE, phi, out)
+    lapphi(1:npfb) = out
+    segment_dtend = omp_get_wtime()
+    segment_time_local(6) = segment_time_local(6) + segment_dtend - segment_dtstart
   end subroutine calc_laplacian_transverse_only_on_bound
 
   subroutine calc_grad2bound(phi, grad2phi)              !! derivatives.F90:191-238

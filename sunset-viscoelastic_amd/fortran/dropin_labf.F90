@@ -5,8 +5,10 @@
 !! filter_coefficients (:1723-1811) run on the device and leave the same arrays
 !! in common_vars as the reference does after each call (ij_w_grad, ij_w_lap,
 !! ij_w_hyp, their sums, vol).  calc_boundary_weights (:505-1039) runs on the
-!! device too and prunes the host lists of boundary nodes as the reference does
-!! (ij_wb_grad2, used only by the compressible rhs.F90, is not kept).
+!! device too and prunes the host lists of boundary nodes as the reference does;
+!! ij_wb_grad2 stays on the device, where the tangential part (:755) serves the
+!! drop-in calc_laplacian_transverse_only_on_bound (read by rhs_di.F90:399-408
+!! and the compressible rhs.F90) and its normal part is not kept.
 !! adapt_stencils (:1079-1721) runs on the device after find_neighbours,
 !! writes the adapted h, h_small of the real nodes back to common_vars and
 !! leaves the device lists rebuilt on them (ghosts kept from build_domain),

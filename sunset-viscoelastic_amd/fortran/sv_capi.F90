@@ -25,7 +25,7 @@ module sv_capi
   end type sv_phys_t
 
   integer(c_int32_t), parameter :: SV_OP_GRAD = 0, SV_OP_LAP = 1, SV_OP_GRAD_X = 2, &
-       SV_OP_GRAD_Y = 3, SV_OP_FILTER = 4, SV_OP_FILTER_TERM = 5
+       SV_OP_GRAD_Y = 3, SV_OP_FILTER = 4, SV_OP_FILTER_TERM = 5, SV_OP_LAP_TRANSVERSE = 6
 
   interface
      integer(c_int) function sv_create(cfg, out) bind(C, name='sv_create')

@@ -61,6 +61,7 @@ EXPORTS = [
     "sv_diagnostics", "sv_diagnostics_noalloc", "sv_set_boundaries", "sv_set_inflow",
     "sv_calc_boundary_weights", "sv_adapt_stencils", "sv_set_state", "sv_filter_variables",
     "sv_set_state_device", "sv_get_state_device", "sv_rollout_vjp_device",
+    "sv_calc_laplacian_transverse_only_on_bound",
 ]
 
 KERNELS = ["dt", "stage1", "stage2", "stage3", "stage4", "filter", "reduce", "apply"]
@@ -96,6 +97,7 @@ def lib():
     L.sv_apply.argtypes = [C.c_void_p, C.c_int32, PD, PD]
     L.sv_calc_gradient.argtypes = [C.c_void_p, PD, PD]
     L.sv_calc_laplacian.argtypes = [C.c_void_p, PD, PD]
+    L.sv_calc_laplacian_transverse_only_on_bound.argtypes = [C.c_void_p, PD, PD]
     L.sv_calc_gradient_only.argtypes = [C.c_void_p, C.c_int32, PD, PD]
     L.sv_calc_filtered_var.argtypes = [C.c_void_p, PD]
     L.sv_calc_filter_term.argtypes = [C.c_void_p, PD, PD]
@@ -312,6 +314,14 @@ class Solver:
         phi = _f64(phi)
         out = np.zeros(self.npfb)
         self._ck(lib().sv_calc_laplacian(self.h, _d(phi), _d(out)))
+        return out
+
+    def laplacian_transverse(self, phi):
+        """calc_laplacian_transverse_only_on_bound (derivatives.F90:306-358):
+        the Laplacian, tangential only on boundary nodes."""
+        phi = _f64(phi)
+        out = np.zeros(self.npfb)
+        self._ck(lib().sv_calc_laplacian_transverse_only_on_bound(self.h, _d(phi), _d(out)))
         return out
 
     def gradient_only(self, dim, phi):

@@ -79,6 +79,7 @@ def lib():
         L.or_psi_from_c.argtypes = [C.c_double] * 3 + [PD] * 3
         L.or_calc_gradient.argtypes = [C.POINTER(Domain), PD, PD, PD]
         L.or_calc_laplacian.argtypes = [C.POINTER(Domain), PD, PD]
+        L.or_calc_laplacian_transverse.argtypes = [C.POINTER(Domain), PD, PD]
         L.or_calc_gradient_only.argtypes = [C.POINTER(Domain), C.c_int, PD, PD]
         L.or_calc_filtered_var.argtypes = [C.POINTER(Domain), PD]
         L.or_calc_filter_term.argtypes = [C.POINTER(Domain), PD, PD]
@@ -309,6 +310,13 @@ class OracleSim:
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         out = np.zeros(self.d.npfb)
         lib().or_calc_laplacian(C.byref(self.d), _pd(phi), _pd(out))
+        return out
+
+    def laplacian_transverse(self, phi):
+        """calc_laplacian_transverse_only_on_bound (derivatives.F90:306-358)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.zeros(self.d.npfb)
+        lib().or_calc_laplacian_transverse(C.byref(self.d), _pd(phi), _pd(out))
         return out
 
     def gradient_only(self, dim, phi):

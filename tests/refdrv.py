@@ -38,19 +38,21 @@ def dropin_binary(order: int) -> str:
 def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e-6,
                   time0=0.0, emax=(1e-5, 1e-5, 1e-5), itime0=0, nplink=0, threads=4,
                   timeout=600, stencil=None, fields_only=False, binary=None, inflow=None,
-                  adapt=False):
+                  adapt=False, transverse=False):
     """stencil (optional): dict(nplink, ij_count, ij_count_small, ij_link (0-based,
     [npfb, nplink]), w_grad [npfb, nplink, 2], w_lap, w_hyp, gsum [npfb, 2], lsum,
     hsum, vol) loaded instead of the reference's own neighbour/weight setup.
     adapt: run adapt_stencils after the first neighbour search (and the search
     again after it); the output gains adapt_h, adapt_h_small, adapt_count,
-    adapt_count_small as adapt_stencils left them."""
+    adapt_count_small as adapt_stencils left them.
+    transverse: also calc_laplacian_transverse_only_on_bound of ro and cxx
+    (lt_ro, lt_cxx) after the operators on ro."""
     phys = dict(L_char=1.0, U_char=1.0, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
                 Mdiff=0.0, grav=(0, 0, 0), drive=(0, 0, 0)) | (phys or {})
     n = ns.npfb
     has_b = getattr(ns, "rnx", None) is not None
     flags = ((2 if stencil is not None else 0) | (4 if fields_only else 0) | (8 if has_b else 0)
-             | (16 if adapt else 0))
+             | (16 if adapt else 0) | (32 if transverse else 0))
     inflow = dict(inflow_type=0, u_control=0, u_start=0.0, u_end=1.0, ramp_time=1.0) | (
         inflow or {})
     hdr = np.array([n, nsteps, 0 if dt_mode == "pid" else 1, nplink, flags, *ns.bc, itime0, 0,
@@ -103,10 +105,10 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
             times[parts[1]] = float(parts[-1])
         if "REFSTEP" in line:
             times["steps"].append(float(line.split()[-1]))
-    return _parse(raw, n, nsteps, fields_only, adapt), times
+    return _parse(raw, n, nsteps, fields_only, adapt, transverse), times
 
 
-def _parse(raw, n, nsteps, fields_only=False, adapt=False):
+def _parse(raw, n, nsteps, fields_only=False, adapt=False, transverse=False):
     off = 0
 
     def take(cnt, dt):
@@ -145,6 +147,9 @@ def _parse(raw, n, nsteps, fields_only=False, adapt=False):
     out["grad_ro_y"] = take(n, np.float64)
     out["lap_ro"] = take(n, np.float64)
     out["filt_ro"] = take(n, np.float64)
+    if transverse:
+        out["lt_ro"] = take(n, np.float64)
+        out["lt_cxx"] = take(n, np.float64)
     return _parse_fields(out, take, n)
 
 

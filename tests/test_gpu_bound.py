@@ -118,8 +118,15 @@ def test_boundary_operators_vs_oracle(kind):
     gx, gy = s.gradient(phi)
     ox, oy = o.gradient(phi)
     for a, b in ((gx, ox), (gy, oy), (s.laplacian(phi), o.laplacian(phi)),
+                 (s.laplacian_transverse(phi), o.laplacian_transverse(phi)),
                  (s.filtered_var(phi), o.filtered_var(phi))):
         assert np.abs(a - b).max() <= 1e-9 * max(np.abs(b).max(), 1.0)
+    # calc_laplacian_transverse_only_on_bound differs from the Laplacian on
+    # the boundary nodes only
+    bnd = (ns.node_type >= 0) & (ns.node_type <= 2)
+    lt, lap = s.laplacian_transverse(phi), s.laplacian(phi)
+    assert np.array_equal(lt[~bnd], lap[~bnd])
+    assert np.abs(lt[bnd] - lap[bnd]).max() > 1e-3 * np.abs(lap[bnd]).max()
     s.close()
 
 

@@ -297,3 +297,52 @@ def test_di_wall_polymer_diffusion_closed_form():
     want = 2e-3 * (-0.1) * (2 * np.pi) ** 2 * np.cos(2 * np.pi * x[walls])
     got = res[2e-3]["xx"][walls] - res[0.0]["xx"][walls]
     assert np.abs(got - want).max() <= 1e-3 * np.abs(want).max()
+
+
+def _lt_ic(ns):
+    n = ns.npfb
+    return dict(ro=1.0 + 0.05 * np.sin(2 * np.pi * ns.x) * np.cos(np.pi * ns.y) + 0.02 * ns.y ** 2,
+                u=np.zeros(n), v=np.zeros(n),
+                cxx=1.0 + 0.1 * np.cos(2 * np.pi * ns.x) * ns.y, cxy=np.zeros(n), cyy=np.ones(n))
+
+
+@pytest.mark.skipif(not refdrv.available(6), reason="oracle/_ref not built")
+@pytest.mark.parametrize("kind,order", [("wall", 6), ("wall", 4), ("io", 6)])
+def test_laplacian_transverse_vs_reference(kind, order):
+    """calc_laplacian_transverse_only_on_bound (derivatives.F90:306-358) of
+    two fields: the Laplacian on internal nodes, the tangential second
+    derivative (ij_wb_grad2(2,:), labf.F90:755, scaled in calc_labf_sums
+    :428-443) on walls, inflow and outflow nodes."""
+    ns = (wall_channel(16, 16, order=order, seed=1) if kind == "wall"
+          else inflow_outflow_channel(16, 16, order=order, seed=2))
+    ic = _lt_ic(ns)
+    R, _ = refdrv.run_reference(ns, order, ic, nsteps=0, transverse=True)
+    o = OL.OracleSim(ns, order).setup()
+    bnd = (ns.node_type >= 0) & (ns.node_type <= 2)
+    for key, f in (("lt_ro", ic["ro"]), ("lt_cxx", ic["cxx"])):
+        got = o.laplacian_transverse(o.extend(f))
+        scale = np.abs(R[key]).max()
+        err_b = np.abs(got[bnd] - R[key][bnd]).max() / scale
+        err_i = np.abs(got[~bnd] - R[key][~bnd]).max() / scale
+        print(f"{kind} order {order} {key}: boundary {err_b:.2e}, internal {err_i:.2e}")
+        assert err_b < 1e-9 and err_i < 1e-9, (key, err_b, err_i)
+        if key == "lt_ro":   # ro has a normal second derivative on the boundary
+            assert np.abs(o.laplacian(o.extend(f))[bnd] - R[key][bnd]).max() > 1e-2 * scale
+
+
+def test_laplacian_transverse_closed_form():
+    """Straight walls: the tangential second derivative of (x - x0)^6 is
+    30 (x - x0)^4 (1D LABFM exact for degree 6 along the wall), and the normal
+    part of (y - 0.2)^4 (12 (y - 0.2)^2) is left out; internal nodes get the
+    full Laplacian."""
+    ns = wall_channel(24, 24, order=6, seed=3)
+    o = OL.OracleSim(ns, 6).setup()
+    x0 = 0.37
+    f = lambda x, y: (x - x0) ** 6 + (y - 0.2) ** 4
+    lt = o.laplacian_transverse(_poly_field(o, f))
+    walls = np.where(ns.node_type == 0)[0]
+    want = 30 * (ns.x[walls] - x0) ** 4
+    assert np.abs(lt[walls] - want).max() < 1e-7 * np.abs(want).max()
+    full = o.laplacian(_poly_field(o, f))
+    inner = np.where(ns.node_type == 999)[0]
+    assert np.array_equal(lt[inner], full[inner])
