@@ -227,7 +227,10 @@ int sv_calc_filter_term(sv_ctx *ctx, const double *phi, double *filtphi);/* :466
 int sv_calc_laplacian_transverse_only_on_bound(sv_ctx *ctx, const double *phi,
                                                double *lapphi);          /* :306-358 */
 /* Device variant over fields stored on real nodes only (ghosts resolved to
- * their periodic parents): nfields fields, each npfb doubles. */
+ * their periodic parents): nfields fields, each npfb doubles, device pointers.
+ * Asynchronous: the launches are queued on `stream` as given (NULL: the null
+ * stream), in order with the caller's work on it; the outputs are complete
+ * when that stream's work is. */
 int sv_apply_dev(sv_ctx *ctx, int32_t op, int32_t nfields, const double *const *phi,
                  double *const *out, void *stream);
 

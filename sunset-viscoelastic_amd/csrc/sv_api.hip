@@ -546,7 +546,10 @@ int sv_apply_dev(sv_ctx *c, int32_t op, int32_t nfields, const double *const *ph
   SV_CHECK(c, c->have_weights && c->have_sums, "weights and sums must be computed first");
   if (int rc = set_device(c)) return rc;
   if (int rc = ensure_sell(c)) return rc;
-  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  // queued on the caller's stream as given: NULL is the null (default)
+  // stream, which orders the launches with the caller's work there (a
+  // framework's default stream is the null stream)
+  hipStream_t st = (hipStream_t)stream;
   for (int f = 0; f < nfields; f++)
     if (int rc = sv_dev_apply(c, op, phi[f], out[f], false, st)) return rc;
   return 0;

@@ -7,7 +7,10 @@ Checks: one HIP runtime mapped; the state set from CUDA tensors reads back
 bit for bit; the rollout loss and its gradient with U0 a CUDA tensor (no host
 copy at the boundary) equal the CPU-tensor path bit for bit, and the gradient
 is a CUDA tensor; Wi, beta, eps gradients equal too; a host pointer is
-refused.  Prints one line per check and "OK" at the end."""
+refused; the device-field operators (sv_apply_dev on CUDA tensors of the real
+nodes, ghosts resolved to their parents) equal the host drop-in operators bit
+for bit on a channel with walls, calc_laplacian_transverse_only_on_bound
+included.  Prints one line per check and "OK" at the end."""
 import os
 import sys
 
@@ -83,7 +86,34 @@ def main():
     assert Ph == Pd, (Ph, Pd)
     print(f"CUDA-tensor rollout = CPU-tensor rollout bit for bit: loss {float(Ld):.6e}, "
           f"|grad| {float(Gd.norm()):.6e}, dL/deps {Pd['eps']:.6e}", flush=True)
+    apply_dev_check(dev)
     print("OK", flush=True)
+
+
+def apply_dev_check(dev):
+    import ctypes as C
+    from sunset_amd.nodes import wall_channel
+    ns = wall_channel(16, 16, order=6, seed=1)
+    s = sunset_amd.Solver(morder=6, form="lc").setup(ns)
+    f = np.sin(2 * np.pi * ns.x) * np.cos(np.pi * ns.y) + ns.y ** 3
+    _, _, irel, _ = s.ghosts()
+    full = np.concatenate([f, f[irel]])   # periodic ghosts: their parents' values
+    host = {1: s.laplacian(full), 2: s.gradient_only(1, full), 3: s.gradient_only(2, full),
+            6: s.laplacian_transverse(full)}
+    L = sunset_amd.lib()
+    fd = torch.tensor(f, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream(dev)
+    for op, want in host.items():
+        out = torch.full((ns.npfb,), float("nan"), dtype=torch.float64, device=dev)
+        pin = (C.c_void_p * 1)(fd.data_ptr())
+        pout = (C.c_void_p * 1)(out.data_ptr())
+        rc = L.sv_apply_dev(s.h, op, 1, pin, pout, C.c_void_p(st.cuda_stream))
+        assert rc == 0, sunset_amd.lib().sv_last_error(s.h)
+        got = out.cpu().numpy()   # ordered after the launches on torch's stream
+        assert np.array_equal(got, want), (op, float(np.abs(got - want).max()))
+    s.close()
+    print("device-field operators = host drop-in operators bit for bit (wall channel, "
+          "transverse Laplacian included)", flush=True)
 
 
 if __name__ == "__main__":
