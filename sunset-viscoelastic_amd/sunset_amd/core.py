@@ -520,7 +520,8 @@ class Ensemble:
         self.sizes = [m.npfb for m in members]
 
     def __len__(self):
-        return len(self.sizes)
+        """Members, as the library counts them (sv_ensemble_size)."""
+        return int(lib().sv_ensemble_size(self.h))
 
     def close(self):
         if getattr(self, "h", None):
