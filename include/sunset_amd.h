@@ -252,7 +252,8 @@ int sv_set_state(sv_ctx *ctx, const double *ro, const double *rou, const double 
  * caller's node order) on the context's GPU in this library's HIP runtime
  * (from a Python process, import torch first so that the library binds to
  * torch's runtime and its tensors' data pointers qualify); the caller's
- * stream (nullable) is synchronised first.  A pointer of another runtime or
+ * stream is synchronised first (NULL: the null stream, a framework's default
+ * stream), the copy is complete on return.  A pointer of another runtime or
  * device is refused with SV_E_ARG.  No counterpart in the reference: these
  * serve device-tensor bindings (no host copy). */
 int sv_set_state_device(sv_ctx *ctx, const double *const U[6], void *stream);
@@ -375,8 +376,8 @@ int sv_rollout_forward(sv_ctx *ctx, int32_t nsteps, double *loss);
 int sv_rollout_vjp(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
                    double *const grad0[6]);
 /* As sv_rollout_vjp with ubar_final (nullable, or null fields) and grad0 as
- * device pointers (see sv_set_state_device); the caller's stream (nullable)
- * is synchronised first, the result is complete on return. */
+ * device pointers (see sv_set_state_device); the caller's stream (NULL: the
+ * null stream) is synchronised first, the result is complete on return. */
 int sv_rollout_vjp_device(sv_ctx *ctx, double loss_bar, const double *const ubar_final[6],
                           double *const grad0[6], void *stream);
 /* gradient of the same quantity with respect to the parameters, from the last

@@ -1056,7 +1056,9 @@ int sv_rollout_vjp_device(sv_ctx *c, double loss_bar, const double *const ubar_f
     if (ubar_final && ubar_final[q])
       if (int rc = sv_check_device_ptr(c, ubar_final[q], "ubar_final")) return rc;
   }
-  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  // the caller's stream, the null stream included (a framework's default
+  // stream): its pending work on the arrays completes first
+  SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
   return rollout_vjp(c, loss_bar, ubar_final, grad0, true);
 }
 

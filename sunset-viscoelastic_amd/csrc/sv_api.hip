@@ -805,7 +805,9 @@ int sv_set_state_device(sv_ctx *c, const double *const U[6], void *stream) {
   if (int rc = set_device(c)) return rc;
   for (int q = 0; q < 6; q++)
     if (int rc = sv_check_device_ptr(c, U[q], "sv_set_state_device")) return rc;
-  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  // the caller's stream, the null stream included (a framework's default
+  // stream): its pending work on the arrays completes first
+  SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
   if (int rc = sv_dev_gather_records(c, c->U, U)) return rc;
   if (int rc = sv_dev_init_state(c)) return rc;   // synchronises the context's stream
   return 0;
@@ -819,7 +821,9 @@ int sv_get_state_device(sv_ctx *c, double *const U[6], void *stream) {
   for (int q = 0; q < 6; q++)
     if (U[q])
       if (int rc = sv_check_device_ptr(c, U[q], "sv_get_state_device")) return rc;
-  if (stream) SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
+  // the caller's stream, the null stream included (a framework's default
+  // stream): its pending work on the arrays completes first
+  SV_HIP(c, hipStreamSynchronize((hipStream_t)stream));
   if (int rc = sv_dev_scatter_records(c, c->U, U)) return rc;
   SV_HIP(c, hipStreamSynchronize(c->stream));
   return 0;
