@@ -31,8 +31,9 @@
 !!          boundary nodes, each followed by its FD rows (types -1..-4) as the
 !!          IPART reader lays them out (setup_domain.F90:179-193)
 !!          bit 5 (32) -- after the operators on ro, calc_laplacian_transverse_
-!!          only_on_bound of ro and of cxx: real64 lt_ro(npfb), lt_cxx(npfb)
-!!          (written only with the lists, bit 2 clear)
+!!          only_on_bound of ro and of cxx and calc_divergence of (rou, rov):
+!!          real64 lt_ro(npfb), lt_cxx(npfb), div(npfb) (written only with the
+!!          lists, bit 2 clear)
 !!          bit 4 (16) -- call adapt_stencils after the first find_neighbours
 !!          and find_neighbours again after it, as the main program does
 !!          without restart (sunset.F90:36-52); the output then starts with
@@ -55,7 +56,7 @@ program sunset_ref
   integer(ikind) :: hdr(12),nsteps,dt_mode,flags,i,k,nm,itime0
   real(rkind) :: rh(24)
   real(rkind),allocatable :: tmp(:),tro(:),tu(:),tv(:),tcxx(:),tcxy(:),tcyy(:)
-  real(rkind),allocatable :: g2(:,:),l1(:),f1(:),lt1(:),lt2(:)
+  real(rkind),allocatable :: g2(:,:),l1(:),f1(:),lt1(:),lt2(:),dv(:)
   real(rkind),allocatable :: ah(:),ahs(:)
   integer(ikind),allocatable :: itmp(:),ac(:),acs(:)
   integer(ikind) :: ib2(2)
@@ -246,13 +247,14 @@ program sunset_ref
   itime = itime0
 
   !! Derivative operators on the initial density (derivatives.F90:26-116, :466-511)
-  allocate(g2(npfb,ithree),l1(npfb),f1(npfb),lt1(npfb),lt2(npfb))
+  allocate(g2(npfb,ithree),l1(npfb),f1(npfb),lt1(npfb),lt2(npfb),dv(npfb))
   call calc_gradient(ro,g2)
   call calc_laplacian(ro,l1)
   call calc_filter_term(ro,f1)
   if(iand(flags,32).ne.0) then   !! derivatives.F90:306-358
      call calc_laplacian_transverse_only_on_bound(ro,lt1)
      call calc_laplacian_transverse_only_on_bound(cxx,lt2)
+     call calc_divergence(rou,rov,dv)         !! derivatives.F90:119-189
   end if
 
   open(31,file=trim(fout),access='stream',form='unformatted',status='replace')
@@ -270,7 +272,7 @@ program sunset_ref
   write(31) ij_w_grad(1:2,1:nplink,1:npfb),ij_w_lap(1:nplink,1:npfb),ij_w_hyp(1:nplink,1:npfb)
   write(31) ij_w_grad_sum(1:2,1:npfb),ij_w_lap_sum(1:npfb),ij_w_hyp_sum(1:npfb),vol(1:npfb)
   write(31) g2(1:npfb,1),g2(1:npfb,2),l1(1:npfb),f1(1:npfb)
-  if(iand(flags,32).ne.0) write(31) lt1(1:npfb),lt2(1:npfb)
+  if(iand(flags,32).ne.0) write(31) lt1(1:npfb),lt2(1:npfb),dv(1:npfb)
   end if
 
   !! Time loop (sunset.F90:75-106) without output/statistics

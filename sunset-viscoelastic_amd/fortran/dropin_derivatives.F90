@@ -109,12 +109,57 @@ contains
     segment_time_local(6) = segment_time_local(6) + segment_dtend - segment_dtstart
   end subroutine calc_laplacian_transverse_only_on_bound
 
+  !! derivatives.F90:119-189 (2D): div = d(phi1)/dx + d(phi2)/dy on the real
+  !! nodes; on wall nodes, whose first-derivative weights are in the boundary
+  !! frame (normal, tangent), n . (d(phi1)/dn, d(phi2)/dn) (:159-181)
+  subroutine calc_divergence(phi1, phi2, divphi)
+    real(rkind), dimension(:), intent(in) :: phi1, phi2
+    real(rkind), dimension(:), intent(inout) :: divphi
+    real(c_double), allocatable :: d1(:), d2(:), dn2(:)
+    integer(ikind) :: jj, i
+    segment_dtstart = omp_get_wtime()
+    allocate(d1(npfb), d2(npfb))
+    call apply_op(SV_OP_GRAD_X, phi1, d1)
+    call apply_op(SV_OP_GRAD_Y, phi2, d2)
+    divphi(1:npfb) = d1 + d2
+    if (nb /= 0) then
+       allocate(dn2(npfb))
+       call apply_op(SV_OP_GRAD_X, phi2, dn2)
+       do jj = 1, nb
+          i = boundary_list(jj)
+          if (node_type(i) == 0) divphi(i) = rnorm(i,1)*d1(i) + rnorm(i,2)*dn2(i)
+       end do
+    end if
+    segment_dtend = omp_get_wtime()
+    segment_time_local(5) = segment_time_local(5) + segment_dtend - segment_dtstart
+  end subroutine calc_divergence
+
+  !! The second-derivative boundary operators (derivatives.F90:191-304) serve
+  !! only the compressible rhs.F90: without boundary nodes they do nothing (as
+  !! the reference's loops over boundary_list), with them the drop-in stops.
   subroutine calc_grad2bound(phi, grad2phi)              !! derivatives.F90:191-238
     real(rkind), dimension(:), intent(in) :: phi
     real(rkind), dimension(:,:), intent(inout) :: grad2phi
+    call no_boundary_operator("calc_grad2bound")
+  end subroutine calc_grad2bound
+
+  subroutine calc_grad2vecbound(phi1, phi2, phi3, g2phi) !! derivatives.F90:240-270
+    real(rkind), dimension(:), intent(in) :: phi1, phi2, phi3
+    real(rkind), dimension(:,:), intent(inout) :: g2phi
+    call no_boundary_operator("calc_grad2vecbound")
+  end subroutine calc_grad2vecbound
+
+  subroutine calc_grad2crossbound(gradphi, g2phi)        !! derivatives.F90:272-304
+    real(rkind), dimension(:,:), intent(in) :: gradphi
+    real(rkind), dimension(:,:), intent(inout) :: g2phi
+    call no_boundary_operator("calc_grad2crossbound")
+  end subroutine calc_grad2crossbound
+
+  subroutine no_boundary_operator(name)
+    character(len=*), intent(in) :: name
     if (nb /= 0) then
-       write(6,*) "sunset_amd: boundary operators (nb>0) have no device implementation"
+       write(6,*) "sunset_amd: ", name, " with boundary nodes has no device implementation"
        stop
     end if
-  end subroutine calc_grad2bound
+  end subroutine no_boundary_operator
 end module derivatives

@@ -46,7 +46,8 @@ def run_reference(ns, order, ic, *, nsteps=0, dt_mode="pid", phys=None, dt0=1.0e
     again after it); the output gains adapt_h, adapt_h_small, adapt_count,
     adapt_count_small as adapt_stencils left them.
     transverse: also calc_laplacian_transverse_only_on_bound of ro and cxx
-    (lt_ro, lt_cxx) after the operators on ro."""
+    (lt_ro, lt_cxx) and calc_divergence of (rou, rov) (div_rouv) after the
+    operators on ro."""
     phys = dict(L_char=1.0, U_char=1.0, Re=1.0, Wi=1.0, beta=0.5, eps=0.0, Ma=0.1,
                 Mdiff=0.0, grav=(0, 0, 0), drive=(0, 0, 0)) | (phys or {})
     n = ns.npfb
@@ -150,6 +151,7 @@ def _parse(raw, n, nsteps, fields_only=False, adapt=False, transverse=False):
     if transverse:
         out["lt_ro"] = take(n, np.float64)
         out["lt_cxx"] = take(n, np.float64)
+        out["div_rouv"] = take(n, np.float64)
     return _parse_fields(out, take, n)
 
 

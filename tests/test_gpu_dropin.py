@@ -128,10 +128,12 @@ def test_dropin_step_with_boundaries_matches_golden(name):
 
 @pytest.mark.parametrize("kind", ["wall", "io"])
 def test_dropin_laplacian_transverse_matches_reference(kind):
-    """calc_laplacian_transverse_only_on_bound through the drop-in
-    derivatives module (the device's tangential boundary weights) against the
-    reference's own routine, on a channel with walls or with an inflow and an
-    outflow (ref_driver flag 32: the operator on ro and on cxx)."""
+    """calc_laplacian_transverse_only_on_bound and calc_divergence through
+    the drop-in derivatives module (the device's tangential boundary weights;
+    the wall divergence from boundary-frame normal derivatives) against the
+    reference's own routines, on a channel with walls or with an inflow and an
+    outflow (ref_driver flag 32: the operator on ro and on cxx, the divergence
+    of (rou, rov))."""
     from sunset_amd.nodes import inflow_outflow_channel, wall_channel
     binary = refdrv.dropin_binary(6)
     if not (os.path.exists(binary) and refdrv.available(6)):
@@ -141,13 +143,14 @@ def test_dropin_laplacian_transverse_matches_reference(kind):
           else inflow_outflow_channel(16, 16, order=6, seed=2))
     n = ns.npfb
     ic = dict(ro=1.0 + 0.05 * np.sin(2 * np.pi * ns.x) * np.cos(np.pi * ns.y) + 0.02 * ns.y ** 2,
-              u=np.zeros(n), v=np.zeros(n),
+              u=0.1 * np.sin(2 * np.pi * ns.x) * (1.0 + ns.y ** 2),
+              v=0.05 * np.cos(2 * np.pi * ns.x) * np.sin(np.pi * ns.y) + 0.02 * ns.y,
               cxx=1.0 + 0.1 * np.cos(2 * np.pi * ns.x) * ns.y, cxy=np.zeros(n), cyy=np.ones(n))
     R, _ = refdrv.run_reference(ns, 6, ic, nsteps=0, transverse=True, timeout=300)
     D, _ = refdrv.run_reference(ns, 6, ic, nsteps=0, transverse=True, binary=binary,
                                 timeout=300)
     bnd = (ns.node_type >= 0) & (ns.node_type <= 2)
-    for key in ("lt_ro", "lt_cxx"):
+    for key in ("lt_ro", "lt_cxx", "div_rouv"):
         scale = np.abs(R[key]).max()
         err = np.abs(D[key] - R[key]).max() / scale
         err_b = np.abs(D[key][bnd] - R[key][bnd]).max() / scale

@@ -302,7 +302,8 @@ def test_di_wall_polymer_diffusion_closed_form():
 def _lt_ic(ns):
     n = ns.npfb
     return dict(ro=1.0 + 0.05 * np.sin(2 * np.pi * ns.x) * np.cos(np.pi * ns.y) + 0.02 * ns.y ** 2,
-                u=np.zeros(n), v=np.zeros(n),
+                u=0.1 * np.sin(2 * np.pi * ns.x) * (1.0 + ns.y ** 2),
+                v=0.05 * np.cos(2 * np.pi * ns.x) * np.sin(np.pi * ns.y) + 0.02 * ns.y,
                 cxx=1.0 + 0.1 * np.cos(2 * np.pi * ns.x) * ns.y, cxy=np.zeros(n), cyy=np.ones(n))
 
 
@@ -328,6 +329,16 @@ def test_laplacian_transverse_vs_reference(kind, order):
         assert err_b < 1e-9 and err_i < 1e-9, (key, err_b, err_i)
         if key == "lt_ro":   # ro has a normal second derivative on the boundary
             assert np.abs(o.laplacian(o.extend(f))[bnd] - R[key][bnd]).max() > 1e-2 * scale
+    # calc_divergence (derivatives.F90:119-189) of (rou, rov): d/dx + d/dy, on
+    # walls n . (d/dn, d/dn) with the boundary-frame first derivatives
+    ux, uy = o.gradient(o.extend(ic["ro"] * ic["u"]))
+    vx, vy = o.gradient(o.extend(ic["ro"] * ic["v"]))
+    div = ux + vy
+    walls = ns.node_type == 0
+    div[walls] = ns.rnx[walls] * ux[walls] + ns.rny[walls] * vx[walls]
+    err = np.abs(div - R["div_rouv"]).max() / np.abs(R["div_rouv"]).max()
+    print(f"{kind} order {order} divergence: {err:.2e}")
+    assert err < 1e-9
 
 
 def test_laplacian_transverse_closed_form():
