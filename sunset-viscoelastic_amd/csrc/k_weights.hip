@@ -208,11 +208,11 @@ struct Lds {
   static constexpr int CH = CH_;
   static constexpr int G = Pack<M>::G;
   static constexpr int ASZ = N * LDA;
-  // 16-neighbour chunks are stored column-major (function f of neighbour k
-  // at f * LDC + k, LDC = 17: the 16 lanes writing one function hit distinct
-  // banks; row-major, all 16 rows started on the same bank); 32-neighbour
-  // chunks row-major (measured faster for them)
-  static constexpr bool TR = (CH == 16);
+  // chunks are stored column-major (function f of neighbour k at f * LDC + k,
+  // LDC = CH + 1 odd: the lanes writing one function hit distinct banks;
+  // row-major, all rows started on the same bank).  Row-major 32-neighbour
+  // chunks measured slower than 16-neighbour ones; column-major they tie.
+  static constexpr bool TR = true;
   static constexpr int LDC = TR ? CH + 1 : 1;
   static constexpr int CHUNK = TR ? 2 * LDC * NP : 2 * CH * NP;
   static constexpr int UBS = NP + 8;
