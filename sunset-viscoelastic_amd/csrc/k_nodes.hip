@@ -17,6 +17,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <thread>
+
 #include "sv_internal.h"
 
 #define SS 2.0
@@ -572,15 +574,13 @@ static int build_halos(sv_ctx *c) {
     SV_HIP(c, hipMemcpy(fl.data(), L.lflip, L.nnz, hipMemcpyDeviceToHost));
     for (int64_t e = 0; e < L.nnz; e++) col[e] |= (int)fl[e] << SV_FLIP_SHIFT;
   }
-  std::vector<int64_t> hoff(nb + 1, 0);
-  std::vector<int> hidx;
+  // blocks are independent: each one's halo (sorted unique link targets) and
+  // its links' block-local indices, on up to 16 host threads over
+  // contiguous block ranges; the halos are concatenated in block order after
+  std::vector<std::vector<int>> hb(nb);
   std::vector<uint16_t> lcol(L.nnz, 0);
-  hidx.reserve((size_t)n * 3);
-  std::vector<int> h;
-  int hmax = 0;
-  bool fits = true;
   constexpr int SPB = SV_BLOCK / SV_WAVE;   // slices per block
-  for (int b = 0; b < nb; b++) {
+  auto one_block = [&](int b, std::vector<int> &h) {
     h.clear();
     for (int s = b * SPB; s < std::min(ns, (b + 1) * SPB); s++)
       for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
@@ -589,9 +589,8 @@ static int build_halos(sv_ctx *c) {
       }
     std::sort(h.begin(), h.end());
     h.erase(std::unique(h.begin(), h.end()), h.end());
-    hmax = std::max(hmax, (int)h.size());
-    if (h.size() > 65535) fits = false;
-    for (int s = b * SPB; fits && s < std::min(ns, (b + 1) * SPB); s++)
+    if (h.size() > 65535) return;   // too large for 16-bit indices: no halo sweeps
+    for (int s = b * SPB; s < std::min(ns, (b + 1) * SPB); s++)
       for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
         const int slot = s * SV_WAVE + l;
         for (int k = 0; k < cnt[slot]; k++) {
@@ -599,9 +598,28 @@ static int build_halos(sv_ctx *c) {
           lcol[e] = (uint16_t)(std::lower_bound(h.begin(), h.end(), col[e]) - h.begin());
         }
       }
-    hidx.insert(hidx.end(), h.begin(), h.end());
-    hoff[b + 1] = (int64_t)hidx.size();
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(nb, 1));
+  {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        for (int b = (int)((int64_t)nb * t / nt); b < (int)((int64_t)nb * (t + 1) / nt); b++)
+          one_block(b, hb[b]);
+      });
+    for (auto &th : pool) th.join();
   }
+  std::vector<int64_t> hoff(nb + 1, 0);
+  int hmax = 0;
+  bool fits = true;
+  for (int b = 0; b < nb; b++) {
+    hmax = std::max(hmax, (int)hb[b].size());
+    if (hb[b].size() > 65535) fits = false;
+    hoff[b + 1] = hoff[b] + (int64_t)hb[b].size();
+  }
+  std::vector<int> hidx((size_t)hoff[nb]);
+  for (int b = 0; b < nb; b++) std::copy(hb[b].begin(), hb[b].end(), hidx.begin() + hoff[b]);
   L.hmax = hmax;
   // the LC sweep keeps 72 bytes per halo record in LDS
   L.halo = c->halo_sweeps && fits && (size_t)hmax * 72 <= SV_HALO_LDS_MAX;
