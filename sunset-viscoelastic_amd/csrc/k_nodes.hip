@@ -651,8 +651,17 @@ int sv_dev_build_loop(sv_ctx *c) {
       const double fy = std::min(std::max((y[i] - c->ymin) * sy, 0.0), 65535.0);
       key[i] = morton2((uint32_t)fx, (uint32_t)fy);
     }
-    std::stable_sort(L.hperm.begin(), L.hperm.end(),
-                     [&](int a, int b) { return key[a] < key[b]; });
+    // stable LSD radix sort of the 32-bit keys, two 16-bit digits: the same
+    // permutation as a stable comparison sort
+    std::vector<int> tmp(n);
+    std::vector<int> cnt16(65537);
+    for (int shift = 0; shift < 32; shift += 16) {
+      std::fill(cnt16.begin(), cnt16.end(), 0);
+      for (int i = 0; i < n; i++) cnt16[((key[L.hperm[i]] >> shift) & 0xffff) + 1]++;
+      for (int d = 1; d <= 65536; d++) cnt16[d] += cnt16[d - 1];
+      for (int i = 0; i < n; i++) tmp[cnt16[(key[L.hperm[i]] >> shift) & 0xffff]++] = L.hperm[i];
+      L.hperm.swap(tmp);
+    }
   }
   std::vector<int> iperm(c->np);
   for (int k = 0; k < n; k++) iperm[L.hperm[k]] = k;

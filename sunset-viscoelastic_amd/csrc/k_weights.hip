@@ -1211,9 +1211,20 @@ int sv_dev_weights(sv_ctx *c, const int *order_map_host) {
       sys.push_back({mf, cnt[i], i, SYS_F});
     }
   }
-  std::stable_sort(sys.begin(), sys.end(), [](const Sys &a, const Sys &b) {
-    return a.m != b.m ? a.m < b.m : a.K > b.K;
-  });
+  // by order, then by decreasing stencil length, stable: a counting sort over
+  // the (order, length) keys gives the order of a stable comparison sort
+  {
+    int kmax = 0;
+    for (const Sys &e : sys) kmax = std::max(kmax, e.K);
+    const int nk = kmax + 1;
+    auto key = [&](const Sys &e) { return (e.m / 2 - 2) * nk + (kmax - e.K); };
+    std::vector<int64_t> start((size_t)4 * nk + 1, 0);
+    for (const Sys &e : sys) start[key(e) + 1]++;
+    for (size_t q = 1; q < start.size(); q++) start[q] += start[q - 1];
+    std::vector<Sys> sorted(sys.size());
+    for (const Sys &e : sys) sorted[start[key(e)]++] = e;
+    sys.swap(sorted);
+  }
   std::vector<int> hn(sys.size()), hk(sys.size());
   for (size_t s = 0; s < sys.size(); s++) { hn[s] = sys[s].node; hk[s] = sys[s].kind; }
   DevTmp<int> dn, dk, dq;
