@@ -531,17 +531,26 @@ def main():
     ns = jittered_lattice(args.n, seed=2, h_over_s=2.4)
     print(f"[bench] {ns.npfb} nodes: setup", file=sys.stderr, flush=True)
     s = sunset_amd.Solver(morder=6, form="lc")
-    s.set_nodes(ns)
+    setup = {}   # wall time of each setup call (host work included)
+
+    def timed(name, f):
+        t = time.perf_counter()
+        r = f()
+        setup[name] = round((time.perf_counter() - t) * 1e3, 1)
+        return r
+
+    timed("set_nodes", lambda: s.set_nodes(ns))
     cnt, cs, link = s.find_neighbours()    # host copies: CPU baseline and oracle check
     t_neigh = s.timing("neighbours")
-    s.calc_weights()
+    timed("calc_weights", s.calc_weights)
     t_w = s.timing("weights")
     n_sys = s.timing("systems")
-    s.calc_sums()
-    s.filter_coefficients()
+    timed("calc_sums", s.calc_sums)
+    timed("filter_coefficients", s.filter_coefficients)
     st = s.stencil_stats()   # halo statistics exist once the time loop has run
     s.set_phys(**PHYS)
-    s.set_initial(taylor_green_ic(ns))
+    ic0 = taylor_green_ic(ns)
+    timed("set_initial", lambda: s.set_initial(ic0))
     print(f"[bench] neighbours {t_neigh:.1f} ms, weights {t_w:.1f} ms "
           f"({n_sys:.0f} systems), K={st['K_mean']:.2f}; warmup {args.warmup}",
           file=sys.stderr, flush=True)
@@ -681,6 +690,7 @@ def main():
         "weights_fp64": fp64_figure(n_sys * weight_flops(6, st["Ks_mean"]), t_w),
         "cylinder_channel_weights": c3,
         "weight_setup_ms": round(t_w, 2), "neighbour_ms": round(t_neigh, 2),
+        "setup_wall_ms": setup,
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()} if kern else None,
         "wall_channel": bnd,
         "ensemble": ens,
