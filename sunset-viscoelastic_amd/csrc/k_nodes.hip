@@ -17,6 +17,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
+#include <cstdint>
 #include <thread>
 
 #include "sv_internal.h"
@@ -580,22 +582,48 @@ static int build_halos(sv_ctx *c) {
   std::vector<std::vector<int>> hb(nb);
   std::vector<uint16_t> lcol(L.nnz, 0);
   constexpr int SPB = SV_BLOCK / SV_WAVE;   // slices per block
-  auto one_block = [&](int b, std::vector<int> &h) {
+  // one block: its halo (the sorted unique link targets) and each link's
+  // rank in it.  Targets within a narrow range (the usual case in Morton
+  // order) are ranked with a dense marker array in one pass; a wide range
+  // (a Morton seam, or flipped mirror keys) takes a sort and binary searches.
+  // Both give the same halo and ranks.
+  auto one_block = [&](int b, std::vector<int> &h, std::vector<int> &mark) {
     h.clear();
+    int lo = INT32_MAX, hi = INT32_MIN;
     for (int s = b * SPB; s < std::min(ns, (b + 1) * SPB); s++)
       for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
         const int slot = s * SV_WAVE + l;
-        for (int k = 0; k < cnt[slot]; k++) h.push_back(col[off[s] + (int64_t)k * SV_WAVE + l]);
+        for (int k = 0; k < cnt[slot]; k++) {
+          const int v = col[off[s] + (int64_t)k * SV_WAVE + l];
+          h.push_back(v);
+          lo = std::min(lo, v);
+          hi = std::max(hi, v);
+        }
       }
-    std::sort(h.begin(), h.end());
-    h.erase(std::unique(h.begin(), h.end()), h.end());
+    if (h.empty()) return;
+    const int64_t range = (int64_t)hi - lo + 1;
+    const bool dense = range <= 4 * (int64_t)h.size() + 4096;
+    if (dense) {
+      mark.assign((size_t)range, -1);
+      for (int v : h) mark[v - lo] = 0;
+      h.clear();
+      for (int64_t q = 0; q < range; q++)
+        if (mark[q] == 0) {
+          mark[q] = (int)h.size();
+          h.push_back(lo + (int)q);
+        }
+    } else {
+      std::sort(h.begin(), h.end());
+      h.erase(std::unique(h.begin(), h.end()), h.end());
+    }
     if (h.size() > 65535) return;   // too large for 16-bit indices: no halo sweeps
     for (int s = b * SPB; s < std::min(ns, (b + 1) * SPB); s++)
       for (int l = 0; l < SV_WAVE && s * SV_WAVE + l < n; l++) {
         const int slot = s * SV_WAVE + l;
         for (int k = 0; k < cnt[slot]; k++) {
           const int64_t e = off[s] + (int64_t)k * SV_WAVE + l;
-          lcol[e] = (uint16_t)(std::lower_bound(h.begin(), h.end(), col[e]) - h.begin());
+          lcol[e] = dense ? (uint16_t)mark[col[e] - lo]
+                          : (uint16_t)(std::lower_bound(h.begin(), h.end(), col[e]) - h.begin());
         }
       }
   };
@@ -605,8 +633,9 @@ static int build_halos(sv_ctx *c) {
     std::vector<std::thread> pool;
     for (int t = 0; t < nt; t++)
       pool.emplace_back([&, t] {
+        std::vector<int> mark;
         for (int b = (int)((int64_t)nb * t / nt); b < (int)((int64_t)nb * (t + 1) / nt); b++)
-          one_block(b, hb[b]);
+          one_block(b, hb[b], mark);
       });
     for (auto &th : pool) th.join();
   }
