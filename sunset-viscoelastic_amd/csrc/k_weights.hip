@@ -379,6 +379,62 @@ __device__ __forceinline__ SysOf sys_of(const SysArgs &S_, int sys) {
 }
 
 // Weights w_k = (psi_q . g_k)/hh (labf.F90:183-215, k <= ij_count) and the
+// node volume (calc_node_volumes, labf.F90:453-503) of system `sys`.  Two
+// lanes per neighbour, 32 neighbours per pass: lane p = l & 1 keeps psi of
+// right-hand sides 2p and 2p+1 (p = 0: d/dx, d/dy; p = 1: Laplacian, filter)
+// in registers and evaluates the neighbour's ABFs once for both sums, each in
+// the order of the unknowns.
+template <int M>
+__device__ __forceinline__ void eval_weights(const SysArgs &S_, int sys, const double *Xs) {
+  constexpr int N = nsize(M);
+  const int l = threadIdx.x, p = l & 1, kl = l >> 1;
+  const SysOf o = sys_of<M>(S_, sys);
+  const int K = o.Kfull;
+  const bool wantB = p == 0 ? o.sysD : o.sysF;   // d/dy or the filter
+  const double hh = o.hh, rhh = 1.0 / hh;
+  const double xi = S_.x[o.i], yi = S_.y[o.i];
+  double psA[N], psB[N];
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    psA[a] = Xs[a * 4 + 2 * p];
+    psB[a] = Xs[a * 4 + 2 * p + 1];
+  }
+  double *const dA = p ? S_.wl : S_.wx;
+  double *const dB = p ? S_.wh : S_.wy;
+  double vsum = 0.0;
+  for (int k = kl; k < K; k += SV_WAVE / 2) {
+    const int j = S_.col[o.base + k];
+    const double x = -(xi - S_.x[j]), y = -(yi - S_.y[j]);
+    const double rad = sqrt(x * x + y * y) * rhh;
+    const double w = wab(rad, S_.h2);
+    double hx[M + 1], hy[M + 1];
+    hermite_all<M>(x * rhh, hx);
+    hermite_all<M>(y * rhh, hy);
+    double accA = 0.0, accB = 0.0;
+    int idx = 0;
+#pragma unroll
+    for (int d = 1; d <= M; d++)
+#pragma unroll
+      for (int b = 0; b <= d; b++) {
+        const int a = d - b;
+        const double v = ((b == 0) ? hx[a] : (a == 0) ? hy[b] : hx[a] * hy[b]) * w;
+        accA += psA[idx] * v;
+        accB += psB[idx] * v;
+        idx++;
+      }
+    if (o.sysD) dA[o.base + k] = p ? accA * rhh * rhh : accA * rhh;
+    if (wantB) dB[o.base + k] = p ? accB : accB * rhh;
+    if (p == 1 && o.sysF) vsum += (9.0 / SV_PI) * exp(-9.0 * rad * rad) / (hh * hh);
+  }
+  if (o.sysF) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) vsum += __shfl_xor(vsum, off);
+    if (l == 0) S_.vol[o.i] = 1.0 / vsum;
+  }
+}
+
+// Four lanes per neighbour (orders 4 and 8, where two would cost a wave per
+// SIMD in registers): weights w_k = (psi_q . g_k)/hh (labf.F90:183-215) and the
 // node volume (calc_node_volumes, labf.F90:453-503) of the `count` (<= G)
 // systems of a unit.  Four lanes per neighbour, lane q = l & 3 for
 // right-hand side q (d/dx, d/dy, Laplacian, filter), 16 neighbours per pass
@@ -387,8 +443,8 @@ __device__ __forceinline__ SysOf sys_of(const SysArgs &S_, int sys) {
 // chains that the compiler interleaves).  A lane past a system's stencil
 // evaluates the node itself and stores nothing.
 template <int M, int G>
-__device__ __forceinline__ void eval_weights(const SysArgs &S_, int first, int count,
-                                             const double *Xs) {
+__device__ __forceinline__ void eval_weights4(const SysArgs &S_, int first, int count,
+                                              const double *Xs) {
   constexpr int N = nsize(M), NP = npad(M);
   const int l = threadIdx.x, q = l & 3, kl = l >> 2;
   int ii[G], K[G];
@@ -466,7 +522,8 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int first, int c
 //     and the correction added (the reference's Golub-Reinsch SVD,
 //     svdlib.F90, is only within cond(A)*eps of the exact solution of the FP64
 //     matrix; the refined solution is at it to rounding);
-//  4. the weights of the unit's systems, four lanes per neighbour (eval_weights).
+//  4. the weights of the unit's systems, two (order 6) or four lanes per
+//     neighbour (eval_weights, eval_weights4).
 template <int M, int CH_>
 __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int count, double *lds) {
   using P = Pack<M>;
@@ -632,9 +689,11 @@ __device__ __forceinline__ void solve_unit(const SysArgs &S_, int first, int cou
   if (sing && row) atomicAdd(S_.singular, 1);
   __syncthreads();
 
-  // ---- 4. weights, one system after the other (two per pass measured no
-  // faster at order 6 and cost the order-4 kernel a wave per SIMD) ----
-  for (int e0 = 0; e0 < count; e0++) eval_weights<M, 1>(S_, first + e0, 1, Xs + e0 * NP * 4);
+  // ---- 4. weights, one system after the other ----
+  for (int e0 = 0; e0 < count; e0++) {
+    if (M == 6) eval_weights<M>(S_, first + e0, Xs + e0 * NP * 4);
+    else eval_weights4<M, 1>(S_, first + e0, 1, Xs + e0 * NP * 4);
+  }
   __syncthreads();
 }
 
