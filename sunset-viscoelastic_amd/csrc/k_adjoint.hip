@@ -829,6 +829,40 @@ __global__ void k_scatter_records(int n, const int *perm, const double *rec, Out
     if (out.f[q]) out.f[q][i] = rec[(size_t)SV_US * k + q];
 }
 
+// The eleven output fields of sv_get_state in the caller's node order, field
+// after field (out[q * n + i]): ro, rou, rov from the registers, u, v from the
+// gather records, psi (LC; left untouched in DI) from the registers, c from
+// the gather records.
+__global__ void k_get_fields(int n, const int *perm, const double *U, const double *G, int gs,
+                             int lc, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+  const double *u = U + (size_t)SV_US * k;
+  const double *g = G + (size_t)gs * k;
+  out[0 * (size_t)n + i] = u[0];
+  out[1 * (size_t)n + i] = u[1];
+  out[2 * (size_t)n + i] = u[2];
+  out[3 * (size_t)n + i] = g[SV_G_U];
+  out[4 * (size_t)n + i] = g[SV_G_V];
+  if (lc) {
+    out[5 * (size_t)n + i] = u[3];
+    out[6 * (size_t)n + i] = u[4];
+    out[7 * (size_t)n + i] = u[5];
+  }
+  out[8 * (size_t)n + i] = g[SV_G_CXX];
+  out[9 * (size_t)n + i] = g[SV_G_CXY];
+  out[10 * (size_t)n + i] = g[SV_G_CYY];
+}
+
+int sv_dev_get_fields(sv_ctx *c, double *out) {
+  const int n = c->npfb;
+  k_get_fields<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->L.perm, c->U, c->GA, c->gs,
+                                                       c->cfg.form == SV_FORM_LC, out);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int sv_dev_gather_records(sv_ctx *c, double *dst, const double *const f[6]) {
   Fields6 in{};
   for (int q = 0; q < 6; q++) in.f[q] = f ? f[q] : nullptr;

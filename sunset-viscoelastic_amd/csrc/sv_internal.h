@@ -199,6 +199,8 @@ struct sv_ctx {
   // both divided by L_char^2 once the sums are computed
   double *wtt = nullptr, *wtt_sum = nullptr;
   int wtt_ld = 0;
+  // sv_get_state's staging area on the device ([11][npfb], kept between calls)
+  double *fields_buf = nullptr;
   bool have_normals = false, have_bweights = false;
   // inflow settings (sv_set_inflow)
   int inflow_type = 0, u_control = 0;
@@ -362,6 +364,9 @@ int sv_dev_mass_correct(sv_ctx *c, double *ro, hipStream_t st);
 // <-> loop-order records on the context's stream
 int sv_dev_gather_records(sv_ctx *c, double *dst, const double *const f[6]);
 int sv_dev_scatter_records(sv_ctx *c, const double *src, double *const f[6]);
+// the eleven fields of sv_get_state in caller order into out[11][npfb]
+// (single context; the psi rows are left untouched in DI)
+int sv_dev_get_fields(sv_ctx *c, double *out);
 // 0 if p is device memory of the context's device in this library's HIP
 // runtime, else an SV_E_ARG failure naming `what`
 int sv_check_device_ptr(sv_ctx *c, const void *p, const char *what);
