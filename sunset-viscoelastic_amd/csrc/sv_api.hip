@@ -774,15 +774,18 @@ int sv_set_state(sv_ctx *c, const double *ro, const double *rou, const double *r
       SV_CHECK(c, xx[i] * yy[i] - xy[i] * xy[i] > 0 && xx[i] > 0,
                "node %d: conformation tensor not positive definite", i);
   }
-  std::vector<double> rec((size_t)SV_US * n);
-  SV_HIP(c, hipMemcpy(rec.data(), c->U, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
-  const std::vector<int> &P = c->L.hperm;   // record k holds the caller's node P[k]
-  for (int k = 0; k < n; k++) {
-    double *r = rec.data() + (size_t)SV_US * k;
-    const int i = P[k];
-    r[0] = ro[i]; r[1] = rou[i]; r[2] = rov[i]; r[3] = xx[i]; r[4] = xy[i]; r[5] = yy[i];
+  // the six arrays to the device as they are, then into the loop-order
+  // registers there (record k holds the caller's node perm[k])
+  if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 11 * (size_t)n))
+    return SV_E_NOMEM;
+  const double *src[6] = {ro, rou, rov, xx, xy, yy};
+  const double *dev[6];
+  for (int q = 0; q < 6; q++) {
+    double *d = c->fields_buf + (size_t)q * n;
+    SV_HIP(c, hipMemcpy(d, src[q], sizeof(double) * n, hipMemcpyHostToDevice));
+    dev[q] = d;
   }
-  SV_HIP(c, hipMemcpy(c->U, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+  if (int rc = sv_dev_gather_records(c, c->U, dev)) return rc;
   return sv_dev_init_state(c);
 }
 
