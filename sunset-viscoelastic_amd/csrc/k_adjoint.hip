@@ -855,6 +855,23 @@ __global__ void k_get_fields(int n, const int *perm, const double *U, const doub
   out[10 * (size_t)n + i] = g[SV_G_CYY];
 }
 
+// nf fields of n values from loop order (in[f * n + k]) to the caller's
+// order (out[f * n + perm[k]])
+__global__ void k_to_caller_order(int n, int nf, const int *perm, const double *in,
+                                  double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+  for (int f = 0; f < nf; f++) out[(size_t)f * n + i] = in[(size_t)f * n + k];
+}
+
+int sv_dev_to_caller_order(sv_ctx *c, int nf, const double *in, double *out) {
+  const int n = c->npfb;
+  k_to_caller_order<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nf, c->L.perm, in, out);
+  SV_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int sv_dev_get_fields(sv_ctx *c, double *out) {
   const int n = c->npfb;
   k_get_fields<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->L.perm, c->U, c->GA, c->gs,

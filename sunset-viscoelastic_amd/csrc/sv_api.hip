@@ -776,7 +776,7 @@ int sv_set_state(sv_ctx *c, const double *ro, const double *rou, const double *r
   }
   // the six arrays to the device as they are, then into the loop-order
   // registers there (record k holds the caller's node perm[k])
-  if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 11 * (size_t)n))
+  if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 12 * (size_t)n))
     return SV_E_NOMEM;
   const double *src[6] = {ro, rou, rov, xx, xy, yy};
   const double *dev[6];
@@ -909,22 +909,20 @@ int sv_calc_all_rhs(sv_ctx *c, int32_t iRKstep, double *r0, double *r1, double *
   SV_CHECK(c, c && c->have_state, "sv_set_initial must come first");
   SV_CHECK(c, !c->is_ensemble, "not available on an ensemble context");
   if (int rc = set_device(c)) return rc;
-  DevTmp<double> out;
-  SV_HIP(c, out.alloc(6 * (size_t)c->npfb));
-  int rc = sv_dev_rhs(c, iRKstep, out);
-  if (!rc) {
-    double *dst[6] = {r0, r1, r2, r3, r4, r5};
-    const int n = c->npfb;
-    const std::vector<int> &P = c->L.hperm;   // loop order -> caller's
-    std::vector<double> tmp(n);
-    for (int f = 0; f < 6; f++)
-      if (dst[f]) {
-        SV_HIP(c, hipMemcpy(tmp.data(), out.p + (size_t)f * n, sizeof(double) * n,
-                            hipMemcpyDeviceToHost));
-        for (int k = 0; k < n; k++) dst[f][P[k]] = tmp[k];
-      }
-  }
-  return rc;
+  const size_t n = c->npfb;
+  // the right-hand sides in loop order in the first half of the staging
+  // area, put in the caller's order in the second half on the device
+  if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 12 * n))
+    return SV_E_NOMEM;
+  double *lo = c->fields_buf, *co = c->fields_buf + 6 * n;
+  if (int rc = sv_dev_rhs(c, iRKstep, lo)) return rc;
+  if (int rc = sv_dev_to_caller_order(c, 6, lo, co)) return rc;
+  SV_HIP(c, hipStreamSynchronize(c->stream));
+  double *dst[6] = {r0, r1, r2, r3, r4, r5};
+  for (int f = 0; f < 6; f++)
+    if (dst[f])
+      SV_HIP(c, hipMemcpy(dst[f], co + (size_t)f * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int sv_run(sv_ctx *c, int32_t nsteps, int32_t use_graph) {
@@ -1036,7 +1034,7 @@ int sv_ensemble_get_state(sv_ctx *c, int32_t member, double *ro, double *rou, do
   if (!c->is_ensemble && c->L.perm) {
     // fields picked and put in the caller's order on the device, then one
     // copy per field (the host scatter of eleven fields cost ~50 ms at 1M)
-    if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 11 * n))
+    if (!c->fields_buf && sv_alloc(c, (void **)&c->fields_buf, sizeof(double) * 12 * n))
       return SV_E_NOMEM;
     if (int rc = sv_dev_get_fields(c, c->fields_buf)) return rc;
     SV_HIP(c, hipStreamSynchronize(c->stream));

@@ -199,7 +199,8 @@ struct sv_ctx {
   // both divided by L_char^2 once the sums are computed
   double *wtt = nullptr, *wtt_sum = nullptr;
   int wtt_ld = 0;
-  // sv_get_state's staging area on the device ([11][npfb], kept between calls)
+  // staging area of the host-array entry points on the device ([12][npfb],
+  // kept between calls): sv_get_state, sv_set_state, sv_calc_all_rhs
   double *fields_buf = nullptr;
   bool have_normals = false, have_bweights = false;
   // inflow settings (sv_set_inflow)
@@ -367,6 +368,8 @@ int sv_dev_scatter_records(sv_ctx *c, const double *src, double *const f[6]);
 // the eleven fields of sv_get_state in caller order into out[11][npfb]
 // (single context; the psi rows are left untouched in DI)
 int sv_dev_get_fields(sv_ctx *c, double *out);
+// nf loop-order fields (in[f][npfb]) to the caller's order (out[f][npfb])
+int sv_dev_to_caller_order(sv_ctx *c, int nf, const double *in, double *out);
 // 0 if p is device memory of the context's device in this library's HIP
 // runtime, else an SV_E_ARG failure naming `what`
 int sv_check_device_ptr(sv_ctx *c, const void *p, const char *what);
