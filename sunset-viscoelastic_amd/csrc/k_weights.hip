@@ -433,8 +433,9 @@ __device__ __forceinline__ void eval_weights(const SysArgs &S_, int sys, const d
   }
 }
 
-// Four lanes per neighbour (orders 4 and 8, where two would cost a wave per
-// SIMD in registers): weights w_k = (psi_q . g_k)/hh (labf.F90:183-215) and the
+// Four lanes per neighbour (orders 4 and 8, where two cost a wave per SIMD in
+// registers: order 4 measured 6% slower with two): weights
+// w_k = (psi_q . g_k)/hh (labf.F90:183-215) and the
 // node volume (calc_node_volumes, labf.F90:453-503) of the `count` (<= G)
 // systems of a unit.  Four lanes per neighbour, lane q = l & 3 for
 // right-hand side q (d/dx, d/dy, Laplacian, filter), 16 neighbours per pass
